@@ -1,0 +1,916 @@
+/*
+ * oracle.c -- FP64 CPU oracle and FP32 reference restatement (see oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg, never by the product.
+ *
+ * Parity pins (what checks each function; the tests named live under tests/):
+ *   ora_box_mesh / ora_box_mesh_f32  golden connectivity from the compiled reference
+ *                                    makeBoxTetMesh (fem.cpp:217-263), test_oracle_golden
+ *   ora_rest_state                   golden restInv/volume/mass/CSR from the compiled
+ *                                    computeRestState (fem.cpp:119-158); exact volume sum
+ *   ora_edge_timestep                golden criticalTimestep (fem.cpp:183-185)
+ *   ora_max_element_omega            brute-force 12x12 eigen-solve vs a closed form for a
+ *                                    right-corner tet, and stability at 0.99 / blow-up
+ *                                    at 1.2 of the bound on a real mesh
+ *   ora_tet_force                    compiled tetForces on the twisted fem_spike mesh;
+ *                                    rigid motion => zero force; sum f = 0, sum x^f = 0;
+ *                                    patch test (uniform strain => exact traction)
+ *   ora_j2_update                    compiled plasticity::update on strain paths;
+ *                                    uniaxial closed form, sigma_y/sqrt3 in shear,
+ *                                    hydrostatic no-yield (test_plasticity.cpp:291-750)
+ *   ora_face_traction                total force = traction * face area (closed form)
+ *   ora_run                          compiled stepCpu trajectory (elastic, FP32 ref vs
+ *                                    FP64 oracle agree to FP32 rounding); rest stays at
+ *                                    rest; rigid translation; energy closes for an
+ *                                    undamped linear run; contact momentum and work
+ *   ora_panel_mesh / ora_box_jitter  positive volumes, counts in closed form, exact
+ *                                    agreement with the product's generators
+ *   ref32_*                          bit-identical to the compiled reference
+ *                                    (fem.cpp, -ffp-contract=off), 1 and N threads
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ======================================================================== */
+/* small FP64 3x3 helpers, column-major m[col*3+row] like fem.cpp:10-16      */
+/* ======================================================================== */
+#define AT(m, r, c) ((m)[(c) * 3 + (r)])
+
+static void mul3(const double* a, const double* b, double* r) {
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 3; ++i) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += AT(a, i, k) * AT(b, k, c);
+            AT(r, i, c) = s;
+        }
+}
+
+static void transpose3(const double* a, double* r) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) AT(r, i, j) = AT(a, j, i);
+}
+
+static double det3(const double* a) {
+    return AT(a, 0, 0) * (AT(a, 1, 1) * AT(a, 2, 2) - AT(a, 1, 2) * AT(a, 2, 1)) -
+           AT(a, 0, 1) * (AT(a, 1, 0) * AT(a, 2, 2) - AT(a, 1, 2) * AT(a, 2, 0)) +
+           AT(a, 0, 2) * (AT(a, 1, 0) * AT(a, 2, 1) - AT(a, 1, 1) * AT(a, 2, 0));
+}
+
+/* cofactor inverse with the reference's singular guard (fem.cpp:42-56) */
+static void inv3(const double* a, double* r) {
+    const double det = det3(a);
+    const double id = fabs(det) > 1e-30 ? 1.0 / det : 0.0;
+    AT(r, 0, 0) = (AT(a, 1, 1) * AT(a, 2, 2) - AT(a, 1, 2) * AT(a, 2, 1)) * id;
+    AT(r, 0, 1) = (AT(a, 0, 2) * AT(a, 2, 1) - AT(a, 0, 1) * AT(a, 2, 2)) * id;
+    AT(r, 0, 2) = (AT(a, 0, 1) * AT(a, 1, 2) - AT(a, 0, 2) * AT(a, 1, 1)) * id;
+    AT(r, 1, 0) = (AT(a, 1, 2) * AT(a, 2, 0) - AT(a, 1, 0) * AT(a, 2, 2)) * id;
+    AT(r, 1, 1) = (AT(a, 0, 0) * AT(a, 2, 2) - AT(a, 0, 2) * AT(a, 2, 0)) * id;
+    AT(r, 1, 2) = (AT(a, 0, 2) * AT(a, 1, 0) - AT(a, 0, 0) * AT(a, 1, 2)) * id;
+    AT(r, 2, 0) = (AT(a, 1, 0) * AT(a, 2, 1) - AT(a, 1, 1) * AT(a, 2, 0)) * id;
+    AT(r, 2, 1) = (AT(a, 0, 1) * AT(a, 2, 0) - AT(a, 0, 0) * AT(a, 2, 1)) * id;
+    AT(r, 2, 2) = (AT(a, 0, 0) * AT(a, 1, 1) - AT(a, 0, 1) * AT(a, 1, 0)) * id;
+}
+
+/* Higham's Newton iteration R <- (R + R^-T)/2, exactly four times (fem.cpp:62-69) */
+static void polar4(const double* f, double* r) {
+    double inv[9], it[9];
+    memcpy(r, f, sizeof(double) * 9);
+    for (int i = 0; i < 4; ++i) {
+        inv3(r, inv);
+        transpose3(inv, it);
+        for (int k = 0; k < 9; ++k) r[k] = 0.5 * (r[k] + it[k]);
+    }
+}
+
+/* Kuhn subdivision of a cell into six tets on the 0-7 diagonal (fem.cpp:236-237);
+ * cell corner c sits at (i + (c&1), j + ((c>>1)&1), k + ((c>>2)&1)). */
+static const int kKuhn[6][4] = {{0, 1, 3, 7}, {0, 1, 7, 5}, {0, 5, 7, 4},
+                                {0, 3, 2, 7}, {0, 2, 6, 7}, {0, 4, 6, 7}};
+
+/* ======================================================================== */
+/* meshes                                                                    */
+/* ======================================================================== */
+
+/* Appends the six Kuhn tets of one cell with corner node ids `corner`, swapping
+ * v1,v2 when the double-precision orientation determinant is negative
+ * (fem.cpp:246-258). */
+static int64_t emit_cell(const double* X, const uint32_t corner[8], uint32_t* index, int64_t t) {
+    for (int k = 0; k < 6; ++k) {
+        uint32_t v[4];
+        for (int c = 0; c < 4; ++c) v[c] = corner[kKuhn[k][c]];
+        double dm[9];
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) AT(dm, a, c) = X[v[c + 1] * 3 + a] - X[v[0] * 3 + a];
+        if (det3(dm) < 0) {
+            uint32_t s = v[1];
+            v[1] = v[2];
+            v[2] = s;
+        }
+        if (index) memcpy(&index[t * 4], v, sizeof v);
+        ++t;
+    }
+    return t;
+}
+
+void ora_box_mesh(double lx, double ly, double lz, int nx, int ny, int nz, int64_t* nodes,
+                  int64_t* tets, double* X, uint32_t* index) {
+    const int sx = nx + 1, sy = ny + 1, sz = nz + 1;
+    if (nodes) *nodes = (int64_t)sx * sy * sz;
+    if (tets) *tets = (int64_t)nx * ny * nz * 6;
+    if (!X || !index) return;
+    for (int k = 0; k < sz; ++k)
+        for (int j = 0; j < sy; ++j)
+            for (int i = 0; i < sx; ++i) {
+                double* p = &X[((int64_t)(k * sy + j) * sx + i) * 3];
+                p[0] = lx * (double)i / (double)nx;
+                p[1] = ly * (double)j / (double)ny;
+                p[2] = lz * (double)k / (double)nz;
+            }
+    int64_t t = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                uint32_t corner[8];
+                for (int c = 0; c < 8; ++c)
+                    corner[c] = (uint32_t)(((k + ((c >> 2) & 1)) * sy + (j + ((c >> 1) & 1))) * sx +
+                                           (i + (c & 1)));
+                t = emit_cell(X, corner, index, t);
+            }
+}
+
+void ora_box_mesh_f32(float lx, float ly, float lz, int nx, int ny, int nz, float* X,
+                      uint32_t* index) {
+    const int sx = nx + 1, sy = ny + 1, sz = nz + 1;
+    for (int k = 0; k < sz; ++k)
+        for (int j = 0; j < sy; ++j)
+            for (int i = 0; i < sx; ++i) {
+                float* p = &X[((int64_t)(k * sy + j) * sx + i) * 3];
+                p[0] = lx * (float)i / (float)nx;
+                p[1] = ly * (float)j / (float)ny;
+                p[2] = lz * (float)k / (float)nz;
+            }
+    int64_t t = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                uint32_t corner[8];
+                for (int c = 0; c < 8; ++c)
+                    corner[c] = (uint32_t)(((k + ((c >> 2) & 1)) * sy + (j + ((c >> 1) & 1))) * sx +
+                                           (i + (c & 1)));
+                for (int q = 0; q < 6; ++q) {
+                    uint32_t v[4];
+                    for (int c = 0; c < 4; ++c) v[c] = corner[kKuhn[q][c]];
+                    float d[9];
+                    for (int c = 0; c < 3; ++c)
+                        for (int a = 0; a < 3; ++a)
+                            d[c * 3 + a] = X[v[c + 1] * 3 + a] - X[v[0] * 3 + a];
+                    /* float determinant, same expression as fem.cpp:36-40 */
+                    float det = d[0] * (d[4] * d[8] - d[7] * d[5]) -
+                                d[3] * (d[1] * d[8] - d[7] * d[2]) +
+                                d[6] * (d[1] * d[5] - d[4] * d[2]);
+                    if (det < 0) {
+                        uint32_t s = v[1];
+                        v[1] = v[2];
+                        v[2] = s;
+                    }
+                    memcpy(&index[t * 4], v, sizeof v);
+                    ++t;
+                }
+            }
+}
+
+double ora_uniform(uint64_t seed, uint64_t counter) {
+    uint64_t z = seed + (counter + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void ora_box_jitter(double* X, int nx, int ny, int nz, double lx, double ly, double lz,
+                    double amp, uint64_t seed) {
+    const int sx = nx + 1, sy = ny + 1, sz = nz + 1;
+    const double h[3] = {lx / nx, ly / ny, lz / nz};
+    for (int k = 0; k < sz; ++k)
+        for (int j = 0; j < sy; ++j)
+            for (int i = 0; i < sx; ++i) {
+                const int64_t n = (int64_t)(k * sy + j) * sx + i;
+                const int onface[3] = {i == 0 || i == nx, j == 0 || j == ny, k == 0 || k == nz};
+                for (int a = 0; a < 3; ++a) {
+                    if (onface[a]) continue;
+                    const double r = ora_uniform(seed, (uint64_t)n * 3 + (uint64_t)a);
+                    X[n * 3 + a] += amp * h[a] * (2.0 * r - 1.0);
+                }
+            }
+}
+
+/* y grid lines of the stiffened panel; returns count of lines, fills web column
+ * cell indices. */
+static int panel_y_lines(const ora_panel_spec* s, double* y, int* web_col) {
+    int n = 0;
+    if (y) y[n] = 0.0;
+    ++n;
+    for (int b = 0; b <= s->n_stiff; ++b) {
+        const double ys_prev = s->ly * (double)b / (double)(s->n_stiff + 1);
+        const double y0 = b == 0 ? 0.0 : ys_prev + 0.5 * s->web_t;
+        const double y1 = b == s->n_stiff
+                              ? s->ly
+                              : s->ly * (double)(b + 1) / (double)(s->n_stiff + 1) - 0.5 * s->web_t;
+        for (int j = 1; j <= s->ny_bay; ++j) {
+            if (y) y[n] = j == s->ny_bay ? y1 : y0 + (y1 - y0) * (double)j / (double)s->ny_bay;
+            ++n;
+        }
+        if (b < s->n_stiff) {
+            if (web_col) web_col[b] = n - 1; /* the cell between line n-1 and n */
+            if (y) y[n] = s->ly * (double)(b + 1) / (double)(s->n_stiff + 1) + 0.5 * s->web_t;
+            ++n;
+        }
+    }
+    return n;
+}
+
+void ora_panel_mesh(const ora_panel_spec* s, int64_t* nodes, int64_t* tets, double* X,
+                    uint32_t* index) {
+    const int ny_lines = panel_y_lines(s, NULL, NULL);
+    const int ny = ny_lines - 1, nx = s->nx, nzc = 1 + s->nz_web;
+    const int sx = nx + 1, sy = ny + 1, sz = nzc + 1;
+    double* y = (double*)malloc(sizeof(double) * ny_lines);
+    int* webc = (int*)malloc(sizeof(int) * (s->n_stiff > 0 ? s->n_stiff : 1));
+    panel_y_lines(s, y, webc);
+    unsigned char* colweb = (unsigned char*)calloc((size_t)ny, 1);
+    for (int b = 0; b < s->n_stiff; ++b) colweb[webc[b]] = 1;
+#define CELL_ACTIVE(i, j, k) ((k) == 0 || colweb[(j)])
+    /* node used flags over the full lattice, then lexicographic numbering */
+    int64_t* id = (int64_t*)malloc(sizeof(int64_t) * (size_t)sx * sy * sz);
+    for (int64_t q = 0; q < (int64_t)sx * sy * sz; ++q) id[q] = -1;
+    int64_t cells = 0;
+    for (int k = 0; k < nzc; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                if (!CELL_ACTIVE(i, j, k)) continue;
+                ++cells;
+                for (int c = 0; c < 8; ++c)
+                    id[((int64_t)(k + ((c >> 2) & 1)) * sy + (j + ((c >> 1) & 1))) * sx +
+                       (i + (c & 1))] = 0;
+            }
+    int64_t nn = 0;
+    for (int k = 0; k < sz; ++k)
+        for (int j = 0; j < sy; ++j)
+            for (int i = 0; i < sx; ++i) {
+                int64_t q = ((int64_t)k * sy + j) * sx + i;
+                if (id[q] < 0) continue;
+                id[q] = nn;
+                if (X) {
+                    X[nn * 3 + 0] = s->lx * (double)i / (double)nx;
+                    X[nn * 3 + 1] = y[j];
+                    X[nn * 3 + 2] = k == 0 ? 0.0
+                                           : s->t_plate + s->web_h * (double)(k - 1) / (double)s->nz_web;
+                }
+                ++nn;
+            }
+    if (nodes) *nodes = nn;
+    if (tets) *tets = cells * 6;
+    if (X && index) {
+        int64_t t = 0;
+        for (int k = 0; k < nzc; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    if (!CELL_ACTIVE(i, j, k)) continue;
+                    uint32_t corner[8];
+                    for (int c = 0; c < 8; ++c)
+                        corner[c] = (uint32_t)id[((int64_t)(k + ((c >> 2) & 1)) * sy +
+                                                  (j + ((c >> 1) & 1))) * sx + (i + (c & 1))];
+                    t = emit_cell(X, corner, index, t);
+                }
+    }
+#undef CELL_ACTIVE
+    free(id);
+    free(colweb);
+    free(webc);
+    free(y);
+}
+
+/* ======================================================================== */
+/* rest state, time step, loads                                             */
+/* ======================================================================== */
+
+void ora_rest_state(const double* X, int64_t N, const uint32_t* index, int64_t T, double density,
+                    double* rest_inv, double* volume, double* mass, uint32_t* adj_offset,
+                    uint32_t* adj_entry) {
+    memset(mass, 0, sizeof(double) * (size_t)N);
+    for (int64_t t = 0; t < T; ++t) {
+        const uint32_t* idx = &index[t * 4];
+        double dm[9];
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) AT(dm, a, c) = X[idx[c + 1] * 3 + a] - X[idx[0] * 3 + a];
+        volume[t] = fabs(det3(dm) / 6.0);
+        inv3(dm, &rest_inv[t * 9]);
+        const double quarter = density * volume[t] * 0.25;
+        for (int c = 0; c < 4; ++c) mass[idx[c]] += quarter;
+    }
+    /* CSR node -> (tet*4 + corner) by counting sort, entries in (tet, corner)
+     * order (fem.cpp:147-157) */
+    memset(adj_offset, 0, sizeof(uint32_t) * (size_t)(N + 1));
+    for (int64_t t = 0; t < T; ++t)
+        for (int c = 0; c < 4; ++c) ++adj_offset[index[t * 4 + c] + 1];
+    for (int64_t n = 0; n < N; ++n) adj_offset[n + 1] += adj_offset[n];
+    uint32_t* cursor = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)N);
+    memcpy(cursor, adj_offset, sizeof(uint32_t) * (size_t)N);
+    for (int64_t t = 0; t < T; ++t)
+        for (uint32_t c = 0; c < 4; ++c) adj_entry[cursor[index[t * 4 + c]]++] = (uint32_t)t * 4 + c;
+    free(cursor);
+}
+
+double ora_edge_timestep(const double* X, const uint32_t* index, int64_t T, double E, double rho,
+                         double safety) {
+    double shortest = INFINITY;
+    for (int64_t t = 0; t < T; ++t) {
+        const uint32_t* idx = &index[t * 4];
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b) {
+                double d2 = 0;
+                for (int k = 0; k < 3; ++k) {
+                    const double d = X[idx[a] * 3 + k] - X[idx[b] * 3 + k];
+                    d2 += d * d;
+                }
+                if (sqrt(d2) < shortest) shortest = sqrt(d2);
+            }
+    }
+    return safety * shortest / sqrt(E / rho);
+}
+
+/* Cyclic Jacobi on a symmetric n x n (row-major), returns the largest eigenvalue. */
+static double jacobi_max_eig(double* a, int n) {
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, diag = 0;
+        for (int p = 0; p < n; ++p) {
+            diag += a[p * n + p] * a[p * n + p];
+            for (int q = p + 1; q < n; ++q) off += a[p * n + q] * a[p * n + q];
+        }
+        if (off <= 1e-30 * diag) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[p * n + q];
+                if (apq == 0.0) continue;
+                const double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - s * akq;
+                    a[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - s * aqk;
+                    a[q * n + k] = s * apk + c * aqk;
+                }
+            }
+    }
+    double m = a[0];
+    for (int p = 1; p < n; ++p)
+        if (a[p * n + p] > m) m = a[p * n + p];
+    return m;
+}
+
+double ora_max_element_omega(const double* rest_inv, int64_t T, double E, double nu, double rho) {
+    const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+    double best = 0;
+    for (int64_t t = 0; t < T; ++t) {
+        const double* di = &rest_inv[t * 9];
+        double g[4][3];
+        /* grad N_c = row (c-1) of Dm^-1 for c = 1..3, grad N_0 = -sum */
+        for (int c = 1; c < 4; ++c)
+            for (int a = 0; a < 3; ++a) g[c][a] = AT(di, c - 1, a);
+        for (int a = 0; a < 3; ++a) g[0][a] = -(g[1][a] + g[2][a] + g[3][a]);
+        /* A = B^T C B by columns: A e_(b,j) is the nodal force of the unit
+         * displacement of node b along j, per unit volume */
+        double A[144];
+        for (int b = 0; b < 4; ++b)
+            for (int j = 0; j < 3; ++j) {
+                double grad[3][3] = {{0}}; /* grad u (row i: component, col: derivative) */
+                for (int k = 0; k < 3; ++k) grad[j][k] = g[b][k];
+                double eps[3][3];
+                for (int p = 0; p < 3; ++p)
+                    for (int q = 0; q < 3; ++q) eps[p][q] = 0.5 * (grad[p][q] + grad[q][p]);
+                const double tr = eps[0][0] + eps[1][1] + eps[2][2];
+                double sig[3][3];
+                for (int p = 0; p < 3; ++p)
+                    for (int q = 0; q < 3; ++q) sig[p][q] = 2 * mu * eps[p][q] + (p == q ? lam * tr : 0);
+                for (int a = 0; a < 4; ++a)
+                    for (int i = 0; i < 3; ++i) {
+                        double f = 0;
+                        for (int k = 0; k < 3; ++k) f += sig[i][k] * g[a][k];
+                        A[(a * 3 + i) * 12 + (b * 3 + j)] = f;
+                    }
+            }
+        const double lmax = jacobi_max_eig(A, 12);
+        const double omega = sqrt(4.0 * lmax / rho);
+        if (omega > best) best = omega;
+    }
+    return best;
+}
+
+void ora_face_traction(const double* X, const uint32_t* index, int64_t T, int axis, double value,
+                       double tol, const double traction[3], double* load) {
+    static const int faces[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    for (int64_t t = 0; t < T; ++t) {
+        const uint32_t* idx = &index[t * 4];
+        for (int f = 0; f < 4; ++f) {
+            int on = 1;
+            for (int q = 0; q < 3; ++q)
+                if (fabs(X[idx[faces[f][q]] * 3 + axis] - value) > tol) on = 0;
+            if (!on) continue;
+            const double* a = &X[idx[faces[f][0]] * 3];
+            const double* b = &X[idx[faces[f][1]] * 3];
+            const double* c = &X[idx[faces[f][2]] * 3];
+            const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+            const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                  e1[0] * e2[1] - e1[1] * e2[0]};
+            const double area = 0.5 * sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            for (int q = 0; q < 3; ++q)
+                for (int k = 0; k < 3; ++k)
+                    load[idx[faces[f][q]] * 3 + k] += traction[k] * area / 3.0;
+        }
+    }
+}
+
+/* ======================================================================== */
+/* element: co-rotational force, elastic or J2                              */
+/* ======================================================================== */
+
+/* Stress from the co-rotated small strain (tensor eps, col-major), returning the
+ * symmetric stress tensor and, for J2, updating the point's plastic state. */
+static void stress_of(const double* eps, double lam, double mu, const ora_j2* j2, double* ep,
+                      double* eqps, double* sig, double* diss, double* energy_density) {
+    if (!j2) {
+        const double tr = AT(eps, 0, 0) + AT(eps, 1, 1) + AT(eps, 2, 2);
+        for (int k = 0; k < 9; ++k) sig[k] = 2.0 * mu * eps[k];
+        for (int i = 0; i < 3; ++i) AT(sig, i, i) += lam * tr;
+        if (diss) *diss = 0;
+        if (energy_density) {
+            double s = 0;
+            for (int k = 0; k < 9; ++k) s += sig[k] * eps[k];
+            *energy_density = 0.5 * s;
+        }
+        return;
+    }
+    const double strain[6] = {AT(eps, 0, 0), AT(eps, 1, 1), AT(eps, 2, 2),
+                              2 * AT(eps, 0, 1), 2 * AT(eps, 1, 2), 2 * AT(eps, 0, 2)};
+    double s6[6], d = 0;
+    ora_j2_update(j2, strain, ep, eqps, s6, &d);
+    if (diss) *diss = d;
+    AT(sig, 0, 0) = s6[0];
+    AT(sig, 1, 1) = s6[1];
+    AT(sig, 2, 2) = s6[2];
+    AT(sig, 0, 1) = AT(sig, 1, 0) = s6[3];
+    AT(sig, 1, 2) = AT(sig, 2, 1) = s6[4];
+    AT(sig, 0, 2) = AT(sig, 2, 0) = s6[5];
+    if (energy_density) {
+        /* 1/2 sigma : (eps - eps_p); engineering shears pair with tensor stress */
+        double s = 0;
+        for (int i = 0; i < 6; ++i) s += s6[i] * (strain[i] - ep[i]);
+        *energy_density = 0.5 * s;
+    }
+}
+
+/* Force of one tet from current node positions x (4 x xyz). Mirrors
+ * fem.cpp:73-117 in double, with the stress law pluggable. */
+static void tet_force_general(const double x[12], const double* dmi, double volume, double lam,
+                              double mu, const ora_j2* j2, double* ep, double* eqps, double out[12],
+                              double* diss, double* energy) {
+    double ds[9], f[9], r[9], rt[9], s[9], eps[9], sig[9], piola[9], dmit[9], h[9];
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) AT(ds, a, c) = x[(c + 1) * 3 + a] - x[a];
+    mul3(ds, dmi, f);
+    polar4(f, r);
+    transpose3(r, rt);
+    mul3(rt, f, s);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) AT(eps, i, j) = 0.5 * (AT(s, i, j) + AT(s, j, i));
+    for (int i = 0; i < 3; ++i) AT(eps, i, i) -= 1.0;
+    double ed = 0;
+    stress_of(eps, lam, mu, j2, ep, eqps, sig, diss, energy ? &ed : NULL);
+    if (diss) *diss *= volume;
+    if (energy) *energy = volume * ed;
+    mul3(r, sig, piola);
+    transpose3(dmi, dmit);
+    mul3(piola, dmit, h);
+    double f0[3] = {0, 0, 0};
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+            const double v = -volume * AT(h, a, c);
+            out[(c + 1) * 3 + a] = v;
+            f0[a] -= v;
+        }
+    for (int a = 0; a < 3; ++a) out[a] = f0[a];
+}
+
+void ora_tet_force(const double x[12], const double rest_inv[9], double volume, double lambda,
+                   double mu, double out[12]) {
+    tet_force_general(x, rest_inv, volume, lambda, mu, NULL, NULL, NULL, out, NULL, NULL);
+}
+
+int ora_j2_update(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
+                  double stress[6], double* dissipation) {
+    const double mu = m->E / (2.0 * (1.0 + m->nu));
+    const double kappa = m->E / (3.0 * (1.0 - 2.0 * m->nu));
+    const double root23 = sqrt(2.0 / 3.0);
+    double el[6];
+    for (int i = 0; i < 6; ++i) el[i] = strain[i] - eps_p[i];
+    const double vol = el[0] + el[1] + el[2];
+    const double p = kappa * vol;
+    double tr[6];
+    for (int i = 0; i < 3; ++i) {
+        tr[i] = 2.0 * mu * (el[i] - vol / 3.0);
+        tr[3 + i] = mu * el[3 + i];
+    }
+    const double norm = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
+                             2.0 * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
+    const double radius = root23 * (m->yield + m->hardening * (*eqps > 0 ? *eqps : 0.0));
+    const double excess = norm - radius;
+    if (!(excess > 0.0)) {
+        for (int i = 0; i < 3; ++i) {
+            stress[i] = p + tr[i];
+            stress[3 + i] = tr[3 + i];
+        }
+        if (dissipation) *dissipation = 0.0;
+        return 0;
+    }
+    /* linear hardening makes the consistency condition affine in dgamma: the
+     * Newton root of plasticity.cpp:370-413 in closed form */
+    const double gamma = excess / (2.0 * mu + (2.0 / 3.0) * m->hardening);
+    double n[6], dep[6];
+    for (int i = 0; i < 6; ++i) n[i] = tr[i] / norm;
+    for (int i = 0; i < 3; ++i) {
+        dep[i] = gamma * n[i];
+        dep[3 + i] = 2.0 * gamma * n[3 + i];
+        stress[i] = p + tr[i] - 2.0 * mu * gamma * n[i];
+        stress[3 + i] = tr[3 + i] - 2.0 * mu * gamma * n[3 + i];
+    }
+    double d = 0;
+    for (int i = 0; i < 6; ++i) d += stress[i] * dep[i];
+    if (dissipation) *dissipation = d;
+    for (int i = 0; i < 6; ++i) eps_p[i] += dep[i];
+    *eqps += root23 * gamma;
+    return 1;
+}
+
+/* ======================================================================== */
+/* explicit solver                                                          */
+/* ======================================================================== */
+
+int ora_case_init(ora_case* c) {
+    c->rest_inv = (double*)malloc(sizeof(double) * 9 * (size_t)c->T);
+    c->volume = (double*)malloc(sizeof(double) * (size_t)c->T);
+    c->mass = (double*)malloc(sizeof(double) * (size_t)c->N);
+    c->adj_offset = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(c->N + 1));
+    c->adj_entry = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (size_t)c->T);
+    if (!c->rest_inv || !c->volume || !c->mass || !c->adj_offset || !c->adj_entry) return 1;
+    ora_rest_state(c->X, c->N, c->index, c->T, c->rho, c->rest_inv, c->volume, c->mass,
+                   c->adj_offset, c->adj_entry);
+    return 0;
+}
+
+void ora_case_free(ora_case* c) {
+    free(c->rest_inv);
+    free(c->volume);
+    free(c->mass);
+    free(c->adj_offset);
+    free(c->adj_entry);
+    c->rest_inv = c->volume = c->mass = NULL;
+    c->adj_offset = c->adj_entry = NULL;
+}
+
+static double load_scale(const ora_case* c, double t) {
+    if (!(c->load_ramp > 0)) return 1.0;
+    const double s = t / c->load_ramp;
+    return s < 1.0 ? (s > 0.0 ? s : 0.0) : 1.0;
+}
+
+static double drive_fraction(const ora_case* c, double t) {
+    if (!(c->drive_ramp > 0)) return 1.0;
+    double s = t / c->drive_ramp;
+    s = s < 0 ? 0 : (s > 1 ? 1 : s);
+    return s * s * (3.0 - 2.0 * s);
+}
+
+void ora_run(const ora_case* c, ora_state* st, int64_t steps, int64_t sample_every,
+             ora_energy* rec, int64_t max_rec, int64_t* nrec) {
+    const double lam = c->E * c->nu / ((1 + c->nu) * (1 - 2 * c->nu));
+    const double mu = c->E / (2 * (1 + c->nu));
+    const ora_j2 j2 = {c->E, c->nu, c->yield, c->hardening};
+    const double dt = c->dt;
+    for (int64_t k = 0; k < steps; ++k) {
+        const int64_t n = st->step;
+        const double t = (double)n * dt;
+        const int sample = sample_every > 0 && n % sample_every == 0 && rec && nrec &&
+                           *nrec < max_rec;
+        /* element pass at u_n */
+        double strain_energy = 0, diss = 0;
+        for (int64_t e = 0; e < c->T; ++e) {
+            const uint32_t* idx = &c->index[e * 4];
+            double x[12];
+            for (int q = 0; q < 4; ++q)
+                for (int a = 0; a < 3; ++a)
+                    x[q * 3 + a] = c->X[idx[q] * 3 + a] + st->u[idx[q] * 3 + a];
+            double de = 0, ue = 0;
+            tet_force_general(x, &c->rest_inv[e * 9], c->volume[e], lam, mu,
+                              c->plastic ? &j2 : NULL, c->plastic ? &st->eps_p[e * 6] : NULL,
+                              c->plastic ? &st->eqps[e] : NULL, &st->force[e * 12], &de,
+                              sample ? &ue : NULL);
+            diss += de;
+            strain_energy += ue;
+        }
+        st->dissipation += diss;
+        /* node pass */
+        const double sl = load_scale(c, t);
+        const double sd = drive_fraction(c, t + dt);
+        double kinetic = 0, half_ext = 0, half_con = 0, fcon[3] = {0, 0, 0};
+        for (int64_t i = 0; i < c->N; ++i) {
+            const uint32_t fl = c->flags ? c->flags[i] : 0u;
+            const double m = c->mass[i];
+            if ((fl & (ORA_PIN_X | ORA_PIN_Y | ORA_PIN_Z)) == (ORA_PIN_X | ORA_PIN_Y | ORA_PIN_Z)) {
+                for (int a = 0; a < 3; ++a) st->v[i * 3 + a] = 0.0;
+                continue;
+            }
+            double f[3] = {0, 0, 0};
+            for (uint32_t q = c->adj_offset[i]; q < c->adj_offset[i + 1]; ++q) {
+                const uint32_t en = c->adj_entry[q];
+                const double* s = &st->force[(en >> 2) * 12 + (en & 3u) * 3];
+                for (int a = 0; a < 3; ++a) f[a] += s[a];
+            }
+            double fe[3] = {0, 0, -c->gravity * m};
+            if (c->load)
+                for (int a = 0; a < 3; ++a) fe[a] += sl * c->load[i * 3 + a];
+            double fc[3] = {0, 0, 0};
+            if (c->indenter && (fl & ORA_CONTACT)) {
+                double d[3], d2 = 0;
+                for (int a = 0; a < 3; ++a) {
+                    d[a] = c->X[i * 3 + a] + st->u[i * 3 + a] -
+                           (c->ind_center[a] + t * c->ind_velocity[a]);
+                    d2 += d[a] * d[a];
+                }
+                const double dist = sqrt(d2);
+                if (dist < c->ind_radius && dist > 0) {
+                    const double g = c->ind_radius - dist;
+                    const double k = c->ind_kappa * m / (dt * dt);
+                    for (int a = 0; a < 3; ++a) fc[a] = k * g * d[a] / dist;
+                }
+            }
+            if (sample)
+                for (int a = 0; a < 3; ++a) fcon[a] += fc[a];
+            for (int a = 0; a < 3; ++a) {
+                double* v = &st->v[i * 3 + a];
+                if (fl & (ORA_PIN_X << a)) {
+                    *v = 0.0;
+                    continue;
+                }
+                const double vo = *v;
+                double vn;
+                if (c->prescribed && (fl & (ORA_DRIVE_X << a))) {
+                    const double vfree = vo + (f[a] + fe[a] + fc[a]) / m * dt;
+                    vn = (c->prescribed[i * 3 + a] * sd - st->u[i * 3 + a]) / dt;
+                    st->w_boundary += m * (vn - vfree) * vn;
+                } else {
+                    vn = vo + (f[a] + fe[a] + fc[a]) / m * dt;
+                    if (c->damping != 1.0) {
+                        st->damping_loss += 0.5 * m * vn * vn * (1.0 - c->damping * c->damping);
+                        vn *= c->damping;
+                    }
+                }
+                st->w_ext += fe[a] * (vo + vn) * 0.5 * dt;
+                st->w_contact += fc[a] * (vo + vn) * 0.5 * dt;
+                if (sample) {
+                    half_ext += fe[a] * vn * 0.5 * dt;
+                    half_con += fc[a] * vn * 0.5 * dt;
+                    kinetic += 0.5 * m * vo * vn;
+                }
+                *v = vn;
+                st->u[i * 3 + a] += vn * dt;
+            }
+        }
+        if (sample) {
+            ora_energy* r = &rec[(*nrec)++];
+            r->step = n;
+            r->time = t;
+            r->strain = strain_energy;
+            r->kinetic = kinetic;
+            r->dissipation = st->dissipation;
+            r->w_ext = st->w_ext - half_ext;
+            r->w_contact = st->w_contact - half_con;
+            r->w_boundary = st->w_boundary;
+            r->damping_loss = st->damping_loss;
+            for (int a = 0; a < 3; ++a) r->contact_force[a] = fcon[a];
+        }
+        st->step = n + 1;
+    }
+}
+
+/* ======================================================================== */
+/* FP32 reference restatement: the float operation order of fem.cpp          */
+/* ======================================================================== */
+typedef struct {
+    float m[9];
+} M3f;
+#define AF(M, r, c) ((M).m[(c) * 3 + (r)])
+
+static M3f f_mul(M3f a, M3f b) {
+    M3f r;
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 3; ++i) {
+            float s = 0;
+            for (int k = 0; k < 3; ++k) s += AF(a, i, k) * AF(b, k, c);
+            AF(r, i, c) = s;
+        }
+    return r;
+}
+static M3f f_tr(M3f a) {
+    M3f r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) AF(r, i, j) = AF(a, j, i);
+    return r;
+}
+static float f_det(M3f a) {
+    return AF(a, 0, 0) * (AF(a, 1, 1) * AF(a, 2, 2) - AF(a, 1, 2) * AF(a, 2, 1)) -
+           AF(a, 0, 1) * (AF(a, 1, 0) * AF(a, 2, 2) - AF(a, 1, 2) * AF(a, 2, 0)) +
+           AF(a, 0, 2) * (AF(a, 1, 0) * AF(a, 2, 1) - AF(a, 1, 1) * AF(a, 2, 0));
+}
+static M3f f_inv(M3f a) {
+    const float det = f_det(a);
+    const float id = fabsf(det) > 1e-30f ? 1.0f / det : 0.0f;
+    M3f r;
+    AF(r, 0, 0) = (AF(a, 1, 1) * AF(a, 2, 2) - AF(a, 1, 2) * AF(a, 2, 1)) * id;
+    AF(r, 0, 1) = (AF(a, 0, 2) * AF(a, 2, 1) - AF(a, 0, 1) * AF(a, 2, 2)) * id;
+    AF(r, 0, 2) = (AF(a, 0, 1) * AF(a, 1, 2) - AF(a, 0, 2) * AF(a, 1, 1)) * id;
+    AF(r, 1, 0) = (AF(a, 1, 2) * AF(a, 2, 0) - AF(a, 1, 0) * AF(a, 2, 2)) * id;
+    AF(r, 1, 1) = (AF(a, 0, 0) * AF(a, 2, 2) - AF(a, 0, 2) * AF(a, 2, 0)) * id;
+    AF(r, 1, 2) = (AF(a, 0, 2) * AF(a, 1, 0) - AF(a, 0, 0) * AF(a, 1, 2)) * id;
+    AF(r, 2, 0) = (AF(a, 1, 0) * AF(a, 2, 1) - AF(a, 1, 1) * AF(a, 2, 0)) * id;
+    AF(r, 2, 1) = (AF(a, 0, 1) * AF(a, 2, 0) - AF(a, 0, 0) * AF(a, 2, 1)) * id;
+    AF(r, 2, 2) = (AF(a, 0, 0) * AF(a, 1, 1) - AF(a, 0, 1) * AF(a, 1, 0)) * id;
+    return r;
+}
+
+void ref32_rest_state(float* position, int64_t N, const uint32_t* index, int64_t T, float density,
+                      float* rest_inv, float* volume, float* mass, uint32_t* adj_offset,
+                      uint32_t* adj_entry) {
+    for (int64_t n = 0; n < N; ++n) mass[n] = 0.0f;
+    for (int64_t t = 0; t < T; ++t) {
+        const uint32_t* idx = &index[t * 4];
+        M3f dm;
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a)
+                AF(dm, a, c) = position[idx[c + 1] * 3 + a] - position[idx[0] * 3 + a];
+        const float vol = f_det(dm) / 6.0f;
+        volume[t] = fabsf(vol);
+        const M3f inv = f_inv(dm);
+        for (int k = 0; k < 9; ++k) rest_inv[t * 9 + k] = inv.m[k];
+        const float quarter = density * volume[t] * 0.25f;
+        for (int c = 0; c < 4; ++c) mass[idx[c]] += quarter;
+    }
+    memset(adj_offset, 0, sizeof(uint32_t) * (size_t)(N + 1));
+    for (int64_t t = 0; t < T; ++t)
+        for (int c = 0; c < 4; ++c) ++adj_offset[index[t * 4 + c] + 1];
+    for (int64_t n = 0; n < N; ++n) adj_offset[n + 1] += adj_offset[n];
+    uint32_t* cursor = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)N);
+    memcpy(cursor, adj_offset, sizeof(uint32_t) * (size_t)N);
+    for (int64_t t = 0; t < T; ++t)
+        for (uint32_t c = 0; c < 4; ++c) adj_entry[cursor[index[t * 4 + c]]++] = (uint32_t)t * 4 + c;
+    free(cursor);
+}
+
+void ref32_tet_forces(const float* p, const uint32_t* index, const float* rest_inv,
+                      const float* volume, int64_t t, float lambda, float mu, float out[12]) {
+    const uint32_t* idx = &index[t * 4];
+    M3f ds, dmi;
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) AF(ds, a, c) = p[idx[c + 1] * 3 + a] - p[idx[0] * 3 + a];
+    for (int k = 0; k < 9; ++k) dmi.m[k] = rest_inv[t * 9 + k];
+    const M3f f = f_mul(ds, dmi);
+    M3f r = f;
+    for (int i = 0; i < 4; ++i) {
+        const M3f rit = f_tr(f_inv(r));
+        for (int k = 0; k < 9; ++k) r.m[k] = 0.5f * (r.m[k] + rit.m[k]);
+    }
+    const M3f s = f_mul(f_tr(r), f);
+    M3f strain;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) AF(strain, i, j) = 0.5f * (AF(s, i, j) + AF(s, j, i));
+    for (int i = 0; i < 3; ++i) AF(strain, i, i) -= 1.0f;
+    const float trace = AF(strain, 0, 0) + AF(strain, 1, 1) + AF(strain, 2, 2);
+    M3f stress;
+    for (int k = 0; k < 9; ++k) stress.m[k] = 2.0f * mu * strain.m[k];
+    for (int i = 0; i < 3; ++i) AF(stress, i, i) += lambda * trace;
+    const M3f piola = f_mul(r, stress);
+    const M3f h = f_mul(piola, f_tr(dmi));
+    const float scale = -volume[t];
+    float f0[3] = {0, 0, 0};
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+            const float v = scale * AF(h, a, c);
+            out[(c + 1) * 3 + a] = v;
+            f0[a] -= v;
+        }
+    for (int a = 0; a < 3; ++a) out[a] = f0[a];
+}
+
+static void ref32_lame(float E, float nu, float* lam, float* mu) {
+    *lam = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    *mu = E / (2.0f * (1.0f + nu));
+}
+
+typedef struct {
+    float *position, *velocity, *tet_force;
+    const float *mass, *rest_inv, *volume;
+    const uint32_t *fixed, *index, *adj_offset, *adj_entry;
+    float lambda, mu, dt, gravity, damping;
+    int64_t lo, hi; /* element or node range */
+} ref32_job;
+
+static void ref32_elements(const ref32_job* j) {
+    for (int64_t t = j->lo; t < j->hi; ++t)
+        ref32_tet_forces(j->position, j->index, j->rest_inv, j->volume, t, j->lambda, j->mu,
+                         &j->tet_force[t * 12]);
+}
+
+static void ref32_nodes(const ref32_job* j) {
+    for (int64_t n = j->lo; n < j->hi; ++n) {
+        if (j->fixed[n]) {
+            for (int k = 0; k < 3; ++k) j->velocity[n * 3 + k] = 0.0f;
+            continue;
+        }
+        float force[3] = {0.0f, 0.0f, -j->gravity * j->mass[n]};
+        for (uint32_t e = j->adj_offset[n]; e < j->adj_offset[n + 1]; ++e) {
+            const uint32_t entry = j->adj_entry[e];
+            const float* f = &j->tet_force[(entry >> 2) * 12 + (entry & 3u) * 3];
+            for (int k = 0; k < 3; ++k) force[k] += f[k];
+        }
+        const float inv_mass = 1.0f / j->mass[n];
+        for (int k = 0; k < 3; ++k) {
+            float v = j->velocity[n * 3 + k] + force[k] * inv_mass * j->dt;
+            v *= j->damping;
+            j->velocity[n * 3 + k] = v;
+            j->position[n * 3 + k] += v * j->dt;
+        }
+    }
+}
+
+void ref32_step(float* position, float* velocity, const float* mass, const uint32_t* fixed,
+                int64_t N, const uint32_t* index, const float* rest_inv, const float* volume,
+                int64_t T, float* tet_force, const uint32_t* adj_offset,
+                const uint32_t* adj_entry, float E, float nu, float dt, float gravity,
+                float damping) {
+    ref32_job j = {position, velocity, tet_force, mass, rest_inv, volume, fixed, index,
+                   adj_offset, adj_entry, 0, 0, dt, gravity, damping, 0, T};
+    ref32_lame(E, nu, &j.lambda, &j.mu);
+    ref32_elements(&j);
+    j.lo = 0;
+    j.hi = N;
+    ref32_nodes(&j);
+}
+
+static void* ref32_elem_thread(void* p) {
+    ref32_elements((const ref32_job*)p);
+    return NULL;
+}
+static void* ref32_node_thread(void* p) {
+    ref32_nodes((const ref32_job*)p);
+    return NULL;
+}
+
+void ref32_step_mt(float* position, float* velocity, const float* mass, const uint32_t* fixed,
+                   int64_t N, const uint32_t* index, const float* rest_inv, const float* volume,
+                   int64_t T, float* tet_force, const uint32_t* adj_offset,
+                   const uint32_t* adj_entry, float E, float nu, float dt, float gravity,
+                   float damping, int threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    ref32_job jobs[256];
+    ref32_job base = {position, velocity, tet_force, mass, rest_inv, volume, fixed, index,
+                      adj_offset, adj_entry, 0, 0, dt, gravity, damping, 0, 0};
+    ref32_lame(E, nu, &base.lambda, &base.mu);
+    /* element pass: each element writes only its own slot */
+    for (int w = 0; w < threads; ++w) {
+        jobs[w] = base;
+        jobs[w].lo = T * w / threads;
+        jobs[w].hi = T * (w + 1) / threads;
+        pthread_create(&tid[w], NULL, ref32_elem_thread, &jobs[w]);
+    }
+    for (int w = 0; w < threads; ++w) pthread_join(tid[w], NULL);
+    /* node pass: each node gathers in the fixed CSR order */
+    for (int w = 0; w < threads; ++w) {
+        jobs[w] = base;
+        jobs[w].lo = N * w / threads;
+        jobs[w].hi = N * (w + 1) / threads;
+        pthread_create(&tid[w], NULL, ref32_node_thread, &jobs[w]);
+    }
+    for (int w = 0; w < threads; ++w) pthread_join(tid[w], NULL);
+}

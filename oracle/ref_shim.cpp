@@ -1,0 +1,115 @@
+// ref_shim.cpp -- extern "C" entry points over the reference's own CPU FEM and J2
+// point law, so tests/golden/make_golden.py can record golden vectors from the
+// compiled reference. TEST INFRASTRUCTURE ONLY.
+//
+// Built by oracle/Makefile together with the reference sources where they lie:
+//   engine/sim/fem.cpp         (makeBoxTetMesh, computeRestState, tetForces, stepCpu)
+//   engine/sim/plasticity.cpp  (update)
+// into oracle/_ref/libref.so. Nothing here restates the reference; it only moves
+// arrays in and out of sim::fem::TetMesh and sim::plasticity::State.
+#include "engine/sim/fem.hpp"
+#include "engine/sim/plasticity.hpp"
+
+#include <cstdint>
+#include <cstring>
+
+namespace {
+
+sim::fem::TetMesh meshFrom(const float* position, int64_t N, const uint32_t* index, int64_t T) {
+    sim::fem::TetMesh mesh;
+    mesh.position.assign(position, position + N * 3);
+    mesh.index.assign(index, index + T * 4);
+    return mesh;
+}
+
+sim::fem::Material materialOf(float E, float nu, float rho) {
+    sim::fem::Material m;
+    m.youngsModulus = E;
+    m.poissonRatio = nu;
+    m.density = rho;
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ref_box_counts(int nx, int ny, int nz, int64_t* nodes, int64_t* tets) {
+    const sim::fem::TetMesh m = sim::fem::makeBoxTetMesh(1.0f, 1.0f, 1.0f, nx, ny, nz);
+    *nodes = static_cast<int64_t>(m.position.size() / 3);
+    *tets = static_cast<int64_t>(m.index.size() / 4);
+}
+
+void ref_box_mesh(float lx, float ly, float lz, int nx, int ny, int nz, float* position,
+                  uint32_t* index) {
+    const sim::fem::TetMesh m = sim::fem::makeBoxTetMesh(lx, ly, lz, nx, ny, nz);
+    std::memcpy(position, m.position.data(), m.position.size() * sizeof(float));
+    std::memcpy(index, m.index.data(), m.index.size() * sizeof(uint32_t));
+}
+
+// computeRestState on the given positions; also returns criticalTimestep(safety).
+float ref_rest_state(const float* position, int64_t N, const uint32_t* index, int64_t T, float E,
+                     float nu, float rho, float safety, float* rest_inv, float* volume,
+                     float* mass, uint32_t* adj_offset, uint32_t* adj_entry) {
+    sim::fem::TetMesh m = meshFrom(position, N, index, T);
+    const sim::fem::Material mat = materialOf(E, nu, rho);
+    m.computeRestState(mat);
+    std::memcpy(rest_inv, m.restInv.data(), m.restInv.size() * sizeof(float));
+    std::memcpy(volume, m.restVolume.data(), m.restVolume.size() * sizeof(float));
+    std::memcpy(mass, m.mass.data(), m.mass.size() * sizeof(float));
+    std::memcpy(adj_offset, m.adjacencyOffset.data(), m.adjacencyOffset.size() * sizeof(uint32_t));
+    std::memcpy(adj_entry, m.adjacencyEntry.data(), m.adjacencyEntry.size() * sizeof(uint32_t));
+    return sim::fem::criticalTimestep(m, mat, safety);
+}
+
+// Rest state from `rest`, then positions replaced by `current` (the fem_spike
+// deform-after-rest pattern, tools/fem_spike/main.cpp:147-159), then tetForces for
+// every tet into out[12T].
+void ref_tet_forces(const float* rest, const float* current, int64_t N, const uint32_t* index,
+                    int64_t T, float E, float nu, float rho, float* out) {
+    sim::fem::TetMesh m = meshFrom(rest, N, index, T);
+    const sim::fem::Material mat = materialOf(E, nu, rho);
+    m.computeRestState(mat);
+    std::memcpy(m.position.data(), current, static_cast<size_t>(N) * 3 * sizeof(float));
+    for (int64_t t = 0; t < T; ++t)
+        sim::fem::tetForces(m, static_cast<size_t>(t), mat.lameLambda(), mat.lameMu(), &out[t * 12]);
+}
+
+// Rest state from `rest`, pins from `fixed`, positions := current, velocity :=
+// velocity; `steps` calls of stepCpu; positions and velocities copied back.
+void ref_step(const float* rest, float* position, float* velocity, const uint32_t* fixed, int64_t N,
+              const uint32_t* index, int64_t T, float E, float nu, float rho, float dt,
+              float gravity, float damping, int steps) {
+    sim::fem::TetMesh m = meshFrom(rest, N, index, T);
+    const sim::fem::Material mat = materialOf(E, nu, rho);
+    m.computeRestState(mat);
+    m.fixed.assign(fixed, fixed + N);
+    std::memcpy(m.position.data(), position, static_cast<size_t>(N) * 3 * sizeof(float));
+    std::memcpy(m.velocity.data(), velocity, static_cast<size_t>(N) * 3 * sizeof(float));
+    for (int s = 0; s < steps; ++s) sim::fem::stepCpu(m, mat, dt, gravity, damping);
+    std::memcpy(position, m.position.data(), static_cast<size_t>(N) * 3 * sizeof(float));
+    std::memcpy(velocity, m.velocity.data(), static_cast<size_t>(N) * 3 * sizeof(float));
+}
+
+// plasticity::update with a linear hardening curve, no kinematic hardening and no
+// failure. State in/out; returns the number of consistency evaluations.
+int ref_j2_update(double E, double nu, double yield, double hardening, const double strain[6],
+                  double eps_p[6], double* eqps, double stress[6], double* dissipation,
+                  int* yielded) {
+    sim::plasticity::Material mat;
+    mat.youngsModulus = E;
+    mat.poissonRatio = nu;
+    mat.flow = sim::plasticity::linearHardening(yield, hardening);
+    sim::plasticity::State st;
+    for (int i = 0; i < 6; ++i) st.plasticStrain[i] = eps_p[i];
+    st.equivalentPlasticStrain = *eqps;
+    const sim::plasticity::Increment inc =
+        sim::plasticity::update(mat, sim::plasticity::kNeverFails, strain, st, stress);
+    for (int i = 0; i < 6; ++i) eps_p[i] = st.plasticStrain[i];
+    *eqps = st.equivalentPlasticStrain;
+    *dissipation = inc.dissipation;
+    *yielded = inc.yielded ? 1 : 0;
+    return inc.iterations;
+}
+
+}  // extern "C"
