@@ -1,0 +1,168 @@
+"""Benchmark of the explicit tet step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plate|cantilever|panel]
+
+Default workload: the large plain-path plate (10 x 4 x 0.02 m, 350x144x4 Kuhn cells,
+1,209,600 tets, nodes jittered +-5%, edges clamped, 200 kPa ramped over 5 ms,
+FP32 + Kahan state), one step = one element pass + one node pass over the whole
+mesh. Prints ONE JSON line with tet-steps/s, the HBM roofline of the dominant kernel
+(HIP events on the solver's stream) and the reference CPU path (the FP32 stepCpu
+restatement, bit-identical to the compiled reference) timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "sinking-simulator_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per tet-step and per node-step of the two-pass design, split by
+# kernel (element pass / node pass). Per tet: connectivity 16 + Dm^-1 36 + volume 4 +
+# force-slot write 48 (element) and force-slot read 48 + CSR entries 16 (node); plus
+# J2 state 28 read + 28 write (element) when plastic. Per node: displacement read by
+# the element pass 12 (element); CSR offset 4 + X 12 + 1/m 4 + load 12 + state read
+# 48 + state write 48 (node). Sum: 168 T + 140 N (+ 56 T plastic) per step.
+ELEM_B_T, ELEM_B_N, NODE_B_T, NODE_B_N, PLASTIC_B_T = 104, 12, 64, 128, 56
+REFERENCE_SUSTAINED = 470e6  # tet-steps/s, GTX 1070 Ti, 491,520 tets (docs/07-fem-spike-findings.md:89)
+
+
+def emit(obj):
+    print(json.dumps(obj), flush=True)
+
+
+def workload(name):
+    from shipfem import configs
+
+    if name == "plate":
+        return configs.plate(), ("plate 10x4x0.02 m, 350x144x4 Kuhn cells x6 tets, nodes jittered "
+                                 "+-5%, edges clamped, 200 kPa ramped over 5 ms, elastic")
+    if name == "cantilever":
+        return configs.cantilever(), ("cantilever 1.0x0.1x0.1 m, 40x4x4 Kuhn cells x6 tets, x=0 "
+                                      "clamped, 1 kN tip load ramped over 1 ms, elastic")
+    if name == "panel":
+        return configs.stiffened_panel(), ("stiffened panel 6x3 m, 12 mm plate + 4 flat bars "
+                                           "200x12 mm, J2 355 MPa / H 1 GPa, r=0.5 m sphere at 5 m/s")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def cpu_baseline(case, budget_s=6.0):
+    """The reference's FP32 CPU step (fem.cpp:187-215) restated bit-exactly, on the
+    same mesh, all host cores (static split) and one core; bounded sample."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    pinned = ((case.flags & 7) == 7).astype(np.uint32)
+    sim = O.Ref32(case.X.astype(np.float32), case.index, pinned)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    dt = 1e-7
+    out = {}
+    for label, th in (("all", threads), ("one", 1)):
+        sim.step(dt, threads=th)  # warm-up
+        t0 = time.perf_counter()
+        n = 0
+        while n < 50 and time.perf_counter() - t0 < budget_s / 2:
+            sim.step(dt, threads=th)
+            n += 1
+        out[label] = (case.T * n / (time.perf_counter() - t0), n)
+    return dict(value=out["all"][0], unit="tet-steps/s", cores=threads, kind="port",
+                sample=(f"reference stepCpu restated in C (FP32, -O2 -ffp-contract=off, bit-identical "
+                        f"to the compiled reference on golden vectors) on the same {case.T}-tet mesh: "
+                        f"{out['all'][1]} steps on {threads} threads; one core "
+                        f"{out['one'][0]:.3e} tet-steps/s over {out['one'][1]} steps"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="plate")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} refused); cases shard across GPUs with no communication",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import shipfem as sf
+
+    info = sf.device_info()
+    if info is None:
+        print("no usable gfx950 device", file=sys.stderr)
+        sys.exit(1)
+    case, desc = workload(args.workload)
+    T, N = case.T, case.N
+    solver = sf.Solver(case)
+    ts = solver.timestep()
+    solver.run(args.warmup)
+    t0 = time.perf_counter()
+    dev_ms = solver.run(args.steps)
+    wall = time.perf_counter() - t0
+    st = solver.readback()
+    if st.status:
+        raise SystemExit("state went non-finite")
+    value = T * args.steps / wall
+    ms_per_step = wall * 1e3 / args.steps
+
+    elem_ms, node_ms = solver.profile(args.profile_steps)
+    plastic = bool(case.plastic)
+    elem_bytes = (ELEM_B_T + (PLASTIC_B_T if plastic else 0)) * T + ELEM_B_N * N
+    node_bytes = NODE_B_T * T + NODE_B_N * N
+    if elem_ms >= node_ms:
+        dom, dom_ms, dom_bytes = "element pass (k_elem)", elem_ms, elem_bytes
+    else:
+        dom, dom_ms, dom_bytes = "node pass (k_node)", node_ms, node_bytes
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(args.workload)
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=(traffic or {}).get(dom.split()[0]), kernel=dom,
+                    kernel_ms=round(dom_ms, 5), algorithmic_bytes=dom_bytes,
+                    elem_ms=round(elem_ms, 5), node_ms=round(node_ms, 5))
+    step_bytes = elem_bytes + node_bytes
+    step_gbs = step_bytes * args.steps / wall / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(case)
+        except Exception as e:  # reported, never fatal for the GPU number
+            cpu = dict(value=None, unit="tet-steps/s", cores=0, kind="port", sample=f"failed: {e}")
+
+    emit(dict(metric="tet-element-steps/s", value=value, unit="tet-steps/s", n_gpus=1,
+              steps=args.steps, warmup=args.warmup, ms_per_step=ms_per_step,
+              higher_is_better=True, scaling="weak",
+              vs_baseline=value / REFERENCE_SUSTAINED,
+              vs_baseline_basis="reference sustained 470e6 tet-steps/s (GTX 1070 Ti, Vulkan, FP32, "
+                                "491,520 tets; docs/07-fem-spike-findings.md:89)",
+              dtype="fp32", data="synthetic",
+              config=dict(workload=desc, tets=T, nodes=N, dt=ts["dt"], cfl=ts["cfl"],
+                          reference_edge_dt=ts["edge_dt"], device=info[0],
+                          device_ms_per_step=dev_ms / args.steps, step_hbm_gbs=round(step_gbs, 1),
+                          step_roofline_frac=round(step_gbs / HBM_PEAK_GBS, 4),
+                          algorithmic_bytes_per_step=step_bytes),
+              roofline=roofline, cpu_baseline=cpu))
+
+
+if __name__ == "__main__":
+    main()

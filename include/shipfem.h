@@ -1,0 +1,308 @@
+/*
+ * shipfem.h -- C ABI of the MI355X explicit tetrahedral FEM step.
+ *
+ * The drop-in boundary for the reference's Vulkan tet solver
+ * (engine/gpu/fem_gpu.hpp:17-45, class gpu::FemGpuSolver) and the CPU set-up
+ * functions it consumes (engine/sim/fem.hpp:38-86). Plain C: opaque handles,
+ * plain pointers and sizes, status codes, no exceptions, no torch types.
+ *
+ * Units: SI. Node arrays are xyz-interleaved per node ([3N]) at the boundary, as
+ * in sim::fem::TetMesh (engine/sim/fem.hpp:38-57); the library stores SoA planes on
+ * the device. Tet connectivity is [4T] node ids.
+ *
+ * Threading: a solver, batch or mesh handle is used by one host thread at a time
+ * (the reference solver is single-threaded and blocking, fem_gpu.cpp:117-125).
+ * A server handle owns its own submission thread; its functions are thread-safe.
+ *
+ * Errors: every entry point returns a shipfem_status. SHIPFEM_NO_DEVICE is the
+ * "degrade, not abort" case of fem_gpu.hpp:21-22: callers skip, they do not fail.
+ */
+#ifndef SHIPFEM_H
+#define SHIPFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHIPFEM_ABI_VERSION 1
+
+typedef enum shipfem_status {
+    SHIPFEM_OK = 0,
+    SHIPFEM_NO_DEVICE = 1,      /* no usable gfx950 device: skip, do not fail */
+    SHIPFEM_INVALID_ARG = 2,    /* refused input, message says which */
+    SHIPFEM_OUT_OF_MEMORY = 3,  /* device or host allocation failed */
+    SHIPFEM_UNSTABLE = 4,       /* a case produced a non-finite value */
+    SHIPFEM_DEVICE_ERROR = 5,   /* HIP runtime error */
+    SHIPFEM_BUSY = 6,           /* server: footprint exceeds the admission budget */
+    SHIPFEM_NOT_FOUND = 7,
+    SHIPFEM_UNSUPPORTED = 8
+} shipfem_status;
+
+const char* shipfem_status_string(shipfem_status s);
+
+/* Node flags (one uint32 per node). PIN_* hold a DOF at zero velocity, as the
+ * reference's per-node `fixed` (fem.cpp:197-200) does for all three; DRIVE_*
+ * prescribe a smoothstep-ramped displacement (zone.cpp:1080-1145); LOADED marks a
+ * node with a nonzero external load; CONTACT makes it a candidate for the indenter. */
+#define SHIPFEM_PIN_X 0x01u
+#define SHIPFEM_PIN_Y 0x02u
+#define SHIPFEM_PIN_Z 0x04u
+#define SHIPFEM_PIN_ALL 0x07u
+#define SHIPFEM_LOADED 0x08u
+#define SHIPFEM_CONTACT 0x10u
+#define SHIPFEM_DRIVE_X 0x20u
+#define SHIPFEM_DRIVE_Y 0x40u
+#define SHIPFEM_DRIVE_Z 0x80u
+
+/* sim::fem::Material (engine/sim/fem.hpp:24-36) plus the linear isotropic J2 flow
+ * curve of sim::plasticity (plasticity.hpp:110-128, Hardening::Linear). */
+typedef struct shipfem_material {
+    double youngs_modulus;    /* Pa */
+    double poisson_ratio;
+    double density;           /* kg/m^3 */
+    int32_t plastic;          /* 0: co-rotational linear elastic; 1: + J2 */
+    double yield_strength;    /* Pa, sigma_y(0) */
+    double hardening_modulus; /* Pa, H of sigma_y = sigma_y0 + H eps_p */
+} shipfem_material;
+
+/* Rigid sphere driven at constant velocity, node-to-surface penalty contact:
+ * f = penalty * m_i / dt^2 * gap * n for candidate nodes inside the sphere. */
+typedef struct shipfem_indenter {
+    int32_t enabled;
+    double radius;      /* m */
+    double center[3];   /* m, at t = 0 */
+    double velocity[3]; /* m/s */
+    double penalty;     /* dimensionless kappa; 0 => 0.1 */
+} shipfem_indenter;
+
+/* One case: mesh, material, boundary conditions, loads, contact and time step.
+ * Arrays are borrowed only for the duration of the create call. */
+typedef struct shipfem_case_desc {
+    uint32_t node_count, tet_count;
+    const double* rest_position;        /* [3N] rest coordinates X */
+    const uint32_t* index;              /* [4T] node ids per tet */
+    const uint32_t* node_flags;         /* [N] SHIPFEM_* or NULL */
+    const double* load;                 /* [3N] nodal force at full ramp, N; or NULL */
+    const double* prescribed;           /* [3N] displacement of DRIVE_* DOF; or NULL */
+    const double* initial_displacement; /* [3N] u at step 0 (x = X + u); or NULL */
+    const double* initial_velocity;     /* [3N] v_{-1/2}; or NULL */
+    shipfem_material material;
+    shipfem_indenter indenter;
+    double load_ramp;    /* s, load scale min(t/load_ramp, 1); 0 => full at once */
+    double drive_ramp;   /* s, smoothstep ramp of the prescribed drive; 0 => immediate */
+    double gravity;      /* m/s^2 along -z (fem.cpp:201) */
+    double damping;      /* velocity multiplier per step (fem.cpp:209-210); 0 => 1 */
+    double dt;           /* s; 0 => dt_safety * 2 / omega_max (FP64, on the device) */
+    double dt_safety;    /* 0 => 0.9 */
+    int32_t sample_every; /* energy-record period in steps, even; 0 => 100 */
+    int64_t max_steps;   /* a case stops advancing at this step; 0 => unbounded */
+} shipfem_case_desc;
+
+/* Per-call step parameters of gpu::FemGpuSolver::run (fem_gpu.hpp:31-32). */
+typedef struct shipfem_step_params {
+    double dt;
+    double gravity;
+    double damping;
+    double youngs_modulus, poisson_ratio; /* <= 0 keeps the case's */
+} shipfem_step_params;
+
+/* Energy account (FP64), modelled on zone.cpp:1014-1070. At a record for step n:
+ * strain = U(u_n), kinetic = 1/2 sum m v_{n-1/2} v_{n+1/2}, works integrated to
+ * t_n by the trapezoid rule, dissipation = sum V sigma : d eps_p through step n.
+ * balance residual = w_ext + w_contact + w_boundary - (strain + kinetic - E_0)
+ *                    - dissipation - damping_loss. */
+typedef struct shipfem_energy_record {
+    int64_t step;
+    double time;
+    double strain, kinetic, dissipation, w_ext, w_contact, w_boundary, damping_loss;
+    double contact_force[3]; /* sum of penalty forces on the structure at step n */
+} shipfem_energy_record;
+
+/* Readback destinations; any pointer may be NULL. */
+typedef struct shipfem_state_view {
+    double* displacement;  /* [3N] u (FP32 + compensation, widened) */
+    double* velocity;      /* [3N] v_{n-1/2} */
+    float* position;       /* [3N] X + u in float (the reference's readback, fem_gpu.cpp:424-428) */
+    float* eqps;           /* [T] equivalent plastic strain */
+    float* plastic_strain; /* [6T] Voigt, engineering shears, co-rotated frame */
+    int64_t step;          /* out */
+    double time;           /* out */
+    uint32_t status;       /* out: nonzero when the case went non-finite */
+} shipfem_state_view;
+
+/* ---- device -------------------------------------------------------------- */
+/* Name and HBM size of device 0; SHIPFEM_NO_DEVICE when there is none. */
+shipfem_status shipfem_device_query(char* name, size_t name_len, uint64_t* hbm_bytes);
+
+/* ---- meshes and case building (host; engine/sim/fem.cpp:119-263) ---------- */
+typedef struct shipfem_mesh shipfem_mesh;
+
+/* Kuhn box, reference numbering and tet order (makeBoxTetMesh, fem.cpp:217-263). */
+shipfem_status shipfem_mesh_box(double lx, double ly, double lz, int32_t nx, int32_t ny,
+                                int32_t nz, shipfem_mesh** out);
+/* Box mesh jitter: amp * spacing * U(-1,1) per axis from a counter-based seeded
+ * generator, except components normal to a boundary face. Refuses a jitter that
+ * inverts a tet. Only valid on meshes made by shipfem_mesh_box. */
+shipfem_status shipfem_mesh_jitter(shipfem_mesh* m, double amp, uint64_t seed);
+/* Stiffened panel: plate lx*ly*t_plate (z in [0,t]) with n_stiff flat bars
+ * web_h*web_t along x on z = t, as one conforming masked Kuhn grid. */
+shipfem_status shipfem_mesh_panel(double lx, double ly, double t_plate, int32_t n_stiff,
+                                  double web_h, double web_t, int32_t nx, int32_t ny_bay,
+                                  int32_t nz_web, shipfem_mesh** out);
+shipfem_status shipfem_mesh_counts(const shipfem_mesh* m, uint32_t* nodes, uint32_t* tets);
+/* Borrowed views of the mesh's own arrays (valid until the next call on m). */
+shipfem_status shipfem_mesh_arrays(const shipfem_mesh* m, const double** rest_position,
+                                   const uint32_t** index, const uint32_t** node_flags,
+                                   const double** load);
+/* OR `flags` into every node with |X[axis] - value| <= tol. */
+shipfem_status shipfem_mesh_flag_plane(shipfem_mesh* m, int32_t axis, double value, double tol,
+                                       uint32_t flags);
+/* OR `flags` into nodes with |X[axis]-value| <= tol within `radius` of (cx, cy)
+ * measured in the two other axes (used to mark indenter candidates). */
+shipfem_status shipfem_mesh_flag_disc(shipfem_mesh* m, int32_t axis, double value, double tol,
+                                      double c0, double c1, double radius, uint32_t flags);
+/* Consistent load of a uniform traction (N/m^2) on every tet face lying in the
+ * plane X[axis] == value: traction * area / 3 per face node; marks LOADED. */
+shipfem_status shipfem_mesh_traction(shipfem_mesh* m, int32_t axis, double value, double tol,
+                                     const double traction[3]);
+/* Fill the mesh fields of a case descriptor (pointers borrowed from m). */
+shipfem_status shipfem_mesh_fill_case(const shipfem_mesh* m, shipfem_case_desc* d);
+void shipfem_mesh_destroy(shipfem_mesh* m);
+
+/* Reference stable step: safety * shortest edge / sqrt(E/rho) (fem.cpp:160-185). */
+double shipfem_edge_timestep(const double* rest_position, const uint32_t* index,
+                             uint32_t tet_count, double youngs_modulus, double density,
+                             double safety);
+
+/* ---- one case (replaces gpu::FemGpuSolver, fem_gpu.hpp:17-45) -------------- */
+typedef struct shipfem_solver shipfem_solver;
+
+/* initialise (fem_gpu.hpp:23-24): computes the rest state (FP64 host, stored
+ * FP32), uploads SoA planes, evaluates the FP64 element-frequency time step on the
+ * device. */
+shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
+                                     char* err, size_t err_len);
+/* cfl = 2/omega_max (FP64 device min over elements), dt in use, and the
+ * reference's edge formula at safety 0.5 for comparison. */
+shipfem_status shipfem_solver_timestep(const shipfem_solver* s, double* cfl, double* dt,
+                                       double* edge_dt);
+/* Changes dt/gravity/damping/E/nu for the following runs (the per-call arguments
+ * of fem_gpu.hpp:31-32). */
+shipfem_status shipfem_solver_set_params(shipfem_solver* s, const shipfem_step_params* p);
+/* run (fem_gpu.hpp:31-32): advances `steps` steps and returns the device time in
+ * ms measured with HIP events on the solver's stream. mode 0 replays captured
+ * hipGraph chunks; mode 1 issues two plain launches per step, as the reference
+ * records two dispatches per step (fem_gpu.cpp:401-413). */
+shipfem_status shipfem_solver_run(shipfem_solver* s, int32_t steps, int32_t mode,
+                                  double* device_ms);
+/* readback (fem_gpu.hpp:34-35). */
+shipfem_status shipfem_solver_readback(shipfem_solver* s, shipfem_state_view* view);
+/* Energy records written so far (one per sample step). */
+shipfem_status shipfem_solver_energy(shipfem_solver* s, shipfem_energy_record* out,
+                                     int32_t max_records, int32_t* count);
+/* Per-tet nodal forces of the last element pass, reference layout [12T]
+ * (tetForce, fem.hpp:49-50). */
+shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out);
+/* Runs `steps` plain steps with HIP events around every launch on the solver's
+ * stream and returns the mean element-pass and node-pass durations in ms (the
+ * per-kernel split of the timings fem_gpu.cpp:388-421 takes around a whole run). */
+shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* elem_ms,
+                                      double* node_ms);
+/* Device bytes held by the solver. */
+shipfem_status shipfem_solver_footprint(const shipfem_solver* s, uint64_t* bytes);
+void shipfem_solver_destroy(shipfem_solver* s);
+
+/* Bytes a case of this shape will hold on the device (admission formula). */
+uint64_t shipfem_case_footprint(uint32_t node_count, uint32_t tet_count, int32_t plastic,
+                                int32_t sample_capacity);
+
+/* ---- many cases in one launch per phase ----------------------------------- */
+typedef struct shipfem_batch shipfem_batch;
+
+/* Launch shape. Any choice gives bit-identical results: reductions are fixed
+ * per 64-item wavefront tile of each case, independent of block size and order. */
+typedef struct shipfem_launch_config {
+    int32_t elem_block; /* threads per element-pass block: 64..1024, multiple of 64 */
+    int32_t node_block; /* threads per node-pass block */
+    int32_t xcd_remap;  /* 1: give each XCD a contiguous range of tiles */
+    int32_t chunk;      /* steps per captured graph (even) */
+} shipfem_launch_config;
+
+shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t case_count,
+                                    const shipfem_launch_config* cfg, shipfem_batch** out,
+                                    char* err, size_t err_len);
+/* All cases advance `steps` (cases that reach max_steps stop). */
+shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device_ms);
+/* Times candidate launch shapes on the batch itself (state is snapshotted and
+ * restored) and keeps the fastest; writes it to *best. */
+shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
+                                  shipfem_launch_config* best, double* best_ms_per_step);
+shipfem_status shipfem_batch_set_config(shipfem_batch* b, const shipfem_launch_config* cfg);
+shipfem_status shipfem_batch_readback(shipfem_batch* b, uint32_t case_index,
+                                      shipfem_state_view* view);
+shipfem_status shipfem_batch_energy(shipfem_batch* b, uint32_t case_index,
+                                    shipfem_energy_record* out, int32_t max_records,
+                                    int32_t* count);
+/* Restart every case from its initial state (for repeated timing). */
+shipfem_status shipfem_batch_reset(shipfem_batch* b);
+void shipfem_batch_destroy(shipfem_batch* b);
+
+/* ---- resident serving loop with continuous batching ---------------------- */
+typedef struct shipfem_server shipfem_server;
+
+typedef struct shipfem_server_opts {
+    uint64_t memory_budget;  /* device bytes for case state; 0 => 60% of HBM */
+    uint32_t max_active;     /* slots; 0 => 256 */
+    int32_t chunk;           /* steps per device round between admissions; 0 => 64 */
+    shipfem_launch_config launch; /* zeros => defaults */
+} shipfem_server_opts;
+
+typedef struct shipfem_result {
+    uint64_t request_id;
+    shipfem_status status;
+    int64_t steps;
+    double sim_time;
+    double submit_s, admit_s, done_s; /* host monotonic clock, seconds */
+    uint32_t node_count, tet_count;
+    double max_displacement;  /* max |u| over nodes, m */
+    double max_eqps;
+    double peak_contact_force; /* max over records of |sum contact force|, N */
+    shipfem_energy_record last; /* energy at the last record */
+} shipfem_result;
+
+typedef struct shipfem_server_stats {
+    uint64_t submitted, admitted, completed, failed, refused;
+    uint64_t active, queued;
+    uint64_t bytes_in_use, bytes_budget;
+    uint64_t rounds;
+    double element_steps; /* total tet-steps advanced */
+    double device_ms;     /* device time of all rounds */
+} shipfem_server_stats;
+
+shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_server** out,
+                                     char* err, size_t err_len);
+/* Copies the case; runs it to `run_to_time` seconds of simulated time (rounded up
+ * to whole steps). Refuses (SHIPFEM_BUSY) a case whose footprint exceeds the budget. */
+shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc* desc,
+                                     double run_to_time, uint64_t* request_id);
+/* Moves up to `max` completed results into out. Optionally keeps the final
+ * displacement of request ids the caller asked for with shipfem_server_keep. */
+shipfem_status shipfem_server_poll(shipfem_server* s, shipfem_result* out, uint32_t max,
+                                   uint32_t* count);
+/* Ask that the final state of request_id be retained for shipfem_server_fetch. */
+shipfem_status shipfem_server_keep(shipfem_server* s, uint64_t request_id);
+shipfem_status shipfem_server_fetch(shipfem_server* s, uint64_t request_id,
+                                    shipfem_state_view* view);
+/* Blocks until every submitted request has completed or `timeout_s` passes. */
+shipfem_status shipfem_server_drain(shipfem_server* s, double timeout_s);
+shipfem_status shipfem_server_stats_get(shipfem_server* s, shipfem_server_stats* out);
+double shipfem_now(void); /* the server's monotonic clock, seconds */
+void shipfem_server_destroy(shipfem_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHIPFEM_H */

@@ -1,0 +1,509 @@
+// api.cpp -- the single-case solver (gpu::FemGpuSolver's replacement) and the
+// many-case batch, behind the C ABI of include/shipfem.h.
+#include "shipfem_host.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+
+using namespace shipfem;
+
+namespace {
+
+const char* kStatusText[] = {"ok", "no usable device", "invalid argument", "out of memory",
+                             "unstable (non-finite state)", "device error", "busy",
+                             "not found", "unsupported"};
+
+// Advances all cases of `p` by `steps`, replaying the captured chunk when the
+// cases sit on a chunk boundary. Samples (energy records) fall on steps n with
+// n % S == 0, in both modes.
+hipError_t advance(Plan& p, int64_t& host_step, int64_t steps, int mode, int S, hipStream_t s) {
+    while (steps > 0) {
+        if (mode == 0 && p.parity == 0 && host_step % S == 0 && steps >= S) {
+            if (!p.graph || p.chunk != S) {
+                const hipError_t e = p.capture(S, s);
+                if (e != hipSuccess) return e;
+            }
+            const hipError_t e = hipGraphLaunch(p.graph, s);
+            if (e != hipSuccess) return e;
+            host_step += S;
+            steps -= S;
+        } else {
+            p.stepPlain(host_step % S == 0, s);
+            ++host_step;
+            --steps;
+        }
+    }
+    return hipGetLastError();
+}
+
+double omegaToCfl(double lambda_max, double rho) { return 2.0 / std::sqrt(4.0 * lambda_max / rho); }
+
+}  // namespace
+
+struct shipfem_solver {
+    HostCase hc;
+    DeviceCase dc;
+    Plan plan;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int64_t host_step = 0;
+    ~shipfem_solver() {
+        plan.dropGraph();
+        if (dc.owns && dc.arena) (void)hipFree(dc.arena);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+struct shipfem_batch {
+    std::vector<HostCase> hcs;
+    std::vector<DeviceCase> dcs;
+    Plan plan;
+    char* arena = nullptr;
+    char* backup = nullptr;
+    uint64_t bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int64_t host_step = 0;
+    int S = 100;
+    ~shipfem_batch() {
+        plan.dropGraph();
+        if (arena) (void)hipFree(arena);
+        if (backup) (void)hipFree(backup);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+const char* shipfem_status_string(shipfem_status s) {
+    const int i = static_cast<int>(s);
+    return (i >= 0 && i < static_cast<int>(sizeof(kStatusText) / sizeof(kStatusText[0]))) ? kStatusText[i]
+                                                                                            : "unknown";
+}
+
+shipfem_status shipfem_device_query(char* name, size_t name_len, uint64_t* hbm_bytes) {
+    std::string why;
+    if (!deviceAvailable(&why)) {
+        setError(name, name_len, why);
+        return SHIPFEM_NO_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return SHIPFEM_DEVICE_ERROR;
+    setError(name, name_len, std::string(prop.name) + " (" + prop.gcnArchName + ")");
+    if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    return SHIPFEM_OK;
+}
+
+uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t cap) {
+    // upper bound of arenaBytes: every optional plane present, 32-wide gather slab
+    const uint64_t Np = roundUp64(N), Tp = roundUp64(T), nw = Np / 64, ew = Tp / 64;
+    uint64_t b = 16ull * T + 40ull * Tp + 48ull * T + (plastic ? 28ull * Tp : 0) + 16ull * N +
+                 36ull * Np + 12ull * Np + 40ull * Np + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
+                 8ull * (3 * ew + 10 * nw) + 24 + sizeof(EnergyRecordDev) * (uint64_t)std::max(cap, 1);
+    return b + 256ull * 26;
+}
+
+shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
+                                     char* err, size_t err_len) {
+    if (!desc || !out) {
+        setError(err, err_len, "null argument");
+        return SHIPFEM_INVALID_ARG;
+    }
+    *out = nullptr;
+    std::string why;
+    if (!deviceAvailable(&why)) {
+        setError(err, err_len, why);
+        return SHIPFEM_NO_DEVICE;
+    }
+    auto* s = new (std::nothrow) shipfem_solver();
+    if (!s) return SHIPFEM_OUT_OF_MEMORY;
+    const std::string msg = prepareCase(*desc, s->hc);
+    if (!msg.empty()) {
+        setError(err, err_len, msg);
+        delete s;
+        return SHIPFEM_INVALID_ARG;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->e0);
+    if (e == hipSuccess) e = hipEventCreate(&s->e1);
+    const uint64_t bytes = arenaBytes(s->hc);
+    char* base = nullptr;
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&base), bytes);
+    if (e == hipSuccess) {
+        s->dc.owns = true;
+        e = uploadCase(s->hc, base, s->dc, s->stream);
+        if (e != hipSuccess) s->dc.arena = base;
+    } else if (base) {
+        (void)hipFree(base);
+    }
+    if (e == hipSuccess) {
+        s->plan.recs = {s->dc.rec};
+        e = s->plan.build(s->stream);
+    }
+    std::vector<double> lmax;
+    if (e == hipSuccess) e = s->plan.omegaMax(lmax, s->stream);
+    if (e != hipSuccess) {
+        setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
+        const shipfem_status st = statusOf(e);
+        delete s;
+        return st;
+    }
+    s->dc.omega = std::sqrt(4.0 * lmax[0] / s->hc.rho);
+    s->dc.cfl = omegaToCfl(lmax[0], s->hc.rho);
+    setTimestep(s->dc, s->hc, s->hc.dt_request > 0 ? s->hc.dt_request : s->hc.dt_safety * s->dc.cfl);
+    s->plan.recs[0] = s->dc.rec;
+    e = s->plan.uploadRecords(s->stream);
+    if (e != hipSuccess) {
+        setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
+        delete s;
+        return statusOf(e);
+    }
+    *out = s;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_solver_timestep(const shipfem_solver* s, double* cfl, double* dt,
+                                       double* edge_dt) {
+    if (!s) return SHIPFEM_INVALID_ARG;
+    if (cfl) *cfl = s->dc.cfl;
+    if (dt) *dt = s->dc.dt;
+    if (edge_dt) *edge_dt = s->hc.edge_dt;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_solver_set_params(shipfem_solver* s, const shipfem_step_params* p) {
+    if (!s || !p) return SHIPFEM_INVALID_ARG;
+    if (!(p->dt > 0) || !std::isfinite(p->gravity)) return SHIPFEM_INVALID_ARG;
+    const double damping = p->damping == 0.0 ? 1.0 : p->damping;
+    if (!(damping > 0 && damping <= 1.0)) return SHIPFEM_INVALID_ARG;
+    HostCase& hc = s->hc;
+    if (p->youngs_modulus > 0) hc.E = p->youngs_modulus;
+    if (p->poisson_ratio > 0 && p->poisson_ratio < 0.5) hc.nu = p->poisson_ratio;
+    hc.lam = hc.E * hc.nu / ((1 + hc.nu) * (1 - 2 * hc.nu));
+    hc.mu = hc.E / (2 * (1 + hc.nu));
+    hc.bulk = hc.E / (3 * (1 - 2 * hc.nu));
+    hc.gravity = p->gravity;
+    hc.damping = damping;
+    CaseDev& r = s->dc.rec;
+    r.lam = static_cast<float>(hc.lam);
+    r.mu = static_cast<float>(hc.mu);
+    r.bulk = static_cast<float>(hc.bulk);
+    r.lam_d = hc.lam;
+    r.mu_d = hc.mu;
+    r.gravity = static_cast<float>(hc.gravity);
+    r.damping = static_cast<float>(hc.damping);
+    hc.features = (hc.features & ~(kFeatGravity | kFeatDamp)) | (hc.gravity != 0 ? kFeatGravity : 0) |
+                  (hc.damping != 1.0 ? kFeatDamp : 0);
+    r.features = hc.features;
+    setTimestep(s->dc, hc, p->dt);
+    s->plan.recs[0] = r;
+    s->plan.feat_union = r.features;
+    s->plan.dropGraph();
+    return statusOf(s->plan.uploadRecords(s->stream));
+}
+
+shipfem_status shipfem_solver_run(shipfem_solver* s, int32_t steps, int32_t mode, double* device_ms) {
+    if (!s || steps < 0 || (mode != 0 && mode != 1)) return SHIPFEM_INVALID_ARG;
+    hipError_t e = hipEventRecord(s->e0, s->stream);
+    if (e == hipSuccess) e = advance(s->plan, s->host_step, steps, mode, s->hc.sample_every, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->e1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(s->e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->e0, s->e1);
+    if (e != hipSuccess) return statusOf(e);
+    if (device_ms) *device_ms = ms;
+    uint32_t status = 0;
+    e = hipMemcpy(&status, s->dc.rec.status, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return statusOf(e);
+    return status ? SHIPFEM_UNSTABLE : SHIPFEM_OK;
+}
+
+shipfem_status shipfem_solver_readback(shipfem_solver* s, shipfem_state_view* view) {
+    if (!s || !view) return SHIPFEM_INVALID_ARG;
+    return statusOf(readbackCase(s->dc, s->hc, view, s->stream));
+}
+
+shipfem_status shipfem_solver_energy(shipfem_solver* s, shipfem_energy_record* out, int32_t max,
+                                     int32_t* count) {
+    if (!s) return SHIPFEM_INVALID_ARG;
+    return statusOf(readEnergy(s->dc, out, max, count, s->stream));
+}
+
+shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out) {
+    if (!s || !out) return SHIPFEM_INVALID_ARG;
+    hipError_t e = hipMemcpyAsync(out, s->dc.rec.force, 48ull * s->hc.T, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return statusOf(e);
+}
+
+shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* elem_ms,
+                                      double* node_ms) {
+    if (!s || steps < 1) return SHIPFEM_INVALID_ARG;
+    Plan& p = s->plan;
+    std::vector<hipEvent_t> ev(3 * static_cast<size_t>(steps), nullptr);
+    hipError_t e = hipSuccess;
+    for (auto& x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    const int S = s->hc.sample_every;
+    for (int k = 0; k < steps && e == hipSuccess; ++k) {
+        const int sample = s->host_step % S == 0 ? 1 : 0;
+        e = hipEventRecord(ev[3 * k], s->stream);
+        launch_elem(p.d_recs, p.d_elem, static_cast<int>(p.elem_tiles.size()), p.shape.elem_block,
+                    p.parity, sample, p.shape.xcd_remap, p.feat_union, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 1], s->stream);
+        launch_node(p.d_recs, p.d_node, static_cast<int>(p.node_tiles.size()), p.shape.node_block,
+                    p.parity, sample, p.shape.xcd_remap, p.feat_union, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 2], s->stream);
+        if (sample) launch_reduce(p.d_recs, static_cast<int>(p.recs.size()), p.parity, s->stream);
+        p.parity ^= 1;
+        ++s->host_step;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    double te = 0, tn = 0;
+    for (int k = 0; k < steps && e == hipSuccess; ++k) {
+        float a = 0, b = 0;
+        e = hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
+        te += a;
+        tn += b;
+    }
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return statusOf(e);
+    if (elem_ms) *elem_ms = te / steps;
+    if (node_ms) *node_ms = tn / steps;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_solver_footprint(const shipfem_solver* s, uint64_t* bytes) {
+    if (!s || !bytes) return SHIPFEM_INVALID_ARG;
+    *bytes = s->dc.bytes;
+    return SHIPFEM_OK;
+}
+
+void shipfem_solver_destroy(shipfem_solver* s) {
+    if (s) (void)hipStreamSynchronize(s->stream);
+    delete s;
+}
+
+// ---------------------------------------------------------------- batch
+
+shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
+                                    const shipfem_launch_config* cfg, shipfem_batch** out,
+                                    char* err, size_t err_len) {
+    if (!cases || !out || n == 0) {
+        setError(err, err_len, "null argument or empty batch");
+        return SHIPFEM_INVALID_ARG;
+    }
+    *out = nullptr;
+    std::string why;
+    if (!deviceAvailable(&why)) {
+        setError(err, err_len, why);
+        return SHIPFEM_NO_DEVICE;
+    }
+    auto* b = new (std::nothrow) shipfem_batch();
+    if (!b) return SHIPFEM_OUT_OF_MEMORY;
+    b->hcs.resize(n);
+    b->dcs.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const std::string msg = prepareCase(cases[i], b->hcs[i]);
+        if (!msg.empty()) {
+            setError(err, err_len, "case " + std::to_string(i) + ": " + msg);
+            delete b;
+            return SHIPFEM_INVALID_ARG;
+        }
+    }
+    b->S = (cfg && cfg->chunk > 0) ? cfg->chunk : b->hcs[0].sample_every;
+    if (b->S < 2 || (b->S & 1)) {
+        setError(err, err_len, "chunk must be an even number >= 2");
+        delete b;
+        return SHIPFEM_INVALID_ARG;
+    }
+    if (cfg) {
+        b->plan.shape.elem_block = cfg->elem_block ? cfg->elem_block : 256;
+        b->plan.shape.node_block = cfg->node_block ? cfg->node_block : 256;
+        b->plan.shape.xcd_remap = cfg->xcd_remap;
+    }
+    std::vector<uint64_t> off(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) off[i + 1] = off[i] + (arenaBytes(b->hcs[i]) + 4095) / 4096 * 4096;
+    b->bytes = off[n];
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->e0);
+    if (e == hipSuccess) e = hipEventCreate(&b->e1);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->arena), b->bytes);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->backup), b->bytes);
+    for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
+        e = uploadCase(b->hcs[i], b->arena + off[i], b->dcs[i], b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);  // host planes stay valid
+    }
+    if (e == hipSuccess) {
+        b->plan.recs.clear();
+        for (auto& d : b->dcs) b->plan.recs.push_back(d.rec);
+        e = b->plan.build(b->stream);
+    }
+    std::vector<double> lmax;
+    if (e == hipSuccess) e = b->plan.omegaMax(lmax, b->stream);
+    if (e == hipSuccess) {
+        for (uint32_t i = 0; i < n; ++i) {
+            HostCase& hc = b->hcs[i];
+            DeviceCase& dc = b->dcs[i];
+            dc.omega = std::sqrt(4.0 * lmax[i] / hc.rho);
+            dc.cfl = omegaToCfl(lmax[i], hc.rho);
+            setTimestep(dc, hc, hc.dt_request > 0 ? hc.dt_request : hc.dt_safety * dc.cfl);
+            b->plan.recs[i] = dc.rec;
+        }
+        e = b->plan.uploadRecords(b->stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->backup, b->arena, b->bytes, hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
+        const shipfem_status st = statusOf(e);
+        delete b;
+        return st;
+    }
+    // the prepared host planes are no longer needed once on the device
+    for (auto& hc : b->hcs) {
+        std::vector<float>().swap(hc.dmi);
+        std::vector<uint32_t>().swap(hc.ell);
+        std::vector<uint32_t>().swap(hc.conn);
+    }
+    *out = b;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device_ms) {
+    if (!b || steps < 0) return SHIPFEM_INVALID_ARG;
+    hipError_t e = hipEventRecord(b->e0, b->stream);
+    if (e == hipSuccess) e = advance(b->plan, b->host_step, steps, 0, b->S, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(b->e1, b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b->e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->e0, b->e1);
+    if (e != hipSuccess) return statusOf(e);
+    if (device_ms) *device_ms = ms;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_batch_set_config(shipfem_batch* b, const shipfem_launch_config* cfg) {
+    if (!b || !cfg) return SHIPFEM_INVALID_ARG;
+    const int eb = cfg->elem_block, nb = cfg->node_block;
+    auto ok = [](int x) { return x == 64 || x == 128 || x == 256 || x == 512; };
+    if (!ok(eb) || !ok(nb)) return SHIPFEM_INVALID_ARG;
+    if (b->host_step % b->S != 0 || b->plan.parity != 0) return SHIPFEM_INVALID_ARG;
+    b->plan.shape.elem_block = eb;
+    b->plan.shape.node_block = nb;
+    b->plan.shape.xcd_remap = cfg->xcd_remap ? 1 : 0;
+    return statusOf(b->plan.build(b->stream));
+}
+
+shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
+                                  shipfem_launch_config* best, double* best_ms) {
+    if (!b || steps_per_trial < 1) return SHIPFEM_INVALID_ARG;
+    if (b->host_step % b->S != 0 || b->plan.parity != 0) return SHIPFEM_INVALID_ARG;
+    // snapshot the live state; every trial starts from it and it is restored at the end
+    char* snap = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&snap), b->bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(snap, b->arena, b->bytes, hipMemcpyDeviceToDevice, b->stream);
+    if (e != hipSuccess) {
+        if (snap) (void)hipFree(snap);
+        return statusOf(e);
+    }
+    const int64_t start = b->host_step;
+    const int trial = std::max<int>(b->S, (steps_per_trial + b->S - 1) / b->S * b->S);
+    auto timeShape = [&](const LaunchShape& sh, double& ms) -> hipError_t {
+        hipError_t r = hipMemcpyAsync(b->arena, snap, b->bytes, hipMemcpyDeviceToDevice, b->stream);
+        b->host_step = start;
+        b->plan.parity = 0;
+        b->plan.shape = sh;
+        if (r == hipSuccess) r = b->plan.build(b->stream);
+        if (r == hipSuccess) r = advance(b->plan, b->host_step, b->S, 0, b->S, b->stream);  // warm-up
+        if (r == hipSuccess) r = hipEventRecord(b->e0, b->stream);
+        if (r == hipSuccess) r = advance(b->plan, b->host_step, trial, 0, b->S, b->stream);
+        if (r == hipSuccess) r = hipEventRecord(b->e1, b->stream);
+        if (r == hipSuccess) r = hipEventSynchronize(b->e1);
+        float f = 0;
+        if (r == hipSuccess) r = hipEventElapsedTime(&f, b->e0, b->e1);
+        ms = f / trial;
+        return r;
+    };
+    // coordinate descent: XCD remap, then element block, then node block
+    LaunchShape cur = b->plan.shape;
+    double curMs = 0;
+    e = timeShape(cur, curMs);
+    const int blocks[] = {64, 128, 256, 512};
+    for (int pass = 0; pass < 3 && e == hipSuccess; ++pass) {
+        std::vector<LaunchShape> cands;
+        for (int v : (pass == 0 ? std::vector<int>{0, 1} : std::vector<int>(blocks, blocks + 4))) {
+            LaunchShape c = cur;
+            if (pass == 0) c.xcd_remap = v;
+            else if (pass == 1) c.elem_block = v;
+            else c.node_block = v;
+            cands.push_back(c);
+        }
+        for (const LaunchShape& c : cands) {
+            if (c.elem_block == cur.elem_block && c.node_block == cur.node_block &&
+                c.xcd_remap == cur.xcd_remap)
+                continue;
+            double ms = 0;
+            e = timeShape(c, ms);
+            if (e != hipSuccess) break;
+            if (ms < curMs) {
+                cur = c;
+                curMs = ms;
+            }
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(b->arena, snap, b->bytes, hipMemcpyDeviceToDevice, b->stream);
+    b->host_step = start;
+    b->plan.parity = 0;
+    b->plan.shape = cur;
+    if (e == hipSuccess) e = b->plan.build(b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(snap);
+    if (e != hipSuccess) return statusOf(e);
+    if (best) {
+        best->elem_block = cur.elem_block;
+        best->node_block = cur.node_block;
+        best->xcd_remap = cur.xcd_remap;
+        best->chunk = b->S;
+    }
+    if (best_ms) *best_ms = curMs;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_batch_readback(shipfem_batch* b, uint32_t i, shipfem_state_view* view) {
+    if (!b || !view || i >= b->dcs.size()) return SHIPFEM_INVALID_ARG;
+    return statusOf(readbackCase(b->dcs[i], b->hcs[i], view, b->stream));
+}
+
+shipfem_status shipfem_batch_energy(shipfem_batch* b, uint32_t i, shipfem_energy_record* out,
+                                    int32_t max, int32_t* count) {
+    if (!b || i >= b->dcs.size()) return SHIPFEM_INVALID_ARG;
+    return statusOf(readEnergy(b->dcs[i], out, max, count, b->stream));
+}
+
+shipfem_status shipfem_batch_reset(shipfem_batch* b) {
+    if (!b) return SHIPFEM_INVALID_ARG;
+    hipError_t e = hipMemcpyAsync(b->arena, b->backup, b->bytes, hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    b->host_step = 0;
+    b->plan.parity = 0;
+    return statusOf(e);
+}
+
+void shipfem_batch_destroy(shipfem_batch* b) {
+    if (b) (void)hipStreamSynchronize(b->stream);
+    delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// engine.cpp -- step plans: tile tables over one or many cases, plain launches and
+// hipGraph capture of a chunk of steps.
+#include "shipfem_host.hpp"
+
+#include <algorithm>
+
+namespace shipfem {
+
+namespace {
+
+template <typename T>
+hipError_t ensure(T*& p, size_t& cap, size_t need) {
+    if (need <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = std::max<size_t>(need, 1);
+    return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
+}
+
+void tilesFor(const std::vector<CaseDev>& recs, int block, bool nodes, std::vector<Tile>& out) {
+    out.clear();
+    for (size_t i = 0; i < recs.size(); ++i) {
+        const uint32_t count = nodes ? recs[i].N : recs[i].T;
+        for (uint32_t first = 0; first < count; first += static_cast<uint32_t>(block))
+            out.push_back(Tile{static_cast<int32_t>(i), first});
+    }
+}
+
+}  // namespace
+
+Plan::~Plan() {
+    dropGraph();
+    if (d_recs) (void)hipFree(d_recs);
+    if (d_elem) (void)hipFree(d_elem);
+    if (d_node) (void)hipFree(d_node);
+    if (d_omega) (void)hipFree(d_omega);
+    if (d_omega_out) (void)hipFree(d_omega_out);
+}
+
+void Plan::dropGraph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_src) (void)hipGraphDestroy(graph_src);
+    graph = nullptr;
+    graph_src = nullptr;
+    chunk = 0;
+}
+
+hipError_t Plan::build(hipStream_t s) {
+    dropGraph();
+    tilesFor(recs, shape.elem_block, false, elem_tiles);
+    tilesFor(recs, shape.node_block, true, node_tiles);
+    tilesFor(recs, 256, false, omega_tiles);
+    feat_union = 0;
+    for (const CaseDev& r : recs) feat_union |= r.features;
+    // tile arrays share one capacity counter each
+    static thread_local size_t dummy = 0;
+    (void)dummy;
+    hipError_t e = hipSuccess;
+    size_t need = std::max({elem_tiles.size(), node_tiles.size(), omega_tiles.size()});
+    if (need > tile_cap || !d_elem) {
+        if (d_elem) (void)hipFree(d_elem);
+        if (d_node) (void)hipFree(d_node);
+        if (d_omega) (void)hipFree(d_omega);
+        d_elem = d_node = d_omega = nullptr;
+        tile_cap = std::max<size_t>(need, 1);
+        e = hipMalloc(reinterpret_cast<void**>(&d_elem), sizeof(Tile) * tile_cap);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_node), sizeof(Tile) * tile_cap);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_omega), sizeof(Tile) * tile_cap);
+        if (e != hipSuccess) return e;
+    }
+    if (recs.size() > case_cap || !d_recs) {
+        if (d_recs) (void)hipFree(d_recs);
+        if (d_omega_out) (void)hipFree(d_omega_out);
+        d_recs = nullptr;
+        d_omega_out = nullptr;
+        case_cap = std::max<size_t>(recs.size(), 1);
+        e = hipMalloc(reinterpret_cast<void**>(&d_recs), sizeof(CaseDev) * case_cap);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_omega_out), sizeof(double) * case_cap);
+        if (e != hipSuccess) return e;
+    }
+    e = hipMemcpyAsync(d_elem, elem_tiles.data(), sizeof(Tile) * elem_tiles.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_node, node_tiles.data(), sizeof(Tile) * node_tiles.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_omega, omega_tiles.data(), sizeof(Tile) * omega_tiles.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = uploadRecords(s);
+    // host-side staging buffers must outlive the async copies
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+hipError_t Plan::uploadRecords(hipStream_t s) {
+    if (recs.empty()) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(d_recs, recs.data(), sizeof(CaseDev) * recs.size(),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+void Plan::stepPlain(bool sample, hipStream_t s) {
+    launch_elem(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, parity,
+                sample ? 1 : 0, shape.xcd_remap, feat_union, s);
+    launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, parity,
+                sample ? 1 : 0, shape.xcd_remap, feat_union, s);
+    if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), parity, s);
+    parity ^= 1;
+}
+
+hipError_t Plan::capture(int steps, hipStream_t s) {
+    dropGraph();
+    if (steps < 2 || (steps & 1)) return hipErrorInvalidValue;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    int par = 0;
+    for (int k = 0; k < steps; ++k) {
+        const int sample = k == 0 ? 1 : 0;
+        launch_elem(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
+                    sample, shape.xcd_remap, feat_union, s);
+        launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, par,
+                    sample, shape.xcd_remap, feat_union, s);
+        if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), par, s);
+        par ^= 1;
+    }
+    e = hipStreamEndCapture(s, &graph_src);
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&graph, graph_src, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph_src);
+        graph_src = nullptr;
+        graph = nullptr;
+        return e;
+    }
+    chunk = steps;
+    return hipSuccess;
+}
+
+hipError_t Plan::omegaMax(std::vector<double>& out, hipStream_t s) {
+    launch_omega(d_recs, d_omega, static_cast<int>(omega_tiles.size()), s);
+    launch_omega_max(d_recs, static_cast<int>(recs.size()), d_omega_out, s);
+    out.assign(recs.size(), 0.0);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out.data(), d_omega_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+}  // namespace shipfem

@@ -1,0 +1,656 @@
+// kernels.hip -- the explicit tet step on gfx950 (wave64).
+//
+// Per step, two passes over a tile table (tiles never span cases):
+//   element pass: one lane per tet. Gathers the four nodes' displacement (float4),
+//     forms F = I + G with G = Du Dm^-1, takes the rotation by Higham's iteration in
+//     perturbation form R = I + W (exactly four iterations, as fem.cpp:62-69), the
+//     co-rotated strain eps = sym(W^T + G + W^T G) (no cancellation against I in
+//     FP32), Hooke or a J2 radial return, P = R sigma, and writes the four nodal
+//     forces -V P Dm^-T to the tet's own 48-byte slot (fem.cpp:73-117).
+//   node pass: one lane per node. Gathers the node's slots in the fixed CSR order
+//     from a wave-transposed slab (coalesced index loads), adds ramped dead load,
+//     gravity and sphere penalty contact, applies pins and the prescribed drive,
+//     and advances v and u with Kahan compensation (fem.cpp:187-215, zone.cpp:1094-1200).
+// No atomics anywhere: energy terms are FP64 per-wave sums (xor butterfly, the same
+// order in every lane) accumulated into per-wave slots, summed in a fixed order by
+// the reduce kernel on sample steps.
+#include "shipfem_dev.hpp"
+
+namespace shipfem {
+namespace {
+
+__device__ __forceinline__ int remap_block(int b, int nb, int xcd) {
+    if (!xcd) return b;
+    // workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
+    // contiguous tile range [x q + min(x, r), ...) so its L2 sees neighbouring tets
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, j = b >> 3;
+    return x * q + (x < r ? x : r) + j;
+}
+
+__device__ __forceinline__ double wave_sum(double s) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+__device__ __forceinline__ double wave_max(double s) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off, 64));
+    return s;
+}
+
+// ---------------------------------------------------------------- element pass
+template <int BLOCK, bool PLASTIC>
+__global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ cases,
+                                                const Tile* __restrict__ tiles, int ntiles,
+                                                int parity, int sample, int xcd) {
+    const int b = remap_block(blockIdx.x, gridDim.x, xcd);
+    const Tile tile = tiles[b];
+    const CaseDev& c = cases[tile.case_index];
+    if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
+
+    const uint32_t e = tile.first + threadIdx.x;
+    const bool live = e < c.T;
+    double diss = 0.0, strain_energy = 0.0;
+
+    if (live) {
+        const uint4 nd = c.conn[e];
+        const float4 p0 = c.u4[nd.x], p1 = c.u4[nd.y], p2 = c.u4[nd.z], p3 = c.u4[nd.w];
+        const uint32_t tp = c.Tpad;
+        float B[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * tp + e];
+        const float V = c.vol[e];
+
+        // Du: columns are corner displacements relative to corner 0
+        const float D[9] = {p1.x - p0.x, p2.x - p0.x, p3.x - p0.x,
+                            p1.y - p0.y, p2.y - p0.y, p3.y - p0.y,
+                            p1.z - p0.z, p2.z - p0.z, p3.z - p0.z};
+        float G[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                G[i * 3 + j] = D[i * 3 + 0] * B[0 * 3 + j] + D[i * 3 + 1] * B[1 * 3 + j] +
+                               D[i * 3 + 2] * B[2 * 3 + j];
+
+        // R = I + W by R <- (R + R^-T)/2 with inv(I+W) - I = (W^2 - (1+t)W - det(W) I)/det(I+W)
+        float W[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) W[k] = G[k];
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            float W2[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    W2[i * 3 + j] = W[i * 3 + 0] * W[0 * 3 + j] + W[i * 3 + 1] * W[1 * 3 + j] +
+                                    W[i * 3 + 2] * W[2 * 3 + j];
+            const float t = W[0] + W[4] + W[8];
+            const float i2 = 0.5f * (t * t - (W2[0] + W2[4] + W2[8]));
+            const float dW = W[0] * (W[4] * W[8] - W[5] * W[7]) -
+                             W[1] * (W[3] * W[8] - W[5] * W[6]) +
+                             W[2] * (W[3] * W[7] - W[4] * W[6]);
+            const float den = 1.0f + (t + i2 + dW);
+            const bool ok = fabsf(den) > 1e-30f;  // the reference's singular guard
+            const float id = ok ? 1.0f / den : 0.0f;
+            float En[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const float diag = i == j ? 1.0f : 0.0f;
+                    En[i * 3 + j] = ok ? (W2[i * 3 + j] - (1.0f + t) * W[i * 3 + j] - dW * diag) * id
+                                       : -diag;
+                }
+            float Wn[9];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) Wn[i * 3 + j] = 0.5f * (W[i * 3 + j] + En[j * 3 + i]);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) W[k] = Wn[k];
+        }
+
+        // co-rotated small strain: R^T F - I = W^T + G + W^T G
+        float A[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                A[i * 3 + j] = W[j * 3 + i] + G[i * 3 + j] +
+                               (W[0 * 3 + i] * G[0 * 3 + j] + W[1 * 3 + i] * G[1 * 3 + j] +
+                                W[2 * 3 + i] * G[2 * 3 + j]);
+        const float e00 = A[0], e11 = A[4], e22 = A[8];
+        const float e01 = 0.5f * (A[1] + A[3]), e12 = 0.5f * (A[5] + A[7]),
+                    e02 = 0.5f * (A[2] + A[6]);
+
+        float s00, s11, s22, s01, s12, s02;
+        if (PLASTIC && (c.features & kFeatPlastic)) {
+            // J2 radial return, linear isotropic hardening (plasticity.cpp:319-455)
+            float ep[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) ep[k] = c.pstate[k * tp + e];
+            const float st[6] = {e00, e11, e22, 2.0f * e01, 2.0f * e12, 2.0f * e02};
+            float el[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) el[k] = st[k] - ep[k];
+            const float vol = el[0] + el[1] + el[2];
+            const float pr = c.bulk * vol;
+            float tr[6];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                tr[k] = 2.0f * c.mu * (el[k] - vol * (1.0f / 3.0f));
+                tr[3 + k] = c.mu * el[3 + k];
+            }
+            const float nrm = sqrtf(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
+                                    2.0f * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
+            const float root23 = 0.81649658092772603f;
+            const float radius = root23 * (c.yield + c.hard * fmaxf(ep[6], 0.0f));
+            float sv[6];
+            if (nrm > radius) {
+                const float gamma = (nrm - radius) / (2.0f * c.mu + (2.0f / 3.0f) * c.hard);
+                const float inv = 1.0f / nrm;
+                float d = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float n0 = tr[k] * inv, n1 = tr[3 + k] * inv;
+                    sv[k] = pr + tr[k] - 2.0f * c.mu * gamma * n0;
+                    sv[3 + k] = tr[3 + k] - 2.0f * c.mu * gamma * n1;
+                    const float dn = gamma * n0, ds = 2.0f * gamma * n1;
+                    d += sv[k] * dn + sv[3 + k] * ds;
+                    ep[k] += dn;
+                    ep[3 + k] += ds;
+                }
+                ep[6] += root23 * gamma;
+                diss = (double)d * (double)V;
+#pragma unroll
+                for (int k = 0; k < 7; ++k) c.pstate[k * tp + e] = ep[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    sv[k] = pr + tr[k];
+                    sv[3 + k] = tr[3 + k];
+                }
+            }
+            if (sample) {
+                float s = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) s += sv[k] * (st[k] - ep[k]);
+                strain_energy = 0.5 * (double)s * (double)V;
+            }
+            s00 = sv[0]; s11 = sv[1]; s22 = sv[2]; s01 = sv[3]; s12 = sv[4]; s02 = sv[5];
+        } else {
+            const float tr = e00 + e11 + e22;
+            const float m2 = 2.0f * c.mu, lt = c.lam * tr;
+            s00 = m2 * e00 + lt; s11 = m2 * e11 + lt; s22 = m2 * e22 + lt;
+            s01 = m2 * e01; s12 = m2 * e12; s02 = m2 * e02;
+            if (sample) {
+                const float s = s00 * e00 + s11 * e11 + s22 * e22 +
+                                2.0f * (s01 * e01 + s12 * e12 + s02 * e02);
+                strain_energy = 0.5 * (double)s * (double)V;
+            }
+        }
+        const float S[9] = {s00, s01, s02, s01, s11, s12, s02, s12, s22};
+        // P = (I + W) sigma, H = P Dm^-T, f_c = -V H(:, c-1), f_0 = -(f_1 + f_2 + f_3)
+        float P[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                P[i * 3 + j] = S[i * 3 + j] + (W[i * 3 + 0] * S[0 * 3 + j] +
+                                               W[i * 3 + 1] * S[1 * 3 + j] +
+                                               W[i * 3 + 2] * S[2 * 3 + j]);
+        float f[12];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                f[(cc + 1) * 3 + a] = -V * (P[a * 3 + 0] * B[cc * 3 + 0] + P[a * 3 + 1] * B[cc * 3 + 1] +
+                                            P[a * 3 + 2] * B[cc * 3 + 2]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f[a] = -(f[3 + a] + f[6 + a] + f[9 + a]);
+        float4* out = reinterpret_cast<float4*>(c.force) + (size_t)e * 3;
+        out[0] = make_float4(f[0], f[1], f[2], f[3]);
+        out[1] = make_float4(f[4], f[5], f[6], f[7]);
+        out[2] = make_float4(f[8], f[9], f[10], f[11]);
+    }
+
+    // per-wave FP64 sums: every lane of the wave takes part (dead lanes add zero)
+    // (a tail block's waves past the case's last wave hold only dead lanes and
+    // write nothing)
+    const uint32_t w = (tile.first + threadIdx.x) >> 6;
+    const bool writer = (threadIdx.x & 63u) == 0 && w < c.elem_waves;
+    if (PLASTIC && (c.features & kFeatPlastic)) {
+        const double s = wave_sum(diss);
+        if (writer) c.acc_e[kAccDiss * c.elem_waves + w] += s;
+    }
+    if (sample) {
+        const double s = wave_sum(strain_energy);
+        if (writer) c.smp_e[kSmpStrain * c.elem_waves + w] = s;
+    }
+}
+
+// ---------------------------------------------------------------- node pass
+template <int BLOCK, bool EXTRA>
+__global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ cases,
+                                                const Tile* __restrict__ tiles, int ntiles,
+                                                int parity, int sample, int xcd) {
+    const int b = remap_block(blockIdx.x, gridDim.x, xcd);
+    const Tile tile = tiles[b];
+    const CaseDev& c = cases[tile.case_index];
+    const int64_t n = c.step[parity];
+    const bool frozen = n >= c.step_end;
+    if (tile.first == 0 && threadIdx.x == 0) c.step[parity ^ 1] = frozen ? n : n + 1;
+    if (frozen) return;
+
+    const uint32_t i = tile.first + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = i >> 6;
+    const bool live = i < c.N;
+    const uint32_t fl = live ? c.nflags[i] : kPinAll;
+    const double dt = c.dt;
+    const float dtf = c.dtf;
+    const uint32_t np = c.Npad;
+
+    double a_ext = 0.0, a_con = 0.0, a_bnd = 0.0, a_damp = 0.0;
+    double s_kin = 0.0, s_hext = 0.0, s_hcon = 0.0, s_f[3] = {0.0, 0.0, 0.0};
+
+    if ((fl & kPinAll) != kPinAll) {
+        // gather this node's slots in CSR (tet, corner) order
+        const uint32_t base = c.ell_off[w];
+        const uint32_t width = (c.ell_off[w + 1] - base) >> 6;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        for (uint32_t q = 0; q < width; ++q) {
+            const uint32_t en = c.ell[base + q * 64u + lane];
+            if (en == kEllSentinel) break;
+            const float* s = c.force + (size_t)(en >> 2) * 12u + (en & 3u) * 3u;
+            fx += s[0];
+            fy += s[1];
+            fz += s[2];
+        }
+        const float m = c.mass[i];
+        const float im = c.invm[i];
+        const double t = (double)n * dt;
+        float fe[3] = {0.0f, 0.0f, 0.0f};
+        if ((c.features & kFeatLoad) && (fl & kLoaded)) {
+            const double sl = c.load_ramp > 0.0 ? fmin(fmax(t / c.load_ramp, 0.0), 1.0) : 1.0;
+            const float slf = (float)sl;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) fe[a] = slf * c.load[a * np + i];
+        }
+        if (c.features & kFeatGravity) fe[2] -= c.gravity * m;
+
+        float4 u = c.u4[i];
+        float uu[3] = {u.x, u.y, u.z};
+        float fc[3] = {0.0f, 0.0f, 0.0f};
+        if (EXTRA && (c.features & kFeatContact) && (fl & kContact)) {
+            // d = (X - c0) + (u - t v_ind); phi = |d|^2 - r^2 from the stored
+            // |X - c0|^2 - r^2 so no term cancels at FP32
+            float xc[3], wv[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                xc[a] = c.xc[a * np + i];
+                wv[a] = uu[a] - (float)(t * c.ind_vel[a]);
+            }
+            const float h = c.xc[3 * np + i];
+            const float phi = h + 2.0f * (xc[0] * wv[0] + xc[1] * wv[1] + xc[2] * wv[2]) +
+                              (wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+            if (phi < 0.0f) {
+                const float r = c.ind_r;
+                const float dist = sqrtf(fmaxf(r * r + phi, 0.0f));
+                if (dist > 0.0f) {
+                    const float gap = -phi / (r + dist);
+                    const float k = c.ind_kdt2 * m * gap / dist;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) fc[a] = k * (xc[a] + wv[a]);
+                }
+            }
+        }
+
+        float vv[3], cvv[3], cuu[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vv[a] = c.v[a * np + i];
+            cvv[a] = c.cv[a * np + i];
+            cuu[a] = c.cu[a * np + i];
+        }
+        const float fint[3] = {fx, fy, fz};
+        const float sd = (EXTRA && (c.features & kFeatDrive))
+                             ? [&] {
+                                   if (!(c.drive_ramp > 0.0)) return 1.0f;
+                                   double s = fmin(fmax((t + dt) / c.drive_ramp, 0.0), 1.0);
+                                   return (float)(s * s * (3.0 - 2.0 * s));
+                               }()
+                             : 1.0f;
+        bool bad = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (fl & (kPinX << a)) {
+                vv[a] = 0.0f;
+                cvv[a] = 0.0f;
+                continue;
+            }
+            const float vo = vv[a];
+            const float ftot = fint[a] + fe[a] + fc[a];
+            float vn, cvn;
+            if (EXTRA && (c.features & kFeatDrive) && (fl & (kDriveX << a))) {
+                const float vfree = vo + ftot * (im * dtf);
+                const float ut = uu[a] - cuu[a];
+                vn = (c.presc[a * np + i] * sd - ut) / dtf;
+                cvn = 0.0f;
+                a_bnd += (double)m * ((double)vn - (double)vfree) * (double)vn;
+            } else {
+                // Kahan: the true velocity is vv - cvv
+                const float y = ftot * (im * dtf) - cvv[a];
+                const float tt = vo + y;
+                cvn = (tt - vo) - y;
+                vn = tt;
+                if (EXTRA && (c.features & kFeatDamp)) {
+                    const float d = c.damping;
+                    a_damp += 0.5 * (double)m * (double)vn * (double)vn * (1.0 - (double)d * (double)d);
+                    vn *= d;
+                    cvn *= d;
+                }
+            }
+            a_ext += (double)fe[a] * ((double)vo + (double)vn) * 0.5 * dt;
+            a_con += (double)fc[a] * ((double)vo + (double)vn) * 0.5 * dt;
+            if (sample) {
+                s_kin += 0.5 * (double)m * (double)vo * (double)vn;
+                s_hext += (double)fe[a] * (double)vn * 0.5 * dt;
+                s_hcon += (double)fc[a] * (double)vn * 0.5 * dt;
+            }
+            // Kahan displacement update: the true displacement is uu - cuu
+            const float y = vn * dtf - cuu[a];
+            const float tt = uu[a] + y;
+            cuu[a] = (tt - uu[a]) - y;
+            uu[a] = tt;
+            vv[a] = vn;
+            cvv[a] = cvn;
+            bad |= !isfinite(vn);
+        }
+        if (sample) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) s_f[a] = fc[a];
+        }
+        c.u4[i] = make_float4(uu[0], uu[1], uu[2], 0.0f);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            c.v[a * np + i] = vv[a];
+            c.cv[a * np + i] = cvv[a];
+            c.cu[a * np + i] = cuu[a];
+        }
+        if (bad) c.status[0] = 1u;
+    } else if (live) {
+        // a fully pinned node keeps v = 0 (fem.cpp:197-200); its state never changes
+    }
+
+    const uint32_t nw = c.node_waves;
+    const bool writer = lane == 0 && w < nw;
+    if (c.features & (kFeatLoad | kFeatGravity)) {
+        const double s = wave_sum(a_ext);
+        if (writer) c.acc_n[kAccWext * nw + w] += s;
+    }
+    if (EXTRA) {
+        if (c.features & kFeatContact) {
+            const double s = wave_sum(a_con);
+            if (writer) c.acc_n[kAccWcon * nw + w] += s;
+        }
+        if (c.features & kFeatDrive) {
+            const double s = wave_sum(a_bnd);
+            if (writer) c.acc_n[kAccWbnd * nw + w] += s;
+        }
+        if (c.features & kFeatDamp) {
+            const double s = wave_sum(a_damp);
+            if (writer) c.acc_n[kAccDamp * nw + w] += s;
+        }
+    }
+    if (sample) {
+        const double vals[6] = {s_kin, s_hext, s_hcon, s_f[0], s_f[1], s_f[2]};
+#pragma unroll
+        for (int k = 0; k < kSmpNodeSlots; ++k) {
+            const double s = wave_sum(vals[k]);
+            if (writer) c.smp_n[k * nw + w] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP64 reductions
+constexpr int kReduceBlock = 256;
+
+__device__ double block_sum(double v, double* sh) {
+    // fixed-shape tree: deterministic for a fixed block size
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kReduceBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ double strided_sum(const double* p, uint32_t count) {
+    double s = 0.0;
+    for (uint32_t k = threadIdx.x; k < count; k += kReduceBlock) s += p[k];
+    return s;
+}
+
+__global__ __launch_bounds__(kReduceBlock) void k_reduce(const CaseDev* __restrict__ cases,
+                                                          int parity) {
+    __shared__ double sh[kReduceBlock];
+    const CaseDev& c = cases[blockIdx.x];
+    const int64_t n0 = c.step[parity];
+    if (c.step[parity ^ 1] != n0 + 1) return;  // the case did not step: no record
+    const uint32_t ew = c.elem_waves, nw = c.node_waves;
+    const double diss = block_sum(strided_sum(c.acc_e + kAccDiss * ew, ew), sh);
+    const double wext = block_sum(strided_sum(c.acc_n + kAccWext * nw, nw), sh);
+    const double wcon = block_sum(strided_sum(c.acc_n + kAccWcon * nw, nw), sh);
+    const double wbnd = block_sum(strided_sum(c.acc_n + kAccWbnd * nw, nw), sh);
+    const double damp = block_sum(strided_sum(c.acc_n + kAccDamp * nw, nw), sh);
+    const double strain = block_sum(strided_sum(c.smp_e + kSmpStrain * ew, ew), sh);
+    double sn[kSmpNodeSlots];
+    for (int k = 0; k < kSmpNodeSlots; ++k) sn[k] = block_sum(strided_sum(c.smp_n + k * nw, nw), sh);
+    if (threadIdx.x == 0) {
+        const int32_t h = *c.hist_count;
+        if (h < c.hist_cap) {
+            EnergyRecordDev& r = c.hist[h];
+            r.step = n0;
+            r.time = (double)n0 * c.dt;
+            r.strain = strain;
+            r.kinetic = sn[kSmpKin];
+            r.dissipation = diss;
+            r.w_ext = wext - sn[kSmpHalfExt];
+            r.w_contact = wcon - sn[kSmpHalfCon];
+            r.w_boundary = wbnd;
+            r.damping_loss = damp;
+            r.contact_force[0] = sn[kSmpFx];
+            r.contact_force[1] = sn[kSmpFy];
+            r.contact_force[2] = sn[kSmpFz];
+            *c.hist_count = h + 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP64 stable step
+// lambda_max(B^T C B) = lambda_max(Q (B B^T) Q), Q = C^{1/2} in Voigt form with
+// engineering shears: C^{1/2} = sqrt(3K) P_vol + sqrt(2 mu) P_dev on the normal
+// block and sqrt(mu) on the shear block.
+__device__ double element_lambda_max(const float* dmi, uint32_t tp, uint32_t e, double lam,
+                                     double mu) {
+    double g[4][3];
+#pragma unroll
+    for (int cc = 1; cc < 4; ++cc)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) g[cc][a] = (double)dmi[((cc - 1) * 3 + a) * tp + e];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[0][a] = -(g[1][a] + g[2][a] + g[3][a]);
+    double M[36];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) M[k] = 0.0;
+#pragma unroll
+    for (int nd = 0; nd < 4; ++nd) {
+        const double gx = g[nd][0], gy = g[nd][1], gz = g[nd][2];
+        // rows of B_a: [gx,0,0] [0,gy,0] [0,0,gz] [gy,gx,0] [0,gz,gy] [gz,0,gx]
+        const double Ba[6][3] = {{gx, 0, 0}, {0, gy, 0}, {0, 0, gz},
+                                 {gy, gx, 0}, {0, gz, gy}, {gz, 0, gx}};
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                M[i * 6 + j] += Ba[i][0] * Ba[j][0] + Ba[i][1] * Ba[j][1] + Ba[i][2] * Ba[j][2];
+    }
+    const double K3 = 3.0 * lam + 2.0 * mu;
+    const double qa = sqrt(K3) / 3.0 + sqrt(2.0 * mu) * (2.0 / 3.0);
+    const double qb = sqrt(K3) / 3.0 - sqrt(2.0 * mu) / 3.0;
+    const double qs = sqrt(mu);
+    // S = Q M Q with Q block-diagonal
+    double Q[36];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) Q[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Q[i * 6 + j] = i == j ? qa : qb;
+        Q[(3 + i) * 6 + 3 + i] = qs;
+    }
+    double QM[36], S[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += Q[i * 6 + k] * M[k * 6 + j];
+            QM[i * 6 + j] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += QM[i * 6 + k] * Q[k * 6 + j];
+            S[i * 6 + j] = s;
+        }
+    // cyclic Jacobi, fixed sweep count
+    for (int sweep = 0; sweep < 12; ++sweep) {
+#pragma unroll
+        for (int p = 0; p < 5; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 6; ++q) {
+                const double apq = S[p * 6 + q];
+                if (apq != 0.0) {
+                    const double theta = (S[q * 6 + q] - S[p * 6 + p]) / (2.0 * apq);
+                    const double tt = (theta >= 0.0 ? 1.0 : -1.0) /
+                                      (fabs(theta) + sqrt(theta * theta + 1.0));
+                    const double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const double akp = S[k * 6 + p], akq = S[k * 6 + q];
+                        S[k * 6 + p] = cs * akp - sn * akq;
+                        S[k * 6 + q] = sn * akp + cs * akq;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const double apk = S[p * 6 + k], aqk = S[q * 6 + k];
+                        S[p * 6 + k] = cs * apk - sn * aqk;
+                        S[q * 6 + k] = sn * apk + cs * aqk;
+                    }
+                }
+            }
+    }
+    double mx = S[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) mx = fmax(mx, S[k * 6 + k]);
+    return mx;
+}
+
+__global__ __launch_bounds__(256) void k_omega(const CaseDev* __restrict__ cases,
+                                               const Tile* __restrict__ tiles) {
+    const Tile tile = tiles[blockIdx.x];
+    const CaseDev& c = cases[tile.case_index];
+    const uint32_t e = tile.first + threadIdx.x;
+    double l = 0.0;
+    if (e < c.T) l = element_lambda_max(c.dmi, c.Tpad, e, c.lam_d, c.mu_d);
+    l = wave_max(l);
+    if ((threadIdx.x & 63u) == 0 && (e >> 6) < c.elem_waves) c.omega_w[e >> 6] = l;
+}
+
+__global__ __launch_bounds__(kReduceBlock) void k_omega_max(const CaseDev* __restrict__ cases,
+                                                             double* out) {
+    __shared__ double sh[kReduceBlock];
+    const CaseDev& c = cases[blockIdx.x];
+    double m = 0.0;
+    for (uint32_t k = threadIdx.x; k < c.elem_waves; k += kReduceBlock) m = fmax(m, c.omega_w[k]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kReduceBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+template <int BLOCK>
+void elem_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
+                 int xcd, bool plastic, hipStream_t s) {
+    if (plastic)
+        hipLaunchKernelGGL((k_elem<BLOCK, true>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                           ntiles, parity, sample, xcd);
+    else
+        hipLaunchKernelGGL((k_elem<BLOCK, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                           ntiles, parity, sample, xcd);
+}
+
+template <int BLOCK>
+void node_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
+                 int xcd, bool extra, hipStream_t s) {
+    if (extra)
+        hipLaunchKernelGGL((k_node<BLOCK, true>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                           ntiles, parity, sample, xcd);
+    else
+        hipLaunchKernelGGL((k_node<BLOCK, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                           ntiles, parity, sample, xcd);
+}
+
+}  // namespace
+
+void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
+                 int sample, int xcd, uint32_t feat_union, hipStream_t stream) {
+    const bool plastic = (feat_union & kFeatPlastic) != 0;
+    switch (block) {
+        case 64: elem_launch<64>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
+        case 128: elem_launch<128>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
+        case 512: elem_launch<512>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
+        default: elem_launch<256>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
+    }
+}
+
+void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
+                 int sample, int xcd, uint32_t feat_union, hipStream_t stream) {
+    const bool extra = (feat_union & (kFeatContact | kFeatDrive | kFeatDamp)) != 0;
+    switch (block) {
+        case 64: node_launch<64>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
+        case 128: node_launch<128>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
+        case 512: node_launch<512>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
+        default: node_launch<256>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
+    }
+}
+
+void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reduce, dim3(ncases), dim3(kReduceBlock), 0, stream, cases, parity);
+}
+
+void launch_omega(const CaseDev* cases, const Tile* tiles, int ntiles, hipStream_t stream) {
+    hipLaunchKernelGGL(k_omega, dim3(ntiles), dim3(256), 0, stream, cases, tiles);
+}
+
+void launch_omega_max(const CaseDev* cases, int ncases, double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_omega_max, dim3(ncases), dim3(kReduceBlock), 0, stream, cases, out);
+}
+
+}  // namespace shipfem

@@ -1,0 +1,478 @@
+// prep.cpp -- case preparation (FP64 rest state on the host, FP32 SoA planes for the
+// device), the per-case device arena, upload and readback.
+#include "shipfem_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace shipfem {
+
+namespace {
+
+constexpr uint32_t kKnownFlags = SHIPFEM_PIN_ALL | SHIPFEM_LOADED | SHIPFEM_CONTACT |
+                                 SHIPFEM_DRIVE_X | SHIPFEM_DRIVE_Y | SHIPFEM_DRIVE_Z;
+
+std::string fmt(const char* what, double v) { return std::string(what) + " (" + std::to_string(v) + ")"; }
+
+// Splits a double into a float and the Kahan compensation c = hi - x (the kernels
+// take the true value as hi - c).
+inline void split(double x, float& hi, float& c) {
+    hi = static_cast<float>(x);
+    c = static_cast<float>(static_cast<double>(hi) - x);
+}
+
+struct Layout {
+    uint64_t off = 0;
+    uint64_t take(uint64_t bytes) {
+        const uint64_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    }
+};
+
+// Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
+struct ArenaMap {
+    uint64_t conn, dmi, vol, force, pstate, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
+        ell_off, ell, acc_e, acc_n, smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
+};
+
+ArenaMap mapArena(const HostCase& hc) {
+    Layout L;
+    ArenaMap m{};
+    const uint64_t T = hc.T, N = hc.N, Tp = hc.Tpad, Np = hc.Npad;
+    m.conn = L.take(16 * T);
+    m.dmi = L.take(36 * Tp);
+    m.vol = L.take(4 * Tp);
+    m.force = L.take(48 * T);
+    m.pstate = L.take((hc.features & kFeatPlastic) ? 28 * Tp : 0);
+    m.u4 = L.take(16 * N);
+    m.cu = L.take(12 * Np);
+    m.v = L.take(12 * Np);
+    m.cv = L.take(12 * Np);
+    m.invm = L.take(4 * Np);
+    m.mass = L.take(4 * Np);
+    m.nflags = L.take(4 * Np);
+    m.load = L.take((hc.features & kFeatLoad) ? 12 * Np : 0);
+    m.xc = L.take((hc.features & kFeatContact) ? 16 * Np : 0);
+    m.presc = L.take((hc.features & kFeatDrive) ? 12 * Np : 0);
+    m.ell_off = L.take(4 * (uint64_t)hc.ell_off.size());
+    m.ell = L.take(4 * (uint64_t)hc.ell.size());
+    m.acc_e = L.take(8ull * kAccElemSlots * hc.elem_waves);
+    m.acc_n = L.take(8ull * kAccNodeSlots * hc.node_waves);
+    m.smp_e = L.take(8ull * kSmpElemSlots * hc.elem_waves);
+    m.smp_n = L.take(8ull * kSmpNodeSlots * hc.node_waves);
+    m.omega_w = L.take(8ull * hc.elem_waves);
+    m.step = L.take(16);
+    m.status = L.take(4);
+    m.hist_count = L.take(4);
+    m.hist = L.take(sizeof(EnergyRecordDev) * (uint64_t)hc.hist_cap);
+    m.total = L.off;
+    return m;
+}
+
+}  // namespace
+
+void setError(char* err, size_t len, const std::string& msg) {
+    if (!err || len == 0) return;
+    const size_t n = std::min(len - 1, msg.size());
+    std::memcpy(err, msg.data(), n);
+    err[n] = '\0';
+}
+
+shipfem_status statusOf(hipError_t e) {
+    if (e == hipSuccess) return SHIPFEM_OK;
+    if (e == hipErrorOutOfMemory) return SHIPFEM_OUT_OF_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SHIPFEM_NO_DEVICE;
+    return SHIPFEM_DEVICE_ERROR;
+}
+
+bool deviceAvailable(std::string* why) {
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+        if (why) *why = std::string("no HIP device (") + hipGetErrorString(e) + ")";
+        return false;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+        if (why) *why = "device 0 properties unavailable";
+        return false;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        if (why) *why = std::string("device 0 is ") + prop.gcnArchName + ", this build is gfx950 only";
+        return false;
+    }
+    return true;
+}
+
+std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap) {
+    const uint32_t N = d.node_count, T = d.tet_count;
+    if (N == 0 || T == 0) return "empty mesh (node_count and tet_count must be positive)";
+    if (!d.rest_position || !d.index) return "rest_position and index are required";
+    if (T > (0xFFFFFFFFu >> 2) - 64) return "tet_count too large for 32-bit slot entries";
+    const shipfem_material& mt = d.material;
+    if (!(mt.youngs_modulus > 0)) return fmt("youngs_modulus must be positive", mt.youngs_modulus);
+    if (!(mt.poisson_ratio > -1.0 && mt.poisson_ratio < 0.5))
+        return fmt("poisson_ratio must lie in (-1, 0.5)", mt.poisson_ratio);
+    if (!(mt.density > 0)) return fmt("density must be positive", mt.density);
+    if (mt.plastic && !(mt.yield_strength > 0)) return fmt("yield_strength must be positive", mt.yield_strength);
+    if (mt.plastic && !(mt.hardening_modulus >= 0)) return fmt("hardening_modulus must be >= 0", mt.hardening_modulus);
+    const double damping = d.damping == 0.0 ? 1.0 : d.damping;
+    if (!(damping > 0 && damping <= 1.0)) return fmt("damping must lie in (0, 1]", d.damping);
+    if (!(d.dt >= 0) || !std::isfinite(d.dt)) return fmt("dt must be >= 0", d.dt);
+    const double safety = d.dt_safety == 0.0 ? 0.9 : d.dt_safety;
+    if (!(safety > 0 && safety <= 4.0)) return fmt("dt_safety must lie in (0, 4]", d.dt_safety);
+    const int32_t S = d.sample_every == 0 ? 100 : d.sample_every;
+    if (S < 2 || (S & 1)) return "sample_every must be an even number >= 2";
+    if (d.max_steps < 0) return "max_steps must be >= 0";
+    if (!(d.load_ramp >= 0) || !(d.drive_ramp >= 0)) return "ramp times must be >= 0";
+    if (!std::isfinite(d.gravity)) return "gravity must be finite";
+    const shipfem_indenter& ind = d.indenter;
+    const double penalty = ind.penalty == 0.0 ? 0.1 : ind.penalty;
+    if (ind.enabled) {
+        if (!(ind.radius > 0)) return fmt("indenter radius must be positive", ind.radius);
+        if (!(penalty > 0 && penalty <= 1.0)) return fmt("indenter penalty must lie in (0, 1]", ind.penalty);
+    }
+
+    hc = HostCase();
+    hc.N = N;
+    hc.T = T;
+    hc.Npad = roundUp64(N);
+    hc.Tpad = roundUp64(T);
+    hc.node_waves = hc.Npad / kWave;
+    hc.elem_waves = hc.Tpad / kWave;
+    hc.E = mt.youngs_modulus;
+    hc.nu = mt.poisson_ratio;
+    hc.rho = mt.density;
+    hc.lam = hc.E * hc.nu / ((1 + hc.nu) * (1 - 2 * hc.nu));
+    hc.mu = hc.E / (2 * (1 + hc.nu));
+    hc.bulk = hc.E / (3 * (1 - 2 * hc.nu));
+    hc.yield = mt.yield_strength;
+    hc.hard = mt.hardening_modulus;
+    hc.load_ramp = d.load_ramp;
+    hc.drive_ramp = d.drive_ramp;
+    hc.gravity = d.gravity;
+    hc.damping = damping;
+    hc.indenter = ind;
+    hc.indenter.penalty = penalty;
+    hc.dt_request = d.dt;
+    hc.dt_safety = safety;
+    hc.sample_every = S;
+    hc.max_steps = d.max_steps;
+    hc.hist_cap = hist_cap > 0 ? hist_cap : 4096;
+
+    // flags
+    hc.nflags.assign(hc.Npad, SHIPFEM_PIN_ALL);  // pad lanes behave as pinned
+    for (uint32_t n = 0; n < N; ++n) {
+        const uint32_t f = d.node_flags ? d.node_flags[n] : 0u;
+        if (f & ~kKnownFlags) return "node " + std::to_string(n) + " has unknown flag bits";
+        hc.nflags[n] = f & ~(SHIPFEM_LOADED | SHIPFEM_CONTACT);  // derived below
+    }
+    for (uint64_t q = 0; q < 4ull * T; ++q)
+        if (d.index[q] >= N) return "tet " + std::to_string(q / 4) + " references node " +
+                                    std::to_string(d.index[q]) + " >= node_count";
+
+    // rest state in FP64: Dm^-1 (row-major planes), volume, lumped mass
+    hc.X.assign(d.rest_position, d.rest_position + 3ull * N);
+    hc.conn.assign(d.index, d.index + 4ull * T);
+    hc.dmi.assign(9ull * hc.Tpad, 0.0f);
+    hc.vol.assign(hc.Tpad, 0.0f);
+    std::vector<double> mass(N, 0.0);
+    const double* X = d.rest_position;
+    double shortest = std::numeric_limits<double>::infinity();
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint32_t* v = &d.index[4ull * t];
+        double Dm[3][3];  // Dm[row][col], col c = X_{c+1} - X_0
+        double longest2 = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) Dm[a][c] = X[v[c + 1] * 3ull + a] - X[v[0] * 3ull + a];
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b) {
+                double d2 = 0;
+                for (int k = 0; k < 3; ++k) {
+                    const double e = X[v[a] * 3ull + k] - X[v[b] * 3ull + k];
+                    d2 += e * e;
+                }
+                longest2 = std::max(longest2, d2);
+                shortest = std::min(shortest, std::sqrt(d2));
+            }
+        const double det = Dm[0][0] * (Dm[1][1] * Dm[2][2] - Dm[1][2] * Dm[2][1]) -
+                           Dm[0][1] * (Dm[1][0] * Dm[2][2] - Dm[1][2] * Dm[2][0]) +
+                           Dm[0][2] * (Dm[1][0] * Dm[2][1] - Dm[1][1] * Dm[2][0]);
+        if (!(std::fabs(det) > 1e-12 * longest2 * std::sqrt(longest2)))
+            return "tet " + std::to_string(t) + " is degenerate (|det Dm| = " + std::to_string(det) + ")";
+        const double id = 1.0 / det;
+        const double inv[9] = {(Dm[1][1] * Dm[2][2] - Dm[1][2] * Dm[2][1]) * id,
+                               (Dm[0][2] * Dm[2][1] - Dm[0][1] * Dm[2][2]) * id,
+                               (Dm[0][1] * Dm[1][2] - Dm[0][2] * Dm[1][1]) * id,
+                               (Dm[1][2] * Dm[2][0] - Dm[1][0] * Dm[2][2]) * id,
+                               (Dm[0][0] * Dm[2][2] - Dm[0][2] * Dm[2][0]) * id,
+                               (Dm[0][2] * Dm[1][0] - Dm[0][0] * Dm[1][2]) * id,
+                               (Dm[1][0] * Dm[2][1] - Dm[1][1] * Dm[2][0]) * id,
+                               (Dm[0][1] * Dm[2][0] - Dm[0][0] * Dm[2][1]) * id,
+                               (Dm[0][0] * Dm[1][1] - Dm[0][1] * Dm[1][0]) * id};
+        for (int k = 0; k < 9; ++k) hc.dmi[k * (uint64_t)hc.Tpad + t] = static_cast<float>(inv[k]);
+        const double V = std::fabs(det) / 6.0;
+        hc.vol[t] = static_cast<float>(V);
+        hc.total_volume += V;
+        for (int c = 0; c < 4; ++c) mass[v[c]] += hc.rho * V * 0.25;
+    }
+    hc.edge_dt = 0.5 * shortest / std::sqrt(hc.E / hc.rho);
+    hc.mass.assign(hc.Npad, 0.0f);
+    hc.invm.assign(hc.Npad, 0.0f);
+    for (uint32_t n = 0; n < N; ++n) {
+        hc.total_mass += mass[n];
+        hc.mass[n] = static_cast<float>(mass[n]);
+        if (mass[n] > 0) hc.invm[n] = static_cast<float>(1.0 / mass[n]);
+        else hc.nflags[n] |= SHIPFEM_PIN_ALL;  // a node in no tet carries no mass: held
+    }
+
+    // loads, contact candidates, prescribed drive
+    bool anyLoad = false, anyContact = false, anyDrive = false;
+    if (d.load) {
+        hc.load.assign(3ull * hc.Npad, 0.0f);
+        for (uint32_t n = 0; n < N; ++n)
+            for (int a = 0; a < 3; ++a) {
+                const double f = d.load[n * 3ull + a];
+                if (!std::isfinite(f)) return "load of node " + std::to_string(n) + " is not finite";
+                if (f != 0.0) {
+                    hc.load[a * (uint64_t)hc.Npad + n] = static_cast<float>(f);
+                    hc.nflags[n] |= SHIPFEM_LOADED;
+                    anyLoad = true;
+                }
+            }
+        if (!anyLoad) hc.load.clear();
+    }
+    if (ind.enabled) {
+        bool flagged = false;
+        for (uint32_t n = 0; n < N && d.node_flags; ++n) flagged |= (d.node_flags[n] & SHIPFEM_CONTACT) != 0;
+        hc.xc.assign(4ull * hc.Npad, 0.0f);
+        for (uint32_t n = 0; n < N; ++n) {
+            if (flagged && !(d.node_flags[n] & SHIPFEM_CONTACT)) continue;
+            if ((hc.nflags[n] & SHIPFEM_PIN_ALL) == SHIPFEM_PIN_ALL) continue;
+            hc.nflags[n] |= SHIPFEM_CONTACT;
+            double r2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                const double x = X[n * 3ull + a] - ind.center[a];
+                hc.xc[a * (uint64_t)hc.Npad + n] = static_cast<float>(x);
+                r2 += x * x;
+            }
+            hc.xc[3ull * hc.Npad + n] = static_cast<float>(r2 - ind.radius * ind.radius);
+            anyContact = true;
+        }
+        if (!anyContact) hc.xc.clear();
+    }
+    if (d.prescribed) {
+        hc.presc.assign(3ull * hc.Npad, 0.0f);
+        for (uint32_t n = 0; n < N; ++n)
+            for (int a = 0; a < 3; ++a)
+                if (hc.nflags[n] & (SHIPFEM_DRIVE_X << a)) {
+                    hc.presc[a * (uint64_t)hc.Npad + n] = static_cast<float>(d.prescribed[n * 3ull + a]);
+                    anyDrive = true;
+                }
+        if (!anyDrive) hc.presc.clear();
+    }
+    hc.features = (mt.plastic ? kFeatPlastic : 0) | (anyLoad ? kFeatLoad : 0) |
+                  (anyContact ? kFeatContact : 0) | (anyDrive ? kFeatDrive : 0) |
+                  (damping != 1.0 ? kFeatDamp : 0) | (d.gravity != 0.0 ? kFeatGravity : 0);
+
+    // CSR node -> (tet*4 + corner) by counting sort, (tet, corner) order (fem.cpp:147-157),
+    // then wave-transposed: wave w holds width_w rows of 64 lanes
+    std::vector<uint32_t> off(N + 1, 0);
+    for (uint64_t q = 0; q < 4ull * T; ++q) ++off[d.index[q] + 1];
+    for (uint32_t n = 0; n < N; ++n) off[n + 1] += off[n];
+    std::vector<uint32_t> ent(4ull * T), cur(off.begin(), off.end() - 1);
+    for (uint32_t t = 0; t < T; ++t)
+        for (uint32_t c = 0; c < 4; ++c) ent[cur[d.index[4ull * t + c]]++] = t * 4 + c;
+    hc.ell_off.assign(hc.node_waves + 1, 0);
+    uint64_t total = 0;
+    for (uint32_t w = 0; w < hc.node_waves; ++w) {
+        uint32_t width = 0;
+        for (uint32_t l = 0; l < kWave; ++l) {
+            const uint32_t n = w * kWave + l;
+            if (n < N && (hc.nflags[n] & SHIPFEM_PIN_ALL) != SHIPFEM_PIN_ALL)
+                width = std::max(width, off[n + 1] - off[n]);
+        }
+        hc.ell_off[w] = static_cast<uint32_t>(total);
+        total += uint64_t(width) * kWave;
+        if (total > 0xFFFFFFF0ull) return "node gather slab exceeds 32-bit indexing";
+    }
+    hc.ell_off[hc.node_waves] = static_cast<uint32_t>(total);
+    hc.ell.assign(total, kEllSentinel);
+    for (uint32_t w = 0; w < hc.node_waves; ++w)
+        for (uint32_t l = 0; l < kWave; ++l) {
+            const uint32_t n = w * kWave + l;
+            if (n >= N || (hc.nflags[n] & SHIPFEM_PIN_ALL) == SHIPFEM_PIN_ALL) continue;
+            for (uint32_t q = off[n]; q < off[n + 1]; ++q)
+                hc.ell[hc.ell_off[w] + (q - off[n]) * kWave + l] = ent[q];
+        }
+
+    // initial state, split into float + Kahan compensation
+    hc.u4.assign(4ull * N, 0.0f);
+    hc.cu.assign(3ull * hc.Npad, 0.0f);
+    hc.v.assign(3ull * hc.Npad, 0.0f);
+    hc.cv.assign(3ull * hc.Npad, 0.0f);
+    for (uint32_t n = 0; n < N; ++n)
+        for (int a = 0; a < 3; ++a) {
+            if (d.initial_displacement)
+                split(d.initial_displacement[n * 3ull + a], hc.u4[n * 4ull + a],
+                      hc.cu[a * (uint64_t)hc.Npad + n]);
+            if (d.initial_velocity && !(hc.nflags[n] & (SHIPFEM_PIN_X << a)))
+                split(d.initial_velocity[n * 3ull + a], hc.v[a * (uint64_t)hc.Npad + n],
+                      hc.cv[a * (uint64_t)hc.Npad + n]);
+        }
+    return {};
+}
+
+uint64_t arenaBytes(const HostCase& hc) { return mapArena(hc).total; }
+
+hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_t s) {
+    const ArenaMap m = mapArena(hc);
+    dc.arena = base;
+    dc.bytes = m.total;
+    CaseDev& r = dc.rec;
+    r = CaseDev{};
+    r.N = hc.N;
+    r.T = hc.T;
+    r.Npad = hc.Npad;
+    r.Tpad = hc.Tpad;
+    r.node_waves = hc.node_waves;
+    r.elem_waves = hc.elem_waves;
+    r.features = hc.features;
+    r.conn = reinterpret_cast<const uint4*>(base + m.conn);
+    r.dmi = reinterpret_cast<const float*>(base + m.dmi);
+    r.vol = reinterpret_cast<const float*>(base + m.vol);
+    r.force = reinterpret_cast<float*>(base + m.force);
+    r.pstate = (hc.features & kFeatPlastic) ? reinterpret_cast<float*>(base + m.pstate) : nullptr;
+    r.u4 = reinterpret_cast<float4*>(base + m.u4);
+    r.cu = reinterpret_cast<float*>(base + m.cu);
+    r.v = reinterpret_cast<float*>(base + m.v);
+    r.cv = reinterpret_cast<float*>(base + m.cv);
+    r.invm = reinterpret_cast<const float*>(base + m.invm);
+    r.mass = reinterpret_cast<const float*>(base + m.mass);
+    r.nflags = reinterpret_cast<const uint32_t*>(base + m.nflags);
+    r.load = (hc.features & kFeatLoad) ? reinterpret_cast<const float*>(base + m.load) : nullptr;
+    r.xc = (hc.features & kFeatContact) ? reinterpret_cast<const float*>(base + m.xc) : nullptr;
+    r.presc = (hc.features & kFeatDrive) ? reinterpret_cast<const float*>(base + m.presc) : nullptr;
+    r.ell_off = reinterpret_cast<const uint32_t*>(base + m.ell_off);
+    r.ell = reinterpret_cast<const uint32_t*>(base + m.ell);
+    r.load_ramp = hc.load_ramp;
+    r.drive_ramp = hc.drive_ramp;
+    for (int a = 0; a < 3; ++a) r.ind_vel[a] = hc.indenter.velocity[a];
+    r.lam = static_cast<float>(hc.lam);
+    r.mu = static_cast<float>(hc.mu);
+    r.bulk = static_cast<float>(hc.bulk);
+    r.yield = static_cast<float>(hc.yield);
+    r.hard = static_cast<float>(hc.hard);
+    r.gravity = static_cast<float>(hc.gravity);
+    r.damping = static_cast<float>(hc.damping);
+    r.ind_r = static_cast<float>(hc.indenter.radius);
+    r.lam_d = hc.lam;
+    r.mu_d = hc.mu;
+    r.acc_e = reinterpret_cast<double*>(base + m.acc_e);
+    r.acc_n = reinterpret_cast<double*>(base + m.acc_n);
+    r.smp_e = reinterpret_cast<double*>(base + m.smp_e);
+    r.smp_n = reinterpret_cast<double*>(base + m.smp_n);
+    r.omega_w = reinterpret_cast<double*>(base + m.omega_w);
+    r.step = reinterpret_cast<int64_t*>(base + m.step);
+    r.step_end = hc.max_steps > 0 ? hc.max_steps : std::numeric_limits<int64_t>::max();
+    r.status = reinterpret_cast<uint32_t*>(base + m.status);
+    r.hist = reinterpret_cast<EnergyRecordDev*>(base + m.hist);
+    r.hist_cap = hc.hist_cap;
+    r.hist_count = reinterpret_cast<int32_t*>(base + m.hist_count);
+
+    hipError_t e = hipMemsetAsync(base, 0, m.total, s);
+    auto up = [&](uint64_t off, const void* src, uint64_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    up(m.conn, hc.conn.data(), 4ull * hc.conn.size());
+    up(m.dmi, hc.dmi.data(), 4ull * hc.dmi.size());
+    up(m.vol, hc.vol.data(), 4ull * hc.vol.size());
+    up(m.u4, hc.u4.data(), 4ull * hc.u4.size());
+    up(m.cu, hc.cu.data(), 4ull * hc.cu.size());
+    up(m.v, hc.v.data(), 4ull * hc.v.size());
+    up(m.cv, hc.cv.data(), 4ull * hc.cv.size());
+    up(m.invm, hc.invm.data(), 4ull * hc.invm.size());
+    up(m.mass, hc.mass.data(), 4ull * hc.mass.size());
+    up(m.nflags, hc.nflags.data(), 4ull * hc.nflags.size());
+    up(m.load, hc.load.data(), 4ull * hc.load.size());
+    up(m.xc, hc.xc.data(), 4ull * hc.xc.size());
+    up(m.presc, hc.presc.data(), 4ull * hc.presc.size());
+    up(m.ell_off, hc.ell_off.data(), 4ull * hc.ell_off.size());
+    up(m.ell, hc.ell.data(), 4ull * hc.ell.size());
+    return e;
+}
+
+void setTimestep(DeviceCase& dc, const HostCase& hc, double dt) {
+    // the kernels use dt in FP32 for the update and in FP64 for time and work: keep
+    // the two identical by taking dt at float precision
+    dt = static_cast<double>(static_cast<float>(dt));
+    dc.dt = dt;
+    dc.rec.dt = dt;
+    dc.rec.dtf = static_cast<float>(dt);
+    dc.rec.ind_kdt2 = static_cast<float>(hc.indenter.penalty / (dt * dt));
+}
+
+hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_view* view,
+                        hipStream_t s) {
+    const CaseDev& r = dc.rec;
+    const uint64_t N = hc.N, Np = hc.Npad;
+    std::vector<float> u4(4 * N), pl(9 * Np);
+    int64_t step[2] = {0, 0};
+    uint32_t status = 0;
+    hipError_t e = hipMemcpyAsync(u4.data(), r.u4, 16 * N, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl.data(), r.cu, 12 * Np, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl.data() + 3 * Np, r.v, 12 * Np, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl.data() + 6 * Np, r.cv, 12 * Np, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(step, r.step, 16, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&status, r.status, 4, hipMemcpyDeviceToHost, s);
+    std::vector<float> ps;
+    if (r.pstate && (view->eqps || view->plastic_strain)) {
+        ps.resize(7 * (uint64_t)hc.Tpad);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ps.data(), r.pstate, 4 * ps.size(), hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const float* cu = pl.data();
+    const float* v = pl.data() + 3 * Np;
+    const float* cv = pl.data() + 6 * Np;
+    for (uint64_t n = 0; n < N; ++n)
+        for (int a = 0; a < 3; ++a) {
+            const double u = (double)u4[n * 4 + a] - (double)cu[a * Np + n];
+            if (view->displacement) view->displacement[n * 3 + a] = u;
+            if (view->velocity) view->velocity[n * 3 + a] = (double)v[a * Np + n] - (double)cv[a * Np + n];
+            if (view->position) view->position[n * 3 + a] = static_cast<float>(hc.X[n * 3 + a] + u);
+        }
+    const uint64_t T = hc.T, Tp = hc.Tpad;
+    if (view->eqps)
+        for (uint64_t t = 0; t < T; ++t) view->eqps[t] = ps.empty() ? 0.0f : ps[6 * Tp + t];
+    if (view->plastic_strain)
+        for (uint64_t t = 0; t < T; ++t)
+            for (int k = 0; k < 6; ++k) view->plastic_strain[t * 6 + k] = ps.empty() ? 0.0f : ps[k * Tp + t];
+    view->step = std::max(step[0], step[1]);
+    view->time = static_cast<double>(view->step) * dc.dt;
+    view->status = status;
+    return hipSuccess;
+}
+
+hipError_t readEnergy(const DeviceCase& dc, shipfem_energy_record* out, int32_t max,
+                      int32_t* count, hipStream_t s) {
+    int32_t n = 0;
+    hipError_t e = hipMemcpyAsync(&n, dc.rec.hist_count, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    n = std::min(n, dc.rec.hist_cap);
+    const int32_t k = out ? std::min(n, max) : 0;
+    if (k > 0) {
+        static_assert(sizeof(EnergyRecordDev) == sizeof(shipfem_energy_record), "record layout");
+        e = hipMemcpyAsync(out, dc.rec.hist, sizeof(EnergyRecordDev) * k, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (count) *count = n;
+    return e;
+}
+
+}  // namespace shipfem

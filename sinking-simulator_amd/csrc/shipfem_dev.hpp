@@ -1,0 +1,122 @@
+// shipfem_dev.hpp -- device-visible case record and kernel launch interface.
+//
+// One CaseDev per case lives in device memory; the element and node kernels read it
+// through a tile table whose entries never span two cases, so every case's
+// arithmetic and every per-wave reduction is the same whatever else is in the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace shipfem {
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kEllSentinel = 0xFFFFFFFFu;
+
+// node flag bits (mirror include/shipfem.h)
+constexpr uint32_t kPinX = 0x01, kPinY = 0x02, kPinZ = 0x04, kPinAll = 0x07;
+constexpr uint32_t kLoaded = 0x08, kContact = 0x10;
+constexpr uint32_t kDriveX = 0x20, kDriveY = 0x40, kDriveZ = 0x80;
+
+// case feature bits: which optional terms the node/element passes evaluate
+enum Feature : uint32_t {
+    kFeatPlastic = 1u << 0,
+    kFeatLoad = 1u << 1,
+    kFeatContact = 1u << 2,
+    kFeatDrive = 1u << 3,
+    kFeatDamp = 1u << 4,
+    kFeatGravity = 1u << 5,
+};
+
+// per-wave FP64 accumulator slots (running sums, read-modify-write every step)
+enum { kAccDiss = 0, kAccElemSlots = 1 };
+enum { kAccWext = 0, kAccWcon = 1, kAccWbnd = 2, kAccDamp = 3, kAccNodeSlots = 4 };
+// per-wave FP64 sample partials (written on sample steps only)
+enum { kSmpStrain = 0, kSmpElemSlots = 1 };
+enum { kSmpKin = 0, kSmpHalfExt = 1, kSmpHalfCon = 2, kSmpFx = 3, kSmpFy = 4, kSmpFz = 5,
+       kSmpNodeSlots = 6 };
+
+// Same layout as shipfem_energy_record.
+struct EnergyRecordDev {
+    int64_t step;
+    double time;
+    double strain, kinetic, dissipation, w_ext, w_contact, w_boundary, damping_loss;
+    double contact_force[3];
+};
+
+struct CaseDev {
+    uint32_t N, T;            // nodes, tets
+    uint32_t Npad, Tpad;      // plane strides (multiples of 64)
+    uint32_t node_waves, elem_waves;
+    uint32_t features;
+    uint32_t pad0;
+    // ---- element arrays
+    const uint4* conn;   // [T] node ids
+    const float* dmi;    // [9][Tpad] Dm^-1, row-major entry k = 3*row + col
+    const float* vol;    // [Tpad]
+    float* force;        // [T][12] per-tet nodal forces, reference slot layout
+    float* pstate;       // [7][Tpad] plastic strain (6, eng. shears) + eqps
+    // ---- node arrays
+    float4* u4;          // [N] displacement (xyz, w = 0), gathered by the element pass
+    float* cu;           // [3][Npad] Kahan compensation of u
+    float* v;            // [3][Npad] half-step velocity
+    float* cv;           // [3][Npad] Kahan compensation of v
+    const float* invm;   // [Npad]
+    const float* mass;   // [Npad]
+    const uint32_t* nflags;  // [Npad]
+    const float* load;   // [3][Npad] nodal load at full ramp (kFeatLoad)
+    const float* xc;     // [4][Npad] X - c0 (3 planes) and |X - c0|^2 - r^2 (kFeatContact)
+    const float* presc;  // [3][Npad] prescribed displacement (kFeatDrive)
+    const uint32_t* ell_off;  // [node_waves + 1] first entry of each wave's slab
+    const uint32_t* ell;      // wave slabs, entry e of lane l at ell_off[w] + 64 e + l
+    // ---- scalars
+    double dt;
+    double load_ramp, drive_ramp;
+    double ind_vel[3];
+    float dtf, lam, mu, bulk;
+    float yield, hard;
+    float gravity, damping;
+    float ind_r, ind_kdt2;   // radius, kappa / dt^2
+    float pad1[2];
+    double lam_d, mu_d;       // FP64 Lame constants for the stable-step evaluation
+    // ---- FP64 accounting
+    double* acc_e;  // [kAccElemSlots][elem_waves]
+    double* acc_n;  // [kAccNodeSlots][node_waves]
+    double* smp_e;  // [kSmpElemSlots][elem_waves]
+    double* smp_n;  // [kSmpNodeSlots][node_waves]
+    double* omega_w;  // [elem_waves] per-wave max of lambda_max(B^T C B) (set-up)
+    // ---- time and status
+    int64_t* step;    // [2] ping-pong step counter
+    int64_t step_end; // stop advancing at this step
+    uint32_t* status; // [1] nonzero: non-finite velocity seen
+    EnergyRecordDev* hist;
+    int32_t hist_cap;
+    int32_t pad2;
+    int32_t* hist_count;
+};
+
+// One block of a pass: which case and the first item (tet or node) it covers.
+struct Tile {
+    int32_t case_index;
+    uint32_t first;
+};
+
+struct LaunchShape {
+    int elem_block = 256;
+    int node_block = 256;
+    int xcd_remap = 1;
+};
+
+// Kernel launchers (kernels.hip). All launch asynchronously on `stream`.
+void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
+                 int sample, int xcd, uint32_t feat_union, hipStream_t stream);
+void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
+                 int sample, int xcd, uint32_t feat_union, hipStream_t stream);
+void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream);
+// FP64 largest eigenvalue of B^T C B per element (the lumped-mass element frequency
+// is omega^2 = 4 lambda / rho), reduced to a per-wave max, then to one value per case.
+void launch_omega(const CaseDev* cases, const Tile* tiles, int ntiles, hipStream_t stream);
+void launch_omega_max(const CaseDev* cases, int ncases, double* out, hipStream_t stream);
+
+}  // namespace shipfem

@@ -1,0 +1,121 @@
+// shipfem_host.hpp -- host-side runtime: case preparation, device arenas, step plans.
+#pragma once
+
+#include "../../include/shipfem.h"
+#include "shipfem_dev.hpp"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+struct shipfem_mesh {
+    enum Kind { Box, Panel } kind = Box;
+    struct {
+        double lx, ly, lz;
+        int32_t nx, ny, nz;
+    } box{};
+    std::vector<double> X;        // [3N]
+    std::vector<uint32_t> index;  // [4T]
+    std::vector<uint32_t> flags;  // [N]
+    std::vector<double> load;     // [3N]
+};
+
+namespace shipfem {
+
+double uniform01(uint64_t seed, uint64_t counter);
+
+inline uint32_t roundUp64(uint64_t n) { return static_cast<uint32_t>((n + 63) / 64 * 64); }
+
+// A case after host preparation: rest state in FP64, stored as the FP32 SoA planes
+// the kernels read, plus the wave-transposed CSR slab of the node gather.
+struct HostCase {
+    uint32_t N = 0, T = 0, Npad = 0, Tpad = 0, node_waves = 0, elem_waves = 0;
+    uint32_t features = 0;
+    std::vector<double> X;        // [3N] rest positions
+    std::vector<uint32_t> conn;   // [4T]
+    std::vector<float> dmi;       // [9][Tpad]
+    std::vector<float> vol;       // [Tpad]
+    std::vector<float> mass, invm;  // [Npad]
+    std::vector<uint32_t> nflags;   // [Npad]
+    std::vector<float> load;        // [3][Npad]
+    std::vector<float> xc;          // [4][Npad]
+    std::vector<float> presc;       // [3][Npad]
+    std::vector<uint32_t> ell_off;  // [node_waves + 1]
+    std::vector<uint32_t> ell;
+    std::vector<float> u4;          // [4N]
+    std::vector<float> cu, v, cv;   // [3][Npad]
+    // material and loading
+    double E = 0, nu = 0, rho = 0, lam = 0, mu = 0, bulk = 0, yield = 0, hard = 0;
+    double load_ramp = 0, drive_ramp = 0, gravity = 0, damping = 1;
+    shipfem_indenter indenter{};
+    double dt_request = 0, dt_safety = 0.9;
+    int32_t sample_every = 100;
+    int64_t max_steps = 0;
+    int32_t hist_cap = 0;
+    double edge_dt = 0;  // the reference's formula at safety 0.5, for the record
+    double total_mass = 0, total_volume = 0;
+};
+
+// Validates and prepares; returns an empty string on success, else the reason.
+std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap = -1);
+
+// Device bytes of a prepared case (its arena), and the admission upper bound.
+uint64_t arenaBytes(const HostCase& hc);
+
+// One case's device memory: a single arena holding every plane of the case.
+struct DeviceCase {
+    char* arena = nullptr;
+    uint64_t bytes = 0;
+    bool owns = false;  // allocated by hipMalloc (else borrowed from a pool)
+    CaseDev rec{};      // device pointers into the arena, scalars filled in
+    double cfl = 0, dt = 0, omega = 0;
+};
+
+// Lays out `hc` in the arena at `base` (which must hold arenaBytes(hc)) and fills
+// rec's pointers and scalars (dt excluded). Uploads on `stream`.
+hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_t stream);
+// Sets dt (and everything derived from it) in the record.
+void setTimestep(DeviceCase& dc, const HostCase& hc, double dt);
+
+// A set of cases stepping together.
+struct Plan {
+    std::vector<CaseDev> recs;     // host copies
+    CaseDev* d_recs = nullptr;     // device array
+    std::vector<Tile> elem_tiles, node_tiles, omega_tiles;
+    Tile* d_elem = nullptr;
+    Tile* d_node = nullptr;
+    Tile* d_omega = nullptr;
+    double* d_omega_out = nullptr;
+    LaunchShape shape;
+    uint32_t feat_union = 0;
+    int parity = 0;
+    int chunk = 0;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_src = nullptr;
+    size_t tile_cap = 0, case_cap = 0;
+
+    ~Plan();
+    // (Re)builds tiles and the device record array for `recs` with `shape`.
+    hipError_t build(hipStream_t stream);
+    // Uploads the host record copies only (after a scalar change).
+    hipError_t uploadRecords(hipStream_t stream);
+    void dropGraph();
+    // One step as plain launches.
+    void stepPlain(bool sample, hipStream_t stream);
+    // Captures `chunk` (even) steps starting at parity 0, sample on the first.
+    hipError_t capture(int chunk, hipStream_t stream);
+    // Evaluates the FP64 stable step of every case: returns omega_max per case.
+    hipError_t omegaMax(std::vector<double>& out, hipStream_t stream);
+};
+
+// Copies a case's state back to the host view.
+hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_view* view,
+                        hipStream_t stream);
+hipError_t readEnergy(const DeviceCase& dc, shipfem_energy_record* out, int32_t max,
+                      int32_t* count, hipStream_t stream);
+
+void setError(char* err, size_t len, const std::string& msg);
+shipfem_status statusOf(hipError_t e);
+bool deviceAvailable(std::string* why);
+
+}  // namespace shipfem

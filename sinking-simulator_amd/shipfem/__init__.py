@@ -1,0 +1,499 @@
+"""Python binding of libshipfem.so (include/shipfem.h) through ctypes.
+
+The compute path is the HIP library; this module only moves numpy arrays across the
+C ABI. It raises ImportError-like RuntimeError when the library is missing: there is
+no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libshipfem.so")
+
+OK, NO_DEVICE, INVALID_ARG, OUT_OF_MEMORY, UNSTABLE, DEVICE_ERROR, BUSY, NOT_FOUND, UNSUPPORTED = range(9)
+PIN_X, PIN_Y, PIN_Z, PIN_ALL = 0x1, 0x2, 0x4, 0x7
+LOADED, CONTACT = 0x8, 0x10
+DRIVE_X, DRIVE_Y, DRIVE_Z = 0x20, 0x40, 0x80
+
+
+class ShipfemError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        super().__init__(f"shipfem status {status} ({status_string(status)}): {msg}")
+
+
+D = C.POINTER(C.c_double)
+F = C.POINTER(C.c_float)
+U32 = C.POINTER(C.c_uint32)
+
+
+class Material(C.Structure):
+    _fields_ = [("youngs_modulus", C.c_double), ("poisson_ratio", C.c_double),
+                ("density", C.c_double), ("plastic", C.c_int32), ("yield_strength", C.c_double),
+                ("hardening_modulus", C.c_double)]
+
+
+class Indenter(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("radius", C.c_double), ("center", C.c_double * 3),
+                ("velocity", C.c_double * 3), ("penalty", C.c_double)]
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [("node_count", C.c_uint32), ("tet_count", C.c_uint32), ("rest_position", D),
+                ("index", U32), ("node_flags", U32), ("load", D), ("prescribed", D),
+                ("initial_displacement", D), ("initial_velocity", D), ("material", Material),
+                ("indenter", Indenter), ("load_ramp", C.c_double), ("drive_ramp", C.c_double),
+                ("gravity", C.c_double), ("damping", C.c_double), ("dt", C.c_double),
+                ("dt_safety", C.c_double), ("sample_every", C.c_int32),
+                ("max_steps", C.c_int64)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double), ("damping", C.c_double),
+                ("youngs_modulus", C.c_double), ("poisson_ratio", C.c_double)]
+
+
+class EnergyRecord(C.Structure):
+    _fields_ = [("step", C.c_int64), ("time", C.c_double), ("strain", C.c_double),
+                ("kinetic", C.c_double), ("dissipation", C.c_double), ("w_ext", C.c_double),
+                ("w_contact", C.c_double), ("w_boundary", C.c_double),
+                ("damping_loss", C.c_double), ("contact_force", C.c_double * 3)]
+
+
+class StateView(C.Structure):
+    _fields_ = [("displacement", D), ("velocity", D), ("position", F), ("eqps", F),
+                ("plastic_strain", F), ("step", C.c_int64), ("time", C.c_double),
+                ("status", C.c_uint32)]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("elem_block", C.c_int32), ("node_block", C.c_int32), ("xcd_remap", C.c_int32),
+                ("chunk", C.c_int32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("max_active", C.c_uint32), ("chunk", C.c_int32),
+                ("launch", LaunchConfig)]
+
+
+class Result(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("status", C.c_int), ("steps", C.c_int64),
+                ("sim_time", C.c_double), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("node_count", C.c_uint32), ("tet_count", C.c_uint32),
+                ("max_displacement", C.c_double), ("max_eqps", C.c_double),
+                ("peak_contact_force", C.c_double), ("last", EnergyRecord)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+                ("failed", C.c_uint64), ("refused", C.c_uint64), ("active", C.c_uint64),
+                ("queued", C.c_uint64), ("bytes_in_use", C.c_uint64),
+                ("bytes_budget", C.c_uint64), ("rounds", C.c_uint64),
+                ("element_steps", C.c_double), ("device_ms", C.c_double)]
+
+
+ENERGY_FIELDS = ["strain", "kinetic", "dissipation", "w_ext", "w_contact", "w_boundary",
+                 "damping_loss"]
+
+# every symbol include/shipfem.h declares, with (restype, argtypes)
+VP = C.c_void_p
+PP = C.POINTER(C.c_void_p)
+S = C.c_int
+SIGNATURES = {
+    "shipfem_status_string": (C.c_char_p, [S]),
+    "shipfem_device_query": (S, [C.c_char_p, C.c_size_t, C.POINTER(C.c_uint64)]),
+    "shipfem_mesh_box": (S, [C.c_double] * 3 + [C.c_int32] * 3 + [PP]),
+    "shipfem_mesh_jitter": (S, [VP, C.c_double, C.c_uint64]),
+    "shipfem_mesh_panel": (S, [C.c_double] * 3 + [C.c_int32] + [C.c_double] * 2 + [C.c_int32] * 3 + [PP]),
+    "shipfem_mesh_counts": (S, [VP, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "shipfem_mesh_arrays": (S, [VP, C.POINTER(D), C.POINTER(U32), C.POINTER(U32), C.POINTER(D)]),
+    "shipfem_mesh_flag_plane": (S, [VP, C.c_int32, C.c_double, C.c_double, C.c_uint32]),
+    "shipfem_mesh_flag_disc": (S, [VP, C.c_int32] + [C.c_double] * 5 + [C.c_uint32]),
+    "shipfem_mesh_traction": (S, [VP, C.c_int32, C.c_double, C.c_double, D]),
+    "shipfem_mesh_fill_case": (S, [VP, C.POINTER(CaseDesc)]),
+    "shipfem_mesh_destroy": (None, [VP]),
+    "shipfem_edge_timestep": (C.c_double, [D, U32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
+    "shipfem_solver_create": (S, [C.POINTER(CaseDesc), PP, C.c_char_p, C.c_size_t]),
+    "shipfem_solver_timestep": (S, [VP, D, D, D]),
+    "shipfem_solver_set_params": (S, [VP, C.POINTER(StepParams)]),
+    "shipfem_solver_run": (S, [VP, C.c_int32, C.c_int32, D]),
+    "shipfem_solver_readback": (S, [VP, C.POINTER(StateView)]),
+    "shipfem_solver_energy": (S, [VP, C.POINTER(EnergyRecord), C.c_int32, C.POINTER(C.c_int32)]),
+    "shipfem_solver_tet_forces": (S, [VP, F]),
+    "shipfem_solver_profile": (S, [VP, C.c_int32, D, D]),
+    "shipfem_solver_footprint": (S, [VP, C.POINTER(C.c_uint64)]),
+    "shipfem_solver_destroy": (None, [VP]),
+    "shipfem_case_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int32]),
+    "shipfem_batch_create": (S, [C.POINTER(CaseDesc), C.c_uint32, C.POINTER(LaunchConfig), PP,
+                                 C.c_char_p, C.c_size_t]),
+    "shipfem_batch_run": (S, [VP, C.c_int32, D]),
+    "shipfem_batch_tune": (S, [VP, C.c_int32, C.POINTER(LaunchConfig), D]),
+    "shipfem_batch_set_config": (S, [VP, C.POINTER(LaunchConfig)]),
+    "shipfem_batch_readback": (S, [VP, C.c_uint32, C.POINTER(StateView)]),
+    "shipfem_batch_energy": (S, [VP, C.c_uint32, C.POINTER(EnergyRecord), C.c_int32,
+                                 C.POINTER(C.c_int32)]),
+    "shipfem_batch_reset": (S, [VP]),
+    "shipfem_batch_destroy": (None, [VP]),
+    "shipfem_server_create": (S, [C.POINTER(ServerOpts), PP, C.c_char_p, C.c_size_t]),
+    "shipfem_server_submit": (S, [VP, C.POINTER(CaseDesc), C.c_double, C.POINTER(C.c_uint64)]),
+    "shipfem_server_poll": (S, [VP, C.POINTER(Result), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "shipfem_server_keep": (S, [VP, C.c_uint64]),
+    "shipfem_server_fetch": (S, [VP, C.c_uint64, C.POINTER(StateView)]),
+    "shipfem_server_drain": (S, [VP, C.c_double]),
+    "shipfem_server_stats_get": (S, [VP, C.POINTER(ServerStats)]),
+    "shipfem_now": (C.c_double, []),
+    "shipfem_server_destroy": (None, [VP]),
+}
+
+_lib = None
+
+
+def lib():
+    """The HIP library. Raises when it is not built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libshipfem.so not built at {LIB_PATH}: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(s):
+    try:
+        return lib().shipfem_status_string(s).decode()
+    except Exception:
+        return "?"
+
+
+def _check(status, msg=""):
+    if status != OK:
+        raise ShipfemError(status, msg)
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_info():
+    """(name, hbm_bytes) or None when there is no usable device."""
+    buf = C.create_string_buffer(256)
+    hbm = C.c_uint64()
+    s = lib().shipfem_device_query(buf, 256, C.byref(hbm))
+    if s == NO_DEVICE:
+        return None
+    _check(s, buf.value.decode())
+    return buf.value.decode(), hbm.value
+
+
+def now():
+    return lib().shipfem_now()
+
+
+# ---------------------------------------------------------------- meshes
+class Mesh:
+    """A case under construction (shipfem_mesh_*). Arrays are views of library memory."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def box(cls, lx, ly, lz, nx, ny, nz):
+        h = C.c_void_p()
+        _check(lib().shipfem_mesh_box(lx, ly, lz, nx, ny, nz, C.byref(h)), "mesh_box")
+        return cls(h.value)
+
+    @classmethod
+    def panel(cls, lx, ly, t_plate, n_stiff, web_h, web_t, nx, ny_bay, nz_web):
+        h = C.c_void_p()
+        _check(lib().shipfem_mesh_panel(lx, ly, t_plate, n_stiff, web_h, web_t, nx, ny_bay,
+                                        nz_web, C.byref(h)), "mesh_panel")
+        return cls(h.value)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().shipfem_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    def jitter(self, amp, seed):
+        _check(lib().shipfem_mesh_jitter(self.h, amp, seed), "mesh_jitter")
+        return self
+
+    def counts(self):
+        n, t = C.c_uint32(), C.c_uint32()
+        lib().shipfem_mesh_counts(self.h, C.byref(n), C.byref(t))
+        return n.value, t.value
+
+    def arrays(self):
+        """Copies: X (N,3) f64, index (T,4) u32, flags (N,) u32, load (N,3) f64."""
+        N, T = self.counts()
+        x, i, f, l = D(), U32(), U32(), D()
+        lib().shipfem_mesh_arrays(self.h, C.byref(x), C.byref(i), C.byref(f), C.byref(l))
+        return (np.ctypeslib.as_array(x, shape=(N, 3)).copy(),
+                np.ctypeslib.as_array(i, shape=(T, 4)).copy(),
+                np.ctypeslib.as_array(f, shape=(N,)).copy(),
+                np.ctypeslib.as_array(l, shape=(N, 3)).copy())
+
+    def flag_plane(self, axis, value, flags, tol=1e-9):
+        _check(lib().shipfem_mesh_flag_plane(self.h, axis, value, tol, flags))
+        return self
+
+    def flag_disc(self, axis, value, c0, c1, radius, flags, tol=1e-9):
+        _check(lib().shipfem_mesh_flag_disc(self.h, axis, value, tol, c0, c1, radius, flags))
+        return self
+
+    def traction(self, axis, value, traction, tol=1e-9):
+        t = (C.c_double * 3)(*traction)
+        _check(lib().shipfem_mesh_traction(self.h, axis, value, tol, t))
+        return self
+
+
+class Case:
+    """A case description: numpy arrays plus scalars, convertible to CaseDesc."""
+
+    def __init__(self, X, index, *, flags=None, load=None, prescribed=None, u0=None, v0=None,
+                 E=210e9, nu=0.3, rho=7850.0, plastic=False, yield_=355e6, hardening=1e9,
+                 indenter=None, load_ramp=0.0, drive_ramp=0.0, gravity=0.0, damping=1.0,
+                 dt=0.0, dt_safety=0.9, sample_every=100, max_steps=0):
+        self.X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
+        self.index = np.ascontiguousarray(index, np.uint32).reshape(-1, 4)
+        c = lambda a, dt_: None if a is None else np.ascontiguousarray(a, dt_)  # noqa: E731
+        self.flags = c(flags, np.uint32)
+        self.load = c(load, np.float64)
+        self.prescribed = c(prescribed, np.float64)
+        self.u0 = c(u0, np.float64)
+        self.v0 = c(v0, np.float64)
+        self.E, self.nu, self.rho = E, nu, rho
+        self.plastic, self.yield_, self.hardening = plastic, yield_, hardening
+        self.indenter = indenter
+        self.load_ramp, self.drive_ramp = load_ramp, drive_ramp
+        self.gravity, self.damping = gravity, damping
+        self.dt, self.dt_safety = dt, dt_safety
+        self.sample_every, self.max_steps = sample_every, max_steps
+
+    @classmethod
+    def from_mesh(cls, mesh, **kw):
+        X, index, flags, load = mesh.arrays()
+        if not np.any(load):
+            load = None
+        return cls(X, index, flags=flags, load=load, **kw)
+
+    @property
+    def N(self):
+        return len(self.X)
+
+    @property
+    def T(self):
+        return len(self.index)
+
+    def desc(self):
+        d = CaseDesc()
+        d.node_count, d.tet_count = self.N, self.T
+        d.rest_position = _p(self.X, C.c_double)
+        d.index = _p(self.index, C.c_uint32)
+        d.node_flags = _p(self.flags, C.c_uint32)
+        d.load = _p(self.load, C.c_double)
+        d.prescribed = _p(self.prescribed, C.c_double)
+        d.initial_displacement = _p(self.u0, C.c_double)
+        d.initial_velocity = _p(self.v0, C.c_double)
+        d.material = Material(self.E, self.nu, self.rho, 1 if self.plastic else 0, self.yield_,
+                              self.hardening)
+        if self.indenter:
+            ind = self.indenter
+            d.indenter = Indenter(1, ind["radius"], (C.c_double * 3)(*ind["center"]),
+                                  (C.c_double * 3)(*ind["velocity"]), ind.get("kappa", 0.1))
+        d.load_ramp, d.drive_ramp = self.load_ramp, self.drive_ramp
+        d.gravity, d.damping = self.gravity, self.damping
+        d.dt, d.dt_safety = self.dt, self.dt_safety
+        d.sample_every, d.max_steps = self.sample_every, self.max_steps
+        return d
+
+    def oracle_kwargs(self):
+        """Keyword arguments of oracle.Case describing the same problem (test use)."""
+        ind = None
+        if self.indenter:
+            ind = dict(self.indenter)
+            ind.setdefault("kappa", 0.1)
+        return dict(E=self.E, nu=self.nu, rho=self.rho, flags=self.flags, load=self.load,
+                    prescribed=self.prescribed, plastic=self.plastic, yield_=self.yield_,
+                    hardening=self.hardening, load_ramp=self.load_ramp,
+                    drive_ramp=self.drive_ramp, gravity=self.gravity, damping=self.damping,
+                    indenter=ind)
+
+
+def _records(arr, n):
+    out = []
+    for i in range(n):
+        r = arr[i]
+        d = {k: getattr(r, k) for k in ["step", "time"] + ENERGY_FIELDS}
+        d["contact_force"] = tuple(r.contact_force)
+        out.append(d)
+    return out
+
+
+class State:
+    def __init__(self, N, T):
+        self.u = np.zeros((N, 3))
+        self.v = np.zeros((N, 3))
+        self.position = np.zeros((N, 3), np.float32)
+        self.eqps = np.zeros(T, np.float32)
+        self.plastic_strain = np.zeros((T, 6), np.float32)
+        self.step = 0
+        self.time = 0.0
+        self.status = 0
+
+    def view(self):
+        v = StateView()
+        v.displacement = _p(self.u, C.c_double)
+        v.velocity = _p(self.v, C.c_double)
+        v.position = _p(self.position, C.c_float)
+        v.eqps = _p(self.eqps, C.c_float)
+        v.plastic_strain = _p(self.plastic_strain, C.c_float)
+        return v
+
+    def take(self, v):
+        self.step, self.time, self.status = v.step, v.time, v.status
+        return self
+
+
+class Solver:
+    """One case on the device (replaces gpu::FemGpuSolver)."""
+
+    def __init__(self, case):
+        self.case = case
+        self._desc = case.desc()
+        err = C.create_string_buffer(512)
+        h = C.c_void_p()
+        _check(lib().shipfem_solver_create(C.byref(self._desc), C.byref(h), err, 512),
+               err.value.decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shipfem_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timestep(self):
+        cfl, dt, edge = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().shipfem_solver_timestep(self.h, C.byref(cfl), C.byref(dt), C.byref(edge)))
+        return dict(cfl=cfl.value, dt=dt.value, edge_dt=edge.value)
+
+    @property
+    def dt(self):
+        return self.timestep()["dt"]
+
+    def set_params(self, dt, gravity=0.0, damping=1.0, E=0.0, nu=0.0):
+        p = StepParams(dt, gravity, damping, E, nu)
+        _check(lib().shipfem_solver_set_params(self.h, C.byref(p)))
+
+    def run(self, steps, mode=0, allow_unstable=False):
+        ms = C.c_double()
+        s = lib().shipfem_solver_run(self.h, steps, mode, C.byref(ms))
+        if not (allow_unstable and s == UNSTABLE):
+            _check(s, "run")
+        return ms.value
+
+    def readback(self):
+        st = State(self.case.N, self.case.T)
+        v = st.view()
+        _check(lib().shipfem_solver_readback(self.h, C.byref(v)))
+        return st.take(v)
+
+    def energy(self):
+        n = C.c_int32()
+        _check(lib().shipfem_solver_energy(self.h, None, 0, C.byref(n)))
+        arr = (EnergyRecord * max(n.value, 1))()
+        _check(lib().shipfem_solver_energy(self.h, arr, n.value, C.byref(n)))
+        return _records(arr, n.value)
+
+    def tet_forces(self):
+        out = np.zeros((self.case.T, 12), np.float32)
+        _check(lib().shipfem_solver_tet_forces(self.h, _p(out, C.c_float)))
+        return out
+
+    def footprint(self):
+        b = C.c_uint64()
+        _check(lib().shipfem_solver_footprint(self.h, C.byref(b)))
+        return b.value
+
+    def profile(self, steps):
+        """Mean (element-pass ms, node-pass ms) over `steps` event-bracketed steps."""
+        e, n = C.c_double(), C.c_double()
+        _check(lib().shipfem_solver_profile(self.h, steps, C.byref(e), C.byref(n)))
+        return e.value, n.value
+
+
+class Batch:
+    """Many cases advancing in one launch per phase."""
+
+    def __init__(self, cases, elem_block=256, node_block=256, xcd_remap=1, chunk=0):
+        self.cases = list(cases)
+        self._descs = (CaseDesc * len(self.cases))(*[c.desc() for c in self.cases])
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk)
+        err = C.create_string_buffer(512)
+        h = C.c_void_p()
+        _check(lib().shipfem_batch_create(self._descs, len(self.cases), C.byref(cfg), C.byref(h),
+                                          err, 512), err.value.decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shipfem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, steps):
+        ms = C.c_double()
+        _check(lib().shipfem_batch_run(self.h, steps, C.byref(ms)))
+        return ms.value
+
+    def tune(self, steps_per_trial=200):
+        cfg = LaunchConfig()
+        ms = C.c_double()
+        _check(lib().shipfem_batch_tune(self.h, steps_per_trial, C.byref(cfg), C.byref(ms)))
+        return dict(elem_block=cfg.elem_block, node_block=cfg.node_block,
+                    xcd_remap=cfg.xcd_remap, chunk=cfg.chunk, ms_per_step=ms.value)
+
+    def set_config(self, elem_block, node_block, xcd_remap):
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0)
+        _check(lib().shipfem_batch_set_config(self.h, C.byref(cfg)))
+
+    def reset(self):
+        _check(lib().shipfem_batch_reset(self.h))
+
+    def readback(self, i):
+        c = self.cases[i]
+        st = State(c.N, c.T)
+        v = st.view()
+        _check(lib().shipfem_batch_readback(self.h, i, C.byref(v)))
+        return st.take(v)
+
+    def energy(self, i):
+        n = C.c_int32()
+        _check(lib().shipfem_batch_energy(self.h, i, None, 0, C.byref(n)))
+        arr = (EnergyRecord * max(n.value, 1))()
+        _check(lib().shipfem_batch_energy(self.h, i, arr, n.value, C.byref(n)))
+        return _records(arr, n.value)
+
+
+def case_footprint(N, T, plastic=False, records=64):
+    return lib().shipfem_case_footprint(N, T, 1 if plastic else 0, records)
