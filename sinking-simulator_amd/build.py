@@ -51,6 +51,11 @@ def build_library(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [j[0] for j in jobs]
          + ["-lpthread"])
     os.replace(tmp, LIB)
+    # the fem_spike driver, linked against the library (rpath to lib/)
+    tool = os.path.join(LIBDIR, "fem_spike")
+    _run([HIPCC] + COMMON + ["-x", "hip", os.path.join(HERE, "tools", "fem_spike.cpp"), "-o",
+                             tool + ".tmp", "-L", LIBDIR, "-lshipfem", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tool + ".tmp", tool)
     return LIB
 
 

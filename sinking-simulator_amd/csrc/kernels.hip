@@ -595,6 +595,17 @@ __global__ __launch_bounds__(kReduceBlock) void k_omega_max(const CaseDev* __res
     if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
+// Step counter and status of every case, for the host to poll with one copy.
+__global__ __launch_bounds__(256) void k_progress(const CaseDev* __restrict__ cases, int n,
+                                                  int64_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const CaseDev& c = cases[i];
+    const int64_t a = c.step[0], b = c.step[1];
+    out[2 * i] = a > b ? a : b;
+    out[2 * i + 1] = static_cast<int64_t>(c.status[0]);
+}
+
 template <int BLOCK>
 void elem_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
                  int xcd, bool plastic, hipStream_t s) {
@@ -651,6 +662,11 @@ void launch_omega(const CaseDev* cases, const Tile* tiles, int ntiles, hipStream
 
 void launch_omega_max(const CaseDev* cases, int ncases, double* out, hipStream_t stream) {
     hipLaunchKernelGGL(k_omega_max, dim3(ncases), dim3(kReduceBlock), 0, stream, cases, out);
+}
+
+void launch_progress(const CaseDev* cases, int ncases, int64_t* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_progress, dim3((ncases + 255) / 256), dim3(256), 0, stream, cases, ncases,
+                       out);
 }
 
 }  // namespace shipfem

@@ -1,32 +1,571 @@
-// server.cpp -- resident serving loop (continuous batching). Placeholder until the
-// serving runtime lands: every entry point reports SHIPFEM_UNSUPPORTED.
+// server.cpp -- resident serving loop with continuous batching.
+//
+// Requests (damage-case solves of uneven size) arrive at any time. Host preparation
+// runs on a small worker pool; one submission thread owns the device. Between device
+// rounds it admits prepared cases in arrival order while their arena fits the device
+// pool (first-fit, coalescing) and a slot is free, retires finished cases, and runs the
+// next round: `chunk` steps of every resident case in one element launch and one node
+// launch per step over a shared tile table. A case's arithmetic and its per-wave
+// reductions never depend on which other cases share the launch, so its result is
+// bit-identical alone and in the mix.
 #include "shipfem_host.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
 
-extern "C" {
-shipfem_status shipfem_server_create(const shipfem_server_opts*, shipfem_server** out, char* err,
-                                     size_t err_len) {
-    if (out) *out = nullptr;
-    shipfem::setError(err, err_len, "serving loop not built");
-    return SHIPFEM_UNSUPPORTED;
-}
-shipfem_status shipfem_server_submit(shipfem_server*, const shipfem_case_desc*, double, uint64_t*) {
-    return SHIPFEM_UNSUPPORTED;
-}
-shipfem_status shipfem_server_poll(shipfem_server*, shipfem_result*, uint32_t, uint32_t*) {
-    return SHIPFEM_UNSUPPORTED;
-}
-shipfem_status shipfem_server_keep(shipfem_server*, uint64_t) { return SHIPFEM_UNSUPPORTED; }
-shipfem_status shipfem_server_fetch(shipfem_server*, uint64_t, shipfem_state_view*) {
-    return SHIPFEM_UNSUPPORTED;
-}
-shipfem_status shipfem_server_drain(shipfem_server*, double) { return SHIPFEM_UNSUPPORTED; }
-shipfem_status shipfem_server_stats_get(shipfem_server*, shipfem_server_stats*) {
-    return SHIPFEM_UNSUPPORTED;
-}
-double shipfem_now(void) {
+using namespace shipfem;
+
+namespace {
+
+double nowSeconds() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-void shipfem_server_destroy(shipfem_server*) {}
+
+// First-fit allocator over one device buffer; free ranges coalesce on release.
+class DevicePool {
+public:
+    hipError_t init(uint64_t bytes) {
+        size_ = bytes;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&base_), bytes);
+        if (e == hipSuccess) free_[0] = bytes;
+        return e;
+    }
+    ~DevicePool() {
+        if (base_) (void)hipFree(base_);
+    }
+    // offset or UINT64_MAX
+    uint64_t alloc(uint64_t bytes) {
+        bytes = (bytes + 4095) / 4096 * 4096;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const uint64_t off = it->first, len = it->second;
+            free_.erase(it);
+            if (len > bytes) free_[off + bytes] = len - bytes;
+            used_ += bytes;
+            return off;
+        }
+        return UINT64_MAX;
+    }
+    void release(uint64_t off, uint64_t bytes) {
+        bytes = (bytes + 4095) / 4096 * 4096;
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    char* base() const { return base_; }
+    uint64_t size() const { return size_; }
+    uint64_t used() const { return used_; }
+
+private:
+    char* base_ = nullptr;
+    uint64_t size_ = 0, used_ = 0;
+    std::map<uint64_t, uint64_t> free_;
+};
+
+struct Request {
+    uint64_t id = 0;
+    double run_to = 0, submit_s = 0, admit_s = 0;
+    // owned copies of the caller's arrays
+    std::vector<double> X, load, presc, u0, v0;
+    std::vector<uint32_t> index, flags;
+    shipfem_case_desc desc{};
+    HostCase hc;
+    std::string error;
+};
+
+struct Active {
+    std::unique_ptr<Request> req;
+    DeviceCase dc;
+    uint64_t off = 0;
+    int64_t steps = 0;
+    int64_t last = 0;  // step reached at the end of the previous round
+};
+
+struct Kept {
+    HostCase hc;  // X and shapes only
+    std::vector<double> u, v;
+    std::vector<float> eqps;
+    int64_t step = 0;
+    double time = 0;
+    uint32_t status = 0;
+};
+
+}  // namespace
+
+struct shipfem_server {
+    shipfem_server_opts opts{};
+    DevicePool pool;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int64_t* d_progress = nullptr;
+    size_t progress_cap = 0;
+
+    std::mutex m;
+    std::condition_variable cv_raw, cv_ready, cv_done;
+    std::deque<std::unique_ptr<Request>> raw, ready;
+    std::deque<shipfem_result> results;
+    std::unordered_map<uint64_t, bool> keep;
+    std::unordered_map<uint64_t, Kept> kept;
+    std::vector<std::thread> workers;
+    std::thread loop;
+    bool stop = false;
+    uint64_t next_id = 1;
+    shipfem_server_stats st{};
+
+    // owned by the loop thread
+    std::vector<Active> active;
+    Plan plan;
+    bool membership_changed = false;
+
+    void workerMain();
+    void loopMain();
+    bool admit();
+    void finish(size_t i, const int64_t* progress, double now);
+    hipError_t round();
+};
+
+void shipfem_server::workerMain() {
+    for (;;) {
+        std::unique_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv_raw.wait(lk, [&] { return stop || !raw.empty(); });
+            if (stop) return;
+            r = std::move(raw.front());
+            raw.pop_front();
+        }
+        r->error = prepareCase(r->desc, r->hc, 512);
+        // the raw copies are no longer needed; the prepared case holds what it uses
+        std::vector<double>().swap(r->load);
+        std::vector<double>().swap(r->presc);
+        std::vector<double>().swap(r->u0);
+        std::vector<double>().swap(r->v0);
+        std::vector<uint32_t>().swap(r->flags);
+        {
+            std::lock_guard<std::mutex> lk(m);
+            if (!r->error.empty()) {
+                shipfem_result res{};
+                res.request_id = r->id;
+                res.status = SHIPFEM_INVALID_ARG;
+                res.submit_s = r->submit_s;
+                res.done_s = nowSeconds();
+                results.push_back(res);
+                ++st.failed;
+                cv_done.notify_all();
+            } else {
+                ready.push_back(std::move(r));
+                cv_ready.notify_all();
+            }
+        }
+    }
 }
+
+// Moves prepared requests into free slots while their arenas fit. Arrival order:
+// the head of the queue waits for room rather than being overtaken.
+bool shipfem_server::admit() {
+    std::vector<std::unique_ptr<Request>> take;
+    std::vector<uint64_t> offs;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        while (!ready.empty() && active.size() + take.size() < opts.max_active) {
+            const uint64_t bytes = arenaBytes(ready.front()->hc);
+            if (bytes > pool.size()) {
+                // can never fit: refuse it rather than block the queue behind it
+                shipfem_result res{};
+                res.request_id = ready.front()->id;
+                res.status = SHIPFEM_BUSY;
+                res.submit_s = ready.front()->submit_s;
+                res.done_s = nowSeconds();
+                results.push_back(res);
+                ++st.refused;
+                ++st.failed;
+                ready.pop_front();
+                cv_done.notify_all();
+                continue;
+            }
+            const uint64_t off = pool.alloc(bytes);
+            if (off == UINT64_MAX) break;
+            take.push_back(std::move(ready.front()));
+            ready.pop_front();
+            offs.push_back(off);
+        }
+    }
+    if (take.empty()) return false;
+    // upload, then the FP64 stable step of the newcomers in one pass
+    Plan fresh;
+    std::vector<Active> add(take.size());
+    hipError_t e = hipSuccess;
+    for (size_t k = 0; k < take.size() && e == hipSuccess; ++k) {
+        add[k].off = offs[k];
+        e = uploadCase(take[k]->hc, pool.base() + offs[k], add[k].dc, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        fresh.recs.push_back(add[k].dc.rec);
+    }
+    std::vector<double> lmax;
+    if (e == hipSuccess) e = fresh.build(stream);
+    if (e == hipSuccess) e = fresh.omegaMax(lmax, stream);
+    const double now = nowSeconds();
+    for (size_t k = 0; k < take.size(); ++k) {
+        Active& a = add[k];
+        HostCase& hc = take[k]->hc;
+        if (e != hipSuccess) {
+            pool.release(a.off, arenaBytes(hc));
+            std::lock_guard<std::mutex> lk(m);
+            shipfem_result res{};
+            res.request_id = take[k]->id;
+            res.status = statusOf(e);
+            res.submit_s = take[k]->submit_s;
+            res.done_s = now;
+            results.push_back(res);
+            ++st.failed;
+            continue;
+        }
+        a.dc.cfl = 2.0 / std::sqrt(4.0 * lmax[k] / hc.rho);
+        setTimestep(a.dc, hc, hc.dt_request > 0 ? hc.dt_request : hc.dt_safety * a.dc.cfl);
+        a.steps = std::max<int64_t>(1, static_cast<int64_t>(std::ceil(take[k]->run_to / a.dc.dt - 1e-9)));
+        a.dc.rec.step_end = a.steps;
+        take[k]->admit_s = now;
+        a.req = std::move(take[k]);
+        active.push_back(std::move(a));
+    }
+    {
+        std::lock_guard<std::mutex> lk(m);
+        st.admitted += take.size();
+        st.active = active.size();
+    }
+    cv_done.notify_all();
+    membership_changed = true;
+    return true;
+}
+
+void shipfem_server::finish(size_t i, const int64_t* progress, double now) {
+    Active& a = active[i];
+    HostCase& hc = a.req->hc;
+    shipfem_result res{};
+    res.request_id = a.req->id;
+    res.status = progress[2 * i + 1] ? SHIPFEM_UNSTABLE : SHIPFEM_OK;
+    res.submit_s = a.req->submit_s;
+    res.admit_s = a.req->admit_s;
+    res.node_count = hc.N;
+    res.tet_count = hc.T;
+    Kept k;
+    k.u.resize(3ull * hc.N);
+    k.v.resize(3ull * hc.N);
+    k.eqps.resize(hc.T);
+    shipfem_state_view view{};
+    view.displacement = k.u.data();
+    view.velocity = k.v.data();
+    view.eqps = k.eqps.data();
+    hipError_t e = readbackCase(a.dc, hc, &view, stream);
+    std::vector<shipfem_energy_record> rec(static_cast<size_t>(a.dc.rec.hist_cap));
+    int32_t nrec = 0;
+    if (e == hipSuccess) e = readEnergy(a.dc, rec.data(), a.dc.rec.hist_cap, &nrec, stream);
+    if (e != hipSuccess) res.status = statusOf(e);
+    res.steps = view.step;
+    res.sim_time = view.step * a.dc.dt;
+    for (uint32_t n = 0; n < hc.N; ++n) {
+        const double* u = &k.u[3ull * n];
+        res.max_displacement = std::max(res.max_displacement, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    }
+    for (float q : k.eqps) res.max_eqps = std::max(res.max_eqps, static_cast<double>(q));
+    for (int32_t r = 0; r < nrec; ++r) {
+        const double* f = rec[r].contact_force;
+        res.peak_contact_force = std::max(res.peak_contact_force, std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
+    }
+    if (nrec > 0) res.last = rec[nrec - 1];
+    res.done_s = nowSeconds();
+    std::lock_guard<std::mutex> lk(m);
+    pool.release(a.off, arenaBytes(hc));
+    auto it = keep.find(res.request_id);
+    if (it != keep.end()) {
+        k.step = view.step;
+        k.time = view.time;
+        k.status = view.status;
+        k.hc.N = hc.N;
+        k.hc.T = hc.T;
+        k.hc.X = std::move(hc.X);
+        kept[res.request_id] = std::move(k);
+        keep.erase(it);
+    }
+    results.push_back(res);
+    if (res.status == SHIPFEM_OK) ++st.completed;
+    else ++st.failed;
+    (void)now;
+}
+
+hipError_t shipfem_server::round() {
+    const int R = opts.chunk;
+    if (membership_changed) {
+        plan.recs.clear();
+        for (auto& a : active) plan.recs.push_back(a.dc.rec);
+        plan.parity = 0;
+        hipError_t e = plan.build(stream);
+        if (e != hipSuccess) return e;
+        e = plan.capture(R, stream);
+        if (e != hipSuccess) return e;
+        membership_changed = false;
+        if (active.size() > progress_cap) {
+            if (d_progress) (void)hipFree(d_progress);
+            progress_cap = active.size() * 2;
+            e = hipMalloc(reinterpret_cast<void**>(&d_progress), 16 * progress_cap);
+            if (e != hipSuccess) return e;
+        }
+    }
+    hipError_t e = hipEventRecord(e0, stream);
+    if (e == hipSuccess) e = hipGraphLaunch(plan.graph, stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, stream);
+    if (e == hipSuccess) launch_progress(plan.d_recs, static_cast<int>(active.size()), d_progress, stream);
+    std::vector<int64_t> prog(2 * active.size());
+    if (e == hipSuccess) e = hipMemcpyAsync(prog.data(), d_progress, 16 * active.size(), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) return e;
+    // element-steps actually advanced this round (finished cases idle in it)
+    double done_steps = 0;
+    for (size_t i = 0; i < active.size(); ++i) {
+        done_steps += static_cast<double>(active[i].req->hc.T) * static_cast<double>(prog[2 * i] - active[i].last);
+        active[i].last = prog[2 * i];
+    }
+    // retire finished (or failed) cases, newest index first so erasure keeps order valid
+    const double now = nowSeconds();
+    for (size_t i = active.size(); i-- > 0;) {
+        const bool over = prog[2 * i] >= active[i].steps || prog[2 * i + 1] != 0;
+        if (over) {
+            finish(i, prog.data(), now);
+            active.erase(active.begin() + static_cast<std::ptrdiff_t>(i));
+            membership_changed = true;
+        }
+    }
+    std::lock_guard<std::mutex> lk(m);
+    st.rounds += 1;
+    st.device_ms += ms;
+    st.element_steps += done_steps;
+    st.active = active.size();
+    cv_done.notify_all();
+    return hipSuccess;
+}
+
+void shipfem_server::loopMain() {
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            if (active.empty())
+                cv_ready.wait(lk, [&] { return stop || !ready.empty(); });
+            if (stop) return;
+        }
+        admit();
+        if (active.empty()) {
+            // nothing fits yet (cannot happen with an empty device) or admission failed
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
+            continue;
+        }
+        const hipError_t e = round();
+        if (e != hipSuccess) {
+            // a device error fails every resident case; the server keeps serving
+            std::vector<int64_t> prog(2 * active.size(), 0);
+            for (size_t i = 0; i < prog.size() / 2; ++i) prog[2 * i + 1] = 1;
+            const double now = nowSeconds();
+            for (size_t i = active.size(); i-- > 0;) finish(i, prog.data(), now);
+            active.clear();
+            membership_changed = true;
+            std::lock_guard<std::mutex> lk(m);
+            st.active = 0;
+            (void)hipGetLastError();
+        }
+    }
+}
+
+extern "C" {
+
+shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_server** out,
+                                     char* err, size_t err_len) {
+    if (!out) return SHIPFEM_INVALID_ARG;
+    *out = nullptr;
+    std::string why;
+    if (!deviceAvailable(&why)) {
+        setError(err, err_len, why);
+        return SHIPFEM_NO_DEVICE;
+    }
+    auto* s = new (std::nothrow) shipfem_server();
+    if (!s) return SHIPFEM_OUT_OF_MEMORY;
+    if (opts) s->opts = *opts;
+    if (s->opts.max_active == 0) s->opts.max_active = 256;
+    if (s->opts.chunk == 0) s->opts.chunk = 64;
+    if (s->opts.chunk < 2 || (s->opts.chunk & 1)) {
+        setError(err, err_len, "chunk must be an even number >= 2");
+        delete s;
+        return SHIPFEM_INVALID_ARG;
+    }
+    const shipfem_launch_config& L = s->opts.launch;
+    s->plan.shape.elem_block = L.elem_block ? L.elem_block : 256;
+    s->plan.shape.node_block = L.node_block ? L.node_block : 256;
+    s->plan.shape.xcd_remap = (L.elem_block || L.node_block) ? L.xcd_remap : 1;
+    if (s->opts.memory_budget == 0) {
+        size_t freeb = 0, total = 0;
+        (void)hipMemGetInfo(&freeb, &total);
+        s->opts.memory_budget = static_cast<uint64_t>(0.6 * static_cast<double>(total));
+    }
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->e0);
+    if (e == hipSuccess) e = hipEventCreate(&s->e1);
+    if (e == hipSuccess) e = s->pool.init(s->opts.memory_budget);
+    if (e != hipSuccess) {
+        setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
+        const shipfem_status st = statusOf(e);
+        if (s->e0) (void)hipEventDestroy(s->e0);
+        if (s->e1) (void)hipEventDestroy(s->e1);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+        return st;
+    }
+    s->st.bytes_budget = s->opts.memory_budget;
+    const unsigned nworkers = std::max(1u, std::min(4u, std::thread::hardware_concurrency() / 2));
+    for (unsigned w = 0; w < nworkers; ++w) s->workers.emplace_back([s] { s->workerMain(); });
+    s->loop = std::thread([s] { s->loopMain(); });
+    *out = s;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc* d, double run_to,
+                                     uint64_t* request_id) {
+    if (!s || !d || !(run_to > 0) || !d->rest_position || !d->index || d->node_count == 0 ||
+        d->tet_count == 0)
+        return SHIPFEM_INVALID_ARG;
+    const uint64_t bound = shipfem_case_footprint(d->node_count, d->tet_count, d->material.plastic, 512);
+    if (bound > s->opts.memory_budget) {
+        std::lock_guard<std::mutex> lk(s->m);
+        ++s->st.refused;
+        return SHIPFEM_BUSY;
+    }
+    auto r = std::make_unique<Request>();
+    const uint64_t N = d->node_count, T = d->tet_count;
+    r->X.assign(d->rest_position, d->rest_position + 3 * N);
+    r->index.assign(d->index, d->index + 4 * T);
+    if (d->node_flags) r->flags.assign(d->node_flags, d->node_flags + N);
+    if (d->load) r->load.assign(d->load, d->load + 3 * N);
+    if (d->prescribed) r->presc.assign(d->prescribed, d->prescribed + 3 * N);
+    if (d->initial_displacement) r->u0.assign(d->initial_displacement, d->initial_displacement + 3 * N);
+    if (d->initial_velocity) r->v0.assign(d->initial_velocity, d->initial_velocity + 3 * N);
+    r->desc = *d;
+    r->desc.rest_position = r->X.data();
+    r->desc.index = r->index.data();
+    r->desc.node_flags = r->flags.empty() ? nullptr : r->flags.data();
+    r->desc.load = r->load.empty() ? nullptr : r->load.data();
+    r->desc.prescribed = r->presc.empty() ? nullptr : r->presc.data();
+    r->desc.initial_displacement = r->u0.empty() ? nullptr : r->u0.data();
+    r->desc.initial_velocity = r->v0.empty() ? nullptr : r->v0.data();
+    // every round samples its first step: records land on multiples of the chunk
+    r->desc.sample_every = s->opts.chunk;
+    r->run_to = run_to;
+    r->submit_s = nowSeconds();
+    std::lock_guard<std::mutex> lk(s->m);
+    r->id = s->next_id++;
+    if (request_id) *request_id = r->id;
+    s->raw.push_back(std::move(r));
+    ++s->st.submitted;
+    s->cv_raw.notify_one();
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_server_poll(shipfem_server* s, shipfem_result* out, uint32_t max,
+                                   uint32_t* count) {
+    if (!s || (!out && max)) return SHIPFEM_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->m);
+    uint32_t n = 0;
+    while (n < max && !s->results.empty()) {
+        out[n++] = s->results.front();
+        s->results.pop_front();
+    }
+    if (count) *count = n;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_server_keep(shipfem_server* s, uint64_t id) {
+    if (!s) return SHIPFEM_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->m);
+    s->keep[id] = true;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_server_fetch(shipfem_server* s, uint64_t id, shipfem_state_view* view) {
+    if (!s || !view) return SHIPFEM_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->kept.find(id);
+    if (it == s->kept.end()) return SHIPFEM_NOT_FOUND;
+    const Kept& k = it->second;
+    for (size_t q = 0; q < k.u.size(); ++q) {
+        if (view->displacement) view->displacement[q] = k.u[q];
+        if (view->velocity) view->velocity[q] = k.v[q];
+        if (view->position) view->position[q] = static_cast<float>(k.hc.X[q] + k.u[q]);
+    }
+    if (view->eqps) std::copy(k.eqps.begin(), k.eqps.end(), view->eqps);
+    view->step = k.step;
+    view->time = k.time;
+    view->status = k.status;
+    s->kept.erase(it);
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_server_drain(shipfem_server* s, double timeout_s) {
+    if (!s) return SHIPFEM_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->m);
+    const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
+        return s->st.completed + s->st.failed >= s->st.submitted;
+    });
+    return ok ? SHIPFEM_OK : SHIPFEM_BUSY;
+}
+
+shipfem_status shipfem_server_stats_get(shipfem_server* s, shipfem_server_stats* out) {
+    if (!s || !out) return SHIPFEM_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->m);
+    *out = s->st;
+    out->queued = s->raw.size() + s->ready.size();
+    out->bytes_in_use = s->pool.used();
+    return SHIPFEM_OK;
+}
+
+double shipfem_now(void) { return nowSeconds(); }
+
+void shipfem_server_destroy(shipfem_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+    }
+    s->cv_raw.notify_all();
+    s->cv_ready.notify_all();
+    for (auto& w : s->workers) w.join();
+    if (s->loop.joinable()) s->loop.join();
+    (void)hipStreamSynchronize(s->stream);
+    s->plan.dropGraph();
+    s->active.clear();
+    if (s->d_progress) (void)hipFree(s->d_progress);
+    (void)hipEventDestroy(s->e0);
+    (void)hipEventDestroy(s->e1);
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // extern "C"

@@ -118,5 +118,7 @@ void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t str
 // is omega^2 = 4 lambda / rho), reduced to a per-wave max, then to one value per case.
 void launch_omega(const CaseDev* cases, const Tile* tiles, int ntiles, hipStream_t stream);
 void launch_omega_max(const CaseDev* cases, int ncases, double* out, hipStream_t stream);
+// out[2i] = current step of case i, out[2i+1] = its status word.
+void launch_progress(const CaseDev* cases, int ncases, int64_t* out, hipStream_t stream);
 
 }  // namespace shipfem

@@ -495,5 +495,64 @@ class Batch:
         return _records(arr, n.value)
 
 
+class Server:
+    """Resident serving loop: submit cases at any time, poll results."""
+
+    def __init__(self, memory_budget=0, max_active=0, chunk=0, elem_block=0, node_block=0,
+                 xcd_remap=1):
+        opts = ServerOpts(memory_budget, max_active, chunk,
+                          LaunchConfig(elem_block, node_block, xcd_remap, 0))
+        err = C.create_string_buffer(512)
+        h = C.c_void_p()
+        _check(lib().shipfem_server_create(C.byref(opts), C.byref(h), err, 512), err.value.decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shipfem_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, case, run_to, keep=False):
+        d = case.desc()
+        rid = C.c_uint64()
+        _check(lib().shipfem_server_submit(self.h, C.byref(d), run_to, C.byref(rid)), "submit")
+        if keep:
+            _check(lib().shipfem_server_keep(self.h, rid.value))
+        return rid.value
+
+    def poll(self, max_results=4096):
+        arr = (Result * max_results)()
+        n = C.c_uint32()
+        _check(lib().shipfem_server_poll(self.h, arr, max_results, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = arr[i]
+            d = {k: getattr(r, k) for k, _ in Result._fields_ if k != "last"}
+            d["last"] = _records([r.last], 1)[0]
+            out.append(d)
+        return out
+
+    def drain(self, timeout=60.0):
+        return lib().shipfem_server_drain(self.h, timeout) == OK
+
+    def fetch(self, rid, case):
+        st = State(case.N, case.T)
+        v = st.view()
+        v.plastic_strain = None
+        _check(lib().shipfem_server_fetch(self.h, rid, C.byref(v)), f"fetch {rid}")
+        return st.take(v)
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().shipfem_server_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+
 def case_footprint(N, T, plastic=False, records=64):
     return lib().shipfem_case_footprint(N, T, 1 if plastic else 0, records)
