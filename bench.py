@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--workload", default="plate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=40)
+    ap.add_argument("--variant", default="tiled", choices=["tiled", "slots"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -112,7 +113,8 @@ def main():
         sys.exit(1)
     case, desc = workload(args.workload)
     T, N = case.T, case.N
-    solver = sf.Solver(case)
+    variant = sf.VARIANT_TILED if args.variant == "tiled" else sf.VARIANT_SLOTS
+    solver = sf.Solver(case).set_launch(variant)
     ts = solver.timestep()
     solver.run(args.warmup)
     t0 = time.perf_counter()
@@ -128,8 +130,9 @@ def main():
     plastic = bool(case.plastic)
     elem_bytes = (ELEM_B_T + (PLASTIC_B_T if plastic else 0)) * T + ELEM_B_N * N
     node_bytes = NODE_B_T * T + NODE_B_N * N
+    ek = "k_elem_tiled" if args.variant == "tiled" else "k_elem"
     if elem_ms >= node_ms:
-        dom, dom_ms, dom_bytes = "element pass (k_elem)", elem_ms, elem_bytes
+        dom, dom_ms, dom_bytes = f"element pass ({ek})", elem_ms, elem_bytes
     else:
         dom, dom_ms, dom_bytes = "node pass (k_node)", node_ms, node_bytes
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -156,7 +159,8 @@ def main():
               vs_baseline_basis="reference sustained 470e6 tet-steps/s (GTX 1070 Ti, Vulkan, FP32, "
                                 "491,520 tets; docs/07-fem-spike-findings.md:89)",
               dtype="fp32", data="synthetic",
-              config=dict(workload=desc, tets=T, nodes=N, dt=ts["dt"], cfl=ts["cfl"],
+              config=dict(workload=desc, tets=T, nodes=N, variant=args.variant, dt=ts["dt"],
+                          cfl=ts["cfl"],
                           reference_edge_dt=ts["edge_dt"], device=info[0],
                           device_ms_per_step=dev_ms / args.steps, step_hbm_gbs=round(step_gbs, 1),
                           step_roofline_frac=round(step_gbs / HBM_PEAK_GBS, 4),

@@ -177,6 +177,25 @@ double shipfem_edge_timestep(const double* rest_position, const uint32_t* index,
                              uint32_t tet_count, double youngs_modulus, double density,
                              double safety);
 
+/* Element-pass variant. TILED (default): 256-tet tiles stage their nodes in LDS and
+ * sum corner forces per tile node before one write; SLOTS: one 48-byte force slot per
+ * tet gathered through the CSR node map, the reference's own scheme
+ * (tet_forces.comp + node_integrate.comp). The two sum a node's forces in a
+ * different association, so their FP32 results differ in the last bits. */
+#define SHIPFEM_VARIANT_TILED 0
+#define SHIPFEM_VARIANT_SLOTS 1
+
+/* Launch shape. For a given variant every choice gives bit-identical results:
+ * reductions are fixed per 64-item wavefront and per 256-tet tile of each case,
+ * independent of block size and order. */
+typedef struct shipfem_launch_config {
+    int32_t elem_block; /* SLOTS: threads per element-pass block (64, 128, 256, 512) */
+    int32_t node_block; /* threads per node-pass block (64, 128, 256, 512) */
+    int32_t xcd_remap;  /* 1: give each XCD a contiguous range of tiles */
+    int32_t chunk;      /* steps per captured graph (even) */
+    int32_t variant;    /* SHIPFEM_VARIANT_* */
+} shipfem_launch_config;
+
 /* ---- one case (replaces gpu::FemGpuSolver, fem_gpu.hpp:17-45) -------------- */
 typedef struct shipfem_solver shipfem_solver;
 
@@ -204,8 +223,11 @@ shipfem_status shipfem_solver_readback(shipfem_solver* s, shipfem_state_view* vi
 shipfem_status shipfem_solver_energy(shipfem_solver* s, shipfem_energy_record* out,
                                      int32_t max_records, int32_t* count);
 /* Per-tet nodal forces of the last element pass, reference layout [12T]
- * (tetForce, fem.hpp:49-50). */
+ * (tetForce, fem.hpp:49-50). SLOTS variant only (SHIPFEM_UNSUPPORTED otherwise). */
 shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out);
+/* Launch shape and element-pass variant of this solver (zeros keep the defaults:
+ * tiled, 256-thread node blocks, XCD remap on). */
+shipfem_status shipfem_solver_set_launch(shipfem_solver* s, const shipfem_launch_config* cfg);
 /* Runs `steps` plain steps with HIP events around every launch on the solver's
  * stream and returns the mean element-pass and node-pass durations in ms (the
  * per-kernel split of the timings fem_gpu.cpp:388-421 takes around a whole run). */
@@ -222,14 +244,6 @@ uint64_t shipfem_case_footprint(uint32_t node_count, uint32_t tet_count, int32_t
 /* ---- many cases in one launch per phase ----------------------------------- */
 typedef struct shipfem_batch shipfem_batch;
 
-/* Launch shape. Any choice gives bit-identical results: reductions are fixed
- * per 64-item wavefront tile of each case, independent of block size and order. */
-typedef struct shipfem_launch_config {
-    int32_t elem_block; /* threads per element-pass block: 64..1024, multiple of 64 */
-    int32_t node_block; /* threads per node-pass block */
-    int32_t xcd_remap;  /* 1: give each XCD a contiguous range of tiles */
-    int32_t chunk;      /* steps per captured graph (even) */
-} shipfem_launch_config;
 
 shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t case_count,
                                     const shipfem_launch_config* cfg, shipfem_batch** out,

@@ -19,7 +19,10 @@ LIB = os.path.join(LIBDIR, "libshipfem.so")
 SOURCES = ["kernels.hip", "mesh.cpp", "prep.cpp", "engine.cpp", "api.cpp", "server.cpp",
            "fem_gpu.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall",
+# -ffp-contract=on: a multiply-add fuses only within one source expression, so every
+# template instantiation of a kernel rounds identically (results do not depend on the
+# launch shape or on which other cases share a launch).
+COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall", "-ffp-contract=on",
           "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include")]
 

@@ -42,7 +42,27 @@ hipError_t advance(Plan& p, int64_t& host_step, int64_t steps, int mode, int S, 
 
 double omegaToCfl(double lambda_max, double rho) { return 2.0 / std::sqrt(4.0 * lambda_max / rho); }
 
+bool blockOk(int x) { return x == 64 || x == 128 || x == 256 || x == 512; }
+
 }  // namespace
+
+namespace shipfem {
+// A launch configuration from the C ABI: zero fields keep the defaults. Returns false
+// for a value the kernels are not built for.
+bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out) {
+    LaunchShape sh;
+    if (cfg) {
+        if (cfg->elem_block) sh.elem_block = cfg->elem_block;
+        if (cfg->node_block) sh.node_block = cfg->node_block;
+        if (cfg->elem_block || cfg->node_block) sh.xcd_remap = cfg->xcd_remap ? 1 : 0;
+        if (cfg->variant != SHIPFEM_VARIANT_TILED && cfg->variant != SHIPFEM_VARIANT_SLOTS) return false;
+        sh.variant = cfg->variant == SHIPFEM_VARIANT_SLOTS ? kSlots : kTiled;
+    }
+    if (!blockOk(sh.elem_block) || !blockOk(sh.node_block)) return false;
+    out = sh;
+    return true;
+}
+}  // namespace shipfem
 
 struct shipfem_solver {
     HostCase hc;
@@ -103,12 +123,16 @@ shipfem_status shipfem_device_query(char* name, size_t name_len, uint64_t* hbm_b
 }
 
 uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t cap) {
-    // upper bound of arenaBytes: every optional plane present, 32-wide gather slab
+    // arenaBytes with every optional plane present, node degree <= 32 and at most two
+    // (tile, node) slots per tet (Kuhn-type meshes hold about 0.75)
     const uint64_t Np = roundUp64(N), Tp = roundUp64(T), nw = Np / 64, ew = Tp / 64;
+    const uint64_t nt = (T + kTile - 1) / kTile, P = 2ull * T;
     uint64_t b = 16ull * T + 40ull * Tp + 48ull * T + (plastic ? 28ull * Tp : 0) + 16ull * N +
                  36ull * Np + 12ull * Np + 40ull * Np + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
+                 24ull * N + 8ull * T + 4ull * (nt + 1) + 4ull * P + 2ull * (P + nt) +
+                 2ull * 4 * kTile * nt + 16ull * P + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
                  8ull * (3 * ew + 10 * nw) + 24 + sizeof(EnergyRecordDev) * (uint64_t)std::max(cap, 1);
-    return b + 256ull * 26;
+    return b + 256ull * 36;
 }
 
 shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
@@ -239,8 +263,22 @@ shipfem_status shipfem_solver_energy(shipfem_solver* s, shipfem_energy_record* o
 
 shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out) {
     if (!s || !out) return SHIPFEM_INVALID_ARG;
+    // only the slots variant keeps per-tet force slots; the tiled one sums in LDS
+    if (s->plan.shape.variant != kSlots) return SHIPFEM_UNSUPPORTED;
     hipError_t e = hipMemcpyAsync(out, s->dc.rec.force, 48ull * s->hc.T, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return statusOf(e);
+}
+
+shipfem_status shipfem_solver_set_launch(shipfem_solver* s, const shipfem_launch_config* cfg) {
+    if (!s || !cfg) return SHIPFEM_INVALID_ARG;
+    LaunchShape sh;
+    if (!shapeFrom(cfg, sh)) return SHIPFEM_INVALID_ARG;
+    // a captured chunk starts at parity 0; changing the launch keeps the cadence
+    s->plan.shape = sh;
+    const int parity = s->plan.parity;
+    const hipError_t e = s->plan.build(s->stream);
+    s->plan.parity = parity;
     return statusOf(e);
 }
 
@@ -256,11 +294,9 @@ shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* 
     for (int k = 0; k < steps && e == hipSuccess; ++k) {
         const int sample = s->host_step % S == 0 ? 1 : 0;
         e = hipEventRecord(ev[3 * k], s->stream);
-        launch_elem(p.d_recs, p.d_elem, static_cast<int>(p.elem_tiles.size()), p.shape.elem_block,
-                    p.parity, sample, p.shape.xcd_remap, p.feat_union, s->stream);
+        p.launchElem(p.parity, sample, s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 1], s->stream);
-        launch_node(p.d_recs, p.d_node, static_cast<int>(p.node_tiles.size()), p.shape.node_block,
-                    p.parity, sample, p.shape.xcd_remap, p.feat_union, s->stream);
+        p.launchNode(p.parity, sample, s->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 2], s->stream);
         if (sample) launch_reduce(p.d_recs, static_cast<int>(p.recs.size()), p.parity, s->stream);
         p.parity ^= 1;
@@ -327,10 +363,10 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
         delete b;
         return SHIPFEM_INVALID_ARG;
     }
-    if (cfg) {
-        b->plan.shape.elem_block = cfg->elem_block ? cfg->elem_block : 256;
-        b->plan.shape.node_block = cfg->node_block ? cfg->node_block : 256;
-        b->plan.shape.xcd_remap = cfg->xcd_remap;
+    if (!shapeFrom(cfg, b->plan.shape)) {
+        setError(err, err_len, "launch config: blocks must be 64/128/256/512, variant TILED or SLOTS");
+        delete b;
+        return SHIPFEM_INVALID_ARG;
     }
     std::vector<uint64_t> off(n + 1, 0);
     for (uint32_t i = 0; i < n; ++i) off[i + 1] = off[i] + (arenaBytes(b->hcs[i]) + 4095) / 4096 * 4096;
@@ -376,6 +412,10 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
         std::vector<float>().swap(hc.dmi);
         std::vector<uint32_t>().swap(hc.ell);
         std::vector<uint32_t>().swap(hc.conn);
+        std::vector<uint32_t>().swap(hc.lconn);
+        std::vector<uint32_t>().swap(hc.tnode);
+        std::vector<uint16_t>().swap(hc.lent);
+        std::vector<uint32_t>().swap(hc.pell);
     }
     *out = b;
     return SHIPFEM_OK;
@@ -396,13 +436,10 @@ shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device
 
 shipfem_status shipfem_batch_set_config(shipfem_batch* b, const shipfem_launch_config* cfg) {
     if (!b || !cfg) return SHIPFEM_INVALID_ARG;
-    const int eb = cfg->elem_block, nb = cfg->node_block;
-    auto ok = [](int x) { return x == 64 || x == 128 || x == 256 || x == 512; };
-    if (!ok(eb) || !ok(nb)) return SHIPFEM_INVALID_ARG;
+    LaunchShape sh;
+    if (!shapeFrom(cfg, sh)) return SHIPFEM_INVALID_ARG;
     if (b->host_step % b->S != 0 || b->plan.parity != 0) return SHIPFEM_INVALID_ARG;
-    b->plan.shape.elem_block = eb;
-    b->plan.shape.node_block = nb;
-    b->plan.shape.xcd_remap = cfg->xcd_remap ? 1 : 0;
+    b->plan.shape = sh;
     return statusOf(b->plan.build(b->stream));
 }
 
@@ -442,6 +479,8 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
     e = timeShape(cur, curMs);
     const int blocks[] = {64, 128, 256, 512};
     for (int pass = 0; pass < 3 && e == hipSuccess; ++pass) {
+        // the tiled variant's element block is its 256-tet tile: not a free parameter
+        if (pass == 1 && cur.variant == kTiled) continue;
         std::vector<LaunchShape> cands;
         for (int v : (pass == 0 ? std::vector<int>{0, 1} : std::vector<int>(blocks, blocks + 4))) {
             LaunchShape c = cur;
@@ -476,6 +515,7 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
         best->node_block = cur.node_block;
         best->xcd_remap = cur.xcd_remap;
         best->chunk = b->S;
+        best->variant = cur.variant == kSlots ? SHIPFEM_VARIANT_SLOTS : SHIPFEM_VARIANT_TILED;
     }
     if (best_ms) *best_ms = curMs;
     return SHIPFEM_OK;

@@ -8,15 +8,6 @@ namespace shipfem {
 
 namespace {
 
-template <typename T>
-hipError_t ensure(T*& p, size_t& cap, size_t need) {
-    if (need <= cap && p) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = std::max<size_t>(need, 1);
-    return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
-}
-
 void tilesFor(const std::vector<CaseDev>& recs, int block, bool nodes, std::vector<Tile>& out) {
     out.clear();
     for (size_t i = 0; i < recs.size(); ++i) {
@@ -47,14 +38,13 @@ void Plan::dropGraph() {
 
 hipError_t Plan::build(hipStream_t s) {
     dropGraph();
-    tilesFor(recs, shape.elem_block, false, elem_tiles);
+    tilesFor(recs, shape.variant == kTiled ? kTile : shape.elem_block, false, elem_tiles);
+    lmax = 0;
+    for (const CaseDev& r : recs) lmax = std::max(lmax, r.lmax);
     tilesFor(recs, shape.node_block, true, node_tiles);
     tilesFor(recs, 256, false, omega_tiles);
     feat_union = 0;
     for (const CaseDev& r : recs) feat_union |= r.features;
-    // tile arrays share one capacity counter each
-    static thread_local size_t dummy = 0;
-    (void)dummy;
     hipError_t e = hipSuccess;
     size_t need = std::max({elem_tiles.size(), node_tiles.size(), omega_tiles.size()});
     if (need > tile_cap || !d_elem) {
@@ -97,11 +87,27 @@ hipError_t Plan::uploadRecords(hipStream_t s) {
     return e;
 }
 
+void Plan::launchStep(int par, int sample, hipStream_t s) {
+    launchElem(par, sample, s);
+    launchNode(par, sample, s);
+}
+
+void Plan::launchElem(int par, int sample, hipStream_t s) {
+    if (shape.variant == kTiled)
+        launch_elem_tiled(d_recs, d_elem, static_cast<int>(elem_tiles.size()), lmax, par, sample,
+                          shape.xcd_remap, feat_union, s);
+    else
+        launch_elem(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
+                    sample, shape.xcd_remap, feat_union, s);
+}
+
+void Plan::launchNode(int par, int sample, hipStream_t s) {
+    launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, par, sample,
+                shape.xcd_remap, feat_union, shape.variant, s);
+}
+
 void Plan::stepPlain(bool sample, hipStream_t s) {
-    launch_elem(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, parity,
-                sample ? 1 : 0, shape.xcd_remap, feat_union, s);
-    launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, parity,
-                sample ? 1 : 0, shape.xcd_remap, feat_union, s);
+    launchStep(parity, sample ? 1 : 0, s);
     if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), parity, s);
     parity ^= 1;
 }
@@ -114,10 +120,7 @@ hipError_t Plan::capture(int steps, hipStream_t s) {
     int par = 0;
     for (int k = 0; k < steps; ++k) {
         const int sample = k == 0 ? 1 : 0;
-        launch_elem(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
-                    sample, shape.xcd_remap, feat_union, s);
-        launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, par,
-                    sample, shape.xcd_remap, feat_union, s);
+        launchStep(par, sample, s);
         if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), par, s);
         par ^= 1;
     }

@@ -40,7 +40,180 @@ __device__ __forceinline__ double wave_max(double s) {
     return s;
 }
 
-// ---------------------------------------------------------------- element pass
+// ---------------------------------------------------------------- element math
+// Forces of one tet from its corner displacements p0..p3, Dm^-1 (B, row-major) and
+// rest volume V. PLASTIC cases read and update the tet's J2 state. Adds V sigma:d eps_p
+// to diss and, on sample steps, the strain energy 1/2 V sigma:(eps - eps_p) to se.
+template <bool PLASTIC>
+__device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, const float4& p0,
+                                              const float4& p1, const float4& p2, const float4& p3,
+                                              const float B[9], float V, int sample, float f[12],
+                                              double& diss, double& se) {
+    // Du: columns are corner displacements relative to corner 0
+    const float D[9] = {p1.x - p0.x, p2.x - p0.x, p3.x - p0.x,
+                        p1.y - p0.y, p2.y - p0.y, p3.y - p0.y,
+                        p1.z - p0.z, p2.z - p0.z, p3.z - p0.z};
+    float G[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            G[i * 3 + j] = D[i * 3 + 0] * B[0 * 3 + j] + D[i * 3 + 1] * B[1 * 3 + j] +
+                           D[i * 3 + 2] * B[2 * 3 + j];
+
+    // R = I + W by R <- (R + R^-T)/2 with inv(I+W) - I = (W^2 - (1+t)W - det(W) I)/det(I+W)
+    float W[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) W[k] = G[k];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        float W2[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                W2[i * 3 + j] = W[i * 3 + 0] * W[0 * 3 + j] + W[i * 3 + 1] * W[1 * 3 + j] +
+                                W[i * 3 + 2] * W[2 * 3 + j];
+        const float t = W[0] + W[4] + W[8];
+        const float i2 = 0.5f * (t * t - (W2[0] + W2[4] + W2[8]));
+        const float dW = W[0] * (W[4] * W[8] - W[5] * W[7]) - W[1] * (W[3] * W[8] - W[5] * W[6]) +
+                         W[2] * (W[3] * W[7] - W[4] * W[6]);
+        const float den = 1.0f + (t + i2 + dW);
+        const bool ok = fabsf(den) > 1e-30f;  // the reference's singular guard
+        const float id = ok ? 1.0f / den : 0.0f;
+        float En[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float diag = i == j ? 1.0f : 0.0f;
+                En[i * 3 + j] = ok ? (W2[i * 3 + j] - (1.0f + t) * W[i * 3 + j] - dW * diag) * id : -diag;
+            }
+        float Wn[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Wn[i * 3 + j] = 0.5f * (W[i * 3 + j] + En[j * 3 + i]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) W[k] = Wn[k];
+    }
+
+    // co-rotated small strain: R^T F - I = W^T + G + W^T G
+    float A[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            A[i * 3 + j] = W[j * 3 + i] + G[i * 3 + j] +
+                           (W[0 * 3 + i] * G[0 * 3 + j] + W[1 * 3 + i] * G[1 * 3 + j] +
+                            W[2 * 3 + i] * G[2 * 3 + j]);
+    const float e00 = A[0], e11 = A[4], e22 = A[8];
+    const float e01 = 0.5f * (A[1] + A[3]), e12 = 0.5f * (A[5] + A[7]), e02 = 0.5f * (A[2] + A[6]);
+
+    float s00, s11, s22, s01, s12, s02;
+    if (PLASTIC && (c.features & kFeatPlastic)) {
+        // J2 radial return, linear isotropic hardening (plasticity.cpp:319-455)
+        const uint32_t tp = c.Tpad;
+        float ep[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) ep[k] = c.pstate[k * tp + e];
+        const float st[6] = {e00, e11, e22, 2.0f * e01, 2.0f * e12, 2.0f * e02};
+        float el[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) el[k] = st[k] - ep[k];
+        const float vol = el[0] + el[1] + el[2];
+        const float pr = c.bulk * vol;
+        float tr[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            tr[k] = 2.0f * c.mu * (el[k] - vol * (1.0f / 3.0f));
+            tr[3 + k] = c.mu * el[3 + k];
+        }
+        const float nrm = sqrtf(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
+                                2.0f * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
+        const float root23 = 0.81649658092772603f;
+        const float radius = root23 * (c.yield + c.hard * fmaxf(ep[6], 0.0f));
+        float sv[6];
+        if (nrm > radius) {
+            const float gamma = (nrm - radius) / (2.0f * c.mu + (2.0f / 3.0f) * c.hard);
+            const float inv = 1.0f / nrm;
+            float d = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float n0 = tr[k] * inv, n1 = tr[3 + k] * inv;
+                sv[k] = pr + tr[k] - 2.0f * c.mu * gamma * n0;
+                sv[3 + k] = tr[3 + k] - 2.0f * c.mu * gamma * n1;
+                const float dn = gamma * n0, ds = 2.0f * gamma * n1;
+                d += sv[k] * dn + sv[3 + k] * ds;
+                ep[k] += dn;
+                ep[3 + k] += ds;
+            }
+            ep[6] += root23 * gamma;
+            diss += (double)d * (double)V;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) c.pstate[k * tp + e] = ep[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                sv[k] = pr + tr[k];
+                sv[3 + k] = tr[3 + k];
+            }
+        }
+        if (sample) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += sv[k] * (st[k] - ep[k]);
+            se += 0.5 * (double)s * (double)V;
+        }
+        s00 = sv[0]; s11 = sv[1]; s22 = sv[2]; s01 = sv[3]; s12 = sv[4]; s02 = sv[5];
+    } else {
+        const float tr = e00 + e11 + e22;
+        const float m2 = 2.0f * c.mu, lt = c.lam * tr;
+        s00 = m2 * e00 + lt; s11 = m2 * e11 + lt; s22 = m2 * e22 + lt;
+        s01 = m2 * e01; s12 = m2 * e12; s02 = m2 * e02;
+        if (sample) {
+            const float s = s00 * e00 + s11 * e11 + s22 * e22 + 2.0f * (s01 * e01 + s12 * e12 + s02 * e02);
+            se += 0.5 * (double)s * (double)V;
+        }
+    }
+    const float S[9] = {s00, s01, s02, s01, s11, s12, s02, s12, s22};
+    // P = (I + W) sigma, H = P Dm^-T, f_c = -V H(:, c-1), f_0 = -(f_1 + f_2 + f_3)
+    float P[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            P[i * 3 + j] = S[i * 3 + j] + (W[i * 3 + 0] * S[0 * 3 + j] + W[i * 3 + 1] * S[1 * 3 + j] +
+                                           W[i * 3 + 2] * S[2 * 3 + j]);
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            f[(cc + 1) * 3 + a] = -V * (P[a * 3 + 0] * B[cc * 3 + 0] + P[a * 3 + 1] * B[cc * 3 + 1] +
+                                        P[a * 3 + 2] * B[cc * 3 + 2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[a] = -(f[3 + a] + f[6 + a] + f[9 + a]);
+}
+
+// Per-wave FP64 sums of the element pass into the case's wave slots. Every lane of the
+// wave takes part (dead lanes add zero); a tail block's waves past the case's last wave
+// hold only dead lanes and write nothing.
+template <bool PLASTIC>
+__device__ __forceinline__ void element_wave_sums(const CaseDev& c, uint32_t e, int sample,
+                                                  double diss, double se) {
+    const uint32_t w = e >> 6;
+    const bool writer = (threadIdx.x & 63u) == 0 && w < c.elem_waves;
+    if (PLASTIC && (c.features & kFeatPlastic)) {
+        const double s = wave_sum(diss);
+        if (writer) c.acc_e[kAccDiss * c.elem_waves + w] += s;
+    }
+    if (sample) {
+        const double s = wave_sum(se);
+        if (writer) c.smp_e[kSmpStrain * c.elem_waves + w] = s;
+    }
+}
+
+// ---------------------------------------------------------------- element pass, slots
 template <int BLOCK, bool PLASTIC>
 __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ cases,
                                                 const Tile* __restrict__ tiles, int ntiles,
@@ -51,190 +224,102 @@ __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ case
     if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
 
     const uint32_t e = tile.first + threadIdx.x;
-    const bool live = e < c.T;
-    double diss = 0.0, strain_energy = 0.0;
-
-    if (live) {
+    double diss = 0.0, se = 0.0;
+    if (e < c.T) {
         const uint4 nd = c.conn[e];
         const float4 p0 = c.u4[nd.x], p1 = c.u4[nd.y], p2 = c.u4[nd.z], p3 = c.u4[nd.w];
-        const uint32_t tp = c.Tpad;
         float B[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * tp + e];
-        const float V = c.vol[e];
-
-        // Du: columns are corner displacements relative to corner 0
-        const float D[9] = {p1.x - p0.x, p2.x - p0.x, p3.x - p0.x,
-                            p1.y - p0.y, p2.y - p0.y, p3.y - p0.y,
-                            p1.z - p0.z, p2.z - p0.z, p3.z - p0.z};
-        float G[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                G[i * 3 + j] = D[i * 3 + 0] * B[0 * 3 + j] + D[i * 3 + 1] * B[1 * 3 + j] +
-                               D[i * 3 + 2] * B[2 * 3 + j];
-
-        // R = I + W by R <- (R + R^-T)/2 with inv(I+W) - I = (W^2 - (1+t)W - det(W) I)/det(I+W)
-        float W[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) W[k] = G[k];
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            float W2[9];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    W2[i * 3 + j] = W[i * 3 + 0] * W[0 * 3 + j] + W[i * 3 + 1] * W[1 * 3 + j] +
-                                    W[i * 3 + 2] * W[2 * 3 + j];
-            const float t = W[0] + W[4] + W[8];
-            const float i2 = 0.5f * (t * t - (W2[0] + W2[4] + W2[8]));
-            const float dW = W[0] * (W[4] * W[8] - W[5] * W[7]) -
-                             W[1] * (W[3] * W[8] - W[5] * W[6]) +
-                             W[2] * (W[3] * W[7] - W[4] * W[6]);
-            const float den = 1.0f + (t + i2 + dW);
-            const bool ok = fabsf(den) > 1e-30f;  // the reference's singular guard
-            const float id = ok ? 1.0f / den : 0.0f;
-            float En[9];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const float diag = i == j ? 1.0f : 0.0f;
-                    En[i * 3 + j] = ok ? (W2[i * 3 + j] - (1.0f + t) * W[i * 3 + j] - dW * diag) * id
-                                       : -diag;
-                }
-            float Wn[9];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) Wn[i * 3 + j] = 0.5f * (W[i * 3 + j] + En[j * 3 + i]);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) W[k] = Wn[k];
-        }
-
-        // co-rotated small strain: R^T F - I = W^T + G + W^T G
-        float A[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                A[i * 3 + j] = W[j * 3 + i] + G[i * 3 + j] +
-                               (W[0 * 3 + i] * G[0 * 3 + j] + W[1 * 3 + i] * G[1 * 3 + j] +
-                                W[2 * 3 + i] * G[2 * 3 + j]);
-        const float e00 = A[0], e11 = A[4], e22 = A[8];
-        const float e01 = 0.5f * (A[1] + A[3]), e12 = 0.5f * (A[5] + A[7]),
-                    e02 = 0.5f * (A[2] + A[6]);
-
-        float s00, s11, s22, s01, s12, s02;
-        if (PLASTIC && (c.features & kFeatPlastic)) {
-            // J2 radial return, linear isotropic hardening (plasticity.cpp:319-455)
-            float ep[7];
-#pragma unroll
-            for (int k = 0; k < 7; ++k) ep[k] = c.pstate[k * tp + e];
-            const float st[6] = {e00, e11, e22, 2.0f * e01, 2.0f * e12, 2.0f * e02};
-            float el[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) el[k] = st[k] - ep[k];
-            const float vol = el[0] + el[1] + el[2];
-            const float pr = c.bulk * vol;
-            float tr[6];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                tr[k] = 2.0f * c.mu * (el[k] - vol * (1.0f / 3.0f));
-                tr[3 + k] = c.mu * el[3 + k];
-            }
-            const float nrm = sqrtf(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
-                                    2.0f * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
-            const float root23 = 0.81649658092772603f;
-            const float radius = root23 * (c.yield + c.hard * fmaxf(ep[6], 0.0f));
-            float sv[6];
-            if (nrm > radius) {
-                const float gamma = (nrm - radius) / (2.0f * c.mu + (2.0f / 3.0f) * c.hard);
-                const float inv = 1.0f / nrm;
-                float d = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float n0 = tr[k] * inv, n1 = tr[3 + k] * inv;
-                    sv[k] = pr + tr[k] - 2.0f * c.mu * gamma * n0;
-                    sv[3 + k] = tr[3 + k] - 2.0f * c.mu * gamma * n1;
-                    const float dn = gamma * n0, ds = 2.0f * gamma * n1;
-                    d += sv[k] * dn + sv[3 + k] * ds;
-                    ep[k] += dn;
-                    ep[3 + k] += ds;
-                }
-                ep[6] += root23 * gamma;
-                diss = (double)d * (double)V;
-#pragma unroll
-                for (int k = 0; k < 7; ++k) c.pstate[k * tp + e] = ep[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    sv[k] = pr + tr[k];
-                    sv[3 + k] = tr[3 + k];
-                }
-            }
-            if (sample) {
-                float s = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) s += sv[k] * (st[k] - ep[k]);
-                strain_energy = 0.5 * (double)s * (double)V;
-            }
-            s00 = sv[0]; s11 = sv[1]; s22 = sv[2]; s01 = sv[3]; s12 = sv[4]; s02 = sv[5];
-        } else {
-            const float tr = e00 + e11 + e22;
-            const float m2 = 2.0f * c.mu, lt = c.lam * tr;
-            s00 = m2 * e00 + lt; s11 = m2 * e11 + lt; s22 = m2 * e22 + lt;
-            s01 = m2 * e01; s12 = m2 * e12; s02 = m2 * e02;
-            if (sample) {
-                const float s = s00 * e00 + s11 * e11 + s22 * e22 +
-                                2.0f * (s01 * e01 + s12 * e12 + s02 * e02);
-                strain_energy = 0.5 * (double)s * (double)V;
-            }
-        }
-        const float S[9] = {s00, s01, s02, s01, s11, s12, s02, s12, s22};
-        // P = (I + W) sigma, H = P Dm^-T, f_c = -V H(:, c-1), f_0 = -(f_1 + f_2 + f_3)
-        float P[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                P[i * 3 + j] = S[i * 3 + j] + (W[i * 3 + 0] * S[0 * 3 + j] +
-                                               W[i * 3 + 1] * S[1 * 3 + j] +
-                                               W[i * 3 + 2] * S[2 * 3 + j]);
+        for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * c.Tpad + e];
         float f[12];
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                f[(cc + 1) * 3 + a] = -V * (P[a * 3 + 0] * B[cc * 3 + 0] + P[a * 3 + 1] * B[cc * 3 + 1] +
-                                            P[a * 3 + 2] * B[cc * 3 + 2]);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) f[a] = -(f[3 + a] + f[6 + a] + f[9 + a]);
+        element_force<PLASTIC>(c, e, p0, p1, p2, p3, B, c.vol[e], sample, f, diss, se);
         float4* out = reinterpret_cast<float4*>(c.force) + (size_t)e * 3;
         out[0] = make_float4(f[0], f[1], f[2], f[3]);
         out[1] = make_float4(f[4], f[5], f[6], f[7]);
         out[2] = make_float4(f[8], f[9], f[10], f[11]);
     }
+    element_wave_sums<PLASTIC>(c, e, sample, diss, se);
+}
 
-    // per-wave FP64 sums: every lane of the wave takes part (dead lanes add zero)
-    // (a tail block's waves past the case's last wave hold only dead lanes and
-    // write nothing)
-    const uint32_t w = (tile.first + threadIdx.x) >> 6;
-    const bool writer = (threadIdx.x & 63u) == 0 && w < c.elem_waves;
-    if (PLASTIC && (c.features & kFeatPlastic)) {
-        const double s = wave_sum(diss);
-        if (writer) c.acc_e[kAccDiss * c.elem_waves + w] += s;
+// ---------------------------------------------------------------- element pass, tiled
+// One block per 256-tet tile. LDS: corner forces [12][256] floats, then the tile's
+// local nodes: displacement float4[lmax], rest position double[3 lmax].
+template <bool PLASTIC>
+__global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict__ cases,
+                                                      const Tile* __restrict__ tiles, int ntiles,
+                                                      uint32_t lmax, int parity, int sample,
+                                                      int xcd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* sh_f = reinterpret_cast<float*>(smem);
+    float4* sh_u = reinterpret_cast<float4*>(smem + 12 * kTile * sizeof(float));
+    double* sh_x = reinterpret_cast<double*>(smem + 12 * kTile * sizeof(float) + 16 * (size_t)lmax);
+
+    const int b = remap_block(blockIdx.x, gridDim.x, xcd);
+    const Tile tile = tiles[b];
+    const CaseDev& c = cases[tile.case_index];
+    if (c.step[parity] >= c.step_end) return;
+
+    const uint32_t tb = tile.first / kTile;
+    const uint32_t p0 = c.tn_off[tb];
+    const uint32_t L = c.tn_off[tb + 1] - p0;
+    for (uint32_t l = threadIdx.x; l < L; l += kTile) {
+        const uint32_t g = c.tnode[p0 + l];
+        sh_u[l] = c.u4[g];
+        sh_x[3 * l + 0] = c.Xd[3 * (size_t)g + 0];
+        sh_x[3 * l + 1] = c.Xd[3 * (size_t)g + 1];
+        sh_x[3 * l + 2] = c.Xd[3 * (size_t)g + 2];
     }
-    if (sample) {
-        const double s = wave_sum(strain_energy);
-        if (writer) c.smp_e[kSmpStrain * c.elem_waves + w] = s;
+    __syncthreads();
+
+    const uint32_t e = tile.first + threadIdx.x;
+    double diss = 0.0, se = 0.0;
+    if (e < c.T) {
+        const uint2 lc = c.lconn[e];
+        const uint32_t l0 = lc.x & 0xFFFFu, l1 = lc.x >> 16, l2 = lc.y & 0xFFFFu, l3 = lc.y >> 16;
+        // Dm^-1 and V from the rest coordinates, in FP64, rounded once
+        const double* x0 = sh_x + 3 * l0;
+        const double* x1 = sh_x + 3 * l1;
+        const double* x2 = sh_x + 3 * l2;
+        const double* x3 = sh_x + 3 * l3;
+        const double a[9] = {x1[0] - x0[0], x2[0] - x0[0], x3[0] - x0[0],
+                             x1[1] - x0[1], x2[1] - x0[1], x3[1] - x0[1],
+                             x1[2] - x0[2], x2[2] - x0[2], x3[2] - x0[2]};
+        const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                           a[2] * (a[3] * a[7] - a[4] * a[6]);
+        const double id = 1.0 / det;
+        const float B[9] = {(float)((a[4] * a[8] - a[5] * a[7]) * id), (float)((a[2] * a[7] - a[1] * a[8]) * id),
+                            (float)((a[1] * a[5] - a[2] * a[4]) * id), (float)((a[5] * a[6] - a[3] * a[8]) * id),
+                            (float)((a[0] * a[8] - a[2] * a[6]) * id), (float)((a[2] * a[3] - a[0] * a[5]) * id),
+                            (float)((a[3] * a[7] - a[4] * a[6]) * id), (float)((a[1] * a[6] - a[0] * a[7]) * id),
+                            (float)((a[0] * a[4] - a[1] * a[3]) * id)};
+        const float V = (float)(fabs(det) * (1.0 / 6.0));
+        float f[12];
+        element_force<PLASTIC>(c, e, sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3], B, V, sample, f, diss, se);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) sh_f[k * kTile + threadIdx.x] = f[k];
+    }
+    element_wave_sums<PLASTIC>(c, e, sample, diss, se);
+    __syncthreads();
+
+    // one lane per local node sums its corners in the tile's (tet, corner) order
+    const uint16_t* lo = c.lofs + p0 + tb;
+    const uint16_t* le = c.lent + (size_t)tb * (4 * kTile);
+    for (uint32_t l = threadIdx.x; l < L; l += kTile) {
+        const uint32_t q0 = lo[l], q1 = lo[l + 1];
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        for (uint32_t q = q0; q < q1; ++q) {
+            const uint32_t en = le[q];
+            const uint32_t t = en >> 2, cr = (en & 3u) * 3u;
+            fx += sh_f[(cr + 0) * kTile + t];
+            fy += sh_f[(cr + 1) * kTile + t];
+            fz += sh_f[(cr + 2) * kTile + t];
+        }
+        c.partial[p0 + l] = make_float4(fx, fy, fz, 0.0f);
     }
 }
 
 // ---------------------------------------------------------------- node pass
-template <int BLOCK, bool EXTRA>
+template <int BLOCK, bool EXTRA, int VARIANT>
 __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ cases,
                                                 const Tile* __restrict__ tiles, int ntiles,
                                                 int parity, int sample, int xcd) {
@@ -259,17 +344,31 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
     double s_kin = 0.0, s_hext = 0.0, s_hcon = 0.0, s_f[3] = {0.0, 0.0, 0.0};
 
     if ((fl & kPinAll) != kPinAll) {
-        // gather this node's slots in CSR (tet, corner) order
-        const uint32_t base = c.ell_off[w];
-        const uint32_t width = (c.ell_off[w + 1] - base) >> 6;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        for (uint32_t q = 0; q < width; ++q) {
-            const uint32_t en = c.ell[base + q * 64u + lane];
-            if (en == kEllSentinel) break;
-            const float* s = c.force + (size_t)(en >> 2) * 12u + (en & 3u) * 3u;
-            fx += s[0];
-            fy += s[1];
-            fz += s[2];
+        if (VARIANT == kSlots) {
+            // gather this node's slots in CSR (tet, corner) order
+            const uint32_t base = c.ell_off[w];
+            const uint32_t width = (c.ell_off[w + 1] - base) >> 6;
+            for (uint32_t q = 0; q < width; ++q) {
+                const uint32_t en = c.ell[base + q * 64u + lane];
+                if (en == kEllSentinel) break;
+                const float* s = c.force + (size_t)(en >> 2) * 12u + (en & 3u) * 3u;
+                fx += s[0];
+                fy += s[1];
+                fz += s[2];
+            }
+        } else {
+            // gather this node's (tile, node) partials in tile order
+            const uint32_t base = c.pell_off[w];
+            const uint32_t width = (c.pell_off[w + 1] - base) >> 6;
+            for (uint32_t q = 0; q < width; ++q) {
+                const uint32_t p = c.pell[base + q * 64u + lane];
+                if (p == kEllSentinel) break;
+                const float4 s = c.partial[p];
+                fx += s.x;
+                fy += s.y;
+                fz += s.z;
+            }
         }
         const float m = c.mass[i];
         const float im = c.invm[i];
@@ -617,15 +716,24 @@ void elem_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity
                            ntiles, parity, sample, xcd);
 }
 
-template <int BLOCK>
+template <int BLOCK, int VARIANT>
 void node_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
                  int xcd, bool extra, hipStream_t s) {
     if (extra)
-        hipLaunchKernelGGL((k_node<BLOCK, true>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
-                           ntiles, parity, sample, xcd);
+        hipLaunchKernelGGL((k_node<BLOCK, true, VARIANT>), dim3(ntiles), dim3(BLOCK), 0, s, cases,
+                           tiles, ntiles, parity, sample, xcd);
     else
-        hipLaunchKernelGGL((k_node<BLOCK, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
-                           ntiles, parity, sample, xcd);
+        hipLaunchKernelGGL((k_node<BLOCK, false, VARIANT>), dim3(ntiles), dim3(BLOCK), 0, s, cases,
+                           tiles, ntiles, parity, sample, xcd);
+}
+
+template <int BLOCK>
+void node_launch_v(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
+                   int xcd, bool extra, int variant, hipStream_t s) {
+    if (variant == kTiled)
+        node_launch<BLOCK, kTiled>(cases, tiles, ntiles, parity, sample, xcd, extra, s);
+    else
+        node_launch<BLOCK, kSlots>(cases, tiles, ntiles, parity, sample, xcd, extra, s);
 }
 
 }  // namespace
@@ -642,14 +750,25 @@ void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
 }
 
 void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
-                 int sample, int xcd, uint32_t feat_union, hipStream_t stream) {
+                 int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream) {
     const bool extra = (feat_union & (kFeatContact | kFeatDrive | kFeatDamp)) != 0;
     switch (block) {
-        case 64: node_launch<64>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
-        case 128: node_launch<128>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
-        case 512: node_launch<512>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
-        default: node_launch<256>(cases, tiles, ntiles, parity, sample, xcd, extra, stream); break;
+        case 64: node_launch_v<64>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
+        case 128: node_launch_v<128>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
+        case 512: node_launch_v<512>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
+        default: node_launch_v<256>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
     }
+}
+
+void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
+                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream) {
+    const size_t lds = 12 * kTile * sizeof(float) + (16 + 24) * (size_t)lmax;
+    if (feat_union & kFeatPlastic)
+        hipLaunchKernelGGL((k_elem_tiled<true>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
+                           ntiles, lmax, parity, sample, xcd);
+    else
+        hipLaunchKernelGGL((k_elem_tiled<false>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
+                           ntiles, lmax, parity, sample, xcd);
 }
 
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream) {

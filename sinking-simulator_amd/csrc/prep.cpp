@@ -35,7 +35,8 @@ struct Layout {
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
     uint64_t conn, dmi, vol, force, pstate, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
-        ell_off, ell, acc_e, acc_n, smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
+        ell_off, ell, Xd, lconn, tn_off, tnode, lofs, lent, partial, pell_off, pell, acc_e, acc_n,
+        smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
 
 ArenaMap mapArena(const HostCase& hc) {
@@ -59,6 +60,15 @@ ArenaMap mapArena(const HostCase& hc) {
     m.presc = L.take((hc.features & kFeatDrive) ? 12 * Np : 0);
     m.ell_off = L.take(4 * (uint64_t)hc.ell_off.size());
     m.ell = L.take(4 * (uint64_t)hc.ell.size());
+    m.Xd = L.take(24 * N);
+    m.lconn = L.take(8 * T);
+    m.tn_off = L.take(4 * (uint64_t)hc.tn_off.size());
+    m.tnode = L.take(4 * (uint64_t)hc.tnode.size());
+    m.lofs = L.take(2 * (uint64_t)hc.lofs.size());
+    m.lent = L.take(2 * (uint64_t)hc.lent.size());
+    m.partial = L.take(16 * (uint64_t)hc.P);
+    m.pell_off = L.take(4 * (uint64_t)hc.pell_off.size());
+    m.pell = L.take(4 * (uint64_t)hc.pell.size());
     m.acc_e = L.take(8ull * kAccElemSlots * hc.elem_waves);
     m.acc_n = L.take(8ull * kAccNodeSlots * hc.node_waves);
     m.smp_e = L.take(8ull * kSmpElemSlots * hc.elem_waves);
@@ -309,6 +319,84 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
                 hc.ell[hc.ell_off[w] + (q - off[n]) * kWave + l] = ent[q];
         }
 
+    // 256-tet tiles of the tiled element pass: each tile's unique nodes in ascending
+    // id (one partial slot each), 16-bit local connectivity, and a tile-local CSR of
+    // (tet_local*4 + corner) in (tet, corner) order
+    {
+        const uint32_t nt = (T + kTile - 1) / kTile;
+        hc.elem_tiles = nt;
+        hc.lconn.assign(2ull * T, 0u);
+        hc.tn_off.assign(nt + 1, 0u);
+        hc.lent.assign(4ull * kTile * nt, 0u);
+        hc.tnode.clear();
+        hc.lofs.clear();
+        std::vector<uint32_t> stamp(N, 0xFFFFFFFFu), loc(N, 0u);
+        std::vector<uint32_t> nodes;
+        std::vector<uint16_t> cnt;
+        for (uint32_t b = 0; b < nt; ++b) {
+            const uint32_t t0 = b * kTile, t1 = std::min<uint32_t>(T, t0 + kTile);
+            nodes.clear();
+            for (uint32_t t = t0; t < t1; ++t)
+                for (int c = 0; c < 4; ++c) {
+                    const uint32_t g = d.index[4ull * t + c];
+                    if (stamp[g] != b) {
+                        stamp[g] = b;
+                        nodes.push_back(g);
+                    }
+                }
+            std::sort(nodes.begin(), nodes.end());
+            const uint32_t L = static_cast<uint32_t>(nodes.size());
+            for (uint32_t l = 0; l < L; ++l) loc[nodes[l]] = l;
+            hc.tn_off[b + 1] = hc.tn_off[b] + L;
+            hc.lmax = std::max(hc.lmax, L);
+            hc.tnode.insert(hc.tnode.end(), nodes.begin(), nodes.end());
+            for (uint32_t t = t0; t < t1; ++t) {
+                const uint32_t* v = &d.index[4ull * t];
+                hc.lconn[2ull * t] = loc[v[0]] | (loc[v[1]] << 16);
+                hc.lconn[2ull * t + 1] = loc[v[2]] | (loc[v[3]] << 16);
+            }
+            cnt.assign(L + 1, 0);
+            for (uint32_t t = t0; t < t1; ++t)
+                for (int c = 0; c < 4; ++c) ++cnt[loc[d.index[4ull * t + c]] + 1];
+            for (uint32_t l = 0; l < L; ++l) cnt[l + 1] = static_cast<uint16_t>(cnt[l + 1] + cnt[l]);
+            hc.lofs.insert(hc.lofs.end(), cnt.begin(), cnt.end());
+            std::vector<uint16_t> cursor(cnt.begin(), cnt.end() - 1);
+            uint16_t* le = &hc.lent[static_cast<size_t>(b) * 4 * kTile];
+            for (uint32_t t = t0; t < t1; ++t)
+                for (uint32_t c = 0; c < 4; ++c)
+                    le[cursor[loc[d.index[4ull * t + c]]]++] = static_cast<uint16_t>((t - t0) * 4 + c);
+        }
+        hc.P = hc.tn_off[nt];
+        // node -> its partial slots in tile order, wave-transposed like the CSR slab
+        std::vector<uint32_t> poff(N + 1, 0u);
+        for (uint32_t p = 0; p < hc.P; ++p) ++poff[hc.tnode[p] + 1];
+        for (uint32_t n = 0; n < N; ++n) poff[n + 1] += poff[n];
+        std::vector<uint32_t> plist(hc.P), pcur(poff.begin(), poff.end() - 1);
+        for (uint32_t p = 0; p < hc.P; ++p) plist[pcur[hc.tnode[p]]++] = p;
+        hc.pell_off.assign(hc.node_waves + 1, 0u);
+        uint64_t ptotal = 0;
+        for (uint32_t w = 0; w < hc.node_waves; ++w) {
+            uint32_t width = 0;
+            for (uint32_t l = 0; l < kWave; ++l) {
+                const uint32_t n = w * kWave + l;
+                if (n < N && (hc.nflags[n] & SHIPFEM_PIN_ALL) != SHIPFEM_PIN_ALL)
+                    width = std::max(width, poff[n + 1] - poff[n]);
+            }
+            hc.pell_off[w] = static_cast<uint32_t>(ptotal);
+            ptotal += uint64_t(width) * kWave;
+            if (ptotal > 0xFFFFFFF0ull) return "partial gather slab exceeds 32-bit indexing";
+        }
+        hc.pell_off[hc.node_waves] = static_cast<uint32_t>(ptotal);
+        hc.pell.assign(ptotal, kEllSentinel);
+        for (uint32_t w = 0; w < hc.node_waves; ++w)
+            for (uint32_t l = 0; l < kWave; ++l) {
+                const uint32_t n = w * kWave + l;
+                if (n >= N || (hc.nflags[n] & SHIPFEM_PIN_ALL) == SHIPFEM_PIN_ALL) continue;
+                for (uint32_t q = poff[n]; q < poff[n + 1]; ++q)
+                    hc.pell[hc.pell_off[w] + (q - poff[n]) * kWave + l] = plist[q];
+            }
+    }
+
     // initial state, split into float + Kahan compensation
     hc.u4.assign(4ull * N, 0.0f);
     hc.cu.assign(3ull * hc.Npad, 0.0f);
@@ -358,6 +446,18 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.presc = (hc.features & kFeatDrive) ? reinterpret_cast<const float*>(base + m.presc) : nullptr;
     r.ell_off = reinterpret_cast<const uint32_t*>(base + m.ell_off);
     r.ell = reinterpret_cast<const uint32_t*>(base + m.ell);
+    r.Xd = reinterpret_cast<const double*>(base + m.Xd);
+    r.lconn = reinterpret_cast<const uint2*>(base + m.lconn);
+    r.tn_off = reinterpret_cast<const uint32_t*>(base + m.tn_off);
+    r.tnode = reinterpret_cast<const uint32_t*>(base + m.tnode);
+    r.lofs = reinterpret_cast<const uint16_t*>(base + m.lofs);
+    r.lent = reinterpret_cast<const uint16_t*>(base + m.lent);
+    r.partial = reinterpret_cast<float4*>(base + m.partial);
+    r.pell_off = reinterpret_cast<const uint32_t*>(base + m.pell_off);
+    r.pell = reinterpret_cast<const uint32_t*>(base + m.pell);
+    r.elem_tiles = hc.elem_tiles;
+    r.P = hc.P;
+    r.lmax = hc.lmax;
     r.load_ramp = hc.load_ramp;
     r.drive_ramp = hc.drive_ramp;
     for (int a = 0; a < 3; ++a) r.ind_vel[a] = hc.indenter.velocity[a];
@@ -402,6 +502,14 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.presc, hc.presc.data(), 4ull * hc.presc.size());
     up(m.ell_off, hc.ell_off.data(), 4ull * hc.ell_off.size());
     up(m.ell, hc.ell.data(), 4ull * hc.ell.size());
+    up(m.Xd, hc.X.data(), 8ull * hc.X.size());
+    up(m.lconn, hc.lconn.data(), 4ull * hc.lconn.size());
+    up(m.tn_off, hc.tn_off.data(), 4ull * hc.tn_off.size());
+    up(m.tnode, hc.tnode.data(), 4ull * hc.tnode.size());
+    up(m.lofs, hc.lofs.data(), 2ull * hc.lofs.size());
+    up(m.lent, hc.lent.data(), 2ull * hc.lent.size());
+    up(m.pell_off, hc.pell_off.data(), 4ull * hc.pell_off.size());
+    up(m.pell, hc.pell.data(), 4ull * hc.pell.size());
     return e;
 }
 

@@ -418,10 +418,11 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
         delete s;
         return SHIPFEM_INVALID_ARG;
     }
-    const shipfem_launch_config& L = s->opts.launch;
-    s->plan.shape.elem_block = L.elem_block ? L.elem_block : 256;
-    s->plan.shape.node_block = L.node_block ? L.node_block : 256;
-    s->plan.shape.xcd_remap = (L.elem_block || L.node_block) ? L.xcd_remap : 1;
+    if (!shapeFrom(&s->opts.launch, s->plan.shape)) {
+        setError(err, err_len, "launch config: blocks must be 64/128/256/512, variant TILED or SLOTS");
+        delete s;
+        return SHIPFEM_INVALID_ARG;
+    }
     if (s->opts.memory_budget == 0) {
         size_t freeb = 0, total = 0;
         (void)hipMemGetInfo(&freeb, &total);

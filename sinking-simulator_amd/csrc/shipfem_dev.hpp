@@ -70,6 +70,17 @@ struct CaseDev {
     const float* presc;  // [3][Npad] prescribed displacement (kFeatDrive)
     const uint32_t* ell_off;  // [node_waves + 1] first entry of each wave's slab
     const uint32_t* ell;      // wave slabs, entry e of lane l at ell_off[w] + 64 e + l
+    // ---- tiled element pass: 256-tet tiles staged through LDS
+    const double* Xd;         // [N][3] rest positions (FP64; Dm^-1 is re-derived per step)
+    const uint2* lconn;       // [T] four 16-bit tile-local node indices per tet
+    const uint32_t* tn_off;   // [elem_tiles + 1] first (tile, local node) slot of each tile
+    const uint32_t* tnode;    // [P] global node id of each (tile, local node) slot
+    const uint16_t* lofs;     // [P + elem_tiles] per tile: L+1 offsets into its entries
+    const uint16_t* lent;     // [elem_tiles][1024] tile-local CSR: tet_local*4 + corner
+    float4* partial;          // [P] force sum of each (tile, local node)
+    const uint32_t* pell_off; // [node_waves + 1] node -> partial slots, wave slabs
+    const uint32_t* pell;
+    uint32_t elem_tiles, P, lmax, pad3;
     // ---- scalars
     double dt;
     double load_ramp, drive_ramp;
@@ -102,17 +113,30 @@ struct Tile {
     uint32_t first;
 };
 
+// Element-pass variants. kSlots: one lane per tet writes its four corner forces to a
+// 48-byte slot (the reference's tetForce layout) and the node pass gathers slots
+// through the CSR node map. kTiled: a 256-tet tile stages its nodes' displacement and
+// rest coordinates in LDS, re-derives Dm^-1 in FP64, and sums corner forces per local
+// node in LDS; the node pass gathers one partial per (tile, node).
+enum Variant : int { kSlots = 0, kTiled = 1 };
+constexpr int kTile = 256;  // tets per tile of the tiled variant (fixed: it sets the sum order)
+
 struct LaunchShape {
     int elem_block = 256;
     int node_block = 256;
     int xcd_remap = 1;
+    int variant = kTiled;
 };
 
 // Kernel launchers (kernels.hip). All launch asynchronously on `stream`.
+// For the tiled variant the tiles are 256-tet tiles, `block` is ignored and `lmax` sizes
+// the LDS node stage.
 void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
                  int sample, int xcd, uint32_t feat_union, hipStream_t stream);
+void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
+                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream);
 void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
-                 int sample, int xcd, uint32_t feat_union, hipStream_t stream);
+                 int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream);
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream);
 // FP64 largest eigenvalue of B^T C B per element (the lumped-mass element frequency
 // is omega^2 = 4 lambda / rho), reduced to a per-wave max, then to one value per case.

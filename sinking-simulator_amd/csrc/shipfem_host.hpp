@@ -42,6 +42,15 @@ struct HostCase {
     std::vector<float> presc;       // [3][Npad]
     std::vector<uint32_t> ell_off;  // [node_waves + 1]
     std::vector<uint32_t> ell;
+    // tiled element pass (256-tet tiles)
+    uint32_t elem_tiles = 0, P = 0, lmax = 0;
+    std::vector<uint32_t> lconn;    // [2T] packed 16-bit local ids
+    std::vector<uint32_t> tn_off;   // [elem_tiles + 1]
+    std::vector<uint32_t> tnode;    // [P]
+    std::vector<uint16_t> lofs;     // [P + elem_tiles]
+    std::vector<uint16_t> lent;     // [elem_tiles * 1024]
+    std::vector<uint32_t> pell_off; // [node_waves + 1]
+    std::vector<uint32_t> pell;
     std::vector<float> u4;          // [4N]
     std::vector<float> cu, v, cv;   // [3][Npad]
     // material and loading
@@ -88,6 +97,7 @@ struct Plan {
     double* d_omega_out = nullptr;
     LaunchShape shape;
     uint32_t feat_union = 0;
+    uint32_t lmax = 0;  // largest tile node count over the plan's cases (tiled LDS stage)
     int parity = 0;
     int chunk = 0;
     hipGraphExec_t graph = nullptr;
@@ -100,6 +110,10 @@ struct Plan {
     // Uploads the host record copies only (after a scalar change).
     hipError_t uploadRecords(hipStream_t stream);
     void dropGraph();
+    // The element and node launches of one step at `parity` (no reduce).
+    void launchStep(int parity, int sample, hipStream_t stream);
+    void launchElem(int parity, int sample, hipStream_t stream);
+    void launchNode(int parity, int sample, hipStream_t stream);
     // One step as plain launches.
     void stepPlain(bool sample, hipStream_t stream);
     // Captures `chunk` (even) steps starting at parity 0, sample on the first.
@@ -114,6 +128,7 @@ hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_
 hipError_t readEnergy(const DeviceCase& dc, shipfem_energy_record* out, int32_t max,
                       int32_t* count, hipStream_t stream);
 
+bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out);
 void setError(char* err, size_t len, const std::string& msg);
 shipfem_status statusOf(hipError_t e);
 bool deviceAvailable(std::string* why);

@@ -68,9 +68,12 @@ class StateView(C.Structure):
                 ("status", C.c_uint32)]
 
 
+VARIANT_TILED, VARIANT_SLOTS = 0, 1
+
+
 class LaunchConfig(C.Structure):
     _fields_ = [("elem_block", C.c_int32), ("node_block", C.c_int32), ("xcd_remap", C.c_int32),
-                ("chunk", C.c_int32)]
+                ("chunk", C.c_int32), ("variant", C.c_int32)]
 
 
 class ServerOpts(C.Structure):
@@ -122,6 +125,7 @@ SIGNATURES = {
     "shipfem_solver_readback": (S, [VP, C.POINTER(StateView)]),
     "shipfem_solver_energy": (S, [VP, C.POINTER(EnergyRecord), C.c_int32, C.POINTER(C.c_int32)]),
     "shipfem_solver_tet_forces": (S, [VP, F]),
+    "shipfem_solver_set_launch": (S, [VP, C.POINTER(LaunchConfig)]),
     "shipfem_solver_profile": (S, [VP, C.c_int32, D, D]),
     "shipfem_solver_footprint": (S, [VP, C.POINTER(C.c_uint64)]),
     "shipfem_solver_destroy": (None, [VP]),
@@ -396,6 +400,11 @@ class Solver:
     def dt(self):
         return self.timestep()["dt"]
 
+    def set_launch(self, variant=VARIANT_TILED, node_block=256, elem_block=256, xcd_remap=1):
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant)
+        _check(lib().shipfem_solver_set_launch(self.h, C.byref(cfg)), "set_launch")
+        return self
+
     def set_params(self, dt, gravity=0.0, damping=1.0, E=0.0, nu=0.0):
         p = StepParams(dt, gravity, damping, E, nu)
         _check(lib().shipfem_solver_set_params(self.h, C.byref(p)))
@@ -440,10 +449,11 @@ class Solver:
 class Batch:
     """Many cases advancing in one launch per phase."""
 
-    def __init__(self, cases, elem_block=256, node_block=256, xcd_remap=1, chunk=0):
+    def __init__(self, cases, elem_block=256, node_block=256, xcd_remap=1, chunk=0,
+                 variant=VARIANT_TILED):
         self.cases = list(cases)
         self._descs = (CaseDesc * len(self.cases))(*[c.desc() for c in self.cases])
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk)
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk, variant)
         err = C.create_string_buffer(512)
         h = C.c_void_p()
         _check(lib().shipfem_batch_create(self._descs, len(self.cases), C.byref(cfg), C.byref(h),
@@ -471,10 +481,11 @@ class Batch:
         ms = C.c_double()
         _check(lib().shipfem_batch_tune(self.h, steps_per_trial, C.byref(cfg), C.byref(ms)))
         return dict(elem_block=cfg.elem_block, node_block=cfg.node_block,
-                    xcd_remap=cfg.xcd_remap, chunk=cfg.chunk, ms_per_step=ms.value)
+                    xcd_remap=cfg.xcd_remap, chunk=cfg.chunk, variant=cfg.variant,
+                    ms_per_step=ms.value)
 
-    def set_config(self, elem_block, node_block, xcd_remap):
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0)
+    def set_config(self, elem_block, node_block, xcd_remap, variant=VARIANT_TILED):
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant)
         _check(lib().shipfem_batch_set_config(self.h, C.byref(cfg)))
 
     def reset(self):
@@ -499,9 +510,9 @@ class Server:
     """Resident serving loop: submit cases at any time, poll results."""
 
     def __init__(self, memory_budget=0, max_active=0, chunk=0, elem_block=0, node_block=0,
-                 xcd_remap=1):
+                 xcd_remap=1, variant=VARIANT_TILED):
         opts = ServerOpts(memory_budget, max_active, chunk,
-                          LaunchConfig(elem_block, node_block, xcd_remap, 0))
+                          LaunchConfig(elem_block, node_block, xcd_remap, 0, variant))
         err = C.create_string_buffer(512)
         h = C.c_void_p()
         _check(lib().shipfem_server_create(C.byref(opts), C.byref(h), err, 512), err.value.decode())

@@ -60,12 +60,13 @@ def test_case_is_bit_identical_alone_and_in_the_mix():
     assert np.array_equal(st.u, state(both, 1)[0]) and s.energy() == state(both, 1)[3]
 
 
+@pytest.mark.parametrize("variant", [sf.VARIANT_TILED, sf.VARIANT_SLOTS])
 @pytest.mark.parametrize("shape", [(64, 512, 0), (512, 64, 1), (128, 128, 1)])
-def test_launch_shape_does_not_change_any_bit(shape):
+def test_launch_shape_does_not_change_any_bit(shape, variant):
     cases = mix()
-    ref = sf.Batch(cases, 256, 256, 1)
+    ref = sf.Batch(cases, 256, 256, 1, variant=variant)
     ref.run(400)
-    other = sf.Batch(cases, *shape)
+    other = sf.Batch(cases, *shape, variant=variant)
     other.run(400)
     for i in range(len(cases)):
         assert same(state(ref, i), state(other, i)), (shape, i)
@@ -112,7 +113,7 @@ def serve(cases, run_to, budget=2 << 30, chunk=32, keep=(), gaps=None):
     srv = sf.Server(memory_budget=budget, chunk=chunk)
     ids = []
     for k, c in enumerate(cases):
-        if gaps:
+        if gaps is not None:
             time.sleep(gaps[k])
         ids.append(srv.submit(c, run_to, keep=k in keep))
     assert srv.drain(120.0)
