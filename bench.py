@@ -89,6 +89,114 @@ def cpu_baseline(case, budget_s=6.0):
                         f"{out['one'][0]:.3e} tet-steps/s over {out['one'][1]} steps"))
 
 
+def bench_batch(args, sf, info):
+    """64 plate cases (tets log-uniform in [1e4, 3e5], pressure 50-400 kPa, yield
+    235-460 MPa, J2, FP32 + Kahan): the plain path (one case at a time, two plain
+    launches per step) against one batched launch per phase (graph-replayed, tuned)."""
+    from shipfem import configs
+
+    cases = configs.batch_cases(64)
+    tets = sum(c.T for c in cases)
+    # plain: the reference's pattern, one case per solver, stepped case after case
+    plain_steps = min(args.steps, 200)
+    t0 = time.perf_counter()
+    for c in cases:
+        s = sf.Solver(c)
+        s.run(plain_steps, mode=1)
+        s.close()
+    plain_wall_total = time.perf_counter() - t0
+    # device-side plain rate excluding set-up: time the stepping only
+    plain_time = 0.0
+    for c in cases:
+        s = sf.Solver(c)
+        t1 = time.perf_counter()
+        s.run(plain_steps, mode=1)
+        plain_time += time.perf_counter() - t1
+        s.close()
+    plain_rate = tets * plain_steps / plain_time
+    b = sf.Batch(cases, chunk=100)
+    best = b.tune(steps_per_trial=200)
+    b.reset()
+    b.run(args.warmup)
+    b.reset()
+    t0 = time.perf_counter()
+    dev_ms = b.run(args.steps)
+    wall = time.perf_counter() - t0
+    rate = tets * args.steps / wall
+    bad = [i for i in range(len(cases)) if b.readback(i).status]
+    if bad:
+        raise SystemExit(f"non-finite cases {bad}")
+    step_bytes = sum((168 + 56) * c.T + 140 * c.N for c in cases)
+    emit(dict(metric="tet-element-steps/s", value=rate, unit="tet-steps/s", n_gpus=1,
+              steps=args.steps, warmup=args.warmup, ms_per_step=wall * 1e3 / args.steps,
+              higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32",
+              data="synthetic",
+              config=dict(workload=f"batched: 64 plates, {tets} tets total, J2, one launch per "
+                                   f"phase over all cases, graph chunk 100, tuned {best}",
+                          device=info[0], plain_rate=plain_rate, plain_steps=plain_steps,
+                          plain_wall_incl_setup_s=plain_wall_total,
+                          batched_over_plain=rate / plain_rate,
+                          device_ms_per_step=dev_ms / args.steps,
+                          step_hbm_gbs_model=step_bytes * args.steps / wall / 1e9,
+                          step_roofline_frac_model=step_bytes * args.steps / wall / 8e12)))
+
+
+def bench_serve(args, sf, info):
+    """Poisson arrivals of stiffened-panel indentation cases (tets log-uniform in
+    [5e3, 5e5], indenter 2-10 m/s, run to 2 ms simulated), continuous batching."""
+    import math
+
+    import numpy as np
+
+    from shipfem import configs
+
+    rng = np.random.default_rng(11)
+    pool = []
+    for _ in range(args.pool):
+        t = math.exp(rng.uniform(math.log(5e3), math.log(5e5)))
+        pool.append(configs.served_case(t, float(rng.uniform(2.0, 10.0))))
+    rates = [float(r) for r in args.rates.split(",")]
+    rows = []
+    total_tet_steps = 0.0
+    total_wall = 0.0
+    for lam in rates:
+        srv = sf.Server(chunk=64)
+        n = max(4, int(lam * args.duration))
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+        picks = rng.integers(0, len(pool), n)
+        t0 = time.perf_counter()
+        for k in range(n):
+            wait = arrivals[k] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(pool[picks[k]], 2e-3)
+        if not srv.drain(600.0):
+            raise SystemExit("server did not drain")
+        wall = time.perf_counter() - t0
+        res = srv.poll(n + 16)
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r["done_s"] - r["submit_s"] for r in res])
+        qwait = np.array([r["admit_s"] - r["submit_s"] for r in res])
+        ok = sum(1 for r in res if r["status"] == sf.OK)
+        rows.append(dict(rate=lam, requests=n, completed=ok, p50_s=float(np.percentile(lat, 50)),
+                         p99_s=float(np.percentile(lat, 99)),
+                         p99_queue_s=float(np.percentile(qwait, 99)),
+                         tet_steps_per_s=st["element_steps"] / wall,
+                         device_busy=st["device_ms"] * 1e-3 / wall, rounds=st["rounds"]))
+        total_tet_steps += st["element_steps"]
+        total_wall += wall
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    top = max(rows, key=lambda r: r["tet_steps_per_s"])
+    emit(dict(metric="tet-element-steps/s", value=top["tet_steps_per_s"], unit="tet-steps/s",
+              n_gpus=1, steps=None, warmup=0, ms_per_step=None, higher_is_better=True,
+              scaling="weak", vs_baseline=None, dtype="fp32", data="synthetic",
+              config=dict(workload="served: Poisson arrivals of stiffened-panel indentation cases, "
+                                   "tets log-uniform [5e3,5e5], 2-10 m/s, run to 2 ms, "
+                                   f"pool of {len(pool)} meshes",
+                          device=info[0], sweep=rows)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +206,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=40)
     ap.add_argument("--variant", default="tiled", choices=["tiled", "slots"])
+    ap.add_argument("--rates", default="10,50,100,150,200")
+    ap.add_argument("--duration", type=float, default=3.0)
+    ap.add_argument("--pool", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -111,6 +222,10 @@ def main():
     if info is None:
         print("no usable gfx950 device", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batch":
+        return bench_batch(args, sf, info)
+    if args.workload == "serve":
+        return bench_serve(args, sf, info)
     case, desc = workload(args.workload)
     T, N = case.T, case.N
     variant = sf.VARIANT_TILED if args.variant == "tiled" else sf.VARIANT_SLOTS

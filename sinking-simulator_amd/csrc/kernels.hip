@@ -61,33 +61,33 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, cons
             G[i * 3 + j] = D[i * 3 + 0] * B[0 * 3 + j] + D[i * 3 + 1] * B[1 * 3 + j] +
                            D[i * 3 + 2] * B[2 * 3 + j];
 
-    // R = I + W by R <- (R + R^-T)/2 with inv(I+W) - I = (W^2 - (1+t)W - det(W) I)/det(I+W)
+    // R = I + W by R <- (R + R^-T)/2. With adj(W) the adjugate, t = tr W, i2 = tr adj(W):
+    // det(I+W) = 1 + t + i2 + det W and inv(I+W) - I = (adj(W) - W - (i2 + det W) I)/det(I+W),
+    // so no entry is formed as 1 + (small) and cancelled again.
     float W[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) W[k] = G[k];
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
-        float W2[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                W2[i * 3 + j] = W[i * 3 + 0] * W[0 * 3 + j] + W[i * 3 + 1] * W[1 * 3 + j] +
-                                W[i * 3 + 2] * W[2 * 3 + j];
+        const float a[9] = {W[4] * W[8] - W[5] * W[7], W[2] * W[7] - W[1] * W[8], W[1] * W[5] - W[2] * W[4],
+                            W[5] * W[6] - W[3] * W[8], W[0] * W[8] - W[2] * W[6], W[2] * W[3] - W[0] * W[5],
+                            W[3] * W[7] - W[4] * W[6], W[1] * W[6] - W[0] * W[7], W[0] * W[4] - W[1] * W[3]};
         const float t = W[0] + W[4] + W[8];
-        const float i2 = 0.5f * (t * t - (W2[0] + W2[4] + W2[8]));
-        const float dW = W[0] * (W[4] * W[8] - W[5] * W[7]) - W[1] * (W[3] * W[8] - W[5] * W[6]) +
-                         W[2] * (W[3] * W[7] - W[4] * W[6]);
-        const float den = 1.0f + (t + i2 + dW);
+        const float i2 = a[0] + a[4] + a[8];
+        const float dW = W[0] * a[0] + W[1] * a[3] + W[2] * a[6];
+        const float den = 1.0f + (t + (i2 + dW));
         const bool ok = fabsf(den) > 1e-30f;  // the reference's singular guard
-        const float id = ok ? 1.0f / den : 0.0f;
+        // hardware reciprocal (<= 1 ulp): Newton's iteration corrects all but the last
+        // iteration's rounding, which this changes by at most an ulp
+        const float id = ok ? __builtin_amdgcn_rcpf(den) : 0.0f;
+        const float sh = i2 + dW;
         float En[9];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const float diag = i == j ? 1.0f : 0.0f;
-                En[i * 3 + j] = ok ? (W2[i * 3 + j] - (1.0f + t) * W[i * 3 + j] - dW * diag) * id : -diag;
+                En[i * 3 + j] = ok ? (a[i * 3 + j] - W[i * 3 + j] - sh * diag) * id : -diag;
             }
         float Wn[9];
 #pragma unroll
@@ -519,53 +519,57 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
 // ---------------------------------------------------------------- FP64 reductions
 constexpr int kReduceBlock = 256;
 
-__device__ double block_sum(double v, double* sh) {
-    // fixed-shape tree: deterministic for a fixed block size
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kReduceBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
-}
+// One block per case, 1024 threads. Every thread first folds its strided share of all
+// slot arrays in one pass (independent loads in flight together), then each quantity
+// goes through the same fixed tree. The order depends only on the case's wave count.
+constexpr int kRecordBlock = 1024;
+constexpr int kQ = kAccElemSlots + kSmpElemSlots + kAccNodeSlots + kSmpNodeSlots;
 
-__device__ double strided_sum(const double* p, uint32_t count) {
-    double s = 0.0;
-    for (uint32_t k = threadIdx.x; k < count; k += kReduceBlock) s += p[k];
-    return s;
-}
-
-__global__ __launch_bounds__(kReduceBlock) void k_reduce(const CaseDev* __restrict__ cases,
+__global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restrict__ cases,
                                                           int parity) {
-    __shared__ double sh[kReduceBlock];
+    __shared__ double sh[kRecordBlock];
     const CaseDev& c = cases[blockIdx.x];
     const int64_t n0 = c.step[parity];
     if (c.step[parity ^ 1] != n0 + 1) return;  // the case did not step: no record
     const uint32_t ew = c.elem_waves, nw = c.node_waves;
-    const double diss = block_sum(strided_sum(c.acc_e + kAccDiss * ew, ew), sh);
-    const double wext = block_sum(strided_sum(c.acc_n + kAccWext * nw, nw), sh);
-    const double wcon = block_sum(strided_sum(c.acc_n + kAccWcon * nw, nw), sh);
-    const double wbnd = block_sum(strided_sum(c.acc_n + kAccWbnd * nw, nw), sh);
-    const double damp = block_sum(strided_sum(c.acc_n + kAccDamp * nw, nw), sh);
-    const double strain = block_sum(strided_sum(c.smp_e + kSmpStrain * ew, ew), sh);
-    double sn[kSmpNodeSlots];
-    for (int k = 0; k < kSmpNodeSlots; ++k) sn[k] = block_sum(strided_sum(c.smp_n + k * nw, nw), sh);
+    double q[kQ];
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) q[k] = 0.0;
+    for (uint32_t w = threadIdx.x; w < ew; w += kRecordBlock) {
+        q[0] += c.acc_e[kAccDiss * ew + w];
+        q[1] += c.smp_e[kSmpStrain * ew + w];
+    }
+    for (uint32_t w = threadIdx.x; w < nw; w += kRecordBlock) {
+#pragma unroll
+        for (int k = 0; k < kAccNodeSlots; ++k) q[2 + k] += c.acc_n[k * nw + w];
+#pragma unroll
+        for (int k = 0; k < kSmpNodeSlots; ++k) q[2 + kAccNodeSlots + k] += c.smp_n[k * nw + w];
+    }
+    double tot[kQ];
+    for (int k = 0; k < kQ; ++k) {
+        sh[threadIdx.x] = q[k];
+        __syncthreads();
+        for (int s = kRecordBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        tot[k] = sh[0];
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         const int32_t h = *c.hist_count;
         if (h < c.hist_cap) {
+            const double* sn = tot + 2 + kAccNodeSlots;
             EnergyRecordDev& r = c.hist[h];
             r.step = n0;
             r.time = (double)n0 * c.dt;
-            r.strain = strain;
+            r.strain = tot[1];
             r.kinetic = sn[kSmpKin];
-            r.dissipation = diss;
-            r.w_ext = wext - sn[kSmpHalfExt];
-            r.w_contact = wcon - sn[kSmpHalfCon];
-            r.w_boundary = wbnd;
-            r.damping_loss = damp;
+            r.dissipation = tot[0];
+            r.w_ext = tot[2 + kAccWext] - sn[kSmpHalfExt];
+            r.w_contact = tot[2 + kAccWcon] - sn[kSmpHalfCon];
+            r.w_boundary = tot[2 + kAccWbnd];
+            r.damping_loss = tot[2 + kAccDamp];
             r.contact_force[0] = sn[kSmpFx];
             r.contact_force[1] = sn[kSmpFy];
             r.contact_force[2] = sn[kSmpFz];
@@ -772,7 +776,7 @@ void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint
 }
 
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream) {
-    hipLaunchKernelGGL(k_reduce, dim3(ncases), dim3(kReduceBlock), 0, stream, cases, parity);
+    hipLaunchKernelGGL(k_reduce, dim3(ncases), dim3(kRecordBlock), 0, stream, cases, parity);
 }
 
 void launch_omega(const CaseDev* cases, const Tile* tiles, int ntiles, hipStream_t stream) {

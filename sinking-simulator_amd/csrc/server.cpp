@@ -442,7 +442,7 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
         return st;
     }
     s->st.bytes_budget = s->opts.memory_budget;
-    const unsigned nworkers = std::max(1u, std::min(4u, std::thread::hardware_concurrency() / 2));
+    const unsigned nworkers = std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
     for (unsigned w = 0; w < nworkers; ++w) s->workers.emplace_back([s] { s->workerMain(); });
     s->loop = std::thread([s] { s->loopMain(); });
     *out = s;
