@@ -209,6 +209,9 @@ def main():
     ap.add_argument("--rates", default="10,50,100,150,200")
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--pool", type=int, default=24)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after timing, create a one-case batch: its creation is one device-to-"
+                         "device copy of a known size, the reference for rocprofv3 byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -259,6 +262,12 @@ def main():
                     elem_ms=round(elem_ms, 5), node_ms=round(node_ms, 5))
     step_bytes = elem_bytes + node_bytes
     step_gbs = step_bytes * args.steps / wall / 1e9
+    copy_probe = None
+    if args.copy_probe:
+        foot = solver.footprint()
+        solver.close()
+        sf.Batch([case]).close()  # one device-to-device copy of the padded arena
+        copy_probe = (foot + 4095) // 4096 * 4096
 
     cpu = None
     if not args.no_cpu:
@@ -279,7 +288,7 @@ def main():
                           reference_edge_dt=ts["edge_dt"], device=info[0],
                           device_ms_per_step=dev_ms / args.steps, step_hbm_gbs=round(step_gbs, 1),
                           step_roofline_frac=round(step_gbs / HBM_PEAK_GBS, 4),
-                          algorithmic_bytes_per_step=step_bytes),
+                          algorithmic_bytes_per_step=step_bytes, copy_probe_bytes=copy_probe),
               roofline=roofline, cpu_baseline=cpu))
 
 

@@ -129,7 +129,7 @@ uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t
     const uint64_t nt = (T + kTile - 1) / kTile, P = 2ull * T;
     uint64_t b = 16ull * T + 40ull * Tp + 48ull * T + (plastic ? 28ull * Tp : 0) + 16ull * N +
                  36ull * Np + 12ull * Np + 40ull * Np + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
-                 24ull * N + 8ull * T + 4ull * (nt + 1) + 4ull * P + 2ull * (P + nt) +
+                 8ull * T + 4ull * (nt + 1) + 4ull * P + 2ull * (P + nt) +
                  2ull * 4 * kTile * nt + 16ull * P + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
                  8ull * (3 * ew + 10 * nw) + 24 + sizeof(EnergyRecordDev) * (uint64_t)std::max(cap, 1);
     return b + 256ull * 36;

@@ -243,7 +243,9 @@ __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ case
 
 // ---------------------------------------------------------------- element pass, tiled
 // One block per 256-tet tile. LDS: corner forces [12][256] floats, then the tile's
-// local nodes: displacement float4[lmax], rest position double[3 lmax].
+// local nodes' displacement float4[lmax]. Dm^-1 and V stream as planes (the kernel is
+// ALU-bound; re-deriving them in FP64 from rest coordinates cost more issue slots than
+// the 40 bytes per tet it saved).
 template <bool PLASTIC>
 __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict__ cases,
                                                       const Tile* __restrict__ tiles, int ntiles,
@@ -252,7 +254,6 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     extern __shared__ __align__(16) unsigned char smem[];
     float* sh_f = reinterpret_cast<float*>(smem);
     float4* sh_u = reinterpret_cast<float4*>(smem + 12 * kTile * sizeof(float));
-    double* sh_x = reinterpret_cast<double*>(smem + 12 * kTile * sizeof(float) + 16 * (size_t)lmax);
 
     const int b = remap_block(blockIdx.x, gridDim.x, xcd);
     const Tile tile = tiles[b];
@@ -262,13 +263,7 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     const uint32_t tb = tile.first / kTile;
     const uint32_t p0 = c.tn_off[tb];
     const uint32_t L = c.tn_off[tb + 1] - p0;
-    for (uint32_t l = threadIdx.x; l < L; l += kTile) {
-        const uint32_t g = c.tnode[p0 + l];
-        sh_u[l] = c.u4[g];
-        sh_x[3 * l + 0] = c.Xd[3 * (size_t)g + 0];
-        sh_x[3 * l + 1] = c.Xd[3 * (size_t)g + 1];
-        sh_x[3 * l + 2] = c.Xd[3 * (size_t)g + 2];
-    }
+    for (uint32_t l = threadIdx.x; l < L; l += kTile) sh_u[l] = c.u4[c.tnode[p0 + l]];
     __syncthreads();
 
     const uint32_t e = tile.first + threadIdx.x;
@@ -276,25 +271,12 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     if (e < c.T) {
         const uint2 lc = c.lconn[e];
         const uint32_t l0 = lc.x & 0xFFFFu, l1 = lc.x >> 16, l2 = lc.y & 0xFFFFu, l3 = lc.y >> 16;
-        // Dm^-1 and V from the rest coordinates, in FP64, rounded once
-        const double* x0 = sh_x + 3 * l0;
-        const double* x1 = sh_x + 3 * l1;
-        const double* x2 = sh_x + 3 * l2;
-        const double* x3 = sh_x + 3 * l3;
-        const double a[9] = {x1[0] - x0[0], x2[0] - x0[0], x3[0] - x0[0],
-                             x1[1] - x0[1], x2[1] - x0[1], x3[1] - x0[1],
-                             x1[2] - x0[2], x2[2] - x0[2], x3[2] - x0[2]};
-        const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
-                           a[2] * (a[3] * a[7] - a[4] * a[6]);
-        const double id = 1.0 / det;
-        const float B[9] = {(float)((a[4] * a[8] - a[5] * a[7]) * id), (float)((a[2] * a[7] - a[1] * a[8]) * id),
-                            (float)((a[1] * a[5] - a[2] * a[4]) * id), (float)((a[5] * a[6] - a[3] * a[8]) * id),
-                            (float)((a[0] * a[8] - a[2] * a[6]) * id), (float)((a[2] * a[3] - a[0] * a[5]) * id),
-                            (float)((a[3] * a[7] - a[4] * a[6]) * id), (float)((a[1] * a[6] - a[0] * a[7]) * id),
-                            (float)((a[0] * a[4] - a[1] * a[3]) * id)};
-        const float V = (float)(fabs(det) * (1.0 / 6.0));
+        float B[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * c.Tpad + e];
         float f[12];
-        element_force<PLASTIC>(c, e, sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3], B, V, sample, f, diss, se);
+        element_force<PLASTIC>(c, e, sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3], B, c.vol[e], sample, f,
+                               diss, se);
 #pragma unroll
         for (int k = 0; k < 12; ++k) sh_f[k * kTile + threadIdx.x] = f[k];
     }
@@ -766,7 +748,7 @@ void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
 
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
                        int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream) {
-    const size_t lds = 12 * kTile * sizeof(float) + (16 + 24) * (size_t)lmax;
+    const size_t lds = 12 * kTile * sizeof(float) + 16 * (size_t)lmax;
     if (feat_union & kFeatPlastic)
         hipLaunchKernelGGL((k_elem_tiled<true>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
                            ntiles, lmax, parity, sample, xcd);

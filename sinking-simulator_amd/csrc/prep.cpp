@@ -35,7 +35,7 @@ struct Layout {
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
     uint64_t conn, dmi, vol, force, pstate, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
-        ell_off, ell, Xd, lconn, tn_off, tnode, lofs, lent, partial, pell_off, pell, acc_e, acc_n,
+        ell_off, ell, lconn, tn_off, tnode, lofs, lent, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
 
@@ -60,7 +60,6 @@ ArenaMap mapArena(const HostCase& hc) {
     m.presc = L.take((hc.features & kFeatDrive) ? 12 * Np : 0);
     m.ell_off = L.take(4 * (uint64_t)hc.ell_off.size());
     m.ell = L.take(4 * (uint64_t)hc.ell.size());
-    m.Xd = L.take(24 * N);
     m.lconn = L.take(8 * T);
     m.tn_off = L.take(4 * (uint64_t)hc.tn_off.size());
     m.tnode = L.take(4 * (uint64_t)hc.tnode.size());
@@ -446,7 +445,6 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.presc = (hc.features & kFeatDrive) ? reinterpret_cast<const float*>(base + m.presc) : nullptr;
     r.ell_off = reinterpret_cast<const uint32_t*>(base + m.ell_off);
     r.ell = reinterpret_cast<const uint32_t*>(base + m.ell);
-    r.Xd = reinterpret_cast<const double*>(base + m.Xd);
     r.lconn = reinterpret_cast<const uint2*>(base + m.lconn);
     r.tn_off = reinterpret_cast<const uint32_t*>(base + m.tn_off);
     r.tnode = reinterpret_cast<const uint32_t*>(base + m.tnode);
@@ -502,7 +500,6 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.presc, hc.presc.data(), 4ull * hc.presc.size());
     up(m.ell_off, hc.ell_off.data(), 4ull * hc.ell_off.size());
     up(m.ell, hc.ell.data(), 4ull * hc.ell.size());
-    up(m.Xd, hc.X.data(), 8ull * hc.X.size());
     up(m.lconn, hc.lconn.data(), 4ull * hc.lconn.size());
     up(m.tn_off, hc.tn_off.data(), 4ull * hc.tn_off.size());
     up(m.tnode, hc.tnode.data(), 4ull * hc.tnode.size());

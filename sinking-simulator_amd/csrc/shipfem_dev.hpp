@@ -71,7 +71,6 @@ struct CaseDev {
     const uint32_t* ell_off;  // [node_waves + 1] first entry of each wave's slab
     const uint32_t* ell;      // wave slabs, entry e of lane l at ell_off[w] + 64 e + l
     // ---- tiled element pass: 256-tet tiles staged through LDS
-    const double* Xd;         // [N][3] rest positions (FP64; Dm^-1 is re-derived per step)
     const uint2* lconn;       // [T] four 16-bit tile-local node indices per tet
     const uint32_t* tn_off;   // [elem_tiles + 1] first (tile, local node) slot of each tile
     const uint32_t* tnode;    // [P] global node id of each (tile, local node) slot
@@ -115,9 +114,9 @@ struct Tile {
 
 // Element-pass variants. kSlots: one lane per tet writes its four corner forces to a
 // 48-byte slot (the reference's tetForce layout) and the node pass gathers slots
-// through the CSR node map. kTiled: a 256-tet tile stages its nodes' displacement and
-// rest coordinates in LDS, re-derives Dm^-1 in FP64, and sums corner forces per local
-// node in LDS; the node pass gathers one partial per (tile, node).
+// through the CSR node map. kTiled: a 256-tet tile stages its nodes' displacement in
+// LDS and sums corner forces per local node in LDS; the node pass gathers one partial
+// per (tile, node).
 enum Variant : int { kSlots = 0, kTiled = 1 };
 constexpr int kTile = 256;  // tets per tile of the tiled variant (fixed: it sets the sum order)
 
