@@ -228,9 +228,11 @@ shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out);
 /* Launch shape and element-pass variant of this solver (zeros keep the defaults:
  * tiled, 256-thread node blocks, XCD remap on). */
 shipfem_status shipfem_solver_set_launch(shipfem_solver* s, const shipfem_launch_config* cfg);
-/* Runs `steps` plain steps with HIP events around every launch on the solver's
- * stream and returns the mean element-pass and node-pass durations in ms (the
- * per-kernel split of the timings fem_gpu.cpp:388-421 takes around a whole run). */
+/* Runs `steps` plain steps on the solver's stream, each element and node launch
+ * carrying its own start/stop events (recorded by the dispatch, so the elapsed
+ * time is the kernel's execution), and returns the mean element-pass and node-pass
+ * durations in ms (the per-kernel split of the timings fem_gpu.cpp:388-421 takes
+ * around a whole run). */
 shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* elem_ms,
                                       double* node_ms);
 /* Device bytes held by the solver. */

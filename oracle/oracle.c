@@ -4,31 +4,40 @@
  * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg, never by the product.
  *
- * Parity pins (what checks each function; the tests named live under tests/):
- *   ora_box_mesh / ora_box_mesh_f32  golden connectivity from the compiled reference
- *                                    makeBoxTetMesh (fem.cpp:217-263), test_oracle_golden
- *   ora_rest_state                   golden restInv/volume/mass/CSR from the compiled
+ * Parity pins (what checks each function). Golden vectors from the compiled reference
+ * are in test_oracle_golden.py; checks fixed by the mathematics in test_oracle_pins.py.
+ *   ora_box_mesh / ora_box_mesh_f32  golden connectivity and positions of the compiled
+ *                                    makeBoxTetMesh (fem.cpp:217-263)
+ *   ora_rest_state                   golden restInv/volume/mass/CSR of the compiled
  *                                    computeRestState (fem.cpp:119-158); exact volume sum
  *   ora_edge_timestep                golden criticalTimestep (fem.cpp:183-185)
- *   ora_max_element_omega            brute-force 12x12 eigen-solve vs a closed form for a
- *                                    right-corner tet, and stability at 0.99 / blow-up
- *                                    at 1.2 of the bound on a real mesh
+ *   ora_max_element_omega            eigenvalues of a stiffness assembled by central
+ *                                    differences of ora_tet_force; leapfrog stable at
+ *                                    0.99 of 2/omega and divergent at 1.03 (single tet)
  *   ora_tet_force                    compiled tetForces on the twisted fem_spike mesh;
- *                                    rigid motion => zero force; sum f = 0, sum x^f = 0;
- *                                    patch test (uniform strain => exact traction)
+ *                                    rigid rotation => zero force; zero net moment;
+ *                                    patch test: uniform strain gives the consistent
+ *                                    nodal loads of the exact traction at every node,
+ *                                    also rotated
  *   ora_j2_update                    compiled plasticity::update on strain paths;
- *                                    uniaxial closed form, sigma_y/sqrt3 in shear,
- *                                    hydrostatic no-yield (test_plasticity.cpp:291-750)
+ *                                    pure-shear and uniaxial-strain closed forms with and
+ *                                    without hardening, step-size independence on radial
+ *                                    paths, hydrostatic no-yield, zero plastic trace,
+ *                                    return onto the surface, idempotence
  *   ora_face_traction                total force = traction * face area (closed form)
- *   ora_run                          compiled stepCpu trajectory (elastic, FP32 ref vs
- *                                    FP64 oracle agree to FP32 rounding); rest stays at
- *                                    rest; rigid translation; energy closes for an
- *                                    undamped linear run; contact momentum and work
- *   ora_panel_mesh / ora_box_jitter  positive volumes, counts in closed form, exact
- *                                    agreement with the product's generators
+ *   ora_run                          compiled stepCpu trajectory (FP32 reference vs FP64
+ *                                    oracle to FP32 rounding); rest and free translation;
+ *                                    energy closes to 1e-9 for load, drive + damping;
+ *                                    settled drive stores 1/2 E A d^2/L with nu contraction;
+ *                                    contact impulse = momentum, contact work closes
+ *   ora_panel_mesh                   positive volumes, closed-form volume and surface area,
+ *                                    conforming (no face shared by three tets), web lines
+ *   ora_box_jitter / ora_uniform     faces kept planar, amplitude bound, seed dependence;
+ *                                    SplitMix64 published outputs
  *   ref32_*                          bit-identical to the compiled reference
  *                                    (fem.cpp, -ffp-contract=off), 1 and N threads
  */
+#define _POSIX_C_SOURCE 200809L /* pthread barriers */
 #include "oracle.h"
 
 #include <math.h>
@@ -601,116 +610,242 @@ static double drive_fraction(const ora_case* c, double t) {
     return s * s * (3.0 - 2.0 * s);
 }
 
-void ora_run(const ora_case* c, ora_state* st, int64_t steps, int64_t sample_every,
-             ora_energy* rec, int64_t max_rec, int64_t* nrec) {
+/* Per-step sums of one range of elements or nodes. */
+typedef struct {
+    double strain, diss;                              /* element range */
+    double w_ext, w_con, w_bnd, damp;                 /* node range, running terms */
+    double kinetic, half_ext, half_con, half_bnd, half_damp, fcon[3]; /* sample terms */
+} ora_sums;
+
+/* element pass at u_n over tets [e0, e1) */
+static void run_elements(const ora_case* c, ora_state* st, int64_t e0, int64_t e1, int sample,
+                         ora_sums* out) {
     const double lam = c->E * c->nu / ((1 + c->nu) * (1 - 2 * c->nu));
     const double mu = c->E / (2 * (1 + c->nu));
     const ora_j2 j2 = {c->E, c->nu, c->yield, c->hardening};
+    for (int64_t e = e0; e < e1; ++e) {
+        const uint32_t* idx = &c->index[e * 4];
+        double x[12];
+        for (int q = 0; q < 4; ++q)
+            for (int a = 0; a < 3; ++a)
+                x[q * 3 + a] = c->X[idx[q] * 3 + a] + st->u[idx[q] * 3 + a];
+        double de = 0, ue = 0;
+        tet_force_general(x, &c->rest_inv[e * 9], c->volume[e], lam, mu,
+                          c->plastic ? &j2 : NULL, c->plastic ? &st->eps_p[e * 6] : NULL,
+                          c->plastic ? &st->eqps[e] : NULL, &st->force[e * 12], &de,
+                          sample ? &ue : NULL);
+        out->diss += de;
+        out->strain += ue;
+    }
+}
+
+/* node pass of step n over nodes [i0, i1): gather in CSR order, loads, contact, pins,
+ * drive, damping; every impulse is worked at the mid-step velocity (vo + vn)/2, the
+ * form under which the shadow kinetic energy makes the account close exactly */
+static void run_nodes(const ora_case* c, ora_state* st, int64_t i0, int64_t i1, int64_t n,
+                      int sample, ora_sums* out) {
     const double dt = c->dt;
-    for (int64_t k = 0; k < steps; ++k) {
-        const int64_t n = st->step;
-        const double t = (double)n * dt;
-        const int sample = sample_every > 0 && n % sample_every == 0 && rec && nrec &&
-                           *nrec < max_rec;
-        /* element pass at u_n */
-        double strain_energy = 0, diss = 0;
-        for (int64_t e = 0; e < c->T; ++e) {
-            const uint32_t* idx = &c->index[e * 4];
-            double x[12];
-            for (int q = 0; q < 4; ++q)
-                for (int a = 0; a < 3; ++a)
-                    x[q * 3 + a] = c->X[idx[q] * 3 + a] + st->u[idx[q] * 3 + a];
-            double de = 0, ue = 0;
-            tet_force_general(x, &c->rest_inv[e * 9], c->volume[e], lam, mu,
-                              c->plastic ? &j2 : NULL, c->plastic ? &st->eps_p[e * 6] : NULL,
-                              c->plastic ? &st->eqps[e] : NULL, &st->force[e * 12], &de,
-                              sample ? &ue : NULL);
-            diss += de;
-            strain_energy += ue;
+    const double t = (double)n * dt;
+    const double sl = load_scale(c, t);
+    const double sd = drive_fraction(c, t + dt);
+    for (int64_t i = i0; i < i1; ++i) {
+        const uint32_t fl = c->flags ? c->flags[i] : 0u;
+        const double m = c->mass[i];
+        if ((fl & (ORA_PIN_X | ORA_PIN_Y | ORA_PIN_Z)) == (ORA_PIN_X | ORA_PIN_Y | ORA_PIN_Z)) {
+            for (int a = 0; a < 3; ++a) st->v[i * 3 + a] = 0.0;
+            continue;
         }
-        st->dissipation += diss;
-        /* node pass */
-        const double sl = load_scale(c, t);
-        const double sd = drive_fraction(c, t + dt);
-        double kinetic = 0, half_ext = 0, half_con = 0, fcon[3] = {0, 0, 0};
-        for (int64_t i = 0; i < c->N; ++i) {
-            const uint32_t fl = c->flags ? c->flags[i] : 0u;
-            const double m = c->mass[i];
-            if ((fl & (ORA_PIN_X | ORA_PIN_Y | ORA_PIN_Z)) == (ORA_PIN_X | ORA_PIN_Y | ORA_PIN_Z)) {
-                for (int a = 0; a < 3; ++a) st->v[i * 3 + a] = 0.0;
+        double f[3] = {0, 0, 0};
+        for (uint32_t q = c->adj_offset[i]; q < c->adj_offset[i + 1]; ++q) {
+            const uint32_t en = c->adj_entry[q];
+            const double* s = &st->force[(en >> 2) * 12 + (en & 3u) * 3];
+            for (int a = 0; a < 3; ++a) f[a] += s[a];
+        }
+        double fe[3] = {0, 0, -c->gravity * m};
+        if (c->load)
+            for (int a = 0; a < 3; ++a) fe[a] += sl * c->load[i * 3 + a];
+        double fc[3] = {0, 0, 0};
+        if (c->indenter && (fl & ORA_CONTACT)) {
+            double d[3], d2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                d[a] = c->X[i * 3 + a] + st->u[i * 3 + a] - (c->ind_center[a] + t * c->ind_velocity[a]);
+                d2 += d[a] * d[a];
+            }
+            const double dist = sqrt(d2);
+            if (dist < c->ind_radius && dist > 0) {
+                const double g = c->ind_radius - dist;
+                const double k = c->ind_kappa * m / (dt * dt);
+                for (int a = 0; a < 3; ++a) fc[a] = k * g * d[a] / dist;
+            }
+        }
+        if (sample)
+            for (int a = 0; a < 3; ++a) out->fcon[a] += fc[a];
+        for (int a = 0; a < 3; ++a) {
+            double* v = &st->v[i * 3 + a];
+            if (fl & (ORA_PIN_X << a)) {
+                *v = 0.0;
                 continue;
             }
-            double f[3] = {0, 0, 0};
-            for (uint32_t q = c->adj_offset[i]; q < c->adj_offset[i + 1]; ++q) {
-                const uint32_t en = c->adj_entry[q];
-                const double* s = &st->force[(en >> 2) * 12 + (en & 3u) * 3];
-                for (int a = 0; a < 3; ++a) f[a] += s[a];
-            }
-            double fe[3] = {0, 0, -c->gravity * m};
-            if (c->load)
-                for (int a = 0; a < 3; ++a) fe[a] += sl * c->load[i * 3 + a];
-            double fc[3] = {0, 0, 0};
-            if (c->indenter && (fl & ORA_CONTACT)) {
-                double d[3], d2 = 0;
-                for (int a = 0; a < 3; ++a) {
-                    d[a] = c->X[i * 3 + a] + st->u[i * 3 + a] -
-                           (c->ind_center[a] + t * c->ind_velocity[a]);
-                    d2 += d[a] * d[a];
-                }
-                const double dist = sqrt(d2);
-                if (dist < c->ind_radius && dist > 0) {
-                    const double g = c->ind_radius - dist;
-                    const double k = c->ind_kappa * m / (dt * dt);
-                    for (int a = 0; a < 3; ++a) fc[a] = k * g * d[a] / dist;
+            const double vo = *v;
+            double vn;
+            if (c->prescribed && (fl & (ORA_DRIVE_X << a))) {
+                /* the constraint impulse m (vn - vfree) of the drive (zone.cpp:1134-1145);
+                 * the reference works it at the end velocity, which leaves an O(dt)
+                 * residual against this account */
+                const double vfree = vo + (f[a] + fe[a] + fc[a]) / m * dt;
+                vn = (c->prescribed[i * 3 + a] * sd - st->u[i * 3 + a]) / dt;
+                const double J = m * (vn - vfree);
+                out->w_bnd += J * (vo + vn) * 0.5;
+                if (sample) out->half_bnd += J * vn * 0.5;
+            } else {
+                vn = vo + (f[a] + fe[a] + fc[a]) / m * dt;
+                if (c->damping != 1.0) {
+                    /* velocity scaling (zone.cpp:1113-1121) as the impulse m (d - 1) v */
+                    const double J = m * (1.0 - c->damping) * vn;
+                    out->damp += J * (vo + c->damping * vn) * 0.5;
+                    if (sample) out->half_damp += J * c->damping * vn * 0.5;
+                    vn *= c->damping;
                 }
             }
-            if (sample)
-                for (int a = 0; a < 3; ++a) fcon[a] += fc[a];
-            for (int a = 0; a < 3; ++a) {
-                double* v = &st->v[i * 3 + a];
-                if (fl & (ORA_PIN_X << a)) {
-                    *v = 0.0;
-                    continue;
-                }
-                const double vo = *v;
-                double vn;
-                if (c->prescribed && (fl & (ORA_DRIVE_X << a))) {
-                    const double vfree = vo + (f[a] + fe[a] + fc[a]) / m * dt;
-                    vn = (c->prescribed[i * 3 + a] * sd - st->u[i * 3 + a]) / dt;
-                    st->w_boundary += m * (vn - vfree) * vn;
-                } else {
-                    vn = vo + (f[a] + fe[a] + fc[a]) / m * dt;
-                    if (c->damping != 1.0) {
-                        st->damping_loss += 0.5 * m * vn * vn * (1.0 - c->damping * c->damping);
-                        vn *= c->damping;
-                    }
-                }
-                st->w_ext += fe[a] * (vo + vn) * 0.5 * dt;
-                st->w_contact += fc[a] * (vo + vn) * 0.5 * dt;
-                if (sample) {
-                    half_ext += fe[a] * vn * 0.5 * dt;
-                    half_con += fc[a] * vn * 0.5 * dt;
-                    kinetic += 0.5 * m * vo * vn;
-                }
-                *v = vn;
-                st->u[i * 3 + a] += vn * dt;
+            out->w_ext += fe[a] * (vo + vn) * 0.5 * dt;
+            out->w_con += fc[a] * (vo + vn) * 0.5 * dt;
+            if (sample) {
+                out->half_ext += fe[a] * vn * 0.5 * dt;
+                out->half_con += fc[a] * vn * 0.5 * dt;
+                out->kinetic += 0.5 * m * vo * vn;
             }
+            *v = vn;
+            st->u[i * 3 + a] += vn * dt;
         }
-        if (sample) {
-            ora_energy* r = &rec[(*nrec)++];
-            r->step = n;
-            r->time = t;
-            r->strain = strain_energy;
-            r->kinetic = kinetic;
-            r->dissipation = st->dissipation;
-            r->w_ext = st->w_ext - half_ext;
-            r->w_contact = st->w_contact - half_con;
-            r->w_boundary = st->w_boundary;
-            r->damping_loss = st->damping_loss;
-            for (int a = 0; a < 3; ++a) r->contact_force[a] = fcon[a];
-        }
-        st->step = n + 1;
     }
+}
+
+static void add_sums(ora_sums* a, const ora_sums* b) {
+    a->strain += b->strain;
+    a->diss += b->diss;
+    a->w_ext += b->w_ext;
+    a->w_con += b->w_con;
+    a->w_bnd += b->w_bnd;
+    a->damp += b->damp;
+    a->kinetic += b->kinetic;
+    a->half_ext += b->half_ext;
+    a->half_con += b->half_con;
+    a->half_bnd += b->half_bnd;
+    a->half_damp += b->half_damp;
+    for (int k = 0; k < 3; ++k) a->fcon[k] += b->fcon[k];
+}
+
+static int sample_now(int64_t n, int64_t sample_every, const ora_energy* rec, int64_t max_rec,
+                      const int64_t* nrec) {
+    return sample_every > 0 && n % sample_every == 0 && rec && nrec && *nrec < max_rec;
+}
+
+/* fold one step's sums into the running account and, on sample steps, a record */
+static void close_step(const ora_case* c, ora_state* st, int64_t n, int sample, const ora_sums* s,
+                       ora_energy* rec, int64_t* nrec) {
+    st->dissipation += s->diss;
+    st->w_ext += s->w_ext;
+    st->w_contact += s->w_con;
+    st->w_boundary += s->w_bnd;
+    st->damping_loss += s->damp;
+    if (sample) {
+        ora_energy* r = &rec[(*nrec)++];
+        r->step = n;
+        r->time = (double)n * c->dt;
+        r->strain = s->strain;
+        r->kinetic = s->kinetic;
+        r->dissipation = st->dissipation;
+        r->w_ext = st->w_ext - s->half_ext;
+        r->w_contact = st->w_contact - s->half_con;
+        r->w_boundary = st->w_boundary - s->half_bnd;
+        r->damping_loss = st->damping_loss - s->half_damp;
+        for (int a = 0; a < 3; ++a) r->contact_force[a] = s->fcon[a];
+    }
+    st->step = n + 1;
+}
+
+void ora_run(const ora_case* c, ora_state* st, int64_t steps, int64_t sample_every,
+             ora_energy* rec, int64_t max_rec, int64_t* nrec) {
+    for (int64_t k = 0; k < steps; ++k) {
+        const int64_t n = st->step;
+        const int sample = sample_now(n, sample_every, rec, max_rec, nrec);
+        ora_sums s;
+        memset(&s, 0, sizeof s);
+        run_elements(c, st, 0, c->T, sample, &s);
+        run_nodes(c, st, 0, c->N, n, sample, &s);
+        close_step(c, st, n, sample, &s, rec, nrec);
+    }
+}
+
+/* The same steps on `threads` threads: a static split of the element and node loops
+ * (each element and node computes exactly as above), per-thread sums folded in
+ * thread order. Deterministic for a given thread count. */
+typedef struct {
+    const ora_case* c;
+    ora_state* st;
+    int64_t steps, sample_every, max_rec;
+    ora_energy* rec;
+    int64_t* nrec;
+    int threads;
+    pthread_barrier_t* bar;
+    ora_sums* sums; /* [threads] */
+} ora_mt_shared;
+
+typedef struct {
+    ora_mt_shared* sh;
+    int id;
+} ora_mt_arg;
+
+static void* ora_mt_worker(void* p) {
+    const ora_mt_arg* a = (const ora_mt_arg*)p;
+    ora_mt_shared* sh = a->sh;
+    const ora_case* c = sh->c;
+    const int id = a->id, nt = sh->threads;
+    const int64_t e0 = c->T * id / nt, e1 = c->T * (id + 1) / nt;
+    const int64_t i0 = c->N * id / nt, i1 = c->N * (id + 1) / nt;
+    for (int64_t k = 0; k < sh->steps; ++k) {
+        const int64_t n = sh->st->step;
+        const int sample = sample_now(n, sh->sample_every, sh->rec, sh->max_rec, sh->nrec);
+        ora_sums* mine = &sh->sums[id];
+        memset(mine, 0, sizeof *mine);
+        run_elements(c, sh->st, e0, e1, sample, mine);
+        pthread_barrier_wait(sh->bar);
+        run_nodes(c, sh->st, i0, i1, n, sample, mine);
+        pthread_barrier_wait(sh->bar);
+        if (id == 0) {
+            ora_sums tot;
+            memset(&tot, 0, sizeof tot);
+            for (int t = 0; t < nt; ++t) add_sums(&tot, &sh->sums[t]);
+            close_step(c, sh->st, n, sample, &tot, sh->rec, sh->nrec);
+        }
+        pthread_barrier_wait(sh->bar);
+    }
+    return NULL;
+}
+
+void ora_run_mt(const ora_case* c, ora_state* st, int64_t steps, int64_t sample_every,
+                ora_energy* rec, int64_t max_rec, int64_t* nrec, int threads) {
+    if (threads <= 1 || c->T < 2 * (int64_t)threads) {
+        ora_run(c, st, steps, sample_every, rec, max_rec, nrec);
+        return;
+    }
+    pthread_barrier_t bar;
+    pthread_barrier_init(&bar, NULL, (unsigned)threads);
+    ora_sums* sums = (ora_sums*)calloc((size_t)threads, sizeof(ora_sums));
+    ora_mt_arg* args = (ora_mt_arg*)calloc((size_t)threads, sizeof(ora_mt_arg));
+    pthread_t* tid = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+    ora_mt_shared sh = {c, st, steps, sample_every, max_rec, rec, nrec, threads, &bar, sums};
+    for (int t = 0; t < threads; ++t) {
+        args[t].sh = &sh;
+        args[t].id = t;
+    }
+    for (int t = 1; t < threads; ++t) pthread_create(&tid[t], NULL, ora_mt_worker, &args[t]);
+    ora_mt_worker(&args[0]);
+    for (int t = 1; t < threads; ++t) pthread_join(tid[t], NULL);
+    pthread_barrier_destroy(&bar);
+    free(tid);
+    free(args);
+    free(sums);
 }
 
 /* ======================================================================== */

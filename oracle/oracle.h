@@ -165,6 +165,12 @@ void ora_case_free(ora_case* c);
 void ora_run(const ora_case* c, ora_state* s, int64_t steps, int64_t sample_every,
              ora_energy* rec, int64_t max_rec, int64_t* nrec);
 
+/* ora_run on `threads` threads (static split of the element and node loops, per-thread
+ * sums folded in thread order): the same per-element and per-node arithmetic, sums in
+ * another order. */
+void ora_run_mt(const ora_case* c, ora_state* s, int64_t steps, int64_t sample_every,
+                ora_energy* rec, int64_t max_rec, int64_t* nrec, int threads);
+
 /* ---- FP32 reference restatement (engine/sim/fem.cpp, float op order) ------- */
 void ref32_rest_state(float* position, int64_t N, const uint32_t* index, int64_t T, float density,
                       float* rest_inv, float* volume, float* mass, uint32_t* adj_offset,

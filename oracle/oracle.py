@@ -102,6 +102,7 @@ def _declare(L):
     L.ora_case_free.argtypes = [C.POINTER(CaseStruct)]
     L.ora_run.argtypes = [C.POINTER(CaseStruct), C.POINTER(StateStruct), C.c_int64, C.c_int64,
                           C.POINTER(EnergyStruct), C.c_int64, I64]
+    L.ora_run_mt.argtypes = L.ora_run.argtypes + [C.c_int]
     L.ref32_rest_state.argtypes = [F, C.c_int64, U32, C.c_int64, C.c_float, F, F, F, U32, U32]
     L.ref32_tet_forces.argtypes = [F, U32, F, F, C.c_int64, C.c_float, C.c_float, F]
     step_args = [F, F, F, U32, C.c_int64, U32, F, F, C.c_int64, F, U32, U32, C.c_float,
@@ -269,13 +270,19 @@ class Case:
     def step(self):
         return self.st.step
 
-    def run(self, steps, sample_every=0):
-        """Advance; returns the energy records (list of dicts) of sampled steps."""
+    def run(self, steps, sample_every=0, threads=1):
+        """Advance; returns the energy records (list of dicts) of sampled steps.
+        threads > 1 splits the element and node loops statically: u, v and the plastic
+        state are bit-identical to one thread; the energy sums differ in rounding."""
         max_rec = (steps // sample_every + 2) if sample_every > 0 else 0
         rec = (EnergyStruct * max(max_rec, 1))()
         n = C.c_int64(0)
-        lib().ora_run(C.byref(self.s), C.byref(self.st), steps, sample_every, rec, max_rec,
-                      C.byref(n))
+        if threads > 1:
+            lib().ora_run_mt(C.byref(self.s), C.byref(self.st), steps, sample_every, rec,
+                             max_rec, C.byref(n), int(threads))
+        else:
+            lib().ora_run(C.byref(self.s), C.byref(self.st), steps, sample_every, rec, max_rec,
+                          C.byref(n))
         out = []
         for i in range(n.value):
             r = rec[i]

@@ -286,19 +286,20 @@ shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* 
                                       double* node_ms) {
     if (!s || steps < 1) return SHIPFEM_INVALID_ARG;
     Plan& p = s->plan;
-    std::vector<hipEvent_t> ev(3 * static_cast<size_t>(steps), nullptr);
+    // two events per kernel, recorded by the dispatches themselves: the elapsed time is
+    // the kernel's execution, the figure a kernel trace reports
+    std::vector<hipEvent_t> ev(4 * static_cast<size_t>(steps), nullptr);
     hipError_t e = hipSuccess;
     for (auto& x : ev)
         if (e == hipSuccess) e = hipEventCreate(&x);
     const int S = s->hc.sample_every;
     for (int k = 0; k < steps && e == hipSuccess; ++k) {
         const int sample = s->host_step % S == 0 ? 1 : 0;
-        e = hipEventRecord(ev[3 * k], s->stream);
-        p.launchElem(p.parity, sample, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 1], s->stream);
-        p.launchNode(p.parity, sample, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(ev[3 * k + 2], s->stream);
+        const KernelTiming te{ev[4 * k], ev[4 * k + 1]}, tn{ev[4 * k + 2], ev[4 * k + 3]};
+        p.launchElem(p.parity, sample, s->stream, &te);
+        p.launchNode(p.parity, sample, s->stream, &tn);
         if (sample) launch_reduce(p.d_recs, static_cast<int>(p.recs.size()), p.parity, s->stream);
+        e = hipGetLastError();
         p.parity ^= 1;
         ++s->host_step;
     }
@@ -306,8 +307,8 @@ shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* 
     double te = 0, tn = 0;
     for (int k = 0; k < steps && e == hipSuccess; ++k) {
         float a = 0, b = 0;
-        e = hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]);
-        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
+        e = hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[4 * k + 2], ev[4 * k + 3]);
         te += a;
         tn += b;
     }

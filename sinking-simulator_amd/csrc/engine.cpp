@@ -92,18 +92,18 @@ void Plan::launchStep(int par, int sample, hipStream_t s) {
     launchNode(par, sample, s);
 }
 
-void Plan::launchElem(int par, int sample, hipStream_t s) {
+void Plan::launchElem(int par, int sample, hipStream_t s, const KernelTiming* t) {
     if (shape.variant == kTiled)
         launch_elem_tiled(d_recs, d_elem, static_cast<int>(elem_tiles.size()), lmax, par, sample,
-                          shape.xcd_remap, feat_union, s);
+                          shape.xcd_remap, feat_union, s, t);
     else
         launch_elem(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
-                    sample, shape.xcd_remap, feat_union, s);
+                    sample, shape.xcd_remap, feat_union, s, t);
 }
 
-void Plan::launchNode(int par, int sample, hipStream_t s) {
+void Plan::launchNode(int par, int sample, hipStream_t s, const KernelTiming* t) {
     launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, par, sample,
-                shape.xcd_remap, feat_union, shape.variant, s);
+                shape.xcd_remap, feat_union, shape.variant, s, t);
 }
 
 void Plan::stepPlain(bool sample, hipStream_t s) {

@@ -3,12 +3,14 @@
 // Per step, two passes over a tile table (tiles never span cases):
 //   element pass: one lane per tet. Gathers the four nodes' displacement (float4),
 //     forms F = I + G with G = Du Dm^-1, takes the rotation by Higham's iteration in
-//     perturbation form R = I + W (exactly four iterations, as fem.cpp:62-69), the
-//     co-rotated strain eps = sym(W^T + G + W^T G) (no cancellation against I in
-//     FP32), Hooke or a J2 radial return, P = R sigma, and writes the four nodal
-//     forces -V P Dm^-T to the tet's own 48-byte slot (fem.cpp:73-117).
-//   node pass: one lane per node. Gathers the node's slots in the fixed CSR order
-//     from a wave-transposed slab (coalesced index loads), adds ramped dead load,
+//     perturbation form R = I + W (four iterations, as fem.cpp:62-69, the last two
+//     skipped once the second has converged below FP32 resolution of the strain),
+//     the co-rotated strain eps = sym(W^T + G + W^T G) (no cancellation against I in
+//     FP32), Hooke or a J2 radial return, P = R sigma, and the four nodal forces
+//     -V P Dm^-T (fem.cpp:73-117). The slots variant writes them to the tet's own
+//     48-byte slot; the tiled variant sums them per tile-local node in LDS.
+//   node pass: one lane per node. Gathers the node's slots (or tile partials) in a
+//     fixed order from a wave-transposed slab (coalesced index loads), adds ramped dead load,
 //     gravity and sphere penalty contact, applies pins and the prescribed drive,
 //     and advances v and u with Kahan compensation (fem.cpp:187-215, zone.cpp:1094-1200).
 // No atomics anywhere: energy terms are FP64 per-wave sums (xor butterfly, the same
@@ -16,8 +18,21 @@
 // the reduce kernel on sample steps.
 #include "shipfem_dev.hpp"
 
+#include <hip/hip_ext.h>
+
 namespace shipfem {
 namespace {
+
+// Plain launch, or with start/stop events carried on the dispatch itself (the events
+// then bracket exactly the kernel's execution, with no marker packets in between).
+#define SHIPFEM_LAUNCH(timing, kern, grid, block, lds, stream, ...)                           \
+    do {                                                                                      \
+        if (timing)                                                                           \
+            hipExtLaunchKernelGGL(kern, grid, block, lds, stream, (timing)->start,             \
+                                  (timing)->stop, 0, __VA_ARGS__);                             \
+        else                                                                                  \
+            hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);                   \
+    } while (0)
 
 __device__ __forceinline__ int remap_block(int b, int nb, int xcd) {
     if (!xcd) return b;
@@ -75,27 +90,32 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, cons
         const float t = W[0] + W[4] + W[8];
         const float i2 = a[0] + a[4] + a[8];
         const float dW = W[0] * a[0] + W[1] * a[3] + W[2] * a[6];
-        const float den = 1.0f + (t + (i2 + dW));
+        const float q = t + (i2 + dW);  // det(I + W) - 1
+        const float den = 1.0f + q;
         const bool ok = fabsf(den) > 1e-30f;  // the reference's singular guard
         // hardware reciprocal (<= 1 ulp): Newton's iteration corrects all but the last
         // iteration's rounding, which this changes by at most an ulp
         const float id = ok ? __builtin_amdgcn_rcpf(den) : 0.0f;
+        // W' = (W + E^T)/2 with E = (adj W - W - sh I)/den, or E = -I when singular.
+        // fma(1/2, W, (id/2) x) rounds exactly as (W + id x)/2: halving is exact.
+        const float hid = 0.5f * id, hnd = ok ? 0.0f : 0.5f;
         const float sh = i2 + dW;
-        float En[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const float diag = i == j ? 1.0f : 0.0f;
-                En[i * 3 + j] = ok ? (a[i * 3 + j] - W[i * 3 + j] - sh * diag) * id : -diag;
-            }
         float Wn[9];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) Wn[i * 3 + j] = 0.5f * (W[i * 3 + j] + En[j * 3 + i]);
+            for (int j = 0; j < 3; ++j) {
+                const float x = a[j * 3 + i] - W[j * 3 + i];
+                Wn[i * 3 + j] = i == j ? fmaf(0.5f, W[i * 3 + j], fmaf(hid, x - sh, -hnd))
+                                       : fmaf(0.5f, W[i * 3 + j], hid * x);
+            }
 #pragma unroll
         for (int k = 0; k < 9; ++k) W[k] = Wn[k];
+        // Quadratic convergence: after the second step the stretch error d left in
+        // R = I + W satisfies 0 <= d_i and sum d_i ~ q, and the next step leaves d^2/2.
+        // With |q| < 2^-18 the two remaining steps would move the co-rotated strain by
+        // less than 2^-37, below FP32 resolution of any strain they could act on.
+        if (it == 1 && fabsf(q) < 0x1p-18f) break;
     }
 
     // co-rotated small strain: R^T F - I = W^T + G + W^T G
@@ -242,10 +262,18 @@ __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ case
 }
 
 // ---------------------------------------------------------------- element pass, tiled
-// One block per 256-tet tile. LDS: corner forces [12][256] floats, then the tile's
-// local nodes' displacement float4[lmax]. Dm^-1 and V stream as planes (the kernel is
-// ALU-bound; re-deriving them in FP64 from rest coordinates cost more issue slots than
-// the 40 bytes per tet it saved).
+// One block per 256-tet tile. LDS: corner forces [12][256] floats, the tile-local CSR
+// (entries [1024] and offsets [lmax + 1], 16-bit), then the tile's local nodes'
+// displacement float4[lmax]. Dm^-1 and V stream as planes (the kernel is ALU-bound;
+// re-deriving them in FP64 from rest coordinates cost more issue slots than the 40
+// bytes per tet it saved). Every global load a lane needs is issued before the first
+// barrier, so the staging gather, the streamed planes and the CSR share one latency.
+constexpr size_t tiled_lds_csr_offset() { return 12 * kTile * sizeof(float); }
+__host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
+    return 12 * kTile * sizeof(float) + 4 * kTile * sizeof(uint16_t) +
+           ((lmax + 1) * sizeof(uint16_t) + 15) / 16 * 16;
+}
+
 template <bool PLASTIC>
 __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict__ cases,
                                                       const Tile* __restrict__ tiles, int ntiles,
@@ -253,7 +281,9 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
                                                       int xcd) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* sh_f = reinterpret_cast<float*>(smem);
-    float4* sh_u = reinterpret_cast<float4*>(smem + 12 * kTile * sizeof(float));
+    uint16_t* sh_le = reinterpret_cast<uint16_t*>(smem + tiled_lds_csr_offset());
+    uint16_t* sh_lo = sh_le + 4 * kTile;
+    float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
 
     const int b = remap_block(blockIdx.x, gridDim.x, xcd);
     const Tile tile = tiles[b];
@@ -263,19 +293,32 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     const uint32_t tb = tile.first / kTile;
     const uint32_t p0 = c.tn_off[tb];
     const uint32_t L = c.tn_off[tb + 1] - p0;
-    for (uint32_t l = threadIdx.x; l < L; l += kTile) sh_u[l] = c.u4[c.tnode[p0 + l]];
-    __syncthreads();
-
     const uint32_t e = tile.first + threadIdx.x;
-    double diss = 0.0, se = 0.0;
-    if (e < c.T) {
-        const uint2 lc = c.lconn[e];
-        const uint32_t l0 = lc.x & 0xFFFFu, l1 = lc.x >> 16, l2 = lc.y & 0xFFFFu, l3 = lc.y >> 16;
-        float B[9];
+    const bool live = e < c.T;
+
+    // streamed per-tet inputs and this lane's share of the tile CSR, all in flight
+    // together with the staging gather below
+    uint2 lc = make_uint2(0u, 0u);
+    float B[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float V = 0.0f;
+    if (live) {
+        lc = c.lconn[e];
 #pragma unroll
         for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * c.Tpad + e];
+        V = c.vol[e];
+    }
+    const uint2 ent = reinterpret_cast<const uint2*>(c.lent + (size_t)tb * (4 * kTile))[threadIdx.x];
+    const uint16_t* lo = c.lofs + p0 + tb;
+    for (uint32_t l = threadIdx.x; l <= L; l += kTile) sh_lo[l] = lo[l];
+    for (uint32_t l = threadIdx.x; l < L; l += kTile) sh_u[l] = c.u4[c.tnode[p0 + l]];
+    reinterpret_cast<uint2*>(sh_le)[threadIdx.x] = ent;
+    __syncthreads();
+
+    double diss = 0.0, se = 0.0;
+    if (live) {
+        const uint32_t l0 = lc.x & 0xFFFFu, l1 = lc.x >> 16, l2 = lc.y & 0xFFFFu, l3 = lc.y >> 16;
         float f[12];
-        element_force<PLASTIC>(c, e, sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3], B, c.vol[e], sample, f,
+        element_force<PLASTIC>(c, e, sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3], B, V, sample, f,
                                diss, se);
 #pragma unroll
         for (int k = 0; k < 12; ++k) sh_f[k * kTile + threadIdx.x] = f[k];
@@ -284,13 +327,11 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     __syncthreads();
 
     // one lane per local node sums its corners in the tile's (tet, corner) order
-    const uint16_t* lo = c.lofs + p0 + tb;
-    const uint16_t* le = c.lent + (size_t)tb * (4 * kTile);
     for (uint32_t l = threadIdx.x; l < L; l += kTile) {
-        const uint32_t q0 = lo[l], q1 = lo[l + 1];
+        const uint32_t q0 = sh_lo[l], q1 = sh_lo[l + 1];
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (uint32_t q = q0; q < q1; ++q) {
-            const uint32_t en = le[q];
+            const uint32_t en = sh_le[q];
             const uint32_t t = en >> 2, cr = (en & 3u) * 3u;
             fx += sh_f[(cr + 0) * kTile + t];
             fy += sh_f[(cr + 1) * kTile + t];
@@ -323,7 +364,8 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
     const uint32_t np = c.Npad;
 
     double a_ext = 0.0, a_con = 0.0, a_bnd = 0.0, a_damp = 0.0;
-    double s_kin = 0.0, s_hext = 0.0, s_hcon = 0.0, s_f[3] = {0.0, 0.0, 0.0};
+    double s_kin = 0.0, s_hext = 0.0, s_hcon = 0.0, s_hbnd = 0.0, s_hdamp = 0.0;
+    double s_f[3] = {0.0, 0.0, 0.0};
 
     if ((fl & kPinAll) != kPinAll) {
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -422,7 +464,11 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
                 const float ut = uu[a] - cuu[a];
                 vn = (c.presc[a * np + i] * sd - ut) / dtf;
                 cvn = 0.0f;
-                a_bnd += (double)m * ((double)vn - (double)vfree) * (double)vn;
+                // the constraint impulse m (vn - vfree) works at the mid-step velocity,
+                // like every other impulse in the account
+                const double J = (double)m * ((double)vn - (double)vfree);
+                a_bnd += J * ((double)vo + (double)vn) * 0.5;
+                if (sample) s_hbnd += J * (double)vn * 0.5;
             } else {
                 // Kahan: the true velocity is vv - cvv
                 const float y = ftot * (im * dtf) - cvv[a];
@@ -430,8 +476,11 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
                 cvn = (tt - vo) - y;
                 vn = tt;
                 if (EXTRA && (c.features & kFeatDamp)) {
+                    // the damping impulse m (d - 1) v removes m (1 - d) v (vo + d v) / 2
                     const float d = c.damping;
-                    a_damp += 0.5 * (double)m * (double)vn * (double)vn * (1.0 - (double)d * (double)d);
+                    const double J = (double)m * (1.0 - (double)d) * (double)vn;
+                    a_damp += J * ((double)vo + (double)d * (double)vn) * 0.5;
+                    if (sample) s_hdamp += J * (double)d * (double)vn * 0.5;
                     vn *= d;
                     cvn *= d;
                 }
@@ -489,7 +538,8 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
         }
     }
     if (sample) {
-        const double vals[6] = {s_kin, s_hext, s_hcon, s_f[0], s_f[1], s_f[2]};
+        const double vals[kSmpNodeSlots] = {s_kin, s_hext, s_hcon, s_f[0], s_f[1], s_f[2],
+                                            s_hbnd, s_hdamp};
 #pragma unroll
         for (int k = 0; k < kSmpNodeSlots; ++k) {
             const double s = wave_sum(vals[k]);
@@ -550,8 +600,8 @@ __global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restri
             r.dissipation = tot[0];
             r.w_ext = tot[2 + kAccWext] - sn[kSmpHalfExt];
             r.w_contact = tot[2 + kAccWcon] - sn[kSmpHalfCon];
-            r.w_boundary = tot[2 + kAccWbnd];
-            r.damping_loss = tot[2 + kAccDamp];
+            r.w_boundary = tot[2 + kAccWbnd] - sn[kSmpHalfBnd];
+            r.damping_loss = tot[2 + kAccDamp] - sn[kSmpHalfDamp];
             r.contact_force[0] = sn[kSmpFx];
             r.contact_force[1] = sn[kSmpFy];
             r.contact_force[2] = sn[kSmpFz];
@@ -693,68 +743,71 @@ __global__ __launch_bounds__(256) void k_progress(const CaseDev* __restrict__ ca
 
 template <int BLOCK>
 void elem_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
-                 int xcd, bool plastic, hipStream_t s) {
+                 int xcd, bool plastic, hipStream_t s, const KernelTiming* t) {
     if (plastic)
-        hipLaunchKernelGGL((k_elem<BLOCK, true>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
-                           ntiles, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_elem<BLOCK, true>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                       ntiles, parity, sample, xcd);
     else
-        hipLaunchKernelGGL((k_elem<BLOCK, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
-                           ntiles, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_elem<BLOCK, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                       ntiles, parity, sample, xcd);
 }
 
 template <int BLOCK, int VARIANT>
 void node_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
-                 int xcd, bool extra, hipStream_t s) {
+                 int xcd, bool extra, hipStream_t s, const KernelTiming* t) {
     if (extra)
-        hipLaunchKernelGGL((k_node<BLOCK, true, VARIANT>), dim3(ntiles), dim3(BLOCK), 0, s, cases,
-                           tiles, ntiles, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_node<BLOCK, true, VARIANT>), dim3(ntiles), dim3(BLOCK), 0, s, cases,
+                       tiles, ntiles, parity, sample, xcd);
     else
-        hipLaunchKernelGGL((k_node<BLOCK, false, VARIANT>), dim3(ntiles), dim3(BLOCK), 0, s, cases,
-                           tiles, ntiles, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_node<BLOCK, false, VARIANT>), dim3(ntiles), dim3(BLOCK), 0, s, cases,
+                       tiles, ntiles, parity, sample, xcd);
 }
 
 template <int BLOCK>
 void node_launch_v(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
-                   int xcd, bool extra, int variant, hipStream_t s) {
+                   int xcd, bool extra, int variant, hipStream_t s, const KernelTiming* t) {
     if (variant == kTiled)
-        node_launch<BLOCK, kTiled>(cases, tiles, ntiles, parity, sample, xcd, extra, s);
+        node_launch<BLOCK, kTiled>(cases, tiles, ntiles, parity, sample, xcd, extra, s, t);
     else
-        node_launch<BLOCK, kSlots>(cases, tiles, ntiles, parity, sample, xcd, extra, s);
+        node_launch<BLOCK, kSlots>(cases, tiles, ntiles, parity, sample, xcd, extra, s, t);
 }
 
 }  // namespace
 
 void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
-                 int sample, int xcd, uint32_t feat_union, hipStream_t stream) {
+                 int sample, int xcd, uint32_t feat_union, hipStream_t stream,
+                 const KernelTiming* t) {
     const bool plastic = (feat_union & kFeatPlastic) != 0;
     switch (block) {
-        case 64: elem_launch<64>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
-        case 128: elem_launch<128>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
-        case 512: elem_launch<512>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
-        default: elem_launch<256>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream); break;
+        case 64: elem_launch<64>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
+        case 128: elem_launch<128>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
+        case 512: elem_launch<512>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
+        default: elem_launch<256>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
     }
 }
 
 void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
-                 int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream) {
+                 int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream,
+                 const KernelTiming* t) {
     const bool extra = (feat_union & (kFeatContact | kFeatDrive | kFeatDamp)) != 0;
     switch (block) {
-        case 64: node_launch_v<64>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
-        case 128: node_launch_v<128>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
-        case 512: node_launch_v<512>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
-        default: node_launch_v<256>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream); break;
+        case 64: node_launch_v<64>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream, t); break;
+        case 128: node_launch_v<128>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream, t); break;
+        case 512: node_launch_v<512>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream, t); break;
+        default: node_launch_v<256>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream, t); break;
     }
 }
 
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
-                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream) {
-    const size_t lds = 12 * kTile * sizeof(float) + 16 * (size_t)lmax;
+                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream,
+                       const KernelTiming* t) {
+    const size_t lds = tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax;
     if (feat_union & kFeatPlastic)
-        hipLaunchKernelGGL((k_elem_tiled<true>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
-                           ntiles, lmax, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_elem_tiled<true>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
+                       ntiles, lmax, parity, sample, xcd);
     else
-        hipLaunchKernelGGL((k_elem_tiled<false>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
-                           ntiles, lmax, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_elem_tiled<false>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
+                       ntiles, lmax, parity, sample, xcd);
 }
 
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream) {

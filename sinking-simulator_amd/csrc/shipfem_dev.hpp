@@ -35,7 +35,7 @@ enum { kAccWext = 0, kAccWcon = 1, kAccWbnd = 2, kAccDamp = 3, kAccNodeSlots = 4
 // per-wave FP64 sample partials (written on sample steps only)
 enum { kSmpStrain = 0, kSmpElemSlots = 1 };
 enum { kSmpKin = 0, kSmpHalfExt = 1, kSmpHalfCon = 2, kSmpFx = 3, kSmpFy = 4, kSmpFz = 5,
-       kSmpNodeSlots = 6 };
+       kSmpHalfBnd = 6, kSmpHalfDamp = 7, kSmpNodeSlots = 8 };
 
 // Same layout as shipfem_energy_record.
 struct EnergyRecordDev {
@@ -127,15 +127,25 @@ struct LaunchShape {
     int variant = kTiled;
 };
 
-// Kernel launchers (kernels.hip). All launch asynchronously on `stream`.
+// Start/stop events recorded by the kernel dispatch itself (hipExtLaunchKernel), so
+// their elapsed time is the kernel's execution alone.
+struct KernelTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+// Kernel launchers (kernels.hip). All launch asynchronously on `stream`; a non-null
+// `timing` times the launch (never used under graph capture).
 // For the tiled variant the tiles are 256-tet tiles, `block` is ignored and `lmax` sizes
 // the LDS node stage.
 void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
-                 int sample, int xcd, uint32_t feat_union, hipStream_t stream);
+                 int sample, int xcd, uint32_t feat_union, hipStream_t stream,
+                 const KernelTiming* timing = nullptr);
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
-                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream);
+                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream,
+                       const KernelTiming* timing = nullptr);
 void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
-                 int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream);
+                 int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream,
+                 const KernelTiming* timing = nullptr);
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream);
 // FP64 largest eigenvalue of B^T C B per element (the lumped-mass element frequency
 // is omega^2 = 4 lambda / rho), reduced to a per-wave max, then to one value per case.

@@ -112,8 +112,10 @@ struct Plan {
     void dropGraph();
     // The element and node launches of one step at `parity` (no reduce).
     void launchStep(int parity, int sample, hipStream_t stream);
-    void launchElem(int parity, int sample, hipStream_t stream);
-    void launchNode(int parity, int sample, hipStream_t stream);
+    void launchElem(int parity, int sample, hipStream_t stream,
+                    const KernelTiming* timing = nullptr);
+    void launchNode(int parity, int sample, hipStream_t stream,
+                    const KernelTiming* timing = nullptr);
     // One step as plain launches.
     void stepPlain(bool sample, hipStream_t stream);
     // Captures `chunk` (even) steps starting at parity 0, sample on the first.

@@ -225,6 +225,43 @@ def test_variants_agree_to_rounding():
     assert rel_l2(a.readback().u, b.readback().u) < 1e-6
 
 
+def test_prescribed_drive_with_damping_matches_oracle():
+    # a bar stretched by a smoothstep drive on one face, rollers and three pins on the
+    # other, velocity damping on: boundary work and damping loss against the oracle
+    lx, ly, lz, d = 0.4, 0.05, 0.05, 1e-5
+    m = sf.Mesh.box(lx, ly, lz, 16, 2, 2)
+    X, idx, _, _ = m.arrays()
+    at = lambda x, y, z: (np.abs(X - [x, y, z]) < 1e-9).all(axis=1)  # noqa: E731
+    fl = np.zeros(len(X), np.uint32)
+    fl[X[:, 0] < 1e-9] |= sf.PIN_X
+    fl[at(0, 0, 0)] |= sf.PIN_Y | sf.PIN_Z
+    fl[at(0, ly, 0)] |= sf.PIN_Z
+    fl[at(0, 0, lz)] |= sf.PIN_Y
+    far = X[:, 0] > lx - 1e-9
+    fl[far] |= sf.DRIVE_X
+    pres = np.zeros_like(X)
+    pres[far, 0] = d
+    case = sf.Case(X, idx, flags=fl, prescribed=pres, drive_ramp=1.5e-3, damping=0.999,
+                   sample_every=100)
+    for v in VARIANTS:
+        s = solver(case, v)
+        o = oracle_for(case, s.dt)
+        ora = o.run(2400, sample_every=100)
+        s.run(2400)
+        st = s.readback()
+        err = rel_l2(st.u, o.u)
+        g = s.energy()
+        wb = ora[-1]["w_boundary"]
+        dw = max(abs(a["w_boundary"] - b["w_boundary"]) for a, b in zip(g, ora)) / wb
+        dd = max(abs(a["damping_loss"] - b["damping_loss"]) for a, b in zip(g, ora)) / wb
+        res = max(abs(residual(a, g[0])) for a in g[1:]) / wb
+        say(f"drive+damping {VNAME[v]}: u rel L2 {err:.2e}, W_bnd {dw:.2e}, damping {dd:.2e}, "
+            f"residual/W_bnd {res:.2e}")
+        assert np.allclose(st.u[far, 0], d, rtol=1e-6)
+        assert err <= REL_U and dw <= ENERGY and dd <= ENERGY and res <= ENERGY
+        assert ora[-1]["damping_loss"] > 1e-4 * wb
+
+
 def test_stable_step_matches_oracle_eigen_bound():
     case = configs.plate(lx=0.5, ly=0.2, nx=10, ny=4)
     s = solver(case)

@@ -1,0 +1,387 @@
+"""The oracle pinned by what the mathematics fixes, independently of its own formulas.
+
+Each check is chosen so that a plausible mistake fails it: a transposed Dm^-1, a wrong
+Lame constant, a sign, a missing volume factor, a dropped hardening term, a wrong
+penalty normal or a lost half step in the energy account. CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+E, NU, RHO = 210e9, 0.3, 7850.0
+LAM, MU = O.lame(E, NU)
+K_BULK = E / (3 * (1 - 2 * NU))
+
+
+def rot(axis, angle):
+    a = np.asarray(axis, float)
+    a /= np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * K @ K
+
+
+def one_tet(seed=3):
+    rng = np.random.default_rng(seed)
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.01
+    X += rng.uniform(-0.002, 0.002, X.shape)
+    rs = O.rest_state(X, np.array([[0, 1, 2, 3]], np.uint32), RHO)
+    assert rs["volume"][0] > 0
+    return X, rs["rest_inv"][0], rs["volume"][0]
+
+
+def assemble(X, index, rs, u):
+    f = np.zeros_like(X)
+    x = X + u
+    for t, tet in enumerate(index):
+        ft = O.tet_force(x[tet].ravel(), rs["rest_inv"][t], rs["volume"][t], LAM, MU)
+        np.add.at(f, tet, ft.reshape(4, 3))
+    return f
+
+
+# ---------------------------------------------------------------- element force
+def test_rigid_rotation_is_force_free():
+    X, dmi, V = one_tet()
+    ref = np.abs(O.tet_force((X @ np.diag([1.001, 1, 1]).T).ravel(), dmi, V, LAM, MU)).max()
+    Q = rot([0.3, -1.0, 0.5], 1.1)
+    x = X @ Q.T + np.array([0.4, -0.2, 0.7])
+    f = O.tet_force(x.ravel(), dmi, V, LAM, MU)
+    assert np.abs(f).max() < 1e-9 * ref
+
+
+def test_forces_carry_no_moment():
+    # sum x_c ^ f_c = 0 holds only with the polar factor and a stress that commutes
+    # with the stretch: a wrong transpose in P = R sigma or H = P Dm^-T breaks it
+    X, dmi, V = one_tet()
+    rng = np.random.default_rng(11)
+    F = rot([1, 2, 3], 0.7) @ (np.eye(3) + 0.04 * rng.standard_normal((3, 3)))
+    x = X @ F.T
+    f = O.tet_force(x.ravel(), dmi, V, LAM, MU).reshape(4, 3)
+    moment = np.cross(x, f).sum(axis=0)
+    assert np.abs(moment).max() < 1e-10 * np.abs(f).max() * np.abs(x).max()
+
+
+@pytest.mark.parametrize("rotated", [False, True])
+def test_patch_test_uniform_strain_gives_the_exact_boundary_traction(rotated):
+    # u = A X with A symmetric: F = I + A is its own stretch (R = I), so the
+    # co-rotational element is exactly linear elastic and the nodal forces are the
+    # consistent nodal loads of the uniform stress on the six faces, node by node.
+    lx, ly, lz, n = 0.3, 0.2, 0.25, (3, 2, 3)
+    X, index = O.box_mesh(lx, ly, lz, *n)
+    X = O.box_jitter(X, *n, lx, ly, lz, 0.2, 99)
+    rs = O.rest_state(X, index, RHO)
+    A = np.array([[2.0, 0.7, -0.4], [0.7, -1.1, 0.3], [-0.4, 0.3, 0.6]]) * 1e-4
+    sig = 2 * MU * A + LAM * np.trace(A) * np.eye(3)
+    Q = rot([1, -1, 2], 0.9) if rotated else np.eye(3)
+    u = X @ (Q @ (np.eye(3) + A)).T - X
+    f = assemble(X, index, rs, u)
+    expect = np.zeros_like(X)
+    for axis, L in enumerate((lx, ly, lz)):
+        for value, sign in ((0.0, -1.0), (L, 1.0)):
+            nrm = np.zeros(3)
+            nrm[axis] = sign
+            O.face_traction(X, index, axis, value, 1e-12, sig @ nrm, expect)
+    expect = -(expect @ Q.T)  # internal force balances the external traction
+    scale = np.abs(expect).max()
+    assert np.abs(f - expect).max() < 1e-9 * scale
+    # and the closed form itself is not vacuous: a wrong Lame pair misses it
+    f_bad = assemble(X, index, rs, X @ (np.eye(3) + 1.01 * A).T - X)
+    assert np.abs(f_bad - expect @ Q).max() > 1e-3 * scale
+
+
+# ---------------------------------------------------------------- J2 point law
+def test_hydrostatic_strain_never_yields():
+    for e in (1e-4, 1e-3, 1e-2):
+        r = O.j2_update(E, NU, 355e6, 1e9, np.array([e, e, e, 0, 0, 0]))
+        assert not r["yielded"] and r["eqps"] == 0.0
+        assert np.allclose(r["stress"][:3], 3 * K_BULK * e, rtol=1e-13)
+        assert np.abs(r["stress"][3:]).max() == 0.0
+
+
+@pytest.mark.parametrize("H", [0.0, 1e9])
+def test_pure_shear_follows_the_closed_form(H):
+    # tau = mu gamma up to sigma_y/sqrt3; then sqrt3 tau = sigma_y + H gamma_p/sqrt3
+    sy = 355e6
+    gy = sy / (math.sqrt(3) * MU)
+    ep, q = np.zeros(6), 0.0
+    for g in np.linspace(0, 5 * gy, 41)[1:]:
+        r = O.j2_update(E, NU, sy, H, np.array([0, 0, 0, g, 0, 0]), ep, q)
+        ep, q = r["eps_p"], r["eqps"]
+        gp = max(0.0, (math.sqrt(3) * MU * g - sy) / (math.sqrt(3) * MU + H / math.sqrt(3)))
+        assert r["stress"][3] == pytest.approx(MU * (g - gp), rel=1e-11)
+        assert q == pytest.approx(gp / math.sqrt(3), rel=1e-10, abs=1e-18)
+        assert ep[3] == pytest.approx(gp, rel=1e-10, abs=1e-18)
+    assert r["stress"][3] == pytest.approx((sy + H * q) / math.sqrt(3), rel=1e-11)
+
+
+def test_uniaxial_strain_follows_the_closed_form_at_any_step_size():
+    # eps = (e, 0, 0): plastic flow (1, -1/2, -1/2) eqps, q = 2 mu (e - 3/2 eqps)
+    sy, H = 355e6, 1e9
+    e_end = 6 * sy / (2 * MU)
+    paths = {n: np.linspace(0, e_end, n + 1)[1:] for n in (1, 7, 60)}
+    final = {}
+    for n, path in paths.items():
+        ep, q = np.zeros(6), 0.0
+        for e in path:
+            r = O.j2_update(E, NU, sy, H, np.array([e, 0, 0, 0, 0, 0]), ep, q)
+            ep, q = r["eps_p"], r["eqps"]
+            qq = (2 * MU * e - sy) / (3 * MU + H)
+            if qq > 0:
+                qv = sy + H * qq
+                assert q == pytest.approx(qq, rel=1e-10)
+                assert r["stress"][0] == pytest.approx(K_BULK * e + 2 * qv / 3, rel=1e-11)
+                assert r["stress"][1] == pytest.approx(K_BULK * e - qv / 3, rel=1e-11)
+                assert np.allclose(ep[:3], qq * np.array([1, -0.5, -0.5]), rtol=1e-10)
+        final[n] = (r["stress"].copy(), q)
+    # radial path: one step equals many (the return is exact for linear hardening)
+    for n in (7, 60):
+        assert np.allclose(final[n][0], final[1][0], rtol=1e-11)
+        assert final[n][1] == pytest.approx(final[1][1], rel=1e-11)
+
+
+def test_plastic_flow_is_deviatoric_dissipative_and_idempotent():
+    rng = np.random.default_rng(5)
+    sy, H = 300e6, 2e9
+    ep, q, strain = np.zeros(6), 0.0, np.zeros(6)
+    total_d = 0.0
+    for _ in range(60):
+        strain = strain + rng.normal(0, 4e-4, 6)
+        r = O.j2_update(E, NU, sy, H, strain, ep, q)
+        assert r["eqps"] >= q and r["dissipation"] >= 0.0
+        ep, q = r["eps_p"], r["eqps"]
+        total_d += r["dissipation"]
+        assert abs(ep[0] + ep[1] + ep[2]) < 1e-18
+        # a second return at the same strain finds the point on the surface
+        again = O.j2_update(E, NU, sy, H, strain, ep.copy(), q)
+        assert again["eqps"] - q <= 1e-14 * max(q, 1e-12)
+        assert np.allclose(again["stress"], r["stress"], rtol=1e-12, atol=1e-6)
+        # the returned deviator lies on the yield surface sqrt(3/2)|s| = sy + H eqps
+        s = r["stress"].copy()
+        p = s[:3].mean()
+        s[:3] -= p
+        seq = math.sqrt(1.5 * (s[0] ** 2 + s[1] ** 2 + s[2] ** 2 + 2 * (s[3:] ** 2).sum()))
+        assert seq <= (sy + H * q) * (1 + 1e-12)
+        if r["yielded"]:
+            assert seq == pytest.approx(sy + H * q, rel=1e-12)
+    assert q > 0 and total_d > 0
+
+
+# ---------------------------------------------------------------- stable step
+def stiffness_by_differences(X, dmi, V):
+    K = np.zeros((12, 12))
+    h = 1e-9
+    x0 = X.ravel()
+    for j in range(12):
+        d = np.zeros(12)
+        d[j] = h
+        K[:, j] = -(O.tet_force(x0 + d, dmi, V, LAM, MU) - O.tet_force(x0 - d, dmi, V, LAM, MU)) / (2 * h)
+    return 0.5 * (K + K.T)
+
+
+def test_element_frequency_matches_a_brute_force_eigen_solve():
+    for seed in (1, 2, 3):
+        X, dmi, V = one_tet(seed)
+        K = stiffness_by_differences(X, dmi, V)
+        m = RHO * V / 4
+        w_brute = math.sqrt(np.linalg.eigvalsh(K / m).max())
+        w = O.max_element_omega(dmi[None, :], E, NU, RHO)
+        assert w == pytest.approx(w_brute, rel=1e-6)
+
+
+def test_leapfrog_is_stable_just_below_the_bound_and_not_above():
+    # one free tet: the element eigenvalue is the global one, so 2/omega is exact
+    X, dmi, V = one_tet(4)
+    index = np.array([[0, 1, 2, 3]], np.uint32)
+    w = O.max_element_omega(dmi[None, :], E, NU, RHO)
+    rng = np.random.default_rng(8)
+    u0 = rng.uniform(-1e-8, 1e-8, X.shape)
+    grow = {}
+    for f, steps in ((0.99, 2000), (1.03, 30)):
+        c = O.Case(X, index, E=E, nu=NU, rho=RHO, dt=f * 2 / w)
+        c.u[:] = u0
+        c.run(steps)
+        grow[f] = np.abs(c.u - c.u.mean(axis=0)).max() / np.abs(u0).max()
+    # above the bound the top mode grows by |lambda| ~ 1.6 per step
+    assert grow[0.99] < 10 and 1e3 < grow[1.03] < 1e30
+
+
+# ---------------------------------------------------------------- solver
+def cantilever_case(**kw):
+    X, index = O.box_mesh(1.0, 0.1, 0.1, 40, 4, 4)
+    flags = np.where(X[:, 0] < 1e-9, O.PIN_ALL, 0).astype(np.uint32)
+    load = O.face_traction(X, index, 0, 1.0, 1e-9, (0.0, 0.0, -1000.0 / 0.01))
+    return X, index, flags, load
+
+
+def test_rest_stays_at_rest_and_a_free_body_translates():
+    X, index = O.box_mesh(0.2, 0.1, 0.05, 4, 2, 2)
+    c = O.Case(X, index, dt=1e-7)
+    c.run(50)
+    # F = Ds Dm^-1 is the identity to FP64 rounding, so rest moves by rounding only
+    assert np.abs(c.u).max() < 1e-14 * 0.2 and np.abs(c.v).max() < 1e-14 * 0.2 / 1e-7
+    c2 = O.Case(X, index, dt=1e-7)
+    v0 = np.array([1.5, -2.0, 0.25])
+    c2.v[:] = v0
+    c2.run(200)
+    # translation adds no force beyond the same rounding-level noise as rest
+    assert np.abs(c2.v - v0).max() < 1e-9 * np.abs(v0).max()
+    assert np.allclose(c2.u, v0 * 200 * 1e-7, rtol=1e-9, atol=0)
+
+
+def test_energy_account_closes_for_an_undamped_elastic_run():
+    # central difference conserves K = 1/2 m v(n-1/2).v(n+1/2) + U against the midpoint
+    # work of the ramped load; in FP64 the account closes to rounding. A dropped half
+    # step of the work (P dt/2, ~W/n) or K taken from one half-step velocity would
+    # leave a residual near 1e-3 of the work, six orders above this bound.
+    X, index, flags, load = cantilever_case()
+    w = O.max_element_omega(O.rest_state(X, index, RHO)["rest_inv"], E, NU, RHO)
+    c = O.Case(X, index, dt=0.9 * 2 / w, flags=flags, load=load, load_ramp=1e-3)
+    rec = c.run(2000, sample_every=100)
+    r0 = rec[0]
+    for r in rec[1:]:
+        resid = r["w_ext"] - (r["strain"] - r0["strain"]) - (r["kinetic"] - r0["kinetic"])
+        assert r["w_ext"] > 0 and abs(resid) < 1e-9 * r["w_ext"]
+        assert r["strain"] > 0.1 * r["w_ext"] and r["kinetic"] > 0.1 * r["w_ext"]
+
+
+def test_prescribed_drive_reaches_its_target_and_its_work_closes_the_account():
+    # a bar stretched by a smoothstep drive on its x = L face, with the x = 0 face on
+    # rollers and three pins that admit the uniform uniaxial-stress field exactly
+    lx, ly, lz, d = 0.4, 0.05, 0.05, 1e-5
+    X, index = O.box_mesh(lx, ly, lz, 16, 2, 2)
+    at = lambda x, y, z: (np.abs(X - [x, y, z]) < 1e-9).all(axis=1)
+    fl = np.zeros(len(X), np.uint32)
+    fl[X[:, 0] < 1e-9] |= O.PIN_X
+    fl[at(0, 0, 0)] |= O.PIN_Y | O.PIN_Z
+    fl[at(0, ly, 0)] |= O.PIN_Z
+    fl[at(0, 0, lz)] |= O.PIN_Y
+    far = X[:, 0] > lx - 1e-9
+    fl[far] |= O.DRIVE_X
+    pres = np.zeros_like(X)
+    pres[far, 0] = d
+    w = O.max_element_omega(O.rest_state(X, index, RHO)["rest_inv"], E, NU, RHO)
+    dt = 0.9 * 2 / w
+    ramp = 20 * lx / math.sqrt(E / RHO)
+    c = O.Case(X, index, dt=dt, flags=fl, prescribed=pres, drive_ramp=ramp, damping=0.999)
+    n = int(3 * ramp / dt)
+    rec = c.run(n, sample_every=n // 10)
+    assert np.allclose(c.u[far, 0], d, rtol=1e-12, atol=0)
+    r0 = rec[0]
+    for r in rec[1:]:
+        resid = (r["w_boundary"] - (r["strain"] - r0["strain"]) - (r["kinetic"] - r0["kinetic"])
+                 - r["damping_loss"])
+        assert r["w_boundary"] > 0 and abs(resid) < 1e-9 * r["w_boundary"]
+    assert rec[-1]["damping_loss"] > 1e-4 * rec[-1]["w_boundary"]  # the damping term is live
+    # settled: the stored energy is the bar's 1/2 E A d^2 / L, the section contracts by nu
+    assert rec[-1]["strain"] == pytest.approx(0.5 * E * ly * lz * d * d / lx, rel=1e-4)
+    corner = np.argmax(X.sum(axis=1))
+    assert c.u[corner, 1] == pytest.approx(-NU * d / lx * ly, rel=2e-2)
+
+
+def contact_block(kappa=0.1, pinned=False):
+    X, index = O.box_mesh(0.08, 0.08, 0.04, 8, 8, 4)
+    flags = np.where(X[:, 2] < 1e-9, O.CONTACT, 0).astype(np.uint32)
+    if pinned:
+        flags |= np.where(X[:, 2] > 0.04 - 1e-9, O.PIN_ALL, 0).astype(np.uint32)
+    R = 0.1
+    ind = dict(radius=R, center=(0.04, 0.04, -R), velocity=(0.0, 0.0, 2.0), kappa=kappa)
+    w = O.max_element_omega(O.rest_state(X, index, RHO)["rest_inv"], E, NU, RHO)
+    return O.Case(X, index, dt=0.9 * 2 / w, flags=flags, indenter=ind), X
+
+
+def test_contact_impulse_equals_momentum_and_its_work_closes_the_account():
+    c, X = contact_block()
+    rec = c.run(600, sample_every=1)
+    dt = c.s.dt
+    fz = np.array([r["contact_force"][2] for r in rec])
+    assert fz.max() > 0 and np.all(fz >= 0)  # the sphere pushes +z, never pulls
+    p = (c.mass[:, None] * c.v).sum(axis=0)
+    impulse = dt * np.array([sum(r["contact_force"][a] for r in rec) for a in range(3)])
+    assert np.allclose(p, impulse, rtol=1e-9, atol=1e-12 * abs(impulse[2]))
+    # the contact work goes into the block's K + U; what is left is the penalty
+    # spring's own stored energy while the sphere is in contact
+    r0 = rec[0]
+    for r in rec[1:]:
+        resid = r["w_contact"] - (r["strain"] - r0["strain"]) - (r["kinetic"] - r0["kinetic"])
+        assert abs(resid) <= 1e-4 * abs(r["w_contact"]) + 1e-15
+    assert rec[-1]["w_contact"] > 0
+    # the penalty acts along the sphere normal: over a patch centred on the axis the
+    # lateral components nearly cancel (the Kuhn mesh is not exactly symmetric)
+    f = np.array(rec[int(np.argmax(fz))]["contact_force"])
+    assert np.abs(f[:2]).max() < 2e-2 * f[2]
+
+
+# ---------------------------------------------------------------- meshes and seeds
+def test_uniform_is_splitmix64():
+    # published first outputs of SplitMix64 from state 0 (Vigna), top 53 bits
+    known = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    for k, z in enumerate(known):
+        assert O.uniform(0, k) == (z >> 11) * 2.0 ** -53
+
+
+def test_jitter_keeps_faces_planar_and_stays_in_bounds():
+    n, L = (6, 5, 4), (0.6, 0.5, 0.2)
+    X0, index = O.box_mesh(*L, *n)
+    X = O.box_jitter(X0, *n, *L, 0.3, 123)
+    h = np.array(L) / np.array(n)
+    assert np.all(np.abs(X - X0) <= 0.3 * h + 1e-15)
+    for a in range(3):
+        on = (np.abs(X0[:, a]) < 1e-12) | (np.abs(X0[:, a] - L[a]) < 1e-12)
+        assert np.array_equal(X[on, a], X0[on, a])
+    assert (O.rest_state(X, index, RHO)["volume"] > 0).all()
+    assert not np.array_equal(X, O.box_jitter(X0, *n, *L, 0.3, 124))
+
+
+def boundary_faces(index):
+    faces = {}
+    for t, tet in enumerate(index):
+        for skip in range(4):
+            f = tuple(sorted(int(v) for k, v in enumerate(tet) if k != skip))
+            faces.setdefault(f, []).append(t)
+    return faces
+
+
+def test_panel_mesh_is_conforming_with_closed_form_volume_and_surface():
+    spec = dict(lx=0.6, ly=0.3, t_plate=0.012, n_stiff=2, web_h=0.05, web_t=0.012, nx=12,
+                ny_bay=3, nz_web=4)
+    X, index = O.panel_mesh(**spec)
+    vol = O.rest_state(X, index, RHO)["volume"]
+    assert (vol > 0).all()
+    lx, ly, t, ns, h, wt = (spec[k] for k in ("lx", "ly", "t_plate", "n_stiff", "web_h", "web_t"))
+    assert vol.sum() == pytest.approx(lx * ly * t + ns * lx * wt * h, rel=1e-12)
+    faces = boundary_faces(index)
+    assert max(len(v) for v in faces.values()) == 2  # no face shared by three tets
+    area = 0.0
+    for f, owners in faces.items():
+        if len(owners) == 1:
+            p = X[list(f)]
+            area += 0.5 * np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+    closed = 2 * (lx * ly + lx * t + ly * t) + ns * (2 * lx * h + 2 * wt * h)
+    assert area == pytest.approx(closed, rel=1e-12)
+    # each web sits on the plate over one cell column of width web_t, centred on its line
+    top = X[:, 2] > t + 1e-9
+    ys = np.unique(np.round(X[top, 1], 12))
+    centres = [ly * (s + 1) / (ns + 1) for s in range(ns)]
+    assert np.allclose(ys, sorted([c - wt / 2 for c in centres] + [c + wt / 2 for c in centres]))
+
+
+def test_threaded_oracle_is_bit_identical_in_state():
+    # the split changes only the order of the energy sums
+    X, index = O.panel_mesh(0.6, 0.3, 0.012, 2, 0.05, 0.012, 24, 3, 3)
+    flags = np.where(X[:, 2] < 1e-9, O.CONTACT, 0).astype(np.uint32)
+    flags |= np.where((X[:, 0] < 1e-9) | (X[:, 0] > 0.6 - 1e-9), O.PIN_ALL, 0).astype(np.uint32)
+    ind = dict(radius=0.05, center=(0.3, 0.15, -0.05), velocity=(0.0, 0.0, 10.0), kappa=0.1)
+    w = O.max_element_omega(O.rest_state(X, index, RHO)["rest_inv"], E, NU, RHO)
+    runs = []
+    for threads in (1, 3):
+        c = O.Case(X, index, dt=0.9 * 2 / w, flags=flags, plastic=True, indenter=ind)
+        rec = c.run(300, sample_every=50, threads=threads)
+        runs.append((c.u.copy(), c.v.copy(), c.eqps.copy(), rec))
+    (u1, v1, q1, r1), (u3, v3, q3, r3) = runs
+    assert np.array_equal(u1, u3) and np.array_equal(v1, v3) and np.array_equal(q1, q3)
+    assert q1.max() > 0
+    for a, b in zip(r1, r3):
+        for k in O.ENERGY_FIELDS:
+            assert a[k] == pytest.approx(b[k], rel=1e-12, abs=1e-12 * abs(r1[-1]["w_contact"]))
