@@ -66,6 +66,16 @@ typedef struct shipfem_material {
     int32_t plastic;          /* 0: co-rotational linear elastic; 1: + J2 */
     double yield_strength;    /* Pa, sigma_y(0) */
     double hardening_modulus; /* Pa, H of sigma_y = sigma_y0 + H eps_p */
+    /* Ductile failure and element deletion (plastic only; engine/sim/plasticity.cpp:457-477,
+     * plasticity.hpp:160-210). failure_strain <= 0 disables it. Damage grows by
+     * d eqps / (failure_strain * exp(-sensitivity (eta - reference_triaxiality))), with eta
+     * the triaxiality of the returned stress and none at or below cutoff_triaxiality; a tet
+     * whose damage reaches 1 tears: it carries no stress from then on. The reference's
+     * constants are 1.5, 1/3 and -1/3; see shipfem_regularised_failure_strain. */
+    double failure_strain;
+    double triaxiality_sensitivity;
+    double reference_triaxiality;
+    double cutoff_triaxiality;
 } shipfem_material;
 
 /* Rigid sphere driven at constant velocity, node-to-surface penalty contact:
@@ -118,6 +128,7 @@ typedef struct shipfem_energy_record {
     int64_t step;
     double time;
     double strain, kinetic, dissipation, w_ext, w_contact, w_boundary, damping_loss;
+    double fracture;         /* stored energy released by torn tets, 1/2 V sigma:(eps - eps_p) */
     double contact_force[3]; /* sum of penalty forces on the structure at step n */
 } shipfem_energy_record;
 
@@ -128,6 +139,7 @@ typedef struct shipfem_state_view {
     float* position;       /* [3N] X + u in float (the reference's readback, fem_gpu.cpp:424-428) */
     float* eqps;           /* [T] equivalent plastic strain */
     float* plastic_strain; /* [6T] Voigt, engineering shears, co-rotated frame */
+    float* damage;         /* [T] ductile damage, 1 = torn (0 without failure) */
     int64_t step;          /* out */
     double time;           /* out */
     uint32_t status;       /* out: nonzero when the case went non-finite */
@@ -239,9 +251,40 @@ shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* 
 shipfem_status shipfem_solver_footprint(const shipfem_solver* s, uint64_t* bytes);
 void shipfem_solver_destroy(shipfem_solver* s);
 
-/* Bytes a case of this shape will hold on the device (admission formula). */
+/* One torn tet: the step whose element pass tore it and the failure plane, the
+ * maximum principal direction of its world-frame stress at the tear with the largest
+ * component positive (the reference's Increment::failureNormal, plasticity.cpp:472,
+ * taken in the world frame). */
+typedef struct shipfem_tear {
+    uint32_t tet;
+    int64_t step;
+    double normal[3];
+} shipfem_tear;
+
+/* The tears so far, ordered by step then tet; *count is their total (it may exceed
+ * max). Empty for a case without failure. */
+shipfem_status shipfem_solver_tears(shipfem_solver* s, shipfem_tear* out, uint32_t max,
+                                    uint32_t* count);
+
+/* Bytes a case of this shape will hold on the device (admission formula).
+ * plastic: 0 elastic, 1 J2, 2 J2 with failure. */
 uint64_t shipfem_case_footprint(uint32_t node_count, uint32_t tet_count, int32_t plastic,
                                 int32_t sample_capacity);
+
+/* The mesh-regularised failure strain (plasticity.cpp:158-169): an element of in-plane
+ * size `length` in a plate of `thickness` reads the neck diluted by t/l,
+ * uniform + (fracture - uniform) * min(thickness / length, 1). */
+double shipfem_regularised_failure_strain(double uniform, double fracture, double length,
+                                          double thickness);
+
+/* The groups (panels) whose torn volume reaches `fraction` of the group's volume
+ * (Solver::tornPanelsAt, zone.cpp:1262-1275, by volume for tets), ascending by id.
+ * tet_group[T] labels each tet; damage[T] is a readback (1 = torn). *count is the
+ * number of such groups (it may exceed max). */
+shipfem_status shipfem_torn_groups(const double* rest_position, const uint32_t* index,
+                                   uint32_t tet_count, const uint32_t* tet_group,
+                                   const float* damage, double fraction, uint32_t* out,
+                                   uint32_t max, uint32_t* count);
 
 /* ---- many cases in one launch per phase ----------------------------------- */
 typedef struct shipfem_batch shipfem_batch;
@@ -262,6 +305,8 @@ shipfem_status shipfem_batch_readback(shipfem_batch* b, uint32_t case_index,
 shipfem_status shipfem_batch_energy(shipfem_batch* b, uint32_t case_index,
                                     shipfem_energy_record* out, int32_t max_records,
                                     int32_t* count);
+shipfem_status shipfem_batch_tears(shipfem_batch* b, uint32_t case_index, shipfem_tear* out,
+                                   uint32_t max, uint32_t* count);
 /* Restart every case from its initial state (for repeated timing). */
 shipfem_status shipfem_batch_reset(shipfem_batch* b);
 void shipfem_batch_destroy(shipfem_batch* b);
@@ -286,6 +331,8 @@ typedef struct shipfem_result {
     double max_displacement;  /* max |u| over nodes, m */
     double max_eqps;
     double peak_contact_force; /* max over records of |sum contact force|, N */
+    uint32_t torn_elements;    /* tets torn by the end (failure cases) */
+    double torn_volume;        /* their rest volume, m^3 */
     shipfem_energy_record last; /* energy at the last record */
 } shipfem_result;
 

@@ -456,8 +456,14 @@ void ora_face_traction(const double* X, const uint32_t* index, int64_t T, int ax
 
 /* Stress from the co-rotated small strain (tensor eps, col-major), returning the
  * symmetric stress tensor and, for J2, updating the point's plastic state. */
+static int j2_return(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
+                     double* damage, double stress[6], double* dissipation, double normal[3],
+                     double* eta);
+
 static void stress_of(const double* eps, double lam, double mu, const ora_j2* j2, double* ep,
-                      double* eqps, double* sig, double* diss, double* energy_density) {
+                      double* eqps, double* damage, double* sig, double* diss,
+                      double* energy_density, int* torn, double stress_torn[6], double* released) {
+    if (torn) *torn = 0;
     if (!j2) {
         const double tr = AT(eps, 0, 0) + AT(eps, 1, 1) + AT(eps, 2, 2);
         for (int k = 0; k < 9; ++k) sig[k] = 2.0 * mu * eps[k];
@@ -473,8 +479,19 @@ static void stress_of(const double* eps, double lam, double mu, const ora_j2* j2
     const double strain[6] = {AT(eps, 0, 0), AT(eps, 1, 1), AT(eps, 2, 2),
                               2 * AT(eps, 0, 1), 2 * AT(eps, 1, 2), 2 * AT(eps, 0, 2)};
     double s6[6], d = 0;
-    ora_j2_update(j2, strain, ep, eqps, s6, &d);
+    const int r = j2_return(j2, strain, ep, eqps, damage, s6, &d, NULL, NULL);
     if (diss) *diss = d;
+    if (r & 2) {
+        /* torn in this step: its stored energy leaves the account as fracture; the
+         * returned stress (co-rotated frame) goes out for the failure plane and, kept
+         * here, gives the forces the caller needs to finish the release and zeroes */
+        double u = 0;
+        for (int i = 0; i < 6; ++i) u += s6[i] * (strain[i] - ep[i]);
+        if (released) *released = 0.5 * u;
+        if (torn) *torn = 1;
+        if (stress_torn)
+            for (int i = 0; i < 6; ++i) stress_torn[i] = s6[i];
+    }
     AT(sig, 0, 0) = s6[0];
     AT(sig, 1, 1) = s6[1];
     AT(sig, 2, 2) = s6[2];
@@ -482,18 +499,20 @@ static void stress_of(const double* eps, double lam, double mu, const ora_j2* j2
     AT(sig, 1, 2) = AT(sig, 2, 1) = s6[4];
     AT(sig, 0, 2) = AT(sig, 2, 0) = s6[5];
     if (energy_density) {
-        /* 1/2 sigma : (eps - eps_p); engineering shears pair with tensor stress */
+        /* 1/2 sigma : (eps - eps_p); engineering shears pair with tensor stress; a
+         * torn point holds none */
         double s = 0;
         for (int i = 0; i < 6; ++i) s += s6[i] * (strain[i] - ep[i]);
-        *energy_density = 0.5 * s;
+        *energy_density = (r & 2) ? 0.0 : 0.5 * s;
     }
 }
 
 /* Force of one tet from current node positions x (4 x xyz). Mirrors
  * fem.cpp:73-117 in double, with the stress law pluggable. */
 static void tet_force_general(const double x[12], const double* dmi, double volume, double lam,
-                              double mu, const ora_j2* j2, double* ep, double* eqps, double out[12],
-                              double* diss, double* energy) {
+                              double mu, const ora_j2* j2, double* ep, double* eqps,
+                              double* damage, double out[12], double* diss, double* energy,
+                              int* torn, double normal_world[3], double* released) {
     double ds[9], f[9], r[9], rt[9], s[9], eps[9], sig[9], piola[9], dmit[9], h[9];
     for (int c = 0; c < 3; ++c)
         for (int a = 0; a < 3; ++a) AT(ds, a, c) = x[(c + 1) * 3 + a] - x[a];
@@ -504,10 +523,26 @@ static void tet_force_general(const double x[12], const double* dmi, double volu
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) AT(eps, i, j) = 0.5 * (AT(s, i, j) + AT(s, j, i));
     for (int i = 0; i < 3; ++i) AT(eps, i, i) -= 1.0;
-    double ed = 0;
-    stress_of(eps, lam, mu, j2, ep, eqps, sig, diss, energy ? &ed : NULL);
+    double ed = 0, rel = 0, sloc[6] = {0, 0, 0, 0, 0, 0};
+    int t = 0;
+    stress_of(eps, lam, mu, j2, ep, eqps, damage, sig, diss, energy ? &ed : NULL, &t, sloc, &rel);
     if (diss) *diss *= volume;
     if (energy) *energy = volume * ed;
+    if (torn) *torn = t;
+    if (t) {
+        /* failure plane: maximum principal direction of the world-frame stress
+         * R sigma R^T at the tear, largest component positive */
+        if (released) *released = volume * rel;
+        if (normal_world) {
+            double sl[9] = {sloc[0], sloc[3], sloc[5], sloc[3], sloc[1], sloc[4], sloc[5], sloc[4],
+                            sloc[2]};
+            double tmp[9], rt2[9], sw[9];
+            mul3(r, sl, tmp);
+            transpose3(r, rt2);
+            mul3(tmp, rt2, sw);
+            ora_world_stress_normal(sw, normal_world);
+        }
+    }
     mul3(r, sig, piola);
     transpose3(dmi, dmit);
     mul3(piola, dmit, h);
@@ -523,11 +558,85 @@ static void tet_force_general(const double x[12], const double* dmi, double volu
 
 void ora_tet_force(const double x[12], const double rest_inv[9], double volume, double lambda,
                    double mu, double out[12]) {
-    tet_force_general(x, rest_inv, volume, lambda, mu, NULL, NULL, NULL, out, NULL, NULL);
+    tet_force_general(x, rest_inv, volume, lambda, mu, NULL, NULL, NULL, NULL, out, NULL, NULL,
+                      NULL, NULL, NULL);
 }
 
-int ora_j2_update(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
-                  double stress[6], double* dissipation) {
+double ora_regularised_failure_strain(double uniform, double fracture, double length,
+                                      double thickness) {
+    /* an element no larger than the plate is thick resolves the neck itself */
+    double share = 1.0;
+    if (length > 0.0 && thickness > 0.0) share = thickness / length < 1.0 ? thickness / length : 1.0;
+    return uniform + (fracture - uniform) * share;
+}
+
+double ora_triaxiality_factor(double sensitivity, double reference, double cutoff, double eta) {
+    if (eta <= cutoff) return INFINITY;
+    return exp(-sensitivity * (eta - reference));
+}
+
+/* Maximum principal direction of a symmetric stress (Voigt, tensor shears) by
+ * cyclic Jacobi, sign fixed so the largest component is positive. */
+static void max_principal_direction(const double s6[6], double out[3]) {
+    double a[9] = {s6[0], s6[3], s6[5], s6[3], s6[1], s6[4], s6[5], s6[4], s6[2]};
+    double v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
+        const double diag = a[0] * a[0] + a[4] * a[4] + a[8] * a[8];
+        if (off <= 1e-30 * diag || off == 0.0) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                const double apq = a[p * 3 + q];
+                if (apq == 0.0) continue;
+                const double theta = (a[q * 3 + q] - a[p * 3 + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = a[k * 3 + p], akq = a[k * 3 + q];
+                    a[k * 3 + p] = c * akp - sn * akq;
+                    a[k * 3 + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = a[p * 3 + k], aqk = a[q * 3 + k];
+                    a[p * 3 + k] = c * apk - sn * aqk;
+                    a[q * 3 + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double vkp = v[k * 3 + p], vkq = v[k * 3 + q];
+                    v[k * 3 + p] = c * vkp - sn * vkq;
+                    v[k * 3 + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 3; ++i)
+        if (a[i * 3 + i] > a[best * 3 + best]) best = i;
+    for (int i = 0; i < 3; ++i) out[i] = v[i * 3 + best];
+    double norm = sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+    if (!(norm > 0.0)) {
+        out[0] = 1.0;
+        out[1] = out[2] = 0.0;
+        return;
+    }
+    int largest = 0;
+    for (int i = 1; i < 3; ++i)
+        if (fabs(out[i]) > fabs(out[largest])) largest = i;
+    if (out[largest] < 0.0) norm = -norm;
+    for (int i = 0; i < 3; ++i) out[i] /= norm;
+}
+
+/* The return with damage; at failure `stress` keeps the returned (pre-failure)
+ * stress and bit 1 is set: the caller zeroes it. */
+static int j2_return(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
+                     double* damage, double stress[6], double* dissipation, double normal[3],
+                     double* eta) {
+    if (eta) *eta = 0.0;
+    if (dissipation) *dissipation = 0.0;
+    /* a failed point carries no stress and its history is frozen (plasticity.cpp:326) */
+    if (damage && *damage >= 1.0) {
+        for (int i = 0; i < 6; ++i) stress[i] = 0.0;
+        return 0;
+    }
     const double mu = m->E / (2.0 * (1.0 + m->nu));
     const double kappa = m->E / (3.0 * (1.0 - 2.0 * m->nu));
     const double root23 = sqrt(2.0 / 3.0);
@@ -549,7 +658,6 @@ int ora_j2_update(const ora_j2* m, const double strain[6], double eps_p[6], doub
             stress[i] = p + tr[i];
             stress[3 + i] = tr[3 + i];
         }
-        if (dissipation) *dissipation = 0.0;
         return 0;
     }
     /* linear hardening makes the consistency condition affine in dgamma: the
@@ -567,8 +675,41 @@ int ora_j2_update(const ora_j2* m, const double strain[6], double eps_p[6], doub
     for (int i = 0; i < 6; ++i) d += stress[i] * dep[i];
     if (dissipation) *dissipation = d;
     for (int i = 0; i < 6; ++i) eps_p[i] += dep[i];
-    *eqps += root23 * gamma;
+    const double deq = root23 * gamma;
+    *eqps += deq;
+    /* triaxiality of the returned stress: mean over von Mises, sqrt(3/2)|s| */
+    const double q = sqrt(1.5) * (norm - 2.0 * mu * gamma);
+    const double e = q > 0.0 ? p / q : 0.0;
+    if (eta) *eta = e;
+    if (!damage || !(m->failure_strain > 0.0)) return 1;
+    const double critical = m->failure_strain * ora_triaxiality_factor(
+        m->sensitivity, m->reference_triaxiality, m->cutoff_triaxiality, e);
+    if (critical > 0.0 && isfinite(critical)) *damage += deq / critical;
+    if (*damage >= 1.0) {
+        *damage = 1.0;
+        if (normal) max_principal_direction(stress, normal);
+        return 3;
+    }
     return 1;
+}
+
+void ora_world_stress_normal(const double s9[9], double normal[3]) {
+    const double v[6] = {s9[0], s9[4], s9[8], s9[1], s9[5], s9[2]};
+    max_principal_direction(v, normal);
+}
+
+int ora_j2_update_damage(const ora_j2* m, const double strain[6], double eps_p[6],
+                         double* eqps, double* damage, double stress[6], double* dissipation,
+                         double normal[3], double* eta) {
+    const int r = j2_return(m, strain, eps_p, eqps, damage, stress, dissipation, normal, eta);
+    if (r & 2)
+        for (int i = 0; i < 6; ++i) stress[i] = 0.0;
+    return r;
+}
+
+int ora_j2_update(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
+                  double stress[6], double* dissipation) {
+    return ora_j2_update_damage(m, strain, eps_p, eqps, NULL, stress, dissipation, NULL, NULL) & 1;
 }
 
 /* ======================================================================== */
@@ -612,30 +753,48 @@ static double drive_fraction(const ora_case* c, double t) {
 
 /* Per-step sums of one range of elements or nodes. */
 typedef struct {
-    double strain, diss;                              /* element range */
+    double strain, diss, fracture;                    /* element range */
     double w_ext, w_con, w_bnd, damp;                 /* node range, running terms */
     double kinetic, half_ext, half_con, half_bnd, half_damp, fcon[3]; /* sample terms */
 } ora_sums;
 
 /* element pass at u_n over tets [e0, e1) */
-static void run_elements(const ora_case* c, ora_state* st, int64_t e0, int64_t e1, int sample,
-                         ora_sums* out) {
+static void run_elements(const ora_case* c, ora_state* st, int64_t e0, int64_t e1, int64_t n,
+                         int sample, ora_sums* out) {
     const double lam = c->E * c->nu / ((1 + c->nu) * (1 - 2 * c->nu));
     const double mu = c->E / (2 * (1 + c->nu));
-    const ora_j2 j2 = {c->E, c->nu, c->yield, c->hardening};
+    const ora_j2 j2 = {c->E, c->nu, c->yield, c->hardening, c->failure_strain, c->sensitivity,
+                       c->reference_triaxiality, c->cutoff_triaxiality};
+    const int dmg = c->plastic && c->failure_strain > 0.0 && st->damage;
     for (int64_t e = e0; e < e1; ++e) {
         const uint32_t* idx = &c->index[e * 4];
         double x[12];
         for (int q = 0; q < 4; ++q)
             for (int a = 0; a < 3; ++a)
                 x[q * 3 + a] = c->X[idx[q] * 3 + a] + st->u[idx[q] * 3 + a];
-        double de = 0, ue = 0;
+        double de = 0, ue = 0, rel = 0, nrm[3] = {0, 0, 0};
+        int torn = 0;
         tet_force_general(x, &c->rest_inv[e * 9], c->volume[e], lam, mu,
                           c->plastic ? &j2 : NULL, c->plastic ? &st->eps_p[e * 6] : NULL,
-                          c->plastic ? &st->eqps[e] : NULL, &st->force[e * 12], &de,
-                          sample ? &ue : NULL);
+                          c->plastic ? &st->eqps[e] : NULL, dmg ? &st->damage[e] : NULL,
+                          &st->force[e * 12], &de, sample ? &ue : NULL, &torn, nrm, &rel);
         out->diss += de;
         out->strain += ue;
+        if (torn) {
+            /* The tet's force vanishes from step n on. Against the shadow kinetic
+             * energy 1/2 m v(n-1/2).v(n+1/2) the potential it carried at u_n is
+             * U_e(u_n) + 1/2 f_e(u_n).(u_n - u_(n-1)), f_e its intact nodal forces:
+             * that whole amount leaves as fracture, then the forces are zeroed. */
+            double* fe = &st->force[e * 12];
+            double half = 0;
+            for (int q = 0; q < 4; ++q)
+                for (int a = 0; a < 3; ++a) half += fe[q * 3 + a] * st->v[idx[q] * 3 + a];
+            out->fracture += rel + 0.5 * c->dt * half;
+            for (int k = 0; k < 12; ++k) fe[k] = 0.0;
+            if (st->tear_step) st->tear_step[e] = n;
+            if (st->tear_normal)
+                for (int a = 0; a < 3; ++a) st->tear_normal[e * 3 + a] = nrm[a];
+        }
     }
 }
 
@@ -723,6 +882,7 @@ static void run_nodes(const ora_case* c, ora_state* st, int64_t i0, int64_t i1, 
 static void add_sums(ora_sums* a, const ora_sums* b) {
     a->strain += b->strain;
     a->diss += b->diss;
+    a->fracture += b->fracture;
     a->w_ext += b->w_ext;
     a->w_con += b->w_con;
     a->w_bnd += b->w_bnd;
@@ -744,6 +904,7 @@ static int sample_now(int64_t n, int64_t sample_every, const ora_energy* rec, in
 static void close_step(const ora_case* c, ora_state* st, int64_t n, int sample, const ora_sums* s,
                        ora_energy* rec, int64_t* nrec) {
     st->dissipation += s->diss;
+    st->fracture += s->fracture;
     st->w_ext += s->w_ext;
     st->w_contact += s->w_con;
     st->w_boundary += s->w_bnd;
@@ -759,6 +920,7 @@ static void close_step(const ora_case* c, ora_state* st, int64_t n, int sample, 
         r->w_contact = st->w_contact - s->half_con;
         r->w_boundary = st->w_boundary - s->half_bnd;
         r->damping_loss = st->damping_loss - s->half_damp;
+        r->fracture = st->fracture;
         for (int a = 0; a < 3; ++a) r->contact_force[a] = s->fcon[a];
     }
     st->step = n + 1;
@@ -771,7 +933,7 @@ void ora_run(const ora_case* c, ora_state* st, int64_t steps, int64_t sample_eve
         const int sample = sample_now(n, sample_every, rec, max_rec, nrec);
         ora_sums s;
         memset(&s, 0, sizeof s);
-        run_elements(c, st, 0, c->T, sample, &s);
+        run_elements(c, st, 0, c->T, n, sample, &s);
         run_nodes(c, st, 0, c->N, n, sample, &s);
         close_step(c, st, n, sample, &s, rec, nrec);
     }
@@ -808,7 +970,7 @@ static void* ora_mt_worker(void* p) {
         const int sample = sample_now(n, sh->sample_every, sh->rec, sh->max_rec, sh->nrec);
         ora_sums* mine = &sh->sums[id];
         memset(mine, 0, sizeof *mine);
-        run_elements(c, sh->st, e0, e1, sample, mine);
+        run_elements(c, sh->st, e0, e1, n, sample, mine);
         pthread_barrier_wait(sh->bar);
         run_nodes(c, sh->st, i0, i1, n, sample, mine);
         pthread_barrier_wait(sh->bar);

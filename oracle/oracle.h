@@ -104,6 +104,11 @@ void ora_tet_force(const double x[12] /*4 nodes xyz*/, const double rest_inv[9],
 
 typedef struct ora_j2 {
     double E, nu, yield, hardening; /* linear isotropic hardening */
+    /* ductile failure (plasticity.hpp:160-210); failure_strain <= 0 disables it */
+    double failure_strain;          /* eps_f, already regularised for the element */
+    double sensitivity;             /* Rice-Tracey: eps_f exp(-sensitivity (eta - eta_ref)) */
+    double reference_triaxiality;   /* eta_ref (1/3, uniaxial tension) */
+    double cutoff_triaxiality;      /* at or below: no damage */
 } ora_j2;
 
 /* Radial return (plasticity.cpp:319-455, kinematic 0, no damage). strain and
@@ -111,6 +116,27 @@ typedef struct ora_j2 {
  * components. Returns 1 when the point yielded. dissipation = sigma : d eps_p. */
 int ora_j2_update(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
                   double stress[6], double* dissipation);
+
+/* The same return with ductile damage (plasticity.cpp:457-477): damage grows by
+ * d eqps / (eps_f * triaxiality factor) and the point fails at damage 1, after which it
+ * carries no stress and its history is frozen. damage >= 1 is the failed state.
+ * Returns bit 0 yielded, bit 1 failed in this call. At failure, normal (may be NULL)
+ * receives the maximum principal direction of the returned stress, largest component
+ * positive. eta (may be NULL) receives the triaxiality of the returned stress. */
+int ora_j2_update_damage(const ora_j2* m, const double strain[6], double eps_p[6],
+                         double* eqps, double* damage, double stress[6], double* dissipation,
+                         double normal[3], double* eta);
+
+/* Maximum principal direction of a symmetric 3x3 stress (column-major), largest
+ * component positive: the failure-plane normal of a torn tet in the world frame. */
+void ora_world_stress_normal(const double s9[9], double normal[3]);
+
+/* eps_u + (eps_frac - eps_u) min(t / l, 1) (plasticity.cpp:158-169). */
+double ora_regularised_failure_strain(double uniform, double fracture, double length,
+                                      double thickness);
+/* exp(-sensitivity (eta - eta_ref)), infinite at or below the cutoff
+ * (plasticity.cpp:171-174). */
+double ora_triaxiality_factor(double sensitivity, double reference, double cutoff, double eta);
 
 /* ---- FP64 explicit solver --------------------------------------------------- */
 typedef struct ora_case {
@@ -126,6 +152,8 @@ typedef struct ora_case {
     double load_ramp;  /* s(t) = min(t/load_ramp, 1); 0 => 1 */
     double drive_ramp; /* smoothstep of t/drive_ramp; 0 => 1 */
     double gravity, damping;
+    /* ductile failure when plastic and failure_strain > 0 (see ora_j2) */
+    double failure_strain, sensitivity, reference_triaxiality, cutoff_triaxiality;
     int indenter;
     double ind_radius, ind_center[3], ind_velocity[3], ind_kappa;
     double dt;
@@ -142,16 +170,21 @@ typedef struct ora_state {
     double* v;      /* 3N, half-step velocity */
     double* eps_p;  /* 6T */
     double* eqps;   /* T */
+    double* damage; /* T, may be NULL when failure is off; >= 1 means torn */
+    int64_t* tear_step;   /* T, step at which the tet tore, -1 intact (with damage) */
+    double* tear_normal;  /* 3T, world-frame failure-plane normal (with damage) */
     double* force;  /* 12T scratch */
     int64_t step;   /* n: the state holds u_n, v_{n-1/2} */
     /* energy account: running sums over the steps taken so far */
     double w_ext, w_contact, w_boundary, dissipation, damping_loss;
+    double fracture; /* stored energy released by torn tets, 1/2 V sigma:(eps - eps_p) */
 } ora_state;
 
 typedef struct ora_energy {
     int64_t step;
     double time;
     double strain, kinetic, dissipation, w_ext, w_contact, w_boundary, damping_loss;
+    double fracture;
     double contact_force[3]; /* sum of contact forces on the structure */
 } ora_energy;
 

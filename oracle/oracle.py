@@ -48,7 +48,14 @@ class PanelSpec(C.Structure):
 
 class J2(C.Structure):
     _fields_ = [("E", C.c_double), ("nu", C.c_double), ("yield_", C.c_double),
-                ("hardening", C.c_double)]
+                ("hardening", C.c_double), ("failure_strain", C.c_double),
+                ("sensitivity", C.c_double), ("reference_triaxiality", C.c_double),
+                ("cutoff_triaxiality", C.c_double)]
+
+
+# Rice-Tracey constants of the reference's Failure defaults (plasticity.hpp:193-199)
+RICE_TRACEY = dict(sensitivity=1.5, reference_triaxiality=1.0 / 3.0,
+                   cutoff_triaxiality=-1.0 / 3.0)
 
 
 class CaseStruct(C.Structure):
@@ -56,7 +63,10 @@ class CaseStruct(C.Structure):
                 ("load", D), ("prescribed", D), ("E", C.c_double), ("nu", C.c_double),
                 ("rho", C.c_double), ("plastic", C.c_int), ("yield_", C.c_double),
                 ("hardening", C.c_double), ("load_ramp", C.c_double), ("drive_ramp", C.c_double),
-                ("gravity", C.c_double), ("damping", C.c_double), ("indenter", C.c_int),
+                ("gravity", C.c_double), ("damping", C.c_double),
+                ("failure_strain", C.c_double), ("sensitivity", C.c_double),
+                ("reference_triaxiality", C.c_double), ("cutoff_triaxiality", C.c_double),
+                ("indenter", C.c_int),
                 ("ind_radius", C.c_double), ("ind_center", C.c_double * 3),
                 ("ind_velocity", C.c_double * 3), ("ind_kappa", C.c_double), ("dt", C.c_double),
                 ("rest_inv", D), ("volume", D), ("mass", D), ("adj_offset", U32),
@@ -64,20 +74,23 @@ class CaseStruct(C.Structure):
 
 
 class StateStruct(C.Structure):
-    _fields_ = [("u", D), ("v", D), ("eps_p", D), ("eqps", D), ("force", D), ("step", C.c_int64),
+    _fields_ = [("u", D), ("v", D), ("eps_p", D), ("eqps", D), ("damage", D),
+                ("tear_step", I64), ("tear_normal", D), ("force", D), ("step", C.c_int64),
                 ("w_ext", C.c_double), ("w_contact", C.c_double), ("w_boundary", C.c_double),
-                ("dissipation", C.c_double), ("damping_loss", C.c_double)]
+                ("dissipation", C.c_double), ("damping_loss", C.c_double),
+                ("fracture", C.c_double)]
 
 
 class EnergyStruct(C.Structure):
     _fields_ = [("step", C.c_int64), ("time", C.c_double), ("strain", C.c_double),
                 ("kinetic", C.c_double), ("dissipation", C.c_double), ("w_ext", C.c_double),
                 ("w_contact", C.c_double), ("w_boundary", C.c_double),
-                ("damping_loss", C.c_double), ("contact_force", C.c_double * 3)]
+                ("damping_loss", C.c_double), ("fracture", C.c_double),
+                ("contact_force", C.c_double * 3)]
 
 
 ENERGY_FIELDS = ["strain", "kinetic", "dissipation", "w_ext", "w_contact", "w_boundary",
-                 "damping_loss"]
+                 "damping_loss", "fracture"]
 
 
 def _declare(L):
@@ -97,6 +110,12 @@ def _declare(L):
     L.ora_tet_force.argtypes = [D, D, C.c_double, C.c_double, C.c_double, D]
     L.ora_j2_update.argtypes = [C.POINTER(J2), D, D, D, D, D]
     L.ora_j2_update.restype = C.c_int
+    L.ora_j2_update_damage.argtypes = [C.POINTER(J2), D, D, D, D, D, D, D, D]
+    L.ora_j2_update_damage.restype = C.c_int
+    L.ora_regularised_failure_strain.argtypes = [C.c_double] * 4
+    L.ora_regularised_failure_strain.restype = C.c_double
+    L.ora_triaxiality_factor.argtypes = [C.c_double] * 4
+    L.ora_triaxiality_factor.restype = C.c_double
     L.ora_case_init.argtypes = [C.POINTER(CaseStruct)]
     L.ora_case_init.restype = C.c_int
     L.ora_case_free.argtypes = [C.POINTER(CaseStruct)]
@@ -198,7 +217,7 @@ def tet_force(x12, dmi9, volume, lam, mu):
 
 
 def j2_update(E, nu, yield_, hardening, strain, eps_p=None, eqps=0.0):
-    m = J2(E, nu, yield_, hardening)
+    m = J2(E, nu, yield_, hardening, 0.0, 0.0, 0.0, 0.0)
     strain = np.ascontiguousarray(strain, np.float64)
     ep = np.zeros(6) if eps_p is None else np.array(eps_p, np.float64)
     q = np.array([eqps], np.float64)
@@ -207,6 +226,32 @@ def j2_update(E, nu, yield_, hardening, strain, eps_p=None, eqps=0.0):
     y = lib().ora_j2_update(C.byref(m), _p(strain, C.c_double), _p(ep, C.c_double),
                             _p(q, C.c_double), _p(stress, C.c_double), _p(diss, C.c_double))
     return dict(yielded=bool(y), stress=stress, eps_p=ep, eqps=float(q[0]), dissipation=float(diss[0]))
+
+
+def j2_update_damage(E, nu, yield_, hardening, failure_strain, strain, eps_p=None, eqps=0.0,
+                     damage=0.0, sensitivity=1.5, reference_triaxiality=1.0 / 3.0,
+                     cutoff_triaxiality=-1.0 / 3.0):
+    m = J2(E, nu, yield_, hardening, failure_strain, sensitivity, reference_triaxiality,
+           cutoff_triaxiality)
+    strain = np.ascontiguousarray(strain, np.float64)
+    ep = np.zeros(6) if eps_p is None else np.array(eps_p, np.float64)
+    q = np.array([eqps], np.float64)
+    dmg = np.array([damage], np.float64)
+    stress, diss, nrm, eta = np.zeros(6), np.zeros(1), np.zeros(3), np.zeros(1)
+    r = lib().ora_j2_update_damage(C.byref(m), _p(strain, C.c_double), _p(ep, C.c_double),
+                                   _p(q, C.c_double), _p(dmg, C.c_double), _p(stress, C.c_double),
+                                   _p(diss, C.c_double), _p(nrm, C.c_double), _p(eta, C.c_double))
+    return dict(yielded=bool(r & 1), failed_now=bool(r & 2), stress=stress, eps_p=ep,
+                eqps=float(q[0]), damage=float(dmg[0]), dissipation=float(diss[0]), normal=nrm,
+                eta=float(eta[0]))
+
+
+def regularised_failure_strain(uniform, fracture, length, thickness):
+    return lib().ora_regularised_failure_strain(uniform, fracture, length, thickness)
+
+
+def triaxiality_factor(eta, sensitivity=1.5, reference=1.0 / 3.0, cutoff=-1.0 / 3.0):
+    return lib().ora_triaxiality_factor(sensitivity, reference, cutoff, eta)
 
 
 def lame(E, nu):
@@ -219,7 +264,7 @@ class Case:
 
     def __init__(self, X, index, *, E=210e9, nu=0.3, rho=7850.0, dt, flags=None, load=None,
                  prescribed=None, plastic=False, yield_=355e6, hardening=1e9, load_ramp=0.0,
-                 drive_ramp=0.0, gravity=0.0, damping=1.0, indenter=None):
+                 drive_ramp=0.0, gravity=0.0, damping=1.0, indenter=None, failure=None):
         self.X = np.ascontiguousarray(X, np.float64)
         self.index = np.ascontiguousarray(index, np.uint32)
         self.N, self.T = len(self.X), len(self.index)
@@ -237,6 +282,13 @@ class Case:
         s.yield_, s.hardening = yield_, hardening
         s.load_ramp, s.drive_ramp = load_ramp, drive_ramp
         s.gravity, s.damping = gravity, damping
+        if failure:
+            f = dict(RICE_TRACEY)
+            f.update(failure)
+            s.failure_strain = f["failure_strain"]
+            s.sensitivity = f["sensitivity"]
+            s.reference_triaxiality = f["reference_triaxiality"]
+            s.cutoff_triaxiality = f["cutoff_triaxiality"]
         if indenter:
             s.indenter = 1
             s.ind_radius = indenter["radius"]
@@ -254,9 +306,15 @@ class Case:
         self.eps_p = np.zeros((self.T, 6))
         self.eqps = np.zeros(self.T)
         self.force = np.zeros((self.T, 12))
+        self.damage = np.zeros(self.T)
+        self.tear_step = np.full(self.T, -1, np.int64)
+        self.tear_normal = np.zeros((self.T, 3))
         st = StateStruct()
         st.u, st.v = _p(self.u, C.c_double), _p(self.v, C.c_double)
         st.eps_p, st.eqps = _p(self.eps_p, C.c_double), _p(self.eqps, C.c_double)
+        st.damage = _p(self.damage, C.c_double)
+        st.tear_step = _p(self.tear_step, C.c_int64)
+        st.tear_normal = _p(self.tear_normal, C.c_double)
         st.force = _p(self.force, C.c_double)
         self.st = st
 
@@ -376,5 +434,12 @@ def ref_lib():
                                C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
         R.ref_j2_update.argtypes = [C.c_double] * 4 + [D, D, D, D, D, C.POINTER(C.c_int)]
         R.ref_j2_update.restype = C.c_int
+        R.ref_j2_damage_update.argtypes = [C.c_double] * 8 + [D, D, D, D, C.POINTER(C.c_int), D,
+                                                              D, D, D, C.POINTER(C.c_int)]
+        R.ref_j2_damage_update.restype = C.c_int
+        R.ref_regularised_failure_strain.argtypes = [C.c_double] * 4
+        R.ref_regularised_failure_strain.restype = C.c_double
+        R.ref_triaxiality_factor.argtypes = [C.c_double] * 4
+        R.ref_triaxiality_factor.restype = C.c_double
         _ref = R
     return _ref

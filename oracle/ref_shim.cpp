@@ -4,7 +4,7 @@
 //
 // Built by oracle/Makefile together with the reference sources where they lie:
 //   engine/sim/fem.cpp         (makeBoxTetMesh, computeRestState, tetForces, stepCpu)
-//   engine/sim/plasticity.cpp  (update)
+//   engine/sim/plasticity.cpp  (update, regularisedFailureStrain, triaxialityFactor)
 // into oracle/_ref/libref.so. Nothing here restates the reference; it only moves
 // arrays in and out of sim::fem::TetMesh and sim::plasticity::State.
 #include "engine/sim/fem.hpp"
@@ -110,6 +110,57 @@ int ref_j2_update(double E, double nu, double yield, double hardening, const dou
     *dissipation = inc.dissipation;
     *yielded = inc.yielded ? 1 : 0;
     return inc.iterations;
+}
+
+// plasticity::update with a linear hardening curve and ductile failure: the given
+// (already regularised) failure strain and the Rice-Tracey constants. State in/out
+// including damage and the failed flag; returns 1 when the point failed in this call.
+int ref_j2_damage_update(double E, double nu, double yield, double hardening,
+                         double failure_strain, double sensitivity, double reference_eta,
+                         double cutoff_eta, const double strain[6], double eps_p[6],
+                         double* eqps, double* damage, int* failed, double stress[6],
+                         double* dissipation, double normal[3], double* triaxiality,
+                         int* yielded) {
+    sim::plasticity::Material mat;
+    mat.youngsModulus = E;
+    mat.poissonRatio = nu;
+    mat.flow = sim::plasticity::linearHardening(yield, hardening);
+    mat.failure.triaxialitySensitivity = sensitivity;
+    mat.failure.referenceTriaxiality = reference_eta;
+    mat.failure.cutoffTriaxiality = cutoff_eta;
+    sim::plasticity::State st;
+    for (int i = 0; i < 6; ++i) st.plasticStrain[i] = eps_p[i];
+    st.equivalentPlasticStrain = *eqps;
+    st.damage = *damage;
+    st.failed = *failed != 0;
+    const sim::plasticity::Increment inc =
+        sim::plasticity::update(mat, failure_strain, strain, st, stress);
+    for (int i = 0; i < 6; ++i) eps_p[i] = st.plasticStrain[i];
+    *eqps = st.equivalentPlasticStrain;
+    *damage = st.damage;
+    *failed = st.failed ? 1 : 0;
+    *dissipation = inc.dissipation;
+    *triaxiality = inc.triaxiality;
+    *yielded = inc.yielded ? 1 : 0;
+    for (int i = 0; i < 3; ++i) normal[i] = inc.failureNormal[i];
+    return inc.failedNow ? 1 : 0;
+}
+
+double ref_regularised_failure_strain(double uniform, double fracture, double length,
+                                      double thickness) {
+    sim::plasticity::Failure f;
+    f.uniformStrain = uniform;
+    f.fractureStrain = fracture;
+    return sim::plasticity::regularisedFailureStrain(f, length, thickness);
+}
+
+double ref_triaxiality_factor(double sensitivity, double reference_eta, double cutoff_eta,
+                              double eta) {
+    sim::plasticity::Failure f;
+    f.triaxialitySensitivity = sensitivity;
+    f.referenceTriaxiality = reference_eta;
+    f.cutoffTriaxiality = cutoff_eta;
+    return sim::plasticity::triaxialityFactor(f, eta);
 }
 
 }  // extern "C"

@@ -127,12 +127,14 @@ uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t
     // (tile, node) slots per tet (Kuhn-type meshes hold about 0.75)
     const uint64_t Np = roundUp64(N), Tp = roundUp64(T), nw = Np / 64, ew = Tp / 64;
     const uint64_t nt = (T + kTile - 1) / kTile, P = 2ull * T;
-    uint64_t b = 16ull * T + 40ull * Tp + 48ull * T + (plastic ? 28ull * Tp : 0) + 16ull * N +
+    const uint64_t pstate = plastic == 0 ? 0 : plastic == 1 ? 28ull * Tp : (32ull + 4 + 24) * Tp;
+    uint64_t b = 16ull * T + 40ull * Tp + 48ull * T + pstate + 16ull * N +
                  36ull * Np + 12ull * Np + 40ull * Np + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
                  8ull * T + 4ull * (nt + 1) + 4ull * P + 2ull * (P + nt) +
                  2ull * 4 * kTile * nt + 16ull * P + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
-                 8ull * (3 * ew + 10 * nw) + 24 + sizeof(EnergyRecordDev) * (uint64_t)std::max(cap, 1);
-    return b + 256ull * 36;
+                 8ull * ((kAccElemSlots + kSmpElemSlots + 1) * ew + (kAccNodeSlots + kSmpNodeSlots) * nw) +
+                 24 + sizeof(EnergyRecordDev) * (uint64_t)std::max(cap, 1);
+    return b + 256ull * 40;  // each of the arena's planes rounds up to 256 bytes
 }
 
 shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
@@ -259,6 +261,25 @@ shipfem_status shipfem_solver_energy(shipfem_solver* s, shipfem_energy_record* o
                                      int32_t* count) {
     if (!s) return SHIPFEM_INVALID_ARG;
     return statusOf(readEnergy(s->dc, out, max, count, s->stream));
+}
+
+namespace {
+shipfem_status copyTears(const std::vector<shipfem_tear>& tears, shipfem_tear* out, uint32_t max,
+                         uint32_t* count) {
+    if (count) *count = static_cast<uint32_t>(tears.size());
+    if (out)
+        for (uint32_t k = 0; k < max && k < tears.size(); ++k) out[k] = tears[k];
+    return SHIPFEM_OK;
+}
+}  // namespace
+
+shipfem_status shipfem_solver_tears(shipfem_solver* s, shipfem_tear* out, uint32_t max,
+                                    uint32_t* count) {
+    if (!s || (!out && max)) return SHIPFEM_INVALID_ARG;
+    std::vector<shipfem_tear> tears;
+    const hipError_t e = readTears(s->dc, s->hc, tears, s->stream);
+    if (e != hipSuccess) return statusOf(e);
+    return copyTears(tears, out, max, count);
 }
 
 shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out) {
@@ -531,6 +552,15 @@ shipfem_status shipfem_batch_energy(shipfem_batch* b, uint32_t i, shipfem_energy
                                     int32_t max, int32_t* count) {
     if (!b || i >= b->dcs.size()) return SHIPFEM_INVALID_ARG;
     return statusOf(readEnergy(b->dcs[i], out, max, count, b->stream));
+}
+
+shipfem_status shipfem_batch_tears(shipfem_batch* b, uint32_t i, shipfem_tear* out, uint32_t max,
+                                   uint32_t* count) {
+    if (!b || i >= b->dcs.size() || (!out && max)) return SHIPFEM_INVALID_ARG;
+    std::vector<shipfem_tear> tears;
+    const hipError_t e = readTears(b->dcs[i], b->hcs[i], tears, b->stream);
+    if (e != hipSuccess) return statusOf(e);
+    return copyTears(tears, out, max, count);
 }
 
 shipfem_status shipfem_batch_reset(shipfem_batch* b) {

@@ -57,13 +57,24 @@ __device__ __forceinline__ double wave_max(double s) {
 
 // ---------------------------------------------------------------- element math
 // Forces of one tet from its corner displacements p0..p3, Dm^-1 (B, row-major) and
-// rest volume V. PLASTIC cases read and update the tet's J2 state. Adds V sigma:d eps_p
-// to diss and, on sample steps, the strain energy 1/2 V sigma:(eps - eps_p) to se.
-template <bool PLASTIC>
-__device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, const float4& p0,
+// rest volume V at step n. PLASTIC cases read and update the tet's J2 state. Adds
+// V sigma:d eps_p to diss and, on sample steps, the strain energy 1/2 V sigma:(eps - eps_p)
+// to se. With damage, a tet that tears releases its stored energy into fract and carries
+// no stress from then on (plasticity.cpp:326, 457-477).
+template <bool PLASTIC, bool DAMAGE>
+__device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, int64_t n,
+                                              const float4& p0,
                                               const float4& p1, const float4& p2, const float4& p3,
                                               const float B[9], float V, int sample, float f[12],
-                                              double& diss, double& se) {
+                                              double& diss, double& se, double& fract,
+                                              bool& tore) {
+    tore = false;
+    if (DAMAGE && (c.features & kFeatDamage) && c.pstate[7 * c.Tpad + e] >= 1.0f) {
+        // torn: no stress, no strain energy, history frozen
+#pragma unroll
+        for (int k = 0; k < 12; ++k) f[k] = 0.0f;
+        return;
+    }
     // Du: columns are corner displacements relative to corner 0
     const float D[9] = {p1.x - p0.x, p2.x - p0.x, p3.x - p0.x,
                         p1.y - p0.y, p2.y - p0.y, p3.y - p0.y,
@@ -172,6 +183,20 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, cons
             diss += (double)d * (double)V;
 #pragma unroll
             for (int k = 0; k < 7; ++k) c.pstate[k * tp + e] = ep[k];
+            if (DAMAGE && (c.features & kFeatDamage)) {
+                // d damage = d eqps / (eps_f exp(-s (eta - eta_ref))), none at or below
+                // the cutoff; eta = mean stress over von Mises of the returned stress
+                const float q = 1.22474487139158905f * (nrm - 2.0f * c.mu * gamma);
+                const float eta = q > 0.0f ? pr / q : 0.0f;
+                float dmg = c.pstate[7 * tp + e];
+                if (eta > c.fail_ecut)
+                    dmg += root23 * gamma * __expf(c.fail_sens * (eta - c.fail_eref)) / c.fail_strain;
+                if (dmg >= 1.0f) {
+                    dmg = 1.0f;
+                    tore = true;
+                }
+                c.pstate[7 * tp + e] = dmg;
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -179,7 +204,32 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, cons
                 sv[3 + k] = tr[3 + k];
             }
         }
-        if (sample) {
+        if (DAMAGE && tore) {
+            // the stored energy leaves as fracture (the caller adds the half step of
+            // work its forces still carry and zeroes them); the world-frame stress
+            // R sigma R^T at the tear is kept for the failure plane (found on the host)
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += sv[k] * (st[k] - ep[k]);
+            fract += 0.5 * (double)s * (double)V;
+            // row by row: (R S)_i. then (R S R^T)_ij for j >= i, Voigt slot of (i, j)
+            const float S3[9] = {sv[0], sv[3], sv[5], sv[3], sv[1], sv[4], sv[5], sv[4], sv[2]};
+            constexpr int kSlot[9] = {0, 3, 5, 3, 1, 4, 5, 4, 2};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                float rs[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    rs[k] = S3[i * 3 + k] + (W[i * 3 + 0] * S3[0 * 3 + k] + W[i * 3 + 1] * S3[1 * 3 + k] +
+                                             W[i * 3 + 2] * S3[2 * 3 + k]);
+#pragma unroll
+                for (int j = i; j < 3; ++j)
+                    c.tear_stress[kSlot[i * 3 + j] * tp + e] =
+                        rs[j] + (rs[0] * W[j * 3 + 0] + rs[1] * W[j * 3 + 1] + rs[2] * W[j * 3 + 2]);
+            }
+            c.tear_step[e] = (int32_t)(n + 1);
+        }
+        if (sample && !(DAMAGE && tore)) {
             float s = 0.0f;
 #pragma unroll
             for (int k = 0; k < 6; ++k) s += sv[k] * (st[k] - ep[k]);
@@ -218,14 +268,37 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, cons
 // Per-wave FP64 sums of the element pass into the case's wave slots. Every lane of the
 // wave takes part (dead lanes add zero); a tail block's waves past the case's last wave
 // hold only dead lanes and write nothing.
-template <bool PLASTIC>
+// A tet that tore in this element pass: against the shadow kinetic energy
+// 1/2 m v(n-1/2).v(n+1/2) the potential it carried at u_n is U_e + 1/2 f_e.(u_n - u_(n-1))
+// with f_e its intact nodal forces, so the half step of work dt/2 f_e.v(n-1/2) joins the
+// fracture release before the forces are zeroed (v read as hi - compensation).
+__device__ __forceinline__ void release_torn(const CaseDev& c, const uint32_t g[4], float f[12],
+                                             double& fract) {
+    double half = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t k = a * c.Npad + g[q];
+            half += (double)f[q * 3 + a] * ((double)c.v[k] - (double)c.cv[k]);
+        }
+    fract += 0.5 * c.dt * half;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) f[k] = 0.0f;
+}
+
+template <bool PLASTIC, bool DAMAGE>
 __device__ __forceinline__ void element_wave_sums(const CaseDev& c, uint32_t e, int sample,
-                                                  double diss, double se) {
+                                                  double diss, double se, double fract) {
     const uint32_t w = e >> 6;
     const bool writer = (threadIdx.x & 63u) == 0 && w < c.elem_waves;
     if (PLASTIC && (c.features & kFeatPlastic)) {
         const double s = wave_sum(diss);
         if (writer) c.acc_e[kAccDiss * c.elem_waves + w] += s;
+    }
+    if (DAMAGE && (c.features & kFeatDamage)) {
+        const double s = wave_sum(fract);
+        if (writer) c.acc_e[kAccFract * c.elem_waves + w] += s;
     }
     if (sample) {
         const double s = wave_sum(se);
@@ -234,7 +307,7 @@ __device__ __forceinline__ void element_wave_sums(const CaseDev& c, uint32_t e, 
 }
 
 // ---------------------------------------------------------------- element pass, slots
-template <int BLOCK, bool PLASTIC>
+template <int BLOCK, bool PLASTIC, bool DAMAGE>
 __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ cases,
                                                 const Tile* __restrict__ tiles, int ntiles,
                                                 int parity, int sample, int xcd) {
@@ -244,7 +317,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ case
     if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
 
     const uint32_t e = tile.first + threadIdx.x;
-    double diss = 0.0, se = 0.0;
+    double diss = 0.0, se = 0.0, fract = 0.0;
     if (e < c.T) {
         const uint4 nd = c.conn[e];
         const float4 p0 = c.u4[nd.x], p1 = c.u4[nd.y], p2 = c.u4[nd.z], p3 = c.u4[nd.w];
@@ -252,13 +325,19 @@ __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ case
 #pragma unroll
         for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * c.Tpad + e];
         float f[12];
-        element_force<PLASTIC>(c, e, p0, p1, p2, p3, B, c.vol[e], sample, f, diss, se);
+        bool tore;
+        element_force<PLASTIC, DAMAGE>(c, e, c.step[parity], p0, p1, p2, p3, B, c.vol[e], sample, f,
+                                       diss, se, fract, tore);
+        if (DAMAGE && tore) {
+            const uint32_t g[4] = {nd.x, nd.y, nd.z, nd.w};
+            release_torn(c, g, f, fract);
+        }
         float4* out = reinterpret_cast<float4*>(c.force) + (size_t)e * 3;
         out[0] = make_float4(f[0], f[1], f[2], f[3]);
         out[1] = make_float4(f[4], f[5], f[6], f[7]);
         out[2] = make_float4(f[8], f[9], f[10], f[11]);
     }
-    element_wave_sums<PLASTIC>(c, e, sample, diss, se);
+    element_wave_sums<PLASTIC, DAMAGE>(c, e, sample, diss, se, fract);
 }
 
 // ---------------------------------------------------------------- element pass, tiled
@@ -274,7 +353,7 @@ __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
            ((lmax + 1) * sizeof(uint16_t) + 15) / 16 * 16;
 }
 
-template <bool PLASTIC>
+template <bool PLASTIC, bool DAMAGE>
 __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict__ cases,
                                                       const Tile* __restrict__ tiles, int ntiles,
                                                       uint32_t lmax, int parity, int sample,
@@ -314,16 +393,22 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     reinterpret_cast<uint2*>(sh_le)[threadIdx.x] = ent;
     __syncthreads();
 
-    double diss = 0.0, se = 0.0;
+    double diss = 0.0, se = 0.0, fract = 0.0;
     if (live) {
         const uint32_t l0 = lc.x & 0xFFFFu, l1 = lc.x >> 16, l2 = lc.y & 0xFFFFu, l3 = lc.y >> 16;
         float f[12];
-        element_force<PLASTIC>(c, e, sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3], B, V, sample, f,
-                               diss, se);
+        bool tore;
+        element_force<PLASTIC, DAMAGE>(c, e, c.step[parity], sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3],
+                                       B, V, sample, f, diss, se, fract, tore);
+        if (DAMAGE && tore) {
+            const uint32_t g[4] = {c.tnode[p0 + l0], c.tnode[p0 + l1], c.tnode[p0 + l2],
+                                   c.tnode[p0 + l3]};
+            release_torn(c, g, f, fract);
+        }
 #pragma unroll
         for (int k = 0; k < 12; ++k) sh_f[k * kTile + threadIdx.x] = f[k];
     }
-    element_wave_sums<PLASTIC>(c, e, sample, diss, se);
+    element_wave_sums<PLASTIC, DAMAGE>(c, e, sample, diss, se, fract);
     __syncthreads();
 
     // one lane per local node sums its corners in the tile's (tet, corner) order
@@ -356,9 +441,9 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
 
     const uint32_t i = tile.first + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t w = i >> 6;
+    // tiles start on multiples of 64 nodes, so the wave index is wave-uniform
+    const uint32_t w = __builtin_amdgcn_readfirstlane(i >> 6);
     const bool live = i < c.N;
-    const uint32_t fl = live ? c.nflags[i] : kPinAll;
     const double dt = c.dt;
     const float dtf = c.dtf;
     const uint32_t np = c.Npad;
@@ -367,46 +452,85 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
     double s_kin = 0.0, s_hext = 0.0, s_hcon = 0.0, s_hbnd = 0.0, s_hdamp = 0.0;
     double s_f[3] = {0.0, 0.0, 0.0};
 
-    if ((fl & kPinAll) != kPinAll) {
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        if (VARIANT == kSlots) {
-            // gather this node's slots in CSR (tet, corner) order
-            const uint32_t base = c.ell_off[w];
-            const uint32_t width = (c.ell_off[w + 1] - base) >> 6;
-            for (uint32_t q = 0; q < width; ++q) {
-                const uint32_t en = c.ell[base + q * 64u + lane];
-                if (en == kEllSentinel) break;
-                const float* s = c.force + (size_t)(en >> 2) * 12u + (en & 3u) * 3u;
-                fx += s[0];
-                fy += s[1];
-                fz += s[2];
-            }
-        } else {
-            // gather this node's (tile, node) partials in tile order
-            const uint32_t base = c.pell_off[w];
-            const uint32_t width = (c.pell_off[w + 1] - base) >> 6;
-            for (uint32_t q = 0; q < width; ++q) {
-                const uint32_t p = c.pell[base + q * 64u + lane];
-                if (p == kEllSentinel) break;
-                const float4 s = c.partial[p];
-                fx += s.x;
-                fy += s.y;
-                fz += s.z;
-            }
+    // Every load that does not depend on the gather is issued first, so a wave waits
+    // for one round trip for them and two for the gather (indices, then data) instead
+    // of a chain of dependent loads. Padding lanes read node 0 and discard it.
+    const uint32_t ii = live ? i : 0u;
+    const uint32_t fl = live ? c.nflags[ii] : kPinAll;
+    const float m = c.mass[ii];
+    const float im = c.invm[ii];
+    const float4 u = c.u4[ii];
+    float vv[3], cvv[3], cuu[3], ld[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        vv[a] = c.v[a * np + ii];
+        cvv[a] = c.cv[a * np + ii];
+        cuu[a] = c.cu[a * np + ii];
+    }
+    if (c.features & kFeatLoad) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ld[a] = c.load[a * np + ii];
+    }
+
+    // gather in the fixed order (slots: CSR (tet, corner); tiled: tile order), in
+    // batches of kGather slab rows: all index loads, then all data loads, then the
+    // adds in order. A lane's column ends in sentinels, which add nothing.
+    constexpr uint32_t kGather = 8;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    {
+        const uint32_t* off = VARIANT == kSlots ? c.ell_off : c.pell_off;
+        const uint32_t* slab = VARIANT == kSlots ? c.ell : c.pell;
+        // a block may overhang the case's last node: waves past its node_waves slabs
+        // have no offsets (off holds node_waves + 1 entries) and gather nothing
+        uint32_t base = 0, width = 0;
+        if (w < c.node_waves) {
+            base = off[w];
+            width = (off[w + 1] - base) >> 6;
         }
-        const float m = c.mass[i];
-        const float im = c.invm[i];
+        for (uint32_t q0 = 0; q0 < width; q0 += kGather) {
+            uint32_t idx[kGather];
+#pragma unroll
+            for (uint32_t k = 0; k < kGather; ++k)
+                idx[k] = q0 + k < width ? slab[base + (q0 + k) * 64u + lane] : kEllSentinel;
+            float g[kGather][3];
+#pragma unroll
+            for (uint32_t k = 0; k < kGather; ++k) {
+                g[k][0] = g[k][1] = g[k][2] = 0.0f;
+                if (idx[k] != kEllSentinel) {
+                    if (VARIANT == kSlots) {
+                        const float* sl = c.force + (size_t)(idx[k] >> 2) * 12u + (idx[k] & 3u) * 3u;
+                        g[k][0] = sl[0];
+                        g[k][1] = sl[1];
+                        g[k][2] = sl[2];
+                    } else {
+                        const float4 pv = c.partial[idx[k]];
+                        g[k][0] = pv.x;
+                        g[k][1] = pv.y;
+                        g[k][2] = pv.z;
+                    }
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kGather; ++k)
+                if (idx[k] != kEllSentinel) {
+                    fx += g[k][0];
+                    fy += g[k][1];
+                    fz += g[k][2];
+                }
+        }
+    }
+
+    if ((fl & kPinAll) != kPinAll) {
         const double t = (double)n * dt;
         float fe[3] = {0.0f, 0.0f, 0.0f};
         if ((c.features & kFeatLoad) && (fl & kLoaded)) {
             const double sl = c.load_ramp > 0.0 ? fmin(fmax(t / c.load_ramp, 0.0), 1.0) : 1.0;
             const float slf = (float)sl;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) fe[a] = slf * c.load[a * np + i];
+            for (int a = 0; a < 3; ++a) fe[a] = slf * ld[a];
         }
         if (c.features & kFeatGravity) fe[2] -= c.gravity * m;
 
-        float4 u = c.u4[i];
         float uu[3] = {u.x, u.y, u.z};
         float fc[3] = {0.0f, 0.0f, 0.0f};
         if (EXTRA && (c.features & kFeatContact) && (fl & kContact)) {
@@ -433,13 +557,6 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
             }
         }
 
-        float vv[3], cvv[3], cuu[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            vv[a] = c.v[a * np + i];
-            cvv[a] = c.cv[a * np + i];
-            cuu[a] = c.cu[a * np + i];
-        }
         const float fint[3] = {fx, fy, fz};
         const float sd = (EXTRA && (c.features & kFeatDrive))
                              ? [&] {
@@ -564,18 +681,21 @@ __global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restri
     const int64_t n0 = c.step[parity];
     if (c.step[parity ^ 1] != n0 + 1) return;  // the case did not step: no record
     const uint32_t ew = c.elem_waves, nw = c.node_waves;
+    // q: element accumulators, the element sample, node accumulators, node samples
+    constexpr int QS = kAccElemSlots, QN = QS + kSmpElemSlots, QM = QN + kAccNodeSlots;
     double q[kQ];
 #pragma unroll
     for (int k = 0; k < kQ; ++k) q[k] = 0.0;
     for (uint32_t w = threadIdx.x; w < ew; w += kRecordBlock) {
-        q[0] += c.acc_e[kAccDiss * ew + w];
-        q[1] += c.smp_e[kSmpStrain * ew + w];
+#pragma unroll
+        for (int k = 0; k < kAccElemSlots; ++k) q[k] += c.acc_e[k * ew + w];
+        q[QS] += c.smp_e[kSmpStrain * ew + w];
     }
     for (uint32_t w = threadIdx.x; w < nw; w += kRecordBlock) {
 #pragma unroll
-        for (int k = 0; k < kAccNodeSlots; ++k) q[2 + k] += c.acc_n[k * nw + w];
+        for (int k = 0; k < kAccNodeSlots; ++k) q[QN + k] += c.acc_n[k * nw + w];
 #pragma unroll
-        for (int k = 0; k < kSmpNodeSlots; ++k) q[2 + kAccNodeSlots + k] += c.smp_n[k * nw + w];
+        for (int k = 0; k < kSmpNodeSlots; ++k) q[QM + k] += c.smp_n[k * nw + w];
     }
     double tot[kQ];
     for (int k = 0; k < kQ; ++k) {
@@ -591,17 +711,18 @@ __global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restri
     if (threadIdx.x == 0) {
         const int32_t h = *c.hist_count;
         if (h < c.hist_cap) {
-            const double* sn = tot + 2 + kAccNodeSlots;
+            const double* sn = tot + QM;
             EnergyRecordDev& r = c.hist[h];
             r.step = n0;
             r.time = (double)n0 * c.dt;
-            r.strain = tot[1];
+            r.strain = tot[QS];
             r.kinetic = sn[kSmpKin];
-            r.dissipation = tot[0];
-            r.w_ext = tot[2 + kAccWext] - sn[kSmpHalfExt];
-            r.w_contact = tot[2 + kAccWcon] - sn[kSmpHalfCon];
-            r.w_boundary = tot[2 + kAccWbnd] - sn[kSmpHalfBnd];
-            r.damping_loss = tot[2 + kAccDamp] - sn[kSmpHalfDamp];
+            r.dissipation = tot[kAccDiss];
+            r.fracture = tot[kAccFract];
+            r.w_ext = tot[QN + kAccWext] - sn[kSmpHalfExt];
+            r.w_contact = tot[QN + kAccWcon] - sn[kSmpHalfCon];
+            r.w_boundary = tot[QN + kAccWbnd] - sn[kSmpHalfBnd];
+            r.damping_loss = tot[QN + kAccDamp] - sn[kSmpHalfDamp];
             r.contact_force[0] = sn[kSmpFx];
             r.contact_force[1] = sn[kSmpFy];
             r.contact_force[2] = sn[kSmpFz];
@@ -741,15 +862,20 @@ __global__ __launch_bounds__(256) void k_progress(const CaseDev* __restrict__ ca
     out[2 * i + 1] = static_cast<int64_t>(c.status[0]);
 }
 
+// PLASTIC / DAMAGE instantiations only add code gated per case by its feature bits, so
+// a case rounds identically in whichever instantiation a mix selects.
 template <int BLOCK>
 void elem_launch(const CaseDev* cases, const Tile* tiles, int ntiles, int parity, int sample,
-                 int xcd, bool plastic, hipStream_t s, const KernelTiming* t) {
-    if (plastic)
-        SHIPFEM_LAUNCH(t, (k_elem<BLOCK, true>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                 int xcd, bool plastic, bool damage, hipStream_t s, const KernelTiming* t) {
+    if (damage)
+        SHIPFEM_LAUNCH(t, (k_elem<BLOCK, true, true>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
+                       ntiles, parity, sample, xcd);
+    else if (plastic)
+        SHIPFEM_LAUNCH(t, (k_elem<BLOCK, true, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
                        ntiles, parity, sample, xcd);
     else
-        SHIPFEM_LAUNCH(t, (k_elem<BLOCK, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases, tiles,
-                       ntiles, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_elem<BLOCK, false, false>), dim3(ntiles), dim3(BLOCK), 0, s, cases,
+                       tiles, ntiles, parity, sample, xcd);
 }
 
 template <int BLOCK, int VARIANT>
@@ -778,11 +904,12 @@ void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
                  int sample, int xcd, uint32_t feat_union, hipStream_t stream,
                  const KernelTiming* t) {
     const bool plastic = (feat_union & kFeatPlastic) != 0;
+    const bool damage = (feat_union & kFeatDamage) != 0;
     switch (block) {
-        case 64: elem_launch<64>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
-        case 128: elem_launch<128>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
-        case 512: elem_launch<512>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
-        default: elem_launch<256>(cases, tiles, ntiles, parity, sample, xcd, plastic, stream, t); break;
+        case 64: elem_launch<64>(cases, tiles, ntiles, parity, sample, xcd, plastic, damage, stream, t); break;
+        case 128: elem_launch<128>(cases, tiles, ntiles, parity, sample, xcd, plastic, damage, stream, t); break;
+        case 512: elem_launch<512>(cases, tiles, ntiles, parity, sample, xcd, plastic, damage, stream, t); break;
+        default: elem_launch<256>(cases, tiles, ntiles, parity, sample, xcd, plastic, damage, stream, t); break;
     }
 }
 
@@ -802,12 +929,15 @@ void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint
                        int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream,
                        const KernelTiming* t) {
     const size_t lds = tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax;
-    if (feat_union & kFeatPlastic)
-        SHIPFEM_LAUNCH(t, (k_elem_tiled<true>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
-                       ntiles, lmax, parity, sample, xcd);
+    if (feat_union & kFeatDamage)
+        SHIPFEM_LAUNCH(t, (k_elem_tiled<true, true>), dim3(ntiles), dim3(kTile), lds, stream, cases,
+                       tiles, ntiles, lmax, parity, sample, xcd);
+    else if (feat_union & kFeatPlastic)
+        SHIPFEM_LAUNCH(t, (k_elem_tiled<true, false>), dim3(ntiles), dim3(kTile), lds, stream, cases,
+                       tiles, ntiles, lmax, parity, sample, xcd);
     else
-        SHIPFEM_LAUNCH(t, (k_elem_tiled<false>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles,
-                       ntiles, lmax, parity, sample, xcd);
+        SHIPFEM_LAUNCH(t, (k_elem_tiled<false, false>), dim3(ntiles), dim3(kTile), lds, stream,
+                       cases, tiles, ntiles, lmax, parity, sample, xcd);
 }
 
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream) {

@@ -6,6 +6,7 @@
 #include "shipfem_host.hpp"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -291,6 +292,43 @@ double shipfem_edge_timestep(const double* X, const uint32_t* index, uint32_t te
             }
     }
     return safety * shortest / std::sqrt(E / rho);
+}
+
+double shipfem_regularised_failure_strain(double uniform, double fracture, double length,
+                                          double thickness) {
+    // an element no larger than the plate is thick resolves the neck itself; degenerate
+    // sizes fall to the unregularised end (plasticity.cpp:158-169)
+    double share = 1.0;
+    if (length > 0.0 && thickness > 0.0) share = std::min(1.0, thickness / length);
+    return uniform + (fracture - uniform) * share;
+}
+
+shipfem_status shipfem_torn_groups(const double* X, const uint32_t* index, uint32_t T,
+                                   const uint32_t* group, const float* damage, double fraction,
+                                   uint32_t* out, uint32_t max, uint32_t* count) {
+    if (!X || !index || !group || !damage || (!out && max)) return SHIPFEM_INVALID_ARG;
+    // per group: total and torn rest volume, then the groups past `fraction`
+    std::map<uint32_t, std::pair<double, double>> vol;
+    for (uint32_t t = 0; t < T; ++t) {
+        const double* p0 = &X[3ull * index[4ull * t]];
+        double e[3][3];
+        for (int c = 1; c < 4; ++c)
+            for (int a = 0; a < 3; ++a) e[c - 1][a] = X[3ull * index[4ull * t + c] + a] - p0[a];
+        const double det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                           e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                           e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+        auto& g = vol[group[t]];
+        g.first += std::fabs(det) / 6.0;
+        if (damage[t] >= 1.0f) g.second += std::fabs(det) / 6.0;
+    }
+    uint32_t n = 0;
+    for (const auto& [id, v] : vol) {
+        if (!(v.second > 0.0) || v.second < fraction * v.first) continue;
+        if (out && n < max) out[n] = id;
+        ++n;
+    }
+    if (count) *count = n;
+    return SHIPFEM_OK;
 }
 
 }  // extern "C"

@@ -34,7 +34,7 @@ struct Layout {
 
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
-    uint64_t conn, dmi, vol, force, pstate, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
+    uint64_t conn, dmi, vol, force, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
         ell_off, ell, lconn, tn_off, tnode, lofs, lent, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
@@ -47,7 +47,10 @@ ArenaMap mapArena(const HostCase& hc) {
     m.dmi = L.take(36 * Tp);
     m.vol = L.take(4 * Tp);
     m.force = L.take(48 * T);
-    m.pstate = L.take((hc.features & kFeatPlastic) ? 28 * Tp : 0);
+    const bool dmg = (hc.features & kFeatDamage) != 0;
+    m.pstate = L.take((hc.features & kFeatPlastic) ? (dmg ? 32 : 28) * Tp : 0);
+    m.tear_step = L.take(dmg ? 4 * Tp : 0);
+    m.tear_stress = L.take(dmg ? 24 * Tp : 0);
     m.u4 = L.take(16 * N);
     m.cu = L.take(12 * Np);
     m.v = L.take(12 * Np);
@@ -128,6 +131,13 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
     if (!(mt.density > 0)) return fmt("density must be positive", mt.density);
     if (mt.plastic && !(mt.yield_strength > 0)) return fmt("yield_strength must be positive", mt.yield_strength);
     if (mt.plastic && !(mt.hardening_modulus >= 0)) return fmt("hardening_modulus must be >= 0", mt.hardening_modulus);
+    const bool failure = mt.plastic && mt.failure_strain > 0.0;
+    if (failure) {
+        if (!std::isfinite(mt.failure_strain)) return fmt("failure_strain must be finite", mt.failure_strain);
+        if (!std::isfinite(mt.triaxiality_sensitivity) || !std::isfinite(mt.reference_triaxiality) ||
+            !std::isfinite(mt.cutoff_triaxiality))
+            return "triaxiality constants must be finite";
+    }
     const double damping = d.damping == 0.0 ? 1.0 : d.damping;
     if (!(damping > 0 && damping <= 1.0)) return fmt("damping must lie in (0, 1]", d.damping);
     if (!(d.dt >= 0) || !std::isfinite(d.dt)) return fmt("dt must be >= 0", d.dt);
@@ -160,6 +170,10 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
     hc.bulk = hc.E / (3 * (1 - 2 * hc.nu));
     hc.yield = mt.yield_strength;
     hc.hard = mt.hardening_modulus;
+    hc.fail_strain = failure ? mt.failure_strain : 0.0;
+    hc.fail_sens = mt.triaxiality_sensitivity;
+    hc.fail_eref = mt.reference_triaxiality;
+    hc.fail_ecut = mt.cutoff_triaxiality;
     hc.load_ramp = d.load_ramp;
     hc.drive_ramp = d.drive_ramp;
     hc.gravity = d.gravity;
@@ -283,7 +297,8 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
                 }
         if (!anyDrive) hc.presc.clear();
     }
-    hc.features = (mt.plastic ? kFeatPlastic : 0) | (anyLoad ? kFeatLoad : 0) |
+    hc.features = (mt.plastic ? kFeatPlastic : 0) | (failure ? kFeatDamage : 0) |
+                  (anyLoad ? kFeatLoad : 0) |
                   (anyContact ? kFeatContact : 0) | (anyDrive ? kFeatDrive : 0) |
                   (damping != 1.0 ? kFeatDamp : 0) | (d.gravity != 0.0 ? kFeatGravity : 0);
 
@@ -433,6 +448,8 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.vol = reinterpret_cast<const float*>(base + m.vol);
     r.force = reinterpret_cast<float*>(base + m.force);
     r.pstate = (hc.features & kFeatPlastic) ? reinterpret_cast<float*>(base + m.pstate) : nullptr;
+    r.tear_step = (hc.features & kFeatDamage) ? reinterpret_cast<int32_t*>(base + m.tear_step) : nullptr;
+    r.tear_stress = (hc.features & kFeatDamage) ? reinterpret_cast<float*>(base + m.tear_stress) : nullptr;
     r.u4 = reinterpret_cast<float4*>(base + m.u4);
     r.cu = reinterpret_cast<float*>(base + m.cu);
     r.v = reinterpret_cast<float*>(base + m.v);
@@ -464,6 +481,10 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.bulk = static_cast<float>(hc.bulk);
     r.yield = static_cast<float>(hc.yield);
     r.hard = static_cast<float>(hc.hard);
+    r.fail_strain = static_cast<float>(hc.fail_strain);
+    r.fail_sens = static_cast<float>(hc.fail_sens);
+    r.fail_eref = static_cast<float>(hc.fail_eref);
+    r.fail_ecut = static_cast<float>(hc.fail_ecut);
     r.gravity = static_cast<float>(hc.gravity);
     r.damping = static_cast<float>(hc.damping);
     r.ind_r = static_cast<float>(hc.indenter.radius);
@@ -534,8 +555,9 @@ hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_
     if (e == hipSuccess) e = hipMemcpyAsync(step, r.step, 16, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&status, r.status, 4, hipMemcpyDeviceToHost, s);
     std::vector<float> ps;
-    if (r.pstate && (view->eqps || view->plastic_strain)) {
-        ps.resize(7 * (uint64_t)hc.Tpad);
+    const bool dmg = (hc.features & kFeatDamage) != 0;
+    if (r.pstate && (view->eqps || view->plastic_strain || view->damage)) {
+        ps.resize((dmg ? 8 : 7) * (uint64_t)hc.Tpad);
         if (e == hipSuccess)
             e = hipMemcpyAsync(ps.data(), r.pstate, 4 * ps.size(), hipMemcpyDeviceToHost, s);
     }
@@ -557,6 +579,8 @@ hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_
     if (view->plastic_strain)
         for (uint64_t t = 0; t < T; ++t)
             for (int k = 0; k < 6; ++k) view->plastic_strain[t * 6 + k] = ps.empty() ? 0.0f : ps[k * Tp + t];
+    if (view->damage)
+        for (uint64_t t = 0; t < T; ++t) view->damage[t] = dmg && !ps.empty() ? ps[7 * Tp + t] : 0.0f;
     view->step = std::max(step[0], step[1]);
     view->time = static_cast<double>(view->step) * dc.dt;
     view->status = status;
@@ -578,6 +602,84 @@ hipError_t readEnergy(const DeviceCase& dc, shipfem_energy_record* out, int32_t 
     }
     if (count) *count = n;
     return e;
+}
+
+hipError_t readTears(const DeviceCase& dc, const HostCase& hc, std::vector<shipfem_tear>& out,
+                     hipStream_t s) {
+    out.clear();
+    if (!(hc.features & kFeatDamage)) return hipSuccess;
+    const uint64_t Tp = hc.Tpad;
+    std::vector<int32_t> step(Tp);
+    std::vector<float> st(6 * Tp);
+    hipError_t e = hipMemcpyAsync(step.data(), dc.rec.tear_step, 4 * Tp, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(st.data(), dc.rec.tear_stress, 24 * Tp, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    for (uint32_t t = 0; t < hc.T; ++t) {
+        if (step[t] == 0) continue;
+        shipfem_tear tear{};
+        tear.tet = t;
+        tear.step = static_cast<int64_t>(step[t]) - 1;
+        double v[6];
+        for (int k = 0; k < 6; ++k) v[k] = st[k * Tp + t];
+        maxPrincipalDirection(v, tear.normal);
+        out.push_back(tear);
+    }
+    std::stable_sort(out.begin(), out.end(), [](const shipfem_tear& a, const shipfem_tear& b) {
+        return a.step < b.step;
+    });
+    return hipSuccess;
+}
+
+// Largest-eigenvalue direction of a symmetric 3x3 (Voigt, tensor shears) by cyclic
+// Jacobi in FP64; sign fixed so the largest component is positive.
+void maxPrincipalDirection(const double s6[6], double out[3]) {
+    double a[9] = {s6[0], s6[3], s6[5], s6[3], s6[1], s6[4], s6[5], s6[4], s6[2]};
+    double v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
+        const double diag = a[0] * a[0] + a[4] * a[4] + a[8] * a[8];
+        if (off == 0.0 || off <= 1e-30 * diag) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                const double apq = a[p * 3 + q];
+                if (apq == 0.0) continue;
+                const double th = (a[q * 3 + q] - a[p * 3 + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    const double x = a[k * 3 + p], y = a[k * 3 + q];
+                    a[k * 3 + p] = c * x - sn * y;
+                    a[k * 3 + q] = sn * x + c * y;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double x = a[p * 3 + k], y = a[q * 3 + k];
+                    a[p * 3 + k] = c * x - sn * y;
+                    a[q * 3 + k] = sn * x + c * y;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double x = v[k * 3 + p], y = v[k * 3 + q];
+                    v[k * 3 + p] = c * x - sn * y;
+                    v[k * 3 + q] = sn * x + c * y;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 3; ++i)
+        if (a[i * 3 + i] > a[best * 3 + best]) best = i;
+    for (int i = 0; i < 3; ++i) out[i] = v[i * 3 + best];
+    double norm = std::sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+    if (!(norm > 0.0)) {
+        out[0] = 1.0;
+        out[1] = out[2] = 0.0;
+        return;
+    }
+    int largest = 0;
+    for (int i = 1; i < 3; ++i)
+        if (std::fabs(out[i]) > std::fabs(out[largest])) largest = i;
+    if (out[largest] < 0.0) norm = -norm;
+    for (int i = 0; i < 3; ++i) out[i] /= norm;
 }
 
 }  // namespace shipfem

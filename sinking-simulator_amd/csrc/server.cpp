@@ -1,11 +1,15 @@
 // server.cpp -- resident serving loop with continuous batching.
 //
 // Requests (damage-case solves of uneven size) arrive at any time. Host preparation
-// runs on a small worker pool; one submission thread owns the device. Between device
-// rounds it admits prepared cases in arrival order while their arena fits the device
-// pool (first-fit, coalescing) and a slot is free, retires finished cases, and runs the
-// next round: `chunk` steps of every resident case in one element launch and one node
-// launch per step over a shared tile table. A case's arithmetic and its per-wave
+// runs on a small worker pool; one submission thread owns the device. The device runs
+// rounds: `chunk` steps of every resident case in one element launch and one node
+// launch per step over a shared tile table, replayed from a captured graph. Two rounds
+// are kept in flight on the run stream: while the device runs one, the submission
+// thread retires the cases the previous round finished (read back on a side stream),
+// admits prepared cases in arrival order while their arena fits the device pool
+// (first-fit, coalescing) and captures the next round. A case's step count after each
+// round is known when the round is launched, so a finishing case is left out of the
+// next round before its last round has even run. A case's arithmetic and its per-wave
 // reductions never depend on which other cases share the launch, so its result is
 // bit-identical alone and in the mix.
 #include "shipfem_host.hpp"
@@ -99,7 +103,21 @@ struct Active {
     DeviceCase dc;
     uint64_t off = 0;
     int64_t steps = 0;
-    int64_t last = 0;  // step reached at the end of the previous round
+    int64_t last = 0;        // step reached at the end of the last completed round
+    int64_t predicted = 0;   // step it reaches when every launched round has run
+    uint64_t last_round = 0; // the last launched round it belongs to (0: none)
+    bool dead = false;       // reported non-finite: joins no further round
+};
+
+// One launched round: its plan, members, timing events and progress buffers.
+struct Round {
+    uint64_t id = 0;
+    int plan = 0;
+    std::vector<Active*> members;
+    hipEvent_t start = nullptr, stop = nullptr, done = nullptr;
+    int64_t* d_prog = nullptr;  // [2 * members] device
+    int64_t* h_prog = nullptr;  // [2 * members] pinned host
+    size_t cap = 0;
 };
 
 struct Kept {
@@ -116,10 +134,8 @@ struct Kept {
 struct shipfem_server {
     shipfem_server_opts opts{};
     DevicePool pool;
-    hipStream_t stream = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int64_t* d_progress = nullptr;
-    size_t progress_cap = 0;
+    hipStream_t run = nullptr;   // rounds
+    hipStream_t side = nullptr;  // uploads, stable steps, capture, readback
 
     std::mutex m;
     std::condition_variable cv_raw, cv_ready, cv_done;
@@ -134,16 +150,44 @@ struct shipfem_server {
     shipfem_server_stats st{};
 
     // owned by the loop thread
-    std::vector<Active> active;
-    Plan plan;
-    bool membership_changed = false;
+    std::vector<std::unique_ptr<Active>> active;
+    Plan plans[2];
+    std::vector<uint64_t> plan_sig[2];  // request ids each plan's graph was captured for
+    int next_plan = 0;
+    Plan omega_plan;                    // stable steps of newcomers
+    Round slots[2];
+    std::deque<int> inflight;           // slot indices, oldest first
+    uint64_t rounds_launched = 0;
 
     void workerMain();
     void loopMain();
     bool admit();
-    void finish(size_t i, const int64_t* progress, double now);
-    hipError_t round();
+    void finish(Active& a, hipError_t fault);
+    hipError_t launchRound(const std::vector<Active*>& members);
+    hipError_t completeOldest();
+    void failAll(hipError_t e);
+    // Device objects: waits for both streams, then frees (safe on partial creation).
+    void release();
 };
+
+void shipfem_server::release() {
+    if (run) (void)hipStreamSynchronize(run);
+    if (side) (void)hipStreamSynchronize(side);
+    plans[0].dropGraph();
+    plans[1].dropGraph();
+    active.clear();
+    for (Round& r : slots) {
+        if (r.d_prog) (void)hipFree(r.d_prog);
+        if (r.h_prog) (void)hipHostFree(r.h_prog);
+        if (r.start) (void)hipEventDestroy(r.start);
+        if (r.stop) (void)hipEventDestroy(r.stop);
+        if (r.done) (void)hipEventDestroy(r.done);
+        r = Round{};
+    }
+    if (run) (void)hipStreamDestroy(run);
+    if (side) (void)hipStreamDestroy(side);
+    run = side = nullptr;
+}
 
 void shipfem_server::workerMain() {
     for (;;) {
@@ -182,7 +226,8 @@ void shipfem_server::workerMain() {
 }
 
 // Moves prepared requests into free slots while their arenas fit. Arrival order:
-// the head of the queue waits for room rather than being overtaken.
+// the head of the queue waits for room rather than being overtaken. Runs while rounds
+// are in flight: it only touches free pool memory and the side stream.
 bool shipfem_server::admit() {
     std::vector<std::unique_ptr<Request>> take;
     std::vector<uint64_t> offs;
@@ -213,25 +258,27 @@ bool shipfem_server::admit() {
     }
     if (take.empty()) return false;
     // upload, then the FP64 stable step of the newcomers in one pass
-    Plan fresh;
-    std::vector<Active> add(take.size());
+    std::vector<std::unique_ptr<Active>> add(take.size());
+    omega_plan.recs.clear();
     hipError_t e = hipSuccess;
-    for (size_t k = 0; k < take.size() && e == hipSuccess; ++k) {
-        add[k].off = offs[k];
-        e = uploadCase(take[k]->hc, pool.base() + offs[k], add[k].dc, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        fresh.recs.push_back(add[k].dc.rec);
+    for (size_t k = 0; k < take.size(); ++k) {
+        add[k] = std::make_unique<Active>();
+        add[k]->off = offs[k];
+        if (e == hipSuccess) e = uploadCase(take[k]->hc, pool.base() + offs[k], add[k]->dc, side);
+        if (e == hipSuccess) e = hipStreamSynchronize(side);
+        omega_plan.recs.push_back(add[k]->dc.rec);
     }
     std::vector<double> lmax;
-    if (e == hipSuccess) e = fresh.build(stream);
-    if (e == hipSuccess) e = fresh.omegaMax(lmax, stream);
+    if (e == hipSuccess) e = omega_plan.build(side);
+    if (e == hipSuccess) e = omega_plan.omegaMax(lmax, side);
     const double now = nowSeconds();
+    size_t admitted = 0;
     for (size_t k = 0; k < take.size(); ++k) {
-        Active& a = add[k];
+        Active& a = *add[k];
         HostCase& hc = take[k]->hc;
         if (e != hipSuccess) {
-            pool.release(a.off, arenaBytes(hc));
             std::lock_guard<std::mutex> lk(m);
+            pool.release(a.off, arenaBytes(hc));
             shipfem_result res{};
             res.request_id = take[k]->id;
             res.status = statusOf(e);
@@ -247,115 +294,171 @@ bool shipfem_server::admit() {
         a.dc.rec.step_end = a.steps;
         take[k]->admit_s = now;
         a.req = std::move(take[k]);
-        active.push_back(std::move(a));
+        active.push_back(std::move(add[k]));
+        ++admitted;
     }
     {
         std::lock_guard<std::mutex> lk(m);
-        st.admitted += take.size();
+        st.admitted += admitted;
         st.active = active.size();
     }
     cv_done.notify_all();
-    membership_changed = true;
-    return true;
+    if (e != hipSuccess) (void)hipGetLastError();
+    return admitted > 0;
 }
 
-void shipfem_server::finish(size_t i, const int64_t* progress, double now) {
-    Active& a = active[i];
+// Reads a retired case back (side stream), reports it and returns its pool memory.
+// The case belongs to no round still in flight.
+void shipfem_server::finish(Active& a, hipError_t fault) {
     HostCase& hc = a.req->hc;
     shipfem_result res{};
     res.request_id = a.req->id;
-    res.status = progress[2 * i + 1] ? SHIPFEM_UNSTABLE : SHIPFEM_OK;
+    res.status = fault != hipSuccess ? statusOf(fault) : (a.dead ? SHIPFEM_UNSTABLE : SHIPFEM_OK);
     res.submit_s = a.req->submit_s;
     res.admit_s = a.req->admit_s;
     res.node_count = hc.N;
     res.tet_count = hc.T;
     Kept k;
-    k.u.resize(3ull * hc.N);
-    k.v.resize(3ull * hc.N);
-    k.eqps.resize(hc.T);
     shipfem_state_view view{};
-    view.displacement = k.u.data();
-    view.velocity = k.v.data();
-    view.eqps = k.eqps.data();
-    hipError_t e = readbackCase(a.dc, hc, &view, stream);
-    std::vector<shipfem_energy_record> rec(static_cast<size_t>(a.dc.rec.hist_cap));
+    std::vector<shipfem_energy_record> rec;
     int32_t nrec = 0;
-    if (e == hipSuccess) e = readEnergy(a.dc, rec.data(), a.dc.rec.hist_cap, &nrec, stream);
-    if (e != hipSuccess) res.status = statusOf(e);
-    res.steps = view.step;
-    res.sim_time = view.step * a.dc.dt;
-    for (uint32_t n = 0; n < hc.N; ++n) {
-        const double* u = &k.u[3ull * n];
-        res.max_displacement = std::max(res.max_displacement, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    if (fault == hipSuccess) {
+        k.u.resize(3ull * hc.N);
+        k.v.resize(3ull * hc.N);
+        k.eqps.resize(hc.T);
+        view.displacement = k.u.data();
+        view.velocity = k.v.data();
+        view.eqps = k.eqps.data();
+        hipError_t e = readbackCase(a.dc, hc, &view, side);
+        rec.resize(static_cast<size_t>(a.dc.rec.hist_cap));
+        if (e == hipSuccess) e = readEnergy(a.dc, rec.data(), a.dc.rec.hist_cap, &nrec, side);
+        if (e != hipSuccess) res.status = statusOf(e);
+        res.steps = view.step;
+        res.sim_time = view.step * a.dc.dt;
+        for (uint32_t n = 0; n < hc.N; ++n) {
+            const double* u = &k.u[3ull * n];
+            res.max_displacement = std::max(res.max_displacement, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+        }
+        for (float q : k.eqps) res.max_eqps = std::max(res.max_eqps, static_cast<double>(q));
+        if (hc.features & kFeatDamage) {
+            std::vector<float> dmg(hc.T);
+            shipfem_state_view dv{};
+            dv.damage = dmg.data();
+            if (readbackCase(a.dc, hc, &dv, side) == hipSuccess)
+                for (uint32_t t = 0; t < hc.T; ++t)
+                    if (dmg[t] >= 1.0f) {
+                        ++res.torn_elements;
+                        res.torn_volume += hc.vol[t];
+                    }
+        }
+        for (int32_t r = 0; r < nrec; ++r) {
+            const double* f = rec[r].contact_force;
+            res.peak_contact_force = std::max(res.peak_contact_force, std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
+        }
+        if (nrec > 0) res.last = rec[nrec - 1];
     }
-    for (float q : k.eqps) res.max_eqps = std::max(res.max_eqps, static_cast<double>(q));
-    for (int32_t r = 0; r < nrec; ++r) {
-        const double* f = rec[r].contact_force;
-        res.peak_contact_force = std::max(res.peak_contact_force, std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
-    }
-    if (nrec > 0) res.last = rec[nrec - 1];
     res.done_s = nowSeconds();
     std::lock_guard<std::mutex> lk(m);
     pool.release(a.off, arenaBytes(hc));
     auto it = keep.find(res.request_id);
     if (it != keep.end()) {
-        k.step = view.step;
-        k.time = view.time;
-        k.status = view.status;
-        k.hc.N = hc.N;
-        k.hc.T = hc.T;
-        k.hc.X = std::move(hc.X);
-        kept[res.request_id] = std::move(k);
+        if (fault == hipSuccess) {
+            k.step = view.step;
+            k.time = view.time;
+            k.status = view.status;
+            k.hc.N = hc.N;
+            k.hc.T = hc.T;
+            k.hc.X = std::move(hc.X);
+            kept[res.request_id] = std::move(k);
+        }
         keep.erase(it);
     }
     results.push_back(res);
     if (res.status == SHIPFEM_OK) ++st.completed;
     else ++st.failed;
-    (void)now;
 }
 
-hipError_t shipfem_server::round() {
+// Captures (or reuses) the graph for `members` on the plan not used by the round in
+// flight, then queues the round, its progress read-out and its events on the run stream.
+hipError_t shipfem_server::launchRound(const std::vector<Active*>& members) {
     const int R = opts.chunk;
-    if (membership_changed) {
-        plan.recs.clear();
-        for (auto& a : active) plan.recs.push_back(a.dc.rec);
-        plan.parity = 0;
-        hipError_t e = plan.build(stream);
+    const int pi = next_plan;
+    Plan& p = plans[pi];
+    std::vector<uint64_t> sig;
+    sig.reserve(members.size());
+    for (const Active* a : members) sig.push_back(a->req->id);
+    hipError_t e = hipSuccess;
+    if (sig != plan_sig[pi] || !p.graph) {
+        plan_sig[pi].clear();
+        p.recs.clear();
+        for (const Active* a : members) p.recs.push_back(a->dc.rec);
+        p.parity = 0;
+        e = p.build(side);
+        if (e == hipSuccess) e = p.capture(R, side);
         if (e != hipSuccess) return e;
-        e = plan.capture(R, stream);
-        if (e != hipSuccess) return e;
-        membership_changed = false;
-        if (active.size() > progress_cap) {
-            if (d_progress) (void)hipFree(d_progress);
-            progress_cap = active.size() * 2;
-            e = hipMalloc(reinterpret_cast<void**>(&d_progress), 16 * progress_cap);
-            if (e != hipSuccess) return e;
-        }
+        plan_sig[pi] = std::move(sig);
     }
-    hipError_t e = hipEventRecord(e0, stream);
-    if (e == hipSuccess) e = hipGraphLaunch(plan.graph, stream);
-    if (e == hipSuccess) e = hipEventRecord(e1, stream);
-    if (e == hipSuccess) launch_progress(plan.d_recs, static_cast<int>(active.size()), d_progress, stream);
-    std::vector<int64_t> prog(2 * active.size());
-    if (e == hipSuccess) e = hipMemcpyAsync(prog.data(), d_progress, 16 * active.size(), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    int slot = 0;
+    while (std::find(inflight.begin(), inflight.end(), slot) != inflight.end()) ++slot;
+    Round& r = slots[slot];
+    const size_t n = members.size();
+    if (n > r.cap) {
+        if (r.d_prog) (void)hipFree(r.d_prog);
+        if (r.h_prog) (void)hipHostFree(r.h_prog);
+        r.d_prog = nullptr;
+        r.h_prog = nullptr;
+        r.cap = 0;
+        const size_t cap = std::max<size_t>(2 * n, 64);
+        e = hipMalloc(reinterpret_cast<void**>(&r.d_prog), 16 * cap);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&r.h_prog), 16 * cap);
+        if (e != hipSuccess) return e;
+        r.cap = cap;
+    }
+    r.id = ++rounds_launched;
+    r.plan = pi;
+    r.members = members;
+    e = hipEventRecord(r.start, run);
+    if (e == hipSuccess) e = hipGraphLaunch(p.graph, run);
+    if (e == hipSuccess) e = hipEventRecord(r.stop, run);
+    if (e == hipSuccess) {
+        launch_progress(p.d_recs, static_cast<int>(n), r.d_prog, run);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(r.h_prog, r.d_prog, 16 * n, hipMemcpyDeviceToHost, run);
+    if (e == hipSuccess) e = hipEventRecord(r.done, run);
     if (e != hipSuccess) return e;
-    // element-steps actually advanced this round (finished cases idle in it)
-    double done_steps = 0;
-    for (size_t i = 0; i < active.size(); ++i) {
-        done_steps += static_cast<double>(active[i].req->hc.T) * static_cast<double>(prog[2 * i] - active[i].last);
-        active[i].last = prog[2 * i];
+    for (Active* a : members) {
+        a->predicted = std::min(a->predicted + R, a->steps);
+        a->last_round = r.id;
     }
-    // retire finished (or failed) cases, newest index first so erasure keeps order valid
-    const double now = nowSeconds();
+    inflight.push_back(slot);
+    next_plan ^= 1;
+    return hipSuccess;
+}
+
+// Waits for the oldest round in flight, accounts it and retires what it finished.
+hipError_t shipfem_server::completeOldest() {
+    Round& r = slots[inflight.front()];
+    hipError_t e = hipEventSynchronize(r.done);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.start, r.stop);
+    if (e != hipSuccess) return e;
+    inflight.pop_front();
+    // element-steps actually advanced (finished cases idle in a round)
+    double done_steps = 0;
+    for (size_t i = 0; i < r.members.size(); ++i) {
+        Active* a = r.members[i];
+        done_steps += static_cast<double>(a->req->hc.T) * static_cast<double>(r.h_prog[2 * i] - a->last);
+        a->last = r.h_prog[2 * i];
+        if (r.h_prog[2 * i + 1] != 0) a->dead = true;
+    }
+    // retire cases whose last round this was and that will join no other
     for (size_t i = active.size(); i-- > 0;) {
-        const bool over = prog[2 * i] >= active[i].steps || prog[2 * i + 1] != 0;
-        if (over) {
-            finish(i, prog.data(), now);
+        Active& a = *active[i];
+        const bool over = a.dead || a.predicted >= a.steps;
+        if (over && a.last_round <= r.id) {
+            finish(a, hipSuccess);
             active.erase(active.begin() + static_cast<std::ptrdiff_t>(i));
-            membership_changed = true;
         }
     }
     std::lock_guard<std::mutex> lk(m);
@@ -367,32 +470,57 @@ hipError_t shipfem_server::round() {
     return hipSuccess;
 }
 
+// A device error fails every resident case; the server keeps serving.
+void shipfem_server::failAll(hipError_t e) {
+    (void)hipStreamSynchronize(run);
+    (void)hipStreamSynchronize(side);
+    (void)hipGetLastError();
+    inflight.clear();
+    for (auto& a : active) finish(*a, e);
+    active.clear();
+    plan_sig[0].clear();
+    plan_sig[1].clear();
+    std::lock_guard<std::mutex> lk(m);
+    st.active = 0;
+    cv_done.notify_all();
+}
+
 void shipfem_server::loopMain() {
     for (;;) {
+        bool stopping = false;
         {
             std::unique_lock<std::mutex> lk(m);
-            if (active.empty())
+            if (active.empty() && inflight.empty())
                 cv_ready.wait(lk, [&] { return stop || !ready.empty(); });
-            if (stop) return;
+            stopping = stop;
+        }
+        if (stopping) {
+            // launch nothing more; let the rounds in flight end before the streams go
+            while (!inflight.empty())
+                if (completeOldest() != hipSuccess) break;
+            return;
         }
         admit();
-        if (active.empty()) {
-            // nothing fits yet (cannot happen with an empty device) or admission failed
-            std::this_thread::sleep_for(std::chrono::milliseconds(1));
-            continue;
+        bool launched = false;
+        if (inflight.size() < 2) {
+            std::vector<Active*> members;
+            for (auto& a : active)
+                if (!a->dead && a->predicted < a->steps) members.push_back(a.get());
+            if (!members.empty()) {
+                const hipError_t e = launchRound(members);
+                if (e != hipSuccess) {
+                    failAll(e);
+                    continue;
+                }
+                launched = true;
+            }
         }
-        const hipError_t e = round();
-        if (e != hipSuccess) {
-            // a device error fails every resident case; the server keeps serving
-            std::vector<int64_t> prog(2 * active.size(), 0);
-            for (size_t i = 0; i < prog.size() / 2; ++i) prog[2 * i + 1] = 1;
-            const double now = nowSeconds();
-            for (size_t i = active.size(); i-- > 0;) finish(i, prog.data(), now);
-            active.clear();
-            membership_changed = true;
-            std::lock_guard<std::mutex> lk(m);
-            st.active = 0;
-            (void)hipGetLastError();
+        if (!inflight.empty() && (inflight.size() == 2 || !launched)) {
+            const hipError_t e = completeOldest();
+            if (e != hipSuccess) failAll(e);
+        } else if (!launched) {
+            // resident cases wait for nothing and nothing fits yet: brief back-off
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
         }
     }
 }
@@ -418,7 +546,7 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
         delete s;
         return SHIPFEM_INVALID_ARG;
     }
-    if (!shapeFrom(&s->opts.launch, s->plan.shape)) {
+    if (!shapeFrom(&s->opts.launch, s->plans[0].shape)) {
         setError(err, err_len, "launch config: blocks must be 64/128/256/512, variant TILED or SLOTS");
         delete s;
         return SHIPFEM_INVALID_ARG;
@@ -428,19 +556,23 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
         (void)hipMemGetInfo(&freeb, &total);
         s->opts.memory_budget = static_cast<uint64_t>(0.6 * static_cast<double>(total));
     }
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&s->e0);
-    if (e == hipSuccess) e = hipEventCreate(&s->e1);
+    hipError_t e = hipStreamCreateWithFlags(&s->run, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking);
+    for (Round& r : s->slots) {
+        if (e == hipSuccess) e = hipEventCreate(&r.start);
+        if (e == hipSuccess) e = hipEventCreate(&r.stop);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r.done, hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = s->pool.init(s->opts.memory_budget);
     if (e != hipSuccess) {
         setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
         const shipfem_status st = statusOf(e);
-        if (s->e0) (void)hipEventDestroy(s->e0);
-        if (s->e1) (void)hipEventDestroy(s->e1);
-        if (s->stream) (void)hipStreamDestroy(s->stream);
+        s->release();
         delete s;
         return st;
     }
+    s->plans[1].shape = s->plans[0].shape;
+    s->omega_plan.shape = s->plans[0].shape;
     s->st.bytes_budget = s->opts.memory_budget;
     const unsigned nworkers = std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
     for (unsigned w = 0; w < nworkers; ++w) s->workers.emplace_back([s] { s->workerMain(); });
@@ -454,7 +586,8 @@ shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc*
     if (!s || !d || !(run_to > 0) || !d->rest_position || !d->index || d->node_count == 0 ||
         d->tet_count == 0)
         return SHIPFEM_INVALID_ARG;
-    const uint64_t bound = shipfem_case_footprint(d->node_count, d->tet_count, d->material.plastic, 512);
+    const int32_t kind = d->material.plastic ? (d->material.failure_strain > 0.0 ? 2 : 1) : 0;
+    const uint64_t bound = shipfem_case_footprint(d->node_count, d->tet_count, kind, 512);
     if (bound > s->opts.memory_budget) {
         std::lock_guard<std::mutex> lk(s->m);
         ++s->st.refused;
@@ -559,13 +692,7 @@ void shipfem_server_destroy(shipfem_server* s) {
     s->cv_ready.notify_all();
     for (auto& w : s->workers) w.join();
     if (s->loop.joinable()) s->loop.join();
-    (void)hipStreamSynchronize(s->stream);
-    s->plan.dropGraph();
-    s->active.clear();
-    if (s->d_progress) (void)hipFree(s->d_progress);
-    (void)hipEventDestroy(s->e0);
-    (void)hipEventDestroy(s->e1);
-    (void)hipStreamDestroy(s->stream);
+    s->release();
     delete s;
 }
 

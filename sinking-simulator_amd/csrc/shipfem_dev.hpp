@@ -27,10 +27,11 @@ enum Feature : uint32_t {
     kFeatDrive = 1u << 3,
     kFeatDamp = 1u << 4,
     kFeatGravity = 1u << 5,
+    kFeatDamage = 1u << 6,  // ductile damage and element deletion (needs kFeatPlastic)
 };
 
 // per-wave FP64 accumulator slots (running sums, read-modify-write every step)
-enum { kAccDiss = 0, kAccElemSlots = 1 };
+enum { kAccDiss = 0, kAccFract = 1, kAccElemSlots = 2 };
 enum { kAccWext = 0, kAccWcon = 1, kAccWbnd = 2, kAccDamp = 3, kAccNodeSlots = 4 };
 // per-wave FP64 sample partials (written on sample steps only)
 enum { kSmpStrain = 0, kSmpElemSlots = 1 };
@@ -42,6 +43,7 @@ struct EnergyRecordDev {
     int64_t step;
     double time;
     double strain, kinetic, dissipation, w_ext, w_contact, w_boundary, damping_loss;
+    double fracture;
     double contact_force[3];
 };
 
@@ -56,7 +58,9 @@ struct CaseDev {
     const float* dmi;    // [9][Tpad] Dm^-1, row-major entry k = 3*row + col
     const float* vol;    // [Tpad]
     float* force;        // [T][12] per-tet nodal forces, reference slot layout
-    float* pstate;       // [7][Tpad] plastic strain (6, eng. shears) + eqps
+    float* pstate;       // [7 or 8][Tpad] plastic strain (6, eng. shears), eqps, damage
+    int32_t* tear_step;  // [Tpad] step + 1 at which the tet tore, 0 intact (kFeatDamage)
+    float* tear_stress;  // [6][Tpad] world-frame stress at the tear (kFeatDamage)
     // ---- node arrays
     float4* u4;          // [N] displacement (xyz, w = 0), gathered by the element pass
     float* cu;           // [3][Npad] Kahan compensation of u
@@ -88,6 +92,9 @@ struct CaseDev {
     float yield, hard;
     float gravity, damping;
     float ind_r, ind_kdt2;   // radius, kappa / dt^2
+    float fail_strain;       // regularised eps_f (kFeatDamage)
+    float fail_sens;         // Rice-Tracey sensitivity
+    float fail_eref, fail_ecut;  // reference and cutoff triaxiality
     float pad1[2];
     double lam_d, mu_d;       // FP64 Lame constants for the stable-step evaluation
     // ---- FP64 accounting

@@ -55,6 +55,7 @@ struct HostCase {
     std::vector<float> cu, v, cv;   // [3][Npad]
     // material and loading
     double E = 0, nu = 0, rho = 0, lam = 0, mu = 0, bulk = 0, yield = 0, hard = 0;
+    double fail_strain = 0, fail_sens = 0, fail_eref = 0, fail_ecut = 0;  // kFeatDamage
     double load_ramp = 0, drive_ramp = 0, gravity = 0, damping = 1;
     shipfem_indenter indenter{};
     double dt_request = 0, dt_safety = 0.9;
@@ -129,6 +130,13 @@ hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_
                         hipStream_t stream);
 hipError_t readEnergy(const DeviceCase& dc, shipfem_energy_record* out, int32_t max,
                       int32_t* count, hipStream_t stream);
+// The case's tears (kFeatDamage), ordered by step then tet, normals from the stored
+// world-frame stress.
+hipError_t readTears(const DeviceCase& dc, const HostCase& hc, std::vector<shipfem_tear>& out,
+                     hipStream_t stream);
+// Largest principal direction of a symmetric stress (Voigt, tensor shears), largest
+// component positive.
+void maxPrincipalDirection(const double s6[6], double out[3]);
 
 bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out);
 void setError(char* err, size_t len, const std::string& msg);

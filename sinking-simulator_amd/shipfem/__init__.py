@@ -32,7 +32,14 @@ U32 = C.POINTER(C.c_uint32)
 class Material(C.Structure):
     _fields_ = [("youngs_modulus", C.c_double), ("poisson_ratio", C.c_double),
                 ("density", C.c_double), ("plastic", C.c_int32), ("yield_strength", C.c_double),
-                ("hardening_modulus", C.c_double)]
+                ("hardening_modulus", C.c_double), ("failure_strain", C.c_double),
+                ("triaxiality_sensitivity", C.c_double), ("reference_triaxiality", C.c_double),
+                ("cutoff_triaxiality", C.c_double)]
+
+
+# the reference's Rice-Tracey constants (engine/sim/plasticity.hpp:193-199)
+RICE_TRACEY = dict(sensitivity=1.5, reference_triaxiality=1.0 / 3.0,
+                   cutoff_triaxiality=-1.0 / 3.0)
 
 
 class Indenter(C.Structure):
@@ -59,13 +66,18 @@ class EnergyRecord(C.Structure):
     _fields_ = [("step", C.c_int64), ("time", C.c_double), ("strain", C.c_double),
                 ("kinetic", C.c_double), ("dissipation", C.c_double), ("w_ext", C.c_double),
                 ("w_contact", C.c_double), ("w_boundary", C.c_double),
-                ("damping_loss", C.c_double), ("contact_force", C.c_double * 3)]
+                ("damping_loss", C.c_double), ("fracture", C.c_double),
+                ("contact_force", C.c_double * 3)]
 
 
 class StateView(C.Structure):
     _fields_ = [("displacement", D), ("velocity", D), ("position", F), ("eqps", F),
-                ("plastic_strain", F), ("step", C.c_int64), ("time", C.c_double),
+                ("plastic_strain", F), ("damage", F), ("step", C.c_int64), ("time", C.c_double),
                 ("status", C.c_uint32)]
+
+
+class Tear(C.Structure):
+    _fields_ = [("tet", C.c_uint32), ("step", C.c_int64), ("normal", C.c_double * 3)]
 
 
 VARIANT_TILED, VARIANT_SLOTS = 0, 1
@@ -86,7 +98,8 @@ class Result(C.Structure):
                 ("sim_time", C.c_double), ("submit_s", C.c_double), ("admit_s", C.c_double),
                 ("done_s", C.c_double), ("node_count", C.c_uint32), ("tet_count", C.c_uint32),
                 ("max_displacement", C.c_double), ("max_eqps", C.c_double),
-                ("peak_contact_force", C.c_double), ("last", EnergyRecord)]
+                ("peak_contact_force", C.c_double), ("torn_elements", C.c_uint32),
+                ("torn_volume", C.c_double), ("last", EnergyRecord)]
 
 
 class ServerStats(C.Structure):
@@ -98,7 +111,7 @@ class ServerStats(C.Structure):
 
 
 ENERGY_FIELDS = ["strain", "kinetic", "dissipation", "w_ext", "w_contact", "w_boundary",
-                 "damping_loss"]
+                 "damping_loss", "fracture"]
 
 # every symbol include/shipfem.h declares, with (restype, argtypes)
 VP = C.c_void_p
@@ -128,6 +141,10 @@ SIGNATURES = {
     "shipfem_solver_set_launch": (S, [VP, C.POINTER(LaunchConfig)]),
     "shipfem_solver_profile": (S, [VP, C.c_int32, D, D]),
     "shipfem_solver_footprint": (S, [VP, C.POINTER(C.c_uint64)]),
+    "shipfem_solver_tears": (S, [VP, C.POINTER(Tear), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "shipfem_regularised_failure_strain": (C.c_double, [C.c_double] * 4),
+    "shipfem_torn_groups": (S, [D, U32, C.c_uint32, U32, F, C.c_double, U32, C.c_uint32,
+                                C.POINTER(C.c_uint32)]),
     "shipfem_solver_destroy": (None, [VP]),
     "shipfem_case_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int32]),
     "shipfem_batch_create": (S, [C.POINTER(CaseDesc), C.c_uint32, C.POINTER(LaunchConfig), PP,
@@ -138,6 +155,8 @@ SIGNATURES = {
     "shipfem_batch_readback": (S, [VP, C.c_uint32, C.POINTER(StateView)]),
     "shipfem_batch_energy": (S, [VP, C.c_uint32, C.POINTER(EnergyRecord), C.c_int32,
                                  C.POINTER(C.c_int32)]),
+    "shipfem_batch_tears": (S, [VP, C.c_uint32, C.POINTER(Tear), C.c_uint32,
+                                C.POINTER(C.c_uint32)]),
     "shipfem_batch_reset": (S, [VP]),
     "shipfem_batch_destroy": (None, [VP]),
     "shipfem_server_create": (S, [C.POINTER(ServerOpts), PP, C.c_char_p, C.c_size_t]),
@@ -266,7 +285,7 @@ class Case:
     def __init__(self, X, index, *, flags=None, load=None, prescribed=None, u0=None, v0=None,
                  E=210e9, nu=0.3, rho=7850.0, plastic=False, yield_=355e6, hardening=1e9,
                  indenter=None, load_ramp=0.0, drive_ramp=0.0, gravity=0.0, damping=1.0,
-                 dt=0.0, dt_safety=0.9, sample_every=100, max_steps=0):
+                 dt=0.0, dt_safety=0.9, sample_every=100, max_steps=0, failure=None):
         self.X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
         self.index = np.ascontiguousarray(index, np.uint32).reshape(-1, 4)
         c = lambda a, dt_: None if a is None else np.ascontiguousarray(a, dt_)  # noqa: E731
@@ -282,6 +301,9 @@ class Case:
         self.gravity, self.damping = gravity, damping
         self.dt, self.dt_safety = dt, dt_safety
         self.sample_every, self.max_steps = sample_every, max_steps
+        # ductile failure: dict(failure_strain=..., [sensitivity, reference_triaxiality,
+        # cutoff_triaxiality]) or None
+        self.failure = dict(failure) if failure else None
 
     @classmethod
     def from_mesh(cls, mesh, **kw):
@@ -308,8 +330,12 @@ class Case:
         d.prescribed = _p(self.prescribed, C.c_double)
         d.initial_displacement = _p(self.u0, C.c_double)
         d.initial_velocity = _p(self.v0, C.c_double)
+        f = dict(RICE_TRACEY)
+        f.update(self.failure or {})
         d.material = Material(self.E, self.nu, self.rho, 1 if self.plastic else 0, self.yield_,
-                              self.hardening)
+                              self.hardening, f.get("failure_strain", 0.0) if self.failure else 0.0,
+                              f["sensitivity"], f["reference_triaxiality"],
+                              f["cutoff_triaxiality"])
         if self.indenter:
             ind = self.indenter
             d.indenter = Indenter(1, ind["radius"], (C.c_double * 3)(*ind["center"]),
@@ -330,7 +356,7 @@ class Case:
                     prescribed=self.prescribed, plastic=self.plastic, yield_=self.yield_,
                     hardening=self.hardening, load_ramp=self.load_ramp,
                     drive_ramp=self.drive_ramp, gravity=self.gravity, damping=self.damping,
-                    indenter=ind)
+                    indenter=ind, failure=self.failure)
 
 
 def _records(arr, n):
@@ -343,6 +369,37 @@ def _records(arr, n):
     return out
 
 
+def _tears(call):
+    n = C.c_uint32()
+    _check(call(None, 0, C.byref(n)), "tears")
+    arr = (Tear * max(n.value, 1))()
+    _check(call(arr, n.value, C.byref(n)), "tears")
+    return [(arr[k].tet, arr[k].step, np.array(arr[k].normal[:])) for k in range(n.value)]
+
+
+def regularised_failure_strain(uniform, fracture, length, thickness):
+    """uniform + (fracture - uniform) min(thickness / length, 1) (plasticity.cpp:158-169)."""
+    return lib().shipfem_regularised_failure_strain(uniform, fracture, length, thickness)
+
+
+def torn_groups(X, index, tet_group, damage, fraction=0.5):
+    """Group ids whose torn volume reaches `fraction` of their volume (zone.cpp:1262-1275)."""
+    X = np.ascontiguousarray(X, np.float64)
+    index = np.ascontiguousarray(index, np.uint32)
+    g = np.ascontiguousarray(tet_group, np.uint32)
+    dmg = np.ascontiguousarray(damage, np.float32)
+    n = C.c_uint32()
+    T = len(index)
+    _check(lib().shipfem_torn_groups(_p(X, C.c_double), _p(index, C.c_uint32), T,
+                                     _p(g, C.c_uint32), _p(dmg, C.c_float), fraction, None, 0,
+                                     C.byref(n)))
+    out = np.zeros(max(n.value, 1), np.uint32)
+    _check(lib().shipfem_torn_groups(_p(X, C.c_double), _p(index, C.c_uint32), T,
+                                     _p(g, C.c_uint32), _p(dmg, C.c_float), fraction,
+                                     _p(out, C.c_uint32), n.value, C.byref(n)))
+    return out[:n.value].tolist()
+
+
 class State:
     def __init__(self, N, T):
         self.u = np.zeros((N, 3))
@@ -350,6 +407,7 @@ class State:
         self.position = np.zeros((N, 3), np.float32)
         self.eqps = np.zeros(T, np.float32)
         self.plastic_strain = np.zeros((T, 6), np.float32)
+        self.damage = np.zeros(T, np.float32)
         self.step = 0
         self.time = 0.0
         self.status = 0
@@ -361,6 +419,7 @@ class State:
         v.position = _p(self.position, C.c_float)
         v.eqps = _p(self.eqps, C.c_float)
         v.plastic_strain = _p(self.plastic_strain, C.c_float)
+        v.damage = _p(self.damage, C.c_float)
         return v
 
     def take(self, v):
@@ -428,6 +487,10 @@ class Solver:
         arr = (EnergyRecord * max(n.value, 1))()
         _check(lib().shipfem_solver_energy(self.h, arr, n.value, C.byref(n)))
         return _records(arr, n.value)
+
+    def tears(self):
+        """[(tet, step, normal[3])] ordered by step then tet."""
+        return _tears(lambda out, mx, n: lib().shipfem_solver_tears(self.h, out, mx, n))
 
     def tet_forces(self):
         out = np.zeros((self.case.T, 12), np.float32)
@@ -505,6 +568,9 @@ class Batch:
         _check(lib().shipfem_batch_energy(self.h, i, arr, n.value, C.byref(n)))
         return _records(arr, n.value)
 
+    def tears(self, i):
+        return _tears(lambda out, mx, n: lib().shipfem_batch_tears(self.h, i, out, mx, n))
+
 
 class Server:
     """Resident serving loop: submit cases at any time, poll results."""
@@ -566,4 +632,5 @@ class Server:
 
 
 def case_footprint(N, T, plastic=False, records=64):
-    return lib().shipfem_case_footprint(N, T, 1 if plastic else 0, records)
+    # plastic: False/0 elastic, True/1 J2, 2 J2 with failure
+    return lib().shipfem_case_footprint(N, T, int(plastic), records)

@@ -13,6 +13,10 @@ What is recorded, and from which reference function:
               9.80665, damping 1, dt = criticalTimestep        engine/sim/fem.cpp:187-215
   j2_*        plasticity::update along strain paths, linear hardening, state carried
               from point to point                               engine/sim/plasticity.cpp:319-455
+  fail_*      regularisedFailureStrain, triaxialityFactor      engine/sim/plasticity.cpp:158-174
+  dmg_*       plasticity::update with ductile failure along pull, rotated pull,
+              squeeze, biaxial and mixed paths (damage, failed, failure normal)
+                                                                engine/sim/plasticity.cpp:457-477
 """
 import ctypes as C
 import os
@@ -117,6 +121,79 @@ def main():
         out[f"j2_{name}_yielded"] = np.array(ylds)
         out[f"j2_{name}_iterations"] = np.array(iters)
     out["j2_material"] = np.array([Ej, nuj, sy, H])
+
+    # Ductile damage on the same point law: a regularised failure strain and the
+    # Rice-Tracey constants, state (damage, failed) carried along each path
+    eu, efr, length, thick = 0.12, 0.8, 0.05, 0.02
+    sens, eta_ref, eta_cut = 1.5, 1.0 / 3.0, -1.0 / 3.0
+    ef = R.ref_regularised_failure_strain(eu, efr, length, thick)
+    lengths = np.array([0.005, 0.01, 0.02, 0.05, 0.1, 1.0, 100.0])
+    out["fail_reg_input"] = np.array([eu, efr, thick])
+    out["fail_reg_length"] = lengths
+    out["fail_reg_strain"] = np.array([R.ref_regularised_failure_strain(eu, efr, l, thick)
+                                       for l in lengths])
+    etas = np.array([-0.5, -1.0 / 3.0 + 1e-9, 0.0, 1.0 / 3.0, 0.5, 2.0 / 3.0, 1.0])
+    out["fail_eta"] = etas
+    out["fail_factor"] = np.array([R.ref_triaxiality_factor(sens, eta_ref, eta_cut, e)
+                                   for e in etas])
+    out["fail_material"] = np.array([Ej, nuj, sy, H, ef, sens, eta_ref, eta_cut])
+
+    def pull(plastic, rot=None):
+        sig = sy + H * plastic
+        a = sig / Ej + plastic
+        lat = -nuj * sig / Ej - 0.5 * plastic
+        if rot is None:
+            return np.array([a, lat, lat, 0, 0, 0])
+        T = rot @ np.diag([a, lat, lat]) @ rot.T
+        return np.array([T[0, 0], T[1, 1], T[2, 2], 2 * T[0, 1], 2 * T[1, 2], 2 * T[0, 2]])
+
+    def biaxial(plastic):
+        # sigma = (s, s, 0): flow (1/2, 1/2, -1) eqps, elastic part from Hooke
+        sig = sy + H * plastic
+        a = sig * (1 - nuj) / Ej + 0.5 * plastic
+        c = -2 * nuj * sig / Ej - plastic
+        return np.array([a, a, c, 0, 0, 0])
+
+    ax = np.array([0.31, 0.62, -0.72])
+    ax /= np.linalg.norm(ax)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    rot = np.eye(3) + np.sin(0.9) * K + (1 - np.cos(0.9)) * K @ K
+    fracs = (0.1, 0.4, 0.8, 0.99, 1.005, 1.2, 0.5)
+    dpaths = {
+        # uniaxial tension sits at the reference triaxiality: damage = eps_p / eps_f
+        "pull": [pull(f * ef) for f in fracs],
+        "rotated": [pull(f * ef, rot) for f in fracs],
+        # a superposed hydrostatic squeeze keeps the triaxiality below the cutoff
+        "squeeze": [np.array([-2 * f - 0.004, f - 0.004, f - 0.004, 0, 0, 0])
+                    for f in np.linspace(0.01, 0.1, 8)],
+        # equibiaxial plane stress: eta = 2/3, fails at exp(-1/2) of the uniaxial strain
+        "biaxial": [biaxial(f * ef) for f in (0.1, 0.3, 0.5, 0.6, 0.61, 0.7)],
+        "mixed": [np.array([0.4, -0.3, -0.05, 0.3, -0.2, 0.15]) * f
+                  for f in np.linspace(0.004, 0.8, 12)],
+    }
+    for name, path in dpaths.items():
+        ep = np.zeros(6)
+        q = np.zeros(1)
+        dmg = np.zeros(1)
+        failed = C.c_int(0)
+        rows = {k: [] for k in ("strain", "stress", "eps_p", "eqps", "damage", "failed",
+                                "failed_now", "normal", "eta", "dissipation", "yielded")}
+        for s_ in path:
+            s_ = np.ascontiguousarray(s_, np.float64)
+            stress, d, nrm, eta = np.zeros(6), np.zeros(1), np.zeros(3), np.zeros(1)
+            y = C.c_int()
+            now = R.ref_j2_damage_update(Ej, nuj, sy, H, ef, sens, eta_ref, eta_cut,
+                                         p(s_, C.c_double), p(ep, C.c_double), p(q, C.c_double),
+                                         p(dmg, C.c_double), C.byref(failed),
+                                         p(stress, C.c_double), p(d, C.c_double),
+                                         p(nrm, C.c_double), p(eta, C.c_double), C.byref(y))
+            for k, v in (("strain", s_), ("stress", stress), ("eps_p", ep.copy()), ("eqps", q[0]),
+                         ("damage", dmg[0]), ("failed", failed.value), ("failed_now", now),
+                         ("normal", nrm), ("eta", eta[0]), ("dissipation", d[0]),
+                         ("yielded", y.value)):
+                rows[k].append(np.array(v, copy=True))
+        for k, v in rows.items():
+            out[f"dmg_{name}_{k}"] = np.array(v)
 
     path = os.path.join(HERE, "reference_fem.npz")
     np.savez_compressed(path, **out)

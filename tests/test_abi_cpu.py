@@ -108,3 +108,23 @@ def test_no_device_is_reported_as_skip_not_failure():
     with pytest.raises(sf.ShipfemError) as e:
         sf.Solver(case)
     assert e.value.status == sf.NO_DEVICE
+
+
+def test_failure_strain_and_torn_groups_on_the_host():
+    # regularisation against the oracle (pinned to the compiled reference)
+    for length in (0.005, 0.02, 0.05, 1.0):
+        assert sf.regularised_failure_strain(0.12, 0.8, length, 0.02) == \
+            O.regularised_failure_strain(0.12, 0.8, length, 0.02)
+    # torn groups: two groups of the cantilever's halves; tear all of the first half's
+    # tets and a third of the second's, by volume
+    X, idx = O.box_mesh(1.0, 0.1, 0.1, 40, 4, 4)
+    cx = X[idx].mean(axis=1)[:, 0]
+    group = np.where(cx < 0.5, 7, 3).astype(np.uint32)
+    damage = np.zeros(len(idx), np.float32)
+    damage[cx < 0.5] = 1.0
+    third = np.nonzero(cx >= 0.5)[0][: (cx >= 0.5).sum() // 3]
+    damage[third] = 1.0
+    assert sf.torn_groups(X, idx, group, damage, 0.5) == [7]
+    assert sf.torn_groups(X, idx, group, damage, 0.3) == [3, 7]
+    damage[:] = 0.999
+    assert sf.torn_groups(X, idx, group, damage, 0.01) == []

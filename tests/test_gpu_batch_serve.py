@@ -60,6 +60,41 @@ def test_case_is_bit_identical_alone_and_in_the_mix():
     assert np.array_equal(st.u, state(both, 1)[0]) and s.energy() == state(both, 1)[3]
 
 
+def test_tearing_case_is_bit_identical_alone_and_in_the_mix():
+    # a case with ductile failure joins the mix: the launch then runs the damage
+    # instantiation for everyone, and no case's bits may change for it
+    tear = small_panel(speed=20.0, failure=dict(failure_strain=0.05))
+    cases = mix() + [tear]
+    both = sf.Batch(cases)
+    both.run(600)
+    alone = sf.Batch([tear])
+    alone.run(600)
+    k = len(cases) - 1
+    assert same(state(alone, 0), state(both, k))
+    ta, tb = alone.tears(0), both.tears(k)
+    assert len(ta) > 5 and [(t, s) for t, s, _ in ta] == [(t, s) for t, s, _ in tb]
+    assert all(np.array_equal(x[2], y[2]) for x, y in zip(ta, tb))
+    assert np.array_equal(both.readback(k).damage, alone.readback(0).damage)
+    for i, c in enumerate(mix()):
+        ref = sf.Batch([c])
+        ref.run(600)
+        assert same(state(ref, 0), state(both, i)), i
+        assert both.tears(i) == []
+
+
+def test_server_reports_tears():
+    tear = small_panel(speed=20.0, failure=dict(failure_strain=0.05))
+    srv, ids, res, kept = serve([small_panel(), tear], 4e-4, keep={1})
+    r = res[ids[1]]
+    assert r["status"] == sf.OK and r["torn_elements"] > 5
+    alone = sf.Batch([tear])
+    alone.run(r["steps"])
+    st = alone.readback(0)
+    assert r["torn_elements"] == int((st.damage >= 1.0).sum())
+    assert np.array_equal(kept[ids[1]].u, st.u)
+    assert res[ids[0]]["torn_elements"] == 0
+
+
 @pytest.mark.parametrize("variant", [sf.VARIANT_TILED, sf.VARIANT_SLOTS])
 @pytest.mark.parametrize("shape", [(64, 512, 0), (512, 64, 1), (128, 128, 1)])
 def test_launch_shape_does_not_change_any_bit(shape, variant):

@@ -1,12 +1,16 @@
-"""The measured workloads at their full sizes, through size-independent properties.
+"""The measured workloads at their full sizes.
 
-The FP64 oracle cannot run 1.2M tets x 2000 steps inside a test, so at full size these
-check what holds at any size: bitwise run-to-run determinism, finiteness, the energy
-account's self-consistency (leapfrog conserves the shadow energy of a small-strain elastic
-run), contact engagement and momentum, and alone-vs-mix identity in a full batch. The
-oracle comparison itself runs at a quarter of the large plate (parity tolerance 1e-5).
+Size-independent properties at full size: bitwise run-to-run determinism, finiteness, the
+energy account's self-consistency, contact engagement, alone-vs-mix identity in a full
+batch. Then the full-size configurations against the FP64 oracle itself, run on the host's
+cores with a static split (bit-identical in state to one thread): the large plate for 2000
+steps, the stiffened panel for 4000 steps, and a seeded sample of the 64-case batch, each at
+the parity tolerances (u relative L2 1e-5, eqps 1e-4 absolute, energy residual 1e-4 of the
+work).
 """
+import os
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -32,7 +36,7 @@ def device():
 def residual(r, r0):
     work = r["w_ext"] + r["w_contact"] + r["w_boundary"]
     return work - (r["strain"] + r["kinetic"] - r0["strain"] - r0["kinetic"]) - \
-        r["dissipation"] - r["damping_loss"]
+        r["dissipation"] - r["damping_loss"] - r["fracture"]
 
 
 def test_large_plate_full_size_is_deterministic_and_conserves_energy():
@@ -97,3 +101,65 @@ def test_full_batch_alone_equals_mixed():
         assert np.array_equal(a.u, m.u) and np.array_equal(a.eqps, m.eqps)
         assert alone.energy(0) == b.energy(i)
     assert all(b.readback(i).status == 0 for i in range(64))
+
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+REL_U, ABS_EQPS, ENERGY = 1e-5, 1e-4, 1e-4
+
+
+def oracle_run(case, dt, steps):
+    o = O.Case(case.X, case.index, dt=dt, **case.oracle_kwargs())
+    t0 = time.perf_counter()
+    rec = o.run(steps, sample_every=case.sample_every, threads=THREADS)
+    say(f"  oracle: {case.T} tets x {steps} steps on {THREADS} threads in "
+        f"{time.perf_counter() - t0:.1f} s")
+    return o, rec
+
+
+def energy_gap(g, o):
+    assert [r["step"] for r in g] == [r["step"] for r in o]
+    work = max(abs(o[-1]["w_ext"]) + abs(o[-1]["w_contact"]) + abs(o[-1]["w_boundary"]), 1e-300)
+    return max(abs(residual(a, g[0]) - residual(b, o[0])) for a, b in zip(g[1:], o[1:])) / work
+
+
+def test_large_plate_full_size_matches_oracle():
+    case = configs.plate()
+    s = sf.Solver(case)
+    s.run(2000)
+    st = s.readback()
+    o, rec = oracle_run(case, s.dt, 2000)
+    err = np.linalg.norm(st.u - o.u) / np.linalg.norm(o.u)
+    gap = energy_gap(s.energy(), rec)
+    say(f"plate full size, 2000 steps: u rel L2 {err:.3e}, energy gap/W {gap:.2e}")
+    assert err <= REL_U and gap <= ENERGY
+
+
+def test_stiffened_panel_full_size_matches_oracle():
+    case = configs.stiffened_panel()
+    s = sf.Solver(case)
+    s.run(4000)
+    st = s.readback()
+    o, rec = oracle_run(case, s.dt, 4000)
+    err = np.linalg.norm(st.u - o.u) / np.linalg.norm(o.u)
+    deq = float(np.max(np.abs(st.eqps - o.eqps)))
+    gap = energy_gap(s.energy(), rec)
+    say(f"panel full size, 4000 steps: u rel L2 {err:.3e}, max |d eqps| {deq:.2e} "
+        f"(max eqps {o.eqps.max():.3e}), energy gap/W {gap:.2e}")
+    assert o.eqps.max() > 1e-3
+    assert err <= REL_U and deq <= ABS_EQPS and gap <= ENERGY
+
+
+def test_full_batch_sample_matches_oracle():
+    cases = configs.batch_cases(64)
+    b = sf.Batch(cases)
+    b.run(1000)
+    rng = np.random.default_rng(2026)
+    for i in sorted(rng.choice(64, size=4, replace=False).tolist()):
+        st = b.readback(i)
+        o, rec = oracle_run(cases[i], st.time / st.step, 1000)
+        err = np.linalg.norm(st.u - o.u) / np.linalg.norm(o.u)
+        deq = float(np.max(np.abs(st.eqps - o.eqps)))
+        gap = energy_gap(b.energy(i), rec)
+        say(f"batch case {i} ({cases[i].T} tets), 1000 steps: u rel L2 {err:.3e}, "
+            f"max |d eqps| {deq:.2e}, energy gap/W {gap:.2e}")
+        assert err <= REL_U and deq <= ABS_EQPS and gap <= ENERGY

@@ -49,7 +49,7 @@ def rel_l2(a, b):
 def residual(r, r0):
     work = r["w_ext"] + r["w_contact"] + r["w_boundary"]
     return work - (r["strain"] + r["kinetic"] - r0["strain"] - r0["kinetic"]) - \
-        r["dissipation"] - r["damping_loss"]
+        r["dissipation"] - r["damping_loss"] - r["fracture"]
 
 
 def check_energy(gpu_rec, ora_rec, tol=ENERGY):
@@ -188,6 +188,42 @@ def test_plastic_indentation_matches_oracle():
             f"contact fz {ora[-1]['contact_force'][2]:.1f} N")
         assert err_u <= REL_U
         assert err_p <= ABS_EQPS
+
+
+def test_tearing_matches_oracle():
+    # the same indentation with ductile failure: tets tear under the sphere. A tear is
+    # a threshold event, so an FP32 damage that crosses 1 a step before or after the
+    # FP64 one moves that tear by a step; the checks are on the torn set, the tear
+    # steps and planes, and the state, at tolerances that allow exactly that
+    case = small_panel(speed=10.0, failure=dict(failure_strain=0.08))
+    n = 1200
+    for v in VARIANTS:
+        s = solver(case, v)
+        o = oracle_for(case, s.dt)
+        ora = o.run(n, sample_every=case.sample_every)
+        s.run(n)
+        st = s.readback()
+        tears = s.tears()
+        torn_o = set(np.nonzero(o.damage >= 1.0)[0].tolist())
+        torn_g = {t for t, _, _ in tears}
+        assert len(torn_o) > 10, "the case is built to tear"
+        assert torn_g == set(np.nonzero(st.damage >= 1.0)[0].tolist())
+        common = sorted(torn_o & torn_g)
+        dstep = np.array([abs(step - o.tear_step[t]) for t, step, _ in tears if t in torn_o])
+        gn = {t: nrm for t, _, nrm in tears}
+        dots = np.array([abs(float(np.dot(gn[t], o.tear_normal[t]))) for t in common])
+        err_u = rel_l2(st.u, o.u)
+        live = (o.damage < 1.0) & (st.damage < 1.0)
+        err_d = float(np.max(np.abs(st.damage[live] - o.damage[live])))
+        worst = check_energy(s.energy(), ora)
+        say(f"tearing {VNAME[v]}: torn {len(torn_g)} gpu / {len(torn_o)} oracle, "
+            f"sym diff {len(torn_o ^ torn_g)}, tear step |d| max {dstep.max()} mean "
+            f"{dstep.mean():.2f}, normal |dot| min {dots.min():.4f}, u rel L2 {err_u:.2e}, "
+            f"damage |d| {err_d:.2e}, fracture {ora[-1]['fracture']:.3e} J, energy {worst:.2e}")
+        assert len(torn_o ^ torn_g) <= max(2, len(torn_o) // 50)
+        assert np.mean(dstep <= 1) >= 0.95 and dstep.max() <= 5
+        assert np.mean(dots >= 0.999) >= 0.95
+        assert err_u <= REL_U and err_d <= ABS_EQPS and worst <= ENERGY
 
 
 @pytest.mark.parametrize("variant", VARIANTS)

@@ -130,3 +130,30 @@ def test_j2_matches_reference_point_law(golden, path):
     # linear hardening: the reference's Newton loop takes exactly two evaluations
     it = golden[f"j2_{path}_iterations"]
     assert set(it[golden[f"j2_{path}_yielded"] == 1].tolist()) <= {2}
+
+
+def test_failure_strain_regularisation_and_triaxiality_match_reference(golden):
+    eu, efr, thick = golden["fail_reg_input"]
+    for length, ref in zip(golden["fail_reg_length"], golden["fail_reg_strain"]):
+        assert O.regularised_failure_strain(eu, efr, length, thick) == pytest.approx(ref, rel=1e-15)
+    for eta, ref in zip(golden["fail_eta"], golden["fail_factor"]):
+        got = O.triaxiality_factor(eta)
+        assert (np.isinf(got) and np.isinf(ref)) or got == pytest.approx(ref, rel=1e-14)
+
+
+@pytest.mark.parametrize("path", ["pull", "rotated", "squeeze", "biaxial", "mixed"])
+def test_j2_damage_matches_reference_point_law(golden, path):
+    # plasticity::update with ductile failure (plasticity.cpp:457-477), state carried
+    Ej, nuj, sy, H, ef, sens, eref, ecut = golden["fail_material"]
+    ep, q, dmg = np.zeros(6), 0.0, 0.0
+    for i, s in enumerate(golden[f"dmg_{path}_strain"]):
+        r = O.j2_update_damage(Ej, nuj, sy, H, ef, s, ep, q, dmg, sens, eref, ecut)
+        ep, q, dmg = r["eps_p"], r["eqps"], r["damage"]
+        assert (dmg >= 1.0) == bool(golden[f"dmg_{path}_failed"][i])
+        assert r["failed_now"] == bool(golden[f"dmg_{path}_failed_now"][i])
+        assert dmg == pytest.approx(golden[f"dmg_{path}_damage"][i], rel=1e-12, abs=1e-15)
+        assert np.max(np.abs(r["stress"] - golden[f"dmg_{path}_stress"][i])) <= 1e-9 * sy
+        assert q == pytest.approx(golden[f"dmg_{path}_eqps"][i], rel=1e-12, abs=1e-15)
+        if r["failed_now"]:
+            # an eigenvector's sign is fixed by its largest component in both
+            assert np.allclose(r["normal"], golden[f"dmg_{path}_normal"][i], atol=1e-9)
