@@ -131,7 +131,7 @@ uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t
     uint64_t b = 16ull * T + 40ull * Tp + 48ull * T + pstate + 16ull * N +
                  36ull * Np + 12ull * Np + 40ull * Np + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
                  8ull * T + 4ull * (nt + 1) + 4ull * P + 2ull * (P + nt) +
-                 2ull * 4 * kTile * nt + 16ull * P + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
+                 16ull * P + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
                  8ull * ((kAccElemSlots + kSmpElemSlots + 1) * ew + (kAccNodeSlots + kSmpNodeSlots) * nw) +
                  24 + sizeof(EnergyRecordDev) * (uint64_t)std::max(cap, 1);
     return b + 256ull * 40;  // each of the arena's planes rounds up to 256 bytes
@@ -436,7 +436,6 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
         std::vector<uint32_t>().swap(hc.conn);
         std::vector<uint32_t>().swap(hc.lconn);
         std::vector<uint32_t>().swap(hc.tnode);
-        std::vector<uint16_t>().swap(hc.lent);
         std::vector<uint32_t>().swap(hc.pell);
     }
     *out = b;

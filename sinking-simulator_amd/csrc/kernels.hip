@@ -341,16 +341,20 @@ __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ case
 }
 
 // ---------------------------------------------------------------- element pass, tiled
-// One block per 256-tet tile. LDS: corner forces [12][256] floats, the tile-local CSR
-// (entries [1024] and offsets [lmax + 1], 16-bit), then the tile's local nodes'
-// displacement float4[lmax]. Dm^-1 and V stream as planes (the kernel is ALU-bound;
-// re-deriving them in FP64 from rest coordinates cost more issue slots than the 40
-// bytes per tet it saved). Every global load a lane needs is issued before the first
-// barrier, so the staging gather, the streamed planes and the CSR share one latency.
-constexpr size_t tiled_lds_csr_offset() { return 12 * kTile * sizeof(float); }
+// One block per 256-tet tile. The tile's local nodes are numbered by descending corner
+// count and each has a contiguous run of 4-kTile corner slots, in (tet, corner) order,
+// starting at its offset lofs[l]. A tet's connectivity word carries, per corner, the
+// local node (10 bits) and the corner's rank in that node's run (6 bits), so a tet
+// lane writes each corner force straight into its slot and a node lane sums one
+// contiguous run: the tile needs no separate node->corner map.
+// LDS: corner-slot forces [3][4 kTile] floats (component planes), the run offsets
+// [lmax + 1] (16-bit), then the tile's local nodes' displacement float4[lmax]. Dm^-1 and
+// V stream as planes. Every global load a lane needs is issued before the first
+// barrier, so the staging gather and the streamed planes share one latency.
+constexpr uint32_t kSlots4 = 4 * kTile;  // corner slots of a tile
+__host__ __device__ constexpr size_t tiled_lds_offsets_offset() { return 3 * kSlots4 * sizeof(float); }
 __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
-    return 12 * kTile * sizeof(float) + 4 * kTile * sizeof(uint16_t) +
-           ((lmax + 1) * sizeof(uint16_t) + 15) / 16 * 16;
+    return tiled_lds_offsets_offset() + ((lmax + 1) * sizeof(uint16_t) + 15) / 16 * 16;
 }
 
 template <bool PLASTIC, bool DAMAGE>
@@ -360,8 +364,7 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
                                                       int xcd) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* sh_f = reinterpret_cast<float*>(smem);
-    uint16_t* sh_le = reinterpret_cast<uint16_t*>(smem + tiled_lds_csr_offset());
-    uint16_t* sh_lo = sh_le + 4 * kTile;
+    uint16_t* sh_lo = reinterpret_cast<uint16_t*>(smem + tiled_lds_offsets_offset());
     float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
 
     const int b = remap_block(blockIdx.x, gridDim.x, xcd);
@@ -375,8 +378,7 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     const uint32_t e = tile.first + threadIdx.x;
     const bool live = e < c.T;
 
-    // streamed per-tet inputs and this lane's share of the tile CSR, all in flight
-    // together with the staging gather below
+    // streamed per-tet inputs, all in flight together with the staging gather below
     uint2 lc = make_uint2(0u, 0u);
     float B[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float V = 0.0f;
@@ -386,41 +388,58 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
         for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * c.Tpad + e];
         V = c.vol[e];
     }
-    const uint2 ent = reinterpret_cast<const uint2*>(c.lent + (size_t)tb * (4 * kTile))[threadIdx.x];
     const uint16_t* lo = c.lofs + p0 + tb;
     for (uint32_t l = threadIdx.x; l <= L; l += kTile) sh_lo[l] = lo[l];
     for (uint32_t l = threadIdx.x; l < L; l += kTile) sh_u[l] = c.u4[c.tnode[p0 + l]];
-    reinterpret_cast<uint2*>(sh_le)[threadIdx.x] = ent;
     __syncthreads();
 
     double diss = 0.0, se = 0.0, fract = 0.0;
     if (live) {
-        const uint32_t l0 = lc.x & 0xFFFFu, l1 = lc.x >> 16, l2 = lc.y & 0xFFFFu, l3 = lc.y >> 16;
+        const uint32_t w4[4] = {lc.x & 0xFFFFu, lc.x >> 16, lc.y & 0xFFFFu, lc.y >> 16};
+        uint32_t ln[4], slot[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ln[q] = w4[q] & 1023u;
+            slot[q] = sh_lo[ln[q]] + (w4[q] >> 10);
+        }
         float f[12];
         bool tore;
-        element_force<PLASTIC, DAMAGE>(c, e, c.step[parity], sh_u[l0], sh_u[l1], sh_u[l2], sh_u[l3],
-                                       B, V, sample, f, diss, se, fract, tore);
+        element_force<PLASTIC, DAMAGE>(c, e, c.step[parity], sh_u[ln[0]], sh_u[ln[1]], sh_u[ln[2]],
+                                       sh_u[ln[3]], B, V, sample, f, diss, se, fract, tore);
         if (DAMAGE && tore) {
-            const uint32_t g[4] = {c.tnode[p0 + l0], c.tnode[p0 + l1], c.tnode[p0 + l2],
-                                   c.tnode[p0 + l3]};
+            const uint32_t g[4] = {c.tnode[p0 + ln[0]], c.tnode[p0 + ln[1]], c.tnode[p0 + ln[2]],
+                                   c.tnode[p0 + ln[3]]};
             release_torn(c, g, f, fract);
         }
 #pragma unroll
-        for (int k = 0; k < 12; ++k) sh_f[k * kTile + threadIdx.x] = f[k];
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sh_f[a * kSlots4 + slot[q]] = f[q * 3 + a];
     }
     element_wave_sums<PLASTIC, DAMAGE>(c, e, sample, diss, se, fract);
     __syncthreads();
 
-    // one lane per local node sums its corners in the tile's (tet, corner) order
+    // one lane per local node sums its run of corner slots in order, four per round
+    // (all reads of a round in flight together). Local nodes are numbered by
+    // descending count, so the lanes of a wave run similar numbers of rounds.
     for (uint32_t l = threadIdx.x; l < L; l += kTile) {
         const uint32_t q0 = sh_lo[l], q1 = sh_lo[l + 1];
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        for (uint32_t q = q0; q < q1; ++q) {
-            const uint32_t en = sh_le[q];
-            const uint32_t t = en >> 2, cr = (en & 3u) * 3u;
-            fx += sh_f[(cr + 0) * kTile + t];
-            fy += sh_f[(cr + 1) * kTile + t];
-            fz += sh_f[(cr + 2) * kTile + t];
+        for (uint32_t q = q0; q < q1; q += 4) {
+            float g[4][3];
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t s = min(q + k, q1 - 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) g[k][a] = sh_f[a * kSlots4 + s];
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k)
+                if (q + k < q1) {
+                    fx += g[k][0];
+                    fy += g[k][1];
+                    fz += g[k][2];
+                }
         }
         c.partial[p0 + l] = make_float4(fx, fy, fz, 0.0f);
     }

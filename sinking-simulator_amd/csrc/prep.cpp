@@ -35,7 +35,7 @@ struct Layout {
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
     uint64_t conn, dmi, vol, force, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
-        ell_off, ell, lconn, tn_off, tnode, lofs, lent, partial, pell_off, pell, acc_e, acc_n,
+        ell_off, ell, lconn, tn_off, tnode, lofs, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
 
@@ -67,7 +67,6 @@ ArenaMap mapArena(const HostCase& hc) {
     m.tn_off = L.take(4 * (uint64_t)hc.tn_off.size());
     m.tnode = L.take(4 * (uint64_t)hc.tnode.size());
     m.lofs = L.take(2 * (uint64_t)hc.lofs.size());
-    m.lent = L.take(2 * (uint64_t)hc.lent.size());
     m.partial = L.take(16 * (uint64_t)hc.P);
     m.pell_off = L.take(4 * (uint64_t)hc.pell_off.size());
     m.pell = L.take(4 * (uint64_t)hc.pell.size());
@@ -333,18 +332,21 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
                 hc.ell[hc.ell_off[w] + (q - off[n]) * kWave + l] = ent[q];
         }
 
-    // 256-tet tiles of the tiled element pass: each tile's unique nodes in ascending
-    // id (one partial slot each), 16-bit local connectivity, and a tile-local CSR of
-    // (tet_local*4 + corner) in (tet, corner) order
+    // 256-tet tiles of the tiled element pass: each tile's unique nodes (one partial
+    // slot each), numbered by descending corner count (ties by ascending id) so the
+    // lanes of one wave of the in-tile sum run similar trip counts. Each local node
+    // owns a contiguous run of the tile's corner slots, in (tet, corner) order, from
+    // its offset lofs[l]; a tet's connectivity packs, per corner, the local node (10
+    // bits) and the corner's rank in that run (6 bits). A node's run, and so its sum,
+    // does not depend on its local number.
     {
         const uint32_t nt = (T + kTile - 1) / kTile;
         hc.elem_tiles = nt;
         hc.lconn.assign(2ull * T, 0u);
         hc.tn_off.assign(nt + 1, 0u);
-        hc.lent.assign(4ull * kTile * nt, 0u);
         hc.tnode.clear();
         hc.lofs.clear();
-        std::vector<uint32_t> stamp(N, 0xFFFFFFFFu), loc(N, 0u);
+        std::vector<uint32_t> stamp(N, 0xFFFFFFFFu), loc(N, 0u), deg(N, 0u), seen(N, 0u);
         std::vector<uint32_t> nodes;
         std::vector<uint16_t> cnt;
         for (uint32_t b = 0; b < nt; ++b) {
@@ -355,30 +357,35 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
                     const uint32_t g = d.index[4ull * t + c];
                     if (stamp[g] != b) {
                         stamp[g] = b;
+                        deg[g] = 0;
+                        seen[g] = 0;
                         nodes.push_back(g);
                     }
+                    ++deg[g];
                 }
-            std::sort(nodes.begin(), nodes.end());
+            std::sort(nodes.begin(), nodes.end(), [&](uint32_t a, uint32_t c) {
+                return deg[a] != deg[c] ? deg[a] > deg[c] : a < c;
+            });
             const uint32_t L = static_cast<uint32_t>(nodes.size());
+            if (deg[nodes[0]] > 64)
+                return "a node belongs to more than 64 tets of one 256-tet tile (node " +
+                       std::to_string(nodes[0]) + ")";
             for (uint32_t l = 0; l < L; ++l) loc[nodes[l]] = l;
             hc.tn_off[b + 1] = hc.tn_off[b] + L;
             hc.lmax = std::max(hc.lmax, L);
             hc.tnode.insert(hc.tnode.end(), nodes.begin(), nodes.end());
-            for (uint32_t t = t0; t < t1; ++t) {
-                const uint32_t* v = &d.index[4ull * t];
-                hc.lconn[2ull * t] = loc[v[0]] | (loc[v[1]] << 16);
-                hc.lconn[2ull * t + 1] = loc[v[2]] | (loc[v[3]] << 16);
-            }
             cnt.assign(L + 1, 0);
-            for (uint32_t t = t0; t < t1; ++t)
-                for (int c = 0; c < 4; ++c) ++cnt[loc[d.index[4ull * t + c]] + 1];
-            for (uint32_t l = 0; l < L; ++l) cnt[l + 1] = static_cast<uint16_t>(cnt[l + 1] + cnt[l]);
+            for (uint32_t l = 0; l < L; ++l) cnt[l + 1] = static_cast<uint16_t>(cnt[l] + deg[nodes[l]]);
             hc.lofs.insert(hc.lofs.end(), cnt.begin(), cnt.end());
-            std::vector<uint16_t> cursor(cnt.begin(), cnt.end() - 1);
-            uint16_t* le = &hc.lent[static_cast<size_t>(b) * 4 * kTile];
-            for (uint32_t t = t0; t < t1; ++t)
-                for (uint32_t c = 0; c < 4; ++c)
-                    le[cursor[loc[d.index[4ull * t + c]]]++] = static_cast<uint16_t>((t - t0) * 4 + c);
+            for (uint32_t t = t0; t < t1; ++t) {
+                uint32_t wv[4];
+                for (int c = 0; c < 4; ++c) {
+                    const uint32_t g = d.index[4ull * t + c];
+                    wv[c] = loc[g] | (seen[g]++ << 10);
+                }
+                hc.lconn[2ull * t] = wv[0] | (wv[1] << 16);
+                hc.lconn[2ull * t + 1] = wv[2] | (wv[3] << 16);
+            }
         }
         hc.P = hc.tn_off[nt];
         // node -> its partial slots in tile order, wave-transposed like the CSR slab
@@ -466,7 +473,6 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.tn_off = reinterpret_cast<const uint32_t*>(base + m.tn_off);
     r.tnode = reinterpret_cast<const uint32_t*>(base + m.tnode);
     r.lofs = reinterpret_cast<const uint16_t*>(base + m.lofs);
-    r.lent = reinterpret_cast<const uint16_t*>(base + m.lent);
     r.partial = reinterpret_cast<float4*>(base + m.partial);
     r.pell_off = reinterpret_cast<const uint32_t*>(base + m.pell_off);
     r.pell = reinterpret_cast<const uint32_t*>(base + m.pell);
@@ -525,7 +531,6 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.tn_off, hc.tn_off.data(), 4ull * hc.tn_off.size());
     up(m.tnode, hc.tnode.data(), 4ull * hc.tnode.size());
     up(m.lofs, hc.lofs.data(), 2ull * hc.lofs.size());
-    up(m.lent, hc.lent.data(), 2ull * hc.lent.size());
     up(m.pell_off, hc.pell_off.data(), 4ull * hc.pell_off.size());
     up(m.pell, hc.pell.data(), 4ull * hc.pell.size());
     return e;

@@ -75,11 +75,12 @@ struct CaseDev {
     const uint32_t* ell_off;  // [node_waves + 1] first entry of each wave's slab
     const uint32_t* ell;      // wave slabs, entry e of lane l at ell_off[w] + 64 e + l
     // ---- tiled element pass: 256-tet tiles staged through LDS
-    const uint2* lconn;       // [T] four 16-bit tile-local node indices per tet
+    const uint2* lconn;       // [T] per corner (16 bits): tile-local node | rank << 10, the
+                              // corner's place in that node's run of corner slots
     const uint32_t* tn_off;   // [elem_tiles + 1] first (tile, local node) slot of each tile
     const uint32_t* tnode;    // [P] global node id of each (tile, local node) slot
-    const uint16_t* lofs;     // [P + elem_tiles] per tile: L+1 offsets into its entries
-    const uint16_t* lent;     // [elem_tiles][1024] tile-local CSR: tet_local*4 + corner
+    const uint16_t* lofs;     // [P + elem_tiles] per tile: L+1 run offsets into its
+                              // 4 kTile corner slots
     float4* partial;          // [P] force sum of each (tile, local node)
     const uint32_t* pell_off; // [node_waves + 1] node -> partial slots, wave slabs
     const uint32_t* pell;

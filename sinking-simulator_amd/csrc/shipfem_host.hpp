@@ -44,11 +44,10 @@ struct HostCase {
     std::vector<uint32_t> ell;
     // tiled element pass (256-tet tiles)
     uint32_t elem_tiles = 0, P = 0, lmax = 0;
-    std::vector<uint32_t> lconn;    // [2T] packed 16-bit local ids
+    std::vector<uint32_t> lconn;    // [2T] per corner: local node | rank in its run << 10
     std::vector<uint32_t> tn_off;   // [elem_tiles + 1]
     std::vector<uint32_t> tnode;    // [P]
     std::vector<uint16_t> lofs;     // [P + elem_tiles]
-    std::vector<uint16_t> lent;     // [elem_tiles * 1024]
     std::vector<uint32_t> pell_off; // [node_waves + 1]
     std::vector<uint32_t> pell;
     std::vector<float> u4;          // [4N]
