@@ -43,6 +43,34 @@ __device__ __forceinline__ int remap_block(int b, int nb, int xcd) {
     return x * q + (x < r ? x : r) + j;
 }
 
+// A load the wave issues on the scalar path (through the constant cache). Only for
+// wave-uniform addresses of data that no lane writes during the kernel; the cache is
+// invalidated at every kernel start, so values written by earlier launches are seen.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+    return *reinterpret_cast<const __attribute__((address_space(4))) T*>(reinterpret_cast<uintptr_t>(p));
+}
+
+// The same pointer in the global address space: its loads are vector-memory only (not
+// flat), so the compiler can wait for one of several loads in flight.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+    return reinterpret_cast<const __attribute__((address_space(1))) T*>(reinterpret_cast<uintptr_t>(p));
+}
+
+// Vector-typed loads through the global address space (HIP's vector structs cannot be
+// copied from an address-space-qualified reference, their native vectors can).
+__device__ __forceinline__ float4 gload(const float4* p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 x = *gmem(reinterpret_cast<const v4*>(p));
+    return make_float4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ uint2 gload(const uint2* p) {
+    typedef uint32_t v2 __attribute__((ext_vector_type(2)));
+    const v2 x = *gmem(reinterpret_cast<const v2*>(p));
+    return make_uint2(x.x, x.y);
+}
+
 __device__ __forceinline__ double wave_sum(double s) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -370,27 +398,44 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     const int b = remap_block(blockIdx.x, gridDim.x, xcd);
     const Tile tile = tiles[b];
     const CaseDev& c = cases[tile.case_index];
-    if (c.step[parity] >= c.step_end) return;
-
     const uint32_t tb = tile.first / kTile;
-    const uint32_t p0 = c.tn_off[tb];
-    const uint32_t L = c.tn_off[tb + 1] - p0;
+    const int64_t n = ld_uniform(c.step + parity);
+    const uint32_t p0 = ld_uniform(c.tn_off + tb);
+    const uint32_t L = ld_uniform(c.tn_off + tb + 1) - p0;
+    if (n >= c.step_end) return;  // the case has finished: whole block idles
     const uint32_t e = tile.first + threadIdx.x;
     const bool live = e < c.T;
 
-    // streamed per-tet inputs, all in flight together with the staging gather below
-    uint2 lc = make_uint2(0u, 0u);
-    float B[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float V = 0.0f;
-    if (live) {
-        lc = c.lconn[e];
+    // Loads in the order their chains need them: the staging gather's node ids head
+    // the longest chain (ids, then displacements), then the run offsets, then the
+    // streamed per-tet planes, so waiting for the ids never waits for the stream.
+    // Indices are clamped into range instead of branched on, so every load is issued
+    // unconditionally; values a lane does not own are discarded. A lane stages local
+    // nodes threadIdx.x and threadIdx.x + kTile; tiles with more (not seen in
+    // Kuhn-type meshes) stage the rest afterwards.
+    const uint32_t la = threadIdx.x, lb = threadIdx.x + kTile;
+    const auto* lo = gmem(c.lofs + p0 + tb);
+    const auto* tn = gmem(c.tnode + p0);
+    const uint32_t ga = tn[min(la, L - 1)];
+    const uint32_t gb = tn[min(lb, L - 1)];
+    const uint32_t oa = lo[min(la, L)];
+    const uint32_t ob = lo[min(lb, L)];
+    const uint32_t ec = min(e, c.T - 1);
+    const uint2 lc = gload(c.lconn + ec);
+    float B[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) B[k] = c.dmi[k * c.Tpad + e];
-        V = c.vol[e];
-    }
-    const uint16_t* lo = c.lofs + p0 + tb;
-    for (uint32_t l = threadIdx.x; l <= L; l += kTile) sh_lo[l] = lo[l];
-    for (uint32_t l = threadIdx.x; l < L; l += kTile) sh_u[l] = c.u4[c.tnode[p0 + l]];
+    for (int k = 0; k < 9; ++k) B[k] = gmem(c.dmi)[k * c.Tpad + ec];
+    const float V = gmem(c.vol)[ec];
+    const float4 ua = gload(c.u4 + ga);
+    const float4 ub = gload(c.u4 + gb);
+    // lanes past the tile's nodes rewrite the last entry with the value it already
+    // holds, so no store waits behind a branch
+    sh_lo[min(la, L)] = static_cast<uint16_t>(oa);
+    sh_lo[min(lb, L)] = static_cast<uint16_t>(ob);
+    sh_u[min(la, L - 1)] = ua;
+    sh_u[min(lb, L - 1)] = ub;
+    for (uint32_t l = threadIdx.x + 2 * kTile; l <= L; l += kTile) sh_lo[l] = lo[l];
+    for (uint32_t l = threadIdx.x + 2 * kTile; l < L; l += kTile) sh_u[l] = c.u4[c.tnode[p0 + l]];
     __syncthreads();
 
     double diss = 0.0, se = 0.0, fract = 0.0;
@@ -404,7 +449,7 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
         }
         float f[12];
         bool tore;
-        element_force<PLASTIC, DAMAGE>(c, e, c.step[parity], sh_u[ln[0]], sh_u[ln[1]], sh_u[ln[2]],
+        element_force<PLASTIC, DAMAGE>(c, e, n, sh_u[ln[0]], sh_u[ln[1]], sh_u[ln[2]],
                                        sh_u[ln[3]], B, V, sample, f, diss, se, fract, tore);
         if (DAMAGE && tore) {
             const uint32_t g[4] = {c.tnode[p0 + ln[0]], c.tnode[p0 + ln[1]], c.tnode[p0 + ln[2]],
@@ -453,15 +498,26 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
     const int b = remap_block(blockIdx.x, gridDim.x, xcd);
     const Tile tile = tiles[b];
     const CaseDev& c = cases[tile.case_index];
-    const int64_t n = c.step[parity];
-    const bool frozen = n >= c.step_end;
-    if (tile.first == 0 && threadIdx.x == 0) c.step[parity ^ 1] = frozen ? n : n + 1;
-    if (frozen) return;
-
     const uint32_t i = tile.first + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     // tiles start on multiples of 64 nodes, so the wave index is wave-uniform
     const uint32_t w = __builtin_amdgcn_readfirstlane(i >> 6);
+    const int64_t n = ld_uniform(c.step + parity);
+    // this wave's gather slab (fixed order: slots in CSR (tet, corner) order, tiled in
+    // tile order). A block may overhang the case's last node: waves past its
+    // node_waves slabs have no offsets (off holds node_waves + 1 entries) and gather
+    // nothing.
+    const uint32_t* off = VARIANT == kSlots ? c.ell_off : c.pell_off;
+    const uint32_t* slab = VARIANT == kSlots ? c.ell : c.pell;
+    uint32_t base = 0, width = 0;
+    if (w < c.node_waves) {
+        base = ld_uniform(off + w);
+        width = (ld_uniform(off + w + 1) - base) >> 6;
+    }
+    const bool frozen = n >= c.step_end;
+    if (tile.first == 0 && threadIdx.x == 0) c.step[parity ^ 1] = frozen ? n : n + 1;
+    if (frozen) return;
+
     const bool live = i < c.N;
     const double dt = c.dt;
     const float dtf = c.dtf;
@@ -471,72 +527,69 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
     double s_kin = 0.0, s_hext = 0.0, s_hcon = 0.0, s_hbnd = 0.0, s_hdamp = 0.0;
     double s_f[3] = {0.0, 0.0, 0.0};
 
-    // Every load that does not depend on the gather is issued first, so a wave waits
-    // for one round trip for them and two for the gather (indices, then data) instead
-    // of a chain of dependent loads. Padding lanes read node 0 and discard it.
+    // The gather's slab indices go out first (they head the longest chain: index, then
+    // data), then every load that does not depend on the gather, then the gathered
+    // data; a wave then waits about two memory round trips. Rows are clamped into
+    // the slab and padding lanes read node 0, so every load is issued unconditionally
+    // and values a lane does not own are discarded. A lane's column ends in
+    // sentinels, which add nothing.
+    constexpr uint32_t kGather = 8;
+    // a wave with an empty slab reads the case's first 64 flags instead and turns
+    // them into sentinels (nflags holds Npad >= 64 entries)
+    const auto* sl = width > 0 ? gmem(slab) + base + lane : gmem(c.nflags) + lane;
+    const uint32_t last = width > 0 ? width - 1 : 0;
+    const uint32_t none = width > 0 ? 0u : kEllSentinel;
+    uint32_t idx[kGather];
+#pragma unroll
+    for (uint32_t k = 0; k < kGather; ++k) idx[k] = sl[min(k, last) * 64u] | none;
     const uint32_t ii = live ? i : 0u;
-    const uint32_t fl = live ? c.nflags[ii] : kPinAll;
-    const float m = c.mass[ii];
-    const float im = c.invm[ii];
-    const float4 u = c.u4[ii];
-    float vv[3], cvv[3], cuu[3], ld[3] = {0.0f, 0.0f, 0.0f};
+    const uint32_t flr = gmem(c.nflags)[ii];
+    const float m = gmem(c.mass)[ii];
+    const float im = gmem(c.invm)[ii];
+    const float4 u = gload(c.u4 + ii);
+    // without a load plane the velocity plane is read in its place and never used
+    const float* lplane = (c.features & kFeatLoad) ? c.load : c.v;
+    float vv[3], cvv[3], cuu[3], ld[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        vv[a] = c.v[a * np + ii];
-        cvv[a] = c.cv[a * np + ii];
-        cuu[a] = c.cu[a * np + ii];
+        vv[a] = gmem(c.v)[a * np + ii];
+        cvv[a] = gmem(c.cv)[a * np + ii];
+        cuu[a] = gmem(c.cu)[a * np + ii];
+        ld[a] = gmem(lplane)[a * np + ii];
     }
-    if (c.features & kFeatLoad) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ld[a] = c.load[a * np + ii];
-    }
+    const uint32_t fl = live ? flr : kPinAll;
 
-    // gather in the fixed order (slots: CSR (tet, corner); tiled: tile order), in
-    // batches of kGather slab rows: all index loads, then all data loads, then the
-    // adds in order. A lane's column ends in sentinels, which add nothing.
-    constexpr uint32_t kGather = 8;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    {
-        const uint32_t* off = VARIANT == kSlots ? c.ell_off : c.pell_off;
-        const uint32_t* slab = VARIANT == kSlots ? c.ell : c.pell;
-        // a block may overhang the case's last node: waves past its node_waves slabs
-        // have no offsets (off holds node_waves + 1 entries) and gather nothing
-        uint32_t base = 0, width = 0;
-        if (w < c.node_waves) {
-            base = off[w];
-            width = (off[w + 1] - base) >> 6;
-        }
-        for (uint32_t q0 = 0; q0 < width; q0 += kGather) {
-            uint32_t idx[kGather];
+    auto gather = [&](uint32_t q0) {
+        float g[kGather][3];
 #pragma unroll
-            for (uint32_t k = 0; k < kGather; ++k)
-                idx[k] = q0 + k < width ? slab[base + (q0 + k) * 64u + lane] : kEllSentinel;
-            float g[kGather][3];
-#pragma unroll
-            for (uint32_t k = 0; k < kGather; ++k) {
-                g[k][0] = g[k][1] = g[k][2] = 0.0f;
-                if (idx[k] != kEllSentinel) {
-                    if (VARIANT == kSlots) {
-                        const float* sl = c.force + (size_t)(idx[k] >> 2) * 12u + (idx[k] & 3u) * 3u;
-                        g[k][0] = sl[0];
-                        g[k][1] = sl[1];
-                        g[k][2] = sl[2];
-                    } else {
-                        const float4 pv = c.partial[idx[k]];
-                        g[k][0] = pv.x;
-                        g[k][1] = pv.y;
-                        g[k][2] = pv.z;
-                    }
-                }
+        for (uint32_t k = 0; k < kGather; ++k) {
+            const uint32_t x = idx[k] == kEllSentinel ? 0u : idx[k];
+            if (VARIANT == kSlots) {
+                const auto* s = gmem(c.force) + (size_t)(x >> 2) * 12u + (x & 3u) * 3u;
+                g[k][0] = s[0];
+                g[k][1] = s[1];
+                g[k][2] = s[2];
+            } else {
+                const float4 pv = gload(c.partial + x);
+                g[k][0] = pv.x;
+                g[k][1] = pv.y;
+                g[k][2] = pv.z;
             }
-#pragma unroll
-            for (uint32_t k = 0; k < kGather; ++k)
-                if (idx[k] != kEllSentinel) {
-                    fx += g[k][0];
-                    fy += g[k][1];
-                    fz += g[k][2];
-                }
         }
+#pragma unroll
+        for (uint32_t k = 0; k < kGather; ++k)
+            if (q0 + k < width && idx[k] != kEllSentinel) {
+                fx += g[k][0];
+                fy += g[k][1];
+                fz += g[k][2];
+            }
+    };
+    gather(0);  // straight-line, so its loads wait only for the indices
+    for (uint32_t q0 = kGather; q0 < width; q0 += kGather) {
+#pragma unroll
+        for (uint32_t k = 0; k < kGather; ++k) idx[k] = sl[min(q0 + k, last) * 64u];
+        gather(q0);
     }
 
     if ((fl & kPinAll) != kPinAll) {
