@@ -286,6 +286,66 @@ shipfem_status shipfem_torn_groups(const double* rest_position, const uint32_t* 
                                    const float* damage, double fraction, uint32_t* out,
                                    uint32_t max, uint32_t* count);
 
+/* ---- hand-off to and from other solvers, checkpoint, driven edge ----------- */
+
+/* Everything the device holds, back in double, in the shape zone::Solver::adopt takes
+ * (gpu::ZoneGpuState, engine/gpu/zone_gpu.hpp:113-126; Solver::adopt,
+ * engine/sim/zone.hpp:963-969). The caller owns the arrays; any may be NULL.
+ * The account's running sums are those of the steps taken so far: an energy record at
+ * step n adds the half of step n's work that lies before t_n, so a record and these
+ * sums agree only at the steps a record is taken after. */
+typedef struct shipfem_handoff {
+    double* position;     /* [3N] X + u */
+    double* displacement; /* [3N] u itself (adopt prefers it to position - X: exact) */
+    double* velocity;     /* [3N] v_{n-1/2} */
+    double* plastic;      /* [8T] per tet: eps_p (6, Voigt, engineering shears, co-rotated
+                           * frame), eqps, damage (1 = torn) */
+    int64_t* tear_step;   /* [T] step whose element pass tore the tet, -1 intact */
+    double* tear_stress;  /* [6T] world-frame stress at the tear, Voigt */
+    int64_t steps;        /* n: the state holds u_n and v_{n-1/2} */
+    double time;          /* n dt */
+    double penetration;   /* indenter travel |v_ind| t; 0 without an indenter */
+    double work;          /* w_ext + w_contact + w_boundary */
+    double w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture;
+    uint32_t torn;        /* tets torn so far */
+} shipfem_handoff;
+
+/* ZoneGpuSolver::readback (engine/gpu/zone_gpu.hpp:205-207): the state and the
+ * account's running sums, for a CPU solver to adopt and continue. */
+shipfem_status shipfem_solver_export(shipfem_solver* s, shipfem_handoff* out);
+
+/* zone::Solver::adopt (engine/sim/zone.hpp:963-969, zone.cpp:1321-1368) on this
+ * solver: take a state computed elsewhere and continue as if this solver had produced
+ * it. velocity and position (or displacement) are required; plastic (and with it tear_step/tear_stress)
+ * is taken when non-NULL and the case is plastic. steps sets the step counter and time
+ * must equal steps * dt (to 1e-9 relative), else SHIPFEM_INVALID_ARG. displacement,
+ * when non-NULL, is taken instead of position - X (a double u of an FP32 + compensation
+ * state comes back to the same two floats, so a run adopting its own export continues
+ * with the same bits; position - X loses u's last bits to X's magnitude). The running
+ * sums continue from w_ext .. fracture. Nothing is stepped, so the plastic history
+ * comes back exactly as handed over. The energy history restarts empty. */
+shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in);
+
+/* Bitwise checkpoint of the solver's evolving device state (displacement and velocity
+ * with their compensations, plastic and tear state, the account's per-wave sums, step
+ * counter, status and energy history). With buf NULL, *len receives the size. A solver
+ * of the same case restored from it continues with the same bits as the run that wrote
+ * it. No reference counterpart: the reference has no checkpoint of a GPU solve. */
+shipfem_status shipfem_solver_checkpoint(shipfem_solver* s, void* buf, uint64_t* len);
+/* SHIPFEM_INVALID_ARG when buf was written by a solver of another case shape,
+ * material features or time step. */
+shipfem_status shipfem_solver_restore(shipfem_solver* s, const void* buf, uint64_t len);
+
+/* coupling::applyEdgeDrive (engine/sim/coupling.hpp:256-270, coupling.cpp:110-125):
+ * drives every DOF whose driven[3 n + a] is non-zero to displacement[3 n + a] (Tier-1's
+ * interface displacement, on the drive_ramp smoothstep) by setting its DRIVE bit,
+ * clearing its PIN bit (the drive replaces a zero clamp) and writing prescribed. It
+ * never un-pins: a DOF the flags pin and the drive does not name stays clamped.
+ * Returns the number of DOF written. */
+uint64_t shipfem_apply_edge_drive(uint32_t node_count, const double* displacement,
+                                  const uint8_t* driven, uint32_t* node_flags,
+                                  double* prescribed);
+
 /* ---- many cases in one launch per phase ----------------------------------- */
 typedef struct shipfem_batch shipfem_batch;
 

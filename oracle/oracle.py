@@ -328,6 +328,39 @@ class Case:
     def step(self):
         return self.st.step
 
+    def adopt(self, state):
+        """Continue from a state computed elsewhere, as zone::Solver::adopt does
+        (zone.cpp:1321-1368): a dict with displacement [N,3] (or position), velocity [N,3],
+        optional plastic
+        [T,8] (eps_p 6, eqps, damage) and tear_step [T], steps, and the account's running
+        sums w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture."""
+        if state.get("displacement") is not None:
+            self.u[:] = np.asarray(state["displacement"], np.float64).reshape(self.N, 3)
+        else:
+            self.u[:] = np.asarray(state["position"], np.float64).reshape(self.N, 3) - self.X
+        self.v[:] = np.asarray(state["velocity"], np.float64).reshape(self.N, 3)
+        if state.get("plastic") is not None:
+            pl = np.asarray(state["plastic"], np.float64).reshape(self.T, 8)
+            self.eps_p[:] = pl[:, :6]
+            self.eqps[:] = pl[:, 6]
+            self.damage[:] = pl[:, 7]
+        if state.get("tear_step") is not None:
+            self.tear_step[:] = state["tear_step"]
+        self.st.step = int(state["steps"])
+        for k in ("w_ext", "w_contact", "w_boundary", "damping_loss", "dissipation", "fracture"):
+            setattr(self.st, k, float(state.get(k, 0.0)))
+        return self
+
+    def export(self):
+        """The oracle's state in the same dict shape (position, velocity, plastic, steps,
+        running sums)."""
+        out = dict(position=self.X + self.u, displacement=self.u.copy(), velocity=self.v.copy(),
+                   plastic=np.concatenate([self.eps_p, self.eqps[:, None], self.damage[:, None]], 1),
+                   tear_step=self.tear_step.copy(), steps=self.st.step)
+        for k in ("w_ext", "w_contact", "w_boundary", "damping_loss", "dissipation", "fracture"):
+            out[k] = getattr(self.st, k)
+        return out
+
     def run(self, steps, sample_every=0, threads=1):
         """Advance; returns the energy records (list of dicts) of sampled steps.
         threads > 1 splits the element and node loops statically: u, v and the plastic

@@ -64,22 +64,6 @@ bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out) {
 }
 }  // namespace shipfem
 
-struct shipfem_solver {
-    HostCase hc;
-    DeviceCase dc;
-    Plan plan;
-    hipStream_t stream = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int64_t host_step = 0;
-    ~shipfem_solver() {
-        plan.dropGraph();
-        if (dc.owns && dc.arena) (void)hipFree(dc.arena);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
 struct shipfem_batch {
     std::vector<HostCase> hcs;
     std::vector<DeviceCase> dcs;

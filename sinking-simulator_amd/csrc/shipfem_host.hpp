@@ -143,3 +143,20 @@ shipfem_status statusOf(hipError_t e);
 bool deviceAvailable(std::string* why);
 
 }  // namespace shipfem
+
+// One case on its own stream (the opaque shipfem_solver of the C ABI).
+struct shipfem_solver {
+    shipfem::HostCase hc;
+    shipfem::DeviceCase dc;
+    shipfem::Plan plan;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int64_t host_step = 0;
+    ~shipfem_solver() {
+        plan.dropGraph();
+        if (dc.owns && dc.arena) (void)hipFree(dc.arena);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};

@@ -110,8 +110,19 @@ class ServerStats(C.Structure):
                 ("element_steps", C.c_double), ("device_ms", C.c_double)]
 
 
+class Handoff(C.Structure):
+    _fields_ = [("position", D), ("displacement", D), ("velocity", D), ("plastic", D),
+                ("tear_step", C.POINTER(C.c_int64)), ("tear_stress", D),
+                ("steps", C.c_int64), ("time", C.c_double), ("penetration", C.c_double),
+                ("work", C.c_double), ("w_ext", C.c_double), ("w_contact", C.c_double),
+                ("w_boundary", C.c_double), ("damping_loss", C.c_double),
+                ("dissipation", C.c_double), ("fracture", C.c_double), ("torn", C.c_uint32)]
+
+
 ENERGY_FIELDS = ["strain", "kinetic", "dissipation", "w_ext", "w_contact", "w_boundary",
                  "damping_loss", "fracture"]
+HANDOFF_SCALARS = ["steps", "time", "penetration", "work", "w_ext", "w_contact", "w_boundary",
+                   "damping_loss", "dissipation", "fracture", "torn"]
 
 # every symbol include/shipfem.h declares, with (restype, argtypes)
 VP = C.c_void_p
@@ -145,6 +156,11 @@ SIGNATURES = {
     "shipfem_regularised_failure_strain": (C.c_double, [C.c_double] * 4),
     "shipfem_torn_groups": (S, [D, U32, C.c_uint32, U32, F, C.c_double, U32, C.c_uint32,
                                 C.POINTER(C.c_uint32)]),
+    "shipfem_solver_export": (S, [VP, C.POINTER(Handoff)]),
+    "shipfem_solver_adopt": (S, [VP, C.POINTER(Handoff)]),
+    "shipfem_solver_checkpoint": (S, [VP, VP, C.POINTER(C.c_uint64)]),
+    "shipfem_solver_restore": (S, [VP, VP, C.c_uint64]),
+    "shipfem_apply_edge_drive": (C.c_uint64, [C.c_uint32, D, C.POINTER(C.c_uint8), U32, D]),
     "shipfem_solver_destroy": (None, [VP]),
     "shipfem_case_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int32]),
     "shipfem_batch_create": (S, [C.POINTER(CaseDesc), C.c_uint32, C.POINTER(LaunchConfig), PP,
@@ -400,6 +416,20 @@ def torn_groups(X, index, tet_group, damage, fraction=0.5):
     return out[:n.value].tolist()
 
 
+def apply_edge_drive(flags, prescribed, displacement, driven):
+    """coupling::applyEdgeDrive (coupling.cpp:110-125) on a case's node flags [N] and
+    prescribed displacement [N, 3], in place: every DOF with driven[n, a] != 0 is driven
+    to displacement[n, a] (its PIN bit cleared, DRIVE bit set); returns the DOF count."""
+    assert flags.dtype == np.uint32 and flags.flags.c_contiguous
+    assert prescribed.dtype == np.float64 and prescribed.flags.c_contiguous
+    disp = np.ascontiguousarray(displacement, np.float64)
+    drv = np.ascontiguousarray(driven, np.uint8)
+    N = len(flags)
+    assert prescribed.size == 3 * N and disp.size == 3 * N and drv.size == 3 * N
+    return lib().shipfem_apply_edge_drive(N, _p(disp, C.c_double), _p(drv, C.c_uint8),
+                                          _p(flags, C.c_uint32), _p(prescribed, C.c_double))
+
+
 class State:
     def __init__(self, N, T):
         self.u = np.zeros((N, 3))
@@ -507,6 +537,58 @@ class Solver:
         e, n = C.c_double(), C.c_double()
         _check(lib().shipfem_solver_profile(self.h, steps, C.byref(e), C.byref(n)))
         return e.value, n.value
+
+    def export(self):
+        """The state in the shape zone::Solver::adopt takes (ZoneGpuState): dict with
+        position [N,3], displacement [N,3], velocity [N,3], plastic [T,8] (eps_p 6, eqps, damage), tear_step
+        [T] (-1 intact), tear_stress [T,6] and the scalars steps, time, penetration, work,
+        w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture, torn."""
+        N, T = self.case.N, self.case.T
+        out = dict(position=np.zeros((N, 3)), displacement=np.zeros((N, 3)),
+                   velocity=np.zeros((N, 3)),
+                   plastic=np.zeros((T, 8)), tear_step=np.zeros(T, np.int64),
+                   tear_stress=np.zeros((T, 6)))
+        h = Handoff()
+        h.position, h.velocity = _p(out["position"], C.c_double), _p(out["velocity"], C.c_double)
+        h.displacement = _p(out["displacement"], C.c_double)
+        h.plastic = _p(out["plastic"], C.c_double)
+        h.tear_step = _p(out["tear_step"], C.c_int64)
+        h.tear_stress = _p(out["tear_stress"], C.c_double)
+        _check(lib().shipfem_solver_export(self.h, C.byref(h)), "export")
+        for k in HANDOFF_SCALARS:
+            out[k] = getattr(h, k)
+        return out
+
+    def adopt(self, state):
+        """zone::Solver::adopt on this solver: continue from a state made elsewhere
+        (a dict as export() returns; plastic, tear_step and tear_stress optional)."""
+        keep = {}
+        h = Handoff()
+        for k, ct in (("position", C.c_double), ("displacement", C.c_double),
+                      ("velocity", C.c_double), ("plastic", C.c_double),
+                      ("tear_step", C.c_int64), ("tear_stress", C.c_double)):
+            a = state.get(k)
+            if a is not None:
+                keep[k] = np.ascontiguousarray(a, np.int64 if k == "tear_step" else np.float64)
+                setattr(h, k, _p(keep[k], ct))
+        for k in HANDOFF_SCALARS:
+            if k in state:
+                setattr(h, k, state[k])
+        _check(lib().shipfem_solver_adopt(self.h, C.byref(h)), "adopt")
+        return self
+
+    def checkpoint(self):
+        """Bitwise checkpoint of the evolving device state, as bytes."""
+        n = C.c_uint64()
+        _check(lib().shipfem_solver_checkpoint(self.h, None, C.byref(n)), "checkpoint")
+        buf = C.create_string_buffer(n.value)
+        _check(lib().shipfem_solver_checkpoint(self.h, buf, C.byref(n)), "checkpoint")
+        return buf.raw[:n.value]
+
+    def restore(self, blob):
+        buf = C.create_string_buffer(blob, len(blob))
+        _check(lib().shipfem_solver_restore(self.h, buf, len(blob)), "restore")
+        return self
 
 
 class Batch:
