@@ -1,6 +1,7 @@
 """Benchmark of the explicit tet step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plate|cantilever|panel]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--csv FILE]
+                    [--workload plate|cantilever|panel|batch|serve|sweep]
 
 Default workload: the large plain-path plate (10 x 4 x 0.02 m, 350x144x4 Kuhn cells,
 1,209,600 tets, nodes jittered +-5%, edges clamped, 200 kPa ramped over 5 ms,
@@ -197,6 +198,55 @@ def bench_serve(args, sf, info):
                           device=info[0], sweep=rows)))
 
 
+# the reference's fem_spike throughput meshes: makeBoxTetMesh(2.0, 0.5, 0.05, nx, ny, nz),
+# root face clamped, gravity on, undamped, 200 warm-up + 2000 timed steps
+# (tools/fem_spike/main.cpp:227-262; docs/07-fem-spike-findings.md:85-89)
+SPIKE_MESHES = [(20, 4, 2), (60, 8, 4), (120, 16, 4), (200, 24, 8), (320, 32, 8)]
+SPIKE_REFERENCE = {960: 116e6, 11520: 666e6, 46080: 587e6, 230400: 439e6, 491520: 470e6}
+
+
+def bench_sweep(args, sf, info, rows):
+    """The fem_spike throughput table from one run: every mesh, repeats interleaved,
+    an idle check before each run, one CSV row per run, minimum over repeats."""
+    from shipfem import results
+
+    cases = []
+    for nx, ny, nz in SPIKE_MESHES:
+        m = sf.Mesh.box(2.0, 0.5, 0.05, nx, ny, nz)
+        m.flag_plane(0, 0.0, sf.PIN_ALL, tol=1e-6)
+        cases.append(sf.Case.from_mesh(m, gravity=9.81, sample_every=100))
+    warm, steps = args.warmup, args.steps
+    best = {}
+    for rep in range(args.repeats):
+        for case in cases:
+            busy = results.host_busy()
+            s = sf.Solver(case)
+            s.run(warm)
+            ms = s.run(steps)
+            s.close()
+            rate = case.T * steps / (ms * 1e-3)
+            best[case.T] = max(best.get(case.T, 0.0), rate)
+            if rows:
+                rows.add(steps, workload="fem_spike box 2.0x0.5x0.05", tets=case.T, nodes=case.N,
+                         repeat=rep, warmup=warm, device_ms=round(ms, 4),
+                         ms_per_step=round(ms / steps, 6), tet_steps_per_s=round(rate),
+                         host_busy_before=None if busy is None else round(busy, 3))
+            print(json.dumps(dict(repeat=rep, tets=case.T, ms_per_step=ms / steps,
+                                  tet_steps_per_s=rate)), file=sys.stderr, flush=True)
+    top = cases[-1].T
+    table = [dict(tets=t, tet_steps_per_s=r, reference=SPIKE_REFERENCE.get(t)) for t, r in best.items()]
+    emit(dict(metric="tet-element-steps/s", value=best[top], unit="tet-steps/s", n_gpus=1,
+              steps=steps, warmup=warm, ms_per_step=top / best[top] * 1e3, higher_is_better=True,
+              scaling="weak", vs_baseline=best[top] / SPIKE_REFERENCE[top],
+              vs_baseline_basis="fem_spike's own number on the same mesh (GTX 1070 Ti, "
+                                "docs/07-fem-spike-findings.md:85-89)",
+              dtype="fp32", data="synthetic",
+              config=dict(workload="fem_spike throughput meshes makeBoxTetMesh(2.0,0.5,0.05,...), "
+                                   "root clamped, gravity, undamped; device time by HIP events; "
+                                   f"minimum over {args.repeats} interleaved repeats",
+                          device=info[0], table=table, csv=args.csv)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +259,10 @@ def main():
     ap.add_argument("--rates", default="10,50,100,150,200")
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--pool", type=int, default=24)
+    ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--csv", default=None,
+                    help="write one row per timed run, stamped with run id, commit, source digest, "
+                         "time and step count")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, create a one-case batch: its creation is one device-to-"
                          "device copy of a known size, the reference for rocprofv3 byte counters")
@@ -225,6 +279,13 @@ def main():
     if info is None:
         print("no usable gfx950 device", file=sys.stderr)
         sys.exit(1)
+    rows = None
+    if args.csv:
+        from shipfem import results
+
+        rows = results.Rows(args.csv, results.run_identity(info[0]))
+    if args.workload == "sweep":
+        return bench_sweep(args, sf, info, rows)
     if args.workload == "batch":
         return bench_batch(args, sf, info)
     if args.workload == "serve":
@@ -276,6 +337,11 @@ def main():
         except Exception as e:  # reported, never fatal for the GPU number
             cpu = dict(value=None, unit="tet-steps/s", cores=0, kind="port", sample=f"failed: {e}")
 
+    if rows:
+        rows.add(args.steps, workload=args.workload, tets=T, nodes=N, variant=args.variant,
+                 warmup=args.warmup, wall_ms_per_step=round(ms_per_step, 6),
+                 device_ms_per_step=round(dev_ms / args.steps, 6), tet_steps_per_s=round(value),
+                 elem_ms=round(elem_ms, 6), node_ms=round(node_ms, 6))
     emit(dict(metric="tet-element-steps/s", value=value, unit="tet-steps/s", n_gpus=1,
               steps=args.steps, warmup=args.warmup, ms_per_step=ms_per_step,
               higher_is_better=True, scaling="weak",

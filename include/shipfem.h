@@ -346,6 +346,28 @@ uint64_t shipfem_apply_edge_drive(uint32_t node_count, const double* displacemen
                                   const uint8_t* driven, uint32_t* node_flags,
                                   double* prescribed);
 
+/* ---- case wire format ------------------------------------------------------- */
+/* One self-checking little-endian blob per case: the on-disk case file and the form a
+ * case travels in to the server. A 64-byte header (magic "SHIPFEMC", version, node and
+ * tet counts, which optional arrays follow, payload size, FNV-1a 64 of the payload),
+ * the scalar block, then rest positions, connectivity and the optional arrays of the
+ * descriptor. No reference counterpart: the reference builds its cases in memory; a
+ * run's tables are reproducible from files it can re-read (tools/zone_gpu_probe/
+ * sweep.sh:21-25 is the reason). */
+typedef struct shipfem_case_blob shipfem_case_blob; /* owns a decoded case's arrays */
+uint64_t shipfem_case_encoded_size(const shipfem_case_desc* d);
+shipfem_status shipfem_case_encode(const shipfem_case_desc* d, void* buf, uint64_t len);
+/* Fills *d with pointers into *out (valid until shipfem_case_blob_destroy).
+ * SHIPFEM_INVALID_ARG on a wrong magic or version, inconsistent sizes, a checksum
+ * mismatch or a node id out of range. */
+shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_blob** out,
+                                   shipfem_case_desc* d);
+void shipfem_case_blob_destroy(shipfem_case_blob* b);
+/* The blob to and from a file (written to path.tmp, then renamed). */
+shipfem_status shipfem_case_save(const shipfem_case_desc* d, const char* path);
+shipfem_status shipfem_case_load(const char* path, shipfem_case_blob** out,
+                                 shipfem_case_desc* d);
+
 /* ---- many cases in one launch per phase ----------------------------------- */
 typedef struct shipfem_batch shipfem_batch;
 
@@ -411,6 +433,9 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
  * to whole steps). Refuses (SHIPFEM_BUSY) a case whose footprint exceeds the budget. */
 shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc* desc,
                                      double run_to_time, uint64_t* request_id);
+/* shipfem_server_submit of an encoded case (the wire format above). */
+shipfem_status shipfem_server_submit_encoded(shipfem_server* s, const void* buf, uint64_t len,
+                                             double run_to, uint64_t* request_id);
 /* Moves up to `max` completed results into out. Optionally keeps the final
  * displacement of request ids the caller asked for with shipfem_server_keep. */
 shipfem_status shipfem_server_poll(shipfem_server* s, shipfem_result* out, uint32_t max,

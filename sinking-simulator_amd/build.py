@@ -17,7 +17,7 @@ OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libshipfem.so")
 
 SOURCES = ["kernels.hip", "mesh.cpp", "prep.cpp", "engine.cpp", "api.cpp", "server.cpp", "handoff.cpp",
-           "fem_gpu.cpp"]
+           "casefile.cpp", "fem_gpu.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=on: a multiply-add fuses only within one source expression, so every
 # template instantiation of a kernel rounds identically (results do not depend on the
@@ -69,9 +69,28 @@ def build_oracle():
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
 
 
+def write_build_info():
+    """lib/build_info.json: the commit the library was built from (the tree travels
+    without its git history), whether the tree had changes, and when."""
+    import datetime
+    import json
+
+    def git(*args):
+        r = subprocess.run(["git", "-C", ROOT] + list(args), capture_output=True, text=True)
+        return r.stdout.strip() if r.returncode == 0 else None
+
+    info = dict(commit=git("rev-parse", "HEAD") or "unknown",
+                dirty=bool(git("status", "--porcelain", "--untracked-files=no")),
+                built_utc=datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"))
+    with open(os.path.join(LIBDIR, "build_info.json"), "w") as f:
+        json.dump(info, f, indent=1)
+
+
 def build_all(verbose=False):
     build_oracle()
-    return build_library(verbose)
+    lib = build_library(verbose)
+    write_build_info()
+    return lib
 
 
 if __name__ == "__main__":

@@ -636,6 +636,19 @@ shipfem_status shipfem_server_poll(shipfem_server* s, shipfem_result* out, uint3
     return SHIPFEM_OK;
 }
 
+shipfem_status shipfem_server_submit_encoded(shipfem_server* s, const void* buf, uint64_t len,
+                                             double run_to, uint64_t* request_id) {
+    if (!s) return SHIPFEM_INVALID_ARG;
+    shipfem_case_blob* blob = nullptr;
+    shipfem_case_desc d{};
+    const shipfem_status st = shipfem_case_decode(buf, len, &blob, &d);
+    if (st != SHIPFEM_OK) return st;
+    // submit copies the case, so the decoded arrays can go at once
+    const shipfem_status r = shipfem_server_submit(s, &d, run_to, request_id);
+    shipfem_case_blob_destroy(blob);
+    return r;
+}
+
 shipfem_status shipfem_server_keep(shipfem_server* s, uint64_t id) {
     if (!s) return SHIPFEM_INVALID_ARG;
     std::lock_guard<std::mutex> lk(s->m);

@@ -161,6 +161,13 @@ SIGNATURES = {
     "shipfem_solver_checkpoint": (S, [VP, VP, C.POINTER(C.c_uint64)]),
     "shipfem_solver_restore": (S, [VP, VP, C.c_uint64]),
     "shipfem_apply_edge_drive": (C.c_uint64, [C.c_uint32, D, C.POINTER(C.c_uint8), U32, D]),
+    "shipfem_case_encoded_size": (C.c_uint64, [C.POINTER(CaseDesc)]),
+    "shipfem_case_encode": (S, [C.POINTER(CaseDesc), VP, C.c_uint64]),
+    "shipfem_case_decode": (S, [VP, C.c_uint64, PP, C.POINTER(CaseDesc)]),
+    "shipfem_case_blob_destroy": (None, [VP]),
+    "shipfem_case_save": (S, [C.POINTER(CaseDesc), C.c_char_p]),
+    "shipfem_case_load": (S, [C.c_char_p, PP, C.POINTER(CaseDesc)]),
+    "shipfem_server_submit_encoded": (S, [VP, VP, C.c_uint64, C.c_double, C.POINTER(C.c_uint64)]),
     "shipfem_solver_destroy": (None, [VP]),
     "shipfem_case_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int32]),
     "shipfem_batch_create": (S, [C.POINTER(CaseDesc), C.c_uint32, C.POINTER(LaunchConfig), PP,
@@ -361,6 +368,64 @@ class Case:
         d.dt, d.dt_safety = self.dt, self.dt_safety
         d.sample_every, d.max_steps = self.sample_every, self.max_steps
         return d
+
+    def encode(self):
+        """The case wire format (shipfem_case_encode): one self-checking blob."""
+        d = self.desc()
+        n = lib().shipfem_case_encoded_size(C.byref(d))
+        buf = C.create_string_buffer(n)
+        _check(lib().shipfem_case_encode(C.byref(d), buf, n), "encode")
+        return buf.raw[:n]
+
+    @classmethod
+    def _from_desc(cls, d):
+        N, T = d.node_count, d.tet_count
+        arr = lambda ptr, n, dt_: None if not ptr else np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt_).copy()  # noqa: E731
+        X = arr(d.rest_position, 3 * N, np.float64).reshape(N, 3)
+        index = arr(d.index, 4 * T, np.uint32).reshape(T, 4)
+        vec3 = lambda ptr: None if not ptr else arr(ptr, 3 * N, np.float64).reshape(N, 3)  # noqa: E731
+        m, ind = d.material, d.indenter
+        failure = None
+        if m.plastic and m.failure_strain > 0:
+            failure = dict(failure_strain=m.failure_strain, sensitivity=m.triaxiality_sensitivity,
+                           reference_triaxiality=m.reference_triaxiality,
+                           cutoff_triaxiality=m.cutoff_triaxiality)
+        indenter = None
+        if ind.enabled:
+            indenter = dict(radius=ind.radius, center=tuple(ind.center), velocity=tuple(ind.velocity),
+                            kappa=ind.penalty)
+        return cls(X, index, flags=arr(d.node_flags, N, np.uint32), load=vec3(d.load),
+                   prescribed=vec3(d.prescribed), u0=vec3(d.initial_displacement),
+                   v0=vec3(d.initial_velocity), E=m.youngs_modulus, nu=m.poisson_ratio,
+                   rho=m.density, plastic=bool(m.plastic), yield_=m.yield_strength,
+                   hardening=m.hardening_modulus, indenter=indenter, load_ramp=d.load_ramp,
+                   drive_ramp=d.drive_ramp, gravity=d.gravity, damping=d.damping, dt=d.dt,
+                   dt_safety=d.dt_safety, sample_every=d.sample_every, max_steps=d.max_steps,
+                   failure=failure)
+
+    @classmethod
+    def decode(cls, blob):
+        """A case from the wire format (shipfem_case_decode)."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        h, d = C.c_void_p(), CaseDesc()
+        _check(lib().shipfem_case_decode(buf, len(blob), C.byref(h), C.byref(d)), "decode")
+        try:
+            return cls._from_desc(d)
+        finally:
+            lib().shipfem_case_blob_destroy(h)
+
+    def save(self, path):
+        d = self.desc()
+        _check(lib().shipfem_case_save(C.byref(d), os.fsencode(path)), f"save {path}")
+
+    @classmethod
+    def load(cls, path):
+        h, d = C.c_void_p(), CaseDesc()
+        _check(lib().shipfem_case_load(os.fsencode(path), C.byref(h), C.byref(d)), f"load {path}")
+        try:
+            return cls._from_desc(d)
+        finally:
+            lib().shipfem_case_blob_destroy(h)
 
     def oracle_kwargs(self):
         """Keyword arguments of oracle.Case describing the same problem (test use)."""
@@ -681,6 +746,16 @@ class Server:
         d = case.desc()
         rid = C.c_uint64()
         _check(lib().shipfem_server_submit(self.h, C.byref(d), run_to, C.byref(rid)), "submit")
+        if keep:
+            _check(lib().shipfem_server_keep(self.h, rid.value))
+        return rid.value
+
+    def submit_encoded(self, blob, run_to, keep=False):
+        """submit() of a case in the wire format (Case.encode / a case file's bytes)."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        rid = C.c_uint64()
+        _check(lib().shipfem_server_submit_encoded(self.h, buf, len(blob), run_to, C.byref(rid)),
+               "submit_encoded")
         if keep:
             _check(lib().shipfem_server_keep(self.h, rid.value))
         return rid.value

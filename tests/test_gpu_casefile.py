@@ -1,0 +1,60 @@
+"""Cases that travel in the wire format solve exactly as the cases they were made from:
+a decoded case on a solver, and an encoded case submitted to the server."""
+import numpy as np
+import pytest
+
+import shipfem as sf
+from shipfem import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    if sf.device_info() is None:
+        pytest.skip("no usable gfx950 device")
+
+
+def panel():
+    return configs.stiffened_panel(nx=24, ny_bay=4, nz_web=3, speed=8.0, radius=0.05,
+                                   geometry=dict(lx=0.4, ly=0.2, n_stiff=1, web_h=0.04),
+                                   failure=dict(failure_strain=0.1))
+
+
+def test_decoded_case_solves_with_the_same_bits(tmp_path):
+    c = panel()
+    path = tmp_path / "panel.case"
+    c.save(path)
+    outs = []
+    for case in (c, sf.Case.decode(c.encode()), sf.Case.load(path)):
+        s = sf.Solver(case)
+        s.run(400)
+        st = s.readback()
+        outs.append((st.u, st.eqps, s.energy()))
+    for u, q, e in outs[1:]:
+        assert np.array_equal(u, outs[0][0]) and np.array_equal(q, outs[0][1]) and e == outs[0][2]
+    assert outs[0][1].max() > 0
+
+
+def test_server_takes_a_case_in_the_wire_format():
+    c = panel()
+    results = []
+    for encoded in (False, True):
+        srv = sf.Server(chunk=32)
+        rid = srv.submit_encoded(c.encode(), 2e-4, keep=True) if encoded else \
+            srv.submit(c, 2e-4, keep=True)
+        assert srv.drain(120.0)
+        res = {r["request_id"]: r for r in srv.poll()}[rid]
+        assert res["status"] == sf.OK
+        results.append((res, srv.fetch(rid, c)))
+        srv.close()
+    (ra, sa), (rb, sb) = results
+    assert sa.step == sb.step > 0
+    assert np.array_equal(sa.u, sb.u) and np.array_equal(sa.eqps, sb.eqps)
+    srv = sf.Server(chunk=32)
+    bad = bytearray(c.encode())
+    bad[100] ^= 0xFF
+    with pytest.raises(sf.ShipfemError) as e:
+        srv.submit_encoded(bytes(bad), 2e-4)
+    assert e.value.status == sf.INVALID_ARG
+    srv.close()
