@@ -270,9 +270,14 @@ shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out) {
     if (!s || !out) return SHIPFEM_INVALID_ARG;
     // only the slots variant keeps per-tet force slots; the tiled one sums in LDS
     if (s->plan.shape.variant != kSlots) return SHIPFEM_UNSUPPORTED;
-    hipError_t e = hipMemcpyAsync(out, s->dc.rec.force, 48ull * s->hc.T, hipMemcpyDeviceToHost, s->stream);
+    const HostCase& hc = s->hc;
+    std::vector<float> f(12ull * hc.T);
+    hipError_t e = hipMemcpyAsync(f.data(), s->dc.rec.force, 48ull * hc.T, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    return statusOf(e);
+    if (e != hipSuccess) return statusOf(e);
+    for (uint32_t t = 0; t < hc.T; ++t)  // back in the caller's tet order (corners keep theirs)
+        std::memcpy(out + 12ull * externalTet(hc, t), f.data() + 12ull * t, 48);
+    return SHIPFEM_OK;
 }
 
 shipfem_status shipfem_solver_set_launch(shipfem_solver* s, const shipfem_launch_config* cfg) {

@@ -110,24 +110,28 @@ shipfem_status shipfem_solver_export(shipfem_solver* s, shipfem_handoff* out) {
     const float* cu = pl.data();
     const float* v = pl.data() + 3 * Np;
     const float* cv = pl.data() + 6 * Np;
-    for (uint64_t n = 0; n < N; ++n)
+    // node and tet arrays go out in the caller's orders
+    for (uint32_t n = 0; n < N; ++n) {
+        const uint64_t x = externalNode(hc, n);
         for (int a = 0; a < 3; ++a) {
-            const double u = (double)u4[n * 4 + a] - (double)cu[a * Np + n];
-            if (out->position) out->position[n * 3 + a] = hc.X[n * 3 + a] + u;
-            if (out->displacement) out->displacement[n * 3 + a] = u;
-            if (out->velocity) out->velocity[n * 3 + a] = (double)v[a * Np + n] - (double)cv[a * Np + n];
+            const double u = (double)u4[n * 4ull + a] - (double)cu[a * Np + n];
+            if (out->position) out->position[x * 3 + a] = hc.X[n * 3ull + a] + u;
+            if (out->displacement) out->displacement[x * 3 + a] = u;
+            if (out->velocity) out->velocity[x * 3 + a] = (double)v[a * Np + n] - (double)cv[a * Np + n];
         }
+    }
     if (out->plastic)
-        for (uint64_t t = 0; t < T; ++t)
+        for (uint32_t t = 0; t < T; ++t)
             for (uint32_t k = 0; k < 8; ++k)
-                out->plastic[t * 8 + k] = k < pstatePlanes(hc) ? (double)ps[k * Tp + t] : 0.0;
+                out->plastic[externalTet(hc, t) * 8ull + k] = k < pstatePlanes(hc) ? (double)ps[k * Tp + t] : 0.0;
     uint32_t torn = 0;
-    for (uint64_t t = 0; t < T; ++t) {
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint64_t x = externalTet(hc, t);
         const bool is_torn = dmg && tstep[t] != 0;
         torn += is_torn ? 1u : 0u;
-        if (out->tear_step) out->tear_step[t] = is_torn ? static_cast<int64_t>(tstep[t]) - 1 : -1;
+        if (out->tear_step) out->tear_step[x] = is_torn ? static_cast<int64_t>(tstep[t]) - 1 : -1;
         if (out->tear_stress)
-            for (int k = 0; k < 6; ++k) out->tear_stress[t * 6 + k] = is_torn ? (double)tstress[k * Tp + t] : 0.0;
+            for (int k = 0; k < 6; ++k) out->tear_stress[x * 6 + k] = is_torn ? (double)tstress[k * Tp + t] : 0.0;
     }
     // the account's running sums, wave by wave in order
     auto total = [](const std::vector<double>& a, uint32_t slot, uint32_t waves) {
@@ -165,15 +169,17 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
     float* cu = pl.data();
     float* v = pl.data() + 3 * Np;
     float* cv = pl.data() + 6 * Np;
-    for (uint64_t n = 0; n < N; ++n)
+    for (uint32_t n = 0; n < N; ++n) {
+        const uint64_t e = externalNode(hc, n);
         for (int a = 0; a < 3; ++a) {
-            const double x = in->displacement ? in->displacement[n * 3 + a]
-                                              : in->position[n * 3 + a] - hc.X[n * 3 + a];
-            const double vel = in->velocity[n * 3 + a];
+            const double x = in->displacement ? in->displacement[e * 3 + a]
+                                              : in->position[e * 3 + a] - hc.X[n * 3ull + a];
+            const double vel = in->velocity[e * 3 + a];
             if (!std::isfinite(x) || !std::isfinite(vel)) return SHIPFEM_INVALID_ARG;
-            splitValue(x, u4[n * 4 + a], cu[a * Np + n]);
+            splitValue(x, u4[n * 4ull + a], cu[a * Np + n]);
             splitValue(vel, v[a * Np + n], cv[a * Np + n]);
         }
+    }
     const hipStream_t st = s->stream;
     hipError_t e = toDevice(r.u4, u4.data(), 16 * N, st);
     if (e == hipSuccess) e = toDevice(r.cu, cu, 12 * Np, st);
@@ -185,19 +191,21 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
     std::vector<int32_t> tstep;
     if (in->plastic && planes) {
         ps.assign(planes * Tp, 0.0f);
-        for (uint64_t t = 0; t < T; ++t)
-            for (uint32_t k = 0; k < planes; ++k) ps[k * Tp + t] = static_cast<float>(in->plastic[t * 8 + k]);
+        for (uint32_t t = 0; t < T; ++t)
+            for (uint32_t k = 0; k < planes; ++k)
+                ps[k * Tp + t] = static_cast<float>(in->plastic[externalTet(hc, t) * 8ull + k]);
         if (e == hipSuccess) e = toDevice(r.pstate, ps.data(), 4 * ps.size(), st);
         if (dmg) {
             // a torn tet handed over without its tear step is recorded as torn at `steps`
             tstep.assign(Tp, 0);
             tstress.assign(6 * Tp, 0.0f);
-            for (uint64_t t = 0; t < T; ++t) {
+            for (uint32_t t = 0; t < T; ++t) {
                 if (!(ps[7 * Tp + t] >= 1.0f)) continue;
-                const int64_t at = in->tear_step && in->tear_step[t] >= 0 ? in->tear_step[t] : in->steps;
+                const uint64_t x = externalTet(hc, t);
+                const int64_t at = in->tear_step && in->tear_step[x] >= 0 ? in->tear_step[x] : in->steps;
                 tstep[t] = static_cast<int32_t>(at + 1);
                 if (in->tear_stress)
-                    for (int k = 0; k < 6; ++k) tstress[k * Tp + t] = static_cast<float>(in->tear_stress[t * 6 + k]);
+                    for (int k = 0; k < 6; ++k) tstress[k * Tp + t] = static_cast<float>(in->tear_stress[x * 6 + k]);
             }
             if (e == hipSuccess) e = toDevice(r.tear_step, tstep.data(), 4 * Tp, st);
             if (e == hipSuccess) e = toDevice(r.tear_stress, tstress.data(), 24 * Tp, st);

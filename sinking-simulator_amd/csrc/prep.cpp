@@ -118,7 +118,86 @@ bool deviceAvailable(std::string* why) {
     return true;
 }
 
-std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap) {
+namespace {
+
+// The library's internal tet order: recursive coordinate bisection of the tet centroids
+// into leaves of exactly kTile tets (the remainder leaf last), each leaf in the caller's
+// order. A 256-tet tile of the tiled element pass is then a compact block of cells, not
+// a strip one cell wide, so it touches fewer nodes: fewer partial sums to gather, fewer
+// displacements to stage. Each split is at a multiple of kTile, at that rank along the
+// longest axis of the range's centroid box, ties broken by tet id, so the order depends
+// only on the mesh. perm[k] is the caller's id of internal tet k.
+std::vector<uint32_t> spatialTetOrder(const double* X, const uint32_t* index, uint32_t T) {
+    std::vector<uint32_t> perm(T);
+    for (uint32_t t = 0; t < T; ++t) perm[t] = t;
+    if (T <= static_cast<uint32_t>(kTile)) return perm;
+    std::vector<double> cent(3ull * T);
+    for (uint64_t t = 0; t < T; ++t)
+        for (int a = 0; a < 3; ++a) {
+            double c = 0;
+            for (int q = 0; q < 4; ++q) c += X[index[4 * t + q] * 3ull + a];
+            cent[3 * t + a] = 0.25 * c;
+        }
+    std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, T}};
+    while (!stack.empty()) {
+        const auto [b, e] = stack.back();
+        stack.pop_back();
+        const uint32_t n = e - b;
+        if (n <= static_cast<uint32_t>(kTile)) {
+            std::sort(perm.begin() + b, perm.begin() + e);
+            continue;
+        }
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::numeric_limits<double>::infinity();
+            hi[a] = -lo[a];
+        }
+        for (uint32_t k = b; k < e; ++k)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], cent[3ull * perm[k] + a]);
+                hi[a] = std::max(hi[a], cent[3ull * perm[k] + a]);
+            }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a)
+            if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+        // a multiple of kTile near the middle, never past the last whole leaf
+        const uint32_t whole = n / kTile;
+        const uint32_t half = static_cast<uint32_t>(std::lround(n / (2.0 * kTile)));
+        const uint32_t left = std::min(whole, std::max<uint32_t>(1u, half)) * kTile;
+        std::nth_element(perm.begin() + b, perm.begin() + b + left, perm.begin() + e,
+                         [&](uint32_t x, uint32_t y) {
+                             const double cx = cent[3ull * x + ax], cy = cent[3ull * y + ax];
+                             return cx < cy || (cx == cy && x < y);
+                         });
+        stack.push_back({b + left, e});
+        stack.push_back({b, b + left});
+    }
+    return perm;
+}
+
+// The internal node order: nodes by first appearance in the internal tet order (corner
+// order within a tet), nodes in no tet last in the caller's order. A tile's new nodes
+// are then consecutive, so the node pass's waves read their partials side by side.
+std::vector<uint32_t> tileNodeOrder(const uint32_t* index, const std::vector<uint32_t>& tperm,
+                                    uint32_t N) {
+    std::vector<uint32_t> nperm;
+    nperm.reserve(N);
+    std::vector<uint8_t> seen(N, 0);
+    for (uint32_t t : tperm)
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t g = index[4ull * t + c];
+            if (!seen[g]) {
+                seen[g] = 1;
+                nperm.push_back(g);
+            }
+        }
+    for (uint32_t n = 0; n < N; ++n)
+        if (!seen[n]) nperm.push_back(n);
+    return nperm;
+}
+
+std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap,
+                            std::vector<uint32_t>&& tperm, std::vector<uint32_t>&& nperm) {
     const uint32_t N = d.node_count, T = d.tet_count;
     if (N == 0 || T == 0) return "empty mesh (node_count and tet_count must be positive)";
     if (!d.rest_position || !d.index) return "rest_position and index are required";
@@ -155,6 +234,8 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
     }
 
     hc = HostCase();
+    hc.tperm = std::move(tperm);
+    hc.nperm = std::move(nperm);
     hc.N = N;
     hc.T = T;
     hc.Npad = roundUp64(N);
@@ -189,7 +270,7 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
     hc.nflags.assign(hc.Npad, SHIPFEM_PIN_ALL);  // pad lanes behave as pinned
     for (uint32_t n = 0; n < N; ++n) {
         const uint32_t f = d.node_flags ? d.node_flags[n] : 0u;
-        if (f & ~kKnownFlags) return "node " + std::to_string(n) + " has unknown flag bits";
+        if (f & ~kKnownFlags) return "node " + std::to_string(externalNode(hc, n)) + " has unknown flag bits";
         hc.nflags[n] = f & ~(SHIPFEM_LOADED | SHIPFEM_CONTACT);  // derived below
     }
     for (uint64_t q = 0; q < 4ull * T; ++q)
@@ -224,7 +305,8 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
                            Dm[0][1] * (Dm[1][0] * Dm[2][2] - Dm[1][2] * Dm[2][0]) +
                            Dm[0][2] * (Dm[1][0] * Dm[2][1] - Dm[1][1] * Dm[2][0]);
         if (!(std::fabs(det) > 1e-12 * longest2 * std::sqrt(longest2)))
-            return "tet " + std::to_string(t) + " is degenerate (|det Dm| = " + std::to_string(det) + ")";
+            return "tet " + std::to_string(externalTet(hc, t)) + " is degenerate (|det Dm| = " +
+                   std::to_string(det) + ")";
         const double id = 1.0 / det;
         const double inv[9] = {(Dm[1][1] * Dm[2][2] - Dm[1][2] * Dm[2][1]) * id,
                                (Dm[0][2] * Dm[2][1] - Dm[0][1] * Dm[2][2]) * id,
@@ -258,7 +340,8 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
         for (uint32_t n = 0; n < N; ++n)
             for (int a = 0; a < 3; ++a) {
                 const double f = d.load[n * 3ull + a];
-                if (!std::isfinite(f)) return "load of node " + std::to_string(n) + " is not finite";
+                if (!std::isfinite(f))
+                    return "load of node " + std::to_string(externalNode(hc, n)) + " is not finite";
                 if (f != 0.0) {
                     hc.load[a * (uint64_t)hc.Npad + n] = static_cast<float>(f);
                     hc.nflags[n] |= SHIPFEM_LOADED;
@@ -333,12 +416,11 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
         }
 
     // 256-tet tiles of the tiled element pass: each tile's unique nodes (one partial
-    // slot each), numbered by descending corner count (ties by ascending id) so the
-    // lanes of one wave of the in-tile sum run similar trip counts. Each local node
-    // owns a contiguous run of the tile's corner slots, in (tet, corner) order, from
-    // its offset lofs[l]; a tet's connectivity packs, per corner, the local node (10
-    // bits) and the corner's rank in that run (6 bits). A node's run, and so its sum,
-    // does not depend on its local number.
+    // slot each) in ascending internal id, so a tile's new nodes (consecutive ids) hold
+    // consecutive partial slots. Each local node owns a contiguous run of the tile's
+    // corner slots, in (tet, corner) order, from its offset lofs[l]; a tet's
+    // connectivity packs, per corner, the local node (10 bits) and the corner's rank in
+    // that run (6 bits).
     {
         const uint32_t nt = (T + kTile - 1) / kTile;
         hc.elem_tiles = nt;
@@ -363,13 +445,12 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
                     }
                     ++deg[g];
                 }
-            std::sort(nodes.begin(), nodes.end(), [&](uint32_t a, uint32_t c) {
-                return deg[a] != deg[c] ? deg[a] > deg[c] : a < c;
-            });
+            std::sort(nodes.begin(), nodes.end());
             const uint32_t L = static_cast<uint32_t>(nodes.size());
-            if (deg[nodes[0]] > 64)
-                return "a node belongs to more than 64 tets of one 256-tet tile (node " +
-                       std::to_string(nodes[0]) + ")";
+            for (uint32_t g : nodes)
+                if (deg[g] > 64)
+                    return "node " + std::to_string(externalNode(hc, g)) +
+                           " belongs to more than 64 tets of one 256-tet tile";
             for (uint32_t l = 0; l < L; ++l) loc[nodes[l]] = l;
             hc.tn_off[b + 1] = hc.tn_off[b] + L;
             hc.lmax = std::max(hc.lmax, L);
@@ -433,6 +514,54 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
                       hc.cv[a * (uint64_t)hc.Npad + n]);
         }
     return {};
+}
+
+// A copy of the caller's case in the internal orders.
+struct PermutedCase {
+    std::vector<double> X, load, presc, u0, v0;
+    std::vector<uint32_t> index, flags;
+};
+
+}  // namespace
+
+std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap) {
+    const uint32_t N = d.node_count, T = d.tet_count;
+    if (N == 0 || T == 0) return "empty mesh (node_count and tet_count must be positive)";
+    if (!d.rest_position || !d.index) return "rest_position and index are required";
+    for (uint64_t q = 0; q < 4ull * T; ++q)
+        if (d.index[q] >= N) return "tet " + std::to_string(q / 4) + " references node " +
+                                    std::to_string(d.index[q]) + " >= node_count";
+    std::vector<uint32_t> tperm = spatialTetOrder(d.rest_position, d.index, T);
+    std::vector<uint32_t> nperm = tileNodeOrder(d.index, tperm, N);
+    std::vector<uint32_t> ninv(N);
+    for (uint32_t i = 0; i < N; ++i) ninv[nperm[i]] = i;
+    PermutedCase pc;
+    shipfem_case_desc dp = d;
+    pc.X.resize(3ull * N);
+    for (uint32_t i = 0; i < N; ++i)
+        for (int a = 0; a < 3; ++a) pc.X[3ull * i + a] = d.rest_position[3ull * nperm[i] + a];
+    dp.rest_position = pc.X.data();
+    pc.index.resize(4ull * T);
+    for (uint32_t k = 0; k < T; ++k)
+        for (int c = 0; c < 4; ++c) pc.index[4ull * k + c] = ninv[d.index[4ull * tperm[k] + c]];
+    dp.index = pc.index.data();
+    if (d.node_flags) {
+        pc.flags.resize(N);
+        for (uint32_t i = 0; i < N; ++i) pc.flags[i] = d.node_flags[nperm[i]];
+        dp.node_flags = pc.flags.data();
+    }
+    auto nodeVec = [&](const double* src, std::vector<double>& dst) -> const double* {
+        if (!src) return nullptr;
+        dst.resize(3ull * N);
+        for (uint32_t i = 0; i < N; ++i)
+            for (int a = 0; a < 3; ++a) dst[3ull * i + a] = src[3ull * nperm[i] + a];
+        return dst.data();
+    };
+    dp.load = nodeVec(d.load, pc.load);
+    dp.prescribed = nodeVec(d.prescribed, pc.presc);
+    dp.initial_displacement = nodeVec(d.initial_displacement, pc.u0);
+    dp.initial_velocity = nodeVec(d.initial_velocity, pc.v0);
+    return prepareInternal(dp, hc, hist_cap, std::move(tperm), std::move(nperm));
 }
 
 uint64_t arenaBytes(const HostCase& hc) { return mapArena(hc).total; }
@@ -571,21 +700,27 @@ hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_
     const float* cu = pl.data();
     const float* v = pl.data() + 3 * Np;
     const float* cv = pl.data() + 6 * Np;
-    for (uint64_t n = 0; n < N; ++n)
+    // node and tet results go back in the caller's orders
+    for (uint32_t n = 0; n < N; ++n) {
+        const uint64_t x = externalNode(hc, n);
         for (int a = 0; a < 3; ++a) {
-            const double u = (double)u4[n * 4 + a] - (double)cu[a * Np + n];
-            if (view->displacement) view->displacement[n * 3 + a] = u;
-            if (view->velocity) view->velocity[n * 3 + a] = (double)v[a * Np + n] - (double)cv[a * Np + n];
-            if (view->position) view->position[n * 3 + a] = static_cast<float>(hc.X[n * 3 + a] + u);
+            const double u = (double)u4[n * 4ull + a] - (double)cu[a * Np + n];
+            if (view->displacement) view->displacement[x * 3 + a] = u;
+            if (view->velocity) view->velocity[x * 3 + a] = (double)v[a * Np + n] - (double)cv[a * Np + n];
+            if (view->position) view->position[x * 3 + a] = static_cast<float>(hc.X[n * 3ull + a] + u);
         }
-    const uint64_t T = hc.T, Tp = hc.Tpad;
+    }
+    const uint64_t Tp = hc.Tpad;
+    const uint32_t T = hc.T;
     if (view->eqps)
-        for (uint64_t t = 0; t < T; ++t) view->eqps[t] = ps.empty() ? 0.0f : ps[6 * Tp + t];
+        for (uint32_t t = 0; t < T; ++t) view->eqps[externalTet(hc, t)] = ps.empty() ? 0.0f : ps[6 * Tp + t];
     if (view->plastic_strain)
-        for (uint64_t t = 0; t < T; ++t)
-            for (int k = 0; k < 6; ++k) view->plastic_strain[t * 6 + k] = ps.empty() ? 0.0f : ps[k * Tp + t];
+        for (uint32_t t = 0; t < T; ++t)
+            for (int k = 0; k < 6; ++k)
+                view->plastic_strain[externalTet(hc, t) * 6ull + k] = ps.empty() ? 0.0f : ps[k * Tp + t];
     if (view->damage)
-        for (uint64_t t = 0; t < T; ++t) view->damage[t] = dmg && !ps.empty() ? ps[7 * Tp + t] : 0.0f;
+        for (uint32_t t = 0; t < T; ++t)
+            view->damage[externalTet(hc, t)] = dmg && !ps.empty() ? ps[7 * Tp + t] : 0.0f;
     view->step = std::max(step[0], step[1]);
     view->time = static_cast<double>(view->step) * dc.dt;
     view->status = status;
@@ -624,15 +759,15 @@ hipError_t readTears(const DeviceCase& dc, const HostCase& hc, std::vector<shipf
     for (uint32_t t = 0; t < hc.T; ++t) {
         if (step[t] == 0) continue;
         shipfem_tear tear{};
-        tear.tet = t;
+        tear.tet = externalTet(hc, t);
         tear.step = static_cast<int64_t>(step[t]) - 1;
         double v[6];
         for (int k = 0; k < 6; ++k) v[k] = st[k * Tp + t];
         maxPrincipalDirection(v, tear.normal);
         out.push_back(tear);
     }
-    std::stable_sort(out.begin(), out.end(), [](const shipfem_tear& a, const shipfem_tear& b) {
-        return a.step < b.step;
+    std::sort(out.begin(), out.end(), [](const shipfem_tear& a, const shipfem_tear& b) {
+        return a.step < b.step || (a.step == b.step && a.tet < b.tet);
     });
     return hipSuccess;
 }

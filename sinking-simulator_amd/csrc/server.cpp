@@ -345,8 +345,8 @@ void shipfem_server::finish(Active& a, hipError_t fault) {
             shipfem_state_view dv{};
             dv.damage = dmg.data();
             if (readbackCase(a.dc, hc, &dv, side) == hipSuccess)
-                for (uint32_t t = 0; t < hc.T; ++t)
-                    if (dmg[t] >= 1.0f) {
+                for (uint32_t t = 0; t < hc.T; ++t)  // dmg is in the caller's order, vol internal
+                    if (dmg[externalTet(hc, t)] >= 1.0f) {
                         ++res.torn_elements;
                         res.torn_volume += hc.vol[t];
                     }

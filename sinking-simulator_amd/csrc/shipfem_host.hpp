@@ -31,6 +31,9 @@ inline uint32_t roundUp64(uint64_t n) { return static_cast<uint32_t>((n + 63) / 
 struct HostCase {
     uint32_t N = 0, T = 0, Npad = 0, Tpad = 0, node_waves = 0, elem_waves = 0;
     uint32_t features = 0;
+    // Internal orders: internal tet k is the caller's tet tperm[k], internal node i the
+    // caller's node nperm[i]. Every array below is in the internal orders.
+    std::vector<uint32_t> tperm, nperm;
     std::vector<double> X;        // [3N] rest positions
     std::vector<uint32_t> conn;   // [4T]
     std::vector<float> dmi;       // [9][Tpad]
@@ -65,7 +68,13 @@ struct HostCase {
     double total_mass = 0, total_volume = 0;
 };
 
-// Validates and prepares; returns an empty string on success, else the reason.
+// The caller's ids of an internal tet and node.
+inline uint32_t externalTet(const HostCase& hc, uint32_t t) { return hc.tperm.empty() ? t : hc.tperm[t]; }
+inline uint32_t externalNode(const HostCase& hc, uint32_t n) { return hc.nperm.empty() ? n : hc.nperm[n]; }
+
+// Validates and prepares; returns an empty string on success, else the reason. The
+// case is laid out in the internal tet and node orders; results go back through
+// externalTet / externalNode.
 std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap = -1);
 
 // Device bytes of a prepared case (its arena), and the admission upper bound.
