@@ -544,7 +544,10 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
     for (uint32_t k = 0; k < kGather; ++k) idx[k] = sl[min(k, last) * 64u] | none;
     const uint32_t ii = live ? i : 0u;
     const uint32_t flr = gmem(c.nflags)[ii];
-    const float m = gmem(c.mass)[ii];
+    // m is needed on sample steps and by gravity, contact, the drive and damping; other
+    // steps read the 1/m line again (already fetched) instead of the mass plane
+    const bool need_m = sample || (c.features & (kFeatGravity | kFeatContact | kFeatDrive | kFeatDamp));
+    const float m = gmem(need_m ? c.mass : c.invm)[ii];
     const float im = gmem(c.invm)[ii];
     const float4 u = gload(c.u4 + ii);
     // without a load plane the velocity plane is read in its place and never used
