@@ -58,3 +58,34 @@ def test_server_takes_a_case_in_the_wire_format():
         srv.submit_encoded(bytes(bad), 2e-4)
     assert e.value.status == sf.INVALID_ARG
     srv.close()
+
+
+def test_caller_numbering_does_not_change_the_solve():
+    # the library lays a case out in its own tet and node orders and maps every result
+    # back; the same case under a shuffled numbering of nodes and tets gives the same
+    # answer per node and per tet (the orders depend on geometry, ties broken by id)
+    c = panel()
+    rng = np.random.default_rng(17)
+    pn = rng.permutation(c.N)          # new node k is old node pn[k]
+    inv = np.empty_like(pn)
+    inv[pn] = np.arange(c.N)
+    pt = rng.permutation(c.T)          # new tet k is old tet pt[k]
+    s = sf.Case(c.X[pn], inv[c.index[pt]], flags=c.flags[pn], load=None if c.load is None else c.load[pn],
+                E=c.E, nu=c.nu, rho=c.rho, plastic=c.plastic, yield_=c.yield_, hardening=c.hardening,
+                indenter=c.indenter, sample_every=c.sample_every, failure=c.failure)
+    a, b = sf.Solver(c), sf.Solver(s)
+    assert a.dt == b.dt
+    a.run(500)
+    b.run(500)
+    ra, rb = a.readback(), b.readback()
+    du = np.linalg.norm(rb.u - ra.u[pn]) / np.linalg.norm(ra.u)
+    dq = np.abs(rb.eqps - ra.eqps[pt]).max()
+    ta = sorted((int(np.nonzero(pt == t)[0][0]), st) for t, st, _ in a.tears())
+    tb = sorted((t, st) for t, st, _ in b.tears())
+    assert du <= 1e-6 and dq <= 1e-6, (du, dq)
+    assert ta == tb
+    ea, eb = a.energy(), b.energy()
+    assert [r["step"] for r in ea] == [r["step"] for r in eb]
+    w = abs(ea[-1]["w_contact"])
+    assert max(abs(x["strain"] - y["strain"]) for x, y in zip(ea, eb)) <= 1e-6 * w
+    assert ra.eqps.max() > 0
