@@ -93,7 +93,8 @@ def cpu_baseline(case, budget_s=6.0):
 def bench_batch(args, sf, info):
     """64 plate cases (tets log-uniform in [1e4, 3e5], pressure 50-400 kPa, yield
     235-460 MPa, J2, FP32 + Kahan): the plain path (one case at a time, two plain
-    launches per step) against one batched launch per phase (graph-replayed, tuned)."""
+    launches per step; with this library's tiled kernels, and with the reference's
+    slot scheme) against one batched launch per phase (graph-replayed, tuned)."""
     from shipfem import configs
 
     cases = configs.batch_cases(64)
@@ -115,6 +116,16 @@ def bench_batch(args, sf, info):
         plain_time += time.perf_counter() - t1
         s.close()
     plain_rate = tets * plain_steps / plain_time
+    # the same, with the reference's own scheme: per-tet 48-byte force slots gathered
+    # through the CSR node map (tet_forces.comp + node_integrate.comp)
+    slots_time = 0.0
+    for c in cases:
+        s = sf.Solver(c).set_launch(sf.VARIANT_SLOTS)
+        t1 = time.perf_counter()
+        s.run(plain_steps, mode=1)
+        slots_time += time.perf_counter() - t1
+        s.close()
+    plain_slots_rate = tets * plain_steps / slots_time
     b = sf.Batch(cases, chunk=100)
     best = b.tune(steps_per_trial=200)
     b.reset()
@@ -137,6 +148,8 @@ def bench_batch(args, sf, info):
                           device=info[0], plain_rate=plain_rate, plain_steps=plain_steps,
                           plain_wall_incl_setup_s=plain_wall_total,
                           batched_over_plain=rate / plain_rate,
+                          plain_slots_rate=plain_slots_rate,
+                          batched_over_plain_slots=rate / plain_slots_rate,
                           device_ms_per_step=dev_ms / args.steps,
                           step_hbm_gbs_model=step_bytes * args.steps / wall / 1e9,
                           step_roofline_frac_model=step_bytes * args.steps / wall / 8e12)))
