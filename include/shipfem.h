@@ -368,6 +368,57 @@ shipfem_status shipfem_case_save(const shipfem_case_desc* d, const char* path);
 shipfem_status shipfem_case_load(const char* path, shipfem_case_blob** out,
                                  shipfem_case_desc* d);
 
+/* ---- the solid-shell hexahedron (engine/sim/solid_shell.hpp:111-194) --------- */
+/* An 8-node hex whose nodes 0-3 are the zeta = -1 face and 4-7 the zeta = +1 face,
+ * each counter-clockwise seen from +zeta (solid_shell.hpp:115-125): zeta is the
+ * shell's thickness. Assumed natural transverse shear and thickness strain, seven
+ * enhanced strains condensed at rest, 2x2x2 Gauss. Host FP64. `nodes` is 8 x xyz. */
+
+/* Condensed elastic stiffness, 24 x 24 row-major, global frame
+ * (solidshell::elementStiffness with Formulation::SolidShell, solid_shell.hpp:202-203).
+ * SHIPFEM_INVALID_ARG for a degenerate or inverted element. */
+shipfem_status shipfem_hex_stiffness(const double nodes[24], double youngs_modulus,
+                                     double poisson_ratio, double out[576]);
+/* Lumped mass rho int N_a dV per node (solidshell::elementMass, solid_shell.hpp:330). */
+shipfem_status shipfem_hex_mass(const double nodes[24], double density, double out[8]);
+/* Co-rotational internal force on the nodes, f = -R K (R^T x - X), R the polar factor
+ * of the deformation gradient at the element centre (solidshell::internalForce,
+ * solid_shell.hpp:355-356). */
+shipfem_status shipfem_hex_force(const double stiffness[576], const double rest[24],
+                                 const double current[24], double out[24]);
+/* safety * 2 / omega_max of the element with its lumped mass, omega_max by power
+ * iteration (solidshell::criticalTimestep, solid_shell.hpp:369-370). */
+shipfem_status shipfem_hex_critical_timestep(const double nodes[24], double youngs_modulus,
+                                             double poisson_ratio, double density,
+                                             double safety, double* dt);
+
+/* solidshell::makePlateMesh (solid_shell.hpp:546-552): a plate lx x ly x thickness, nx x ny
+ * elements in plane and nz through it, nodes numbered with the thickness index fastest,
+ * then y, then x, mid-surface at z = 0. Counts only when rest_position and index are
+ * NULL; otherwise rest_position holds 3 * node_count and index 8 * element_count. */
+shipfem_status shipfem_hex_plate_mesh(double lx, double ly, double thickness, int32_t nx,
+                                      int32_t ny, int32_t nz, uint32_t* node_count,
+                                      uint32_t* element_count, double* rest_position,
+                                      uint32_t* index);
+/* solidshell::uniformPressureLoad (solid_shell.hpp:714-719): the consistent nodal load of
+ * a pressure on every element's +zeta face that lies on the mesh boundary, added into
+ * load[3 * node_count]; positive pressure pushes into the element. */
+shipfem_status shipfem_hex_pressure_load(const double* rest_position, uint32_t node_count,
+                                         const uint32_t* index, uint32_t element_count,
+                                         double pressure, double* load);
+
+/* An explicit solve over solid-shell hexes: the elastic zone::Solver step
+ * (engine/sim/zone.cpp:928-1012, 1086-1252 with params.plastic false) on the device.
+ * The case is a shipfem_case_desc read with tet_count as the element count and index
+ * as [8 * tet_count] node ids in the order above; material.plastic must be 0
+ * (SHIPFEM_UNSUPPORTED otherwise). dt = 0 takes dt_safety * 2 / omega_max, omega_max the
+ * largest element frequency (FP64, host). The returned handle takes every
+ * shipfem_solver_* call; shipfem_solver_tet_forces returns [24 * tet_count] (the
+ * element forces of the last element pass, 8 corners x xyz per element), and the
+ * per-element plastic outputs read zero. */
+shipfem_status shipfem_hex_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
+                                         char* err, size_t err_len);
+
 /* ---- many cases in one launch per phase ----------------------------------- */
 typedef struct shipfem_batch shipfem_batch;
 

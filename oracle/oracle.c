@@ -716,7 +716,33 @@ int ora_j2_update(const ora_j2* m, const double strain[6], double eps_p[6], doub
 /* explicit solver                                                          */
 /* ======================================================================== */
 
+static int hex_case(const ora_case* c) { return c->corners == 8; }
+
 int ora_case_init(ora_case* c) {
+    if (hex_case(c)) {
+        /* nodal masses and the (element, corner) CSR of the hex solve (zone.cpp:978-980) */
+        const int64_t N = c->N, CT = 8 * c->T;
+        c->rest_inv = NULL;
+        c->volume = (double*)calloc((size_t)c->T, sizeof(double));
+        c->mass = (double*)calloc((size_t)N, sizeof(double));
+        c->adj_offset = (uint32_t*)calloc((size_t)(N + 1), sizeof(uint32_t));
+        c->adj_entry = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)CT);
+        uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(N + 1));
+        if (!c->volume || !c->mass || !c->adj_offset || !c->adj_entry || !cur) {
+            free(cur);
+            return 1;
+        }
+        for (int64_t q = 0; q < CT; ++q) {
+            c->mass[c->index[q]] += c->hex_mass[q];
+            c->volume[q / 8] += c->hex_mass[q] / c->rho;
+            ++c->adj_offset[c->index[q] + 1];
+        }
+        for (int64_t i = 0; i < N; ++i) c->adj_offset[i + 1] += c->adj_offset[i];
+        memcpy(cur, c->adj_offset, sizeof(uint32_t) * (size_t)N);
+        for (int64_t q = 0; q < CT; ++q) c->adj_entry[cur[c->index[q]]++] = (uint32_t)q;
+        free(cur);
+        return 0;
+    }
     c->rest_inv = (double*)malloc(sizeof(double) * 9 * (size_t)c->T);
     c->volume = (double*)malloc(sizeof(double) * (size_t)c->T);
     c->mass = (double*)malloc(sizeof(double) * (size_t)c->N);
@@ -758,9 +784,86 @@ typedef struct {
     double kinetic, half_ext, half_con, half_bnd, half_damp, fcon[3]; /* sample terms */
 } ora_sums;
 
-/* element pass at u_n over tets [e0, e1) */
+/* Co-rotational force of one solid-shell hex (solidshell::internalForce,
+ * solid_shell.cpp:680-766): R the polar factor of F = J(x) J(X)^-1 at the element centre by
+ * R <- (R + R^-T)/2 until no entry moves by 1e-16 (at most 64 steps), d = R^T x - X,
+ * f = -R K d; adds 1/2 d.K d to *energy when it is not NULL (zone.cpp:1040-1060). */
+static void hex_force(const double* K, const double X[24], const double x[24], double f[24],
+                      double* energy) {
+    static const double sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    static const double sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    static const double sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    double Jx[9], JX[9], JXi[9], r[9], inv[9], it[9];
+    for (int i = 0; i < 3; ++i) {
+        double gx[3] = {0, 0, 0}, gX[3] = {0, 0, 0};
+        for (int a = 0; a < 8; ++a) {
+            const double dn[3] = {0.125 * sx[a], 0.125 * sy[a], 0.125 * sz[a]};
+            for (int k = 0; k < 3; ++k) {
+                gx[k] += dn[k] * x[3 * a + i];
+                gX[k] += dn[k] * X[3 * a + i];
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            AT(Jx, i, k) = gx[k];
+            AT(JX, i, k) = gX[k];
+        }
+    }
+    inv3(JX, JXi);
+    mul3(Jx, JXi, r);
+    for (int step = 0; step < 64; ++step) {
+        inv3(r, inv);
+        transpose3(inv, it);
+        double change = 0;
+        for (int k = 0; k < 9; ++k) {
+            const double nx = 0.5 * (r[k] + it[k]);
+            change = fmax(change, fabs(nx - r[k]));
+            r[k] = nx;
+        }
+        if (change < 1e-16) break;
+    }
+    double d[24], kd[24];
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += AT(r, k, i) * x[3 * a + k];
+            d[3 * a + i] = s - X[3 * a + i];
+        }
+    double e = 0;
+    for (int p = 0; p < 24; ++p) {
+        double s = 0;
+        for (int q = 0; q < 24; ++q) s += K[p * 24 + q] * d[q];
+        kd[p] = s;
+        e += 0.5 * d[p] * s;
+    }
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += AT(r, i, k) * kd[3 * a + k];
+            f[3 * a + i] = -s;
+        }
+    if (energy) *energy += e;
+}
+
+void ora_hex_force(const double K[576], const double X[24], const double x[24], double out[24]) {
+    hex_force(K, X, x, out, NULL);
+}
+
+/* element pass at u_n over elements [e0, e1) */
 static void run_elements(const ora_case* c, ora_state* st, int64_t e0, int64_t e1, int64_t n,
                          int sample, ora_sums* out) {
+    if (hex_case(c)) {
+        for (int64_t e = e0; e < e1; ++e) {
+            const uint32_t* idx = &c->index[e * 8];
+            double X[24], x[24];
+            for (int q = 0; q < 8; ++q)
+                for (int a = 0; a < 3; ++a) {
+                    X[q * 3 + a] = c->X[idx[q] * 3 + a];
+                    x[q * 3 + a] = X[q * 3 + a] + st->u[idx[q] * 3 + a];
+                }
+            hex_force(&c->hex_k[e * 576], X, x, &st->force[e * 24], sample ? &out->strain : NULL);
+        }
+        return;
+    }
     const double lam = c->E * c->nu / ((1 + c->nu) * (1 - 2 * c->nu));
     const double mu = c->E / (2 * (1 + c->nu));
     const ora_j2 j2 = {c->E, c->nu, c->yield, c->hardening, c->failure_strain, c->sensitivity,
@@ -817,7 +920,7 @@ static void run_nodes(const ora_case* c, ora_state* st, int64_t i0, int64_t i1, 
         double f[3] = {0, 0, 0};
         for (uint32_t q = c->adj_offset[i]; q < c->adj_offset[i + 1]; ++q) {
             const uint32_t en = c->adj_entry[q];
-            const double* s = &st->force[(en >> 2) * 12 + (en & 3u) * 3];
+            const double* s = &st->force[(size_t)en * 3]; /* slot of (element, corner) */
             for (int a = 0; a < 3; ++a) f[a] += s[a];
         }
         double fe[3] = {0, 0, -c->gravity * m};

@@ -102,6 +102,10 @@ void ora_face_traction(const double* X, const uint32_t* index, int64_t T, int ax
 void ora_tet_force(const double x[12] /*4 nodes xyz*/, const double rest_inv[9], double volume,
                    double lambda, double mu, double out[12]);
 
+/* Co-rotational force of one solid-shell hex, f = -R K (R^T x - X) with R the polar
+ * factor of the centre deformation gradient (solid_shell.cpp:680-766). */
+void ora_hex_force(const double K[576], const double X[24], const double x[24], double out[24]);
+
 typedef struct ora_j2 {
     double E, nu, yield, hardening; /* linear isotropic hardening */
     /* ductile failure (plasticity.hpp:160-210); failure_strain <= 0 disables it */
@@ -163,6 +167,12 @@ typedef struct ora_case {
     double* mass;
     uint32_t* adj_offset;
     uint32_t* adj_entry;
+    /* Solid-shell hex case when corners == 8 (0 or 4: tets): index holds 8 node ids per
+     * element (T elements), hex_k [T][576] the condensed element stiffness and hex_mass
+     * [T][8] the element masses, both from the caller; elastic only. */
+    int corners;
+    const double* hex_k;
+    const double* hex_mass;
 } ora_case;
 
 typedef struct ora_state {
@@ -173,7 +183,7 @@ typedef struct ora_state {
     double* damage; /* T, may be NULL when failure is off; >= 1 means torn */
     int64_t* tear_step;   /* T, step at which the tet tore, -1 intact (with damage) */
     double* tear_normal;  /* 3T, world-frame failure-plane normal (with damage) */
-    double* force;  /* 12T scratch */
+    double* force;  /* 12T (hex: 24T) scratch */
     int64_t step;   /* n: the state holds u_n, v_{n-1/2} */
     /* energy account: running sums over the steps taken so far */
     double w_ext, w_contact, w_boundary, dissipation, damping_loss;

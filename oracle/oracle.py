@@ -70,7 +70,7 @@ class CaseStruct(C.Structure):
                 ("ind_radius", C.c_double), ("ind_center", C.c_double * 3),
                 ("ind_velocity", C.c_double * 3), ("ind_kappa", C.c_double), ("dt", C.c_double),
                 ("rest_inv", D), ("volume", D), ("mass", D), ("adj_offset", U32),
-                ("adj_entry", U32)]
+                ("adj_entry", U32), ("corners", C.c_int), ("hex_k", D), ("hex_mass", D)]
 
 
 class StateStruct(C.Structure):
@@ -94,6 +94,7 @@ ENERGY_FIELDS = ["strain", "kinetic", "dissipation", "w_ext", "w_contact", "w_bo
 
 
 def _declare(L):
+    L.ora_hex_force.argtypes = [D, D, D, D]
     L.ora_box_mesh.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [I64, I64, D, U32]
     L.ora_box_mesh_f32.argtypes = [C.c_float] * 3 + [C.c_int] * 3 + [F, U32]
     L.ora_uniform.argtypes = [C.c_uint64, C.c_uint64]
@@ -265,9 +266,19 @@ class Case:
     def __init__(self, X, index, *, E=210e9, nu=0.3, rho=7850.0, dt, flags=None, load=None,
                  prescribed=None, plastic=False, yield_=355e6, hardening=1e9, load_ramp=0.0,
                  drive_ramp=0.0, gravity=0.0, damping=1.0, indenter=None, failure=None):
+        """index [T, 4] solves tets; index [T, 8] solves solid-shell hexes (elastic), their
+        condensed stiffness and masses from oracle.hex."""
         self.X = np.ascontiguousarray(X, np.float64)
         self.index = np.ascontiguousarray(index, np.uint32)
         self.N, self.T = len(self.X), len(self.index)
+        self.corners = self.index.shape[1]
+        if self.corners == 8:
+            if plastic:
+                raise ValueError("the hex oracle is elastic")
+            from oracle import hex as H
+            nodes = self.X[self.index]
+            self.hex_k = np.ascontiguousarray(H.stiffness(nodes, E, nu))
+            self.hex_mass = np.ascontiguousarray(H.mass(nodes, rho))
         self.flags = None if flags is None else np.ascontiguousarray(flags, np.uint32)
         self.load = None if load is None else np.ascontiguousarray(load, np.float64)
         self.prescribed = None if prescribed is None else np.ascontiguousarray(prescribed, np.float64)
@@ -296,6 +307,9 @@ class Case:
             s.ind_velocity = (C.c_double * 3)(*indenter["velocity"])
             s.ind_kappa = indenter.get("kappa", 0.1)
         s.dt = dt
+        if self.corners == 8:
+            s.corners = 8
+            s.hex_k, s.hex_mass = _p(self.hex_k, C.c_double), _p(self.hex_mass, C.c_double)
         if lib().ora_case_init(C.byref(s)) != 0:
             raise MemoryError("oracle case init")
         self.s = s
@@ -305,7 +319,7 @@ class Case:
         self.v = np.zeros((self.N, 3))
         self.eps_p = np.zeros((self.T, 6))
         self.eqps = np.zeros(self.T)
-        self.force = np.zeros((self.T, 12))
+        self.force = np.zeros((self.T, 3 * self.corners))
         self.damage = np.zeros(self.T)
         self.tear_step = np.full(self.T, -1, np.int64)
         self.tear_normal = np.zeros((self.T, 3))
@@ -474,5 +488,13 @@ def ref_lib():
         R.ref_regularised_failure_strain.restype = C.c_double
         R.ref_triaxiality_factor.argtypes = [C.c_double] * 4
         R.ref_triaxiality_factor.restype = C.c_double
+        R.ref_hex_stiffness.argtypes = [D, C.c_double, C.c_double, C.c_int, D]
+        R.ref_hex_mass.argtypes = [D, C.c_double, D]
+        R.ref_hex_force.argtypes = [D, D, D, D]
+        R.ref_hex_rotation.argtypes = [D, D, D]
+        R.ref_hex_critical_timestep.argtypes = [D] + [C.c_double] * 4
+        R.ref_hex_critical_timestep.restype = C.c_double
+        R.ref_hex_plate_counts.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [I64, I64]
+        R.ref_hex_plate.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [C.c_double, D, U32, D]
         _ref = R
     return _ref

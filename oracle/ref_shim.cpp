@@ -5,10 +5,13 @@
 // Built by oracle/Makefile together with the reference sources where they lie:
 //   engine/sim/fem.cpp         (makeBoxTetMesh, computeRestState, tetForces, stepCpu)
 //   engine/sim/plasticity.cpp  (update, regularisedFailureStrain, triaxialityFactor)
+//   engine/sim/solid_shell.cpp (elementStiffness, elementMass, internalForce,
+//                               elementRotation, criticalTimestep of the hex element)
 // into oracle/_ref/libref.so. Nothing here restates the reference; it only moves
 // arrays in and out of sim::fem::TetMesh and sim::plasticity::State.
 #include "engine/sim/fem.hpp"
 #include "engine/sim/plasticity.hpp"
+#include "engine/sim/solid_shell.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -20,6 +23,20 @@ sim::fem::TetMesh meshFrom(const float* position, int64_t N, const uint32_t* ind
     mesh.position.assign(position, position + N * 3);
     mesh.index.assign(index, index + T * 4);
     return mesh;
+}
+
+sim::StructuralMaterial structuralOf(double E, double nu, double rho) {
+    sim::StructuralMaterial m;
+    m.youngsModulus = E;
+    m.poissonRatio = nu;
+    m.density = rho;
+    return m;
+}
+
+sim::solidshell::Formulation formOf(int form) {
+    return form == 0   ? sim::solidshell::Formulation::Displacement
+           : form == 1 ? sim::solidshell::Formulation::AssumedNaturalStrain
+                       : sim::solidshell::Formulation::SolidShell;
 }
 
 sim::fem::Material materialOf(float E, float nu, float rho) {
@@ -161,6 +178,48 @@ double ref_triaxiality_factor(double sensitivity, double reference_eta, double c
     f.referenceTriaxiality = reference_eta;
     f.cutoffTriaxiality = cutoff_eta;
     return sim::plasticity::triaxialityFactor(f, eta);
+}
+
+// The solid-shell hex (8 nodes x xyz, node-major). form: 0 Displacement,
+// 1 AssumedNaturalStrain, 2 SolidShell.
+void ref_hex_stiffness(const double nodes[24], double E, double nu, int form, double out[576]) {
+    sim::solidshell::elementStiffness(nodes, structuralOf(E, nu, 7850.0), formOf(form), out);
+}
+
+void ref_hex_mass(const double nodes[24], double rho, double out[8]) {
+    sim::solidshell::elementMass(nodes, rho, out);
+}
+
+void ref_hex_force(const double stiffness[576], const double rest[24], const double current[24],
+                   double out[24]) {
+    sim::solidshell::internalForce(stiffness, rest, current, out);
+}
+
+void ref_hex_rotation(const double rest[24], const double current[24], double out[9]) {
+    sim::solidshell::elementRotation(rest, current, out);
+}
+
+// makePlateMesh counts, then arrays; uniformPressureLoad on it.
+void ref_hex_plate_counts(double lx, double ly, double t, int nx, int ny, int nz, int64_t* nodes,
+                          int64_t* elements) {
+    const sim::solidshell::HexMesh m = sim::solidshell::makePlateMesh(lx, ly, t, nx, ny, nz);
+    *nodes = static_cast<int64_t>(m.nodeCount());
+    *elements = static_cast<int64_t>(m.elementCount());
+}
+
+void ref_hex_plate(double lx, double ly, double t, int nx, int ny, int nz, double pressure,
+                   double* position, uint32_t* index, double* load) {
+    const sim::solidshell::HexMesh m = sim::solidshell::makePlateMesh(lx, ly, t, nx, ny, nz);
+    std::memcpy(position, m.position.data(), m.position.size() * sizeof(double));
+    std::memcpy(index, m.index.data(), m.index.size() * sizeof(uint32_t));
+    const std::vector<double> f = sim::solidshell::uniformPressureLoad(m, pressure);
+    std::memcpy(load, f.data(), f.size() * sizeof(double));
+}
+
+double ref_hex_critical_timestep(const double nodes[24], double E, double nu, double rho,
+                                 double safety) {
+    return sim::solidshell::criticalTimestep(nodes, structuralOf(E, nu, rho),
+                                             sim::solidshell::Formulation::SolidShell, safety);
 }
 
 }  // extern "C"

@@ -121,8 +121,13 @@ uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t
     return b + 256ull * 40;  // each of the arena's planes rounds up to 256 bytes
 }
 
-shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
-                                     char* err, size_t err_len) {
+namespace {
+
+// One case on its own stream: prepares (tet or hex), uploads, builds the plan and sets
+// dt from the largest element frequency (tets: FP64 on the device; hexes: FP64 power
+// iteration on the host during preparation).
+shipfem_status createSolver(const shipfem_case_desc* desc, bool hex, shipfem_solver** out,
+                            char* err, size_t err_len) {
     if (!desc || !out) {
         setError(err, err_len, "null argument");
         return SHIPFEM_INVALID_ARG;
@@ -135,7 +140,7 @@ shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solv
     }
     auto* s = new (std::nothrow) shipfem_solver();
     if (!s) return SHIPFEM_OUT_OF_MEMORY;
-    const std::string msg = prepareCase(*desc, s->hc);
+    const std::string msg = hex ? prepareHexCase(*desc, s->hc) : prepareCase(*desc, s->hc);
     if (!msg.empty()) {
         setError(err, err_len, msg);
         delete s;
@@ -159,15 +164,20 @@ shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solv
         e = s->plan.build(s->stream);
     }
     std::vector<double> lmax;
-    if (e == hipSuccess) e = s->plan.omegaMax(lmax, s->stream);
+    if (e == hipSuccess && !hex) e = s->plan.omegaMax(lmax, s->stream);
     if (e != hipSuccess) {
         setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
         const shipfem_status st = statusOf(e);
         delete s;
         return st;
     }
-    s->dc.omega = std::sqrt(4.0 * lmax[0] / s->hc.rho);
-    s->dc.cfl = omegaToCfl(lmax[0], s->hc.rho);
+    if (hex) {
+        s->dc.omega = s->hc.hex_omega;
+        s->dc.cfl = 2.0 / s->hc.hex_omega;
+    } else {
+        s->dc.omega = std::sqrt(4.0 * lmax[0] / s->hc.rho);
+        s->dc.cfl = omegaToCfl(lmax[0], s->hc.rho);
+    }
     setTimestep(s->dc, s->hc, s->hc.dt_request > 0 ? s->hc.dt_request : s->hc.dt_safety * s->dc.cfl);
     s->plan.recs[0] = s->dc.rec;
     e = s->plan.uploadRecords(s->stream);
@@ -178,6 +188,23 @@ shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solv
     }
     *out = s;
     return SHIPFEM_OK;
+}
+
+}  // namespace
+
+shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
+                                     char* err, size_t err_len) {
+    return createSolver(desc, false, out, err, err_len);
+}
+
+shipfem_status shipfem_hex_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
+                                         char* err, size_t err_len) {
+    if (desc && desc->material.plastic) {
+        setError(err, err_len, "the hex solve is elastic: material.plastic must be 0");
+        if (out) *out = nullptr;
+        return SHIPFEM_UNSUPPORTED;
+    }
+    return createSolver(desc, true, out, err, err_len);
 }
 
 shipfem_status shipfem_solver_timestep(const shipfem_solver* s, double* cfl, double* dt,
@@ -268,15 +295,17 @@ shipfem_status shipfem_solver_tears(shipfem_solver* s, shipfem_tear* out, uint32
 
 shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out) {
     if (!s || !out) return SHIPFEM_INVALID_ARG;
-    // only the slots variant keeps per-tet force slots; the tiled one sums in LDS
-    if (s->plan.shape.variant != kSlots) return SHIPFEM_UNSUPPORTED;
+    // only the slots variant keeps per-tet force slots; the tiled one sums in LDS (a hex
+    // case always writes slots)
+    if (s->plan.shape.variant != kSlots && !s->plan.hex) return SHIPFEM_UNSUPPORTED;
     const HostCase& hc = s->hc;
-    std::vector<float> f(12ull * hc.T);
-    hipError_t e = hipMemcpyAsync(f.data(), s->dc.rec.force, 48ull * hc.T, hipMemcpyDeviceToHost, s->stream);
+    const uint64_t per = 3ull * hc.corners;
+    std::vector<float> f(per * hc.T);
+    hipError_t e = hipMemcpyAsync(f.data(), s->dc.rec.force, 4 * f.size(), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return statusOf(e);
-    for (uint32_t t = 0; t < hc.T; ++t)  // back in the caller's tet order (corners keep theirs)
-        std::memcpy(out + 12ull * externalTet(hc, t), f.data() + 12ull * t, 48);
+    for (uint32_t t = 0; t < hc.T; ++t)  // back in the caller's element order (corners keep theirs)
+        std::memcpy(out + per * externalTet(hc, t), f.data() + per * t, 4 * per);
     return SHIPFEM_OK;
 }
 

@@ -38,7 +38,10 @@ void Plan::dropGraph() {
 
 hipError_t Plan::build(hipStream_t s) {
     dropGraph();
-    tilesFor(recs, shape.variant == kTiled ? kTile : shape.elem_block, false, elem_tiles);
+    hex = !recs.empty() && recs[0].corners == 8;
+    for (const CaseDev& r : recs)
+        if ((r.corners == 8) != hex) return hipErrorInvalidValue;  // one element kind per plan
+    tilesFor(recs, shape.variant == kTiled && !hex ? kTile : shape.elem_block, false, elem_tiles);
     lmax = 0;
     for (const CaseDev& r : recs) lmax = std::max(lmax, r.lmax);
     tilesFor(recs, shape.node_block, true, node_tiles);
@@ -93,7 +96,10 @@ void Plan::launchStep(int par, int sample, hipStream_t s) {
 }
 
 void Plan::launchElem(int par, int sample, hipStream_t s, const KernelTiming* t) {
-    if (shape.variant == kTiled)
+    if (hex)
+        launch_elem_hex(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
+                        sample, shape.xcd_remap, s, t);
+    else if (shape.variant == kTiled)
         launch_elem_tiled(d_recs, d_elem, static_cast<int>(elem_tiles.size()), lmax, par, sample,
                           shape.xcd_remap, feat_union, s, t);
     else
@@ -102,8 +108,9 @@ void Plan::launchElem(int par, int sample, hipStream_t s, const KernelTiming* t)
 }
 
 void Plan::launchNode(int par, int sample, hipStream_t s, const KernelTiming* t) {
+    // a hex case's element pass always writes slots
     launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, par, sample,
-                shape.xcd_remap, feat_union, shape.variant, s, t);
+                shape.xcd_remap, feat_union, hex ? kSlots : shape.variant, s, t);
 }
 
 void Plan::stepPlain(bool sample, hipStream_t s) {

@@ -368,6 +368,142 @@ __global__ __launch_bounds__(BLOCK) void k_elem(const CaseDev* __restrict__ case
     element_wave_sums<PLASTIC, DAMAGE>(c, e, sample, diss, se, fract);
 }
 
+// ---------------------------------------------------------------- element pass, hex
+// One lane per solid-shell hex: f = -R K (R^T x - X) (solid_shell.cpp:735-766) with K the
+// condensed elastic stiffness formed at rest (FP64), R the polar factor of the centre
+// deformation gradient. K annihilates rigid translations, so the element works on the
+// displacements of corners 1..7 relative to corner 0 and the force on corner 0 is minus
+// the sum of the others: only the 21 x 21 block of K that couples corners 1..7 is read
+// (231 packed doubles per element, the pass's dominant traffic). With R = I + W and
+// corner offsets r_a = u_a - u_0 (each displacement taken as its FP32 value minus its
+// Kahan compensation), R^T x_a - X_a - (R^T x_0 - X_0) = r_a + W^T (X_a - X_0 + r_a): every
+// operand is small, so nothing cancels at FP32. The product with K and the strain energy
+// 1/2 d.Kd are FP64; the nodal forces are written as FP32 slots [T][8][3].
+__device__ __forceinline__ void polar_perturbation(float W[9]) {
+    // R = I + W by R <- (R + R^-T)/2 in the same perturbation form as the tet pass, six
+    // steps: quadratic convergence takes a stretch error below 0.5 under 2^-40 of it
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {
+        const float a[9] = {W[4] * W[8] - W[5] * W[7], W[2] * W[7] - W[1] * W[8], W[1] * W[5] - W[2] * W[4],
+                            W[5] * W[6] - W[3] * W[8], W[0] * W[8] - W[2] * W[6], W[2] * W[3] - W[0] * W[5],
+                            W[3] * W[7] - W[4] * W[6], W[1] * W[6] - W[0] * W[7], W[0] * W[4] - W[1] * W[3]};
+        const float t = W[0] + W[4] + W[8];
+        const float i2 = a[0] + a[4] + a[8];
+        const float dW = W[0] * a[0] + W[1] * a[3] + W[2] * a[6];
+        const float den = 1.0f + (t + (i2 + dW));
+        const bool ok = fabsf(den) > 1e-30f;
+        const float id = ok ? __builtin_amdgcn_rcpf(den) : 0.0f;
+        const float hid = 0.5f * id, hnd = ok ? 0.0f : 0.5f;
+        const float sh = i2 + dW;
+        float Wn[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float x = a[j * 3 + i] - W[j * 3 + i];
+                Wn[i * 3 + j] = i == j ? fmaf(0.5f, W[i * 3 + j], fmaf(hid, x - sh, -hnd))
+                                       : fmaf(0.5f, W[i * 3 + j], hid * x);
+            }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) W[k] = Wn[k];
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_elem_hex(const CaseDev* __restrict__ cases,
+                                                    const Tile* __restrict__ tiles, int ntiles,
+                                                    int parity, int sample, int xcd) {
+    const int b = remap_block(blockIdx.x, gridDim.x, xcd);
+    const Tile tile = tiles[b];
+    const CaseDev& c = cases[tile.case_index];
+    if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
+
+    const uint32_t e = tile.first + threadIdx.x;
+    double se = 0.0;
+    if (e < c.T) {
+        const uint32_t tp = c.Tpad, np = c.Npad;
+        const uint4 q0 = c.conn[2 * e], q1 = c.conn[2 * e + 1];
+        const uint32_t g[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        float uh[8][3], uc[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float4 p = c.u4[g[a]];
+            uh[a][0] = p.x;
+            uh[a][1] = p.y;
+            uh[a][2] = p.z;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) uc[a][i] = c.cu[i * np + g[a]];
+        }
+        // offsets from corner 0: the FP32 difference, then the compensations' difference
+        float r[7][3], H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < 7; ++a)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) r[a][i] = (uh[a + 1][i] - uh[0][i]) - (uc[a + 1][i] - uc[0][i]);
+#pragma unroll
+        for (int a = 0; a < 7; ++a) {
+            const float b0[3] = {c.hgeo[(3 * a) * tp + e], c.hgeo[(3 * a + 1) * tp + e],
+                                 c.hgeo[(3 * a + 2) * tp + e]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) H[i * 3 + j] = fmaf(r[a][i], b0[j], H[i * 3 + j]);
+        }
+        polar_perturbation(H);  // H now holds W, R = I + W
+        double d[kHexDof];
+#pragma unroll
+        for (int a = 0; a < 7; ++a) {
+            float y[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) y[i] = c.hgeo[(kHexDof + 3 * a + i) * tp + e] + r[a][i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                d[3 * a + i] = (double)(r[a][i] + (H[0 * 3 + i] * y[0] + H[1 * 3 + i] * y[1] +
+                                                   H[2 * 3 + i] * y[2]));
+        }
+        double kd[kHexDof];
+#pragma unroll
+        for (int i = 0; i < kHexDof; ++i) kd[i] = 0.0;
+        const double* kp = c.hk + e;
+#pragma unroll
+        for (int i = 0; i < kHexDof; ++i)
+#pragma unroll
+            for (int j = i; j < kHexDof; ++j) {
+                const double k = kp[(size_t)(i * kHexDof - i * (i - 1) / 2 + j - i) * tp];
+                kd[i] = fma(k, d[j], kd[i]);
+                if (j != i) kd[j] = fma(k, d[i], kd[j]);
+            }
+        if (sample) {
+#pragma unroll
+            for (int i = 0; i < kHexDof; ++i) se = fma(d[i], kd[i], se);
+            se *= 0.5;
+        }
+        float f[24];
+        double k0[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int a = 0; a < 7; ++a)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) k0[i] -= kd[3 * a + i];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float v[3] = {(float)(a == 0 ? k0[0] : kd[3 * a - 3]),
+                                (float)(a == 0 ? k0[1] : kd[3 * a - 2]),
+                                (float)(a == 0 ? k0[2] : kd[3 * a - 1])};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                f[3 * a + i] = -(v[i] + (H[i * 3 + 0] * v[0] + H[i * 3 + 1] * v[1] + H[i * 3 + 2] * v[2]));
+        }
+        float4* out = reinterpret_cast<float4*>(c.force) + (size_t)e * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+    }
+    if (sample) {
+        const double s = wave_sum(se);
+        const uint32_t w = e >> 6;
+        if ((threadIdx.x & 63u) == 0 && w < c.elem_waves) c.smp_e[kSmpStrain * c.elem_waves + w] = s;
+    }
+}
+
 // ---------------------------------------------------------------- element pass, tiled
 // One block per 256-tet tile. The tile's local nodes are numbered by descending corner
 // count and each has a contiguous run of 4-kTile corner slots, in (tet, corner) order,
@@ -997,6 +1133,28 @@ void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
         case 128: node_launch_v<128>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream, t); break;
         case 512: node_launch_v<512>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream, t); break;
         default: node_launch_v<256>(cases, tiles, ntiles, parity, sample, xcd, extra, variant, stream, t); break;
+    }
+}
+
+void launch_elem_hex(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
+                     int sample, int xcd, hipStream_t stream, const KernelTiming* t) {
+    switch (block) {
+        case 64:
+            SHIPFEM_LAUNCH(t, (k_elem_hex<64>), dim3(ntiles), dim3(64), 0, stream, cases, tiles, ntiles,
+                           parity, sample, xcd);
+            break;
+        case 128:
+            SHIPFEM_LAUNCH(t, (k_elem_hex<128>), dim3(ntiles), dim3(128), 0, stream, cases, tiles, ntiles,
+                           parity, sample, xcd);
+            break;
+        case 512:
+            SHIPFEM_LAUNCH(t, (k_elem_hex<512>), dim3(ntiles), dim3(512), 0, stream, cases, tiles, ntiles,
+                           parity, sample, xcd);
+            break;
+        default:
+            SHIPFEM_LAUNCH(t, (k_elem_hex<256>), dim3(ntiles), dim3(256), 0, stream, cases, tiles, ntiles,
+                           parity, sample, xcd);
+            break;
     }
 }
 

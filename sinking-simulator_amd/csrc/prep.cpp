@@ -34,7 +34,7 @@ struct Layout {
 
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
-    uint64_t conn, dmi, vol, force, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
+    uint64_t conn, dmi, vol, force, hk, hgeo, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
         ell_off, ell, lconn, tn_off, tnode, lofs, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
@@ -43,10 +43,12 @@ ArenaMap mapArena(const HostCase& hc) {
     Layout L;
     ArenaMap m{};
     const uint64_t T = hc.T, N = hc.N, Tp = hc.Tpad, Np = hc.Npad;
-    m.conn = L.take(16 * T);
-    m.dmi = L.take(36 * Tp);
-    m.vol = L.take(4 * Tp);
-    m.force = L.take(48 * T);
+    m.conn = L.take(4ull * hc.corners * T);
+    m.dmi = L.take(4ull * hc.dmi.size());
+    m.vol = L.take(4ull * hc.vol.size());
+    m.force = L.take(12ull * hc.corners * T);
+    m.hk = L.take(8ull * hc.hk.size());
+    m.hgeo = L.take(4ull * hc.hgeo.size());
     const bool dmg = (hc.features & kFeatDamage) != 0;
     m.pstate = L.take((hc.features & kFeatPlastic) ? (dmg ? 32 : 28) * Tp : 0);
     m.tear_step = L.take(dmg ? 4 * Tp : 0);
@@ -63,7 +65,7 @@ ArenaMap mapArena(const HostCase& hc) {
     m.presc = L.take((hc.features & kFeatDrive) ? 12 * Np : 0);
     m.ell_off = L.take(4 * (uint64_t)hc.ell_off.size());
     m.ell = L.take(4 * (uint64_t)hc.ell.size());
-    m.lconn = L.take(8 * T);
+    m.lconn = L.take(4ull * hc.lconn.size());
     m.tn_off = L.take(4 * (uint64_t)hc.tn_off.size());
     m.tnode = L.take(4 * (uint64_t)hc.tnode.size());
     m.lofs = L.take(2 * (uint64_t)hc.lofs.size());
@@ -197,11 +199,12 @@ std::vector<uint32_t> tileNodeOrder(const uint32_t* index, const std::vector<uin
 }
 
 std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap,
-                            std::vector<uint32_t>&& tperm, std::vector<uint32_t>&& nperm) {
+                            std::vector<uint32_t>&& tperm, std::vector<uint32_t>&& nperm,
+                            uint32_t corners) {
     const uint32_t N = d.node_count, T = d.tet_count;
     if (N == 0 || T == 0) return "empty mesh (node_count and tet_count must be positive)";
     if (!d.rest_position || !d.index) return "rest_position and index are required";
-    if (T > (0xFFFFFFFFu >> 2) - 64) return "tet_count too large for 32-bit slot entries";
+    if (T > 0xFFFFFFFFu / corners - 64) return "element count too large for 32-bit slot entries";
     const shipfem_material& mt = d.material;
     if (!(mt.youngs_modulus > 0)) return fmt("youngs_modulus must be positive", mt.youngs_modulus);
     if (!(mt.poisson_ratio > -1.0 && mt.poisson_ratio < 0.5))
@@ -234,6 +237,7 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
     }
 
     hc = HostCase();
+    hc.corners = corners;
     hc.tperm = std::move(tperm);
     hc.nperm = std::move(nperm);
     hc.N = N;
@@ -273,19 +277,25 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
         if (f & ~kKnownFlags) return "node " + std::to_string(externalNode(hc, n)) + " has unknown flag bits";
         hc.nflags[n] = f & ~(SHIPFEM_LOADED | SHIPFEM_CONTACT);  // derived below
     }
-    for (uint64_t q = 0; q < 4ull * T; ++q)
-        if (d.index[q] >= N) return "tet " + std::to_string(q / 4) + " references node " +
+    for (uint64_t q = 0; q < uint64_t(corners) * T; ++q)
+        if (d.index[q] >= N) return "element " + std::to_string(q / corners) + " references node " +
                                     std::to_string(d.index[q]) + " >= node_count";
 
-    // rest state in FP64: Dm^-1 (row-major planes), volume, lumped mass
+    // rest state in FP64: Dm^-1 (row-major planes), volume, lumped mass (tets); the
+    // condensed stiffness, centre gradients and mass of every element (hexes)
     hc.X.assign(d.rest_position, d.rest_position + 3ull * N);
-    hc.conn.assign(d.index, d.index + 4ull * T);
-    hc.dmi.assign(9ull * hc.Tpad, 0.0f);
-    hc.vol.assign(hc.Tpad, 0.0f);
+    hc.conn.assign(d.index, d.index + uint64_t(corners) * T);
     std::vector<double> mass(N, 0.0);
     const double* X = d.rest_position;
     double shortest = std::numeric_limits<double>::infinity();
-    for (uint32_t t = 0; t < T; ++t) {
+    if (corners == 8) {
+        const std::string why = hexRestState(d, hc, mass);
+        if (!why.empty()) return why;
+    } else {
+        hc.dmi.assign(9ull * hc.Tpad, 0.0f);
+        hc.vol.assign(hc.Tpad, 0.0f);
+    }
+    for (uint32_t t = 0; t < T && corners == 4; ++t) {
         const uint32_t* v = &d.index[4ull * t];
         double Dm[3][3];  // Dm[row][col], col c = X_{c+1} - X_0
         double longest2 = 0;
@@ -323,7 +333,7 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
         hc.total_volume += V;
         for (int c = 0; c < 4; ++c) mass[v[c]] += hc.rho * V * 0.25;
     }
-    hc.edge_dt = 0.5 * shortest / std::sqrt(hc.E / hc.rho);
+    hc.edge_dt = corners == 4 ? 0.5 * shortest / std::sqrt(hc.E / hc.rho) : 0.0;
     hc.mass.assign(hc.Npad, 0.0f);
     hc.invm.assign(hc.Npad, 0.0f);
     for (uint32_t n = 0; n < N; ++n) {
@@ -384,14 +394,15 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
                   (anyContact ? kFeatContact : 0) | (anyDrive ? kFeatDrive : 0) |
                   (damping != 1.0 ? kFeatDamp : 0) | (d.gravity != 0.0 ? kFeatGravity : 0);
 
-    // CSR node -> (tet*4 + corner) by counting sort, (tet, corner) order (fem.cpp:147-157),
-    // then wave-transposed: wave w holds width_w rows of 64 lanes
+    // CSR node -> (element*corners + corner) by counting sort, (element, corner) order
+    // (fem.cpp:147-157, zone.cpp:978-980), then wave-transposed: wave w holds width_w
+    // rows of 64 lanes
+    const uint64_t CT = uint64_t(corners) * T;
     std::vector<uint32_t> off(N + 1, 0);
-    for (uint64_t q = 0; q < 4ull * T; ++q) ++off[d.index[q] + 1];
+    for (uint64_t q = 0; q < CT; ++q) ++off[d.index[q] + 1];
     for (uint32_t n = 0; n < N; ++n) off[n + 1] += off[n];
-    std::vector<uint32_t> ent(4ull * T), cur(off.begin(), off.end() - 1);
-    for (uint32_t t = 0; t < T; ++t)
-        for (uint32_t c = 0; c < 4; ++c) ent[cur[d.index[4ull * t + c]]++] = t * 4 + c;
+    std::vector<uint32_t> ent(CT), cur(off.begin(), off.end() - 1);
+    for (uint64_t q = 0; q < CT; ++q) ent[cur[d.index[q]]++] = static_cast<uint32_t>(q);
     hc.ell_off.assign(hc.node_waves + 1, 0);
     uint64_t total = 0;
     for (uint32_t w = 0; w < hc.node_waves; ++w) {
@@ -420,8 +431,8 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
     // consecutive partial slots. Each local node owns a contiguous run of the tile's
     // corner slots, in (tet, corner) order, from its offset lofs[l]; a tet's
     // connectivity packs, per corner, the local node (10 bits) and the corner's rank in
-    // that run (6 bits).
-    {
+    // that run (6 bits). Hex cases have no tiles: their element pass writes slots.
+    if (corners == 4) {
         const uint32_t nt = (T + kTile - 1) / kTile;
         hc.elem_tiles = nt;
         hc.lconn.assign(2ull * T, 0u);
@@ -561,7 +572,12 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
     dp.prescribed = nodeVec(d.prescribed, pc.presc);
     dp.initial_displacement = nodeVec(d.initial_displacement, pc.u0);
     dp.initial_velocity = nodeVec(d.initial_velocity, pc.v0);
-    return prepareInternal(dp, hc, hist_cap, std::move(tperm), std::move(nperm));
+    return prepareInternal(dp, hc, hist_cap, std::move(tperm), std::move(nperm), 4);
+}
+
+std::string prepareHexCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap) {
+    if (d.material.plastic) return "the hex solve is elastic: material.plastic must be 0";
+    return prepareInternal(d, hc, hist_cap, {}, {}, 8);
 }
 
 uint64_t arenaBytes(const HostCase& hc) { return mapArena(hc).total; }
@@ -579,7 +595,10 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.node_waves = hc.node_waves;
     r.elem_waves = hc.elem_waves;
     r.features = hc.features;
+    r.corners = hc.corners;
     r.conn = reinterpret_cast<const uint4*>(base + m.conn);
+    r.hk = hc.hk.empty() ? nullptr : reinterpret_cast<const double*>(base + m.hk);
+    r.hgeo = hc.hgeo.empty() ? nullptr : reinterpret_cast<const float*>(base + m.hgeo);
     r.dmi = reinterpret_cast<const float*>(base + m.dmi);
     r.vol = reinterpret_cast<const float*>(base + m.vol);
     r.force = reinterpret_cast<float*>(base + m.force);
@@ -644,6 +663,8 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.conn, hc.conn.data(), 4ull * hc.conn.size());
     up(m.dmi, hc.dmi.data(), 4ull * hc.dmi.size());
     up(m.vol, hc.vol.data(), 4ull * hc.vol.size());
+    up(m.hk, hc.hk.data(), 8ull * hc.hk.size());
+    up(m.hgeo, hc.hgeo.data(), 4ull * hc.hgeo.size());
     up(m.u4, hc.u4.data(), 4ull * hc.u4.size());
     up(m.cu, hc.cu.data(), 4ull * hc.cu.size());
     up(m.v, hc.v.data(), 4ull * hc.v.size());

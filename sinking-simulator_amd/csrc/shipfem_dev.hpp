@@ -47,17 +47,27 @@ struct EnergyRecordDev {
     double contact_force[3];
 };
 
+// Solid-shell hex: the 21 x 21 block of the condensed stiffness that couples corners
+// 1..7, packed upper triangle row by row (entry (i, j), i <= j, at i*21 - i(i-1)/2 + j - i).
+constexpr int kHexDof = 21;
+constexpr int kHexPacked = kHexDof * (kHexDof + 1) / 2;  // 231
+constexpr int kHexGeo = 42;  // centre gradients of corners 1..7, then X_a - X_0
+
 struct CaseDev {
-    uint32_t N, T;            // nodes, tets
+    uint32_t N, T;            // nodes, elements (tets, or hexes when corners == 8)
     uint32_t Npad, Tpad;      // plane strides (multiples of 64)
     uint32_t node_waves, elem_waves;
     uint32_t features;
-    uint32_t pad0;
+    uint32_t corners;         // 4: tet, 8: solid-shell hex
     // ---- element arrays
-    const uint4* conn;   // [T] node ids
+    const uint4* conn;   // [T] node ids (tet), or [2T] (hex: corners 0-3, then 4-7)
     const float* dmi;    // [9][Tpad] Dm^-1, row-major entry k = 3*row + col
     const float* vol;    // [Tpad]
-    float* force;        // [T][12] per-tet nodal forces, reference slot layout
+    float* force;        // [T][3 corners] per-element nodal forces, reference slot layout
+                         // (the slot of corner entry x = corners * element + corner is 3x)
+    const double* hk;    // hex: [kHexPacked][Tpad] stiffness block of corners 1..7 (FP64)
+    const float* hgeo;   // hex: [kHexGeo][Tpad] d N_a/dX at the centre (21: 3(a-1) + j),
+                         // then rest X_a - X_0 (21: 21 + 3(a-1) + i), corners a = 1..7
     float* pstate;       // [7 or 8][Tpad] plastic strain (6, eng. shears), eqps, damage
     int32_t* tear_step;  // [Tpad] step + 1 at which the tet tore, 0 intact (kFeatDamage)
     float* tear_stress;  // [6][Tpad] world-frame stress at the tear (kFeatDamage)
@@ -148,6 +158,9 @@ struct KernelTiming {
 void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
                  int sample, int xcd, uint32_t feat_union, hipStream_t stream,
                  const KernelTiming* timing = nullptr);
+// The solid-shell hex element pass: one lane per element, `block` threads per block.
+void launch_elem_hex(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
+                     int sample, int xcd, hipStream_t stream, const KernelTiming* timing = nullptr);
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
                        int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream,
                        const KernelTiming* timing = nullptr);

@@ -35,7 +35,11 @@ struct HostCase {
     // caller's node nperm[i]. Every array below is in the internal orders.
     std::vector<uint32_t> tperm, nperm;
     std::vector<double> X;        // [3N] rest positions
-    std::vector<uint32_t> conn;   // [4T]
+    uint32_t corners = 4;         // 4: tet, 8: solid-shell hex
+    std::vector<uint32_t> conn;   // [corners T]
+    std::vector<double> hk;       // hex: [kHexPacked][Tpad]
+    std::vector<float> hgeo;      // hex: [kHexGeo][Tpad]
+    double hex_omega = 0;         // hex: largest element frequency (FP64, host)
     std::vector<float> dmi;       // [9][Tpad]
     std::vector<float> vol;       // [Tpad]
     std::vector<float> mass, invm;  // [Npad]
@@ -76,6 +80,17 @@ inline uint32_t externalNode(const HostCase& hc, uint32_t n) { return hc.nperm.e
 // case is laid out in the internal tet and node orders; results go back through
 // externalTet / externalNode.
 std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap = -1);
+// The same for a solid-shell hex case: index holds 8 node ids per element (tet_count
+// elements), kept in the caller's orders.
+std::string prepareHexCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap = -1);
+// Hex rest state (hex.cpp): the packed stiffness block, centre gradients and rest offsets
+// of every element, nodal masses accumulated into `mass`, and hc.hex_omega.
+std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<double>& mass);
+// FP64 solid-shell element (hex.cpp).
+bool hexStiffness(const double* x, double E, double nu, double K[576]);
+bool hexMass(const double* x, double rho, double m[8]);
+bool hexCentreGradient(const double* x, double b0[8][3]);
+double hexOmegaMax(const double K[576], const double m[8]);
 
 // Device bytes of a prepared case (its arena), and the admission upper bound.
 uint64_t arenaBytes(const HostCase& hc);
@@ -107,6 +122,7 @@ struct Plan {
     LaunchShape shape;
     uint32_t feat_union = 0;
     uint32_t lmax = 0;  // largest tile node count over the plan's cases (tiled LDS stage)
+    bool hex = false;   // the plan's cases are solid-shell hexes (all or none)
     int parity = 0;
     int chunk = 0;
     hipGraphExec_t graph = nullptr;

@@ -169,6 +169,14 @@ SIGNATURES = {
     "shipfem_case_load": (S, [C.c_char_p, PP, C.POINTER(CaseDesc)]),
     "shipfem_server_submit_encoded": (S, [VP, VP, C.c_uint64, C.c_double, C.POINTER(C.c_uint64)]),
     "shipfem_solver_destroy": (None, [VP]),
+    "shipfem_hex_stiffness": (S, [D, C.c_double, C.c_double, D]),
+    "shipfem_hex_mass": (S, [D, C.c_double, D]),
+    "shipfem_hex_force": (S, [D, D, D, D]),
+    "shipfem_hex_critical_timestep": (S, [D] + [C.c_double] * 4 + [D]),
+    "shipfem_hex_solver_create": (S, [C.POINTER(CaseDesc), PP, C.c_char_p, C.c_size_t]),
+    "shipfem_hex_plate_mesh": (S, [C.c_double] * 3 + [C.c_int32] * 3 + [C.POINTER(C.c_uint32)] * 2
+                               + [D, U32]),
+    "shipfem_hex_pressure_load": (S, [D, C.c_uint32, U32, C.c_uint32, C.c_double, D]),
     "shipfem_case_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int32]),
     "shipfem_batch_create": (S, [C.POINTER(CaseDesc), C.c_uint32, C.POINTER(LaunchConfig), PP,
                                  C.c_char_p, C.c_size_t]),
@@ -305,12 +313,14 @@ class Mesh:
 class Case:
     """A case description: numpy arrays plus scalars, convertible to CaseDesc."""
 
+    corners = 4  # nodes per element: tets
+
     def __init__(self, X, index, *, flags=None, load=None, prescribed=None, u0=None, v0=None,
                  E=210e9, nu=0.3, rho=7850.0, plastic=False, yield_=355e6, hardening=1e9,
                  indenter=None, load_ramp=0.0, drive_ramp=0.0, gravity=0.0, damping=1.0,
                  dt=0.0, dt_safety=0.9, sample_every=100, max_steps=0, failure=None):
         self.X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
-        self.index = np.ascontiguousarray(index, np.uint32).reshape(-1, 4)
+        self.index = np.ascontiguousarray(index, np.uint32).reshape(-1, self.corners)
         c = lambda a, dt_: None if a is None else np.ascontiguousarray(a, dt_)  # noqa: E731
         self.flags = c(flags, np.uint32)
         self.load = c(load, np.float64)
@@ -440,6 +450,76 @@ class Case:
                     indenter=ind, failure=self.failure)
 
 
+class HexCase(Case):
+    """A solid-shell hex case: index [T, 8], nodes 0-3 on the element's -zeta face and 4-7
+    on its +zeta face (the thickness direction), each counter-clockwise seen from +zeta
+    (solid_shell.hpp:115-125). Elastic only."""
+
+    corners = 8
+
+    @classmethod
+    def plate(cls, lx, ly, thickness, nx, ny, nz=1, pressure=0.0, **kw):
+        """makePlateMesh (solid_shell.cpp:1152-1185), with uniformPressureLoad of
+        `pressure` on the top face when nonzero (solid_shell.cpp:1505-1561)."""
+        X, index = hex_plate_mesh(lx, ly, thickness, nx, ny, nz)
+        load = hex_pressure_load(X, index, pressure) if pressure else None
+        return cls(X, index, load=load, **kw)
+
+
+def hex_plate_mesh(lx, ly, thickness, nx, ny, nz=1):
+    n, e = C.c_uint32(), C.c_uint32()
+    _check(lib().shipfem_hex_plate_mesh(lx, ly, thickness, nx, ny, nz, C.byref(n), C.byref(e),
+                                        None, None), "hex_plate_mesh")
+    X = np.zeros((n.value, 3))
+    index = np.zeros((e.value, 8), np.uint32)
+    _check(lib().shipfem_hex_plate_mesh(lx, ly, thickness, nx, ny, nz, None, None,
+                                        _p(X, C.c_double), _p(index, C.c_uint32)))
+    return X, index
+
+
+def hex_pressure_load(X, index, pressure, load=None):
+    X = np.ascontiguousarray(X, np.float64)
+    index = np.ascontiguousarray(index, np.uint32)
+    out = np.zeros((len(X), 3)) if load is None else load
+    _check(lib().shipfem_hex_pressure_load(_p(X, C.c_double), len(X), _p(index, C.c_uint32),
+                                           len(index), pressure, _p(out, C.c_double)))
+    return out
+
+
+def _nodes24(nodes):
+    return np.ascontiguousarray(np.asarray(nodes, np.float64).reshape(24))
+
+
+def hex_stiffness(nodes, E=210e9, nu=0.3):
+    """Condensed elastic stiffness [24, 24] of one solid-shell hex (elementStiffness)."""
+    out = np.zeros((24, 24))
+    _check(lib().shipfem_hex_stiffness(_p(_nodes24(nodes), C.c_double), E, nu,
+                                       _p(out, C.c_double)), "hex_stiffness")
+    return out
+
+
+def hex_mass(nodes, rho=7850.0):
+    out = np.zeros(8)
+    _check(lib().shipfem_hex_mass(_p(_nodes24(nodes), C.c_double), rho, _p(out, C.c_double)),
+           "hex_mass")
+    return out
+
+
+def hex_force(K, rest, current):
+    out = np.zeros(24)
+    K = np.ascontiguousarray(K, np.float64)
+    _check(lib().shipfem_hex_force(_p(K, C.c_double), _p(_nodes24(rest), C.c_double),
+                                   _p(_nodes24(current), C.c_double), _p(out, C.c_double)))
+    return out
+
+
+def hex_critical_timestep(nodes, E=210e9, nu=0.3, rho=7850.0, safety=0.9):
+    dt = C.c_double()
+    _check(lib().shipfem_hex_critical_timestep(_p(_nodes24(nodes), C.c_double), E, nu, rho, safety,
+                                               C.byref(dt)), "hex_critical_timestep")
+    return dt.value
+
+
 def _records(arr, n):
     out = []
     for i in range(n):
@@ -523,15 +603,16 @@ class State:
 
 
 class Solver:
-    """One case on the device (replaces gpu::FemGpuSolver)."""
+    """One case on the device (replaces gpu::FemGpuSolver; a HexCase runs the elastic
+    solid-shell solve of zone::Solver)."""
 
     def __init__(self, case):
         self.case = case
         self._desc = case.desc()
         err = C.create_string_buffer(512)
         h = C.c_void_p()
-        _check(lib().shipfem_solver_create(C.byref(self._desc), C.byref(h), err, 512),
-               err.value.decode())
+        create = lib().shipfem_hex_solver_create if case.corners == 8 else lib().shipfem_solver_create
+        _check(create(C.byref(self._desc), C.byref(h), err, 512), err.value.decode())
         self.h = h
 
     def close(self):
@@ -588,7 +669,9 @@ class Solver:
         return _tears(lambda out, mx, n: lib().shipfem_solver_tears(self.h, out, mx, n))
 
     def tet_forces(self):
-        out = np.zeros((self.case.T, 12), np.float32)
+        """Per-element nodal forces of the last element pass: [T, 12] (tets) or [T, 24]
+        (hexes)."""
+        out = np.zeros((self.case.T, 3 * self.case.corners), np.float32)
         _check(lib().shipfem_solver_tet_forces(self.h, _p(out, C.c_float)))
         return out
 

@@ -9,6 +9,11 @@
   batch_cases()     64 plates with tet counts log-uniform in [1e4, 3e5], pressure 50-400 kPa,
                     yield 235-460 MPa
   served_case()     a stiffened panel scaled to a target tet count with indenter speed 2-10 m/s
+  hex_plate()       solid-shell plating 30 x 25 m x 12 mm, 1200 x 1000 x 1 hexes
+                    (1,200,000 elements), edges clamped, 50 kPa on the top face ramped over
+                    5 ms, elastic
+  hex_zone()        a zone patch of solid-shell plating, 1.2 x 0.6 m x 12 mm, edges clamped,
+                    a rigid sphere punched into its middle (the elastic zone::Solver solve)
 
 Every builder takes a `scale` (or explicit sizes) so the same geometry can be run at
 a size the FP64 oracle finishes in seconds.
@@ -17,7 +22,7 @@ import math
 
 import numpy as np
 
-from . import CONTACT, PIN_ALL, Case, Mesh
+from . import CONTACT, PIN_ALL, Case, HexCase, Mesh
 
 STEEL = dict(E=210e9, nu=0.3, rho=7850.0)
 TIP_NODE = (2 * 5 + 2) * 41 + 40  # node (i, j, k) = (40, 2, 2) of the 40x4x4 cantilever
@@ -114,3 +119,46 @@ def batch_cases(n=64, seed=7, lo=1e4, hi=3e5, **kw):
         cases.append(plate(lx=nx * h, ly=ny * h, nx=nx, ny=ny, pressure=p, ramp=2.5e-4,
                            jitter=0.0, plastic=True, yield_=sy, **kw))
     return cases
+
+
+def _clamp_edges(X, lx, ly, tol=1e-9):
+    flags = np.zeros(len(X), np.uint32)
+    edge = (X[:, 0] < tol) | (X[:, 0] > lx - tol) | (X[:, 1] < tol) | (X[:, 1] > ly - tol)
+    flags[edge] = PIN_ALL
+    return flags
+
+
+def hex_plate(lx=30.0, ly=25.0, thickness=0.012, nx=1200, ny=1000, nz=1, pressure=50e3,
+              ramp=5e-3, origin=(0.0, 0.0, 0.0), **kw):
+    """Solid-shell plate (makePlateMesh) with clamped edges under a ramped pressure.
+    `origin` shifts the whole plate (the rest coordinates the solve works from)."""
+    opts = dict(STEEL, load_ramp=ramp, dt_safety=0.9, sample_every=100)
+    opts.update(kw)
+    c = HexCase.plate(lx, ly, thickness, nx, ny, nz, pressure=pressure, **opts)
+    c.flags = _clamp_edges(c.X, lx, ly)
+    c.X = np.ascontiguousarray(c.X + np.asarray(origin, np.float64))
+    return c
+
+
+def hex_plate_scaled(scale):
+    """The hex plate's shape at 1/scale of its elements per in-plane axis."""
+    return hex_plate(nx=max(2, round(1200 / scale)), ny=max(2, round(1000 / scale)))
+
+
+def hex_zone(lx=1.2, ly=0.6, thickness=0.012, nx=48, ny=24, nz=1, radius=0.15, speed=2.0,
+             origin=(0.0, 0.0, 0.0), **kw):
+    """A zone patch: clamped edges, a sphere of `radius` moving at `speed` into the middle
+    of the bottom face, contact on every free node (zone.cpp's indenter, elastic)."""
+    c = HexCase.plate(lx, ly, thickness, nx, ny, nz)
+    c.flags = _clamp_edges(c.X, lx, ly)
+    c.flags[(c.flags & PIN_ALL) == 0] |= CONTACT
+    o = np.asarray(origin, np.float64)
+    c.X = np.ascontiguousarray(c.X + o)
+    centre = o + np.array([0.5 * lx, 0.5 * ly, -0.5 * thickness - radius])
+    c.indenter = dict(radius=radius, center=tuple(centre), velocity=(0.0, 0.0, speed), kappa=0.1)
+    c.E, c.nu, c.rho = STEEL["E"], STEEL["nu"], STEEL["rho"]
+    c.dt_safety, c.sample_every = 0.9, 100
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+

@@ -1,8 +1,9 @@
 """Write golden vectors from the compiled reference (oracle/_ref/libref.so).
 
 Run in a container that has the reference tree and the built oracle/_ref:
-    make -C oracle && python tests/golden/make_golden.py
-Output: tests/golden/reference_fem.npz (inputs and the reference's outputs only).
+    make -C oracle && python tests/golden/make_golden.py [fem|hex|all]
+Output: tests/golden/reference_fem.npz and tests/golden/reference_hex.npz (inputs and
+the reference's outputs only).
 
 What is recorded, and from which reference function:
   box_*       makeBoxTetMesh(0.4, 0.1, 0.02, 16, 4, 2)          engine/sim/fem.cpp:217-263
@@ -17,6 +18,24 @@ What is recorded, and from which reference function:
   dmg_*       plasticity::update with ductile failure along pull, rotated pull,
               squeeze, biaxial and mixed paths (damage, failed, failure normal)
                                                                 engine/sim/plasticity.cpp:457-477
+
+reference_hex.npz, from engine/sim/solid_shell.cpp (Formulation::SolidShell, steel
+E 206 GPa, nu 0.3, rho 7850):
+  el_nodes    eight elements: a 0.6 x 0.6 x 0.012 m plate element, the unit cube, a
+              0.3 x 0.2 x 0.01 m element with seeded corner noise, that element rotated
+              and moved to (3, -9.9, 8) m, a sheared and tapered element, a 2:1 brick
+              at nu 0.45, a thick skewed element and a 0.1 x 0.05 x 0.02 m web element
+  el_nu       Poisson ratio of each element (0.45 for the sixth, else 0.3)
+  el_K        elementStiffness [24 x 24]                   solid_shell.cpp:500-526
+  el_mass     elementMass                                   solid_shell.cpp:655-663
+  el_dt       criticalTimestep(safety 0.9)                  solid_shell.cpp:1088-1134
+  el_current  three current configurations per element: a small seeded deformation,
+              a 35 degree rotation plus that deformation, and 5% stretch plus shear
+  el_force    internalForce(el_K, rest, current)            solid_shell.cpp:735-766
+  el_rotation elementRotation(rest, current), column-major  solid_shell.cpp:680-733
+  plate_*     makePlateMesh(0.6, 0.3, 0.012, 3, 2, 1) and (0.4, 0.2, 0.02, 2, 2, 2) with
+              uniformPressureLoad(1e5)                      solid_shell.cpp:1152-1185,
+                                                            1505-1561
 """
 import ctypes as C
 import os
@@ -200,5 +219,88 @@ def main():
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
 
 
+def hex_elements(rng):
+    """The eight test elements (node order of solid_shell.hpp:115-125)."""
+    unit = np.stack([[-1, 1, 1, -1, -1, 1, 1, -1], [-1, -1, 1, 1, -1, -1, 1, 1],
+                     [-1, -1, -1, -1, 1, 1, 1, 1]], 1) * 0.5 + 0.5
+    plate = unit * np.array([0.6, 0.6, 0.012]) - np.array([0, 0, 0.006])
+    noisy = unit * np.array([0.3, 0.2, 0.01]) + rng.normal(0, 1, (8, 3)) * np.array([3e-3, 2e-3, 1e-4])
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    moved = noisy @ q.T + np.array([3.0, -9.9, 8.0])
+    sheared = unit * np.array([0.4, 0.3, 0.015])
+    sheared[:, 0] += 0.1 * sheared[:, 1]           # in-plane shear
+    sheared[:, 1] *= 1.0 - 0.3 * unit[:, 0]        # taper along x
+    sheared[:, 2] += 0.02 * unit[:, 0]             # warp
+    brick = unit * np.array([0.2, 0.1, 0.05])
+    thick = unit * np.array([0.1, 0.1, 0.08])
+    thick[:, 0] += 0.3 * thick[:, 2]
+    web = unit * np.array([0.1, 0.05, 0.02])
+    return [plate, unit, noisy, moved, sheared, brick, thick, web], [0.3] * 5 + [0.45] + [0.3] * 2
+
+
+def main_hex():
+    R = O.ref_lib()
+    if R is None:
+        raise SystemExit("oracle/_ref/libref.so not built (make -C oracle)")
+    E, rho = 206e9, 7850.0
+    rng = np.random.default_rng(20261203)
+    els, nus = hex_elements(rng)
+    out = {"el_nodes": np.array(els), "el_nu": np.array(nus)}
+    Ks, ms, dts, curs, fs, rots = [], [], [], [], [], []
+    for x, nu in zip(els, nus):
+        x = np.ascontiguousarray(x)
+        K = np.zeros(576)
+        R.ref_hex_stiffness(p(x, C.c_double), E, nu, 2, p(K, C.c_double))
+        m = np.zeros(8)
+        R.ref_hex_mass(p(x, C.c_double), rho, p(m, C.c_double))
+        dts.append(R.ref_hex_critical_timestep(p(x, C.c_double), E, nu, rho, 0.9))
+        size = np.ptp(x, axis=0).max()
+        a = np.deg2rad(35.0)
+        ax = np.array([1.0, 2.0, 0.5]) / np.linalg.norm([1.0, 2.0, 0.5])
+        W = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+        Rot = np.eye(3) + np.sin(a) * W + (1 - np.cos(a)) * W @ W
+        small = rng.normal(0, 1e-3 * size, (8, 3))
+        c0 = x.mean(0)
+        stretch = np.array([[1.05, 0.02, 0.0], [0.0, 0.98, 0.01], [0.0, 0.0, 1.01]])
+        cur3 = [x + small, (x - c0) @ Rot.T + c0 + small, (x - c0) @ stretch.T + c0]
+        fe, re = [], []
+        for cur in cur3:
+            cur = np.ascontiguousarray(cur)
+            f = np.zeros(24)
+            R.ref_hex_force(p(K, C.c_double), p(x, C.c_double), p(cur, C.c_double), p(f, C.c_double))
+            r9 = np.zeros(9)
+            R.ref_hex_rotation(p(x, C.c_double), p(cur, C.c_double), p(r9, C.c_double))
+            fe.append(f)
+            re.append(r9)
+        Ks.append(K.reshape(24, 24))
+        ms.append(m)
+        curs.append(np.array(cur3))
+        fs.append(np.array(fe))
+        rots.append(np.array(re))
+    out.update(el_K=np.array(Ks), el_mass=np.array(ms), el_dt=np.array(dts),
+               el_current=np.array(curs), el_force=np.array(fs), el_rotation=np.array(rots),
+               el_material=np.array([E, rho]))
+    for name, (lx, ly, t, nx, ny, nz) in (("plate_a", (0.6, 0.3, 0.012, 3, 2, 1)),
+                                          ("plate_b", (0.4, 0.2, 0.02, 2, 2, 2))):
+        n, e = C.c_int64(), C.c_int64()
+        R.ref_hex_plate_counts(lx, ly, t, nx, ny, nz, C.byref(n), C.byref(e))
+        X = np.zeros((n.value, 3))
+        idx = np.zeros((e.value, 8), np.uint32)
+        load = np.zeros((n.value, 3))
+        R.ref_hex_plate(lx, ly, t, nx, ny, nz, 1e5, p(X, C.c_double), p(idx, C.c_uint32),
+                        p(load, C.c_double))
+        out[f"{name}_spec"] = np.array([lx, ly, t, nx, ny, nz])
+        out[f"{name}_position"], out[f"{name}_index"], out[f"{name}_load"] = X, idx, load
+    path = os.path.join(HERE, "reference_hex.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
+
+
 if __name__ == "__main__":
-    main()
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("fem", "all"):
+        main()
+    if what in ("hex", "all"):
+        main_hex()
