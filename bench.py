@@ -1,7 +1,7 @@
 """Benchmark of the explicit tet step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--csv FILE]
-                    [--workload plate|cantilever|panel|batch|serve|sweep]
+                    [--workload plate|cantilever|panel|batch|serve|sweep|hex]
 
 Default workload: the large plain-path plate (10 x 4 x 0.02 m, 350x144x4 Kuhn cells,
 1,209,600 tets, nodes jittered +-5%, edges clamped, 200 kPa ramped over 5 ms,
@@ -28,6 +28,11 @@ HBM_PEAK_GBS = 8000.0
 # the element pass 12 (element); CSR offset 4 + X 12 + 1/m 4 + load 12 + state read
 # 48 + state write 48 (node). Sum: 168 T + 140 N (+ 56 T plastic) per step.
 ELEM_B_T, ELEM_B_N, NODE_B_T, NODE_B_N, PLASTIC_B_T = 104, 12, 64, 128, 56
+# Solid-shell hex pass, per element: connectivity 32 + the packed 21x21 stiffness block
+# 231 x 8 + centre gradients and rest offsets 42 x 4 + force-slot write 96; per node the
+# displacement and its compensation 28. Node pass per element: 8 slot reads of 12 + 8
+# gather entries of 4; per node as for tets.
+HEX_ELEM_B_T, HEX_ELEM_B_N, HEX_NODE_B_T = 32 + 231 * 8 + 42 * 4 + 96, 28, 8 * 16
 REFERENCE_SUSTAINED = 470e6  # tet-steps/s, GTX 1070 Ti, 491,520 tets (docs/07-fem-spike-findings.md:89)
 
 
@@ -260,6 +265,93 @@ def bench_sweep(args, sf, info, rows):
                           device=info[0], table=table, csv=args.csv)))
 
 
+def hex_cpu_baseline(budget_s=8.0):
+    """The elastic zone step's element work (solidshell::internalForce with a 24x24 FP64
+    stiffness per element, zone.cpp:928-1012) and node update, as the oracle's C hex
+    solve: a bounded sample, a 100 x 100-element patch of the same plating."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from shipfem import configs
+
+    c = configs.hex_plate(lx=2.5, ly=2.5, nx=100, ny=100, pressure=50e3)
+    o = O.Case(c.X, c.index, dt=1e-7, **c.oracle_kwargs())
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = {}
+    for label, th in (("all", threads), ("one", 1)):
+        o.run(1, threads=th)
+        t0 = time.perf_counter()
+        n = 0
+        while n < 400 and time.perf_counter() - t0 < budget_s / 2:
+            o.run(5, threads=th)
+            n += 5
+        out[label] = (c.T * n / (time.perf_counter() - t0), n)
+    del np
+    return dict(value=out["all"][0], unit="hex-steps/s", cores=threads, kind="port",
+                sample=(f"oracle hex solve in C (FP64: polar rotation to convergence, full 24x24 "
+                        f"stiffness per element as the reference's CPU zone step) on a {c.T}-element "
+                        f"patch of the same plating: {out['all'][1]} steps on {threads} threads; "
+                        f"one core {out['one'][0]:.3e} hex-steps/s over {out['one'][1]} steps"))
+
+
+def bench_hex(args, sf, info, rows):
+    """The elastic solid-shell solve on the 1.2M-element hex plate."""
+    from shipfem import configs
+
+    case = configs.hex_plate()
+    T, N = case.T, case.N
+    t0 = time.perf_counter()
+    solver = sf.Solver(case)
+    prep_s = time.perf_counter() - t0
+    ts = solver.timestep()
+    solver.run(args.warmup)
+    t0 = time.perf_counter()
+    dev_ms = solver.run(args.steps)
+    wall = time.perf_counter() - t0
+    st = solver.readback()
+    if st.status:
+        raise SystemExit("state went non-finite")
+    value = T * args.steps / wall
+    elem_ms, node_ms = solver.profile(args.profile_steps)
+    elem_bytes = HEX_ELEM_B_T * T + HEX_ELEM_B_N * N
+    node_bytes = HEX_NODE_B_T * T + NODE_B_N * N
+    if elem_ms >= node_ms:
+        dom, dom_ms, dom_bytes = "element pass (k_elem_hex)", elem_ms, elem_bytes
+    else:
+        dom, dom_ms, dom_bytes = "node pass (k_node)", node_ms, node_bytes
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles("hex")
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=(traffic or {}).get(dom.split()[0]), kernel=dom,
+                    kernel_ms=round(dom_ms, 5), algorithmic_bytes=dom_bytes,
+                    elem_ms=round(elem_ms, 5), node_ms=round(node_ms, 5))
+    step_bytes = elem_bytes + node_bytes
+    step_gbs = step_bytes * args.steps / wall / 1e9
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = hex_cpu_baseline()
+        except Exception as e:
+            cpu = dict(value=None, unit="hex-steps/s", cores=0, kind="port", sample=f"failed: {e}")
+    if rows:
+        rows.add(args.steps, workload="hex", elements=T, nodes=N, warmup=args.warmup,
+                 wall_ms_per_step=round(wall * 1e3 / args.steps, 6),
+                 device_ms_per_step=round(dev_ms / args.steps, 6), hex_steps_per_s=round(value),
+                 elem_ms=round(elem_ms, 6), node_ms=round(node_ms, 6))
+    emit(dict(metric="hex-element-steps/s", value=value, unit="hex-steps/s", n_gpus=1,
+              steps=args.steps, warmup=args.warmup, ms_per_step=wall * 1e3 / args.steps,
+              higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32+fp64",
+              data="synthetic",
+              config=dict(workload=("solid-shell hex plate 30x25 m x 12 mm, 1200x1000x1 elements, "
+                                    "edges clamped, 50 kPa ramped over 5 ms, elastic co-rotational "
+                                    "(FP64 stiffness block, FP32 + Kahan state)"),
+                          elements=T, nodes=N, dt=ts["dt"], cfl=ts["cfl"], device=info[0],
+                          host_prep_s=round(prep_s, 2), device_ms_per_step=dev_ms / args.steps,
+                          step_hbm_gbs=round(step_gbs, 1), algorithmic_bytes_per_step=step_bytes),
+              roofline=roofline, cpu_baseline=cpu))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +395,8 @@ def main():
         return bench_batch(args, sf, info)
     if args.workload == "serve":
         return bench_serve(args, sf, info)
+    if args.workload == "hex":
+        return bench_hex(args, sf, info, rows)
     case, desc = workload(args.workload)
     T, N = case.T, case.N
     variant = sf.VARIANT_TILED if args.variant == "tiled" else sf.VARIANT_SLOTS
