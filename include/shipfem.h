@@ -426,6 +426,11 @@ typedef struct shipfem_batch shipfem_batch;
 shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t case_count,
                                     const shipfem_launch_config* cfg, shipfem_batch** out,
                                     char* err, size_t err_len);
+/* shipfem_batch_create for solid-shell hex cases (each read as shipfem_hex_solver_create
+ * reads its case); every batch call then applies. Elastic only (SHIPFEM_UNSUPPORTED). */
+shipfem_status shipfem_hex_batch_create(const shipfem_case_desc* cases, uint32_t case_count,
+                                        const shipfem_launch_config* cfg, shipfem_batch** out,
+                                        char* err, size_t err_len);
 /* All cases advance `steps` (cases that reach max_steps stop). */
 shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device_ms);
 /* Times candidate launch shapes on the batch itself (state is snapshotted and

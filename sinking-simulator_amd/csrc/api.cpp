@@ -372,9 +372,11 @@ void shipfem_solver_destroy(shipfem_solver* s) {
 
 // ---------------------------------------------------------------- batch
 
-shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
-                                    const shipfem_launch_config* cfg, shipfem_batch** out,
-                                    char* err, size_t err_len) {
+namespace {
+
+shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
+                           const shipfem_launch_config* cfg, bool hex, shipfem_batch** out,
+                           char* err, size_t err_len) {
     if (!cases || !out || n == 0) {
         setError(err, err_len, "null argument or empty batch");
         return SHIPFEM_INVALID_ARG;
@@ -390,7 +392,7 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
     b->hcs.resize(n);
     b->dcs.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
-        const std::string msg = prepareCase(cases[i], b->hcs[i]);
+        const std::string msg = hex ? prepareHexCase(cases[i], b->hcs[i]) : prepareCase(cases[i], b->hcs[i]);
         if (!msg.empty()) {
             setError(err, err_len, "case " + std::to_string(i) + ": " + msg);
             delete b;
@@ -426,13 +428,13 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
         e = b->plan.build(b->stream);
     }
     std::vector<double> lmax;
-    if (e == hipSuccess) e = b->plan.omegaMax(lmax, b->stream);
+    if (e == hipSuccess && !hex) e = b->plan.omegaMax(lmax, b->stream);
     if (e == hipSuccess) {
         for (uint32_t i = 0; i < n; ++i) {
             HostCase& hc = b->hcs[i];
             DeviceCase& dc = b->dcs[i];
-            dc.omega = std::sqrt(4.0 * lmax[i] / hc.rho);
-            dc.cfl = omegaToCfl(lmax[i], hc.rho);
+            dc.omega = hex ? hc.hex_omega : std::sqrt(4.0 * lmax[i] / hc.rho);
+            dc.cfl = hex ? 2.0 / hc.hex_omega : omegaToCfl(lmax[i], hc.rho);
             setTimestep(dc, hc, hc.dt_request > 0 ? hc.dt_request : hc.dt_safety * dc.cfl);
             b->plan.recs[i] = dc.rec;
         }
@@ -455,9 +457,31 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
         std::vector<uint32_t>().swap(hc.lconn);
         std::vector<uint32_t>().swap(hc.tnode);
         std::vector<uint32_t>().swap(hc.pell);
+        std::vector<double>().swap(hc.hk);
+        std::vector<float>().swap(hc.hgeo);
     }
     *out = b;
     return SHIPFEM_OK;
+}
+
+}  // namespace
+
+shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
+                                    const shipfem_launch_config* cfg, shipfem_batch** out,
+                                    char* err, size_t err_len) {
+    return createBatch(cases, n, cfg, false, out, err, err_len);
+}
+
+shipfem_status shipfem_hex_batch_create(const shipfem_case_desc* cases, uint32_t n,
+                                        const shipfem_launch_config* cfg, shipfem_batch** out,
+                                        char* err, size_t err_len) {
+    for (uint32_t i = 0; cases && i < n; ++i)
+        if (cases[i].material.plastic) {
+            setError(err, err_len, "case " + std::to_string(i) + ": the hex solve is elastic");
+            if (out) *out = nullptr;
+            return SHIPFEM_UNSUPPORTED;
+        }
+    return createBatch(cases, n, cfg, true, out, err, err_len);
 }
 
 shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device_ms) {
@@ -518,8 +542,9 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
     e = timeShape(cur, curMs);
     const int blocks[] = {64, 128, 256, 512};
     for (int pass = 0; pass < 3 && e == hipSuccess; ++pass) {
-        // the tiled variant's element block is its 256-tet tile: not a free parameter
-        if (pass == 1 && cur.variant == kTiled) continue;
+        // the tiled variant's element block is its 256-tet tile: not a free parameter (the
+        // hex pass always takes one lane per element, so for hexes it is)
+        if (pass == 1 && cur.variant == kTiled && !b->plan.hex) continue;
         std::vector<LaunchShape> cands;
         for (int v : (pass == 0 ? std::vector<int>{0, 1} : std::vector<int>(blocks, blocks + 4))) {
             LaunchShape c = cur;

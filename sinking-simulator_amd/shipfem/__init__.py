@@ -180,6 +180,8 @@ SIGNATURES = {
     "shipfem_case_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int32]),
     "shipfem_batch_create": (S, [C.POINTER(CaseDesc), C.c_uint32, C.POINTER(LaunchConfig), PP,
                                  C.c_char_p, C.c_size_t]),
+    "shipfem_hex_batch_create": (S, [C.POINTER(CaseDesc), C.c_uint32, C.POINTER(LaunchConfig), PP,
+                                     C.c_char_p, C.c_size_t]),
     "shipfem_batch_run": (S, [VP, C.c_int32, D]),
     "shipfem_batch_tune": (S, [VP, C.c_int32, C.POINTER(LaunchConfig), D]),
     "shipfem_batch_set_config": (S, [VP, C.POINTER(LaunchConfig)]),
@@ -749,8 +751,12 @@ class Batch:
         cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk, variant)
         err = C.create_string_buffer(512)
         h = C.c_void_p()
-        _check(lib().shipfem_batch_create(self._descs, len(self.cases), C.byref(cfg), C.byref(h),
-                                          err, 512), err.value.decode())
+        kinds = {c.corners for c in self.cases}
+        if len(kinds) != 1:
+            raise ValueError("a batch holds one element kind (all tets or all hexes)")
+        create = lib().shipfem_hex_batch_create if kinds == {8} else lib().shipfem_batch_create
+        _check(create(self._descs, len(self.cases), C.byref(cfg), C.byref(h), err, 512),
+               err.value.decode())
         self.h = h
 
     def close(self):

@@ -188,3 +188,27 @@ def test_hex_solve_is_deterministic_and_graph_replay_matches_plain_launches():
     d = sf.Solver(case).set_launch(sf.VARIANT_SLOTS, node_block=64, elem_block=128)
     d.run(600)
     assert np.array_equal(d.readback().u, ua.u)
+
+
+def test_hex_batch_matches_each_case_alone_and_tunes():
+    # one launch per phase over several hex cases: every case ends with the bits it has
+    # when solved alone, whatever else shares the launch and whatever shape the tuner picks
+    rng = np.random.default_rng(3)
+    cases = [configs.hex_zone(nx=int(n), ny=int(n) // 2, speed=float(v))
+             for n, v in zip(rng.integers(8, 40, 6), rng.uniform(2.0, 8.0, 6))]
+    cases.append(configs.hex_plate(lx=0.6, ly=0.3, nx=20, ny=10, pressure=1e5, ramp=1e-3))
+    b = sf.Batch(cases)
+    b.run(400)
+    alone = []
+    for c in cases:
+        s = sf.Solver(c)
+        s.run(400)
+        alone.append(s.readback().u)
+    for i, u in enumerate(alone):
+        assert np.array_equal(b.readback(i).u, u), i
+    best = b.tune(steps_per_trial=100)
+    say(f"hex batch of {len(cases)}: tuned {best}")
+    b.reset()
+    b.run(400)
+    for i, u in enumerate(alone):
+        assert np.array_equal(b.readback(i).u, u), i
