@@ -627,14 +627,39 @@ static void max_principal_direction(const double s6[6], double out[3]) {
 
 /* The return with damage; at failure `stress` keeps the returned (pre-failure)
  * stress and bit 1 is set: the caller zeroes it. */
+static int j2_return_t(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
+                       double* damage, double stress[6], double* dissipation, double normal[3],
+                       double* eta, double tangent[36]);
+
 static int j2_return(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
                      double* damage, double stress[6], double* dissipation, double normal[3],
                      double* eta) {
+    return j2_return_t(m, strain, eps_p, eqps, damage, stress, dissipation, normal, eta, NULL);
+}
+
+/* The elastic moduli in Voigt form with engineering shears (plasticity.cpp elasticModuli). */
+static void elastic_moduli(double mu, double kappa, double c[36]) {
+    for (int k = 0; k < 36; ++k) c[k] = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) c[i * 6 + j] = kappa + 2.0 * mu * ((i == j) - 1.0 / 3.0);
+        c[(3 + i) * 6 + 3 + i] = mu;
+    }
+}
+
+/* With `tangent`, also the algorithmic tangent of the return (plasticity.cpp:481-505):
+ * kappa 1(x)1 + 2 mu theta I_dev - 2 mu thetaBar n(x)n, theta = 1 - 2 mu dgamma/|s_trial|,
+ * thetaBar = 1/(1 + H/(3 mu)) - (1 - theta); the elastic moduli when the point does not
+ * yield, zero for a failed point. */
+static int j2_return_t(const ora_j2* m, const double strain[6], double eps_p[6], double* eqps,
+                       double* damage, double stress[6], double* dissipation, double normal[3],
+                       double* eta, double tangent[36]) {
     if (eta) *eta = 0.0;
     if (dissipation) *dissipation = 0.0;
     /* a failed point carries no stress and its history is frozen (plasticity.cpp:326) */
     if (damage && *damage >= 1.0) {
         for (int i = 0; i < 6; ++i) stress[i] = 0.0;
+        if (tangent)
+            for (int k = 0; k < 36; ++k) tangent[k] = 0.0;
         return 0;
     }
     const double mu = m->E / (2.0 * (1.0 + m->nu));
@@ -658,6 +683,7 @@ static int j2_return(const ora_j2* m, const double strain[6], double eps_p[6], d
             stress[i] = p + tr[i];
             stress[3 + i] = tr[3 + i];
         }
+        if (tangent) elastic_moduli(mu, kappa, tangent);
         return 0;
     }
     /* linear hardening makes the consistency condition affine in dgamma: the
@@ -681,14 +707,24 @@ static int j2_return(const ora_j2* m, const double strain[6], double eps_p[6], d
     const double q = sqrt(1.5) * (norm - 2.0 * mu * gamma);
     const double e = q > 0.0 ? p / q : 0.0;
     if (eta) *eta = e;
-    if (!damage || !(m->failure_strain > 0.0)) return 1;
-    const double critical = m->failure_strain * ora_triaxiality_factor(
-        m->sensitivity, m->reference_triaxiality, m->cutoff_triaxiality, e);
-    if (critical > 0.0 && isfinite(critical)) *damage += deq / critical;
-    if (*damage >= 1.0) {
-        *damage = 1.0;
-        if (normal) max_principal_direction(stress, normal);
-        return 3;
+    if (damage && m->failure_strain > 0.0) {
+        const double critical = m->failure_strain * ora_triaxiality_factor(
+            m->sensitivity, m->reference_triaxiality, m->cutoff_triaxiality, e);
+        if (critical > 0.0 && isfinite(critical)) *damage += deq / critical;
+        if (*damage >= 1.0) {
+            *damage = 1.0;
+            if (normal) max_principal_direction(stress, normal);
+            if (tangent)
+                for (int k = 0; k < 36; ++k) tangent[k] = 0.0;
+            return 3;
+        }
+    }
+    if (tangent) {
+        const double theta = 1.0 - 2.0 * mu * gamma / norm;
+        const double thetabar = 1.0 / (1.0 + m->hardening / (3.0 * mu)) - (1.0 - theta);
+        elastic_moduli(mu * theta, kappa, tangent);
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) tangent[i * 6 + j] -= 2.0 * mu * thetabar * n[i] * n[j];
     }
     return 1;
 }
@@ -848,19 +884,281 @@ void ora_hex_force(const double K[576], const double X[24], const double x[24], 
     hex_force(K, X, x, out, NULL);
 }
 
+/* Polar factor of the centre deformation gradient of a hex, to convergence (column-major),
+ * as hex_force forms it. */
+static void hex_rotation(const double X[24], const double x[24], double r[9]) {
+    static const double sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    static const double sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    static const double sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    double Jx[9], JX[9], JXi[9], inv[9], it[9];
+    for (int i = 0; i < 3; ++i) {
+        double gx[3] = {0, 0, 0}, gX[3] = {0, 0, 0};
+        for (int a = 0; a < 8; ++a) {
+            const double dn[3] = {0.125 * sx[a], 0.125 * sy[a], 0.125 * sz[a]};
+            for (int k = 0; k < 3; ++k) {
+                gx[k] += dn[k] * x[3 * a + i];
+                gX[k] += dn[k] * X[3 * a + i];
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            AT(Jx, i, k) = gx[k];
+            AT(JX, i, k) = gX[k];
+        }
+    }
+    inv3(JX, JXi);
+    mul3(Jx, JXi, r);
+    for (int step = 0; step < 64; ++step) {
+        inv3(r, inv);
+        transpose3(inv, it);
+        double change = 0;
+        for (int k = 0; k < 9; ++k) {
+            const double nx = 0.5 * (r[k] + it[k]);
+            change = fmax(change, fabs(nx - r[k]));
+            r[k] = nx;
+        }
+        if (change < 1e-16) break;
+    }
+}
+
+/* In-place Cholesky solve of the n x n system a (row-major, stride 7) for b; 0 when a
+ * pivot is not positive. */
+static int chol_solve7(const double* a, int n, double* b) {
+    double l[49];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double t = a[i * 7 + j];
+            for (int k = 0; k < j; ++k) t -= l[i * 7 + k] * l[j * 7 + k];
+            if (i == j) {
+                if (!(t > 0.0)) return 0;
+                l[i * 7 + i] = sqrt(t);
+            } else {
+                l[i * 7 + j] = t / l[j * 7 + j];
+            }
+        }
+    for (int i = 0; i < n; ++i) {
+        double t = b[i];
+        for (int k = 0; k < i; ++k) t -= l[i * 7 + k] * b[k];
+        b[i] = t / l[i * 7 + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double t = b[i];
+        for (int k = i + 1; k < n; ++k) t -= l[k * 7 + i] * b[k];
+        b[i] = t / l[i * 7 + i];
+    }
+    return 1;
+}
+
+
+/* Stored energy density 1/2 s : C^-1 : s of a Voigt stress (tensor shears), as
+ * zone.cpp:1022-1036 forms it. */
+static double stored_density(const double s[6], double mu, double kappa) {
+    const double p = (s[0] + s[1] + s[2]) / 3.0;
+    double d = 0;
+    for (int i = 0; i < 3; ++i) d += 0.5 * s[i] * ((s[i] - p) / (2.0 * mu) + p / (3.0 * kappa));
+    for (int i = 3; i < 6; ++i) d += 0.5 * s[i] * (s[i] / mu);
+    return d;
+}
+
+static int hex_plastic(const double* B, const double* G, const double* w, const double X[24],
+                       const double x[24], const ora_j2* m, double* point, double enhanced[7],
+                       double force[24], double stress[48], double info[6], double* released);
+
+int ora_hex_plastic_update(const double* B, const double* G, const double* w,
+                           const double X[24], const double x[24], const ora_j2* m,
+                           double* point, double enhanced[7], double force[24],
+                           double stress[48], double info[6]) {
+    return hex_plastic(B, G, w, X, x, m, point, enhanced, force, stress, info, NULL);
+}
+
+/* With `released`: the stored energy w 1/2 s : C^-1 : s of every point that fails in this
+ * update, s its returned stress at failure (that energy leaves the element with it). */
+static int hex_plastic(const double* B, const double* G, const double* w, const double X[24],
+                       const double x[24], const ora_j2* m, double* point, double enhanced[7],
+                       double force[24], double stress[48], double info[6], double* released) {
+    const double mu = m->E / (2.0 * (1.0 + m->nu));
+    const double kappa = m->E / (3.0 * (1.0 - 2.0 * m->nu));
+    for (int k = 0; k < 24; ++k) force[k] = 0.0;
+    for (int k = 0; k < 6; ++k) info[k] = 0.0;
+    double r[9], d[24];
+    hex_rotation(X, x, r);
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) {
+            double t = 0;
+            for (int k = 0; k < 3; ++k) t += AT(r, k, i) * x[3 * a + k];
+            d[3 * a + i] = t - X[3 * a + i];
+        }
+    /* an element with a failed point has dropped its enhanced modes (solid_shell.cpp:860-866) */
+    int eas = 7;
+    for (int g = 0; g < 8; ++g)
+        if (point[9 * g + 7] >= 1.0) eas = 0;
+    double volume = 0;
+    for (int g = 0; g < 8; ++g) volume += w[g];
+    const double scale = m->yield * volume;
+    /* the base strain B d of each point, and the history at the start of the step */
+    double eu[8][6], start[8][8], trial[8][8], sg[8][6], cg[8][36], diss[8], rel[8];
+    int flags[8];
+    for (int g = 0; g < 8; ++g) {
+        for (int i = 0; i < 6; ++i) {
+            double t = 0;
+            for (int j = 0; j < 24; ++j) t += B[(g * 6 + i) * 24 + j] * d[j];
+            eu[g][i] = t;
+        }
+        for (int k = 0; k < 8; ++k) start[g][k] = point[9 * g + k];
+    }
+    double alpha[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < eas; ++k) alpha[k] = enhanced[k];
+    int converged = 0, iterations = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        converged = 0;
+        for (int iter = 0;; ++iter) {
+            for (int g = 0; g < 8; ++g) {
+                double e[6];
+                for (int i = 0; i < 6; ++i) {
+                    double t = eu[g][i];
+                    for (int k = 0; k < eas; ++k) t += G[(g * 6 + i) * 7 + k] * alpha[k];
+                    e[i] = t;
+                }
+                for (int k = 0; k < 8; ++k) trial[g][k] = start[g][k];
+                flags[g] = j2_return_t(m, e, trial[g], &trial[g][6], &trial[g][7], sg[g], &diss[g],
+                                       NULL, NULL, cg[g]);
+                rel[g] = 0.0;
+                if (flags[g] & 2) {
+                    rel[g] = w[g] * stored_density(sg[g], mu, kappa);
+                    for (int i = 0; i < 6; ++i) sg[g][i] = 0.0;
+                }
+            }
+            iterations = iter + 1;
+            if (eas == 0) {
+                converged = 1;
+                break;
+            }
+            double res[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int g = 0; g < 8; ++g)
+                for (int k = 0; k < eas; ++k) {
+                    double t = 0;
+                    for (int i = 0; i < 6; ++i) t += G[(g * 6 + i) * 7 + k] * sg[g][i];
+                    res[k] += w[g] * t;
+                }
+            double nrm = 0;
+            for (int k = 0; k < eas; ++k) nrm += res[k] * res[k];
+            if (sqrt(nrm) <= 1e-12 * scale) {
+                converged = 1;
+                break;
+            }
+            if (iter + 1 >= 40) break;
+            double kaa[49];
+            for (int k = 0; k < 49; ++k) kaa[k] = 0.0;
+            for (int g = 0; g < 8; ++g) {
+                double cgm[6][7];
+                for (int i = 0; i < 6; ++i)
+                    for (int k = 0; k < eas; ++k) {
+                        double t = 0;
+                        for (int j = 0; j < 6; ++j) t += cg[g][i * 6 + j] * G[(g * 6 + j) * 7 + k];
+                        cgm[i][k] = t;
+                    }
+                for (int p = 0; p < eas; ++p)
+                    for (int q = 0; q < eas; ++q) {
+                        double t = 0;
+                        for (int i = 0; i < 6; ++i) t += G[(g * 6 + i) * 7 + p] * cgm[i][q];
+                        kaa[p * 7 + q] += w[g] * t;
+                    }
+            }
+            double delta[7];
+            for (int k = 0; k < eas; ++k) delta[k] = -res[k];
+            if (!chol_solve7(kaa, eas, delta)) break;
+            double work = 0;
+            for (int k = 0; k < eas; ++k) work += delta[k] * res[k];
+            if (fabs(work) <= 1e-16 * scale) {
+                converged = 1;
+                break;
+            }
+            for (int k = 0; k < eas; ++k) alpha[k] += delta[k];
+        }
+        int died = 0;
+        for (int g = 0; g < 8; ++g) died |= trial[g][7] >= 1.0;
+        if (!(died && eas > 0)) break;
+        /* a point died inside this step: redo the element without the enhanced modes */
+        eas = 0;
+        for (int k = 0; k < 7; ++k) alpha[k] = 0.0;
+    }
+    int yielded = 0, failed = 0;
+    double dissipation = 0;
+    for (int g = 0; g < 8; ++g) {
+        for (int k = 0; k < 8; ++k) point[9 * g + k] = trial[g][k];
+        point[9 * g + 8] = trial[g][7] >= 1.0 ? 1.0 : 0.0;
+        dissipation += w[g] * diss[g];
+        yielded += (flags[g] & 1) != 0;
+        failed += trial[g][7] >= 1.0;
+        if (stress)
+            for (int i = 0; i < 6; ++i) stress[6 * g + i] = sg[g][i];
+    }
+    for (int k = 0; k < 7; ++k) enhanced[k] = alpha[k];
+    double internal[24];
+    for (int j = 0; j < 24; ++j) {
+        double t = 0;
+        for (int g = 0; g < 8; ++g) {
+            double u = 0;
+            for (int i = 0; i < 6; ++i) u += B[(g * 6 + i) * 24 + j] * sg[g][i];
+            t += w[g] * u;
+        }
+        internal[j] = t;
+    }
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) {
+            double t = 0;
+            for (int k = 0; k < 3; ++k) t += AT(r, i, k) * internal[3 * a + k];
+            force[3 * a + i] = -t;
+        }
+    info[0] = dissipation;
+    info[1] = iterations;
+    info[2] = converged;
+    info[3] = yielded;
+    info[4] = failed;
+    info[5] = failed == 8;
+    if (released) {
+        *released = 0.0;
+        for (int g = 0; g < 8; ++g) *released += rel[g];
+    }
+    return failed == 8;
+}
+
 /* element pass at u_n over elements [e0, e1) */
 static void run_elements(const ora_case* c, ora_state* st, int64_t e0, int64_t e1, int64_t n,
                          int sample, ora_sums* out) {
     if (hex_case(c)) {
+        const ora_j2 j2 = {c->E, c->nu, c->yield, c->hardening, c->failure_strain, c->sensitivity,
+                           c->reference_triaxiality, c->cutoff_triaxiality};
+        const double mu = c->E / (2 * (1 + c->nu)), kappa = c->E / (3 * (1 - 2 * c->nu));
         for (int64_t e = e0; e < e1; ++e) {
             const uint32_t* idx = &c->index[e * 8];
+            double* f = &st->force[e * 24];
             double X[24], x[24];
             for (int q = 0; q < 8; ++q)
                 for (int a = 0; a < 3; ++a) {
                     X[q * 3 + a] = c->X[idx[q] * 3 + a];
                     x[q * 3 + a] = X[q * 3 + a] + st->u[idx[q] * 3 + a];
                 }
-            hex_force(&c->hex_k[e * 576], X, x, &st->force[e * 24], sample ? &out->strain : NULL);
+            if (!c->plastic) {
+                hex_force(&c->hex_k[e * 576], X, x, f, sample ? &out->strain : NULL);
+                continue;
+            }
+            /* zone.cpp:935-962: a torn element carries nothing and is not updated */
+            double* point = &st->hex_point[e * 72];
+            int torn = 1;
+            for (int g = 0; g < 8; ++g) torn &= point[9 * g + 7] >= 1.0;
+            if (torn) {
+                for (int k = 0; k < 24; ++k) f[k] = 0.0;
+                continue;
+            }
+            double stress[48], info[6], rel = 0.0;
+            const double* w = &c->hex_w[e * 8];
+            const int now = hex_plastic(&c->hex_B[e * 1152], &c->hex_G[e * 336], w, X, x, &j2, point,
+                                        &st->hex_enhanced[e * 7], f, stress, info, &rel);
+            out->diss += info[0];
+            out->fracture += rel;
+            if (now && st->tear_step) st->tear_step[e] = n;
+            if (sample)
+                for (int g = 0; g < 8; ++g) out->strain += w[g] * stored_density(&stress[6 * g], mu, kappa);
         }
         return;
     }

@@ -115,6 +115,18 @@ typedef struct ora_j2 {
     double cutoff_triaxiality;      /* at or below: no damage */
 } ora_j2;
 
+/* elementPlasticUpdate (solid_shell.cpp:826-1029) on precomputed forms: B [8][6][24] and
+ * G [8][6][7] of each Gauss point, w [8] its volume. point [8][9] per Gauss point: plastic
+ * strain 6 (engineering shears), eqps, damage, failed; enhanced [7] the warm start in, the
+ * solution out. m: E, nu, yield, hardening and the failure strain (<= 0 never fails) with
+ * the Rice-Tracey constants. force [24] on the nodes, stress [48] (8 x 6, may be NULL),
+ * info [6] = dissipation, iterations, converged, yielded, failed points, torn. Returns
+ * torn. */
+int ora_hex_plastic_update(const double* B, const double* G, const double* w,
+                           const double X[24], const double x[24], const ora_j2* m,
+                           double* point, double enhanced[7], double force[24],
+                           double stress[48], double info[6]);
+
 /* Radial return (plasticity.cpp:319-455, kinematic 0, no damage). strain and
  * eps_p are Voigt [xx,yy,zz,xy,yz,zx] with engineering shears; stress has tensor
  * components. Returns 1 when the point yielded. dissipation = sigma : d eps_p. */
@@ -173,6 +185,11 @@ typedef struct ora_case {
     int corners;
     const double* hex_k;
     const double* hex_mass;
+    /* plastic hexes: the forms of every element (B [T][8][6][24], G [T][8][6][7], Gauss
+     * volumes w [T][8]) */
+    const double* hex_B;
+    const double* hex_G;
+    const double* hex_w;
 } ora_case;
 
 typedef struct ora_state {
@@ -188,6 +205,10 @@ typedef struct ora_state {
     /* energy account: running sums over the steps taken so far */
     double w_ext, w_contact, w_boundary, dissipation, damping_loss;
     double fracture; /* stored energy released by torn tets, 1/2 V sigma:(eps - eps_p) */
+    /* plastic hexes: per element 8 points x (plastic strain 6, eqps, damage, failed) and
+     * the 7 enhanced parameters; tear_step (above) gets the step an element tore */
+    double* hex_point;    /* [T][72] */
+    double* hex_enhanced; /* [T][7] */
 } ora_state;
 
 typedef struct ora_energy {

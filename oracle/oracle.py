@@ -70,7 +70,8 @@ class CaseStruct(C.Structure):
                 ("ind_radius", C.c_double), ("ind_center", C.c_double * 3),
                 ("ind_velocity", C.c_double * 3), ("ind_kappa", C.c_double), ("dt", C.c_double),
                 ("rest_inv", D), ("volume", D), ("mass", D), ("adj_offset", U32),
-                ("adj_entry", U32), ("corners", C.c_int), ("hex_k", D), ("hex_mass", D)]
+                ("adj_entry", U32), ("corners", C.c_int), ("hex_k", D), ("hex_mass", D),
+                ("hex_B", D), ("hex_G", D), ("hex_w", D)]
 
 
 class StateStruct(C.Structure):
@@ -78,7 +79,7 @@ class StateStruct(C.Structure):
                 ("tear_step", I64), ("tear_normal", D), ("force", D), ("step", C.c_int64),
                 ("w_ext", C.c_double), ("w_contact", C.c_double), ("w_boundary", C.c_double),
                 ("dissipation", C.c_double), ("damping_loss", C.c_double),
-                ("fracture", C.c_double)]
+                ("fracture", C.c_double), ("hex_point", D), ("hex_enhanced", D)]
 
 
 class EnergyStruct(C.Structure):
@@ -95,6 +96,8 @@ ENERGY_FIELDS = ["strain", "kinetic", "dissipation", "w_ext", "w_contact", "w_bo
 
 def _declare(L):
     L.ora_hex_force.argtypes = [D, D, D, D]
+    L.ora_hex_plastic_update.argtypes = [D, D, D, D, D, C.POINTER(J2), D, D, D, D, D]
+    L.ora_hex_plastic_update.restype = C.c_int
     L.ora_box_mesh.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [I64, I64, D, U32]
     L.ora_box_mesh_f32.argtypes = [C.c_float] * 3 + [C.c_int] * 3 + [F, U32]
     L.ora_uniform.argtypes = [C.c_uint64, C.c_uint64]
@@ -273,12 +276,18 @@ class Case:
         self.N, self.T = len(self.X), len(self.index)
         self.corners = self.index.shape[1]
         if self.corners == 8:
-            if plastic:
-                raise ValueError("the hex oracle is elastic")
             from oracle import hex as H
             nodes = self.X[self.index]
-            self.hex_k = np.ascontiguousarray(H.stiffness(nodes, E, nu))
             self.hex_mass = np.ascontiguousarray(H.mass(nodes, rho))
+            if plastic:
+                B, G, w = H.forms(nodes)
+                self.hex_B, self.hex_G = np.ascontiguousarray(B), np.ascontiguousarray(G)
+                self.hex_w = np.ascontiguousarray(w)
+                self.hex_k = None
+            else:
+                self.hex_k = np.ascontiguousarray(H.stiffness(nodes, E, nu))
+            self.hex_point = np.zeros((self.T, 8, 9))
+            self.hex_enhanced = np.zeros((self.T, 7))
         self.flags = None if flags is None else np.ascontiguousarray(flags, np.uint32)
         self.load = None if load is None else np.ascontiguousarray(load, np.float64)
         self.prescribed = None if prescribed is None else np.ascontiguousarray(prescribed, np.float64)
@@ -310,6 +319,9 @@ class Case:
         if self.corners == 8:
             s.corners = 8
             s.hex_k, s.hex_mass = _p(self.hex_k, C.c_double), _p(self.hex_mass, C.c_double)
+            if plastic:
+                s.hex_B, s.hex_G = _p(self.hex_B, C.c_double), _p(self.hex_G, C.c_double)
+                s.hex_w = _p(self.hex_w, C.c_double)
         if lib().ora_case_init(C.byref(s)) != 0:
             raise MemoryError("oracle case init")
         self.s = s
@@ -330,6 +342,9 @@ class Case:
         st.tear_step = _p(self.tear_step, C.c_int64)
         st.tear_normal = _p(self.tear_normal, C.c_double)
         st.force = _p(self.force, C.c_double)
+        if self.corners == 8:
+            st.hex_point = _p(self.hex_point, C.c_double)
+            st.hex_enhanced = _p(self.hex_enhanced, C.c_double)
         self.st = st
 
     def __del__(self):
@@ -341,6 +356,17 @@ class Case:
     @property
     def step(self):
         return self.st.step
+
+    @property
+    def hex_eqps(self):
+        """Per hex element: the largest equivalent plastic strain of its points."""
+        return self.hex_point[:, :, 6].max(1)
+
+    @property
+    def hex_damage(self):
+        """Per hex element: the smallest damage of its points (1: every point failed, the
+        element is torn)."""
+        return self.hex_point[:, :, 7].min(1)
 
     def adopt(self, state):
         """Continue from a state computed elsewhere, as zone::Solver::adopt does
@@ -494,6 +520,9 @@ def ref_lib():
         R.ref_hex_rotation.argtypes = [D, D, D]
         R.ref_hex_critical_timestep.argtypes = [D] + [C.c_double] * 4
         R.ref_hex_critical_timestep.restype = C.c_double
+        R.ref_hex_plastic_path.argtypes = [D, D, C.c_int] + [C.c_double] * 8 + [D] * 5
+        R.ref_hex_failure_strain.argtypes = [D, C.c_double, C.c_double]
+        R.ref_hex_failure_strain.restype = C.c_double
         R.ref_hex_plate_counts.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [I64, I64]
         R.ref_hex_plate.argtypes = [C.c_double] * 3 + [C.c_int] * 3 + [C.c_double, D, U32, D]
         _ref = R

@@ -199,6 +199,57 @@ void ref_hex_rotation(const double rest[24], const double current[24], double ou
     sim::solidshell::elementRotation(rest, current, out);
 }
 
+// elementPlasticUpdate (solid_shell.cpp:826-1029) along a path of K current
+// configurations from a fresh state whose failure strain is `failure_strain` (<= 0: the
+// point law's never-fails value). Per step k: force[24], stress[48] (8 points x 6),
+// point state[8 x 9] (plastic strain 6, eqps, damage, failed), enhanced[7], and
+// info[6] = {dissipation, iterations, converged, yielded points, failed points, torn}.
+void ref_hex_plastic_path(const double rest[24], const double* currents, int K, double E,
+                          double nu, double yield, double hardening, double failure_strain,
+                          double sensitivity, double reference_eta, double cutoff_eta,
+                          double* force, double* stress, double* state, double* enhanced,
+                          double* info) {
+    sim::plasticity::Material mat;
+    mat.youngsModulus = E;
+    mat.poissonRatio = nu;
+    mat.flow = sim::plasticity::linearHardening(yield, hardening);
+    mat.failure.triaxialitySensitivity = sensitivity;
+    mat.failure.referenceTriaxiality = reference_eta;
+    mat.failure.cutoffTriaxiality = cutoff_eta;
+    sim::solidshell::ElementPlasticState st;
+    st.failureStrain = failure_strain > 0 ? failure_strain : sim::plasticity::kNeverFails;
+    for (int k = 0; k < K; ++k) {
+        const sim::solidshell::PlasticUpdate u = sim::solidshell::elementPlasticUpdate(
+            rest, currents + 24 * k, mat, sim::solidshell::Formulation::SolidShell, st,
+            force + 24 * k, stress + 48 * k);
+        for (int g = 0; g < 8; ++g) {
+            double* q = state + 72 * k + 9 * g;
+            for (int i = 0; i < 6; ++i) q[i] = st.point[g].plasticStrain[i];
+            q[6] = st.point[g].equivalentPlasticStrain;
+            q[7] = st.point[g].damage;
+            q[8] = st.point[g].failed ? 1.0 : 0.0;
+        }
+        for (int i = 0; i < 7; ++i) enhanced[7 * k + i] = st.enhanced[i];
+        double* in = info + 6 * k;
+        in[0] = u.dissipation;
+        in[1] = u.iterations;
+        in[2] = u.converged ? 1.0 : 0.0;
+        in[3] = u.yieldedPoints;
+        in[4] = u.failedPoints;
+        in[5] = st.torn ? 1.0 : 0.0;
+    }
+}
+
+// initialisePlasticState's failure strain for an element (regularised by its size).
+double ref_hex_failure_strain(const double nodes[24], double uniform, double fracture) {
+    sim::plasticity::Material mat;
+    mat.failure.uniformStrain = uniform;
+    mat.failure.fractureStrain = fracture;
+    sim::solidshell::ElementPlasticState st;
+    sim::solidshell::initialisePlasticState(nodes, mat, st);
+    return st.failureStrain;
+}
+
 // makePlateMesh counts, then arrays; uniformPressureLoad on it.
 void ref_hex_plate_counts(double lx, double ly, double t, int nx, int ny, int nz, int64_t* nodes,
                           int64_t* elements) {

@@ -36,6 +36,16 @@ E 206 GPa, nu 0.3, rho 7850):
   plate_*     makePlateMesh(0.6, 0.3, 0.012, 3, 2, 1) and (0.4, 0.2, 0.02, 2, 2, 2) with
               uniformPressureLoad(1e5)                      solid_shell.cpp:1152-1185,
                                                             1505-1561
+  pl{0,1,2}_* elementPlasticUpdate along three load paths of 40 configurations from a fresh
+              state (sigma_y 355 MPa, H 1 GPa; Rice-Tracey 1.5, 1/3, -1/3): a thin plate
+              element pulled in x to 25% at constant volume and bent to 1.5 /m, failure strain 0.12
+              (its points fail and the element tears); the noisy element bent and rotated
+              20 degrees without failure; a thick skewed element in simple shear to 0.8 with
+              a twist (shear varying across it), failure strain 0.25.
+              Force, point stress and state, enhanced parameters, dissipation,
+              iterations, yielded/failed counts, torn   solid_shell.cpp:826-1029
+  fs_*        initialisePlasticState's regularised failure strain of each test element
+              (uniform 0.12, fracture 0.8)                  solid_shell.cpp:770-815
 """
 import ctypes as C
 import os
@@ -293,6 +303,55 @@ def main_hex():
                         p(load, C.c_double))
         out[f"{name}_spec"] = np.array([lx, ly, t, nx, ny, nz])
         out[f"{name}_position"], out[f"{name}_index"], out[f"{name}_load"] = X, idx, load
+    # plastic paths
+    Ej, nuj, sy, H = 206e9, 0.3, 355e6, 1e9
+    out["pl_material"] = np.array([Ej, nuj, sy, H, 1.5, 1.0 / 3.0, -1.0 / 3.0])
+    K = 40
+    specs = [(0, 0.12), (2, 0.0), (6, 0.25)]
+    for j, (k, ef) in enumerate(specs):
+        x = np.ascontiguousarray(els[k])
+        c0 = x.mean(0)
+        size = np.ptp(x, axis=0)
+        cur = []
+        for step in range(1, K + 1):
+            t = step / K
+            if j == 0:
+                # near-incompressible pull with a superposed bending curvature
+                a = 0.25 * t
+                F = np.diag([1 + a, 1 / np.sqrt(1 + a), 1 / np.sqrt(1 + a)])
+                y = (x - c0) @ F.T + c0
+                kap = 1.5 * t  # 1/m, Kirchhoff bending about y on top of the pull
+                y[:, 0] -= kap * (x[:, 2] - c0[2]) * (x[:, 0] - c0[0])
+                y[:, 2] += 0.5 * kap * (x[:, 0] - c0[0]) ** 2
+            elif j == 1:
+                a = np.deg2rad(20.0) * t
+                Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+                bend = np.zeros_like(x)
+                bend[:, 0] = 0.08 * t * (x[:, 2] - c0[2]) * (x[:, 0] - c0[0]) / size[2] / 10
+                bend[:, 2] = -0.5 * 0.08 * t * (x[:, 0] - c0[0]) ** 2 / size[2] / 10
+                y = (x - c0 + bend) @ Rz.T + c0
+            else:
+                # simple shear with a shear gradient through the thickness
+                F = np.array([[1.0, 0.8 * t, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+                y = (x - c0) @ F.T + c0
+                y[:, 0] += 0.6 * t * (x[:, 2] - c0[2]) * (x[:, 1] - c0[1]) / size[1]
+            cur.append(y)
+        cur = np.ascontiguousarray(np.array(cur))
+        f = np.zeros((K, 24))
+        st = np.zeros((K, 48))
+        state = np.zeros((K, 72))
+        enh = np.zeros((K, 7))
+        info = np.zeros((K, 6))
+        R.ref_hex_plastic_path(p(x, C.c_double), p(cur, C.c_double), K, Ej, nuj, sy, H, ef, 1.5,
+                               1.0 / 3.0, -1.0 / 3.0, p(f, C.c_double), p(st, C.c_double),
+                               p(state, C.c_double), p(enh, C.c_double), p(info, C.c_double))
+        out.update({f"pl{j}_rest": x, f"pl{j}_current": cur, f"pl{j}_failure_strain": np.float64(ef),
+                    f"pl{j}_force": f, f"pl{j}_stress": st.reshape(K, 8, 6),
+                    f"pl{j}_state": state.reshape(K, 8, 9), f"pl{j}_enhanced": enh,
+                    f"pl{j}_info": info})
+    out["fs_strain"] = np.array([R.ref_hex_failure_strain(p(np.ascontiguousarray(x), C.c_double),
+                                                          0.12, 0.8) for x in els])
+
     path = os.path.join(HERE, "reference_hex.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path} ({os.path.getsize(path)} bytes, {len(out)} arrays)")
