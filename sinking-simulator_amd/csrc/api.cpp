@@ -199,11 +199,6 @@ shipfem_status shipfem_solver_create(const shipfem_case_desc* desc, shipfem_solv
 
 shipfem_status shipfem_hex_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
                                          char* err, size_t err_len) {
-    if (desc && desc->material.plastic) {
-        setError(err, err_len, "the hex solve is elastic: material.plastic must be 0");
-        if (out) *out = nullptr;
-        return SHIPFEM_UNSUPPORTED;
-    }
     return createSolver(desc, true, out, err, err_len);
 }
 
@@ -459,6 +454,10 @@ shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
         std::vector<uint32_t>().swap(hc.pell);
         std::vector<double>().swap(hc.hk);
         std::vector<float>().swap(hc.hgeo);
+        std::vector<double>().swap(hc.hB);
+        std::vector<double>().swap(hc.hG);
+        std::vector<double>().swap(hc.hw);
+        std::vector<double>().swap(hc.hgeod);
     }
     *out = b;
     return SHIPFEM_OK;
@@ -475,12 +474,6 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
 shipfem_status shipfem_hex_batch_create(const shipfem_case_desc* cases, uint32_t n,
                                         const shipfem_launch_config* cfg, shipfem_batch** out,
                                         char* err, size_t err_len) {
-    for (uint32_t i = 0; cases && i < n; ++i)
-        if (cases[i].material.plastic) {
-            setError(err, err_len, "case " + std::to_string(i) + ": the hex solve is elastic");
-            if (out) *out = nullptr;
-            return SHIPFEM_UNSUPPORTED;
-        }
     return createBatch(cases, n, cfg, true, out, err, err_len);
 }
 

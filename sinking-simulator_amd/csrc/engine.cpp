@@ -39,8 +39,10 @@ void Plan::dropGraph() {
 hipError_t Plan::build(hipStream_t s) {
     dropGraph();
     hex = !recs.empty() && recs[0].corners == 8;
+    // one element kind per plan, and hex plans all elastic or all plastic
     for (const CaseDev& r : recs)
-        if ((r.corners == 8) != hex) return hipErrorInvalidValue;  // one element kind per plan
+        if ((r.corners == 8) != hex || (hex && ((r.features ^ recs[0].features) & kFeatPlastic)))
+            return hipErrorInvalidValue;
     tilesFor(recs, shape.variant == kTiled && !hex ? kTile : shape.elem_block, false, elem_tiles);
     lmax = 0;
     for (const CaseDev& r : recs) lmax = std::max(lmax, r.lmax);
@@ -96,7 +98,10 @@ void Plan::launchStep(int par, int sample, hipStream_t s) {
 }
 
 void Plan::launchElem(int par, int sample, hipStream_t s, const KernelTiming* t) {
-    if (hex)
+    if (hex && (feat_union & kFeatPlastic))
+        launch_elem_hex_plastic(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block,
+                                par, sample, shape.xcd_remap, s, t);
+    else if (hex)
         launch_elem_hex(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
                         sample, shape.xcd_remap, s, t);
     else if (shape.variant == kTiled)

@@ -27,7 +27,7 @@ hipError_t toDevice(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
 }
 
 uint32_t pstatePlanes(const HostCase& hc) {
-    if (!(hc.features & kFeatPlastic)) return 0;
+    if (!(hc.features & kFeatPlastic) || hc.corners == 8) return 0;
     return (hc.features & kFeatDamage) ? 8 : 7;
 }
 
@@ -49,6 +49,8 @@ std::vector<Piece> evolvingState(const shipfem_solver* s) {
         {r.v, 12 * Np},
         {r.cv, 12 * Np},
         {r.pstate, 4ull * pstatePlanes(hc) * Tp},
+        {r.hpoint, r.hpoint ? 8ull * kHexPoint * Tp : 0},
+        {r.henh, r.henh ? 8ull * 7 * Tp : 0},
         {r.tear_step, dmg ? 4 * Tp : 0},
         {r.tear_stress, dmg ? 24 * Tp : 0},
         {r.acc_e, 8ull * kAccElemSlots * hc.elem_waves},
@@ -85,6 +87,9 @@ extern "C" {
 shipfem_status shipfem_solver_export(shipfem_solver* s, shipfem_handoff* out) {
     if (!s || !out) return SHIPFEM_INVALID_ARG;
     const HostCase& hc = s->hc;
+    // a plastic hex keeps eight points of history per element: not the per-element shape
+    // of the hand-off (checkpoint and restore carry it)
+    if (hc.corners == 8 && (hc.features & kFeatPlastic)) return SHIPFEM_UNSUPPORTED;
     const CaseDev& r = s->dc.rec;
     const uint64_t N = hc.N, T = hc.T, Np = hc.Npad, Tp = hc.Tpad;
     const hipStream_t st = s->stream;
@@ -161,6 +166,7 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
         return SHIPFEM_INVALID_ARG;
     const HostCase& hc = s->hc;
     const CaseDev& r = s->dc.rec;
+    if (hc.corners == 8 && (hc.features & kFeatPlastic)) return SHIPFEM_UNSUPPORTED;
     const double want = static_cast<double>(in->steps) * s->dc.dt;
     if (!(std::fabs(in->time - want) <= 1e-9 * std::max(std::fabs(want), s->dc.dt)))
         return SHIPFEM_INVALID_ARG;

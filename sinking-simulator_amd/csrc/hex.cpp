@@ -125,6 +125,9 @@ struct Forms {
     double G[kG][6][kE];
     double w[kG];
     double Jinv0[3][3];
+    double t0[6][6];       // the centre's covariant-to-Cartesian map
+    double scale[kG];      // det J0 / det J of each point
+    double invNorm[kE];    // each mode's normalisation
     bool ok = false;
 };
 
@@ -133,7 +136,7 @@ bool restForms(const double* x, Forms& f) {
     natural(x, 0, 0, 0, c);
     const double det0 = inverse3(c.J, f.Jinv0);
     if (!(det0 > 0)) return false;
-    double t0[6][6];
+    double (&t0)[6][6] = f.t0;
     toCartesian(f.Jinv0, t0);
     const double q = 1.0 / std::sqrt(3.0);
     // assumed-strain samples at the two zeta levels of the rule
@@ -189,6 +192,7 @@ bool restForms(const double* x, Forms& f) {
         m[3][5] = xi;
         m[3][6] = eta;
         const double scale = det0 / det;
+        f.scale[g] = scale;
         for (int i = 0; i < 6; ++i)
             for (int j = 0; j < kE; ++j) {
                 double s = 0;
@@ -200,8 +204,10 @@ bool restForms(const double* x, Forms& f) {
         double n2 = 0;
         for (int g = 0; g < kG; ++g)
             for (int i = 0; i < 6; ++i) n2 += f.w[g] * f.G[g][i][j] * f.G[g][i][j];
+        f.invNorm[j] = 1.0;
         if (!(n2 > 0)) continue;
         const double s = 1.0 / std::sqrt(n2);
+        f.invNorm[j] = s;
         for (int g = 0; g < kG; ++g)
             for (int i = 0; i < 6; ++i) f.G[g][i][j] *= s;
     }
@@ -398,7 +404,15 @@ void hexForce(const double K[kD * kD], const double* X, const double* x, double 
 std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<double>& mass) {
     const uint32_t T = hc.T;
     const uint64_t tp = hc.Tpad;
-    hc.hk.assign(kHexPacked * tp, 0.0);
+    const bool plastic = d.material.plastic != 0;
+    if (plastic) {
+        hc.hB.assign(uint64_t(kHexB) * tp, 0.0);
+        hc.hG.assign(uint64_t(kHexG) * tp, 0.0);
+        hc.hw.assign(8ull * tp, 0.0);
+        hc.hgeod.assign(kHexGeo * tp, 0.0);
+    } else {
+        hc.hk.assign(kHexPacked * tp, 0.0);
+    }
     hc.hgeo.assign(kHexGeo * tp, 0.0f);
     std::vector<double> em(8ull * T), omega(T, 0.0), volume(T, 0.0);
     std::vector<int> bad(T, 0);
@@ -414,6 +428,25 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
                 bad[e] = 1;
                 continue;
             }
+            if (plastic) {
+                // each point's strain rows for corners 1..7 (B annihilates translations, so
+                // corner 0's columns are minus the sum of the others), G in compact form
+                // (columns 0-3 of the centre's map, each point's det J0/det J, each mode's
+                // normalisation) and the points' volumes
+                Forms f;
+                restForms(x, f);
+                for (int g = 0; g < kG; ++g)
+                    for (int i = 0; i < 6; ++i)
+                        for (int j = 0; j < kHexDof; ++j)
+                            hc.hB[((g * 6 + i) * kHexDof + j) * tp + e] = f.B[g][i][3 + j];
+                for (int i = 0; i < 6; ++i)
+                    for (int c = 0; c < 4; ++c) hc.hG[(i * 4 + c) * tp + e] = f.t0[i][c];
+                for (int g = 0; g < kG; ++g) {
+                    hc.hG[(24 + g) * tp + e] = f.scale[g];
+                    hc.hw[g * tp + e] = f.w[g];
+                }
+                for (int k = 0; k < kE; ++k) hc.hG[(32 + k) * tp + e] = f.invNorm[k];
+            }
             for (int a = 0; a < kN; ++a) {
                 em[8ull * e + a] = m[a];
                 volume[e] += m[a];
@@ -422,12 +455,16 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
             omega[e] = hexOmegaMax(K, m);
             // corners 1..7: rows and columns 3..23 of K
             int p = 0;
-            for (int i = 0; i < kHexDof; ++i)
+            for (int i = 0; i < kHexDof && !plastic; ++i)
                 for (int j = i; j < kHexDof; ++j) hc.hk[(p++) * tp + e] = K[(3 + i) * kD + 3 + j];
             for (int a = 1; a < kN; ++a)
                 for (int i = 0; i < 3; ++i) {
                     hc.hgeo[(3 * (a - 1) + i) * tp + e] = static_cast<float>(b0[a][i]);
                     hc.hgeo[(kHexDof + 3 * (a - 1) + i) * tp + e] = static_cast<float>(x[3 * a + i] - x[i]);
+                    if (plastic) {
+                        hc.hgeod[(3 * (a - 1) + i) * tp + e] = b0[a][i];
+                        hc.hgeod[(kHexDof + 3 * (a - 1) + i) * tp + e] = x[3 * a + i] - x[i];
+                    }
                 }
         }
     };

@@ -34,7 +34,7 @@ struct Layout {
 
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
-    uint64_t conn, dmi, vol, force, hk, hgeo, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
+    uint64_t conn, dmi, vol, force, hk, hgeo, hB, hG, hw, hgeod, hpoint, henh, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
         ell_off, ell, lconn, tn_off, tnode, lofs, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
@@ -49,8 +49,15 @@ ArenaMap mapArena(const HostCase& hc) {
     m.force = L.take(12ull * hc.corners * T);
     m.hk = L.take(8ull * hc.hk.size());
     m.hgeo = L.take(4ull * hc.hgeo.size());
+    m.hB = L.take(8ull * hc.hB.size());
+    m.hG = L.take(8ull * hc.hG.size());
+    m.hw = L.take(8ull * hc.hw.size());
+    m.hgeod = L.take(8ull * hc.hgeod.size());
+    const bool hexPlastic = hc.corners == 8 && (hc.features & kFeatPlastic);
+    m.hpoint = L.take(hexPlastic ? 8ull * kHexPoint * Tp : 0);
+    m.henh = L.take(hexPlastic ? 8ull * 7 * Tp : 0);
     const bool dmg = (hc.features & kFeatDamage) != 0;
-    m.pstate = L.take((hc.features & kFeatPlastic) ? (dmg ? 32 : 28) * Tp : 0);
+    m.pstate = L.take((hc.features & kFeatPlastic) && !hexPlastic ? (dmg ? 32 : 28) * Tp : 0);
     m.tear_step = L.take(dmg ? 4 * Tp : 0);
     m.tear_stress = L.take(dmg ? 24 * Tp : 0);
     m.u4 = L.take(16 * N);
@@ -576,7 +583,6 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
 }
 
 std::string prepareHexCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_cap) {
-    if (d.material.plastic) return "the hex solve is elastic: material.plastic must be 0";
     return prepareInternal(d, hc, hist_cap, {}, {}, 8);
 }
 
@@ -599,10 +605,18 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.conn = reinterpret_cast<const uint4*>(base + m.conn);
     r.hk = hc.hk.empty() ? nullptr : reinterpret_cast<const double*>(base + m.hk);
     r.hgeo = hc.hgeo.empty() ? nullptr : reinterpret_cast<const float*>(base + m.hgeo);
+    const bool hexPlastic = hc.corners == 8 && (hc.features & kFeatPlastic);
+    r.hB = hexPlastic ? reinterpret_cast<const double*>(base + m.hB) : nullptr;
+    r.hG = hexPlastic ? reinterpret_cast<const double*>(base + m.hG) : nullptr;
+    r.hw = hexPlastic ? reinterpret_cast<const double*>(base + m.hw) : nullptr;
+    r.hgeod = hexPlastic ? reinterpret_cast<const double*>(base + m.hgeod) : nullptr;
+    r.hpoint = hexPlastic ? reinterpret_cast<double*>(base + m.hpoint) : nullptr;
+    r.henh = hexPlastic ? reinterpret_cast<double*>(base + m.henh) : nullptr;
     r.dmi = reinterpret_cast<const float*>(base + m.dmi);
     r.vol = reinterpret_cast<const float*>(base + m.vol);
     r.force = reinterpret_cast<float*>(base + m.force);
-    r.pstate = (hc.features & kFeatPlastic) ? reinterpret_cast<float*>(base + m.pstate) : nullptr;
+    r.pstate = (hc.features & kFeatPlastic) && !hexPlastic ? reinterpret_cast<float*>(base + m.pstate)
+                                                           : nullptr;
     r.tear_step = (hc.features & kFeatDamage) ? reinterpret_cast<int32_t*>(base + m.tear_step) : nullptr;
     r.tear_stress = (hc.features & kFeatDamage) ? reinterpret_cast<float*>(base + m.tear_stress) : nullptr;
     r.u4 = reinterpret_cast<float4*>(base + m.u4);
@@ -644,6 +658,12 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.ind_r = static_cast<float>(hc.indenter.radius);
     r.lam_d = hc.lam;
     r.mu_d = hc.mu;
+    r.yield_d = hc.yield;
+    r.hard_d = hc.hard;
+    r.fail_d = hc.fail_strain;
+    r.sens_d = hc.fail_sens;
+    r.eref_d = hc.fail_eref;
+    r.ecut_d = hc.fail_ecut;
     r.acc_e = reinterpret_cast<double*>(base + m.acc_e);
     r.acc_n = reinterpret_cast<double*>(base + m.acc_n);
     r.smp_e = reinterpret_cast<double*>(base + m.smp_e);
@@ -665,6 +685,10 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.vol, hc.vol.data(), 4ull * hc.vol.size());
     up(m.hk, hc.hk.data(), 8ull * hc.hk.size());
     up(m.hgeo, hc.hgeo.data(), 4ull * hc.hgeo.size());
+    up(m.hB, hc.hB.data(), 8ull * hc.hB.size());
+    up(m.hG, hc.hG.data(), 8ull * hc.hG.size());
+    up(m.hw, hc.hw.data(), 8ull * hc.hw.size());
+    up(m.hgeod, hc.hgeod.data(), 8ull * hc.hgeod.size());
     up(m.u4, hc.u4.data(), 4ull * hc.u4.size());
     up(m.cu, hc.cu.data(), 4ull * hc.cu.size());
     up(m.v, hc.v.data(), 4ull * hc.v.size());
@@ -710,6 +734,12 @@ hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_
     if (e == hipSuccess) e = hipMemcpyAsync(step, r.step, 16, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&status, r.status, 4, hipMemcpyDeviceToHost, s);
     std::vector<float> ps;
+    std::vector<double> hp;  // plastic hex points
+    if (r.hpoint && (view->eqps || view->plastic_strain || view->damage)) {
+        hp.resize(uint64_t(kHexPoint) * hc.Tpad);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hp.data(), r.hpoint, 8 * hp.size(), hipMemcpyDeviceToHost, s);
+    }
     const bool dmg = (hc.features & kFeatDamage) != 0;
     if (r.pstate && (view->eqps || view->plastic_strain || view->damage)) {
         ps.resize((dmg ? 8 : 7) * (uint64_t)hc.Tpad);
@@ -733,6 +763,26 @@ hipError_t readbackCase(const DeviceCase& dc, const HostCase& hc, shipfem_state_
     }
     const uint64_t Tp = hc.Tpad;
     const uint32_t T = hc.T;
+    if (!hp.empty()) {
+        // a hex element reports its largest point eqps, its points' mean plastic strain and
+        // its smallest point damage (1: every point has failed, the element is torn)
+        for (uint32_t t = 0; t < T; ++t) {
+            double q = 0, dmin = 1.0, ep[6] = {0, 0, 0, 0, 0, 0};
+            for (int g = 0; g < 8; ++g) {
+                q = std::max(q, hp[(8 * g + 6) * Tp + t]);
+                dmin = std::min(dmin, hp[(8 * g + 7) * Tp + t]);
+                for (int k = 0; k < 6; ++k) ep[k] += 0.125 * hp[(8 * g + k) * Tp + t];
+            }
+            if (view->eqps) view->eqps[t] = static_cast<float>(q);
+            if (view->damage) view->damage[t] = dmg ? static_cast<float>(dmin) : 0.0f;
+            if (view->plastic_strain)
+                for (int k = 0; k < 6; ++k) view->plastic_strain[t * 6ull + k] = static_cast<float>(ep[k]);
+        }
+        view->step = std::max(step[0], step[1]);
+        view->time = static_cast<double>(view->step) * dc.dt;
+        view->status = status;
+        return hipSuccess;
+    }
     if (view->eqps)
         for (uint32_t t = 0; t < T; ++t) view->eqps[externalTet(hc, t)] = ps.empty() ? 0.0f : ps[6 * Tp + t];
     if (view->plastic_strain)
@@ -784,7 +834,7 @@ hipError_t readTears(const DeviceCase& dc, const HostCase& hc, std::vector<shipf
         tear.step = static_cast<int64_t>(step[t]) - 1;
         double v[6];
         for (int k = 0; k < 6; ++k) v[k] = st[k * Tp + t];
-        maxPrincipalDirection(v, tear.normal);
+        if (hc.corners == 4) maxPrincipalDirection(v, tear.normal);  // a hex reports no plane
         out.push_back(tear);
     }
     std::sort(out.begin(), out.end(), [](const shipfem_tear& a, const shipfem_tear& b) {

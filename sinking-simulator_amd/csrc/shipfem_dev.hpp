@@ -52,6 +52,13 @@ struct EnergyRecordDev {
 constexpr int kHexDof = 21;
 constexpr int kHexPacked = kHexDof * (kHexDof + 1) / 2;  // 231
 constexpr int kHexGeo = 42;  // centre gradients of corners 1..7, then X_a - X_0
+// Plastic solid-shell hex: per Gauss point g the strain rows for corners 1..7
+// (B[g][i][j], i < 6, j < 21), G in compact form (the centre's map columns 0-3 [6][4], the
+// points' det J0/det J [8], the modes' normalisations [7]), and per element 8 points x
+// (plastic strain 6, eqps, damage) plus the 7 enhanced parameters.
+constexpr int kHexB = 8 * 6 * kHexDof;  // 1008
+constexpr int kHexG = 24 + 8 + 7;       // 39
+constexpr int kHexPoint = 8 * 8;        // 64
 
 struct CaseDev {
     uint32_t N, T;            // nodes, elements (tets, or hexes when corners == 8)
@@ -68,6 +75,12 @@ struct CaseDev {
     const double* hk;    // hex: [kHexPacked][Tpad] stiffness block of corners 1..7 (FP64)
     const float* hgeo;   // hex: [kHexGeo][Tpad] d N_a/dX at the centre (21: 3(a-1) + j),
                          // then rest X_a - X_0 (21: 21 + 3(a-1) + i), corners a = 1..7
+    const double* hB;    // plastic hex: [kHexB][Tpad], entry (g*6 + i)*21 + j
+    const double* hG;    // plastic hex: [kHexG][Tpad]
+    const double* hw;    // plastic hex: [8][Tpad] Gauss-point volumes
+    const double* hgeod; // plastic hex: hgeo in FP64 [kHexGeo][Tpad]
+    double* hpoint;      // plastic hex: [kHexPoint][Tpad], entry 8 g + k
+    double* henh;        // plastic hex: [7][Tpad] enhanced parameters (warm start)
     float* pstate;       // [7 or 8][Tpad] plastic strain (6, eng. shears), eqps, damage
     int32_t* tear_step;  // [Tpad] step + 1 at which the tet tore, 0 intact (kFeatDamage)
     float* tear_stress;  // [6][Tpad] world-frame stress at the tear (kFeatDamage)
@@ -108,6 +121,8 @@ struct CaseDev {
     float fail_eref, fail_ecut;  // reference and cutoff triaxiality
     float pad1[2];
     double lam_d, mu_d;       // FP64 Lame constants for the stable-step evaluation
+    double yield_d, hard_d;   // FP64 flow curve and failure constants (plastic hexes)
+    double fail_d, sens_d, eref_d, ecut_d;
     // ---- FP64 accounting
     double* acc_e;  // [kAccElemSlots][elem_waves]
     double* acc_n;  // [kAccNodeSlots][node_waves]
@@ -161,6 +176,10 @@ void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
 // The solid-shell hex element pass: one lane per element, `block` threads per block.
 void launch_elem_hex(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
                      int sample, int xcd, hipStream_t stream, const KernelTiming* timing = nullptr);
+// The plastic solid-shell hex element pass: one lane per element, `block` threads per block.
+void launch_elem_hex_plastic(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
+                             int parity, int sample, int xcd, hipStream_t stream,
+                             const KernelTiming* timing = nullptr);
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
                        int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream,
                        const KernelTiming* timing = nullptr);

@@ -212,3 +212,89 @@ def test_hex_batch_matches_each_case_alone_and_tunes():
     b.run(400)
     for i, u in enumerate(alone):
         assert np.array_equal(b.readback(i).u, u), i
+
+
+# ---------------------------------------------------------------- plastic solid-shell
+PLASTIC = dict(plastic=True, yield_=355e6, hardening=1e9)
+ABS_EQPS = 1e-4
+
+
+def plastic_zone(speed, failure=None, nx=24, ny=12, **kw):
+    return configs.hex_zone(lx=0.3, ly=0.15, nx=nx, ny=ny, radius=0.04, speed=speed,
+                            failure=failure, **PLASTIC, **kw)
+
+
+def test_plastic_element_pass_matches_the_compiled_reference_update():
+    # one element pass from a fresh history at configurations of the golden plastic paths
+    # (tests/golden/reference_hex.npz): forces and the committed point history against the
+    # oracle's update, which those vectors pin to the compiled elementPlasticUpdate
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_hex.npz"))
+    E_, nu_, sy, Hh, sens, er, ec = g["pl_material"]
+    for j in range(3):
+        X = g[f"pl{j}_rest"]
+        ef = float(g[f"pl{j}_failure_strain"])
+        k = 30 if j == 1 else 4  # configurations past first yield on each path
+        cur = g[f"pl{j}_current"][k]
+        case = sf.HexCase(X, np.arange(8, dtype=np.uint32)[None], E=E_, nu=nu_, plastic=True,
+                          yield_=sy, hardening=Hh, u0=cur - X,
+                          failure=dict(failure_strain=ef) if ef > 0 else None)
+        s = sf.Solver(case)
+        s.run(1)
+        f = s.tet_forces().astype(np.float64)[0]
+        o = oracle_for(case, s.dt)
+        o.run(1)
+        err = float(np.abs(f - o.force[0]).max() / np.abs(o.force[0]).max())
+        eq = float(abs(s.readback().eqps[0] - o.hex_eqps[0]))
+        say(f"plastic element pass, path {j}: force max rel {err:.2e}, eqps |d| {eq:.2e} "
+            f"(eqps {o.hex_eqps[0]:.3e})")
+        assert err <= 1e-5 and eq <= 1e-7 and o.hex_eqps[0] > 0
+
+
+def test_plastic_punch_matches_oracle():
+    case = plastic_zone(5.0)
+    s = sf.Solver(case)
+    o = oracle_for(case, s.dt)
+    ro = o.run(3000, sample_every=100, threads=8)
+    s.run(3000)
+    st = s.readback()
+    g = s.energy()
+    err = rel_l2(st.u, o.u)
+    deq = float(np.abs(st.eqps - o.hex_eqps).max())
+    gap = energy_gap(g, ro)
+    dd = abs(g[-1]["dissipation"] - ro[-1]["dissipation"]) / ro[-1]["dissipation"]
+    say(f"plastic punch 3000 steps: u rel L2 {err:.3e}, eqps |d| {deq:.2e} (max {o.hex_eqps.max():.3f}), "
+        f"dissipation {ro[-1]['dissipation']:.3e} J (rel {dd:.1e}), energy gap {gap:.3e}")
+    assert o.hex_eqps.max() > 0.01
+    assert err <= REL_U and deq <= ABS_EQPS and gap <= ENERGY and dd <= 1e-4
+
+
+def test_plastic_punch_with_tearing_matches_oracle():
+    case = plastic_zone(10.0, failure=dict(failure_strain=0.06))
+    s = sf.Solver(case)
+    o = oracle_for(case, s.dt)
+    ro = o.run(1500, sample_every=100, threads=8)
+    s.run(1500)
+    st = s.readback()
+    torn_g = set(np.nonzero(st.damage >= 1)[0])
+    torn_o = set(np.nonzero(o.hex_damage >= 1)[0])
+    tg = {t: st_ for t, st_, _ in s.tears()}
+    same_steps = sum(1 for t in torn_g & torn_o if tg[t] == o.tear_step[t])
+    err = rel_l2(st.u, o.u)
+    deq = float(np.abs(st.eqps - o.hex_eqps).max())
+    say(f"plastic punch with tearing, 1500 steps: torn {len(torn_g)} / {len(torn_o)} "
+        f"(same step {same_steps}), u rel L2 {err:.3e}, eqps |d| {deq:.2e}, "
+        f"fracture {ro[-1]['fracture']:.3e} J")
+    assert len(torn_o) > 0 and len(torn_g ^ torn_o) <= max(1, len(torn_o) // 50)
+    assert err <= REL_U and deq <= ABS_EQPS
+    # a checkpoint carries the eight points of every element: the restored solver
+    # continues with the same bits
+    blob = s.checkpoint()
+    s.run(200)
+    r = sf.Solver(case).restore(blob)
+    r.run(200)
+    a, b = s.readback(), r.readback()
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.eqps, b.eqps)
+    with pytest.raises(sf.ShipfemError) as e:
+        s.export()
+    assert e.value.status == sf.UNSUPPORTED

@@ -209,13 +209,6 @@ def test_oracle_hex_solve_closes_its_energy_account_and_translates_freely():
     assert np.abs(f.force).max() <= 1e-3
 
 
-def test_hex_solver_refuses_a_plastic_case_before_touching_a_device():
-    case = sf.HexCase.plate(0.2, 0.1, 0.01, 2, 2, 1, plastic=True)
-    with pytest.raises(sf.ShipfemError) as e:
-        sf.Solver(case)
-    assert e.value.status == sf.UNSUPPORTED
-
-
 def plastic_update(X, x, j2, point, enh, B=None, G=None, w=None):
     if B is None:
         B, G, w = (np.ascontiguousarray(a[0]) for a in H.forms(X[None]))
