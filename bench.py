@@ -1,7 +1,7 @@
 """Benchmark of the explicit tet step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--csv FILE]
-                    [--workload plate|cantilever|panel|batch|serve|sweep|hex]
+                    [--workload plate|cantilever|panel|batch|serve|sweep|hex|hex_plastic]
 
 Default workload: the large plain-path plate (10 x 4 x 0.02 m, 350x144x4 Kuhn cells,
 1,209,600 tets, nodes jittered +-5%, edges clamped, 200 kPa ramped over 5 ms,
@@ -265,16 +265,21 @@ def bench_sweep(args, sf, info, rows):
                           device=info[0], table=table, csv=args.csv)))
 
 
-def hex_cpu_baseline(budget_s=8.0):
-    """The elastic zone step's element work (solidshell::internalForce with a 24x24 FP64
-    stiffness per element, zone.cpp:928-1012) and node update, as the oracle's C hex
-    solve: a bounded sample, a 100 x 100-element patch of the same plating."""
+def hex_cpu_baseline(budget_s=8.0, plastic=False):
+    """The zone step's element work (elastic: solidshell::internalForce with a 24x24 FP64
+    stiffness per element; plastic: elementPlasticUpdate; zone.cpp:928-1012) and node
+    update, as the oracle's C hex solve: a bounded sample, a 100 x 100-element patch of
+    the same plating (plastic: 40 x 20 elements, punched)."""
     import numpy as np
 
     from oracle import oracle as O
     from shipfem import configs
 
-    c = configs.hex_plate(lx=2.5, ly=2.5, nx=100, ny=100, pressure=50e3)
+    if plastic:
+        c = configs.hex_zone(lx=0.4, ly=0.2, nx=40, ny=20, radius=0.05, speed=5.0, plastic=True,
+                             yield_=355e6, hardening=1e9, failure=dict(failure_strain=0.1))
+    else:
+        c = configs.hex_plate(lx=2.5, ly=2.5, nx=100, ny=100, pressure=50e3)
     o = O.Case(c.X, c.index, dt=1e-7, **c.oracle_kwargs())
     threads = max(1, min(16, os.cpu_count() or 1))
     out = {}
@@ -287,18 +292,38 @@ def hex_cpu_baseline(budget_s=8.0):
             n += 5
         out[label] = (c.T * n / (time.perf_counter() - t0), n)
     del np
+    what = ("the enhanced-strain Newton and eight J2 returns per element, as the reference's CPU "
+            "zone step" if plastic else
+            "polar rotation to convergence, full 24x24 stiffness per element as the reference's "
+            "CPU zone step")
     return dict(value=out["all"][0], unit="hex-steps/s", cores=threads, kind="port",
-                sample=(f"oracle hex solve in C (FP64: polar rotation to convergence, full 24x24 "
-                        f"stiffness per element as the reference's CPU zone step) on a {c.T}-element "
-                        f"patch of the same plating: {out['all'][1]} steps on {threads} threads; "
-                        f"one core {out['one'][0]:.3e} hex-steps/s over {out['one'][1]} steps"))
+                sample=(f"oracle hex solve in C (FP64: {what}) on a {c.T}-element patch of the same "
+                        f"plating: {out['all'][1]} steps on {threads} threads; one core "
+                        f"{out['one'][0]:.3e} hex-steps/s over {out['one'][1]} steps"))
 
 
-def bench_hex(args, sf, info, rows):
-    """The elastic solid-shell solve on the 1.2M-element hex plate."""
+# Plastic solid-shell pass, per element: connectivity 32 + forms (B 1008 x 8, G compact
+# 39 x 8, Gauss volumes 8 x 8, rest geometry 42 x 8) + point history and enhanced
+# parameters read and written 2 x 71 x 8 + force-slot write 96; per node 28.
+HEX_PL_ELEM_B_T = 32 + (1008 + 39 + 8 + 42) * 8 + 2 * 71 * 8 + 96
+
+
+def bench_hex(args, sf, info, rows, plastic=False):
+    """The elastic solid-shell solve on the 1.2M-element hex plate, or (plastic) the plastic
+    solve of a 12 x 6 m zone of 12 mm plating punched by a sphere."""
     from shipfem import configs
 
-    case = configs.hex_plate()
+    if plastic:
+        case = configs.hex_zone(lx=12.0, ly=6.0, nx=1200, ny=600, radius=0.5, speed=5.0,
+                                plastic=True, yield_=355e6, hardening=1e9,
+                                failure=dict(failure_strain=0.1))
+        desc = ("solid-shell zone 12x6 m x 12 mm, 1200x600x1 elements, edges clamped, r=0.5 m "
+                "sphere at 5 m/s, J2 355 MPa / H 1 GPa with ductile failure 0.1 (FP64 enhanced-strain "
+                "Newton per element)")
+    else:
+        case = configs.hex_plate()
+        desc = ("solid-shell hex plate 30x25 m x 12 mm, 1200x1000x1 elements, edges clamped, 50 kPa "
+                "ramped over 5 ms, elastic co-rotational (FP64 stiffness block, FP32 + Kahan state)")
     T, N = case.T, case.N
     t0 = time.perf_counter()
     solver = sf.Solver(case)
@@ -313,14 +338,15 @@ def bench_hex(args, sf, info, rows):
         raise SystemExit("state went non-finite")
     value = T * args.steps / wall
     elem_ms, node_ms = solver.profile(args.profile_steps)
-    elem_bytes = HEX_ELEM_B_T * T + HEX_ELEM_B_N * N
+    elem_bytes = (HEX_PL_ELEM_B_T if plastic else HEX_ELEM_B_T) * T + HEX_ELEM_B_N * N
     node_bytes = HEX_NODE_B_T * T + NODE_B_N * N
+    ek = "k_elem_hex_plastic" if plastic else "k_elem_hex"
     if elem_ms >= node_ms:
-        dom, dom_ms, dom_bytes = "element pass (k_elem_hex)", elem_ms, elem_bytes
+        dom, dom_ms, dom_bytes = f"element pass ({ek})", elem_ms, elem_bytes
     else:
         dom, dom_ms, dom_bytes = "node pass (k_node)", node_ms, node_bytes
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles("hex")
+    traffic = traffic_from_profiles("hex_plastic" if plastic else "hex")
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=(traffic or {}).get(dom.split()[0]), kernel=dom,
@@ -331,11 +357,12 @@ def bench_hex(args, sf, info, rows):
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = hex_cpu_baseline()
+            cpu = hex_cpu_baseline(plastic=plastic)
         except Exception as e:
             cpu = dict(value=None, unit="hex-steps/s", cores=0, kind="port", sample=f"failed: {e}")
     if rows:
-        rows.add(args.steps, workload="hex", elements=T, nodes=N, warmup=args.warmup,
+        rows.add(args.steps, workload="hex_plastic" if plastic else "hex", elements=T, nodes=N,
+                 warmup=args.warmup,
                  wall_ms_per_step=round(wall * 1e3 / args.steps, 6),
                  device_ms_per_step=round(dev_ms / args.steps, 6), hex_steps_per_s=round(value),
                  elem_ms=round(elem_ms, 6), node_ms=round(node_ms, 6))
@@ -343,10 +370,7 @@ def bench_hex(args, sf, info, rows):
               steps=args.steps, warmup=args.warmup, ms_per_step=wall * 1e3 / args.steps,
               higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32+fp64",
               data="synthetic",
-              config=dict(workload=("solid-shell hex plate 30x25 m x 12 mm, 1200x1000x1 elements, "
-                                    "edges clamped, 50 kPa ramped over 5 ms, elastic co-rotational "
-                                    "(FP64 stiffness block, FP32 + Kahan state)"),
-                          elements=T, nodes=N, dt=ts["dt"], cfl=ts["cfl"], device=info[0],
+              config=dict(workload=desc, elements=T, nodes=N, dt=ts["dt"], cfl=ts["cfl"], device=info[0],
                           host_prep_s=round(prep_s, 2), device_ms_per_step=dev_ms / args.steps,
                           step_hbm_gbs=round(step_gbs, 1), algorithmic_bytes_per_step=step_bytes),
               roofline=roofline, cpu_baseline=cpu))
@@ -395,8 +419,8 @@ def main():
         return bench_batch(args, sf, info)
     if args.workload == "serve":
         return bench_serve(args, sf, info)
-    if args.workload == "hex":
-        return bench_hex(args, sf, info, rows)
+    if args.workload in ("hex", "hex_plastic"):
+        return bench_hex(args, sf, info, rows, plastic=args.workload == "hex_plastic")
     case, desc = workload(args.workload)
     T, N = case.T, case.N
     variant = sf.VARIANT_TILED if args.variant == "tiled" else sf.VARIANT_SLOTS
