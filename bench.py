@@ -302,10 +302,10 @@ def hex_cpu_baseline(budget_s=8.0, plastic=False):
                         f"{out['one'][0]:.3e} hex-steps/s over {out['one'][1]} steps"))
 
 
-# Plastic solid-shell pass, per element: connectivity 32 + forms (B 1008 x 8, G compact
-# 39 x 8, Gauss volumes 8 x 8, rest geometry 42 x 8) + point history and enhanced
-# parameters read and written 2 x 71 x 8 + force-slot write 96; per node 28.
-HEX_PL_ELEM_B_T = 32 + (1008 + 39 + 8 + 42) * 8 + 2 * 71 * 8 + 96
+# Plastic solid-shell pass, per element: connectivity 32 + rest geometry in FP64 42 x 8 (the
+# strain operator is formed from it in the pass) + point history and enhanced parameters
+# read and written 2 x 71 x 8 + force-slot write 96; per node 28.
+HEX_PL_ELEM_B_T = 32 + 42 * 8 + 2 * 71 * 8 + 96
 
 
 def bench_hex(args, sf, info, rows, plastic=False):

@@ -454,9 +454,6 @@ shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
         std::vector<uint32_t>().swap(hc.pell);
         std::vector<double>().swap(hc.hk);
         std::vector<float>().swap(hc.hgeo);
-        std::vector<double>().swap(hc.hB);
-        std::vector<double>().swap(hc.hG);
-        std::vector<double>().swap(hc.hw);
         std::vector<double>().swap(hc.hgeod);
     }
     *out = b;

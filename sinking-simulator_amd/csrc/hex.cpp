@@ -125,9 +125,6 @@ struct Forms {
     double G[kG][6][kE];
     double w[kG];
     double Jinv0[3][3];
-    double t0[6][6];       // the centre's covariant-to-Cartesian map
-    double scale[kG];      // det J0 / det J of each point
-    double invNorm[kE];    // each mode's normalisation
     bool ok = false;
 };
 
@@ -136,7 +133,7 @@ bool restForms(const double* x, Forms& f) {
     natural(x, 0, 0, 0, c);
     const double det0 = inverse3(c.J, f.Jinv0);
     if (!(det0 > 0)) return false;
-    double (&t0)[6][6] = f.t0;
+    double t0[6][6];
     toCartesian(f.Jinv0, t0);
     const double q = 1.0 / std::sqrt(3.0);
     // assumed-strain samples at the two zeta levels of the rule
@@ -192,7 +189,6 @@ bool restForms(const double* x, Forms& f) {
         m[3][5] = xi;
         m[3][6] = eta;
         const double scale = det0 / det;
-        f.scale[g] = scale;
         for (int i = 0; i < 6; ++i)
             for (int j = 0; j < kE; ++j) {
                 double s = 0;
@@ -204,10 +200,8 @@ bool restForms(const double* x, Forms& f) {
         double n2 = 0;
         for (int g = 0; g < kG; ++g)
             for (int i = 0; i < 6; ++i) n2 += f.w[g] * f.G[g][i][j] * f.G[g][i][j];
-        f.invNorm[j] = 1.0;
         if (!(n2 > 0)) continue;
         const double s = 1.0 / std::sqrt(n2);
-        f.invNorm[j] = s;
         for (int g = 0; g < kG; ++g)
             for (int i = 0; i < 6; ++i) f.G[g][i][j] *= s;
     }
@@ -406,9 +400,6 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
     const uint64_t tp = hc.Tpad;
     const bool plastic = d.material.plastic != 0;
     if (plastic) {
-        hc.hB.assign(uint64_t(kHexB) * tp, 0.0);
-        hc.hG.assign(uint64_t(kHexG) * tp, 0.0);
-        hc.hw.assign(8ull * tp, 0.0);
         hc.hgeod.assign(kHexGeo * tp, 0.0);
     } else {
         hc.hk.assign(kHexPacked * tp, 0.0);
@@ -427,25 +418,6 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
                 !hexCentreGradient(x, b0)) {
                 bad[e] = 1;
                 continue;
-            }
-            if (plastic) {
-                // each point's strain rows for corners 1..7 (B annihilates translations, so
-                // corner 0's columns are minus the sum of the others), G in compact form
-                // (columns 0-3 of the centre's map, each point's det J0/det J, each mode's
-                // normalisation) and the points' volumes
-                Forms f;
-                restForms(x, f);
-                for (int g = 0; g < kG; ++g)
-                    for (int i = 0; i < 6; ++i)
-                        for (int j = 0; j < kHexDof; ++j)
-                            hc.hB[((g * 6 + i) * kHexDof + j) * tp + e] = f.B[g][i][3 + j];
-                for (int i = 0; i < 6; ++i)
-                    for (int c = 0; c < 4; ++c) hc.hG[(i * 4 + c) * tp + e] = f.t0[i][c];
-                for (int g = 0; g < kG; ++g) {
-                    hc.hG[(24 + g) * tp + e] = f.scale[g];
-                    hc.hw[g * tp + e] = f.w[g];
-                }
-                for (int k = 0; k < kE; ++k) hc.hG[(32 + k) * tp + e] = f.invNorm[k];
             }
             for (int a = 0; a < kN; ++a) {
                 em[8ull * e + a] = m[a];

@@ -1,0 +1,471 @@
+// hex_plastic.hpp -- the plastic solid-shell element update in FP64, with its strain operator
+// formed from the element's rest coordinates rather than read from memory.
+//
+// elementPlasticUpdate (engine/sim/solid_shell.cpp:826-1029) on the forms of
+// computeForms (solid_shell.cpp:237-365), evaluated where they are used:
+//   - covariant strain rows [E11, E22, E33, 2E12, 2E23, 2E31] of the rest base vectors g_k
+//     and the displacement's natural derivatives (solid_shell.cpp:140-158);
+//   - assumed natural transverse shear from (xi, eta) = (-1, 0), (+1, 0) for 2E23 and
+//     (0, -1), (0, +1) for 2E31, assumed natural thickness strain from the four in-plane
+//     corners, each at the Gauss point's zeta (solid_shell.cpp:181-235);
+//   - the push to Cartesian strain with A = J^-1, eps = A^T E A (solid_shell.cpp:116-138);
+//   - seven enhanced modes: column k at point g is (det J0/det J_g) m_k(g) T0[:, r_k],
+//     normalised in the weighted L2 norm over the points (solid_shell.cpp:160-179, 313-365).
+// The force is the same operator transposed: the Cartesian stress is pulled back to the
+// covariant frame, Q = A S A^T, and spread over the nodes through the same rows, the
+// assumed rows through their sampling points. Everything works on corner offsets from
+// corner 0 (X_0 = 0, d_0 = 0): the operator annihilates translations.
+//
+// Shared by the device pass and by host code; the caller supplies the co-rotated corner
+// offsets d and receives the internal force of corners 1..7 in the co-rotated frame.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace shipfem::hexp {
+
+struct Material {
+    double mu, kappa, yield, hard;
+    double fail, sens, eref, ecut;  // failure strain (<= 0: none) and Rice-Tracey constants
+    bool damage;
+};
+
+struct Result {
+    double dissipation = 0.0;  // sum w sigma : d eps_p
+    double released = 0.0;     // stored energy of the points that failed in this update
+    double strain = 0.0;       // sum w 1/2 sigma : C^-1 : sigma (when asked)
+    int failedStart = 0, failedEnd = 0;
+};
+
+// d N_a / d(xi, eta, zeta) of corner a at a natural point.
+__host__ __device__ inline void dshape(int a, double xi, double eta, double zeta, double out[3]) {
+    const double sx = (a == 1 || a == 2 || a == 5 || a == 6) ? 1.0 : -1.0;
+    const double sy = (a == 2 || a == 3 || a == 6 || a == 7) ? 1.0 : -1.0;
+    const double sz = a >= 4 ? 1.0 : -1.0;
+    out[0] = 0.125 * sx * (1 + eta * sy) * (1 + zeta * sz);
+    out[1] = 0.125 * sy * (1 + xi * sx) * (1 + zeta * sz);
+    out[2] = 0.125 * sz * (1 + xi * sx) * (1 + eta * sy);
+}
+
+// Natural derivatives at a point of a field given at corners 1..7 (corner 0 at zero):
+// out[i][k] = d v_i / d xi_k (for the rest offsets this is the Jacobian).
+__host__ __device__ inline void natural(const double* v, double xi, double eta, double zeta,
+                                        double out[3][3]) {
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) out[i][k] = 0.0;
+    for (int a = 1; a < 8; ++a) {
+        double n[3];
+        dshape(a, xi, eta, zeta, n);
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) out[i][k] += n[k] * v[3 * (a - 1) + i];
+    }
+}
+
+__host__ __device__ inline double inverse3(const double m[3][3], double r[3][3]) {
+    const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+    const double id = 1.0 / det;
+    r[0][0] = c00 * id;
+    r[1][0] = c01 * id;
+    r[2][0] = c02 * id;
+    r[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+    r[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+    r[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+    r[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+    r[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+    r[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+    return det;
+}
+
+// Cartesian Voigt strain (engineering shears) of a covariant Voigt strain: A^T E A.
+__host__ __device__ inline void push(const double A[3][3], const double e[6], double out[6]) {
+    const double E[3][3] = {{e[0], 0.5 * e[3], 0.5 * e[5]},
+                            {0.5 * e[3], e[1], 0.5 * e[4]},
+                            {0.5 * e[5], 0.5 * e[4], e[2]}};
+    double EA[3][3], M[3][3];
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) EA[k][j] = E[k][0] * A[0][j] + E[k][1] * A[1][j] + E[k][2] * A[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[i][j] = A[0][i] * EA[0][j] + A[1][i] * EA[1][j] + A[2][i] * EA[2][j];
+    out[0] = M[0][0];
+    out[1] = M[1][1];
+    out[2] = M[2][2];
+    out[3] = 2.0 * M[0][1];
+    out[4] = 2.0 * M[1][2];
+    out[5] = 2.0 * M[2][0];
+}
+
+// The work-conjugate of push: covariant Voigt stress tau = T^T sigma, Q = A S A^T.
+__host__ __device__ inline void pull(const double A[3][3], const double s[6], double out[6]) {
+    const double S[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+    double SA[3][3], Q[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int l = 0; l < 3; ++l) SA[i][l] = S[i][0] * A[l][0] + S[i][1] * A[l][1] + S[i][2] * A[l][2];
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) Q[k][l] = A[k][0] * SA[0][l] + A[k][1] * SA[1][l] + A[k][2] * SA[2][l];
+    out[0] = Q[0][0];
+    out[1] = Q[1][1];
+    out[2] = Q[2][2];
+    out[3] = Q[0][1];
+    out[4] = Q[1][2];
+    out[5] = Q[2][0];
+}
+
+__host__ __device__ inline double gauss(int g, int axis) {
+    const double q = 0.57735026918962576451;
+    return (g >> axis) & 1 ? q : -q;
+}
+
+// the enhanced mode k's natural-coordinate monomial and its covariant row
+__host__ __device__ inline double monomial(int k, double xi, double eta, double zeta) {
+    return k == 0 ? xi : k == 1 ? eta : k == 2 ? zeta : k == 3 ? xi * zeta : k == 4 ? eta * zeta
+                                                              : k == 5 ? xi : eta;
+}
+__host__ __device__ inline int modeRow(int k) { return k < 2 ? k : k < 5 ? 2 : 3; }
+
+// Assumed-strain samples of one zeta level: 2E23 at (-1, 0), (+1, 0); 2E31 at (0, -1),
+// (0, +1); E33 at the corners (-1,-1), (1,-1), (1,1), (-1,1).
+constexpr double kShear[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+constexpr double kCorner[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+
+// The covariant strain of the displacement d at Gauss point g (rows 0, 1, 3 at the point,
+// rows 2, 4, 5 from the samples of its level).
+__host__ __device__ inline void covariantStrain(const double* X, const double* d, int g,
+                                                const double smp[8], double e[6]) {
+    const double xi = gauss(g, 0), eta = gauss(g, 1), zeta = gauss(g, 2);
+    double J[3][3], U[3][3];
+    natural(X, xi, eta, zeta, J);
+    natural(d, xi, eta, zeta, U);
+    e[0] = J[0][0] * U[0][0] + J[1][0] * U[1][0] + J[2][0] * U[2][0];
+    e[1] = J[0][1] * U[0][1] + J[1][1] * U[1][1] + J[2][1] * U[2][1];
+    e[3] = J[0][0] * U[0][1] + J[1][0] * U[1][1] + J[2][0] * U[2][1] +
+           J[0][1] * U[0][0] + J[1][1] * U[1][0] + J[2][1] * U[2][0];
+    e[4] = 0.5 * (1 - xi) * smp[0] + 0.5 * (1 + xi) * smp[1];
+    e[5] = 0.5 * (1 - eta) * smp[2] + 0.5 * (1 + eta) * smp[3];
+    double t = 0.0;
+    for (int s = 0; s < 4; ++s) t += 0.25 * (1 + xi * kCorner[s][0]) * (1 + eta * kCorner[s][1]) * smp[4 + s];
+    e[2] = t;
+}
+
+// The samples of one zeta level for a displacement d.
+__host__ __device__ inline void samples(const double* X, const double* d, double zeta, double smp[8]) {
+    for (int s = 0; s < 4; ++s) {
+        double J[3][3], U[3][3];
+        natural(X, kShear[s][0], kShear[s][1], zeta, J);
+        natural(d, kShear[s][0], kShear[s][1], zeta, U);
+        if (s < 2)  // 2E23 = g_1 . d,zeta + g_2 . d,eta
+            smp[s] = J[0][1] * U[0][2] + J[1][1] * U[1][2] + J[2][1] * U[2][2] +
+                     J[0][2] * U[0][1] + J[1][2] * U[1][1] + J[2][2] * U[2][1];
+        else  // 2E31 = g_2 . d,xi + g_0 . d,zeta
+            smp[s] = J[0][2] * U[0][0] + J[1][2] * U[1][0] + J[2][2] * U[2][0] +
+                     J[0][0] * U[0][2] + J[1][0] * U[1][2] + J[2][0] * U[2][2];
+        natural(X, kCorner[s][0], kCorner[s][1], zeta, J);
+        natural(d, kCorner[s][0], kCorner[s][1], zeta, U);
+        smp[4 + s] = J[0][2] * U[0][2] + J[1][2] * U[1][2] + J[2][2] * U[2][2];
+    }
+}
+
+// Adds c * (g_k dN_a,l + g_l dN_a,k) (or, for k == l, c * g_k dN_a,k) at a natural point to
+// the force of corners 1..7.
+__host__ __device__ inline void spread(const double* X, double xi, double eta, double zeta, int k,
+                                       int l, double c, double f[21]) {
+    double J[3][3];
+    natural(X, xi, eta, zeta, J);
+    for (int a = 1; a < 8; ++a) {
+        double n[3];
+        dshape(a, xi, eta, zeta, n);
+        for (int i = 0; i < 3; ++i) {
+            double v = J[i][k] * n[l];
+            if (k != l) v += J[i][l] * n[k];
+            f[3 * (a - 1) + i] += c * v;
+        }
+    }
+}
+
+// One point's return from its history h0 (plastic strain 6, eqps, damage) for strain e
+// (plasticity.cpp:319-505, linear hardening, no back stress). Returns bit 0 yielded, bit 1
+// failed now, bit 2 failed (no stress, no tangent).
+__host__ __device__ inline int pointReturn(const Material& m, const double e[6], const double h0[8],
+                                           double h[8], double sig[6], double& diss,
+                                           double& theta, double& thetabar, double n[6],
+                                           double& released) {
+    diss = 0.0;
+    released = 0.0;
+    theta = 1.0;
+    thetabar = 0.0;
+    for (int k = 0; k < 8; ++k) h[k] = h0[k];
+    for (int i = 0; i < 6; ++i) n[i] = 0.0;
+    if (h0[7] >= 1.0) {
+        for (int i = 0; i < 6; ++i) sig[i] = 0.0;
+        return 4;
+    }
+    const double mu = m.mu, kappa = m.kappa;
+    double el[6];
+    for (int i = 0; i < 6; ++i) el[i] = e[i] - h0[i];
+    const double vol = el[0] + el[1] + el[2];
+    const double p = kappa * vol;
+    double tr[6];
+    for (int i = 0; i < 3; ++i) {
+        tr[i] = 2.0 * mu * (el[i] - vol / 3.0);
+        tr[3 + i] = mu * el[3 + i];
+    }
+    const double nrm = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
+                            2.0 * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
+    const double root23 = 0.81649658092772603273;
+    const double radius = root23 * (m.yield + m.hard * fmax(h0[6], 0.0));
+    const double excess = nrm - radius;
+    if (!(excess > 0.0)) {
+        for (int i = 0; i < 3; ++i) {
+            sig[i] = p + tr[i];
+            sig[3 + i] = tr[3 + i];
+        }
+        return 0;
+    }
+    const double gamma = excess / (2.0 * mu + (2.0 / 3.0) * m.hard);
+    double dep[6];
+    for (int i = 0; i < 6; ++i) n[i] = tr[i] / nrm;
+    for (int i = 0; i < 3; ++i) {
+        dep[i] = gamma * n[i];
+        dep[3 + i] = 2.0 * gamma * n[3 + i];
+        sig[i] = p + tr[i] - 2.0 * mu * gamma * n[i];
+        sig[3 + i] = tr[3 + i] - 2.0 * mu * gamma * n[3 + i];
+    }
+    double dd = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        dd += sig[i] * dep[i];
+        h[i] = h0[i] + dep[i];
+    }
+    diss = dd;
+    const double deq = root23 * gamma;
+    h[6] = h0[6] + deq;
+    theta = 1.0 - 2.0 * mu * gamma / nrm;
+    thetabar = 1.0 / (1.0 + m.hard / (3.0 * mu)) - (1.0 - theta);
+    if (m.damage && m.fail > 0.0) {
+        const double q = 1.22474487139158904909 * (nrm - 2.0 * mu * gamma);
+        const double eta = q > 0.0 ? p / q : 0.0;
+        if (eta > m.ecut) {
+            const double critical = m.fail * exp(-m.sens * (eta - m.eref));
+            if (critical > 0.0 && critical <= 1.7976931348623157e308)  // positive and finite
+                h[7] = h0[7] + deq / critical;
+        }
+        if (h[7] >= 1.0) {
+            h[7] = 1.0;
+            const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
+            double u = 0.0;
+            for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * mu) + pm / (3.0 * kappa));
+            for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / mu);
+            released = u;
+            for (int i = 0; i < 6; ++i) sig[i] = 0.0;
+            return 1 | 2 | 4;
+        }
+    }
+    return 1;
+}
+
+// The element update. X: rest offsets of corners 1..7 from corner 0; d: co-rotated corner
+// offsets; hist: 8 points x (plastic strain 6, eqps, damage) at hist[(8 g + k) * stride];
+// enh: 7 enhanced parameters at enh[k * stride]. Writes the internal force of corners
+// 1..7 (corner 0 carries minus their sum) in the co-rotated frame. A torn element (all
+// points failed) is left as it is and returns a zero force.
+__host__ __device__ inline Result update(const Material& m, const double* X, const double* d,
+                                         double* hist, double* enh, uint32_t stride,
+                                         bool wantStrain, double f[21]) {
+    Result res;
+    for (int j = 0; j < 21; ++j) f[j] = 0.0;
+    for (int g = 0; g < 8; ++g) res.failedStart += hist[(8 * g + 7) * stride] >= 1.0;
+    if (m.damage && res.failedStart == 8) {
+        res.failedEnd = 8;
+        return res;
+    }
+    // the centre's map T0 (columns 0-3), the points' volumes and det J0/det J, the modes'
+    // normalisations, and the base strain of every point
+    double J0[3][3], A0[3][3];
+    natural(X, 0.0, 0.0, 0.0, J0);
+    const double det0 = inverse3(J0, A0);
+    double t0[4][6];
+    for (int r = 0; r < 4; ++r) {
+        double unit[6] = {0, 0, 0, 0, 0, 0};
+        unit[r] = 1.0;
+        push(A0, unit, t0[r]);
+    }
+    double w[8], sc[8], eu[8][6];
+    {
+        double smp[2][8];
+        samples(X, d, -0.57735026918962576451, smp[0]);
+        samples(X, d, 0.57735026918962576451, smp[1]);
+        for (int g = 0; g < 8; ++g) {
+            double J[3][3], A[3][3], ec[6];
+            natural(X, gauss(g, 0), gauss(g, 1), gauss(g, 2), J);
+            w[g] = inverse3(J, A);
+            sc[g] = det0 / w[g];
+            covariantStrain(X, d, g, smp[(g >> 2) & 1], ec);
+            push(A, ec, eu[g]);
+        }
+    }
+    double inv[7];
+    for (int k = 0; k < 7; ++k) {
+        const int r = modeRow(k);
+        double c2 = 0.0, s = 0.0;
+        for (int i = 0; i < 6; ++i) c2 += t0[r][i] * t0[r][i];
+        for (int g = 0; g < 8; ++g) {
+            const double mk = monomial(k, gauss(g, 0), gauss(g, 1), gauss(g, 2));
+            s += w[g] * sc[g] * sc[g] * mk * mk;
+        }
+        inv[k] = c2 * s > 0.0 ? 1.0 / sqrt(c2 * s) : 1.0;
+    }
+    // column k of G at point g
+    auto G = [&](int g, int k, double v[6]) {
+        const double c = sc[g] * monomial(k, gauss(g, 0), gauss(g, 1), gauss(g, 2)) * inv[k];
+        for (int i = 0; i < 6; ++i) v[i] = c * t0[modeRow(k)][i];
+    };
+    double vol = 0.0;
+    for (int g = 0; g < 8; ++g) vol += w[g];
+    const double scale = m.yield * vol;
+    int eas = res.failedStart > 0 ? 0 : 7;
+    double alpha[7];
+    for (int k = 0; k < 7; ++k) alpha[k] = eas ? enh[k * stride] : 0.0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        bool died = false;
+        for (int iter = 0;; ++iter) {
+            double r[7] = {0, 0, 0, 0, 0, 0, 0}, kaa[28];
+            for (int k = 0; k < 28; ++k) kaa[k] = 0.0;
+            died = false;
+            for (int g = 0; g < 8; ++g) {
+                double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel, v[7][6];
+                for (int k = 0; k < 7; ++k) G(g, k, v[k]);
+                for (int i = 0; i < 6; ++i) {
+                    double t = eu[g][i];
+                    for (int k = 0; k < eas; ++k) t += v[k][i] * alpha[k];
+                    st[i] = t;
+                }
+                for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
+                const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
+                died = died || (fl & 2);
+                if (!eas) continue;
+                double cv[7][6];
+                for (int k = 0; k < 7; ++k) {
+                    if (fl & 4) {
+                        for (int i = 0; i < 6; ++i) cv[k][i] = 0.0;
+                        continue;
+                    }
+                    const double tr = v[k][0] + v[k][1] + v[k][2];
+                    double nv = 0.0;
+                    for (int i = 0; i < 6; ++i) nv += n6[i] * v[k][i];
+                    for (int i = 0; i < 3; ++i)
+                        cv[k][i] = m.kappa * tr + 2.0 * m.mu * th * (v[k][i] - tr / 3.0) -
+                                   2.0 * m.mu * thb * n6[i] * nv;
+                    for (int i = 3; i < 6; ++i) cv[k][i] = m.mu * th * v[k][i] - 2.0 * m.mu * thb * n6[i] * nv;
+                }
+                for (int k = 0; k < 7; ++k) {
+                    double t = 0.0;
+                    for (int i = 0; i < 6; ++i) t += v[k][i] * sig[i];
+                    r[k] += w[g] * t;
+                }
+                for (int p = 0; p < 7; ++p)
+                    for (int q = 0; q <= p; ++q) {
+                        double t = 0.0;
+                        for (int i = 0; i < 6; ++i) t += v[p][i] * cv[q][i];
+                        kaa[p * (p + 1) / 2 + q] += w[g] * t;
+                    }
+            }
+            if (!eas) break;
+            double nrm = 0.0;
+            for (int k = 0; k < 7; ++k) nrm += r[k] * r[k];
+            if (sqrt(nrm) <= 1e-12 * scale) break;
+            if (iter + 1 >= 40) break;
+            double L[28];
+            bool ok = true;
+            for (int i = 0; i < 7; ++i)
+                for (int j = 0; j <= i; ++j) {
+                    double t = kaa[i * (i + 1) / 2 + j];
+                    for (int k = 0; k < j; ++k) t -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+                    if (i == j) {
+                        ok = ok && t > 0.0;
+                        L[i * (i + 1) / 2 + i] = sqrt(fmax(t, 1e-300));
+                    } else {
+                        L[i * (i + 1) / 2 + j] = t / L[j * (j + 1) / 2 + j];
+                    }
+                }
+            if (!ok) break;
+            double del[7];
+            for (int i = 0; i < 7; ++i) {
+                double t = -r[i];
+                for (int k = 0; k < i; ++k) t -= L[i * (i + 1) / 2 + k] * del[k];
+                del[i] = t / L[i * (i + 1) / 2 + i];
+            }
+            for (int i = 6; i >= 0; --i) {
+                double t = del[i];
+                for (int k = i + 1; k < 7; ++k) t -= L[k * (k + 1) / 2 + i] * del[k];
+                del[i] = t / L[i * (i + 1) / 2 + i];
+            }
+            double work = 0.0;
+            for (int k = 0; k < 7; ++k) work += del[k] * r[k];
+            if (fabs(work) <= 1e-16 * scale) break;
+            for (int k = 0; k < 7; ++k) alpha[k] += del[k];
+        }
+        if (!(died && eas)) break;
+        eas = 0;
+        for (int k = 0; k < 7; ++k) alpha[k] = 0.0;
+    }
+    // commit at the final alpha; the force through the same rows, the assumed rows through
+    // their samples
+    double cs[2][8];
+    for (int l = 0; l < 2; ++l)
+        for (int s = 0; s < 8; ++s) cs[l][s] = 0.0;
+    for (int g = 0; g < 8; ++g) {
+        double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel, v[6];
+        for (int i = 0; i < 6; ++i) st[i] = eu[g][i];
+        for (int k = 0; k < eas; ++k) {
+            G(g, k, v);
+            for (int i = 0; i < 6; ++i) st[i] += v[i] * alpha[k];
+        }
+        for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
+        const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
+        for (int k = 0; k < 8; ++k) hist[(8 * g + k) * stride] = h[k];
+        res.failedEnd += (fl & 4) != 0;
+        res.dissipation += w[g] * dg;
+        res.released += w[g] * rel;
+        if (wantStrain) {
+            const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
+            double u = 0.0;
+            for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * m.mu) + pm / (3.0 * m.kappa));
+            for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / m.mu);
+            res.strain += w[g] * u;
+        }
+        if (fl & 4) continue;  // a failed point carries no stress
+        const double xi = gauss(g, 0), eta = gauss(g, 1), zeta = gauss(g, 2);
+        double J[3][3], A[3][3], tau[6];
+        natural(X, xi, eta, zeta, J);
+        inverse3(J, A);
+        pull(A, sig, tau);
+        spread(X, xi, eta, zeta, 0, 0, w[g] * tau[0], f);
+        spread(X, xi, eta, zeta, 1, 1, w[g] * tau[1], f);
+        spread(X, xi, eta, zeta, 0, 1, w[g] * tau[3], f);
+        const int lv = (g >> 2) & 1;
+        cs[lv][0] += w[g] * tau[4] * 0.5 * (1 - xi);
+        cs[lv][1] += w[g] * tau[4] * 0.5 * (1 + xi);
+        cs[lv][2] += w[g] * tau[5] * 0.5 * (1 - eta);
+        cs[lv][3] += w[g] * tau[5] * 0.5 * (1 + eta);
+        for (int s = 0; s < 4; ++s)
+            cs[lv][4 + s] += w[g] * tau[2] * 0.25 * (1 + xi * kCorner[s][0]) * (1 + eta * kCorner[s][1]);
+    }
+    for (int lv = 0; lv < 2; ++lv) {
+        const double zeta = lv ? 0.57735026918962576451 : -0.57735026918962576451;
+        for (int s = 0; s < 4; ++s) {
+            if (s < 2)
+                spread(X, kShear[s][0], kShear[s][1], zeta, 1, 2, cs[lv][s], f);
+            else
+                spread(X, kShear[s][0], kShear[s][1], zeta, 2, 0, cs[lv][s], f);
+            spread(X, kCorner[s][0], kCorner[s][1], zeta, 2, 2, cs[lv][4 + s], f);
+        }
+    }
+    for (int k = 0; k < 7; ++k) enh[k * stride] = alpha[k];
+    return res;
+}
+
+}  // namespace shipfem::hexp

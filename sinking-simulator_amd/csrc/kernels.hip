@@ -17,6 +17,7 @@
 // order in every lane) accumulated into per-wave slots, summed in a fixed order by
 // the reduce kernel on sample steps.
 #include "shipfem_dev.hpp"
+#include "hex_plastic.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -505,122 +506,14 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex(const CaseDev* __restrict__ 
 }
 
 // ---------------------------------------------------------------- element pass, plastic hex
-// One lane per solid-shell hex, FP64: elementPlasticUpdate (solid_shell.cpp:826-1029) on
-// rest forms stored per element. The co-rotated corner offsets d (as the elastic pass
-// forms them) give each Gauss point's base strain B d, kept in registers. A Newton solve
-// on the seven enhanced parameters alpha follows: each iterate returns all eight points
-// from their history at the start of the step, accumulates r = sum w G^T sigma and
-// Kaa = sum w G^T C G (C the points' algorithmic tangents), stops on |r| <= 1e-12 sigma_y V
-// or on a correction doing <= 1e-16 sigma_y V of work, else solves Kaa delta = -r by
-// Cholesky. A point that fails inside the step makes the element redo the step without
-// enhanced modes, and so does any failed point thereafter. A last pass at the final alpha
-// commits the points' history, and returns the force -R sum w B^T sigma, the dissipation,
-// the stored energy that failing points take with them, and on sample steps the stored
-// energy sum w 1/2 sigma : C^-1 : sigma.
+// One lane per solid-shell hex, FP64: elementPlasticUpdate (solid_shell.cpp:826-1029) with
+// the strain operator formed from the element's rest offsets (hex_plastic.hpp). The lane
+// forms the co-rotated corner offsets d in FP64 from the compensated displacement, runs the
+// update on the element's point history in place, and writes -R times the internal force.
+// The elastic pressure kappa tr(eps) of a thin element is where FP32 offsets would show: a
+// 12 mm element displaced 7.5 mm carries 4e-8 of strain in them, 7 kPa at kappa = 172 GPa.
 
-// The enhanced mode k of Gauss point g: column r_k of the centre's map times the point's
-// natural-coordinate monomial.
-__device__ __forceinline__ double eas_monomial(int k, double xi, double eta, double zeta) {
-    return k == 0 ? xi : k == 1 ? eta : k == 2 ? zeta : k == 3 ? xi * zeta : k == 4 ? eta * zeta
-                                                                : k == 5 ? xi : eta;
-}
-__device__ __forceinline__ int eas_row(int k) { return k < 2 ? k : k < 5 ? 2 : 3; }
-
-// Return of one point from its history h0 (plastic strain 6, eqps, damage) for total strain
-// e (plasticity.cpp:319-505, linear hardening, no back stress): stress sig, history h after
-// the step, dissipation density, the tangent's theta, thetaBar and flow direction n, and
-// the stored energy density of the returned stress when the point fails now. Returns bit 0
-// yielded, bit 1 failed now, bit 2 failed (before or now: no stress, no tangent).
-__device__ int hex_point_return(const CaseDev& c, double mu, double kappa, const double e[6],
-                                const double h0[8], double h[8], double sig[6], double& diss,
-                                double& theta, double& thetabar, double n[6], double& released) {
-    diss = 0.0;
-    released = 0.0;
-    theta = 1.0;
-    thetabar = 0.0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) h[k] = h0[k];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) n[i] = 0.0;
-    if (h0[7] >= 1.0) {
-#pragma unroll
-        for (int i = 0; i < 6; ++i) sig[i] = 0.0;
-        return 4;
-    }
-    double el[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) el[i] = e[i] - h0[i];
-    const double vol = el[0] + el[1] + el[2];
-    const double p = kappa * vol;
-    double tr[6];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        tr[i] = 2.0 * mu * (el[i] - vol / 3.0);
-        tr[3 + i] = mu * el[3 + i];
-    }
-    const double nrm = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
-                            2.0 * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
-    const double root23 = 0.81649658092772603273;
-    const double radius = root23 * (c.yield_d + c.hard_d * fmax(h0[6], 0.0));
-    const double excess = nrm - radius;
-    if (!(excess > 0.0)) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            sig[i] = p + tr[i];
-            sig[3 + i] = tr[3 + i];
-        }
-        return 0;
-    }
-    const double gamma = excess / (2.0 * mu + (2.0 / 3.0) * c.hard_d);
-    double dep[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) n[i] = tr[i] / nrm;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        dep[i] = gamma * n[i];
-        dep[3 + i] = 2.0 * gamma * n[3 + i];
-        sig[i] = p + tr[i] - 2.0 * mu * gamma * n[i];
-        sig[3 + i] = tr[3 + i] - 2.0 * mu * gamma * n[3 + i];
-    }
-    double d = 0.0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        d += sig[i] * dep[i];
-        h[i] = h0[i] + dep[i];
-    }
-    diss = d;
-    const double deq = root23 * gamma;
-    h[6] = h0[6] + deq;
-    theta = 1.0 - 2.0 * mu * gamma / nrm;
-    thetabar = 1.0 / (1.0 + c.hard_d / (3.0 * mu)) - (1.0 - theta);
-    if ((c.features & kFeatDamage) && c.fail_d > 0.0) {
-        // Rice-Tracey: damage grows by d eqps / (eps_f exp(-s (eta - eta_ref))), none at or
-        // below the cutoff triaxiality (plasticity.cpp:457-477)
-        const double q = 1.22474487139158904909 * (nrm - 2.0 * mu * gamma);
-        const double eta = q > 0.0 ? p / q : 0.0;
-        if (eta > c.ecut_d) {
-            const double critical = c.fail_d * exp(-c.sens_d * (eta - c.eref_d));
-            if (critical > 0.0 && isfinite(critical)) h[7] = h0[7] + deq / critical;
-        }
-        if (h[7] >= 1.0) {
-            h[7] = 1.0;
-            const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
-            double u = 0.0;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * mu) + pm / (3.0 * kappa));
-#pragma unroll
-            for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / mu);
-            released = u;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) sig[i] = 0.0;
-            return 1 | 2 | 4;
-        }
-    }
-    return 1;
-}
-
-// polar_perturbation in FP64 (the plastic pass: its stresses carry the elastic pressure
-// kappa tr(eps) of thin elements, where FP32 offsets would show)
+// polar_perturbation in FP64
 __device__ __forceinline__ void polar_perturbation_d(double W[9]) {
 #pragma unroll
     for (int it = 0; it < 6; ++it) {
@@ -662,8 +555,6 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
     double diss = 0.0, se = 0.0, fract = 0.0;
     if (e < c.T) {
         const uint32_t tp = c.Tpad, np = c.Npad;
-        const double mu = c.mu_d, kappa = c.lam_d + (2.0 / 3.0) * c.mu_d;
-        const double q = 0.57735026918962576451;
         const bool damage = (c.features & kFeatDamage) != 0;
         float4* out = reinterpret_cast<float4*>(c.force) + (size_t)e * 6;
         int failedStart = 0;
@@ -674,8 +565,6 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 #pragma unroll
             for (int k = 0; k < 6; ++k) out[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         } else {
-            // co-rotated corner offsets as the elastic pass forms them, here in FP64 from the
-            // displacement and its compensation
             const uint4 q0 = c.conn[2 * e], q1 = c.conn[2 * e + 1];
             const uint32_t gi[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
             double ux[8][3];
@@ -686,11 +575,14 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
                 ux[a][1] = (double)pv.y - (double)c.cu[np + gi[a]];
                 ux[a][2] = (double)pv.z - (double)c.cu[2 * np + gi[a]];
             }
-            double r[7][3], W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            double r[7][3], W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, X[kHexDof];
 #pragma unroll
             for (int a = 0; a < 7; ++a)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) r[a][i] = ux[a + 1][i] - ux[0][i];
+                for (int i = 0; i < 3; ++i) {
+                    r[a][i] = ux[a + 1][i] - ux[0][i];
+                    X[3 * a + i] = c.hgeod[(kHexDof + 3 * a + i) * tp + e];
+                }
 #pragma unroll
             for (int a = 0; a < 7; ++a) {
                 const double b0[3] = {c.hgeod[(3 * a) * tp + e], c.hgeod[(3 * a + 1) * tp + e],
@@ -704,212 +596,30 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
             double d[kHexDof];
 #pragma unroll
             for (int a = 0; a < 7; ++a) {
-                double y[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) y[i] = c.hgeod[(kHexDof + 3 * a + i) * tp + e] + r[a][i];
+                const double y[3] = {X[3 * a] + r[a][0], X[3 * a + 1] + r[a][1], X[3 * a + 2] + r[a][2]};
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     d[3 * a + i] = r[a][i] + (W[0 * 3 + i] * y[0] + W[1 * 3 + i] * y[1] + W[2 * 3 + i] * y[2]);
             }
-            // base strain of every point
-            double eu[8][6];
-#pragma unroll
-            for (int g = 0; g < 8; ++g)
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int j = 0; j < kHexDof; ++j)
-                        t = fma(c.hB[(size_t)((g * 6 + i) * kHexDof + j) * tp + e], d[j], t);
-                    eu[g][i] = t;
-                }
-            const double* hG = c.hG + e;
-            double vol = 0.0;
-#pragma unroll
-            for (int g = 0; g < 8; ++g) vol += c.hw[g * tp + e];
-            const double scale = c.yield_d * vol;
-            int eas = failedStart > 0 ? 0 : 7;
-            double alpha[7];
-#pragma unroll
-            for (int k = 0; k < 7; ++k) alpha[k] = eas ? c.henh[k * tp + e] : 0.0;
-            // G of point g, row i, mode k: (det J0/det J)_g m_k(g) T0[i][r_k] / norm_k
-            auto Gx = [&](int g, int i, int k) {
-                const double xi = (g & 1) ? q : -q, eta = (g & 2) ? q : -q, zeta = (g & 4) ? q : -q;
-                return hG[(size_t)(24 + g) * tp] * eas_monomial(k, xi, eta, zeta) *
-                       hG[(size_t)(i * 4 + eas_row(k)) * tp] * hG[(size_t)(32 + k) * tp];
-            };
-            for (int attempt = 0; attempt < 2; ++attempt) {
-                bool died = false;
-                for (int iter = 0;; ++iter) {
-                    double res[7] = {0, 0, 0, 0, 0, 0, 0};
-                    double kaa[28];
-#pragma unroll
-                    for (int k = 0; k < 28; ++k) kaa[k] = 0.0;
-                    died = false;
-                    for (int g = 0; g < 8; ++g) {
-                        const double w = c.hw[g * tp + e];
-                        double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
-#pragma unroll
-                        for (int i = 0; i < 6; ++i) {
-                            double t = eu[g][i];
-                            if (eas)
-#pragma unroll
-                                for (int k = 0; k < 7; ++k) t += Gx(g, i, k) * alpha[k];
-                            st[i] = t;
-                        }
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) h0[k] = c.hpoint[(8 * g + k) * tp + e];
-                        const int fl = hex_point_return(c, mu, kappa, st, h0, h, sig, dg, th, thb, n6, rel);
-                        died |= (fl & 2) != 0;
-                        if (!eas) continue;
-                        // mode columns of this point and the tangent applied to them
-                        double v[7][6], cv[7][6];
-#pragma unroll
-                        for (int k = 0; k < 7; ++k) {
-#pragma unroll
-                            for (int i = 0; i < 6; ++i) v[k][i] = Gx(g, i, k);
-                            if (fl & 4) {
-#pragma unroll
-                                for (int i = 0; i < 6; ++i) cv[k][i] = 0.0;
-                                continue;
-                            }
-                            const double tr = v[k][0] + v[k][1] + v[k][2];
-                            double nv = 0.0;
-#pragma unroll
-                            for (int i = 0; i < 6; ++i) nv += n6[i] * v[k][i];
-#pragma unroll
-                            for (int i = 0; i < 3; ++i)
-                                cv[k][i] = kappa * tr + 2.0 * mu * th * (v[k][i] - tr / 3.0) -
-                                           2.0 * mu * thb * n6[i] * nv;
-#pragma unroll
-                            for (int i = 3; i < 6; ++i)
-                                cv[k][i] = mu * th * v[k][i] - 2.0 * mu * thb * n6[i] * nv;
-                        }
-#pragma unroll
-                        for (int k = 0; k < 7; ++k) {
-                            double t = 0.0;
-#pragma unroll
-                            for (int i = 0; i < 6; ++i) t += v[k][i] * sig[i];
-                            res[k] += w * t;
-                        }
-#pragma unroll
-                        for (int p = 0; p < 7; ++p)
-#pragma unroll
-                            for (int qq = 0; qq <= p; ++qq) {
-                                double t = 0.0;
-#pragma unroll
-                                for (int i = 0; i < 6; ++i) t += v[p][i] * cv[qq][i];
-                                kaa[p * (p + 1) / 2 + qq] += w * t;
-                            }
-                    }
-                    if (!eas) break;
-                    double nrm = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 7; ++k) nrm += res[k] * res[k];
-                    if (sqrt(nrm) <= 1e-12 * scale) break;
-                    if (iter + 1 >= 40) break;
-                    // Cholesky of Kaa, then the correction
-                    double L[28];
-                    bool ok = true;
-#pragma unroll
-                    for (int i = 0; i < 7; ++i)
-#pragma unroll
-                        for (int j = 0; j <= i; ++j) {
-                            double t = kaa[i * (i + 1) / 2 + j];
-#pragma unroll
-                            for (int k = 0; k < j; ++k) t -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
-                            if (i == j) {
-                                ok = ok && t > 0.0;
-                                L[i * (i + 1) / 2 + i] = sqrt(fmax(t, 1e-300));
-                            } else {
-                                L[i * (i + 1) / 2 + j] = t / L[j * (j + 1) / 2 + j];
-                            }
-                        }
-                    if (!ok) break;
-                    double del[7];
-#pragma unroll
-                    for (int i = 0; i < 7; ++i) {
-                        double t = -res[i];
-#pragma unroll
-                        for (int k = 0; k < i; ++k) t -= L[i * (i + 1) / 2 + k] * del[k];
-                        del[i] = t / L[i * (i + 1) / 2 + i];
-                    }
-#pragma unroll
-                    for (int i = 6; i >= 0; --i) {
-                        double t = del[i];
-#pragma unroll
-                        for (int k = i + 1; k < 7; ++k) t -= L[k * (k + 1) / 2 + i] * del[k];
-                        del[i] = t / L[i * (i + 1) / 2 + i];
-                    }
-                    double work = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 7; ++k) work += del[k] * res[k];
-                    if (fabs(work) <= 1e-16 * scale) break;
-#pragma unroll
-                    for (int k = 0; k < 7; ++k) alpha[k] += del[k];
-                }
-                if (!(died && eas)) break;
-                eas = 0;
-#pragma unroll
-                for (int k = 0; k < 7; ++k) alpha[k] = 0.0;
-            }
-            // commit at the final alpha: history, dissipation, released energy, stored
-            // energy, and the nodal forces
-            double inter[kHexDof];
-#pragma unroll
-            for (int j = 0; j < kHexDof; ++j) inter[j] = 0.0;
-            int failedEnd = 0;
-            for (int g = 0; g < 8; ++g) {
-                const double w = c.hw[g * tp + e];
-                double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    double t = eu[g][i];
-                    if (eas)
-#pragma unroll
-                        for (int k = 0; k < 7; ++k) t += Gx(g, i, k) * alpha[k];
-                    st[i] = t;
-                }
-#pragma unroll
-                for (int k = 0; k < 8; ++k) h0[k] = c.hpoint[(8 * g + k) * tp + e];
-                const int fl = hex_point_return(c, mu, kappa, st, h0, h, sig, dg, th, thb, n6, rel);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) c.hpoint[(8 * g + k) * tp + e] = h[k];
-                failedEnd += (fl & 4) != 0;
-                diss += w * dg;
-                fract += w * rel;
-                if (sample) {
-                    const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
-                    double u = 0.0;
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-                        u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * mu) + pm / (3.0 * kappa));
-#pragma unroll
-                    for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / mu);
-                    se += w * u;
-                }
-#pragma unroll
-                for (int j = 0; j < kHexDof; ++j) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int i = 0; i < 6; ++i)
-                        t = fma(c.hB[(size_t)((g * 6 + i) * kHexDof + j) * tp + e], sig[i], t);
-                    inter[j] += w * t;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 7; ++k) c.henh[k * tp + e] = alpha[k];
-            if (damage && failedEnd == 8 && failedStart < 8) c.tear_step[e] = (int32_t)(c.step[parity] + 1);
+            const hexp::Material m{c.mu_d, c.lam_d + (2.0 / 3.0) * c.mu_d, c.yield_d, c.hard_d,
+                                   c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage};
+            double fi[kHexDof];
+            const hexp::Result res = hexp::update(m, X, d, c.hpoint + e, c.henh + e, tp, sample != 0, fi);
+            diss = res.dissipation;
+            fract = res.released;
+            se = res.strain;
+            if (damage && res.failedEnd == 8 && res.failedStart < 8)
+                c.tear_step[e] = (int32_t)(c.step[parity] + 1);
             double k0[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int a = 0; a < 7; ++a)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) k0[i] -= inter[3 * a + i];
+                for (int i = 0; i < 3; ++i) k0[i] -= fi[3 * a + i];
             float f[24];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
-                const double v[3] = {a == 0 ? k0[0] : inter[3 * a - 3], a == 0 ? k0[1] : inter[3 * a - 2],
-                                     a == 0 ? k0[2] : inter[3 * a - 1]};
+                const double v[3] = {a == 0 ? k0[0] : fi[3 * a - 3], a == 0 ? k0[1] : fi[3 * a - 2],
+                                     a == 0 ? k0[2] : fi[3 * a - 1]};
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     f[3 * a + i] = (float)-(v[i] + (W[i * 3 + 0] * v[0] + W[i * 3 + 1] * v[1] + W[i * 3 + 2] * v[2]));

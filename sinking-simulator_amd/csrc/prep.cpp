@@ -34,7 +34,7 @@ struct Layout {
 
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
-    uint64_t conn, dmi, vol, force, hk, hgeo, hB, hG, hw, hgeod, hpoint, henh, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
+    uint64_t conn, dmi, vol, force, hk, hgeo, hgeod, hpoint, henh, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
         ell_off, ell, lconn, tn_off, tnode, lofs, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
@@ -49,9 +49,6 @@ ArenaMap mapArena(const HostCase& hc) {
     m.force = L.take(12ull * hc.corners * T);
     m.hk = L.take(8ull * hc.hk.size());
     m.hgeo = L.take(4ull * hc.hgeo.size());
-    m.hB = L.take(8ull * hc.hB.size());
-    m.hG = L.take(8ull * hc.hG.size());
-    m.hw = L.take(8ull * hc.hw.size());
     m.hgeod = L.take(8ull * hc.hgeod.size());
     const bool hexPlastic = hc.corners == 8 && (hc.features & kFeatPlastic);
     m.hpoint = L.take(hexPlastic ? 8ull * kHexPoint * Tp : 0);
@@ -606,9 +603,6 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.hk = hc.hk.empty() ? nullptr : reinterpret_cast<const double*>(base + m.hk);
     r.hgeo = hc.hgeo.empty() ? nullptr : reinterpret_cast<const float*>(base + m.hgeo);
     const bool hexPlastic = hc.corners == 8 && (hc.features & kFeatPlastic);
-    r.hB = hexPlastic ? reinterpret_cast<const double*>(base + m.hB) : nullptr;
-    r.hG = hexPlastic ? reinterpret_cast<const double*>(base + m.hG) : nullptr;
-    r.hw = hexPlastic ? reinterpret_cast<const double*>(base + m.hw) : nullptr;
     r.hgeod = hexPlastic ? reinterpret_cast<const double*>(base + m.hgeod) : nullptr;
     r.hpoint = hexPlastic ? reinterpret_cast<double*>(base + m.hpoint) : nullptr;
     r.henh = hexPlastic ? reinterpret_cast<double*>(base + m.henh) : nullptr;
@@ -685,9 +679,6 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.vol, hc.vol.data(), 4ull * hc.vol.size());
     up(m.hk, hc.hk.data(), 8ull * hc.hk.size());
     up(m.hgeo, hc.hgeo.data(), 4ull * hc.hgeo.size());
-    up(m.hB, hc.hB.data(), 8ull * hc.hB.size());
-    up(m.hG, hc.hG.data(), 8ull * hc.hG.size());
-    up(m.hw, hc.hw.data(), 8ull * hc.hw.size());
     up(m.hgeod, hc.hgeod.data(), 8ull * hc.hgeod.size());
     up(m.u4, hc.u4.data(), 4ull * hc.u4.size());
     up(m.cu, hc.cu.data(), 4ull * hc.cu.size());

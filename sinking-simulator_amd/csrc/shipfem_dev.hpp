@@ -52,13 +52,9 @@ struct EnergyRecordDev {
 constexpr int kHexDof = 21;
 constexpr int kHexPacked = kHexDof * (kHexDof + 1) / 2;  // 231
 constexpr int kHexGeo = 42;  // centre gradients of corners 1..7, then X_a - X_0
-// Plastic solid-shell hex: per Gauss point g the strain rows for corners 1..7
-// (B[g][i][j], i < 6, j < 21), G in compact form (the centre's map columns 0-3 [6][4], the
-// points' det J0/det J [8], the modes' normalisations [7]), and per element 8 points x
-// (plastic strain 6, eqps, damage) plus the 7 enhanced parameters.
-constexpr int kHexB = 8 * 6 * kHexDof;  // 1008
-constexpr int kHexG = 24 + 8 + 7;       // 39
-constexpr int kHexPoint = 8 * 8;        // 64
+// Plastic solid-shell hex: per element 8 points x (plastic strain 6, eqps, damage) plus the 7
+// enhanced parameters.
+constexpr int kHexPoint = 8 * 8;  // 64
 
 struct CaseDev {
     uint32_t N, T;            // nodes, elements (tets, or hexes when corners == 8)
@@ -75,9 +71,6 @@ struct CaseDev {
     const double* hk;    // hex: [kHexPacked][Tpad] stiffness block of corners 1..7 (FP64)
     const float* hgeo;   // hex: [kHexGeo][Tpad] d N_a/dX at the centre (21: 3(a-1) + j),
                          // then rest X_a - X_0 (21: 21 + 3(a-1) + i), corners a = 1..7
-    const double* hB;    // plastic hex: [kHexB][Tpad], entry (g*6 + i)*21 + j
-    const double* hG;    // plastic hex: [kHexG][Tpad]
-    const double* hw;    // plastic hex: [8][Tpad] Gauss-point volumes
     const double* hgeod; // plastic hex: hgeo in FP64 [kHexGeo][Tpad]
     double* hpoint;      // plastic hex: [kHexPoint][Tpad], entry 8 g + k
     double* henh;        // plastic hex: [7][Tpad] enhanced parameters (warm start)

@@ -38,8 +38,7 @@ struct HostCase {
     uint32_t corners = 4;         // 4: tet, 8: solid-shell hex
     std::vector<uint32_t> conn;   // [corners T]
     std::vector<double> hk;       // hex: [kHexPacked][Tpad]
-    std::vector<double> hB, hG, hw;  // plastic hex: [kHexB], [kHexG], [8] x [Tpad]
-    std::vector<double> hgeod;       // plastic hex: [kHexGeo][Tpad]
+    std::vector<double> hgeod;    // plastic hex: [kHexGeo][Tpad]
     std::vector<float> hgeo;      // hex: [kHexGeo][Tpad]
     double hex_omega = 0;         // hex: largest element frequency (FP64, host)
     std::vector<float> dmi;       // [9][Tpad]
