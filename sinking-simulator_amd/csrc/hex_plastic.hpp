@@ -318,17 +318,24 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         }
         inv[k] = c2 * s > 0.0 ? 1.0 / sqrt(c2 * s) : 1.0;
     }
-    // column k of G at point g
-    auto G = [&](int g, int k, double v[6]) {
-        const double c = sc[g] * monomial(k, gauss(g, 0), gauss(g, 1), gauss(g, 2)) * inv[k];
-        for (int i = 0; i < 6; ++i) v[i] = c * t0[modeRow(k)][i];
+    // Column k of G at point g is c_k(g) T0[:, r_k], c_k(g) = (det J0/det J)_g m_k(g) / |mode k|:
+    // every column is a multiple of one of four vectors, so G alpha needs four sums and
+    // G^T C G needs only the 4 x 4 form T0[:, r]^T C T0[:, s] of each point.
+    int eas = res.failedStart > 0 ? 0 : 7;
+    double alpha[7];
+    for (int k = 0; k < 7; ++k) alpha[k] = eas ? enh[k * stride] : 0.0;
+    auto coef = [&](int g, int k) {
+        return sc[g] * monomial(k, gauss(g, 0), gauss(g, 1), gauss(g, 2)) * inv[k];
+    };
+    auto strainAt = [&](int g, int n, double st[6]) {
+        double a4[4] = {0, 0, 0, 0};
+        for (int k = 0; k < n; ++k) a4[modeRow(k)] += coef(g, k) * alpha[k];
+        for (int i = 0; i < 6; ++i)
+            st[i] = eu[g][i] + (a4[0] * t0[0][i] + a4[1] * t0[1][i] + a4[2] * t0[2][i] + a4[3] * t0[3][i]);
     };
     double vol = 0.0;
     for (int g = 0; g < 8; ++g) vol += w[g];
     const double scale = m.yield * vol;
-    int eas = res.failedStart > 0 ? 0 : 7;
-    double alpha[7];
-    for (int k = 0; k < 7; ++k) alpha[k] = eas ? enh[k * stride] : 0.0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         bool died = false;
         for (int iter = 0;; ++iter) {
@@ -336,41 +343,47 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
             for (int k = 0; k < 28; ++k) kaa[k] = 0.0;
             died = false;
             for (int g = 0; g < 8; ++g) {
-                double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel, v[7][6];
-                for (int k = 0; k < 7; ++k) G(g, k, v[k]);
-                for (int i = 0; i < 6; ++i) {
-                    double t = eu[g][i];
-                    for (int k = 0; k < eas; ++k) t += v[k][i] * alpha[k];
-                    st[i] = t;
-                }
+                double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
+                strainAt(g, eas, st);
                 for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
                 const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
                 died = died || (fl & 2);
                 if (!eas) continue;
-                double cv[7][6];
-                for (int k = 0; k < 7; ++k) {
-                    if (fl & 4) {
-                        for (int i = 0; i < 6; ++i) cv[k][i] = 0.0;
-                        continue;
-                    }
-                    const double tr = v[k][0] + v[k][1] + v[k][2];
-                    double nv = 0.0;
-                    for (int i = 0; i < 6; ++i) nv += n6[i] * v[k][i];
-                    for (int i = 0; i < 3; ++i)
-                        cv[k][i] = m.kappa * tr + 2.0 * m.mu * th * (v[k][i] - tr / 3.0) -
-                                   2.0 * m.mu * thb * n6[i] * nv;
-                    for (int i = 3; i < 6; ++i) cv[k][i] = m.mu * th * v[k][i] - 2.0 * m.mu * thb * n6[i] * nv;
-                }
-                for (int k = 0; k < 7; ++k) {
+                double ts[4], M[10];
+                for (int q = 0; q < 4; ++q) {
                     double t = 0.0;
-                    for (int i = 0; i < 6; ++i) t += v[k][i] * sig[i];
-                    r[k] += w[g] * t;
+                    for (int i = 0; i < 6; ++i) t += t0[q][i] * sig[i];
+                    ts[q] = t;
+                }
+                for (int q = 0; q < 4; ++q) {
+                    // C T0[:, q] with C = kappa 1(x)1 + 2 mu theta I_dev - 2 mu thetaBar n(x)n
+                    // (zero for a failed point)
+                    double cq[6] = {0, 0, 0, 0, 0, 0};
+                    if (!(fl & 4)) {
+                        const double* v = t0[q];
+                        const double tr = v[0] + v[1] + v[2];
+                        double nv = 0.0;
+                        for (int i = 0; i < 6; ++i) nv += n6[i] * v[i];
+                        for (int i = 0; i < 3; ++i)
+                            cq[i] = m.kappa * tr + 2.0 * m.mu * th * (v[i] - tr / 3.0) - 2.0 * m.mu * thb * n6[i] * nv;
+                        for (int i = 3; i < 6; ++i) cq[i] = m.mu * th * v[i] - 2.0 * m.mu * thb * n6[i] * nv;
+                    }
+                    for (int p = q; p < 4; ++p) {
+                        double t = 0.0;
+                        for (int i = 0; i < 6; ++i) t += t0[p][i] * cq[i];
+                        M[p * (p + 1) / 2 + q] = t;
+                    }
+                }
+                double cg[7];
+                for (int k = 0; k < 7; ++k) {
+                    cg[k] = coef(g, k);
+                    r[k] += w[g] * cg[k] * ts[modeRow(k)];
                 }
                 for (int p = 0; p < 7; ++p)
                     for (int q = 0; q <= p; ++q) {
-                        double t = 0.0;
-                        for (int i = 0; i < 6; ++i) t += v[p][i] * cv[q][i];
-                        kaa[p * (p + 1) / 2 + q] += w[g] * t;
+                        const int a = modeRow(p), b = modeRow(q);
+                        const double mpq = a >= b ? M[a * (a + 1) / 2 + b] : M[b * (b + 1) / 2 + a];
+                        kaa[p * (p + 1) / 2 + q] += w[g] * cg[p] * cg[q] * mpq;
                     }
             }
             if (!eas) break;
@@ -418,12 +431,8 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
     for (int l = 0; l < 2; ++l)
         for (int s = 0; s < 8; ++s) cs[l][s] = 0.0;
     for (int g = 0; g < 8; ++g) {
-        double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel, v[6];
-        for (int i = 0; i < 6; ++i) st[i] = eu[g][i];
-        for (int k = 0; k < eas; ++k) {
-            G(g, k, v);
-            for (int i = 0; i < 6; ++i) st[i] += v[i] * alpha[k];
-        }
+        double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
+        strainAt(g, eas, st);
         for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
         const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
         for (int k = 0; k < 8; ++k) hist[(8 * g + k) * stride] = h[k];
