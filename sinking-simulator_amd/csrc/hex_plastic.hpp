@@ -342,6 +342,7 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
             double r[7] = {0, 0, 0, 0, 0, 0, 0}, kaa[28];
             for (int k = 0; k < 28; ++k) kaa[k] = 0.0;
             died = false;
+#pragma unroll 1
             for (int g = 0; g < 8; ++g) {
                 double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
                 strainAt(g, eas, st);
@@ -430,6 +431,7 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
     double cs[2][8];
     for (int l = 0; l < 2; ++l)
         for (int s = 0; s < 8; ++s) cs[l][s] = 0.0;
+#pragma unroll 1
     for (int g = 0; g < 8; ++g) {
         double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
         strainAt(g, eas, st);
