@@ -298,3 +298,65 @@ def test_plastic_punch_with_tearing_matches_oracle():
     with pytest.raises(sf.ShipfemError) as e:
         s.export()
     assert e.value.status == sf.UNSUPPORTED
+
+
+def jittered(case, amp, seed, lx, ly):
+    # in-plane jitter of every node off the clamped edges (the same offset through the
+    # thickness keeps the element's thickness direction), plus a small warp out of plane:
+    # trapezoidal, warped, non-affine elements
+    rng = np.random.default_rng(seed)
+    X = case.X.copy()
+    nxy = np.unique(np.round(X[:, :2], 12), axis=0, return_inverse=True)[1].ravel()
+    off = rng.uniform(-amp, amp, (nxy.max() + 1, 3))
+    off[:, 2] *= 0.05
+    inner = (X[:, 0] > 1e-9) & (X[:, 0] < lx - 1e-9) & (X[:, 1] > 1e-9) & (X[:, 1] < ly - 1e-9)
+    X[inner] += off[nxy[inner]]
+    case.X = np.ascontiguousarray(X)
+    return case
+
+
+def test_jittered_two_layer_plate_matches_oracle():
+    # elastic: non-affine, warped elements, two through the thickness
+    lx, ly = 0.6, 0.3
+    case = configs.hex_plate(lx=lx, ly=ly, thickness=0.02, nx=24, ny=12, nz=2, pressure=2e5,
+                             ramp=1e-3, damping=0.9995)
+    case = jittered(case, 0.25 * lx / 24, 11, lx, ly)
+    s = sf.Solver(case)
+    o = oracle_for(case, s.dt)
+    ro = o.run(2000, sample_every=100)
+    s.run(2000)
+    err = rel_l2(s.readback().u, o.u)
+    gap = energy_gap(s.energy(), ro)
+    say(f"jittered two-layer plate, {case.T} elements, 2000 steps: u rel L2 {err:.3e}, gap {gap:.3e}")
+    assert err <= REL_U and gap <= ENERGY
+
+
+def test_jittered_plastic_zone_matches_oracle():
+    case = plastic_zone(5.0, failure=dict(failure_strain=0.08), nx=20, ny=10)
+    case = jittered(case, 0.2 * 0.3 / 20, 5, 0.3, 0.15)
+    s = sf.Solver(case)
+    o = oracle_for(case, s.dt)
+    o.run(2500, threads=8)
+    s.run(2500)
+    st = s.readback()
+    err = rel_l2(st.u, o.u)
+    deq = float(np.abs(st.eqps - o.hex_eqps).max())
+    torn_g = set(np.nonzero(st.damage >= 1)[0])
+    torn_o = set(np.nonzero(o.hex_damage >= 1)[0])
+    say(f"jittered plastic zone, 2500 steps: u rel L2 {err:.3e}, eqps |d| {deq:.2e} "
+        f"(max {o.hex_eqps.max():.3f}), torn {len(torn_g)} / {len(torn_o)}")
+    assert o.hex_eqps.max() > 0.01
+    assert err <= REL_U and deq <= ABS_EQPS and len(torn_g ^ torn_o) <= max(1, len(torn_o) // 50)
+
+
+def test_plastic_hex_batch_matches_each_case_alone():
+    cases = [plastic_zone(v, failure=dict(failure_strain=f), nx=n, ny=n // 2)
+             for v, f, n in ((4.0, 0.1, 16), (6.0, 0.06, 20), (8.0, 0.08, 12))]
+    b = sf.Batch(cases)
+    b.run(600)
+    for i, c in enumerate(cases):
+        s = sf.Solver(c)
+        s.run(600)
+        a, bb = s.readback(), b.readback(i)
+        assert np.array_equal(a.u, bb.u) and np.array_equal(a.eqps, bb.eqps), i
+        assert np.array_equal(a.damage, bb.damage), i
