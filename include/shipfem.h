@@ -407,15 +407,24 @@ shipfem_status shipfem_hex_pressure_load(const double* rest_position, uint32_t n
                                          const uint32_t* index, uint32_t element_count,
                                          double pressure, double* load);
 
-/* An explicit solve over solid-shell hexes: the elastic zone::Solver step
- * (engine/sim/zone.cpp:928-1012, 1086-1252 with params.plastic false) on the device.
- * The case is a shipfem_case_desc read with tet_count as the element count and index
- * as [8 * tet_count] node ids in the order above; material.plastic must be 0
- * (SHIPFEM_UNSUPPORTED otherwise). dt = 0 takes dt_safety * 2 / omega_max, omega_max the
- * largest element frequency (FP64, host). The returned handle takes every
- * shipfem_solver_* call; shipfem_solver_tet_forces returns [24 * tet_count] (the
- * element forces of the last element pass, 8 corners x xyz per element), and the
- * per-element plastic outputs read zero. */
+/* An explicit solve over solid-shell hexes: the zone::Solver step
+ * (engine/sim/zone.cpp:928-1070, 1086-1252) on the device. The case is a
+ * shipfem_case_desc read with tet_count as the element count and index as
+ * [8 * tet_count] node ids in the order above.
+ *   material.plastic 0: co-rotational elastic, the condensed stiffness formed at rest
+ *     (solidshell::internalForce, solid_shell.cpp:735-766).
+ *   material.plastic 1: elementPlasticUpdate (solid_shell.cpp:826-1029) in FP64 at eight
+ *     points per element: J2 with linear hardening, the enhanced-strain Newton, ductile
+ *     damage when failure_strain > 0 (taken as given for every element; the reference
+ *     regularises it per element size, see shipfem_regularised_failure_strain), and torn
+ *     elements (every point failed) carrying nothing.
+ * dt = 0 takes dt_safety * 2 / omega_max, omega_max the largest element frequency (FP64,
+ * host). The returned handle takes every shipfem_solver_* call;
+ * shipfem_solver_tet_forces returns [24 * tet_count] (8 corners x xyz per element). The
+ * readback's per-element eqps is the largest of the element's points, plastic_strain
+ * their mean and damage the smallest (1: torn); tears carry no plane (zeros).
+ * shipfem_solver_export / adopt refuse a plastic hex case (SHIPFEM_UNSUPPORTED: eight
+ * points of history per element); checkpoint and restore carry it. */
 shipfem_status shipfem_hex_solver_create(const shipfem_case_desc* desc, shipfem_solver** out,
                                          char* err, size_t err_len);
 
@@ -427,7 +436,7 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t cas
                                     const shipfem_launch_config* cfg, shipfem_batch** out,
                                     char* err, size_t err_len);
 /* shipfem_batch_create for solid-shell hex cases (each read as shipfem_hex_solver_create
- * reads its case); every batch call then applies. Elastic only (SHIPFEM_UNSUPPORTED). */
+ * reads its case); every batch call then applies. A batch is all elastic or all plastic. */
 shipfem_status shipfem_hex_batch_create(const shipfem_case_desc* cases, uint32_t case_count,
                                         const shipfem_launch_config* cfg, shipfem_batch** out,
                                         char* err, size_t err_len);
