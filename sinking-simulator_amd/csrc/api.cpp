@@ -471,6 +471,14 @@ shipfem_status shipfem_batch_create(const shipfem_case_desc* cases, uint32_t n,
 shipfem_status shipfem_hex_batch_create(const shipfem_case_desc* cases, uint32_t n,
                                         const shipfem_launch_config* cfg, shipfem_batch** out,
                                         char* err, size_t err_len) {
+    // one launch runs one element pass: the batch is all elastic or all plastic
+    for (uint32_t i = 1; cases && i < n; ++i)
+        if ((cases[i].material.plastic != 0) != (cases[0].material.plastic != 0)) {
+            setError(err, err_len, "case " + std::to_string(i) +
+                                       ": a hex batch is all elastic or all plastic");
+            if (out) *out = nullptr;
+            return SHIPFEM_INVALID_ARG;
+        }
     return createBatch(cases, n, cfg, true, out, err, err_len);
 }
 

@@ -302,3 +302,11 @@ def test_oracle_plastic_hex_punch_account_and_tearing():
     frozen = c.hex_point[torn].copy()
     c.run(50, threads=4)
     assert np.array_equal(c.hex_point[torn], frozen)
+
+
+def test_hex_batch_refuses_mixed_elastic_and_plastic_before_touching_a_device():
+    a = sf.HexCase.plate(0.2, 0.1, 0.01, 2, 2, 1)
+    b = sf.HexCase.plate(0.2, 0.1, 0.01, 2, 2, 1, plastic=True)
+    with pytest.raises(sf.ShipfemError) as e:
+        sf.Batch([a, b])
+    assert e.value.status == sf.INVALID_ARG
