@@ -43,6 +43,8 @@ hipError_t Plan::build(hipStream_t s) {
     for (const CaseDev& r : recs)
         if ((r.corners == 8) != hex || (hex && ((r.features ^ recs[0].features) & kFeatPlastic)))
             return hipErrorInvalidValue;
+    // the plastic hex pass keeps 48 doubles a lane in LDS: at most 256 lanes a block
+    if (hex && (recs[0].features & kFeatPlastic)) shape.elem_block = std::min(shape.elem_block, 256);
     tilesFor(recs, shape.variant == kTiled && !hex ? kTile : shape.elem_block, false, elem_tiles);
     lmax = 0;
     for (const CaseDev& r : recs) lmax = std::max(lmax, r.lmax);

@@ -274,7 +274,8 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
 // points failed) is left as it is and returns a zero force.
 __host__ __device__ inline Result update(const Material& m, const double* X, const double* d,
                                          double* hist, double* enh, uint32_t stride,
-                                         bool wantStrain, double f[21]) {
+                                         bool wantStrain, double f[21], double* eu,
+                                         uint32_t euStride) {
     Result res;
     for (int j = 0; j < 21; ++j) f[j] = 0.0;
     for (int g = 0; g < 8; ++g) res.failedStart += hist[(8 * g + 7) * stride] >= 1.0;
@@ -293,7 +294,9 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         unit[r] = 1.0;
         push(A0, unit, t0[r]);
     }
-    double w[8], sc[8], eu[8][6];
+    // eu: the base strain of point g, row i at eu[(6 g + i) * euStride] (the caller's
+    // buffer: on the device, this lane's column of a block-shared array)
+    double w[8], sc[8];
     {
         double smp[2][8];
         samples(X, d, -0.57735026918962576451, smp[0]);
@@ -304,7 +307,9 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
             w[g] = inverse3(J, A);
             sc[g] = det0 / w[g];
             covariantStrain(X, d, g, smp[(g >> 2) & 1], ec);
-            push(A, ec, eu[g]);
+            double e6[6];
+            push(A, ec, e6);
+            for (int i = 0; i < 6; ++i) eu[(6 * g + i) * euStride] = e6[i];
         }
     }
     double inv[7];
@@ -331,7 +336,8 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         double a4[4] = {0, 0, 0, 0};
         for (int k = 0; k < n; ++k) a4[modeRow(k)] += coef(g, k) * alpha[k];
         for (int i = 0; i < 6; ++i)
-            st[i] = eu[g][i] + (a4[0] * t0[0][i] + a4[1] * t0[1][i] + a4[2] * t0[2][i] + a4[3] * t0[3][i]);
+            st[i] = eu[(6 * g + i) * euStride] +
+                    (a4[0] * t0[0][i] + a4[1] * t0[1][i] + a4[2] * t0[2][i] + a4[3] * t0[3][i]);
     };
     double vol = 0.0;
     for (int g = 0; g < 8; ++g) vol += w[g];

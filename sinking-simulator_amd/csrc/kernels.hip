@@ -551,6 +551,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
     const CaseDev& c = cases[tile.case_index];
     if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
 
+    // each lane's base strains of its element's eight points, lane-interleaved
+    __shared__ double sh_eu[48 * BLOCK];
     const uint32_t e = tile.first + threadIdx.x;
     double diss = 0.0, se = 0.0, fract = 0.0;
     if (e < c.T) {
@@ -604,7 +606,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
             const hexp::Material m{c.mu_d, c.lam_d + (2.0 / 3.0) * c.mu_d, c.yield_d, c.hard_d,
                                    c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage};
             double fi[kHexDof];
-            const hexp::Result res = hexp::update(m, X, d, c.hpoint + e, c.henh + e, tp, sample != 0, fi);
+            const hexp::Result res = hexp::update(m, X, d, c.hpoint + e, c.henh + e, tp, sample != 0, fi,
+                                                  sh_eu + threadIdx.x, BLOCK);
             diss = res.dissipation;
             fract = res.released;
             se = res.strain;
@@ -1308,10 +1311,6 @@ void launch_elem_hex_plastic(const CaseDev* cases, const Tile* tiles, int ntiles
             break;
         case 128:
             SHIPFEM_LAUNCH(t, (k_elem_hex_plastic<128>), dim3(ntiles), dim3(128), 0, stream, cases, tiles,
-                           ntiles, parity, sample, xcd);
-            break;
-        case 512:
-            SHIPFEM_LAUNCH(t, (k_elem_hex_plastic<512>), dim3(ntiles), dim3(512), 0, stream, cases, tiles,
                            ntiles, parity, sample, xcd);
             break;
         default:

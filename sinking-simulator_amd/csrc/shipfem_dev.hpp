@@ -169,7 +169,8 @@ void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
 // The solid-shell hex element pass: one lane per element, `block` threads per block.
 void launch_elem_hex(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
                      int sample, int xcd, hipStream_t stream, const KernelTiming* timing = nullptr);
-// The plastic solid-shell hex element pass: one lane per element, `block` threads per block.
+// The plastic solid-shell hex element pass: one lane per element, `block` (64, 128 or 256;
+// larger is launched as 256) threads per block.
 void launch_elem_hex_plastic(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
                              int parity, int sample, int xcd, hipStream_t stream,
                              const KernelTiming* timing = nullptr);
