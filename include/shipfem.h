@@ -347,8 +347,8 @@ uint64_t shipfem_apply_edge_drive(uint32_t node_count, const double* displacemen
                                   double* prescribed);
 
 /* ---- case wire format ------------------------------------------------------- */
-/* One self-checking little-endian blob per case: the on-disk case file and the form a
- * case travels in to the server. A 64-byte header (magic "SHIPFEMC", version, node and
+/* One self-checking little-endian blob per tet case (4 node ids per element): the on-disk
+ * case file and the form a case travels in to the server. A 64-byte header (magic "SHIPFEMC", version, node and
  * tet counts, which optional arrays follow, payload size, FNV-1a 64 of the payload),
  * the scalar block, then rest positions, connectivity and the optional arrays of the
  * descriptor. No reference counterpart: the reference builds its cases in memory; a

@@ -459,6 +459,13 @@ class HexCase(Case):
 
     corners = 8
 
+    def encode(self):
+        # the wire format's index block is 4 ids per element
+        raise NotImplementedError("the case wire format carries tet cases")
+
+    def save(self, path):
+        raise NotImplementedError("the case wire format carries tet cases")
+
     @classmethod
     def plate(cls, lx, ly, thickness, nx, ny, nz=1, pressure=0.0, **kw):
         """makePlateMesh (solid_shell.cpp:1152-1185), with uniformPressureLoad of

@@ -310,3 +310,9 @@ def test_hex_batch_refuses_mixed_elastic_and_plastic_before_touching_a_device():
     with pytest.raises(sf.ShipfemError) as e:
         sf.Batch([a, b])
     assert e.value.status == sf.INVALID_ARG
+
+
+def test_hex_cases_do_not_pass_through_the_tet_wire_format():
+    c = sf.HexCase.plate(0.2, 0.1, 0.01, 2, 2, 1)
+    with pytest.raises(NotImplementedError):
+        c.encode()
