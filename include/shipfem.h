@@ -381,6 +381,11 @@ shipfem_status shipfem_hex_stiffness(const double nodes[24], double youngs_modul
                                      double poisson_ratio, double out[576]);
 /* Lumped mass rho int N_a dV per node (solidshell::elementMass, solid_shell.hpp:330). */
 shipfem_status shipfem_hex_mass(const double nodes[24], double density, double out[8]);
+/* The rotation the element carries: the polar factor of the deformation gradient at the
+ * element centre, column-major (solidshell::elementRotation, solid_shell.hpp:143). A rest
+ * element with a singular centre Jacobian gives the identity and SHIPFEM_INVALID_ARG. */
+shipfem_status shipfem_hex_rotation(const double rest[24], const double current[24],
+                                    double out[9]);
 /* Co-rotational internal force on the nodes, f = -R K (R^T x - X), R the polar factor
  * of the deformation gradient at the element centre (solidshell::internalForce,
  * solid_shell.hpp:355-356). */

@@ -356,13 +356,17 @@ double hexOmegaMax(const double K[kD * kD], const double m[kN]) {
     return std::sqrt(std::max(lam, 0.0));
 }
 
-void hexForce(const double K[kD * kD], const double* X, const double* x, double out[kD]) {
+bool hexRotation(const double* X, const double* x, double R[3][3]) {
     // R: polar factor of F = J(x) J(X)^-1 at the centre, Newton-Higham to convergence
     Natural cx, cX;
     natural(x, 0, 0, 0, cx);
     natural(X, 0, 0, 0, cX);
-    double A[3][3], R[3][3];
-    inverse3(cX.J, A);
+    double A[3][3];
+    if (inverse3(cX.J, A) == 0.0) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) R[i][j] = i == j ? 1.0 : 0.0;
+        return false;
+    }
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) R[i][j] = cx.J[i][0] * A[0][j] + cx.J[i][1] * A[1][j] + cx.J[i][2] * A[2][j];
     for (int it = 0; it < 64; ++it) {
@@ -377,6 +381,12 @@ void hexForce(const double K[kD * kD], const double* X, const double* x, double 
             }
         if (change < 1e-16) break;
     }
+    return true;
+}
+
+void hexForce(const double K[kD * kD], const double* X, const double* x, double out[kD]) {
+    double R[3][3];
+    hexRotation(X, x, R);
     double d[kD], kd[kD];
     for (int a = 0; a < kN; ++a)
         for (int i = 0; i < 3; ++i)
@@ -569,6 +579,16 @@ shipfem_status shipfem_hex_pressure_load(const double* rest_position, uint32_t n
         }
     }
     return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_hex_rotation(const double rest[24], const double current[24],
+                                    double out[9]) {
+    if (!rest || !current || !out) return SHIPFEM_INVALID_ARG;
+    double R[3][3];
+    const bool ok = shipfem::hexRotation(rest, current, R);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) out[3 * c + r] = R[r][c];
+    return ok ? SHIPFEM_OK : SHIPFEM_INVALID_ARG;
 }
 
 shipfem_status shipfem_hex_force(const double stiffness[576], const double rest[24],

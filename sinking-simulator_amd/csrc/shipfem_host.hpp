@@ -92,6 +92,10 @@ bool hexStiffness(const double* x, double E, double nu, double K[576]);
 bool hexMass(const double* x, double rho, double m[8]);
 bool hexCentreGradient(const double* x, double b0[8][3]);
 double hexOmegaMax(const double K[576], const double m[8]);
+// Polar factor of the centre deformation gradient (row-major); identity and false for a
+// rest element whose centre Jacobian is singular.
+bool hexRotation(const double* X, const double* x, double R[3][3]);
+void hexForce(const double K[576], const double* X, const double* x, double out[24]);
 
 // Device bytes of a prepared case (its arena), and the admission upper bound.
 uint64_t arenaBytes(const HostCase& hc);

@@ -172,6 +172,7 @@ SIGNATURES = {
     "shipfem_hex_stiffness": (S, [D, C.c_double, C.c_double, D]),
     "shipfem_hex_mass": (S, [D, C.c_double, D]),
     "shipfem_hex_force": (S, [D, D, D, D]),
+    "shipfem_hex_rotation": (S, [D, D, D]),
     "shipfem_hex_critical_timestep": (S, [D] + [C.c_double] * 4 + [D]),
     "shipfem_hex_solver_create": (S, [C.POINTER(CaseDesc), PP, C.c_char_p, C.c_size_t]),
     "shipfem_hex_plate_mesh": (S, [C.c_double] * 3 + [C.c_int32] * 3 + [C.POINTER(C.c_uint32)] * 2
@@ -520,6 +521,15 @@ def hex_force(K, rest, current):
     _check(lib().shipfem_hex_force(_p(K, C.c_double), _p(_nodes24(rest), C.c_double),
                                    _p(_nodes24(current), C.c_double), _p(out, C.c_double)))
     return out
+
+
+def hex_rotation(rest, current):
+    """Centre rotation [3, 3] (row-major R[i, j]) of one element (elementRotation)."""
+    out = np.zeros(9)
+    _check(lib().shipfem_hex_rotation(_p(_nodes24(rest), C.c_double),
+                                      _p(_nodes24(current), C.c_double), _p(out, C.c_double)),
+           "hex_rotation")
+    return out.reshape(3, 3).T
 
 
 def hex_critical_timestep(nodes, E=210e9, nu=0.3, rho=7850.0, safety=0.9):

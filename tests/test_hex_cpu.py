@@ -34,6 +34,8 @@ def test_library_element_matches_compiled_reference(k):
         f = sf.hex_force(GOLD["el_K"][k], x, GOLD["el_current"][k, c])
         # the moved element works at |x| ~ 13 m: FP64 cancellation in R^T x - X
         assert rel(f, GOLD["el_force"][k, c]) <= (1e-9 if k == 3 else 1e-12)
+        gold = GOLD["el_rotation"][k, c].reshape(3, 3).T  # column-major
+        assert np.abs(sf.hex_rotation(x, GOLD["el_current"][k, c]) - gold).max() <= 1e-12
 
 
 @pytest.mark.parametrize("k", range(len(GOLD["el_nodes"])))
