@@ -386,6 +386,23 @@ shipfem_status shipfem_hex_mass(const double nodes[24], double density, double o
  * element with a singular centre Jacobian gives the identity and SHIPFEM_INVALID_ARG. */
 shipfem_status shipfem_hex_rotation(const double rest[24], const double current[24],
                                     double out[9]);
+/* One element's elastoplastic update (solidshell::elementPlasticUpdate with
+ * Formulation::SolidShell, solid_shell.hpp:311-316), on the host, by the same element code
+ * the plastic hex solve runs on the device: co-rotated at the centre's polar factor, the
+ * seven enhanced parameters solved by Newton on the algorithmic tangent, J2 with linear
+ * hardening at each Gauss point, Rice-Tracey damage when failure_strain > 0, the step re-run
+ * without enhanced modes when a point fails. point_state[8 * 8]: per point the plastic
+ * strain (6, engineering shears), eqps and damage (1 = failed); enhanced[7]: the warm
+ * start. Both are advanced in place. force: what the element applies to its nodes. An
+ * element whose eight points have all failed is left as it is, with a zero force. */
+shipfem_status shipfem_hex_plastic_update(const double rest[24], const double current[24],
+                                          double youngs_modulus, double poisson_ratio,
+                                          double yield_strength, double hardening,
+                                          double failure_strain, double sensitivity,
+                                          double reference_triaxiality, double cutoff_triaxiality,
+                                          double point_state[64], double enhanced[7],
+                                          double force[24], double* dissipation,
+                                          int32_t* failed_points);
 /* Co-rotational internal force on the nodes, f = -R K (R^T x - X), R the polar factor
  * of the deformation gradient at the element centre (solidshell::internalForce,
  * solid_shell.hpp:355-356). */

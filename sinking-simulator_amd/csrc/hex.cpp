@@ -18,6 +18,7 @@
 // this library's own. Mass is the consistent row sum rho int N_a dV (solid_shell.cpp:
 // 655-663); the force is f = -R K (R^T x - X) with R the polar factor of the deformation
 // gradient at the element centre (solid_shell.cpp:680-766).
+#include "hex_plastic.hpp"
 #include "shipfem.h"
 #include "shipfem_host.hpp"
 
@@ -589,6 +590,52 @@ shipfem_status shipfem_hex_rotation(const double rest[24], const double current[
     for (int c = 0; c < 3; ++c)
         for (int r = 0; r < 3; ++r) out[3 * c + r] = R[r][c];
     return ok ? SHIPFEM_OK : SHIPFEM_INVALID_ARG;
+}
+
+shipfem_status shipfem_hex_plastic_update(const double rest[24], const double current[24],
+                                          double youngs_modulus, double poisson_ratio,
+                                          double yield_strength, double hardening,
+                                          double failure_strain, double sensitivity,
+                                          double reference_triaxiality, double cutoff_triaxiality,
+                                          double point_state[64], double enhanced[7],
+                                          double force[24], double* dissipation,
+                                          int32_t* failed_points) {
+    if (!rest || !current || !point_state || !enhanced || !force || !(youngs_modulus > 0) ||
+        !(poisson_ratio > -1 && poisson_ratio < 0.5) || !(yield_strength > 0) || !(hardening >= 0))
+        return SHIPFEM_INVALID_ARG;
+    double b0[8][3];
+    if (!shipfem::hexCentreGradient(rest, b0)) return SHIPFEM_INVALID_ARG;
+    double R[3][3];
+    shipfem::hexRotation(rest, current, R);
+    // corner offsets from corner 0: rest X, co-rotated displacement d = R^T (x - x_0) - X
+    double X[21], d[21];
+    for (int a = 1; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) {
+            X[3 * (a - 1) + i] = rest[3 * a + i] - rest[i];
+            double t = 0.0;
+            for (int k = 0; k < 3; ++k) t += R[k][i] * (current[3 * a + k] - current[k]);
+            d[3 * (a - 1) + i] = t - X[3 * (a - 1) + i];
+        }
+    const double mu = youngs_modulus / (2.0 * (1.0 + poisson_ratio));
+    const double lam = youngs_modulus * poisson_ratio / ((1.0 + poisson_ratio) * (1.0 - 2.0 * poisson_ratio));
+    const shipfem::hexp::Material m{mu, lam + (2.0 / 3.0) * mu, yield_strength, hardening,
+                                    failure_strain, sensitivity, reference_triaxiality,
+                                    cutoff_triaxiality, failure_strain > 0.0};
+    double fi[21], eu[48];
+    const shipfem::hexp::Result res =
+        shipfem::hexp::update(m, X, d, point_state, enhanced, 1, false, fi, eu, 1);
+    // the force on the nodes, -R f, corner 0 carrying minus the others
+    double f0[3] = {0.0, 0.0, 0.0};
+    for (int a = 1; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) f0[i] -= fi[3 * (a - 1) + i];
+    for (int a = 0; a < 8; ++a)
+        for (int i = 0; i < 3; ++i) {
+            const double* v = a == 0 ? f0 : fi + 3 * (a - 1);
+            force[3 * a + i] = -(R[i][0] * v[0] + R[i][1] * v[1] + R[i][2] * v[2]);
+        }
+    if (dissipation) *dissipation = res.dissipation;
+    if (failed_points) *failed_points = res.failedEnd;
+    return SHIPFEM_OK;
 }
 
 shipfem_status shipfem_hex_force(const double stiffness[576], const double rest[24],

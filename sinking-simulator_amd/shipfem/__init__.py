@@ -173,6 +173,7 @@ SIGNATURES = {
     "shipfem_hex_mass": (S, [D, C.c_double, D]),
     "shipfem_hex_force": (S, [D, D, D, D]),
     "shipfem_hex_rotation": (S, [D, D, D]),
+    "shipfem_hex_plastic_update": (S, [D, D] + [C.c_double] * 8 + [D, D, D, D, C.POINTER(C.c_int32)]),
     "shipfem_hex_critical_timestep": (S, [D] + [C.c_double] * 4 + [D]),
     "shipfem_hex_solver_create": (S, [C.POINTER(CaseDesc), PP, C.c_char_p, C.c_size_t]),
     "shipfem_hex_plate_mesh": (S, [C.c_double] * 3 + [C.c_int32] * 3 + [C.POINTER(C.c_uint32)] * 2
@@ -530,6 +531,27 @@ def hex_rotation(rest, current):
                                       _p(_nodes24(current), C.c_double), _p(out, C.c_double)),
            "hex_rotation")
     return out.reshape(3, 3).T
+
+
+def hex_plastic_update(rest, current, point_state, enhanced, E=210e9, nu=0.3, yield_strength=355e6,
+                       hardening=1e9, failure_strain=0.0, sensitivity=1.5,
+                       reference_triaxiality=1 / 3, cutoff_triaxiality=-1 / 3):
+    """One element's elastoplastic update (elementPlasticUpdate). `point_state` [64] (per
+    point: plastic strain 6, eqps, damage) and `enhanced` [7] are float64 arrays advanced
+    in place. Returns (force [24], dissipation, failed points)."""
+    for a, n in ((point_state, 64), (enhanced, 7)):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.size == n
+                and a.flags.c_contiguous):
+            raise ValueError(f"hex_plastic_update: need a contiguous float64 array of {n}")
+    out = np.zeros(24)
+    diss = C.c_double()
+    failed = C.c_int32()
+    _check(lib().shipfem_hex_plastic_update(
+        _p(_nodes24(rest), C.c_double), _p(_nodes24(current), C.c_double), E, nu, yield_strength,
+        hardening, failure_strain, sensitivity, reference_triaxiality, cutoff_triaxiality,
+        _p(point_state, C.c_double), _p(enhanced, C.c_double), _p(out, C.c_double),
+        C.byref(diss), C.byref(failed)), "hex_plastic_update")
+    return out, diss.value, failed.value
 
 
 def hex_critical_timestep(nodes, E=210e9, nu=0.3, rho=7850.0, safety=0.9):

@@ -250,6 +250,32 @@ def test_oracle_plastic_update_matches_compiled_reference(j):
         assert info_all[:, 1].max() == 3 and info_all[-1, 3] == 8
 
 
+@pytest.mark.parametrize("j", range(3))
+def test_library_plastic_update_matches_compiled_reference(j):
+    # the element code the device pass runs (hex_plastic.hpp), on the host through
+    # shipfem_hex_plastic_update, along the same three paths: the co-rotation, the enhanced
+    # Newton, the point returns with damage, the rerun without enhanced modes, the torn
+    # element
+    E_, nu_, sy, Hh, sens, er, ec = GOLD["pl_material"]
+    X, cur = GOLD[f"pl{j}_rest"], GOLD[f"pl{j}_current"]
+    ef = float(GOLD[f"pl{j}_failure_strain"])
+    point, enh = np.zeros(64), np.zeros(7)
+    fscale = np.abs(GOLD[f"pl{j}_force"]).max()
+    escale = max(np.abs(GOLD[f"pl{j}_enhanced"]).max(), 1e-30)
+    for k in range(len(cur)):
+        gi = GOLD[f"pl{j}_info"][k]
+        f, diss, failed = sf.hex_plastic_update(X, cur[k], point, enh, E_, nu_, sy, Hh, ef, sens, er, ec)
+        gs = GOLD[f"pl{j}_state"][k]
+        assert np.abs(f - GOLD[f"pl{j}_force"][k]).max() <= 1e-10 * fscale, k
+        assert np.abs(point.reshape(8, 8) - gs[:, :8]).max() <= 1e-12, k
+        assert np.array_equal(point.reshape(8, 8)[:, 7] >= 1.0, gs[:, 8] == 1.0)
+        # the operator is formed in another order than the reference's stored rows: the
+        # enhanced parameters carry Kaa's conditioning of that rounding (about 1e-12)
+        assert np.abs(enh - GOLD[f"pl{j}_enhanced"][k]).max() <= 1e-10 * escale, k
+        assert diss == pytest.approx(gi[0], rel=1e-9, abs=1e-12 * sy * 1e-3)
+        assert failed == gi[4]
+
+
 def test_plastic_update_below_yield_is_the_condensed_elastic_force():
     # the reference's tie between the two paths: a material that never yields gives
     # internalForce with the condensed stiffness (the EAS Newton solves the linear system
