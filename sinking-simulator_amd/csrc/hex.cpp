@@ -440,15 +440,21 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
             int p = 0;
             for (int i = 0; i < kHexDof && !plastic; ++i)
                 for (int j = i; j < kHexDof; ++j) hc.hk[(p++) * tp + e] = K[(3 + i) * kD + 3 + j];
+            double off[kHexDof], cX[kHexDof];
             for (int a = 1; a < kN; ++a)
                 for (int i = 0; i < 3; ++i) {
+                    off[3 * (a - 1) + i] = x[3 * a + i] - x[i];
                     hc.hgeo[(3 * (a - 1) + i) * tp + e] = static_cast<float>(b0[a][i]);
-                    hc.hgeo[(kHexDof + 3 * (a - 1) + i) * tp + e] = static_cast<float>(x[3 * a + i] - x[i]);
-                    if (plastic) {
-                        hc.hgeod[(3 * (a - 1) + i) * tp + e] = b0[a][i];
-                        hc.hgeod[(kHexDof + 3 * (a - 1) + i) * tp + e] = x[3 * a + i] - x[i];
-                    }
+                    hc.hgeo[(kHexDof + 3 * (a - 1) + i) * tp + e] = static_cast<float>(off[3 * (a - 1) + i]);
                 }
+            if (plastic) {
+                // the plastic pass reads the rest geometry as its basis coefficients
+                hexp::basis(off, cX);
+                for (int j = 0; j < kHexDof; ++j) {
+                    hc.hgeod[j * tp + e] = b0[1 + j / 3][j % 3];
+                    hc.hgeod[(kHexDof + j) * tp + e] = cX[j];
+                }
+            }
         }
     };
     const uint32_t threads =
@@ -621,9 +627,11 @@ shipfem_status shipfem_hex_plastic_update(const double rest[24], const double cu
     const shipfem::hexp::Material m{mu, lam + (2.0 / 3.0) * mu, yield_strength, hardening,
                                     failure_strain, sensitivity, reference_triaxiality,
                                     cutoff_triaxiality, failure_strain > 0.0};
-    double fi[21], eu[48];
+    double cX[21], cd[21], fi[21], eu[48];
+    shipfem::hexp::basis(X, cX);
+    shipfem::hexp::basis(d, cd);
     const shipfem::hexp::Result res =
-        shipfem::hexp::update(m, X, d, point_state, enhanced, 1, false, fi, eu, 1);
+        shipfem::hexp::update(m, cX, cd, point_state, enhanced, 1, false, fi, eu, 1);
     // the force on the nodes, -R f, corner 0 carrying minus the others
     double f0[3] = {0.0, 0.0, 0.0};
     for (int a = 1; a < 8; ++a)

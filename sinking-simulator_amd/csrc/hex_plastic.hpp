@@ -16,8 +16,12 @@
 // assumed rows through their sampling points. Everything works on corner offsets from
 // corner 0 (X_0 = 0, d_0 = 0): the operator annihilates translations.
 //
-// Shared by the device pass and by host code; the caller supplies the co-rotated corner
-// offsets d and receives the internal force of corners 1..7 in the co-rotated frame.
+// Geometry and displacement enter as the coefficients of their trilinear fields over the
+// seven non-constant monomials (basis() below), so every natural derivative is a handful of
+// multiply-adds and the force is assembled on those coefficients, then mapped to corners once.
+// Shared by the device pass and by host code; the caller supplies the basis coefficients of
+// the rest and co-rotated corner offsets and receives the internal force of corners 1..7 in
+// the co-rotated frame.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,27 +44,83 @@ struct Result {
     int failedStart = 0, failedEnd = 0;
 };
 
-// d N_a / d(xi, eta, zeta) of corner a at a natural point.
-__host__ __device__ inline void dshape(int a, double xi, double eta, double zeta, double out[3]) {
-    const double sx = (a == 1 || a == 2 || a == 5 || a == 6) ? 1.0 : -1.0;
-    const double sy = (a == 2 || a == 3 || a == 6 || a == 7) ? 1.0 : -1.0;
-    const double sz = a >= 4 ? 1.0 : -1.0;
-    out[0] = 0.125 * sx * (1 + eta * sy) * (1 + zeta * sz);
-    out[1] = 0.125 * sy * (1 + xi * sx) * (1 + zeta * sz);
-    out[2] = 0.125 * sz * (1 + xi * sx) * (1 + eta * sy);
+// The trilinear field of values v_a at the corners, written over the monomials
+//   phi = (xi, eta, zeta, xi eta, eta zeta, zeta xi, xi eta zeta)
+// as v(xi, eta, zeta) = v_centre + sum_m phi_m c_m with c_m = 1/8 sum_a s_m(a) v_a, s_m(a) the
+// monomial at corner a's natural coordinates. A natural derivative anywhere is then one
+// coefficient plus three scaled ones, and a force conjugate to the c_m maps back to the
+// corners through the same signs: f_a = 1/8 sum_m s_m(a) F_m.
+__host__ __device__ constexpr double cornerSign(int m, int a) {
+    const int x = (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : -1;
+    const int y = (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : -1;
+    const int z = a >= 4 ? 1 : -1;
+    const int s[7] = {x, y, z, x * y, y * z, z * x, x * y * z};
+    return s[m];
 }
 
-// Natural derivatives at a point of a field given at corners 1..7 (corner 0 at zero):
-// out[i][k] = d v_i / d xi_k (for the rest offsets this is the Jacobian).
-__host__ __device__ inline void natural(const double* v, double xi, double eta, double zeta,
-                                        double out[3][3]) {
-    for (int i = 0; i < 3; ++i)
-        for (int k = 0; k < 3; ++k) out[i][k] = 0.0;
-    for (int a = 1; a < 8; ++a) {
-        double n[3];
-        dshape(a, xi, eta, zeta, n);
-        for (int i = 0; i < 3; ++i)
-            for (int k = 0; k < 3; ++k) out[i][k] += n[k] * v[3 * (a - 1) + i];
+// c[3 m + i] of values v[3 (a - 1) + i] at corners 1..7, corner 0 at zero.
+__host__ __device__ inline void basis(const double* v, double c[21]) {
+#pragma unroll
+    for (int m = 0; m < 7; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 1; a < 8; ++a) t += cornerSign(m, a) * v[3 * (a - 1) + i];
+            c[3 * m + i] = 0.125 * t;
+        }
+}
+
+// Column l of the natural derivative at (xi, eta, zeta): out[i] = d v_i / d xi_l.
+//   l = 0: c_0 + eta c_3 + zeta c_5 + eta zeta c_6
+//   l = 1: c_1 + xi c_3 + zeta c_4 + xi zeta c_6
+//   l = 2: c_2 + eta c_4 + xi c_5 + xi eta c_6
+__host__ __device__ inline void column(const double* c, int l, double xi, double eta, double zeta,
+                                       double out[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (l == 0)
+            out[i] = c[i] + eta * c[9 + i] + zeta * c[15 + i] + eta * zeta * c[18 + i];
+        else if (l == 1)
+            out[i] = c[3 + i] + xi * c[9 + i] + zeta * c[12 + i] + xi * zeta * c[18 + i];
+        else
+            out[i] = c[6 + i] + eta * c[12 + i] + xi * c[15 + i] + xi * eta * c[18 + i];
+    }
+}
+
+// J[i][k] = d v_i / d xi_k at a point.
+__host__ __device__ inline void jacobian(const double* c, double xi, double eta, double zeta,
+                                         double J[3][3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double col[3];
+        column(c, k, xi, eta, zeta, col);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) J[i][k] = col[i];
+    }
+}
+
+// F_m += P d phi_m / d xi_l at (xi, eta, zeta).
+__host__ __device__ inline void addGrad(double F[21], const double P[3], int l, double xi,
+                                        double eta, double zeta) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (l == 0) {
+            F[0 + i] += P[i];
+            F[9 + i] += eta * P[i];
+            F[15 + i] += zeta * P[i];
+            F[18 + i] += eta * zeta * P[i];
+        } else if (l == 1) {
+            F[3 + i] += P[i];
+            F[9 + i] += xi * P[i];
+            F[12 + i] += zeta * P[i];
+            F[18 + i] += xi * zeta * P[i];
+        } else {
+            F[6 + i] += P[i];
+            F[12 + i] += eta * P[i];
+            F[15 + i] += xi * P[i];
+            F[18 + i] += xi * eta * P[i];
+        }
     }
 }
 
@@ -133,18 +193,23 @@ __host__ __device__ inline int modeRow(int k) { return k < 2 ? k : k < 5 ? 2 : 3
 constexpr double kShear[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
 constexpr double kCorner[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
 
-// The covariant strain of the displacement d at Gauss point g (rows 0, 1, 3 at the point,
-// rows 2, 4, 5 from the samples of its level).
-__host__ __device__ inline void covariantStrain(const double* X, const double* d, int g,
+__host__ __device__ inline double dot3(const double a[3], const double b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// The covariant strain at Gauss point g of the displacement with coefficients cd, over the
+// rest geometry cX (rows 0, 1, 3 at the point, rows 2, 4, 5 from the samples of its level).
+// J is the rest Jacobian at the point.
+__host__ __device__ inline void covariantStrain(const double J[3][3], const double* cd, int g,
                                                 const double smp[8], double e[6]) {
     const double xi = gauss(g, 0), eta = gauss(g, 1), zeta = gauss(g, 2);
-    double J[3][3], U[3][3];
-    natural(X, xi, eta, zeta, J);
-    natural(d, xi, eta, zeta, U);
-    e[0] = J[0][0] * U[0][0] + J[1][0] * U[1][0] + J[2][0] * U[2][0];
-    e[1] = J[0][1] * U[0][1] + J[1][1] * U[1][1] + J[2][1] * U[2][1];
-    e[3] = J[0][0] * U[0][1] + J[1][0] * U[1][1] + J[2][0] * U[2][1] +
-           J[0][1] * U[0][0] + J[1][1] * U[1][0] + J[2][1] * U[2][0];
+    double u0[3], u1[3];
+    column(cd, 0, xi, eta, zeta, u0);
+    column(cd, 1, xi, eta, zeta, u1);
+    const double g0[3] = {J[0][0], J[1][0], J[2][0]}, g1[3] = {J[0][1], J[1][1], J[2][1]};
+    e[0] = dot3(g0, u0);
+    e[1] = dot3(g1, u1);
+    e[3] = dot3(g0, u1) + dot3(g1, u0);
     e[4] = 0.5 * (1 - xi) * smp[0] + 0.5 * (1 + xi) * smp[1];
     e[5] = 0.5 * (1 - eta) * smp[2] + 0.5 * (1 + eta) * smp[3];
     double t = 0.0;
@@ -152,38 +217,22 @@ __host__ __device__ inline void covariantStrain(const double* X, const double* d
     e[2] = t;
 }
 
-// The samples of one zeta level for a displacement d.
-__host__ __device__ inline void samples(const double* X, const double* d, double zeta, double smp[8]) {
+// The samples of one zeta level for the displacement with coefficients cd.
+__host__ __device__ inline void samples(const double* cX, const double* cd, double zeta, double smp[8]) {
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
-        double J[3][3], U[3][3];
-        natural(X, kShear[s][0], kShear[s][1], zeta, J);
-        natural(d, kShear[s][0], kShear[s][1], zeta, U);
-        if (s < 2)  // 2E23 = g_1 . d,zeta + g_2 . d,eta
-            smp[s] = J[0][1] * U[0][2] + J[1][1] * U[1][2] + J[2][1] * U[2][2] +
-                     J[0][2] * U[0][1] + J[1][2] * U[1][1] + J[2][2] * U[2][1];
-        else  // 2E31 = g_2 . d,xi + g_0 . d,zeta
-            smp[s] = J[0][2] * U[0][0] + J[1][2] * U[1][0] + J[2][2] * U[2][0] +
-                     J[0][0] * U[0][2] + J[1][0] * U[1][2] + J[2][0] * U[2][2];
-        natural(X, kCorner[s][0], kCorner[s][1], zeta, J);
-        natural(d, kCorner[s][0], kCorner[s][1], zeta, U);
-        smp[4 + s] = J[0][2] * U[0][2] + J[1][2] * U[1][2] + J[2][2] * U[2][2];
-    }
-}
-
-// Adds c * (g_k dN_a,l + g_l dN_a,k) (or, for k == l, c * g_k dN_a,k) at a natural point to
-// the force of corners 1..7.
-__host__ __device__ inline void spread(const double* X, double xi, double eta, double zeta, int k,
-                                       int l, double c, double f[21]) {
-    double J[3][3];
-    natural(X, xi, eta, zeta, J);
-    for (int a = 1; a < 8; ++a) {
-        double n[3];
-        dshape(a, xi, eta, zeta, n);
-        for (int i = 0; i < 3; ++i) {
-            double v = J[i][k] * n[l];
-            if (k != l) v += J[i][l] * n[k];
-            f[3 * (a - 1) + i] += c * v;
-        }
+        const double xi = kShear[s][0], eta = kShear[s][1];
+        // 2E23 = g_1 . d,zeta + g_2 . d,eta; 2E31 = g_2 . d,xi + g_0 . d,zeta
+        const int l = s < 2 ? 1 : 0;
+        double ga[3], gz[3], ua[3], uz[3];
+        column(cX, l, xi, eta, zeta, ga);
+        column(cX, 2, xi, eta, zeta, gz);
+        column(cd, l, xi, eta, zeta, ua);
+        column(cd, 2, xi, eta, zeta, uz);
+        smp[s] = dot3(ga, uz) + dot3(gz, ua);
+        column(cX, 2, kCorner[s][0], kCorner[s][1], zeta, gz);
+        column(cd, 2, kCorner[s][0], kCorner[s][1], zeta, uz);
+        smp[4 + s] = dot3(gz, uz);
     }
 }
 
@@ -267,12 +316,13 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
     return 1;
 }
 
-// The element update. X: rest offsets of corners 1..7 from corner 0; d: co-rotated corner
-// offsets; hist: 8 points x (plastic strain 6, eqps, damage) at hist[(8 g + k) * stride];
-// enh: 7 enhanced parameters at enh[k * stride]. Writes the internal force of corners
-// 1..7 (corner 0 carries minus their sum) in the co-rotated frame. A torn element (all
-// points failed) is left as it is and returns a zero force.
-__host__ __device__ inline Result update(const Material& m, const double* X, const double* d,
+// The element update. cX: basis coefficients of the rest corner offsets (corner 0 at
+// zero); cd: those of the co-rotated displacement offsets; hist: 8 points x (plastic strain
+// 6, eqps, damage) at hist[(8 g + k) * stride]; enh: 7 enhanced parameters at
+// enh[k * stride]. Writes the internal force of corners 1..7 (corner 0 carries minus their
+// sum) in the co-rotated frame. A torn element (all points failed) is left as it is and
+// returns a zero force.
+__host__ __device__ inline Result update(const Material& m, const double* cX, const double* cd,
                                          double* hist, double* enh, uint32_t stride,
                                          bool wantStrain, double f[21], double* eu,
                                          uint32_t euStride) {
@@ -283,10 +333,10 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         res.failedEnd = 8;
         return res;
     }
-    // the centre's map T0 (columns 0-3), the points' volumes and det J0/det J, the modes'
-    // normalisations, and the base strain of every point
+    // the centre's map T0 (columns 0-3): the centre Jacobian's columns are c_0, c_1, c_2
     double J0[3][3], A0[3][3];
-    natural(X, 0.0, 0.0, 0.0, J0);
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) J0[i][k] = cX[3 * k + i];
     const double det0 = inverse3(J0, A0);
     double t0[4][6];
     for (int r = 0; r < 4; ++r) {
@@ -294,19 +344,31 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         unit[r] = 1.0;
         push(A0, unit, t0[r]);
     }
+    // T0[:, p]^T C T0[:, q] = kappa tr_p tr_q + mu theta D_pq - 2 mu thetaBar nv_p nv_q for
+    // C = kappa 1(x)1 + 2 mu theta I_dev - 2 mu thetaBar n(x)n, engineering shears: tr and D
+    // are the element's, nv = n . T0[:, q] the point's
+    double tr4[4], D[10];
+    for (int p = 0; p < 4; ++p) tr4[p] = t0[p][0] + t0[p][1] + t0[p][2];
+    for (int p = 0; p < 4; ++p)
+        for (int q = 0; q <= p; ++q) {
+            double t = 0.0;
+            for (int i = 0; i < 3; ++i) t += 2.0 * t0[p][i] * (t0[q][i] - tr4[q] / 3.0);
+            for (int i = 3; i < 6; ++i) t += t0[p][i] * t0[q][i];
+            D[p * (p + 1) / 2 + q] = t;
+        }
     // eu: the base strain of point g, row i at eu[(6 g + i) * euStride] (the caller's
     // buffer: on the device, this lane's column of a block-shared array)
     double w[8], sc[8];
     {
         double smp[2][8];
-        samples(X, d, -0.57735026918962576451, smp[0]);
-        samples(X, d, 0.57735026918962576451, smp[1]);
+        samples(cX, cd, -0.57735026918962576451, smp[0]);
+        samples(cX, cd, 0.57735026918962576451, smp[1]);
         for (int g = 0; g < 8; ++g) {
             double J[3][3], A[3][3], ec[6];
-            natural(X, gauss(g, 0), gauss(g, 1), gauss(g, 2), J);
+            jacobian(cX, gauss(g, 0), gauss(g, 1), gauss(g, 2), J);
             w[g] = inverse3(J, A);
             sc[g] = det0 / w[g];
-            covariantStrain(X, d, g, smp[(g >> 2) & 1], ec);
+            covariantStrain(J, cd, g, smp[(g >> 2) & 1], ec);
             double e6[6];
             push(A, ec, e6);
             for (int i = 0; i < 6; ++i) eu[(6 * g + i) * euStride] = e6[i];
@@ -356,31 +418,24 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
                 const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
                 died = died || (fl & 2);
                 if (!eas) continue;
-                double ts[4], M[10];
+                double ts[4], nv[4], M[10];
                 for (int q = 0; q < 4; ++q) {
-                    double t = 0.0;
-                    for (int i = 0; i < 6; ++i) t += t0[q][i] * sig[i];
+                    double t = 0.0, u = 0.0;
+                    for (int i = 0; i < 6; ++i) {
+                        t += t0[q][i] * sig[i];
+                        u += t0[q][i] * n6[i];
+                    }
                     ts[q] = t;
+                    nv[q] = u;
                 }
-                for (int q = 0; q < 4; ++q) {
-                    // C T0[:, q] with C = kappa 1(x)1 + 2 mu theta I_dev - 2 mu thetaBar n(x)n
-                    // (zero for a failed point)
-                    double cq[6] = {0, 0, 0, 0, 0, 0};
-                    if (!(fl & 4)) {
-                        const double* v = t0[q];
-                        const double tr = v[0] + v[1] + v[2];
-                        double nv = 0.0;
-                        for (int i = 0; i < 6; ++i) nv += n6[i] * v[i];
-                        for (int i = 0; i < 3; ++i)
-                            cq[i] = m.kappa * tr + 2.0 * m.mu * th * (v[i] - tr / 3.0) - 2.0 * m.mu * thb * n6[i] * nv;
-                        for (int i = 3; i < 6; ++i) cq[i] = m.mu * th * v[i] - 2.0 * m.mu * thb * n6[i] * nv;
+                // (zero for a failed point)
+                const double ck = (fl & 4) ? 0.0 : m.kappa, cm = (fl & 4) ? 0.0 : m.mu * th,
+                             cn = (fl & 4) ? 0.0 : 2.0 * m.mu * thb;
+                for (int p = 0; p < 4; ++p)
+                    for (int q = 0; q <= p; ++q) {
+                        const int i = p * (p + 1) / 2 + q;
+                        M[i] = ck * tr4[p] * tr4[q] + cm * D[i] - cn * nv[p] * nv[q];
                     }
-                    for (int p = q; p < 4; ++p) {
-                        double t = 0.0;
-                        for (int i = 0; i < 6; ++i) t += t0[p][i] * cq[i];
-                        M[p * (p + 1) / 2 + q] = t;
-                    }
-                }
                 double cg[7];
                 for (int k = 0; k < 7; ++k) {
                     cg[k] = coef(g, k);
@@ -432,9 +487,10 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         eas = 0;
         for (int k = 0; k < 7; ++k) alpha[k] = 0.0;
     }
-    // commit at the final alpha; the force through the same rows, the assumed rows through
-    // their samples
-    double cs[2][8];
+    // commit at the final alpha; the force through the same rows, conjugate to the basis
+    // coefficients (F), the assumed rows through their samples
+    double F[21], cs[2][8];
+    for (int j = 0; j < 21; ++j) F[j] = 0.0;
     for (int l = 0; l < 2; ++l)
         for (int s = 0; s < 8; ++s) cs[l][s] = 0.0;
 #pragma unroll 1
@@ -457,12 +513,18 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         if (fl & 4) continue;  // a failed point carries no stress
         const double xi = gauss(g, 0), eta = gauss(g, 1), zeta = gauss(g, 2);
         double J[3][3], A[3][3], tau[6];
-        natural(X, xi, eta, zeta, J);
+        jacobian(cX, xi, eta, zeta, J);
         inverse3(J, A);
         pull(A, sig, tau);
-        spread(X, xi, eta, zeta, 0, 0, w[g] * tau[0], f);
-        spread(X, xi, eta, zeta, 1, 1, w[g] * tau[1], f);
-        spread(X, xi, eta, zeta, 0, 1, w[g] * tau[3], f);
+        // rows E11, E22, 2E12 at the point: tau_0 g_0 + tau_3 g_1 against d phi / d xi and
+        // tau_1 g_1 + tau_3 g_0 against d phi / d eta
+        double p0[3], p1[3];
+        for (int i = 0; i < 3; ++i) {
+            p0[i] = w[g] * (tau[0] * J[i][0] + tau[3] * J[i][1]);
+            p1[i] = w[g] * (tau[1] * J[i][1] + tau[3] * J[i][0]);
+        }
+        addGrad(F, p0, 0, xi, eta, zeta);
+        addGrad(F, p1, 1, xi, eta, zeta);
         const int lv = (g >> 2) & 1;
         cs[lv][0] += w[g] * tau[4] * 0.5 * (1 - xi);
         cs[lv][1] += w[g] * tau[4] * 0.5 * (1 + xi);
@@ -471,16 +533,40 @@ __host__ __device__ inline Result update(const Material& m, const double* X, con
         for (int s = 0; s < 4; ++s)
             cs[lv][4 + s] += w[g] * tau[2] * 0.25 * (1 + xi * kCorner[s][0]) * (1 + eta * kCorner[s][1]);
     }
+#pragma unroll 1
     for (int lv = 0; lv < 2; ++lv) {
         const double zeta = lv ? 0.57735026918962576451 : -0.57735026918962576451;
+#pragma unroll
         for (int s = 0; s < 4; ++s) {
-            if (s < 2)
-                spread(X, kShear[s][0], kShear[s][1], zeta, 1, 2, cs[lv][s], f);
-            else
-                spread(X, kShear[s][0], kShear[s][1], zeta, 2, 0, cs[lv][s], f);
-            spread(X, kCorner[s][0], kCorner[s][1], zeta, 2, 2, cs[lv][4 + s], f);
+            const double xi = kShear[s][0], eta = kShear[s][1];
+            // 2E23 (s < 2) couples g_1 with d,zeta and g_2 with d,eta; 2E31 couples g_2 with
+            // d,xi and g_0 with d,zeta
+            const int l = s < 2 ? 1 : 0;
+            double ga[3], gz[3], pa[3], pz[3];
+            column(cX, l, xi, eta, zeta, ga);
+            column(cX, 2, xi, eta, zeta, gz);
+            for (int i = 0; i < 3; ++i) {
+                pz[i] = cs[lv][s] * ga[i];
+                pa[i] = cs[lv][s] * gz[i];
+            }
+            addGrad(F, pz, 2, xi, eta, zeta);
+            addGrad(F, pa, l, xi, eta, zeta);
+            // E33 at the corner sample
+            column(cX, 2, kCorner[s][0], kCorner[s][1], zeta, gz);
+            for (int i = 0; i < 3; ++i) pz[i] = cs[lv][4 + s] * gz[i];
+            addGrad(F, pz, 2, kCorner[s][0], kCorner[s][1], zeta);
         }
     }
+    // back to corners 1..7
+#pragma unroll
+    for (int a = 1; a < 8; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double t = 0.0;
+#pragma unroll
+            for (int mm = 0; mm < 7; ++mm) t += cornerSign(mm, a) * F[3 * mm + i];
+            f[3 * (a - 1) + i] = 0.125 * t;
+        }
     for (int k = 0; k < 7; ++k) enh[k * stride] = alpha[k];
     return res;
 }

@@ -577,14 +577,11 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
                 ux[a][1] = (double)pv.y - (double)c.cu[np + gi[a]];
                 ux[a][2] = (double)pv.z - (double)c.cu[2 * np + gi[a]];
             }
-            double r[7][3], W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, X[kHexDof];
+            double r[7][3], W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int a = 0; a < 7; ++a)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    r[a][i] = ux[a + 1][i] - ux[0][i];
-                    X[3 * a + i] = c.hgeod[(kHexDof + 3 * a + i) * tp + e];
-                }
+                for (int i = 0; i < 3; ++i) r[a][i] = ux[a + 1][i] - ux[0][i];
 #pragma unroll
             for (int a = 0; a < 7; ++a) {
                 const double b0[3] = {c.hgeod[(3 * a) * tp + e], c.hgeod[(3 * a + 1) * tp + e],
@@ -595,18 +592,24 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
                     for (int j = 0; j < 3; ++j) W[i * 3 + j] = fma(r[a][i], b0[j], W[i * 3 + j]);
             }
             polar_perturbation_d(W);
-            double d[kHexDof];
+            // the co-rotated displacement d = r + W^T (X + r), taken to basis coefficients
+            // (the map is linear): cd = cr + W^T (cX + cr)
+            double cX[kHexDof], cd[kHexDof];
+            hexp::basis(&r[0][0], cd);
 #pragma unroll
-            for (int a = 0; a < 7; ++a) {
-                const double y[3] = {X[3 * a] + r[a][0], X[3 * a + 1] + r[a][1], X[3 * a + 2] + r[a][2]};
+            for (int j = 0; j < kHexDof; ++j) cX[j] = c.hgeod[(kHexDof + j) * tp + e];
+#pragma unroll
+            for (int mm = 0; mm < 7; ++mm) {
+                const double y[3] = {cX[3 * mm] + cd[3 * mm], cX[3 * mm + 1] + cd[3 * mm + 1],
+                                     cX[3 * mm + 2] + cd[3 * mm + 2]};
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    d[3 * a + i] = r[a][i] + (W[0 * 3 + i] * y[0] + W[1 * 3 + i] * y[1] + W[2 * 3 + i] * y[2]);
+                    cd[3 * mm + i] += W[0 * 3 + i] * y[0] + W[1 * 3 + i] * y[1] + W[2 * 3 + i] * y[2];
             }
             const hexp::Material m{c.mu_d, c.lam_d + (2.0 / 3.0) * c.mu_d, c.yield_d, c.hard_d,
                                    c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage};
             double fi[kHexDof];
-            const hexp::Result res = hexp::update(m, X, d, c.hpoint + e, c.henh + e, tp, sample != 0, fi,
+            const hexp::Result res = hexp::update(m, cX, cd, c.hpoint + e, c.henh + e, tp, sample != 0, fi,
                                                   sh_eu + threadIdx.x, BLOCK);
             diss = res.dissipation;
             fract = res.released;

@@ -71,7 +71,9 @@ struct CaseDev {
     const double* hk;    // hex: [kHexPacked][Tpad] stiffness block of corners 1..7 (FP64)
     const float* hgeo;   // hex: [kHexGeo][Tpad] d N_a/dX at the centre (21: 3(a-1) + j),
                          // then rest X_a - X_0 (21: 21 + 3(a-1) + i), corners a = 1..7
-    const double* hgeod; // plastic hex: hgeo in FP64 [kHexGeo][Tpad]
+    const double* hgeod; // plastic hex: [kHexGeo][Tpad] FP64, d N_a/dX at the centre as in
+                         // hgeo, then the rest offsets' coefficients over the monomials
+                         // (xi, eta, zeta, xi eta, eta zeta, zeta xi, xi eta zeta), 3 m + i
     double* hpoint;      // plastic hex: [kHexPoint][Tpad], entry 8 g + k
     double* henh;        // plastic hex: [7][Tpad] enhanced parameters (warm start)
     float* pstate;       // [7 or 8][Tpad] plastic strain (6, eng. shears), eqps, damage
