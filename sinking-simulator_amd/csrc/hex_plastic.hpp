@@ -404,12 +404,78 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
     double vol = 0.0;
     for (int g = 0; g < 8; ++g) vol += w[g];
     const double scale = m.yield * vol;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        bool died = false;
-        for (int iter = 0;; ++iter) {
+    // The commit at the final alpha: history, energies, and the force through the same
+    // rows, conjugate to the basis coefficients (F), the assumed rows through their
+    // samples. With onlyElastic it gives up (false) at the first point that yields,
+    // before writing anything: a point that does not yield keeps its history bit for bit,
+    // so the points before it have changed nothing.
+    double F[21], cs[2][8];
+    auto commit = [&](bool onlyElastic) -> bool {
+        for (int j = 0; j < 21; ++j) F[j] = 0.0;
+        for (int l = 0; l < 2; ++l)
+            for (int s = 0; s < 8; ++s) cs[l][s] = 0.0;
+        double dsum = 0.0, rsum = 0.0, usum = 0.0;
+        int failed = 0;
+#pragma unroll 1
+        for (int g = 0; g < 8; ++g) {
+            double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
+            strainAt(g, eas, st);
+            for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
+            const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
+            if (onlyElastic && fl != 0) return false;
+            if (fl & 1)  // only a yielding point changes its history
+                for (int k = 0; k < 8; ++k) hist[(8 * g + k) * stride] = h[k];
+            failed += (fl & 4) != 0;
+            dsum += w[g] * dg;
+            rsum += w[g] * rel;
+            if (wantStrain) {
+                const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
+                double u = 0.0;
+                for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * m.mu) + pm / (3.0 * m.kappa));
+                for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / m.mu);
+                usum += w[g] * u;
+            }
+            if (fl & 4) continue;  // a failed point carries no stress
+            const double xi = gauss(g, 0), eta = gauss(g, 1), zeta = gauss(g, 2);
+            double J[3][3], A[3][3], tau[6];
+            jacobian(cX, xi, eta, zeta, J);
+            inverse3(J, A);
+            pull(A, sig, tau);
+            // rows E11, E22, 2E12 at the point: tau_0 g_0 + tau_3 g_1 against d phi / d xi and
+            // tau_1 g_1 + tau_3 g_0 against d phi / d eta
+            double p0[3], p1[3];
+            for (int i = 0; i < 3; ++i) {
+                p0[i] = w[g] * (tau[0] * J[i][0] + tau[3] * J[i][1]);
+                p1[i] = w[g] * (tau[1] * J[i][1] + tau[3] * J[i][0]);
+            }
+            addGrad(F, p0, 0, xi, eta, zeta);
+            addGrad(F, p1, 1, xi, eta, zeta);
+            const int lv = (g >> 2) & 1;
+            cs[lv][0] += w[g] * tau[4] * 0.5 * (1 - xi);
+            cs[lv][1] += w[g] * tau[4] * 0.5 * (1 + xi);
+            cs[lv][2] += w[g] * tau[5] * 0.5 * (1 - eta);
+            cs[lv][3] += w[g] * tau[5] * 0.5 * (1 + eta);
+            for (int s = 0; s < 4; ++s)
+                cs[lv][4 + s] += w[g] * tau[2] * 0.25 * (1 + xi * kCorner[s][0]) * (1 + eta * kCorner[s][1]);
+        }
+        res.failedEnd = failed;
+        res.dissipation = dsum;
+        res.released = rsum;
+        res.strain = usum;
+        return true;
+    };
+    // The enhanced Newton (and, when a point fails in it, the pass without enhanced modes),
+    // then the commit. When every point answered elastically the residual is affine in
+    // alpha and the first correction zeroes it: the commit is tried at once, and when a
+    // point yields there the Newton goes on from that alpha as it would have.
+    int iter = 0;
+    for (;;) {
+        bool died = false, tryFast = false;
+        for (;;) {
             double r[7] = {0, 0, 0, 0, 0, 0, 0}, kaa[28];
             for (int k = 0; k < 28; ++k) kaa[k] = 0.0;
             died = false;
+            bool linear = true;
 #pragma unroll 1
             for (int g = 0; g < 8; ++g) {
                 double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
@@ -417,6 +483,7 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
                 for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
                 const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
                 died = died || (fl & 2);
+                linear = linear && fl == 0;
                 if (!eas) continue;
                 double ts[4], nv[4], M[10];
                 for (int q = 0; q < 4; ++q) {
@@ -482,57 +549,21 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
             for (int k = 0; k < 7; ++k) work += del[k] * r[k];
             if (fabs(work) <= 1e-16 * scale) break;
             for (int k = 0; k < 7; ++k) alpha[k] += del[k];
+            ++iter;
+            if (linear && iter == 1) {
+                tryFast = true;
+                break;
+            }
         }
-        if (!(died && eas)) break;
-        eas = 0;
-        for (int k = 0; k < 7; ++k) alpha[k] = 0.0;
+        if (!tryFast && died && eas) {
+            // a point failed in this step: rerun it without the enhanced modes
+            eas = 0;
+            for (int k = 0; k < 7; ++k) alpha[k] = 0.0;
+            continue;
+        }
+        if (commit(tryFast)) break;
     }
-    // commit at the final alpha; the force through the same rows, conjugate to the basis
-    // coefficients (F), the assumed rows through their samples
-    double F[21], cs[2][8];
-    for (int j = 0; j < 21; ++j) F[j] = 0.0;
-    for (int l = 0; l < 2; ++l)
-        for (int s = 0; s < 8; ++s) cs[l][s] = 0.0;
-#pragma unroll 1
-    for (int g = 0; g < 8; ++g) {
-        double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
-        strainAt(g, eas, st);
-        for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
-        const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
-        for (int k = 0; k < 8; ++k) hist[(8 * g + k) * stride] = h[k];
-        res.failedEnd += (fl & 4) != 0;
-        res.dissipation += w[g] * dg;
-        res.released += w[g] * rel;
-        if (wantStrain) {
-            const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
-            double u = 0.0;
-            for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * m.mu) + pm / (3.0 * m.kappa));
-            for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / m.mu);
-            res.strain += w[g] * u;
-        }
-        if (fl & 4) continue;  // a failed point carries no stress
-        const double xi = gauss(g, 0), eta = gauss(g, 1), zeta = gauss(g, 2);
-        double J[3][3], A[3][3], tau[6];
-        jacobian(cX, xi, eta, zeta, J);
-        inverse3(J, A);
-        pull(A, sig, tau);
-        // rows E11, E22, 2E12 at the point: tau_0 g_0 + tau_3 g_1 against d phi / d xi and
-        // tau_1 g_1 + tau_3 g_0 against d phi / d eta
-        double p0[3], p1[3];
-        for (int i = 0; i < 3; ++i) {
-            p0[i] = w[g] * (tau[0] * J[i][0] + tau[3] * J[i][1]);
-            p1[i] = w[g] * (tau[1] * J[i][1] + tau[3] * J[i][0]);
-        }
-        addGrad(F, p0, 0, xi, eta, zeta);
-        addGrad(F, p1, 1, xi, eta, zeta);
-        const int lv = (g >> 2) & 1;
-        cs[lv][0] += w[g] * tau[4] * 0.5 * (1 - xi);
-        cs[lv][1] += w[g] * tau[4] * 0.5 * (1 + xi);
-        cs[lv][2] += w[g] * tau[5] * 0.5 * (1 - eta);
-        cs[lv][3] += w[g] * tau[5] * 0.5 * (1 + eta);
-        for (int s = 0; s < 4; ++s)
-            cs[lv][4 + s] += w[g] * tau[2] * 0.25 * (1 + xi * kCorner[s][0]) * (1 + eta * kCorner[s][1]);
-    }
+    // the assumed rows through their samples
 #pragma unroll 1
     for (int lv = 0; lv < 2; ++lv) {
         const double zeta = lv ? 0.57735026918962576451 : -0.57735026918962576451;
