@@ -263,10 +263,12 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
         tr[i] = 2.0 * mu * (el[i] - vol / 3.0);
         tr[3 + i] = mu * el[3 + i];
     }
-    const double nrm = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
-                            2.0 * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
+    const double n2 = tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
+                      2.0 * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]);
     const double root23 = 0.81649658092772603273;
     const double radius = root23 * (m.yield + m.hard * fmax(h0[6], 0.0));
+    // inside the surface by the squared norms first: an elastic point takes no root
+    const double nrm = n2 > radius * radius ? sqrt(n2) : 0.0;
     const double excess = nrm - radius;
     if (!(excess > 0.0)) {
         for (int i = 0; i < 3; ++i) {
@@ -276,8 +278,9 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
         return 0;
     }
     const double gamma = excess / (2.0 * mu + (2.0 / 3.0) * m.hard);
+    const double inrm = 1.0 / nrm;
     double dep[6];
-    for (int i = 0; i < 6; ++i) n[i] = tr[i] / nrm;
+    for (int i = 0; i < 6; ++i) n[i] = tr[i] * inrm;
     for (int i = 0; i < 3; ++i) {
         dep[i] = gamma * n[i];
         dep[3 + i] = 2.0 * gamma * n[3 + i];
@@ -520,7 +523,8 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
             for (int k = 0; k < 7; ++k) nrm += r[k] * r[k];
             if (sqrt(nrm) <= 1e-12 * scale) break;
             if (iter + 1 >= 40) break;
-            double L[28];
+            // Cholesky with the pivots' reciprocals (one division per column)
+            double L[28], id[7];
             bool ok = true;
             for (int i = 0; i < 7; ++i)
                 for (int j = 0; j <= i; ++j) {
@@ -529,8 +533,9 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
                     if (i == j) {
                         ok = ok && t > 0.0;
                         L[i * (i + 1) / 2 + i] = sqrt(fmax(t, 1e-300));
+                        id[i] = 1.0 / L[i * (i + 1) / 2 + i];
                     } else {
-                        L[i * (i + 1) / 2 + j] = t / L[j * (j + 1) / 2 + j];
+                        L[i * (i + 1) / 2 + j] = t * id[j];
                     }
                 }
             if (!ok) break;
@@ -538,12 +543,12 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
             for (int i = 0; i < 7; ++i) {
                 double t = -r[i];
                 for (int k = 0; k < i; ++k) t -= L[i * (i + 1) / 2 + k] * del[k];
-                del[i] = t / L[i * (i + 1) / 2 + i];
+                del[i] = t * id[i];
             }
             for (int i = 6; i >= 0; --i) {
                 double t = del[i];
                 for (int k = i + 1; k < 7; ++k) t -= L[k * (k + 1) / 2 + i] * del[k];
-                del[i] = t / L[i * (i + 1) / 2 + i];
+                del[i] = t * id[i];
             }
             double work = 0.0;
             for (int k = 0; k < 7; ++k) work += del[k] * r[k];
