@@ -47,6 +47,15 @@ bool blockOk(int x) { return x == 64 || x == 128 || x == 256 || x == 512; }
 }  // namespace
 
 namespace shipfem {
+// The default launch of a plastic hex plan. Its element pass is bound by FP64 arithmetic,
+// not memory, and its cost per element varies (yielding elements iterate): 64-element
+// blocks dealt round-robin over the XCDs balance it better than XCD-contiguous blocks
+// (0.573 against 0.606 ms a pass on a 720,000-element punched zone).
+void plasticHexShape(LaunchShape& sh) {
+    sh.elem_block = 64;
+    sh.xcd_remap = 0;
+}
+
 // A launch configuration from the C ABI: zero fields keep the defaults. Returns false
 // for a value the kernels are not built for.
 bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out) {
@@ -161,6 +170,7 @@ shipfem_status createSolver(const shipfem_case_desc* desc, bool hex, shipfem_sol
     }
     if (e == hipSuccess) {
         s->plan.recs = {s->dc.rec};
+        if (hex && desc->material.plastic) plasticHexShape(s->plan.shape);
         e = s->plan.build(s->stream);
     }
     std::vector<double> lmax;
@@ -405,6 +415,8 @@ shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
         delete b;
         return SHIPFEM_INVALID_ARG;
     }
+    if (hex && cases[0].material.plastic && !(cfg && (cfg->elem_block || cfg->node_block)))
+        plasticHexShape(b->plan.shape);
     std::vector<uint64_t> off(n + 1, 0);
     for (uint32_t i = 0; i < n; ++i) off[i + 1] = off[i] + (arenaBytes(b->hcs[i]) + 4095) / 4096 * 4096;
     b->bytes = off[n];
