@@ -520,6 +520,12 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
  * to whole steps). Refuses (SHIPFEM_BUSY) a case whose footprint exceeds the budget. */
 shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc* desc,
                                      double run_to_time, uint64_t* request_id);
+/* shipfem_server_submit for a solid-shell hex case, read as shipfem_hex_solver_create
+ * reads it (elastic or plastic). Rounds run one element kind at a time (tets, elastic
+ * hexes, plastic hexes, in turn). The footprint is measured after the case's set-up: one
+ * that can never fit the budget completes with SHIPFEM_BUSY in its result. */
+shipfem_status shipfem_server_submit_hex(shipfem_server* s, const shipfem_case_desc* desc,
+                                         double run_to_time, uint64_t* request_id);
 /* shipfem_server_submit of an encoded case (the wire format above). */
 shipfem_status shipfem_server_submit_encoded(shipfem_server* s, const void* buf, uint64_t len,
                                              double run_to, uint64_t* request_id);

@@ -465,6 +465,7 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
                           static_cast<uint32_t>(uint64_t(T) * (t + 1) / threads));
     for (auto& th : pool) th.join();
     hc.hex_omega = 0;
+    hc.hvol = volume;
     for (uint32_t e = 0; e < T; ++e) {
         if (bad[e]) return "hex element " + std::to_string(e) + " is degenerate or inverted";
         hc.hex_omega = std::max(hc.hex_omega, omega[e]);

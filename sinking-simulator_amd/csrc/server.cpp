@@ -94,9 +94,16 @@ struct Request {
     std::vector<double> X, load, presc, u0, v0;
     std::vector<uint32_t> index, flags;
     shipfem_case_desc desc{};
+    bool hex = false;  // a solid-shell case: 8 node ids per element
     HostCase hc;
     std::string error;
 };
+
+// Rounds run one element kind at a time: tets, elastic hexes, plastic hexes.
+enum { kKindTet = 0, kKindHex = 1, kKindHexPlastic = 2, kKinds = 3 };
+int kindOf(const HostCase& hc) {
+    return hc.corners != 8 ? kKindTet : (hc.features & kFeatPlastic) ? kKindHexPlastic : kKindHex;
+}
 
 struct Active {
     std::unique_ptr<Request> req;
@@ -112,7 +119,7 @@ struct Active {
 // One launched round: its plan, members, timing events and progress buffers.
 struct Round {
     uint64_t id = 0;
-    int plan = 0;
+    int kind = 0, plan = 0;
     std::vector<Active*> members;
     hipEvent_t start = nullptr, stop = nullptr, done = nullptr;
     int64_t* d_prog = nullptr;  // [2 * members] device
@@ -151,10 +158,13 @@ struct shipfem_server {
 
     // owned by the loop thread
     std::vector<std::unique_ptr<Active>> active;
-    Plan plans[2];
-    std::vector<uint64_t> plan_sig[2];  // request ids each plan's graph was captured for
-    int next_plan = 0;
-    Plan omega_plan;                    // stable steps of newcomers
+    // per element kind, two plans used in turn, so a round's graph is never rebuilt while
+    // the kind's previous round is in flight
+    Plan plans[kKinds][2];
+    std::vector<uint64_t> plan_sig[kKinds][2];  // request ids each graph was captured for
+    int next_plan[kKinds] = {0, 0, 0};
+    int next_kind = 0;
+    Plan omega_plan;                    // stable steps of tet newcomers
     Round slots[2];
     std::deque<int> inflight;           // slot indices, oldest first
     uint64_t rounds_launched = 0;
@@ -163,7 +173,7 @@ struct shipfem_server {
     void loopMain();
     bool admit();
     void finish(Active& a, hipError_t fault);
-    hipError_t launchRound(const std::vector<Active*>& members);
+    hipError_t launchRound(const std::vector<Active*>& members, int kind);
     hipError_t completeOldest();
     void failAll(hipError_t e);
     // Device objects: waits for both streams, then frees (safe on partial creation).
@@ -173,8 +183,8 @@ struct shipfem_server {
 void shipfem_server::release() {
     if (run) (void)hipStreamSynchronize(run);
     if (side) (void)hipStreamSynchronize(side);
-    plans[0].dropGraph();
-    plans[1].dropGraph();
+    for (auto& k : plans)
+        for (Plan& p : k) p.dropGraph();
     active.clear();
     for (Round& r : slots) {
         if (r.d_prog) (void)hipFree(r.d_prog);
@@ -199,7 +209,7 @@ void shipfem_server::workerMain() {
             r = std::move(raw.front());
             raw.pop_front();
         }
-        r->error = prepareCase(r->desc, r->hc, 512);
+        r->error = r->hex ? prepareHexCase(r->desc, r->hc, 512) : prepareCase(r->desc, r->hc, 512);
         // the raw copies are no longer needed; the prepared case holds what it uses
         std::vector<double>().swap(r->load);
         std::vector<double>().swap(r->presc);
@@ -257,7 +267,8 @@ bool shipfem_server::admit() {
         }
     }
     if (take.empty()) return false;
-    // upload, then the FP64 stable step of the newcomers in one pass
+    // upload, then the FP64 stable step of the tet newcomers in one pass (a hex case's
+    // comes from its host set-up)
     std::vector<std::unique_ptr<Active>> add(take.size());
     omega_plan.recs.clear();
     hipError_t e = hipSuccess;
@@ -266,11 +277,12 @@ bool shipfem_server::admit() {
         add[k]->off = offs[k];
         if (e == hipSuccess) e = uploadCase(take[k]->hc, pool.base() + offs[k], add[k]->dc, side);
         if (e == hipSuccess) e = hipStreamSynchronize(side);
-        omega_plan.recs.push_back(add[k]->dc.rec);
+        if (!take[k]->hex) omega_plan.recs.push_back(add[k]->dc.rec);
     }
     std::vector<double> lmax;
-    if (e == hipSuccess) e = omega_plan.build(side);
-    if (e == hipSuccess) e = omega_plan.omegaMax(lmax, side);
+    if (e == hipSuccess && !omega_plan.recs.empty()) e = omega_plan.build(side);
+    if (e == hipSuccess && !omega_plan.recs.empty()) e = omega_plan.omegaMax(lmax, side);
+    size_t tet_index = 0;
     const double now = nowSeconds();
     size_t admitted = 0;
     for (size_t k = 0; k < take.size(); ++k) {
@@ -288,7 +300,7 @@ bool shipfem_server::admit() {
             ++st.failed;
             continue;
         }
-        a.dc.cfl = 2.0 / std::sqrt(4.0 * lmax[k] / hc.rho);
+        a.dc.cfl = take[k]->hex ? 2.0 / hc.hex_omega : 2.0 / std::sqrt(4.0 * lmax[tet_index++] / hc.rho);
         setTimestep(a.dc, hc, hc.dt_request > 0 ? hc.dt_request : hc.dt_safety * a.dc.cfl);
         a.steps = std::max<int64_t>(1, static_cast<int64_t>(std::ceil(take[k]->run_to / a.dc.dt - 1e-9)));
         a.dc.rec.step_end = a.steps;
@@ -348,7 +360,7 @@ void shipfem_server::finish(Active& a, hipError_t fault) {
                 for (uint32_t t = 0; t < hc.T; ++t)  // dmg is in the caller's order, vol internal
                     if (dmg[externalTet(hc, t)] >= 1.0f) {
                         ++res.torn_elements;
-                        res.torn_volume += hc.vol[t];
+                        res.torn_volume += hc.corners == 8 ? hc.hvol[t] : hc.vol[t];
                     }
         }
         for (int32_t r = 0; r < nrec; ++r) {
@@ -378,25 +390,26 @@ void shipfem_server::finish(Active& a, hipError_t fault) {
     else ++st.failed;
 }
 
-// Captures (or reuses) the graph for `members` on the plan not used by the round in
-// flight, then queues the round, its progress read-out and its events on the run stream.
-hipError_t shipfem_server::launchRound(const std::vector<Active*>& members) {
+// Captures (or reuses) the graph for `members` (all of one element kind) on the kind's plan
+// not used by its round in flight, then queues the round, its progress read-out and its
+// events on the run stream.
+hipError_t shipfem_server::launchRound(const std::vector<Active*>& members, int kind) {
     const int R = opts.chunk;
-    const int pi = next_plan;
-    Plan& p = plans[pi];
+    const int pi = next_plan[kind];
+    Plan& p = plans[kind][pi];
     std::vector<uint64_t> sig;
     sig.reserve(members.size());
     for (const Active* a : members) sig.push_back(a->req->id);
     hipError_t e = hipSuccess;
-    if (sig != plan_sig[pi] || !p.graph) {
-        plan_sig[pi].clear();
+    if (sig != plan_sig[kind][pi] || !p.graph) {
+        plan_sig[kind][pi].clear();
         p.recs.clear();
         for (const Active* a : members) p.recs.push_back(a->dc.rec);
         p.parity = 0;
         e = p.build(side);
         if (e == hipSuccess) e = p.capture(R, side);
         if (e != hipSuccess) return e;
-        plan_sig[pi] = std::move(sig);
+        plan_sig[kind][pi] = std::move(sig);
     }
     int slot = 0;
     while (std::find(inflight.begin(), inflight.end(), slot) != inflight.end()) ++slot;
@@ -415,6 +428,7 @@ hipError_t shipfem_server::launchRound(const std::vector<Active*>& members) {
         r.cap = cap;
     }
     r.id = ++rounds_launched;
+    r.kind = kind;
     r.plan = pi;
     r.members = members;
     e = hipEventRecord(r.start, run);
@@ -432,7 +446,7 @@ hipError_t shipfem_server::launchRound(const std::vector<Active*>& members) {
         a->last_round = r.id;
     }
     inflight.push_back(slot);
-    next_plan ^= 1;
+    next_plan[kind] ^= 1;
     return hipSuccess;
 }
 
@@ -478,8 +492,8 @@ void shipfem_server::failAll(hipError_t e) {
     inflight.clear();
     for (auto& a : active) finish(*a, e);
     active.clear();
-    plan_sig[0].clear();
-    plan_sig[1].clear();
+    for (auto& k : plan_sig)
+        for (auto& sg : k) sg.clear();
     std::lock_guard<std::mutex> lk(m);
     st.active = 0;
     cv_done.notify_all();
@@ -503,11 +517,18 @@ void shipfem_server::loopMain() {
         admit();
         bool launched = false;
         if (inflight.size() < 2) {
+            // the next element kind in turn that has cases to advance
             std::vector<Active*> members;
-            for (auto& a : active)
-                if (!a->dead && a->predicted < a->steps) members.push_back(a.get());
+            int kind = next_kind;
+            for (int t = 0; t < kKinds && members.empty(); ++t) {
+                kind = (next_kind + t) % kKinds;
+                for (auto& a : active)
+                    if (!a->dead && a->predicted < a->steps && kindOf(a->req->hc) == kind)
+                        members.push_back(a.get());
+            }
             if (!members.empty()) {
-                const hipError_t e = launchRound(members);
+                next_kind = (kind + 1) % kKinds;
+                const hipError_t e = launchRound(members, kind);
                 if (e != hipSuccess) {
                     failAll(e);
                     continue;
@@ -546,7 +567,7 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
         delete s;
         return SHIPFEM_INVALID_ARG;
     }
-    if (!shapeFrom(&s->opts.launch, s->plans[0].shape)) {
+    if (!shapeFrom(&s->opts.launch, s->plans[0][0].shape)) {
         setError(err, err_len, "launch config: blocks must be 64/128/256/512, variant TILED or SLOTS");
         delete s;
         return SHIPFEM_INVALID_ARG;
@@ -571,8 +592,14 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
         delete s;
         return st;
     }
-    s->plans[1].shape = s->plans[0].shape;
-    s->omega_plan.shape = s->plans[0].shape;
+    // the hex kinds take the hex plans' defaults unless the caller chose a launch
+    const bool chosen = s->opts.launch.elem_block || s->opts.launch.node_block;
+    for (int k = 0; k < kKinds; ++k)
+        for (int i = 0; i < 2; ++i) {
+            s->plans[k][i].shape = s->plans[0][0].shape;
+            if (k != kKindTet && !chosen) hexShape(s->plans[k][i].shape, k == kKindHexPlastic);
+        }
+    s->omega_plan.shape = s->plans[0][0].shape;
     s->st.bytes_budget = s->opts.memory_budget;
     const unsigned nworkers = std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
     for (unsigned w = 0; w < nworkers; ++w) s->workers.emplace_back([s] { s->workerMain(); });
@@ -581,22 +608,29 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
     return SHIPFEM_OK;
 }
 
-shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc* d, double run_to,
-                                     uint64_t* request_id) {
+namespace {
+
+shipfem_status submitCase(shipfem_server* s, const shipfem_case_desc* d, double run_to,
+                          uint64_t* request_id, bool hex) {
     if (!s || !d || !(run_to > 0) || !d->rest_position || !d->index || d->node_count == 0 ||
         d->tet_count == 0)
         return SHIPFEM_INVALID_ARG;
-    const int32_t kind = d->material.plastic ? (d->material.failure_strain > 0.0 ? 2 : 1) : 0;
-    const uint64_t bound = shipfem_case_footprint(d->node_count, d->tet_count, kind, 512);
-    if (bound > s->opts.memory_budget) {
-        std::lock_guard<std::mutex> lk(s->m);
-        ++s->st.refused;
-        return SHIPFEM_BUSY;
+    // a tet case is refused at once when its footprint bound exceeds the budget; a hex
+    // case is measured after its set-up (a result with SHIPFEM_BUSY when it can never fit)
+    if (!hex) {
+        const int32_t kind = d->material.plastic ? (d->material.failure_strain > 0.0 ? 2 : 1) : 0;
+        const uint64_t bound = shipfem_case_footprint(d->node_count, d->tet_count, kind, 512);
+        if (bound > s->opts.memory_budget) {
+            std::lock_guard<std::mutex> lk(s->m);
+            ++s->st.refused;
+            return SHIPFEM_BUSY;
+        }
     }
     auto r = std::make_unique<Request>();
     const uint64_t N = d->node_count, T = d->tet_count;
+    r->hex = hex;
     r->X.assign(d->rest_position, d->rest_position + 3 * N);
-    r->index.assign(d->index, d->index + 4 * T);
+    r->index.assign(d->index, d->index + (hex ? 8 : 4) * T);
     if (d->node_flags) r->flags.assign(d->node_flags, d->node_flags + N);
     if (d->load) r->load.assign(d->load, d->load + 3 * N);
     if (d->prescribed) r->presc.assign(d->prescribed, d->prescribed + 3 * N);
@@ -621,6 +655,18 @@ shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc*
     ++s->st.submitted;
     s->cv_raw.notify_one();
     return SHIPFEM_OK;
+}
+
+}  // namespace
+
+shipfem_status shipfem_server_submit(shipfem_server* s, const shipfem_case_desc* d, double run_to,
+                                     uint64_t* request_id) {
+    return submitCase(s, d, run_to, request_id, false);
+}
+
+shipfem_status shipfem_server_submit_hex(shipfem_server* s, const shipfem_case_desc* d,
+                                         double run_to, uint64_t* request_id) {
+    return submitCase(s, d, run_to, request_id, true);
 }
 
 shipfem_status shipfem_server_poll(shipfem_server* s, shipfem_result* out, uint32_t max,

@@ -41,6 +41,7 @@ struct HostCase {
     std::vector<double> hgeod;    // plastic hex: [kHexGeo][Tpad]
     std::vector<float> hgeo;      // hex: [kHexGeo][Tpad]
     double hex_omega = 0;         // hex: largest element frequency (FP64, host)
+    std::vector<double> hvol;     // hex: [T] element volumes (host only)
     std::vector<float> dmi;       // [9][Tpad]
     std::vector<float> vol;       // [Tpad]
     std::vector<float> mass, invm;  // [Npad]
@@ -168,6 +169,8 @@ hipError_t readTears(const DeviceCase& dc, const HostCase& hc, std::vector<shipf
 void maxPrincipalDirection(const double s6[6], double out[3]);
 
 bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out);
+// The default launch of a hex plan (plastic: the plastic pass's block).
+void hexShape(LaunchShape& sh, bool plastic);
 void setError(char* err, size_t len, const std::string& msg);
 shipfem_status statusOf(hipError_t e);
 bool deviceAvailable(std::string* why);

@@ -196,6 +196,7 @@ SIGNATURES = {
     "shipfem_batch_destroy": (None, [VP]),
     "shipfem_server_create": (S, [C.POINTER(ServerOpts), PP, C.c_char_p, C.c_size_t]),
     "shipfem_server_submit": (S, [VP, C.POINTER(CaseDesc), C.c_double, C.POINTER(C.c_uint64)]),
+    "shipfem_server_submit_hex": (S, [VP, C.POINTER(CaseDesc), C.c_double, C.POINTER(C.c_uint64)]),
     "shipfem_server_poll": (S, [VP, C.POINTER(Result), C.c_uint32, C.POINTER(C.c_uint32)]),
     "shipfem_server_keep": (S, [VP, C.c_uint64]),
     "shipfem_server_fetch": (S, [VP, C.c_uint64, C.POINTER(StateView)]),
@@ -873,7 +874,8 @@ class Server:
     def submit(self, case, run_to, keep=False):
         d = case.desc()
         rid = C.c_uint64()
-        _check(lib().shipfem_server_submit(self.h, C.byref(d), run_to, C.byref(rid)), "submit")
+        fn = lib().shipfem_server_submit_hex if case.corners == 8 else lib().shipfem_server_submit
+        _check(fn(self.h, C.byref(d), run_to, C.byref(rid)), "submit")
         if keep:
             _check(lib().shipfem_server_keep(self.h, rid.value))
         return rid.value

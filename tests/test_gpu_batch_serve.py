@@ -205,3 +205,30 @@ def test_server_reports_a_bad_case_and_keeps_serving():
     srv, ids, res, _ = serve([good, bad, good], 1e-4)
     assert res[ids[0]]["status"] == sf.OK and res[ids[2]]["status"] == sf.OK
     assert res[ids[1]]["status"] == sf.INVALID_ARG
+
+
+def test_server_serves_hex_cases_beside_tets():
+    # rounds run one element kind at a time (tets, elastic hexes, plastic hexes in turn);
+    # a served hex case ends where the single-case solver takes it, bit for bit
+    plastic = dict(plastic=True, yield_=355e6, hardening=1e9)
+    hexes = [configs.hex_zone(lx=0.3, ly=0.15, nx=24, ny=12, radius=0.04, speed=5.0),
+             configs.hex_zone(lx=0.3, ly=0.15, nx=24, ny=12, radius=0.04, speed=30.0,
+                              failure=dict(failure_strain=0.02), **plastic),
+             configs.hex_zone(lx=0.2, ly=0.1, nx=16, ny=8, radius=0.03, speed=8.0, **plastic)]
+    cases = [small_panel(nx=20), hexes[0], hexes[1], small_panel(speed=4.0, nx=16), hexes[2]]
+    run_to = 6e-4
+    print("server mix:", [(type(c).__name__, c.T) for c in cases], flush=True)
+    srv, ids, res, kept = serve(cases, run_to, keep=set(range(len(cases))))
+    assert all(res[i]["status"] == sf.OK for i in ids), [res[i]["status"] for i in ids]
+    assert srv.stats()["completed"] == len(cases)
+    for k in (1, 2, 4):
+        c, a = cases[k], kept[ids[k]]
+        s = sf.Solver(c)
+        assert a.time / a.step == pytest.approx(s.timestep()["dt"], rel=1e-15)
+        s.run(a.step)
+        st = s.readback()
+        assert np.array_equal(a.u, st.u) and np.array_equal(a.eqps, st.eqps), k
+        torn = int(np.sum(st.damage >= 1.0)) if st.damage is not None else 0
+        assert res[ids[k]]["torn_elements"] == torn, k
+    assert res[ids[2]]["torn_elements"] > 0 and res[ids[2]]["max_eqps"] > 0.0
+    assert res[ids[4]]["max_eqps"] > 0.0 and res[ids[1]]["max_eqps"] == 0.0
