@@ -347,7 +347,8 @@ uint64_t shipfem_apply_edge_drive(uint32_t node_count, const double* displacemen
                                   double* prescribed);
 
 /* ---- case wire format ------------------------------------------------------- */
-/* One self-checking little-endian blob per tet case (4 node ids per element): the on-disk
+/* One self-checking little-endian blob per case (tets, or solid-shell hexes through the
+ * shipfem_hex_case_* calls, marked in the header): the on-disk
  * case file and the form a case travels in to the server. A 64-byte header (magic "SHIPFEMC", version, node and
  * tet counts, which optional arrays follow, payload size, FNV-1a 64 of the payload),
  * the scalar block, then rest positions, connectivity and the optional arrays of the
@@ -367,6 +368,12 @@ void shipfem_case_blob_destroy(shipfem_case_blob* b);
 shipfem_status shipfem_case_save(const shipfem_case_desc* d, const char* path);
 shipfem_status shipfem_case_load(const char* path, shipfem_case_blob** out,
                                  shipfem_case_desc* d);
+/* The same for a solid-shell hex case (8 node ids per element); decode and load read
+ * either kind, and shipfem_case_blob_corners tells which (4 or 8). */
+uint64_t shipfem_hex_case_encoded_size(const shipfem_case_desc* d);
+shipfem_status shipfem_hex_case_encode(const shipfem_case_desc* d, void* buf, uint64_t len);
+shipfem_status shipfem_hex_case_save(const shipfem_case_desc* d, const char* path);
+uint32_t shipfem_case_blob_corners(const shipfem_case_blob* b);
 
 /* ---- the solid-shell hexahedron (engine/sim/solid_shell.hpp:111-194) --------- */
 /* An 8-node hex whose nodes 0-3 are the zeta = -1 face and 4-7 the zeta = +1 face,

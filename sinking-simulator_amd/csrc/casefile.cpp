@@ -1,7 +1,7 @@
 // casefile.cpp -- the case wire format: one self-checking little-endian blob per case,
 // written to disk or handed to the server. A header (magic, version, counts, which
-// optional arrays are present, payload size, FNV-1a 64 of the payload), then the scalar
-// block, then the arrays in a fixed order.
+// optional arrays are present and whether the elements are solid-shell hexes, payload
+// size, FNV-1a 64 of the payload), then the scalar block, then the arrays in a fixed order.
 #include "shipfem.h"
 #include "shipfem_host.hpp"
 
@@ -23,7 +23,9 @@ enum Present : uint32_t {
     kHasPrescribed = 1u << 2,
     kHasU0 = 1u << 3,
     kHasV0 = 1u << 4,
+    kHexElements = 1u << 5,  // solid-shell hexes: 8 node ids per element
 };
+constexpr uint32_t kPresentMask = 0x3Fu;
 
 struct Header {
     char magic[8];
@@ -58,19 +60,27 @@ uint64_t fnv1a(const unsigned char* p, uint64_t n) {
     return h;
 }
 
-uint32_t presentOf(const shipfem_case_desc& d) {
+uint32_t presentOf(const shipfem_case_desc& d, bool hex) {
     return (d.node_flags ? kHasFlags : 0u) | (d.load ? kHasLoad : 0u) |
            (d.prescribed ? kHasPrescribed : 0u) | (d.initial_displacement ? kHasU0 : 0u) |
-           (d.initial_velocity ? kHasV0 : 0u);
+           (d.initial_velocity ? kHasV0 : 0u) | (hex ? kHexElements : 0u);
 }
 
 uint64_t payloadBytes(uint64_t N, uint64_t T, uint32_t present) {
-    uint64_t b = sizeof(Scalars) + 24 * N + 16 * T;
+    uint64_t b = sizeof(Scalars) + 24 * N + ((present & kHexElements) ? 32 : 16) * T;
     if (present & kHasFlags) b += 4 * N;
     for (uint32_t bit : {kHasLoad, kHasPrescribed, kHasU0, kHasV0})
         if (present & bit) b += 24 * N;
     return b;
 }
+
+uint64_t encodedSize(const shipfem_case_desc* d, bool hex) {
+    if (!d) return 0;
+    return sizeof(Header) + payloadBytes(d->node_count, d->tet_count, presentOf(*d, hex));
+}
+
+shipfem_status encode(const shipfem_case_desc* d, void* buf, uint64_t len, bool hex);
+shipfem_status save(const shipfem_case_desc* d, const char* path, bool hex);
 
 }  // namespace
 
@@ -78,19 +88,15 @@ uint64_t payloadBytes(uint64_t N, uint64_t T, uint32_t present) {
 struct shipfem_case_blob {
     std::vector<double> X, load, prescribed, u0, v0;
     std::vector<uint32_t> index, flags;
+    uint32_t corners = 4;
 };
 
-extern "C" {
+namespace {
 
-uint64_t shipfem_case_encoded_size(const shipfem_case_desc* d) {
-    if (!d) return 0;
-    return sizeof(Header) + payloadBytes(d->node_count, d->tet_count, presentOf(*d));
-}
-
-shipfem_status shipfem_case_encode(const shipfem_case_desc* d, void* buf, uint64_t len) {
+shipfem_status encode(const shipfem_case_desc* d, void* buf, uint64_t len, bool hex) {
     if (!d || !buf || !d->rest_position || !d->index || d->node_count == 0 || d->tet_count == 0)
         return SHIPFEM_INVALID_ARG;
-    const uint64_t need = shipfem_case_encoded_size(d);
+    const uint64_t need = encodedSize(d, hex);
     if (len < need) return SHIPFEM_INVALID_ARG;
     const uint64_t N = d->node_count, T = d->tet_count;
     Header h{};
@@ -98,7 +104,7 @@ shipfem_status shipfem_case_encode(const shipfem_case_desc* d, void* buf, uint64
     h.version = kVersion;
     h.node_count = d->node_count;
     h.tet_count = d->tet_count;
-    h.present = presentOf(*d);
+    h.present = presentOf(*d, hex);
     h.payload = need - sizeof(Header);
     const shipfem_material& m = d->material;
     const shipfem_indenter& ind = d->indenter;
@@ -136,7 +142,7 @@ shipfem_status shipfem_case_encode(const shipfem_case_desc* d, void* buf, uint64
     };
     put(&s, sizeof s);
     put(d->rest_position, 24 * N);
-    put(d->index, 16 * T);
+    put(d->index, (hex ? 32 : 16) * T);
     if (d->node_flags) put(d->node_flags, 4 * N);
     if (d->load) put(d->load, 24 * N);
     if (d->prescribed) put(d->prescribed, 24 * N);
@@ -147,6 +153,37 @@ shipfem_status shipfem_case_encode(const shipfem_case_desc* d, void* buf, uint64
     return SHIPFEM_OK;
 }
 
+shipfem_status save(const shipfem_case_desc* d, const char* path, bool hex) {
+    if (!d || !path) return SHIPFEM_INVALID_ARG;
+    std::vector<unsigned char> buf(encodedSize(d, hex));
+    const shipfem_status st = encode(d, buf.data(), buf.size(), hex);
+    if (st != SHIPFEM_OK) return st;
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return SHIPFEM_INVALID_ARG;
+    const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return SHIPFEM_INVALID_ARG;
+    }
+    return SHIPFEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t shipfem_case_encoded_size(const shipfem_case_desc* d) { return encodedSize(d, false); }
+uint64_t shipfem_hex_case_encoded_size(const shipfem_case_desc* d) { return encodedSize(d, true); }
+
+shipfem_status shipfem_case_encode(const shipfem_case_desc* d, void* buf, uint64_t len) {
+    return encode(d, buf, len, false);
+}
+
+shipfem_status shipfem_hex_case_encode(const shipfem_case_desc* d, void* buf, uint64_t len) {
+    return encode(d, buf, len, true);
+}
+
 shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_blob** out,
                                    shipfem_case_desc* d) {
     if (!buf || !out || !d || len < sizeof(Header)) return SHIPFEM_INVALID_ARG;
@@ -154,7 +191,7 @@ shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_b
     Header h;
     std::memcpy(&h, buf, sizeof h);
     if (std::memcmp(h.magic, kMagic, 8) != 0 || h.version != kVersion || h.node_count == 0 ||
-        h.tet_count == 0 || (h.present & ~0x1Fu) != 0)
+        h.tet_count == 0 || (h.present & ~kPresentMask) != 0)
         return SHIPFEM_INVALID_ARG;
     const uint64_t N = h.node_count, T = h.tet_count;
     if (h.payload != payloadBytes(N, T, h.present) || len < sizeof(Header) + h.payload)
@@ -172,7 +209,8 @@ shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_b
         p += count * sizeof(vec[0]);
     };
     take(b->X, 3 * N);
-    take(b->index, 4 * T);
+    b->corners = (h.present & kHexElements) ? 8 : 4;
+    take(b->index, b->corners * T);
     if (h.present & kHasFlags) take(b->flags, N);
     if (h.present & kHasLoad) take(b->load, 3 * N);
     if (h.present & kHasPrescribed) take(b->prescribed, 3 * N);
@@ -223,20 +261,14 @@ shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_b
 
 void shipfem_case_blob_destroy(shipfem_case_blob* b) { delete b; }
 
+uint32_t shipfem_case_blob_corners(const shipfem_case_blob* b) { return b ? b->corners : 0; }
+
 shipfem_status shipfem_case_save(const shipfem_case_desc* d, const char* path) {
-    if (!d || !path) return SHIPFEM_INVALID_ARG;
-    std::vector<unsigned char> buf(shipfem_case_encoded_size(d));
-    const shipfem_status st = shipfem_case_encode(d, buf.data(), buf.size());
-    if (st != SHIPFEM_OK) return st;
-    const std::string tmp = std::string(path) + ".tmp";
-    FILE* f = std::fopen(tmp.c_str(), "wb");
-    if (!f) return SHIPFEM_INVALID_ARG;
-    const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
-    if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), path) != 0) {
-        std::remove(tmp.c_str());
-        return SHIPFEM_INVALID_ARG;
-    }
-    return SHIPFEM_OK;
+    return save(d, path, false);
+}
+
+shipfem_status shipfem_hex_case_save(const shipfem_case_desc* d, const char* path) {
+    return save(d, path, true);
 }
 
 shipfem_status shipfem_case_load(const char* path, shipfem_case_blob** out, shipfem_case_desc* d) {

@@ -690,7 +690,9 @@ shipfem_status shipfem_server_submit_encoded(shipfem_server* s, const void* buf,
     const shipfem_status st = shipfem_case_decode(buf, len, &blob, &d);
     if (st != SHIPFEM_OK) return st;
     // submit copies the case, so the decoded arrays can go at once
-    const shipfem_status r = shipfem_server_submit(s, &d, run_to, request_id);
+    const shipfem_status r = shipfem_case_blob_corners(blob) == 8
+                                 ? shipfem_server_submit_hex(s, &d, run_to, request_id)
+                                 : shipfem_server_submit(s, &d, run_to, request_id);
     shipfem_case_blob_destroy(blob);
     return r;
 }

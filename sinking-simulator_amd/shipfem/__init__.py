@@ -166,6 +166,10 @@ SIGNATURES = {
     "shipfem_case_decode": (S, [VP, C.c_uint64, PP, C.POINTER(CaseDesc)]),
     "shipfem_case_blob_destroy": (None, [VP]),
     "shipfem_case_save": (S, [C.POINTER(CaseDesc), C.c_char_p]),
+    "shipfem_hex_case_encoded_size": (C.c_uint64, [C.POINTER(CaseDesc)]),
+    "shipfem_hex_case_encode": (S, [C.POINTER(CaseDesc), VP, C.c_uint64]),
+    "shipfem_hex_case_save": (S, [C.POINTER(CaseDesc), C.c_char_p]),
+    "shipfem_case_blob_corners": (C.c_uint32, [VP]),
     "shipfem_case_load": (S, [C.c_char_p, PP, C.POINTER(CaseDesc)]),
     "shipfem_server_submit_encoded": (S, [VP, VP, C.c_uint64, C.c_double, C.POINTER(C.c_uint64)]),
     "shipfem_solver_destroy": (None, [VP]),
@@ -385,19 +389,24 @@ class Case:
         return d
 
     def encode(self):
-        """The case wire format (shipfem_case_encode): one self-checking blob."""
+        """The case wire format (shipfem_case_encode, shipfem_hex_case_encode for a hex
+        case): one self-checking blob."""
         d = self.desc()
-        n = lib().shipfem_case_encoded_size(C.byref(d))
+        hx = self.corners == 8
+        n = (lib().shipfem_hex_case_encoded_size if hx else lib().shipfem_case_encoded_size)(C.byref(d))
         buf = C.create_string_buffer(n)
-        _check(lib().shipfem_case_encode(C.byref(d), buf, n), "encode")
+        _check((lib().shipfem_hex_case_encode if hx else lib().shipfem_case_encode)(C.byref(d), buf, n),
+               "encode")
         return buf.raw[:n]
 
     @classmethod
-    def _from_desc(cls, d):
+    def _from_desc(cls, d, corners=4):
+        # the blob says which kind of case it holds
+        cls = HexCase if corners == 8 else Case
         N, T = d.node_count, d.tet_count
         arr = lambda ptr, n, dt_: None if not ptr else np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt_).copy()  # noqa: E731
         X = arr(d.rest_position, 3 * N, np.float64).reshape(N, 3)
-        index = arr(d.index, 4 * T, np.uint32).reshape(T, 4)
+        index = arr(d.index, corners * T, np.uint32).reshape(T, corners)
         vec3 = lambda ptr: None if not ptr else arr(ptr, 3 * N, np.float64).reshape(N, 3)  # noqa: E731
         m, ind = d.material, d.indenter
         failure = None
@@ -425,20 +434,21 @@ class Case:
         h, d = C.c_void_p(), CaseDesc()
         _check(lib().shipfem_case_decode(buf, len(blob), C.byref(h), C.byref(d)), "decode")
         try:
-            return cls._from_desc(d)
+            return cls._from_desc(d, lib().shipfem_case_blob_corners(h))
         finally:
             lib().shipfem_case_blob_destroy(h)
 
     def save(self, path):
         d = self.desc()
-        _check(lib().shipfem_case_save(C.byref(d), os.fsencode(path)), f"save {path}")
+        fn = lib().shipfem_hex_case_save if self.corners == 8 else lib().shipfem_case_save
+        _check(fn(C.byref(d), os.fsencode(path)), f"save {path}")
 
     @classmethod
     def load(cls, path):
         h, d = C.c_void_p(), CaseDesc()
         _check(lib().shipfem_case_load(os.fsencode(path), C.byref(h), C.byref(d)), f"load {path}")
         try:
-            return cls._from_desc(d)
+            return cls._from_desc(d, lib().shipfem_case_blob_corners(h))
         finally:
             lib().shipfem_case_blob_destroy(h)
 
@@ -458,16 +468,9 @@ class Case:
 class HexCase(Case):
     """A solid-shell hex case: index [T, 8], nodes 0-3 on the element's -zeta face and 4-7
     on its +zeta face (the thickness direction), each counter-clockwise seen from +zeta
-    (solid_shell.hpp:115-125). Elastic only."""
+    (solid_shell.hpp:115-125). Elastic, or plastic with `plastic` set."""
 
     corners = 8
-
-    def encode(self):
-        # the wire format's index block is 4 ids per element
-        raise NotImplementedError("the case wire format carries tet cases")
-
-    def save(self, path):
-        raise NotImplementedError("the case wire format carries tet cases")
 
     @classmethod
     def plate(cls, lx, ly, thickness, nx, ny, nz=1, pressure=0.0, **kw):

@@ -89,3 +89,27 @@ def test_caller_numbering_does_not_change_the_solve():
     w = abs(ea[-1]["w_contact"])
     assert max(abs(x["strain"] - y["strain"]) for x, y in zip(ea, eb)) <= 1e-6 * w
     assert ra.eqps.max() > 0
+
+
+def test_hex_case_in_the_wire_format_solves_with_the_same_bits():
+    # a plastic solid-shell zone, encoded: decoded onto a solver, and submitted encoded to
+    # the server, it ends where the original case does
+    c = configs.hex_zone(lx=0.3, ly=0.15, nx=24, ny=12, radius=0.04, speed=10.0, plastic=True,
+                         yield_=355e6, hardening=1e9, failure=dict(failure_strain=0.05))
+    blob = c.encode()
+    a, b = sf.Solver(c), sf.Solver(sf.Case.decode(blob))
+    a.run(300)
+    b.run(300)
+    sa, sb = a.readback(), b.readback()
+    assert np.array_equal(sa.u, sb.u) and np.array_equal(sa.eqps, sb.eqps)
+    srv = sf.Server(chunk=32)
+    rid = srv.submit_encoded(blob, 3e-4, keep=True)
+    assert srv.drain(120.0)
+    res = {r["request_id"]: r for r in srv.poll()}[rid]
+    assert res["status"] == sf.OK
+    st = srv.fetch(rid, c)
+    srv.close()
+    s = sf.Solver(c)
+    s.run(st.step)
+    ref = s.readback()
+    assert np.array_equal(st.u, ref.u) and np.array_equal(st.eqps, ref.eqps)

@@ -340,7 +340,26 @@ def test_hex_batch_refuses_mixed_elastic_and_plastic_before_touching_a_device():
     assert e.value.status == sf.INVALID_ARG
 
 
-def test_hex_cases_do_not_pass_through_the_tet_wire_format():
-    c = sf.HexCase.plate(0.2, 0.1, 0.01, 2, 2, 1)
-    with pytest.raises(NotImplementedError):
-        c.encode()
+def test_hex_cases_round_trip_through_the_wire_format(tmp_path):
+    # the header marks the hex kind: decode and load give back a HexCase with 8 ids per
+    # element and every field; a flipped payload byte fails the checksum
+    from shipfem import configs
+    c = configs.hex_zone(lx=0.3, ly=0.15, nx=6, ny=3, radius=0.04, speed=5.0, plastic=True,
+                         yield_=355e6, hardening=1e9, failure=dict(failure_strain=0.05))
+    blob = c.encode()
+    assert len(blob) == 64 + 216 + 24 * c.N + 32 * c.T + 4 * c.N
+    d = sf.Case.decode(blob)
+    assert isinstance(d, sf.HexCase) and d.index.shape == (c.T, 8)
+    assert np.array_equal(d.X, c.X) and np.array_equal(d.index, c.index)
+    assert np.array_equal(d.flags, c.flags) and d.plastic and d.yield_ == c.yield_
+    assert d.failure["failure_strain"] == 0.05 and d.indenter["radius"] == 0.04
+    assert d.encode() == blob
+    c.save(str(tmp_path / "zone.case"))
+    assert sf.Case.load(str(tmp_path / "zone.case")).encode() == blob
+    bad = bytearray(blob)
+    bad[64 + 216 + 8] ^= 1
+    with pytest.raises(sf.ShipfemError):
+        sf.Case.decode(bytes(bad))
+    # a tet case stays a tet case
+    t = sf.Case.decode(configs.plate(lx=0.2, ly=0.1, nx=4, ny=2).encode())
+    assert type(t) is sf.Case and t.index.shape[1] == 4
