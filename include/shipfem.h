@@ -109,6 +109,12 @@ typedef struct shipfem_case_desc {
     double dt_safety;    /* 0 => 0.9 */
     int32_t sample_every; /* energy-record period in steps, even; 0 => 100 */
     int64_t max_steps;   /* a case stops advancing at this step; 0 => unbounded */
+    /* [N] lumped nodal mass, kg, in place of density * volume / corners per element corner
+     * (the caller-owned TetMesh::mass of fem.hpp:42); or NULL. A node that is not fully
+     * pinned and lies in an element needs a positive mass. The default stable step is then
+     * the density bound times sqrt(min over such nodes of mass / (density share)), which
+     * bounds the lumped system's largest frequency from below. */
+    const double* mass;
 } shipfem_case_desc;
 
 /* Per-call step parameters of gpu::FemGpuSolver::run (fem_gpu.hpp:31-32). */
@@ -188,6 +194,29 @@ void shipfem_mesh_destroy(shipfem_mesh* m);
 double shipfem_edge_timestep(const double* rest_position, const uint32_t* index,
                              uint32_t tet_count, double youngs_modulus, double density,
                              double safety);
+
+/* ---- the host FP32 step (replaces sim::fem::tetForces and stepCpu) -------------- */
+/* tetForces (fem.hpp:84-86, fem.cpp:73-117): the four nodal forces [12] of tet `tet` at
+ * the current positions [3N], with rest_inv [9T] column-major (fem.cpp:10-16) and
+ * rest_volume [T] from computeRestState. With rest_position [3N] (the rest coordinates in
+ * double; may be NULL) the deformation gradient is formed from displacements, as the
+ * device pass forms it; without, from the positions. Co-rotational linear elastic, the
+ * rotation by four Higham iterations. */
+shipfem_status shipfem_cpu_tet_forces(const float* position, const double* rest_position,
+                                      const uint32_t* index, const float* rest_inv,
+                                      const float* rest_volume, uint64_t tet, float lambda, float mu,
+                                      float out[12]);
+/* stepCpu (fem.hpp:77-82, fem.cpp:187-215): one explicit step in place on the caller's
+ * TetMesh arrays. Every tet's forces go to tet_force [12T]; then per node, fixed (may be
+ * NULL) nonzero: velocity 0 and position held; otherwise f = its corner forces gathered in
+ * adjacency (CSR, entry = tet * 4 + corner) order minus gravity * mass along z, and
+ * v = (v + f dt / mass) damping, x += v dt. Single-threaded, deterministic. */
+shipfem_status shipfem_cpu_step(uint32_t node_count, uint32_t tet_count, float* position, float* velocity,
+                                const float* mass, const uint32_t* fixed, const uint32_t* index,
+                                const float* rest_inv, const float* rest_volume, float* tet_force,
+                                const uint32_t* adjacency_offset, const uint32_t* adjacency_entry,
+                                const double* rest_position, float lambda, float mu, float dt,
+                                float gravity, float damping);
 
 /* Element-pass variant. TILED (default): 256-tet tiles stage their nodes in LDS and
  * sum corner forces per tile node before one write; SLOTS: one 48-byte force slot per
@@ -542,6 +571,10 @@ shipfem_status shipfem_server_poll(shipfem_server* s, shipfem_result* out, uint3
                                    uint32_t* count);
 /* Ask that the final state of request_id be retained for shipfem_server_fetch. */
 shipfem_status shipfem_server_keep(shipfem_server* s, uint64_t request_id);
+/* The kept final state of a completed request, once: every field of the view that is
+ * not NULL (displacement, velocity, position X + u, eqps, plastic strain, damage), in the
+ * caller's node and element orders, as shipfem_solver_readback returns it.
+ * SHIPFEM_NOT_FOUND for a request not kept, not finished or already fetched. */
 shipfem_status shipfem_server_fetch(shipfem_server* s, uint64_t request_id,
                                     shipfem_state_view* view);
 /* Blocks until every submitted request has completed or `timeout_s` passes. */

@@ -268,9 +268,11 @@ class Case:
 
     def __init__(self, X, index, *, E=210e9, nu=0.3, rho=7850.0, dt, flags=None, load=None,
                  prescribed=None, plastic=False, yield_=355e6, hardening=1e9, load_ramp=0.0,
-                 drive_ramp=0.0, gravity=0.0, damping=1.0, indenter=None, failure=None):
+                 drive_ramp=0.0, gravity=0.0, damping=1.0, indenter=None, failure=None,
+                 mass=None):
         """index [T, 4] solves tets; index [T, 8] solves solid-shell hexes (elastic), their
-        condensed stiffness and masses from oracle.hex."""
+        condensed stiffness and masses from oracle.hex. `mass` [N] replaces the lumped
+        nodal masses the rest state computes from rho."""
         self.X = np.ascontiguousarray(X, np.float64)
         self.index = np.ascontiguousarray(index, np.uint32)
         self.N, self.T = len(self.X), len(self.index)
@@ -325,6 +327,8 @@ class Case:
         if lib().ora_case_init(C.byref(s)) != 0:
             raise MemoryError("oracle case init")
         self.s = s
+        if mass is not None:
+            np.ctypeslib.as_array(s.mass, shape=(self.N,))[:] = np.asarray(mass, np.float64)
         self.mass = np.ctypeslib.as_array(s.mass, shape=(self.N,)).copy()
         self.volume = np.ctypeslib.as_array(s.volume, shape=(self.T,)).copy()
         self.u = np.zeros((self.N, 3))

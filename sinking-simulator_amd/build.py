@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libshipfem.so")
 
-SOURCES = ["kernels.hip", "mesh.cpp", "prep.cpp", "engine.cpp", "api.cpp", "server.cpp", "handoff.cpp",
+SOURCES = ["kernels.hip", "mesh.cpp", "prep.cpp", "engine.cpp", "api.cpp", "server.cpp", "handoff.cpp", "cpu_step.cpp",
            "casefile.cpp", "hex.cpp", "fem_gpu.cpp", "solid_shell.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=on: a multiply-add fuses only within one source expression, so every
@@ -86,10 +86,20 @@ def write_build_info():
         json.dump(info, f, indent=1)
 
 
+def build_spike():
+    """The reference's own fem_spike driver switched over to this library (oracle/Makefile
+    target `spike`; only where the reference tree is present)."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "spike"], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError("fem_spike switch-over build failed:\n" + r.stdout + r.stderr)
+
+
 def build_all(verbose=False):
     build_oracle()
     lib = build_library(verbose)
     write_build_info()
+    build_spike()
     return lib
 
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <new>
@@ -38,6 +39,25 @@ hipError_t advance(Plan& p, int64_t& host_step, int64_t steps, int mode, int S, 
         }
     }
     return hipGetLastError();
+}
+
+// The board's name: the runtime's marketing name, else the PCI board's product name
+// from sysfs (some boxes leave the runtime's empty), else the PCI address.
+std::string boardName(const hipDeviceProp_t& prop) {
+    if (prop.name[0]) return prop.name;
+    char bus[32] = {0};
+    std::snprintf(bus, sizeof bus, "%04x:%02x:%02x.0", prop.pciDomainID, prop.pciBusID, prop.pciDeviceID);
+    for (const char* leaf : {"product_name", "product_number"}) {
+        std::FILE* f = std::fopen((std::string("/sys/bus/pci/devices/") + bus + "/" + leaf).c_str(), "r");
+        if (!f) continue;
+        char line[128] = {0};
+        const bool got = std::fgets(line, sizeof line, f) != nullptr;
+        std::fclose(f);
+        std::string v = got ? line : "";
+        while (!v.empty() && (v.back() == '\n' || v.back() == ' ')) v.pop_back();
+        if (!v.empty()) return v;
+    }
+    return std::string("AMD GPU at PCI ") + bus;
 }
 
 double omegaToCfl(double lambda_max, double rho) { return 2.0 / std::sqrt(4.0 * lambda_max / rho); }
@@ -111,7 +131,7 @@ shipfem_status shipfem_device_query(char* name, size_t name_len, uint64_t* hbm_b
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return SHIPFEM_DEVICE_ERROR;
-    setError(name, name_len, std::string(prop.name) + " (" + prop.gcnArchName + ")");
+    setError(name, name_len, boardName(prop) + " (" + prop.gcnArchName + ")");
     if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
     return SHIPFEM_OK;
 }
@@ -184,10 +204,10 @@ shipfem_status createSolver(const shipfem_case_desc* desc, bool hex, shipfem_sol
     }
     if (hex) {
         s->dc.omega = s->hc.hex_omega;
-        s->dc.cfl = 2.0 / s->hc.hex_omega;
+        s->dc.cfl = 2.0 / s->hc.hex_omega * s->hc.cfl_scale;
     } else {
         s->dc.omega = std::sqrt(4.0 * lmax[0] / s->hc.rho);
-        s->dc.cfl = omegaToCfl(lmax[0], s->hc.rho);
+        s->dc.cfl = omegaToCfl(lmax[0], s->hc.rho) * s->hc.cfl_scale;
     }
     setTimestep(s->dc, s->hc, s->hc.dt_request > 0 ? s->hc.dt_request : s->hc.dt_safety * s->dc.cfl);
     s->plan.recs[0] = s->dc.rec;
@@ -442,7 +462,7 @@ shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
             HostCase& hc = b->hcs[i];
             DeviceCase& dc = b->dcs[i];
             dc.omega = hex ? hc.hex_omega : std::sqrt(4.0 * lmax[i] / hc.rho);
-            dc.cfl = hex ? 2.0 / hc.hex_omega : omegaToCfl(lmax[i], hc.rho);
+            dc.cfl = (hex ? 2.0 / hc.hex_omega : omegaToCfl(lmax[i], hc.rho)) * hc.cfl_scale;
             setTimestep(dc, hc, hc.dt_request > 0 ? hc.dt_request : hc.dt_safety * dc.cfl);
             b->plan.recs[i] = dc.rec;
         }

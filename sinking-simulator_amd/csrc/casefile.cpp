@@ -24,8 +24,9 @@ enum Present : uint32_t {
     kHasU0 = 1u << 3,
     kHasV0 = 1u << 4,
     kHexElements = 1u << 5,  // solid-shell hexes: 8 node ids per element
+    kHasMass = 1u << 6,      // the caller's lumped nodal masses
 };
-constexpr uint32_t kPresentMask = 0x3Fu;
+constexpr uint32_t kPresentMask = 0x7Fu;
 
 struct Header {
     char magic[8];
@@ -63,7 +64,7 @@ uint64_t fnv1a(const unsigned char* p, uint64_t n) {
 uint32_t presentOf(const shipfem_case_desc& d, bool hex) {
     return (d.node_flags ? kHasFlags : 0u) | (d.load ? kHasLoad : 0u) |
            (d.prescribed ? kHasPrescribed : 0u) | (d.initial_displacement ? kHasU0 : 0u) |
-           (d.initial_velocity ? kHasV0 : 0u) | (hex ? kHexElements : 0u);
+           (d.initial_velocity ? kHasV0 : 0u) | (hex ? kHexElements : 0u) | (d.mass ? kHasMass : 0u);
 }
 
 uint64_t payloadBytes(uint64_t N, uint64_t T, uint32_t present) {
@@ -71,6 +72,7 @@ uint64_t payloadBytes(uint64_t N, uint64_t T, uint32_t present) {
     if (present & kHasFlags) b += 4 * N;
     for (uint32_t bit : {kHasLoad, kHasPrescribed, kHasU0, kHasV0})
         if (present & bit) b += 24 * N;
+    if (present & kHasMass) b += 8 * N;
     return b;
 }
 
@@ -86,7 +88,7 @@ shipfem_status save(const shipfem_case_desc* d, const char* path, bool hex);
 
 // Decoded case: owns the arrays a shipfem_case_desc borrows.
 struct shipfem_case_blob {
-    std::vector<double> X, load, prescribed, u0, v0;
+    std::vector<double> X, load, prescribed, u0, v0, mass;
     std::vector<uint32_t> index, flags;
     uint32_t corners = 4;
 };
@@ -148,6 +150,7 @@ shipfem_status encode(const shipfem_case_desc* d, void* buf, uint64_t len, bool 
     if (d->prescribed) put(d->prescribed, 24 * N);
     if (d->initial_displacement) put(d->initial_displacement, 24 * N);
     if (d->initial_velocity) put(d->initial_velocity, 24 * N);
+    if (d->mass) put(d->mass, 8 * N);
     h.checksum = fnv1a(body, h.payload);
     std::memcpy(buf, &h, sizeof h);
     return SHIPFEM_OK;
@@ -216,6 +219,7 @@ shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_b
     if (h.present & kHasPrescribed) take(b->prescribed, 3 * N);
     if (h.present & kHasU0) take(b->u0, 3 * N);
     if (h.present & kHasV0) take(b->v0, 3 * N);
+    if (h.present & kHasMass) take(b->mass, N);
     for (uint32_t v : b->index)
         if (v >= N) return SHIPFEM_INVALID_ARG;
     *d = shipfem_case_desc{};
@@ -228,6 +232,7 @@ shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_b
     d->prescribed = b->prescribed.empty() ? nullptr : b->prescribed.data();
     d->initial_displacement = b->u0.empty() ? nullptr : b->u0.data();
     d->initial_velocity = b->v0.empty() ? nullptr : b->v0.data();
+    d->mass = b->mass.empty() ? nullptr : b->mass.data();
     shipfem_material& m = d->material;
     m.youngs_modulus = s.E;
     m.poisson_ratio = s.nu;

@@ -91,6 +91,28 @@ float criticalTimestep(const TetMesh& mesh, const Material& material, float safe
     return safety * mesh.shortestEdge() / material.waveSpeed();
 }
 
+namespace {
+const double* restOf(const TetMesh& mesh) {
+    return mesh.restPosition.size() == mesh.position.size() ? mesh.restPosition.data() : nullptr;
+}
+}  // namespace
+
+void stepCpu(TetMesh& mesh, const Material& material, float dt, float gravity, float damping) {
+    const std::size_t N = mesh.nodeCount(), T = mesh.tetCount();
+    if (mesh.tetForce.size() != 12 * T) mesh.tetForce.assign(12 * T, 0.0f);
+    if (mesh.velocity.size() != 3 * N) mesh.velocity.assign(3 * N, 0.0f);
+    shipfem_cpu_step(static_cast<uint32_t>(N), static_cast<uint32_t>(T), mesh.position.data(),
+                     mesh.velocity.data(), mesh.mass.data(), mesh.fixed.size() == N ? mesh.fixed.data() : nullptr,
+                     mesh.index.data(), mesh.restInv.data(), mesh.restVolume.data(), mesh.tetForce.data(),
+                     mesh.adjacencyOffset.data(), mesh.adjacencyEntry.data(), restOf(mesh),
+                     material.lameLambda(), material.lameMu(), dt, gravity, damping);
+}
+
+void tetForces(const TetMesh& mesh, std::size_t tet, float lambda, float mu, float out[12]) {
+    shipfem_cpu_tet_forces(mesh.position.data(), restOf(mesh), mesh.index.data(), mesh.restInv.data(),
+                           mesh.restVolume.data(), tet, lambda, mu, out);
+}
+
 }  // namespace shipfem::fem
 
 namespace shipfem::gpu {
@@ -129,12 +151,11 @@ bool FemGpuSolver::initialise(const fem::TetMesh& mesh, const std::string&, std:
     d.initial_displacement = u0.data();
     d.initial_velocity = v0.data();
     d.material = shipfem_material{210.0e9, 0.30, 7850.0, 0, 0.0, 0.0};
-    if (!mesh.mass.empty() && !mesh.restVolume.empty()) {
-        // the density that reproduces the caller's lumped mass
-        double m = 0, v = 0;
-        for (float x : mesh.mass) m += x;
-        for (float x : mesh.restVolume) v += x;
-        if (v > 0 && m > 0) d.material.density = m / v;
+    // the caller's lumped masses, node by node (TetMesh::mass, fem.hpp:42)
+    std::vector<double> mass;
+    if (mesh.mass.size() == N) {
+        mass.assign(mesh.mass.begin(), mesh.mass.end());
+        d.mass = mass.data();
     }
     char err[512] = {0};
     const shipfem_status s = shipfem_solver_create(&d, &solver_, err, sizeof err);

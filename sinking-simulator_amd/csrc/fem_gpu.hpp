@@ -1,7 +1,7 @@
 // fem_gpu.hpp -- C++ mirror of the reference's tet FEM interface on top of the C ABI.
 //
 //   shipfem::fem    mirrors sim::fem   (engine/sim/fem.hpp:22-88): Material, TetMesh,
-//                   makeBoxTetMesh, criticalTimestep.
+//                   makeBoxTetMesh, criticalTimestep, stepCpu, tetForces.
 //   shipfem::gpu    mirrors gpu::FemGpuSolver (engine/gpu/fem_gpu.hpp:17-45):
 //                   initialise / run / readback / deviceName / valid.
 //
@@ -50,6 +50,14 @@ struct TetMesh {
 
 TetMesh makeBoxTetMesh(float lx, float ly, float lz, int nx, int ny, int nz);
 float criticalTimestep(const TetMesh& mesh, const Material& material, float safety = 0.5f);
+
+// One explicit FP32 step on the host (the library's shipfem_cpu_step): element forces into
+// tetForce, CSR gather, symplectic Euler. `damping` multiplies velocity each step.
+void stepCpu(TetMesh& mesh, const Material& material, float dt, float gravity, float damping);
+
+// The four nodal forces one tet contributes at the current positions
+// (shipfem_cpu_tet_forces, displacement form when restPosition is set).
+void tetForces(const TetMesh& mesh, std::size_t tet, float lambda, float mu, float out[12]);
 
 }  // namespace shipfem::fem
 

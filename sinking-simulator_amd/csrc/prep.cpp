@@ -338,6 +338,25 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
         for (int c = 0; c < 4; ++c) mass[v[c]] += hc.rho * V * 0.25;
     }
     hc.edge_dt = corners == 4 ? 0.5 * shortest / std::sqrt(hc.E / hc.rho) : 0.0;
+    if (d.mass) {
+        // the caller's lumped masses (TetMesh::mass). With m >= s m_density at every node
+        // that carries stiffness, x.Kx / x.Mx <= (x.Kx / x.M_density x) / s, so the largest
+        // frequency is at most the density bound over sqrt(s).
+        double smin = std::numeric_limits<double>::infinity();
+        for (uint32_t n = 0; n < N; ++n) {
+            const double mc = d.mass[n];
+            if (!std::isfinite(mc) || mc < 0)
+                return fmt(("mass of node " + std::to_string(externalNode(hc, n)) + " must be finite and >= 0").c_str(), mc);
+            const bool held = (hc.nflags[n] & SHIPFEM_PIN_ALL) == SHIPFEM_PIN_ALL;
+            if (mass[n] > 0 && !held) {
+                if (!(mc > 0))
+                    return "node " + std::to_string(externalNode(hc, n)) + " is free but its mass is zero";
+                smin = std::min(smin, mc / mass[n]);
+            }
+            mass[n] = mc;
+        }
+        if (std::isfinite(smin)) hc.cfl_scale = std::sqrt(smin);
+    }
     hc.mass.assign(hc.Npad, 0.0f);
     hc.invm.assign(hc.Npad, 0.0f);
     for (uint32_t n = 0; n < N; ++n) {
@@ -533,7 +552,7 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
 
 // A copy of the caller's case in the internal orders.
 struct PermutedCase {
-    std::vector<double> X, load, presc, u0, v0;
+    std::vector<double> X, load, presc, u0, v0, mass;
     std::vector<uint32_t> index, flags;
 };
 
@@ -576,6 +595,11 @@ std::string prepareCase(const shipfem_case_desc& d, HostCase& hc, int32_t hist_c
     dp.prescribed = nodeVec(d.prescribed, pc.presc);
     dp.initial_displacement = nodeVec(d.initial_displacement, pc.u0);
     dp.initial_velocity = nodeVec(d.initial_velocity, pc.v0);
+    if (d.mass) {
+        pc.mass.resize(N);
+        for (uint32_t i = 0; i < N; ++i) pc.mass[i] = d.mass[nperm[i]];
+        dp.mass = pc.mass.data();
+    }
     return prepareInternal(dp, hc, hist_cap, std::move(tperm), std::move(nperm), 4);
 }
 

@@ -91,7 +91,7 @@ struct Request {
     uint64_t id = 0;
     double run_to = 0, submit_s = 0, admit_s = 0;
     // owned copies of the caller's arrays
-    std::vector<double> X, load, presc, u0, v0;
+    std::vector<double> X, load, presc, u0, v0, mass;
     std::vector<uint32_t> index, flags;
     shipfem_case_desc desc{};
     bool hex = false;  // a solid-shell case: 8 node ids per element
@@ -127,10 +127,11 @@ struct Round {
     size_t cap = 0;
 };
 
+// A retired case's state kept for shipfem_server_fetch, every array in the caller's node
+// and element orders (as readbackCase writes them).
 struct Kept {
-    HostCase hc;  // X and shapes only
     std::vector<double> u, v;
-    std::vector<float> eqps;
+    std::vector<float> position, eqps, plastic_strain, damage;
     int64_t step = 0;
     double time = 0;
     uint32_t status = 0;
@@ -215,6 +216,7 @@ void shipfem_server::workerMain() {
         std::vector<double>().swap(r->presc);
         std::vector<double>().swap(r->u0);
         std::vector<double>().swap(r->v0);
+        std::vector<double>().swap(r->mass);
         std::vector<uint32_t>().swap(r->flags);
         {
             std::lock_guard<std::mutex> lk(m);
@@ -300,7 +302,8 @@ bool shipfem_server::admit() {
             ++st.failed;
             continue;
         }
-        a.dc.cfl = take[k]->hex ? 2.0 / hc.hex_omega : 2.0 / std::sqrt(4.0 * lmax[tet_index++] / hc.rho);
+        a.dc.cfl = (take[k]->hex ? 2.0 / hc.hex_omega : 2.0 / std::sqrt(4.0 * lmax[tet_index++] / hc.rho)) *
+                   hc.cfl_scale;
         setTimestep(a.dc, hc, hc.dt_request > 0 ? hc.dt_request : hc.dt_safety * a.dc.cfl);
         a.steps = std::max<int64_t>(1, static_cast<int64_t>(std::ceil(take[k]->run_to / a.dc.dt - 1e-9)));
         a.dc.rec.step_end = a.steps;
@@ -334,13 +337,27 @@ void shipfem_server::finish(Active& a, hipError_t fault) {
     shipfem_state_view view{};
     std::vector<shipfem_energy_record> rec;
     int32_t nrec = 0;
+    bool keeping = false;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        keeping = keep.count(res.request_id) != 0;
+    }
     if (fault == hipSuccess) {
         k.u.resize(3ull * hc.N);
         k.v.resize(3ull * hc.N);
         k.eqps.resize(hc.T);
+        k.damage.resize(hc.T);
         view.displacement = k.u.data();
         view.velocity = k.v.data();
         view.eqps = k.eqps.data();
+        view.damage = k.damage.data();
+        if (keeping) {
+            // one readback gives every kept field in the caller's orders
+            k.position.resize(3ull * hc.N);
+            k.plastic_strain.resize(6ull * hc.T);
+            view.position = k.position.data();
+            view.plastic_strain = k.plastic_strain.data();
+        }
         hipError_t e = readbackCase(a.dc, hc, &view, side);
         rec.resize(static_cast<size_t>(a.dc.rec.hist_cap));
         if (e == hipSuccess) e = readEnergy(a.dc, rec.data(), a.dc.rec.hist_cap, &nrec, side);
@@ -352,17 +369,12 @@ void shipfem_server::finish(Active& a, hipError_t fault) {
             res.max_displacement = std::max(res.max_displacement, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
         }
         for (float q : k.eqps) res.max_eqps = std::max(res.max_eqps, static_cast<double>(q));
-        if (hc.features & kFeatDamage) {
-            std::vector<float> dmg(hc.T);
-            shipfem_state_view dv{};
-            dv.damage = dmg.data();
-            if (readbackCase(a.dc, hc, &dv, side) == hipSuccess)
-                for (uint32_t t = 0; t < hc.T; ++t)  // dmg is in the caller's order, vol internal
-                    if (dmg[externalTet(hc, t)] >= 1.0f) {
-                        ++res.torn_elements;
-                        res.torn_volume += hc.corners == 8 ? hc.hvol[t] : hc.vol[t];
-                    }
-        }
+        if ((hc.features & kFeatDamage) && e == hipSuccess)
+            for (uint32_t t = 0; t < hc.T; ++t)  // damage is in the caller's order, vol internal
+                if (k.damage[externalTet(hc, t)] >= 1.0f) {
+                    ++res.torn_elements;
+                    res.torn_volume += hc.corners == 8 ? hc.hvol[t] : hc.vol[t];
+                }
         for (int32_t r = 0; r < nrec; ++r) {
             const double* f = rec[r].contact_force;
             res.peak_contact_force = std::max(res.peak_contact_force, std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]));
@@ -378,9 +390,6 @@ void shipfem_server::finish(Active& a, hipError_t fault) {
             k.step = view.step;
             k.time = view.time;
             k.status = view.status;
-            k.hc.N = hc.N;
-            k.hc.T = hc.T;
-            k.hc.X = std::move(hc.X);
             kept[res.request_id] = std::move(k);
         }
         keep.erase(it);
@@ -636,6 +645,7 @@ shipfem_status submitCase(shipfem_server* s, const shipfem_case_desc* d, double 
     if (d->prescribed) r->presc.assign(d->prescribed, d->prescribed + 3 * N);
     if (d->initial_displacement) r->u0.assign(d->initial_displacement, d->initial_displacement + 3 * N);
     if (d->initial_velocity) r->v0.assign(d->initial_velocity, d->initial_velocity + 3 * N);
+    if (d->mass) r->mass.assign(d->mass, d->mass + N);
     r->desc = *d;
     r->desc.rest_position = r->X.data();
     r->desc.index = r->index.data();
@@ -644,6 +654,7 @@ shipfem_status submitCase(shipfem_server* s, const shipfem_case_desc* d, double 
     r->desc.prescribed = r->presc.empty() ? nullptr : r->presc.data();
     r->desc.initial_displacement = r->u0.empty() ? nullptr : r->u0.data();
     r->desc.initial_velocity = r->v0.empty() ? nullptr : r->v0.data();
+    r->desc.mass = r->mass.empty() ? nullptr : r->mass.data();
     // every round samples its first step: records land on multiples of the chunk
     r->desc.sample_every = s->opts.chunk;
     r->run_to = run_to;
@@ -710,12 +721,12 @@ shipfem_status shipfem_server_fetch(shipfem_server* s, uint64_t id, shipfem_stat
     auto it = s->kept.find(id);
     if (it == s->kept.end()) return SHIPFEM_NOT_FOUND;
     const Kept& k = it->second;
-    for (size_t q = 0; q < k.u.size(); ++q) {
-        if (view->displacement) view->displacement[q] = k.u[q];
-        if (view->velocity) view->velocity[q] = k.v[q];
-        if (view->position) view->position[q] = static_cast<float>(k.hc.X[q] + k.u[q]);
-    }
+    if (view->displacement) std::copy(k.u.begin(), k.u.end(), view->displacement);
+    if (view->velocity) std::copy(k.v.begin(), k.v.end(), view->velocity);
+    if (view->position) std::copy(k.position.begin(), k.position.end(), view->position);
     if (view->eqps) std::copy(k.eqps.begin(), k.eqps.end(), view->eqps);
+    if (view->plastic_strain) std::copy(k.plastic_strain.begin(), k.plastic_strain.end(), view->plastic_strain);
+    if (view->damage) std::copy(k.damage.begin(), k.damage.end(), view->damage);
     view->step = k.step;
     view->time = k.time;
     view->status = k.status;

@@ -71,6 +71,9 @@ struct HostCase {
     int64_t max_steps = 0;
     int32_t hist_cap = 0;
     double edge_dt = 0;  // the reference's formula at safety 0.5, for the record
+    // sqrt(min m_caller / m_density) over free nodes with mass, when the caller gives the
+    // lumped masses (1 otherwise): the factor on the density-based stable step
+    double cfl_scale = 1.0;
     double total_mass = 0, total_volume = 0;
 };
 

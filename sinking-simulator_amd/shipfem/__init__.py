@@ -54,7 +54,7 @@ class CaseDesc(C.Structure):
                 ("indenter", Indenter), ("load_ramp", C.c_double), ("drive_ramp", C.c_double),
                 ("gravity", C.c_double), ("damping", C.c_double), ("dt", C.c_double),
                 ("dt_safety", C.c_double), ("sample_every", C.c_int32),
-                ("max_steps", C.c_int64)]
+                ("max_steps", C.c_int64), ("mass", D)]
 
 
 class StepParams(C.Structure):
@@ -142,6 +142,9 @@ SIGNATURES = {
     "shipfem_mesh_fill_case": (S, [VP, C.POINTER(CaseDesc)]),
     "shipfem_mesh_destroy": (None, [VP]),
     "shipfem_edge_timestep": (C.c_double, [D, U32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
+    "shipfem_cpu_tet_forces": (S, [F, D, U32, F, F, C.c_uint64, C.c_float, C.c_float, F]),
+    "shipfem_cpu_step": (S, [C.c_uint32, C.c_uint32, F, F, F, U32, U32, F, F, F, U32, U32, D] +
+                         [C.c_float] * 5),
     "shipfem_solver_create": (S, [C.POINTER(CaseDesc), PP, C.c_char_p, C.c_size_t]),
     "shipfem_solver_timestep": (S, [VP, D, D, D]),
     "shipfem_solver_set_params": (S, [VP, C.POINTER(StepParams)]),
@@ -327,7 +330,7 @@ class Case:
     def __init__(self, X, index, *, flags=None, load=None, prescribed=None, u0=None, v0=None,
                  E=210e9, nu=0.3, rho=7850.0, plastic=False, yield_=355e6, hardening=1e9,
                  indenter=None, load_ramp=0.0, drive_ramp=0.0, gravity=0.0, damping=1.0,
-                 dt=0.0, dt_safety=0.9, sample_every=100, max_steps=0, failure=None):
+                 dt=0.0, dt_safety=0.9, sample_every=100, max_steps=0, failure=None, mass=None):
         self.X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
         self.index = np.ascontiguousarray(index, np.uint32).reshape(-1, self.corners)
         c = lambda a, dt_: None if a is None else np.ascontiguousarray(a, dt_)  # noqa: E731
@@ -336,6 +339,7 @@ class Case:
         self.prescribed = c(prescribed, np.float64)
         self.u0 = c(u0, np.float64)
         self.v0 = c(v0, np.float64)
+        self.mass = c(mass, np.float64)  # lumped nodal masses in place of rho V / corners
         self.E, self.nu, self.rho = E, nu, rho
         self.plastic, self.yield_, self.hardening = plastic, yield_, hardening
         self.indenter = indenter
@@ -372,6 +376,7 @@ class Case:
         d.prescribed = _p(self.prescribed, C.c_double)
         d.initial_displacement = _p(self.u0, C.c_double)
         d.initial_velocity = _p(self.v0, C.c_double)
+        d.mass = _p(self.mass, C.c_double)
         f = dict(RICE_TRACEY)
         f.update(self.failure or {})
         d.material = Material(self.E, self.nu, self.rho, 1 if self.plastic else 0, self.yield_,
@@ -425,7 +430,7 @@ class Case:
                    hardening=m.hardening_modulus, indenter=indenter, load_ramp=d.load_ramp,
                    drive_ramp=d.drive_ramp, gravity=d.gravity, damping=d.damping, dt=d.dt,
                    dt_safety=d.dt_safety, sample_every=d.sample_every, max_steps=d.max_steps,
-                   failure=failure)
+                   failure=failure, mass=arr(d.mass, N, np.float64))
 
     @classmethod
     def decode(cls, blob):
@@ -462,7 +467,7 @@ class Case:
                     prescribed=self.prescribed, plastic=self.plastic, yield_=self.yield_,
                     hardening=self.hardening, load_ramp=self.load_ramp,
                     drive_ramp=self.drive_ramp, gravity=self.gravity, damping=self.damping,
-                    indenter=ind, failure=self.failure)
+                    indenter=ind, failure=self.failure, mass=self.mass)
 
 
 class HexCase(Case):
@@ -581,6 +586,42 @@ def _tears(call):
     arr = (Tear * max(n.value, 1))()
     _check(call(arr, n.value, C.byref(n)), "tears")
     return [(arr[k].tet, arr[k].step, np.array(arr[k].normal[:])) for k in range(n.value)]
+
+
+def cpu_tet_forces(position, index, rest_inv, rest_volume, E=210e9, nu=0.3, rest=None):
+    """tetForces (fem.hpp:84-86) of every tet on the host, FP32: [T, 12]. position [N, 3]
+    float32, rest_inv [T, 9] column-major and rest_volume [T] as computeRestState gives
+    them; rest [N, 3] double selects the displacement form."""
+    pos = np.ascontiguousarray(position, np.float32)
+    idx = np.ascontiguousarray(index, np.uint32)
+    ri = np.ascontiguousarray(rest_inv, np.float32)
+    rv = np.ascontiguousarray(rest_volume, np.float32)
+    rs = None if rest is None else np.ascontiguousarray(rest, np.float64)
+    lam = np.float32(E) * np.float32(nu) / ((np.float32(1) + np.float32(nu)) * (np.float32(1) - np.float32(2) * np.float32(nu)))
+    mu = np.float32(E) / (np.float32(2) * (np.float32(1) + np.float32(nu)))
+    out = np.zeros((len(idx), 12), np.float32)
+    for t in range(len(idx)):
+        _check(lib().shipfem_cpu_tet_forces(_p(pos, C.c_float), _p(rs, C.c_double), _p(idx, C.c_uint32),
+                                            _p(ri, C.c_float), _p(rv, C.c_float), t, float(lam),
+                                            float(mu), _p(out[t], C.c_float)))
+    return out
+
+
+def cpu_step(state, dt, gravity=0.0, damping=1.0, E=210e9, nu=0.3):
+    """stepCpu (fem.hpp:77-82) on the host, in place on a dict of TetMesh arrays:
+    position [N, 3] and velocity [N, 3] float32, mass, fixed, index, rest_inv,
+    rest_volume, tet_force, adj_offset, adj_entry and optionally rest [N, 3] double."""
+    g = state
+    lam = np.float32(E) * np.float32(nu) / ((np.float32(1) + np.float32(nu)) * (np.float32(1) - np.float32(2) * np.float32(nu)))
+    mu = np.float32(E) / (np.float32(2) * (np.float32(1) + np.float32(nu)))
+    N, T = len(g["position"]), len(g["index"])
+    _check(lib().shipfem_cpu_step(N, T, _p(g["position"], C.c_float), _p(g["velocity"], C.c_float),
+                                  _p(g["mass"], C.c_float), _p(g.get("fixed"), C.c_uint32),
+                                  _p(g["index"], C.c_uint32), _p(g["rest_inv"], C.c_float),
+                                  _p(g["rest_volume"], C.c_float), _p(g["tet_force"], C.c_float),
+                                  _p(g["adj_offset"], C.c_uint32), _p(g["adj_entry"], C.c_uint32),
+                                  _p(g.get("rest"), C.c_double), float(lam), float(mu), dt, gravity,
+                                  damping))
 
 
 def regularised_failure_strain(uniform, fracture, length, thickness):
@@ -911,7 +952,6 @@ class Server:
     def fetch(self, rid, case):
         st = State(case.N, case.T)
         v = st.view()
-        v.plastic_strain = None
         _check(lib().shipfem_server_fetch(self.h, rid, C.byref(v)), f"fetch {rid}")
         return st.take(v)
 

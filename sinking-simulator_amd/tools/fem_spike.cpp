@@ -3,15 +3,19 @@
 //
 //   1. element formulation vs Euler-Bernoulli beam theory (main.cpp:68-139): refine
 //      1/2/4, damped settle under gravity, converging and finest within 20%
-//   2. determinism: the same run twice gives the same bits (the reference's GPU path is
+//   2. GPU vs the host FP32 stepCpu from a twisted state (main.cpp:143-223): RMS dv / RMS v
+//      after 1, 10, 100, 1000 steps, with the reference's bounds
+//   3. determinism: the same run twice gives the same bits (the reference's GPU path is
 //      not reproducible, docs/07-fem-spike-findings.md:48-63; this one is)
-//   3. throughput on the reference's five meshes, 200 warm-up + 2000 timed steps
+//   4. throughput on the reference's five meshes, 200 warm-up + 2000 timed steps
 //      (main.cpp:227-262), device time from HIP events
+//   5. the host stepCpu on one core, for scale (main.cpp:264-281)
 //
-// The GPU-vs-CPU agreement section (main.cpp:143-223) lives in the test suite, against
-// the FP64 oracle. Exit code 0 when all checks pass; 0 with a SKIP line without a device.
+// Exit code 0 when all checks pass; 0 with a SKIP line without a device.
 #include "../csrc/fem_gpu.hpp"
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -86,19 +90,77 @@ bool cantilever() {
     return true;
 }
 
+// Positions of the fem_spike bar bent and twisted off its rest state (main.cpp:163-168).
+void twist(fem::TetMesh& mesh) {
+    for (std::size_t n = 0; n < mesh.nodeCount(); ++n) {
+        const float x = mesh.position[n * 3];
+        mesh.position[n * 3 + 2] += 0.02f * x * x;
+        mesh.position[n * 3 + 1] += 0.01f * x;
+    }
+}
+
+void gpuMatchesCpu() {
+    std::printf("\n2. GPU vs the host FP32 step\n");
+    const fem::Material steel;
+    fem::TetMesh cpu = fem::makeBoxTetMesh(0.4f, 0.1f, 0.02f, 16, 4, 2);
+    cpu.computeRestState(steel);
+    clampRootFace(cpu);
+    twist(cpu);
+    fem::TetMesh dev = cpu;
+    std::string error;
+    gpu::FemGpuSolver solver;
+    if (!solver.initialise(dev, ".", error)) return;
+    const float dt = fem::criticalTimestep(cpu, steel);
+    std::printf("     %8s %18s %18s\n", "steps", "RMS dv / RMS v", "max |dx| (m)");
+    float ratio[4] = {0, 0, 0, 0}, worst = 0;
+    int done = 0, k = 0;
+    for (int target : {1, 10, 100, 1000}) {
+        for (int i = done; i < target; ++i) fem::stepCpu(cpu, steel, dt, 9.80665f, 1.0f);
+        solver.run(target - done, steel, dt, 9.80665f, 1.0f);
+        solver.readback(dev);
+        done = target;
+        double dv2 = 0, v2 = 0;
+        worst = 0;
+        for (std::size_t q = 0; q < cpu.position.size(); ++q) {
+            worst = std::max(worst, std::abs(cpu.position[q] - dev.position[q]));
+            const double d = static_cast<double>(cpu.velocity[q]) - dev.velocity[q];
+            dv2 += d * d;
+            v2 += static_cast<double>(cpu.velocity[q]) * cpu.velocity[q];
+        }
+        ratio[k] = static_cast<float>(std::sqrt(dv2 / std::max(v2, 1e-30)));
+        std::printf("     %8d %18.3g %18.3g\n", target, static_cast<double>(ratio[k]), static_cast<double>(worst));
+        ++k;
+    }
+    check("one step agrees to better than 1e-4 relative", ratio[0] < 1e-4f);
+    check("ten steps still agree to better than 1e-3 relative", ratio[1] < 1e-3f);
+    check("divergence grows smoothly rather than blowing up", ratio[3] < 0.5f && std::isfinite(ratio[3]));
+    check("positions agree to better than 10 micrometres after 1000 steps", worst < 1e-5f);
+}
+
+void cpuForScale() {
+    std::printf("\n5. The host FP32 step on one core, for scale\n");
+    const fem::Material steel;
+    fem::TetMesh mesh = fem::makeBoxTetMesh(2.0f, 0.5f, 0.05f, 120, 16, 4);
+    mesh.computeRestState(steel);
+    clampRootFace(mesh);
+    const float dt = fem::criticalTimestep(mesh, steel);
+    const int steps = 50;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; ++i) fem::stepCpu(mesh, steel, dt, 9.80665f, 1.0f);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+    std::printf("     %zu tets: %.4f ms/step, %.1f Melem-upd/s (one core)\n", mesh.tetCount(), ms,
+                mesh.tetCount() / (ms * 1e-3) / 1e6);
+}
+
 void determinism() {
-    std::printf("\n2. Run-to-run determinism\n");
+    std::printf("\n3. Run-to-run determinism\n");
     const fem::Material steel;
     std::vector<float> out[2];
     for (int r = 0; r < 2; ++r) {
         fem::TetMesh mesh = fem::makeBoxTetMesh(0.4f, 0.1f, 0.02f, 16, 4, 2);
         mesh.computeRestState(steel);
         clampRootFace(mesh);
-        for (std::size_t n = 0; n < mesh.nodeCount(); ++n) {
-            const float x = mesh.position[n * 3];
-            mesh.position[n * 3 + 2] += 0.02f * x * x;
-            mesh.position[n * 3 + 1] += 0.01f * x;
-        }
+        twist(mesh);
         std::string error;
         gpu::FemGpuSolver solver;
         if (!solver.initialise(mesh, ".", error)) return;
@@ -112,7 +174,7 @@ void determinism() {
 }
 
 void throughput() {
-    std::printf("\n3. Throughput on the GPU\n");
+    std::printf("\n4. Throughput on the GPU\n");
     std::printf("     %10s %10s %12s %14s %12s\n", "tets", "nodes", "ms/step", "Melem-upd/s", "sim ms/s @dt");
     const fem::Material steel;
     struct Size { int nx, ny, nz; };
@@ -143,11 +205,14 @@ void throughput() {
 int main() {
     std::printf("shipfem - explicit tetrahedral FEM spike on MI355X\n");
     if (!cantilever()) {
+        cpuForScale();
         std::printf("\nall checks passed, but the GPU sections were skipped for want of a device\n");
         return 0;
     }
+    gpuMatchesCpu();
     determinism();
     throughput();
+    cpuForScale();
     if (failures != 0) std::printf("\nCHECKS FAILED\n");
     else std::printf("\nall checks passed\n");
     return failures == 0 ? 0 : 1;

@@ -232,3 +232,27 @@ def test_server_serves_hex_cases_beside_tets():
         assert res[ids[k]]["torn_elements"] == torn, k
     assert res[ids[2]]["torn_elements"] > 0 and res[ids[2]]["max_eqps"] > 0.0
     assert res[ids[4]]["max_eqps"] > 0.0 and res[ids[1]]["max_eqps"] == 0.0
+
+
+def test_fetch_returns_position_damage_and_plastic_strain_in_the_callers_order():
+    # the library lays a tet case out in its own tet and node orders; a kept result comes
+    # back in the caller's: position is X + u node by node, and the per-tet fields equal
+    # the single-case solver's readback
+    c = small_panel(speed=10.0, failure=dict(failure_strain=0.08))
+    run_to = 1200 * sf.Solver(c).timestep()["dt"] * (1 - 1e-9)
+    print("fetch case:", c.N, "nodes", c.T, "tets", flush=True)
+    srv, ids, res, kept = serve([c], run_to, keep={0})
+    a = kept[ids[0]]
+    assert res[ids[0]]["status"] == sf.OK
+    assert np.array_equal(a.position, (c.X + a.u).astype(np.float32))
+    s = sf.Solver(c)
+    s.run(a.step)
+    st = s.readback()
+    assert np.array_equal(a.u, st.u) and np.array_equal(a.position, st.position)
+    assert np.array_equal(a.damage, st.damage) and np.array_equal(a.plastic_strain, st.plastic_strain)
+    torn = int(np.sum(a.damage >= 1.0))
+    assert torn > 10 and res[ids[0]]["torn_elements"] == torn
+    # a fetched result is handed over once
+    with pytest.raises(sf.ShipfemError) as e:
+        srv.fetch(ids[0], c)
+    assert e.value.status == sf.NOT_FOUND
