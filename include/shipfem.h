@@ -194,6 +194,15 @@ void shipfem_mesh_destroy(shipfem_mesh* m);
 double shipfem_edge_timestep(const double* rest_position, const uint32_t* index,
                              uint32_t tet_count, double youngs_modulus, double density,
                              double safety);
+/* The stable step the solver uses instead of criticalTimestep (fem.cpp:183-185), without a
+ * solver: cfl = 2 / omega_max with omega_e^2 = 4 lambda_max(B^T C B) / rho the largest
+ * lumped-mass frequency of tet e (Irons-Hughes), evaluated in FP64 on device 0 from the
+ * rest coordinates [3N] and connectivity [4T]. Multiply by a safety factor (the solver
+ * takes 0.9) for dt. Uses device memory of its own for the call. */
+shipfem_status shipfem_stable_timestep(const double* rest_position, const uint32_t* index,
+                                       uint32_t node_count, uint32_t tet_count,
+                                       double youngs_modulus, double poisson_ratio, double density,
+                                       double* cfl, char* err, size_t err_len);
 
 /* ---- the host FP32 step (replaces sim::fem::tetForces and stepCpu) -------------- */
 /* tetForces (fem.hpp:84-86, fem.cpp:73-117): the four nodal forces [12] of tet `tet` at

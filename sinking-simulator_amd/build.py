@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libshipfem.so")
 
-SOURCES = ["kernels.hip", "mesh.cpp", "prep.cpp", "engine.cpp", "api.cpp", "server.cpp", "handoff.cpp", "cpu_step.cpp",
+SOURCES = ["kernels.hip", "mesh.cpp", "prep.cpp", "engine.cpp", "api.cpp", "server.cpp", "handoff.cpp", "cpu_step.cpp", "timestep.hip",
            "casefile.cpp", "hex.cpp", "fem_gpu.cpp", "solid_shell.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=on: a multiply-add fuses only within one source expression, so every

@@ -308,6 +308,26 @@ def test_stable_step_matches_oracle_eigen_bound():
     assert abs(s.dt - 0.9 * cfl) <= 1e-6 * cfl
 
 
+def test_stable_timestep_without_a_solver_matches_oracle():
+    # shipfem_stable_timestep forms Dm^-1 in FP64 on the device: the oracle's FP64 bound to
+    # Jacobi rounding; the solver's (from FP32-stored planes) to FP32 rounding
+    for case in (configs.plate(lx=0.5, ly=0.2, nx=10, ny=4),
+                 configs.stiffened_panel(nx=24, ny_bay=4, nz_web=3, radius=0.05,
+                                         geometry=dict(lx=0.36, ly=0.2, n_stiff=1, web_h=0.04))):
+        rs = O.rest_state(case.X, case.index, 7850.0)
+        omega = O.max_element_omega(rs["rest_inv"], 210e9, 0.3, 7850.0)
+        cfl = sf.stable_timestep(case.X, case.index)
+        assert abs(cfl - 2.0 / omega) <= 1e-11 * cfl
+        assert abs(cfl - solver(case).timestep()["cfl"]) <= 1e-6 * cfl
+    # one tet (one wave, one lane): the closed path through the fold
+    X, idx = O.box_mesh(0.1, 0.1, 0.1, 1, 1, 1)
+    rs = O.rest_state(X, idx[:1], 7850.0)
+    assert abs(sf.stable_timestep(X, idx[:1]) * O.max_element_omega(rs["rest_inv"], 210e9, 0.3, 7850.0) - 2.0) <= 1e-11
+    with pytest.raises(sf.ShipfemError) as e:
+        sf.stable_timestep(X, np.array([[0, 1, 2, len(X)]], np.uint32))
+    assert e.value.status == sf.INVALID_ARG and "node_count" in str(e.value)
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_rest_stays_at_rest_and_translation_is_free(variant):
     X, idx = O.box_mesh(0.3, 0.1, 0.05, 6, 2, 2)
