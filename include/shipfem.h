@@ -203,6 +203,18 @@ shipfem_status shipfem_stable_timestep(const double* rest_position, const uint32
                                        uint32_t node_count, uint32_t tet_count,
                                        double youngs_modulus, double poisson_ratio, double density,
                                        double* cfl, char* err, size_t err_len);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * evaluation and result as shipfem_stable_timestep, bit for bit, with no allocation. It
+ * stages the mesh in a 32 MiB device array of the library's own (one per GPU, held until
+ * the process ends) and runs on the null stream of the current device; a mesh needing more
+ * (24 bytes per node + 16 per tet + 8 per 64 tets, each part rounded up to 256) is refused
+ * with SHIPFEM_INVALID_ARG and a message. Calls on one GPU share that array: calls that
+ * overlap on one GPU (from several host threads) race and must be serialised by the caller. */
+shipfem_status shipfem_stable_timestep_noalloc(const double* rest_position, const uint32_t* index,
+                                               uint32_t node_count, uint32_t tet_count,
+                                               double youngs_modulus, double poisson_ratio,
+                                               double density, double* cfl, char* err, size_t err_len);
 
 /* ---- the host FP32 step (replaces sim::fem::tetForces and stepCpu) -------------- */
 /* tetForces (fem.hpp:84-86, fem.cpp:73-117): the four nodal forces [12] of tet `tet` at

@@ -12,7 +12,14 @@
 #include "shipfem_host.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <string>
+
+// Device scratch of shipfem_stable_timestep_noalloc: one copy per GPU, held until the process
+// ends. In host code `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress, once per device (scratchOnDevice below).
+constexpr uint64_t kScratchBytes = 32ull << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
 
 namespace shipfem {
 namespace {
@@ -124,6 +131,29 @@ hipError_t evaluate(char* base, const double* X, const uint32_t* index, uint32_t
     return hipSuccess;
 }
 
+// The current device's copy of `scratch`, looked up on the first call on that device (with
+// it current) and kept per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch[kMaxDevices] = {};
+
+hipError_t scratchOnDevice(char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 }  // namespace shipfem
 
@@ -155,6 +185,41 @@ shipfem_status shipfem_stable_timestep(const double* rest_position, const uint32
                      stream);
     if (base) (void)hipFree(base);
     if (stream) (void)hipStreamDestroy(stream);
+    if (e != hipSuccess) {
+        setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
+        return statusOf(e);
+    }
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_stable_timestep_noalloc(const double* rest_position, const uint32_t* index,
+                                               uint32_t node_count, uint32_t tet_count, double youngs_modulus,
+                                               double poisson_ratio, double density, double* cfl, char* err,
+                                               size_t err_len) {
+    const std::string why = checkArgs(rest_position, index, node_count, tet_count, youngs_modulus, poisson_ratio,
+                                      density, cfl);
+    if (!why.empty()) {
+        setError(err, err_len, why);
+        return SHIPFEM_INVALID_ARG;
+    }
+    const StepLayout L(node_count, tet_count);
+    if (L.total > kScratchBytes) {
+        setError(err, err_len, "the mesh needs " + std::to_string(L.total) + " bytes of device scratch (24 per node, " +
+                                   "16 per tet, 8 per 64 tets), more than the " + std::to_string(kScratchBytes) +
+                                   " this entry point holds; use shipfem_stable_timestep");
+        return SHIPFEM_INVALID_ARG;
+    }
+    std::string dev;
+    if (!deviceAvailable(&dev)) {
+        setError(err, err_len, dev);
+        return SHIPFEM_NO_DEVICE;
+    }
+    char* base = nullptr;
+    hipError_t e = scratchOnDevice(&base);
+    // every wave slot and the result are written before they are read: nothing to clear
+    if (e == hipSuccess)
+        e = evaluate(base, rest_position, index, node_count, tet_count, youngs_modulus, poisson_ratio, density, cfl,
+                     nullptr);
     if (e != hipSuccess) {
         setError(err, err_len, std::string("HIP: ") + hipGetErrorString(e));
         return statusOf(e);

@@ -144,6 +144,8 @@ SIGNATURES = {
     "shipfem_edge_timestep": (C.c_double, [D, U32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
     "shipfem_stable_timestep": (S, [D, U32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, D,
                                     C.c_char_p, C.c_size_t]),
+    "shipfem_stable_timestep_noalloc": (S, [D, U32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                            D, C.c_char_p, C.c_size_t]),
     "shipfem_cpu_tet_forces": (S, [F, D, U32, F, F, C.c_uint64, C.c_float, C.c_float, F]),
     "shipfem_cpu_step": (S, [C.c_uint32, C.c_uint32, F, F, F, U32, U32, F, F, F, U32, U32, D] +
                          [C.c_float] * 5),
@@ -590,12 +592,14 @@ def _tears(call):
     return [(arr[k].tet, arr[k].step, np.array(arr[k].normal[:])) for k in range(n.value)]
 
 
-def stable_timestep(X, index, E=210e9, nu=0.3, rho=7850.0):
-    """2 / omega_max of a tet mesh in FP64 on the device (shipfem_stable_timestep)."""
+def stable_timestep(X, index, E=210e9, nu=0.3, rho=7850.0, noalloc=False):
+    """2 / omega_max of a tet mesh in FP64 on the device (shipfem_stable_timestep; with
+    noalloc, shipfem_stable_timestep_noalloc, which allocates nothing)."""
     X = np.ascontiguousarray(X, np.float64)
     idx = np.ascontiguousarray(index, np.uint32)
     cfl, err = C.c_double(), C.create_string_buffer(512)
-    _check(lib().shipfem_stable_timestep(_p(X, C.c_double), _p(idx, C.c_uint32), len(X), len(idx), E, nu, rho,
+    fn = lib().shipfem_stable_timestep_noalloc if noalloc else lib().shipfem_stable_timestep
+    _check(fn(_p(X, C.c_double), _p(idx, C.c_uint32), len(X), len(idx), E, nu, rho,
                                          C.byref(cfl), err, 512), err.value.decode())
     return cfl.value
 
