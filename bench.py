@@ -1,14 +1,18 @@
 """Benchmark of the explicit tet step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--csv FILE]
-                    [--workload plate|cantilever|panel|batch|serve|sweep|hex|hex_plastic]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs LIST] [--csv FILE]
+                    [--workload c4|plate|cantilever|panel|batch|serve|sweep|hex|hex_plastic]
 
-Default workload: the large plain-path plate (10 x 4 x 0.02 m, 350x144x4 Kuhn cells,
-1,209,600 tets, nodes jittered +-5%, edges clamped, 200 kPa ramped over 5 ms,
-FP32 + Kahan state), one step = one element pass + one node pass over the whole
-mesh. Prints ONE JSON line with tet-steps/s, the HBM roofline of the dominant kernel
-(HIP events on the solver's stream) and the reference CPU path (the FP32 stepCpu
-restatement, bit-identical to the compiled reference) timed on this host.
+Default workload (c4): the batched configuration, 64 independent clamped 20 mm steel plates
+(tets log-uniform in [1e4, 3e5], pressure 50-400 kPa, yield 235-460 MPa, J2 with H = 1 GPa,
+FP32 + Kahan state), one step = one element pass + one node pass over every case, replayed
+from a captured graph. `value` is batched tet-steps/s over exactly K timed steps. Beside it,
+measured in the same run on the same 64 cases: the plain path (one case per solver, two plain
+launches per step) and the ratio with its HBM bound; the roofline of the dominant kernel on
+the bytes its layout must move, timed live with HIP events on the batch's stream; the large
+plate (1,209,600 tets) on the plain path; a short served leg (Poisson arrivals of stiffened
+panels) with p50/p99 latency; and the reference's FP32 CPU step on this host's cores.
+Prints ONE JSON line.
 """
 import argparse
 import json
@@ -378,12 +382,323 @@ def bench_hex(args, sf, info, rows, plastic=False):
               roofline=roofline, cpu_baseline=cpu))
 
 
+# ---------------------------------------------------------------- the default (c4) line
+def host_cpus():
+    """Threads the CPU legs use: the CPUs this process may run on, capped by a cgroup CPU
+    quota when one is set. Returns (threads, dict describing the host)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    from shipfem import results
+
+    threads = min(aff, quota) if quota else aff
+    return threads, dict(cpu_model=results.cpu_model(), logical_cpus=os.cpu_count(), affinity=aff,
+                         cgroup_quota_cpus=quota)
+
+
+def cpu_baseline_c4(cases, repeats=5):
+    """The reference's FP32 stepCpu (fem.cpp:187-215), restated bit-exactly in C, on the
+    64 batched meshes: all threads this process may use (static split) and one core, and
+    the compiled reference itself on one core for the largest case. Interleaved repeats,
+    each after an idle check (/proc/stat busy fraction), reduced by the minimum time.
+    The reference has no J2 on tets: the FP64 oracle (with J2, the workload's own
+    algorithm) is timed beside it for information."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from shipfem import results
+
+    threads, host = host_cpus()
+    sims = [O.Ref32(c.X.astype(np.float32), c.index, ((c.flags & 7) == 7).astype(np.uint32)) for c in cases]
+    tets = sum(c.T for c in cases)
+    big = max(cases, key=lambda c: c.T)
+    R = O.ref_lib()
+    oracle = [O.Case(c.X, c.index, dt=1e-7, **c.oracle_kwargs()) for c in cases[:16]]
+    otets = sum(c.T for c in cases[:16])
+    best = dict(all=float("inf"), one=float("inf"), ref=float("inf"), oracle=float("inf"))
+    busy = []
+    dt = 1e-7
+    for c in sims:  # warm-up: first touch of every array
+        c.step(dt, threads=threads)
+
+    def ref_step(steps):
+        pos = big.X.astype(np.float32).copy()
+        vel = np.zeros_like(pos)
+        fixed = ((big.flags & 7) == 7).astype(np.uint32)
+        t0 = time.perf_counter()
+        R.ref_step(O._p(big.X.astype(np.float32), O.C.c_float), O._p(pos, O.C.c_float),
+                   O._p(vel, O.C.c_float), O._p(fixed, O.C.c_uint32), big.N,
+                   O._p(np.ascontiguousarray(big.index, np.uint32), O.C.c_uint32), big.T, 210e9, 0.3,
+                   7850.0, dt, 0.0, 1.0, steps)
+        return time.perf_counter() - t0
+
+    for _ in range(repeats):
+        for leg in ("all", "one", "ref", "oracle"):
+            if leg == "ref" and R is None:
+                continue
+            b = results.host_busy(0.2)
+            if b is not None:
+                busy.append(b)
+            if leg in ("all", "one"):
+                th = threads if leg == "all" else 1
+                t0 = time.perf_counter()
+                for c in sims:
+                    c.step(dt, threads=th)
+                t = time.perf_counter() - t0
+            elif leg == "ref":
+                t = ref_step(3) - ref_step(0)  # stepCpu alone: the rest-state set-up cancels
+                t /= 3
+            else:
+                t0 = time.perf_counter()
+                for o in oracle:
+                    o.run(1, threads=threads)
+                t = time.perf_counter() - t0
+            best[leg] = min(best[leg], t)
+    rate = tets / best["all"]
+    extra = dict(one_core=tets / best["one"],
+                 reference_one_core=(big.T / best["ref"]) if R is not None else None,
+                 oracle_fp64_j2_all_threads=otets / best["oracle"],
+                 repeats=repeats, reduce="minimum", host_busy_max=max(busy) if busy else None, **host)
+    return dict(value=rate, unit="tet-steps/s", cores=threads, kind="port",
+                sample=(f"the reference's FP32 stepCpu restated in C (bit-identical to the compiled "
+                        f"reference on golden vectors), elastic (the reference has no J2 on tets), one "
+                        f"step of all 64 batched meshes ({tets} tets) per repeat on {threads} threads; "
+                        f"{repeats} interleaved repeats, minimum; one-core leg the same step on one "
+                        f"thread; compiled reference on one core: 3 steps of the largest case "
+                        f"({big.T} tets) net of its set-up; FP64 oracle with J2: one step of the first "
+                        f"16 cases ({otets} tets)"), **extra)
+
+
+def roofline_entry(kernel, ms, need, model_bytes, counters=None):
+    """Roofline of one pass: `need` is the bytes its layout must move (read + write)."""
+    achieved = need / (ms * 1e-3) / 1e9
+    r = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+             frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=kernel,
+             kernel_ms=round(ms, 5), algorithmic_bytes=int(need),
+             model_bytes=int(model_bytes), model_frac=round(model_bytes / (ms * 1e-3) / 8e12, 4))
+    if counters:
+        r["traffic"] = counters.get("bytes")
+        r["traffic_source"] = counters.get("source")
+    return r
+
+
+def counters_for(name, kernel_key):
+    t = traffic_from_profiles(name)
+    if not t or kernel_key not in t:
+        return None
+    return dict(bytes=t[kernel_key], source=t.get("source", f"profiles/traffic_{name}.json"))
+
+
+PRE_STEPS = 500  # untimed steps before the timed ones in the batched configuration
+
+
+def chunk_for(steps):
+    """Graph chunk: even, at most 100, at most the timed step count."""
+    return max(2, min(100, steps - steps % 2))
+
+
+def model_bytes(c):
+    """The survey's per-step model (168 T + 140 N, + 56 T with J2), split by pass."""
+    e = (ELEM_B_T + (PLASTIC_B_T if c.plastic else 0)) * c.T + ELEM_B_N * c.N
+    n = NODE_B_T * c.T + NODE_B_N * c.N
+    return e, n
+
+
+def bench_headline(args, sf, info):
+    import numpy as np
+
+    from shipfem import configs
+
+    legs = set(args.legs.split(","))
+    K, W = args.steps, args.warmup
+    cases = configs.batch_cases(64)
+    tets = sum(c.T for c in cases)
+    nodes = sum(c.N for c in cases)
+    S = chunk_for(K)
+    t0 = time.perf_counter()
+    b = sf.Batch(cases, chunk=S)
+    prep_s = time.perf_counter() - t0
+    arena = b.footprint()  # creation copied this many bytes device to device once
+    # into the loaded regime (every case's load is at full ramp and the stiffest plates
+    # yield): the timed steps follow `pre` untimed ones, as in the middle of a 1000-step run
+    pre = S * max(1, round(PRE_STEPS / S))
+    b.run(pre)
+    best = b.tune(steps_per_trial=max(S, 100))  # restores the state it started from
+    b.run(S * -(-W // S))  # warm-up, whole chunks
+    b.run(0)  # the batch's stream is idle: the timed region starts from a synchronized device
+    t0 = time.perf_counter()
+    dev_ms = b.run(K)  # returns after the stream has drained
+    wall = time.perf_counter() - t0
+    states = [b.readback(i) for i in range(len(cases))]
+    bad = [i for i, st in enumerate(states) if st.status]
+    if bad:
+        raise SystemExit(f"non-finite cases {bad}")
+    value = tets * K / wall
+    eqps_max = max(float(np.max(st.eqps)) for st in states)
+    yielded = sum(int(np.count_nonzero(st.eqps)) for st in states)
+    timed_from = int(states[0].step) - K
+    del states
+    # per-kernel durations, live: the timed region's device time per step (HIP events around
+    # the graph replay on the batch's stream) apportioned between the two passes by their
+    # durations when each dispatch carries its own events (those read a few us long each:
+    # the event markers' own latency, which the ratio cancels)
+    b.run(-int(timed_from + K) % S)  # back onto a chunk boundary
+    ev_elem, ev_node = b.profile(S * max(1, -(-40 // S)))
+    step_ms = dev_ms / K
+    elem_ms = step_ms * ev_elem / (ev_elem + ev_node)
+    node_ms = step_ms * ev_node / (ev_elem + ev_node)
+    tr = b.traffic()
+    elem_need = tr["elem_read"] + tr["elem_write"]
+    node_need = tr["node_read"] + tr["node_write"]
+    me = sum(model_bytes(c)[0] for c in cases)
+    mn = sum(model_bytes(c)[1] for c in cases)
+    if elem_ms >= node_ms:
+        roofline = roofline_entry("element pass (k_elem_tiled, J2), all 64 cases in one launch", elem_ms,
+                                  elem_need, me, counters_for("batch", "element"))
+    else:
+        roofline = roofline_entry("node pass (k_node), all 64 cases in one launch", node_ms, node_need, mn,
+                                  counters_for("batch", "node"))
+    roofline.update(elem_ms=round(elem_ms, 5), node_ms=round(node_ms, 5),
+                    event_elem_ms=round(ev_elem, 5), event_node_ms=round(ev_node, 5),
+                    elem_bytes=int(elem_need), node_bytes=int(node_need),
+                    node_frac=round(node_need / (node_ms * 1e-3) / 8e12, 4))
+    step_need = elem_need + node_need
+    # the ceiling of the batched rate: every step's compulsory bytes at the HBM peak
+    batched_bound = tets / (step_need / 8e12)
+    b.close()
+    out = dict(metric="tet-element-steps/s", value=value, unit="tet-steps/s", n_gpus=1, steps=K, warmup=W,
+               ms_per_step=wall * 1e3 / K, higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="fp32", data="synthetic")
+    cfg = dict(workload=("batched: 64 clamped steel plates 20 mm thick, 4 tets through the thickness, tets "
+                         "log-uniform in [1e4, 3e5] (seed 7), 50-400 kPa ramped over 0.25 ms, J2 yield "
+                         "235-460 MPa, H 1 GPa, FP32 + Kahan state; one element and one node launch per "
+                         f"step over all cases, graph chunk {S}"),
+               cases=64, tets=tets, nodes=nodes, device=info[0], host_prep_s=round(prep_s, 2),
+               batch_arena_bytes=arena,
+               tuned=best, device_ms_per_step=dev_ms / K, timed_steps_from=timed_from,
+               eqps_max=eqps_max, yielded_tets=yielded,
+               step_bytes=int(step_need), step_hbm_frac=round(step_need * K / (dev_ms * 1e-3) / 8e12, 4),
+               batched_bound_at_hbm_peak=batched_bound)
+    if "plain" in legs:
+        # the plain path on the same 64 cases: one solver per case, two plain launches per
+        # step (the reference's dispatch pattern), the cases stepped one after another
+        per = []
+        plain_time = 0.0
+        for c in cases:
+            s = sf.Solver(c)
+            s.run(timed_from - W)  # from the state the batched steps were timed from
+            s.run(W, mode=1)
+            t1 = time.perf_counter()
+            s.run(K, mode=1)
+            dt_ = time.perf_counter() - t1
+            plain_time += dt_
+            per.append((c.T, c.T * K / dt_))
+            s.close()
+        plain_rate = tets * K / plain_time
+        bins = [(1e4, 3e4), (3e4, 1e5), (1e5, 3.01e5)]
+        by_bin = []
+        for lo, hi in bins:
+            xs = [(t, r) for t, r in per if lo <= t < hi]
+            if xs:
+                T_ = sum(t for t, _ in xs)
+                by_bin.append(dict(tets=[int(lo), int(hi)], cases=len(xs),
+                                   rate=T_ / sum(t / r for t, r in xs),
+                                   us_per_step=1e6 * sum(t / r for t, r in xs) / len(xs)))
+        cfg.update(plain_rate=plain_rate, batched_over_plain=value / plain_rate,
+                   batched_over_plain_bound=batched_bound / plain_rate, plain_by_size=by_bin)
+    if "plate" in legs:
+        case, desc = workload("plate")
+        s = sf.Solver(case)
+        s.run(20)
+        t1 = time.perf_counter()
+        n = 200
+        pdev = s.run(n)
+        pw = time.perf_counter() - t1
+        qe, qn = s.profile(40)
+        pe, pn = pdev / n * qe / (qe + qn), pdev / n * qn / (qe + qn)
+        ptr = s.traffic()
+        pe_need = ptr["elem_read"] + ptr["elem_write"]
+        cfg["plate"] = dict(workload=desc, tets=case.T, nodes=case.N, steps=n, rate=case.T * n / pw,
+                            ms_per_step=pw * 1e3 / n,
+                            roofline=roofline_entry("element pass (k_elem_tiled)", pe, pe_need,
+                                                    model_bytes(case)[0], counters_for("plate", "element")),
+                            node_ms=round(pn, 5), node_bytes=ptr["node_read"] + ptr["node_write"],
+                            node_frac=round((ptr["node_read"] + ptr["node_write"]) / (pn * 1e-3) / 8e12, 4),
+                            vs_reference_sustained=case.T * n / pw / REFERENCE_SUSTAINED)
+        s.close()
+    if "serve" in legs:
+        cfg["served"] = serve_leg(sf, rates=(20.0, 100.0), duration=args.duration, pool_n=12)
+    cpu = None
+    if "cpu" in legs and not args.no_cpu:
+        try:
+            cpu = cpu_baseline_c4(cases)
+        except Exception as e:  # reported, never fatal for the GPU number
+            cpu = dict(value=None, unit="tet-steps/s", cores=0, kind="port", sample=f"failed: {e}")
+    out.update(config=cfg, roofline=roofline, cpu_baseline=cpu)
+    emit(out)
+
+
+def serve_leg(sf, rates, duration, pool_n, seed=11):
+    """Poisson arrivals of stiffened-panel indentation cases (tets log-uniform in
+    [5e3, 5e5], indenter 2-10 m/s, run to 2 ms): p50/p99 latency and throughput."""
+    import math
+
+    import numpy as np
+
+    from shipfem import configs
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(pool_n):
+        t = math.exp(rng.uniform(math.log(5e3), math.log(5e5)))
+        pool.append(configs.served_case(t, float(rng.uniform(2.0, 10.0))))
+    rows = []
+    for lam in rates:
+        srv = sf.Server(chunk=64)
+        n = max(8, int(lam * duration))
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+        picks = rng.integers(0, len(pool), n)
+        t0 = time.perf_counter()
+        for k in range(n):
+            wait = arrivals[k] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(pool[picks[k]], 2e-3)
+        if not srv.drain(600.0):
+            raise SystemExit("server did not drain")
+        wall = time.perf_counter() - t0
+        res = srv.poll(n + 16)
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r["done_s"] - r["submit_s"] for r in res])
+        ok = sum(1 for r in res if r["status"] == sf.OK)
+        rows.append(dict(rate=lam, requests=n, completed=ok, p50_s=float(np.percentile(lat, 50)),
+                         p99_s=float(np.percentile(lat, 99)),
+                         tet_steps_per_s=st["element_steps"] / wall,
+                         device_busy=st["device_ms"] * 1e-3 / wall))
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    return dict(workload="Poisson arrivals of 6x3 m stiffened panels (12 mm plate, 4 flat bars), tets "
+                         f"log-uniform [5e3, 5e5], r = 0.5 m sphere at 2-10 m/s, run to 2 ms; pool of "
+                         f"{pool_n} meshes", rates=rows)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="plate")
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="c4")
+    ap.add_argument("--legs", default="batch,plain,plate,serve,cpu",
+                    help="c4 only: which legs beside the batched one to run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=40)
     ap.add_argument("--variant", default="tiled", choices=["tiled", "slots"])
@@ -415,6 +730,8 @@ def main():
         from shipfem import results
 
         rows = results.Rows(args.csv, results.run_identity(info[0]))
+    if args.workload == "c4":
+        return bench_headline(args, sf, info)
     if args.workload == "sweep":
         return bench_sweep(args, sf, info, rows)
     if args.workload == "batch":

@@ -299,6 +299,12 @@ shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* 
                                       double* node_ms);
 /* Device bytes held by the solver. */
 shipfem_status shipfem_solver_footprint(const shipfem_solver* s, uint64_t* bytes);
+/* The HBM bytes one step's passes must move, by the layout of this case: out[0], out[1]
+ * the element pass's reads and writes, out[2], out[3] the node pass's. Every plane a pass
+ * reads is counted once (a node's displacement once, however many tiles gather it) and
+ * every plane it writes once; not counted are the J2 history a yielding tet writes back and
+ * the per-wave FP64 sums. The roofline of a pass is these bytes over its duration. */
+shipfem_status shipfem_solver_traffic(const shipfem_solver* s, uint64_t out[4]);
 void shipfem_solver_destroy(shipfem_solver* s);
 
 /* One torn tet: the step whose element pass tore it and the failure plane, the
@@ -526,6 +532,14 @@ shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device
 shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
                                   shipfem_launch_config* best, double* best_ms_per_step);
 shipfem_status shipfem_batch_set_config(shipfem_batch* b, const shipfem_launch_config* cfg);
+/* shipfem_solver_profile for a batch: `steps` (whole chunks, from a chunk boundary) plain
+ * steps with per-dispatch events; mean element-pass and node-pass kernel durations. */
+shipfem_status shipfem_batch_profile(shipfem_batch* b, int32_t steps, double* elem_ms, double* node_ms);
+/* Device bytes of the batch's case arenas (one allocation; creation copies it once,
+ * device to device, as the state reset() restores). */
+shipfem_status shipfem_batch_footprint(const shipfem_batch* b, uint64_t* bytes);
+/* shipfem_solver_traffic summed over the batch's cases. */
+shipfem_status shipfem_batch_traffic(const shipfem_batch* b, uint64_t out[4]);
 shipfem_status shipfem_batch_readback(shipfem_batch* b, uint32_t case_index,
                                       shipfem_state_view* view);
 shipfem_status shipfem_batch_energy(shipfem_batch* b, uint32_t case_index,

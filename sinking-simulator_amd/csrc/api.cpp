@@ -347,28 +347,27 @@ shipfem_status shipfem_solver_set_launch(shipfem_solver* s, const shipfem_launch
     return statusOf(e);
 }
 
-shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* elem_ms,
-                                      double* node_ms) {
-    if (!s || steps < 1) return SHIPFEM_INVALID_ARG;
-    Plan& p = s->plan;
-    // two events per kernel, recorded by the dispatches themselves: the elapsed time is
-    // the kernel's execution, the figure a kernel trace reports
+namespace {
+// Per-kernel timing of `steps` plain steps of a plan: every element and node launch carries
+// its own start/stop events (recorded by the dispatch), so the elapsed time is the
+// kernel's execution. Samples fall on host steps that are multiples of S, as in a run.
+hipError_t profilePlan(Plan& p, int64_t& host_step, int S, int steps, hipStream_t stream, double* elem_ms,
+                       double* node_ms) {
     std::vector<hipEvent_t> ev(4 * static_cast<size_t>(steps), nullptr);
     hipError_t e = hipSuccess;
     for (auto& x : ev)
         if (e == hipSuccess) e = hipEventCreate(&x);
-    const int S = s->hc.sample_every;
     for (int k = 0; k < steps && e == hipSuccess; ++k) {
-        const int sample = s->host_step % S == 0 ? 1 : 0;
+        const int sample = host_step % S == 0 ? 1 : 0;
         const KernelTiming te{ev[4 * k], ev[4 * k + 1]}, tn{ev[4 * k + 2], ev[4 * k + 3]};
-        p.launchElem(p.parity, sample, s->stream, &te);
-        p.launchNode(p.parity, sample, s->stream, &tn);
-        if (sample) launch_reduce(p.d_recs, static_cast<int>(p.recs.size()), p.parity, s->stream);
+        p.launchElem(p.parity, sample, stream, &te);
+        p.launchNode(p.parity, sample, stream, &tn);
+        if (sample) launch_reduce(p.d_recs, static_cast<int>(p.recs.size()), p.parity, stream);
         e = hipGetLastError();
         p.parity ^= 1;
-        ++s->host_step;
+        ++host_step;
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     double te = 0, tn = 0;
     for (int k = 0; k < steps && e == hipSuccess; ++k) {
         float a = 0, b = 0;
@@ -379,9 +378,31 @@ shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* 
     }
     for (auto& x : ev)
         if (x) (void)hipEventDestroy(x);
-    if (e != hipSuccess) return statusOf(e);
-    if (elem_ms) *elem_ms = te / steps;
-    if (node_ms) *node_ms = tn / steps;
+    if (e == hipSuccess) {
+        if (elem_ms) *elem_ms = te / steps;
+        if (node_ms) *node_ms = tn / steps;
+    }
+    return e;
+}
+
+void addBytes(const PassBytes& b, uint64_t out[4]) {
+    out[0] += b.elem_read;
+    out[1] += b.elem_write;
+    out[2] += b.node_read;
+    out[3] += b.node_write;
+}
+}  // namespace
+
+shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* elem_ms,
+                                      double* node_ms) {
+    if (!s || steps < 1) return SHIPFEM_INVALID_ARG;
+    return statusOf(profilePlan(s->plan, s->host_step, s->hc.sample_every, steps, s->stream, elem_ms, node_ms));
+}
+
+shipfem_status shipfem_solver_traffic(const shipfem_solver* s, uint64_t out[4]) {
+    if (!s || !out) return SHIPFEM_INVALID_ARG;
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    addBytes(compulsoryBytes(s->hc, s->plan.shape.variant), out);
     return SHIPFEM_OK;
 }
 
@@ -525,6 +546,26 @@ shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->e0, b->e1);
     if (e != hipSuccess) return statusOf(e);
     if (device_ms) *device_ms = ms;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_batch_profile(shipfem_batch* b, int32_t steps, double* elem_ms, double* node_ms) {
+    if (!b || steps < 1) return SHIPFEM_INVALID_ARG;
+    // a captured chunk starts at parity 0 on a chunk boundary: profile whole chunks
+    if (b->host_step % b->S != 0 || b->plan.parity != 0 || steps % b->S != 0) return SHIPFEM_INVALID_ARG;
+    return statusOf(profilePlan(b->plan, b->host_step, b->S, steps, b->stream, elem_ms, node_ms));
+}
+
+shipfem_status shipfem_batch_footprint(const shipfem_batch* b, uint64_t* bytes) {
+    if (!b || !bytes) return SHIPFEM_INVALID_ARG;
+    *bytes = b->bytes;
+    return SHIPFEM_OK;
+}
+
+shipfem_status shipfem_batch_traffic(const shipfem_batch* b, uint64_t out[4]) {
+    if (!b || !out) return SHIPFEM_INVALID_ARG;
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    for (const HostCase& hc : b->hcs) addBytes(compulsoryBytes(hc, b->plan.shape.variant), out);
     return SHIPFEM_OK;
 }
 

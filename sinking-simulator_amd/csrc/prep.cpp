@@ -609,6 +609,46 @@ std::string prepareHexCase(const shipfem_case_desc& d, HostCase& hc, int32_t his
 
 uint64_t arenaBytes(const HostCase& hc) { return mapArena(hc).total; }
 
+PassBytes compulsoryBytes(const HostCase& hc, int variant) {
+    PassBytes b;
+    const uint64_t T = hc.T, N = hc.N, Np = hc.Npad, P = hc.P, nt = hc.elem_tiles;
+    const uint32_t f = hc.features;
+    uint64_t freeNodes = 0;
+    for (uint32_t n = 0; n < hc.N && n < hc.nflags.size(); ++n)
+        freeNodes += (hc.nflags[n] & SHIPFEM_PIN_ALL) != SHIPFEM_PIN_ALL;
+    const uint64_t history = (f & kFeatPlastic) ? ((f & kFeatDamage) ? 32 : 28) * T : 0;
+    if (hc.corners == 8) {
+        const bool plastic = (f & kFeatPlastic) != 0;
+        // connectivity, the stiffness block (elastic) or the FP64 rest geometry and the point
+        // history with the enhanced parameters (plastic), and the displacement with its
+        // compensation of every node; the 24-float force slots out (and the enhanced
+        // parameters, plastic)
+        b.elem_read = 32 * T + 28 * N +
+                      (plastic ? (8ull * kHexGeo + 8ull * (kHexPoint + 7)) * T : (8ull * kHexPacked + 4ull * kHexGeo) * T);
+        b.elem_write = 96 * T + (plastic ? 56 * T : 0);
+    } else if (variant == kSlots) {
+        b.elem_read = 16 * T + 36 * T + 4 * T + 16 * N + history;
+        b.elem_write = 48 * T;
+    } else {
+        // tile offsets, each tile's node ids and run offsets, 16-bit local connectivity,
+        // Dm^-1 and V, every node's displacement once; one float4 partial per (tile, node)
+        b.elem_read = 4 * (nt + 1) + 4 * P + 2 * (P + nt) + 8 * T + 36 * T + 4 * T + 16 * N + history;
+        b.elem_write = 16 * P;
+    }
+    const bool slots = hc.corners == 8 || variant == kSlots;
+    const uint64_t slab = slots ? (hc.ell_off.empty() ? 0 : hc.ell_off.back()) : (hc.pell_off.empty() ? 0 : hc.pell_off.back());
+    const uint64_t gathered = slots ? 12ull * hc.corners * T : 16 * P;
+    const bool needMass = (f & (kFeatGravity | kFeatContact | kFeatDrive | kFeatDamp)) != 0;
+    // the gather slab and its offsets, the gathered forces, flags, 1/m (and m), u, its
+    // compensation, v and its compensation, the optional load, contact and drive planes;
+    // a free node writes u, v and both compensations
+    b.node_read = 4 * (hc.node_waves + 1) + 4 * slab + gathered + 4 * Np + 4 * Np + (needMass ? 4 * Np : 0) +
+                  16 * N + 36 * Np + ((f & kFeatLoad) ? 12 * Np : 0) + ((f & kFeatContact) ? 16 * Np : 0) +
+                  ((f & kFeatDrive) ? 12 * Np : 0);
+    b.node_write = 52 * freeNodes;
+    return b;
+}
+
 hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_t s) {
     const ArenaMap m = mapArena(hc);
     dc.arena = base;

@@ -104,6 +104,15 @@ void hexForce(const double K[576], const double* X, const double* x, double out[
 // Device bytes of a prepared case (its arena), and the admission upper bound.
 uint64_t arenaBytes(const HostCase& hc);
 
+// The HBM bytes one element pass and one node pass of a case must move (a step that is not
+// a sample step): every plane the pass reads, once, and every plane it writes, once. The
+// J2 history a yielding tet writes back is not counted (an elastic tet writes none), nor
+// the per-wave FP64 sums (8 bytes per 64 items). `variant`: kTiled or kSlots.
+struct PassBytes {
+    uint64_t elem_read = 0, elem_write = 0, node_read = 0, node_write = 0;
+};
+PassBytes compulsoryBytes(const HostCase& hc, int variant);
+
 // One case's device memory: a single arena holding every plane of the case.
 struct DeviceCase {
     char* arena = nullptr;

@@ -159,6 +159,10 @@ SIGNATURES = {
     "shipfem_solver_set_launch": (S, [VP, C.POINTER(LaunchConfig)]),
     "shipfem_solver_profile": (S, [VP, C.c_int32, D, D]),
     "shipfem_solver_footprint": (S, [VP, C.POINTER(C.c_uint64)]),
+    "shipfem_solver_traffic": (S, [VP, C.POINTER(C.c_uint64)]),
+    "shipfem_batch_profile": (S, [VP, C.c_int32, D, D]),
+    "shipfem_batch_traffic": (S, [VP, C.POINTER(C.c_uint64)]),
+    "shipfem_batch_footprint": (S, [VP, C.POINTER(C.c_uint64)]),
     "shipfem_solver_tears": (S, [VP, C.POINTER(Tear), C.c_uint32, C.POINTER(C.c_uint32)]),
     "shipfem_regularised_failure_strain": (C.c_double, [C.c_double] * 4),
     "shipfem_torn_groups": (S, [D, U32, C.c_uint32, U32, F, C.c_double, U32, C.c_uint32,
@@ -788,6 +792,13 @@ class Solver:
         _check(lib().shipfem_solver_profile(self.h, steps, C.byref(e), C.byref(n)))
         return e.value, n.value
 
+    def traffic(self):
+        """The bytes one step's passes must move: dict elem_read, elem_write, node_read,
+        node_write (shipfem_solver_traffic)."""
+        out = (C.c_uint64 * 4)()
+        _check(lib().shipfem_solver_traffic(self.h, out))
+        return dict(zip(("elem_read", "elem_write", "node_read", "node_write"), list(out)))
+
     def export(self):
         """The state in the shape zone::Solver::adopt takes (ZoneGpuState): dict with
         position [N,3], displacement [N,3], velocity [N,3], plastic [T,8] (eps_p 6, eqps, damage), tear_step
@@ -874,6 +885,24 @@ class Batch:
         ms = C.c_double()
         _check(lib().shipfem_batch_run(self.h, steps, C.byref(ms)))
         return ms.value
+
+    def profile(self, steps):
+        """Mean (element-pass ms, node-pass ms) over `steps` (whole chunks) event-bracketed
+        steps (shipfem_batch_profile)."""
+        e, n = C.c_double(), C.c_double()
+        _check(lib().shipfem_batch_profile(self.h, steps, C.byref(e), C.byref(n)))
+        return e.value, n.value
+
+    def footprint(self):
+        b = C.c_uint64()
+        _check(lib().shipfem_batch_footprint(self.h, C.byref(b)))
+        return b.value
+
+    def traffic(self):
+        """shipfem_batch_traffic: bytes one step's passes must move, over all cases."""
+        out = (C.c_uint64 * 4)()
+        _check(lib().shipfem_batch_traffic(self.h, out))
+        return dict(zip(("elem_read", "elem_write", "node_read", "node_write"), list(out)))
 
     def tune(self, steps_per_trial=200):
         cfg = LaunchConfig()
