@@ -322,10 +322,14 @@ typedef struct shipfem_tear {
 shipfem_status shipfem_solver_tears(shipfem_solver* s, shipfem_tear* out, uint32_t max,
                                     uint32_t* count);
 
-/* Bytes a case of this shape will hold on the device (admission formula).
- * plastic: 0 elastic, 1 J2, 2 J2 with failure. */
+/* Bytes a tet case of this shape will hold on the device, for meshes whose nodes lie in at
+ * most 32 tets (Kuhn-type and panel meshes). plastic: 0 elastic, 1 J2, 2 J2 with failure. */
 uint64_t shipfem_case_footprint(uint32_t node_count, uint32_t tet_count, int32_t plastic,
                                 int32_t sample_capacity);
+/* An upper bound for any tet mesh: the same formula with the mesh's own largest node
+ * degree and up to four tile slots per tet (0 for an empty descriptor). The server refuses
+ * at submit a case whose bound exceeds its budget. */
+uint64_t shipfem_case_footprint_mesh(const shipfem_case_desc* desc, int32_t sample_capacity);
 
 /* The mesh-regularised failure strain (plasticity.cpp:158-169): an element of in-plane
  * size `length` in a plate of `thickness` reads the neck diluted by t/l,

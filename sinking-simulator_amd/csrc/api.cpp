@@ -136,19 +136,40 @@ shipfem_status shipfem_device_query(char* name, size_t name_len, uint64_t* hbm_b
     return SHIPFEM_OK;
 }
 
-uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t cap) {
-    // arenaBytes with every optional plane present, node degree <= 32 and at most two
-    // (tile, node) slots per tet (Kuhn-type meshes hold about 0.75)
+namespace {
+// arenaBytes with every optional plane present, no node in more than `maxdeg` elements (the
+// width of a wave's gather slabs) and at most P (tile, node) slots.
+uint64_t footprintBound(uint32_t N, uint32_t T, int32_t plastic, int32_t cap, uint64_t maxdeg, uint64_t P) {
     const uint64_t Np = roundUp64(N), Tp = roundUp64(T), nw = Np / 64, ew = Tp / 64;
-    const uint64_t nt = (T + kTile - 1) / kTile, P = 2ull * T;
+    const uint64_t nt = (T + kTile - 1) / kTile;
     const uint64_t pstate = plastic == 0 ? 0 : plastic == 1 ? 28ull * Tp : (32ull + 4 + 24) * Tp;
     uint64_t b = 16ull * T + 40ull * Tp + 48ull * T + pstate + 16ull * N +
-                 36ull * Np + 12ull * Np + 40ull * Np + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
+                 36ull * Np + 12ull * Np + 40ull * Np + 4ull * (nw + 1) + 4ull * 64 * maxdeg * nw +
                  8ull * T + 4ull * (nt + 1) + 4ull * P + 2ull * (P + nt) +
-                 16ull * P + 4ull * (nw + 1) + 4ull * 64 * 32 * nw +
+                 16ull * P + 4ull * (nw + 1) + 4ull * 64 * maxdeg * nw +
                  8ull * ((kAccElemSlots + kSmpElemSlots + 1) * ew + (kAccNodeSlots + kSmpNodeSlots) * nw) +
                  24 + sizeof(EnergyRecordDev) * (uint64_t)std::max(cap, 1);
     return b + 256ull * 40;  // each of the arena's planes rounds up to 256 bytes
+}
+}  // namespace
+
+uint64_t shipfem_case_footprint(uint32_t N, uint32_t T, int32_t plastic, int32_t cap) {
+    // Kuhn-type meshes: node degree <= 32, at most two (tile, node) slots per tet (they hold
+    // about 0.75)
+    return footprintBound(N, T, plastic, cap, 32, 2ull * T);
+}
+
+uint64_t shipfem_case_footprint_mesh(const shipfem_case_desc* d, int32_t cap) {
+    if (!d || !d->index || d->node_count == 0 || d->tet_count == 0) return 0;
+    // the mesh's own largest node degree bounds every wave's slab width; a tile holds at
+    // most 4 slots per tet
+    std::vector<uint32_t> deg(d->node_count, 0u);
+    uint64_t maxdeg = 0;
+    for (uint64_t q = 0; q < 4ull * d->tet_count; ++q)
+        if (d->index[q] < d->node_count) maxdeg = std::max<uint64_t>(maxdeg, ++deg[d->index[q]]);
+    const shipfem_material& m = d->material;
+    const int32_t kind = m.plastic ? (m.failure_strain > 0.0 ? 2 : 1) : 0;
+    return footprintBound(d->node_count, d->tet_count, kind, cap, maxdeg, 4ull * d->tet_count);
 }
 
 namespace {
@@ -268,7 +289,6 @@ shipfem_status shipfem_solver_set_params(shipfem_solver* s, const shipfem_step_p
     r.features = hc.features;
     setTimestep(s->dc, hc, p->dt);
     s->plan.recs[0] = r;
-    s->plan.feat_union = r.features;
     s->plan.dropGraph();
     return statusOf(s->plan.uploadRecords(s->stream));
 }
@@ -323,7 +343,7 @@ shipfem_status shipfem_solver_tet_forces(shipfem_solver* s, float* out) {
     if (!s || !out) return SHIPFEM_INVALID_ARG;
     // only the slots variant keeps per-tet force slots; the tiled one sums in LDS (a hex
     // case always writes slots)
-    if (s->plan.shape.variant != kSlots && !s->plan.hex) return SHIPFEM_UNSUPPORTED;
+    if (s->plan.shape.variant != kSlots && !s->plan.hex && s->hc.elem_tiles > 0) return SHIPFEM_UNSUPPORTED;
     const HostCase& hc = s->hc;
     const uint64_t per = 3ull * hc.corners;
     std::vector<float> f(per * hc.T);

@@ -8,13 +8,16 @@ namespace shipfem {
 
 namespace {
 
+// Appends the tiles of case i: `block` items each over its elements or nodes.
+void tilesOf(const std::vector<CaseDev>& recs, size_t i, int block, bool nodes, std::vector<Tile>& out) {
+    const uint32_t count = nodes ? recs[i].N : recs[i].T;
+    for (uint32_t first = 0; first < count; first += static_cast<uint32_t>(block))
+        out.push_back(Tile{static_cast<int32_t>(i), first});
+}
+
 void tilesFor(const std::vector<CaseDev>& recs, int block, bool nodes, std::vector<Tile>& out) {
     out.clear();
-    for (size_t i = 0; i < recs.size(); ++i) {
-        const uint32_t count = nodes ? recs[i].N : recs[i].T;
-        for (uint32_t first = 0; first < count; first += static_cast<uint32_t>(block))
-            out.push_back(Tile{static_cast<int32_t>(i), first});
-    }
+    for (size_t i = 0; i < recs.size(); ++i) tilesOf(recs, i, block, nodes, out);
 }
 
 }  // namespace
@@ -45,13 +48,24 @@ hipError_t Plan::build(hipStream_t s) {
             return hipErrorInvalidValue;
     // the plastic hex pass keeps 48 doubles a lane in LDS: at most 256 lanes a block
     if (hex && (recs[0].features & kFeatPlastic)) shape.elem_block = std::min(shape.elem_block, 256);
-    tilesFor(recs, shape.variant == kTiled && !hex ? kTile : shape.elem_block, false, elem_tiles);
+    // the tiled group first, then the slots group
+    elem_tiles.clear();
+    node_tiles.clear();
     lmax = 0;
-    for (const CaseDev& r : recs) lmax = std::max(lmax, r.lmax);
-    tilesFor(recs, shape.node_block, true, node_tiles);
+    for (int group = 0; group < 2; ++group) {
+        for (size_t i = 0; i < recs.size(); ++i) {
+            if (inTiledGroup(recs[i]) != (group == 0)) continue;
+            tilesOf(recs, i, group == 0 ? kTile : shape.elem_block, false, elem_tiles);
+            tilesOf(recs, i, shape.node_block, true, node_tiles);
+            if (group == 0) lmax = std::max(lmax, recs[i].lmax);
+        }
+        if (group == 0) {
+            elem_split = elem_tiles.size();
+            node_split = node_tiles.size();
+        }
+    }
     tilesFor(recs, 256, false, omega_tiles);
-    feat_union = 0;
-    for (const CaseDev& r : recs) feat_union |= r.features;
+    refreshFeatures();
     hipError_t e = hipSuccess;
     size_t need = std::max({elem_tiles.size(), node_tiles.size(), omega_tiles.size()});
     if (need > tile_cap || !d_elem) {
@@ -86,7 +100,17 @@ hipError_t Plan::build(hipStream_t s) {
     return e;
 }
 
+bool Plan::inTiledGroup(const CaseDev& r) const { return !hex && shape.variant == kTiled && r.elem_tiles > 0; }
+
+void Plan::refreshFeatures() {
+    feat_tiled = feat_slots = 0;
+    for (const CaseDev& r : recs) (inTiledGroup(r) ? feat_tiled : feat_slots) |= r.features;
+    feat_union = feat_tiled | feat_slots;
+}
+
 hipError_t Plan::uploadRecords(hipStream_t s) {
+    // a record's features choose the instantiations its group's launches take
+    refreshFeatures();
     if (recs.empty()) return hipSuccess;
     hipError_t e = hipMemcpyAsync(d_recs, recs.data(), sizeof(CaseDev) * recs.size(),
                                   hipMemcpyHostToDevice, s);
@@ -99,25 +123,48 @@ void Plan::launchStep(int par, int sample, hipStream_t s) {
     launchNode(par, sample, s);
 }
 
+namespace {
+// Timing of a phase issued as two launches: the start event on the first, the stop on the
+// second (one launch takes both).
+void splitTiming(const KernelTiming* t, bool both, KernelTiming& first, KernelTiming& second) {
+    if (!t) return;
+    first = *t;
+    second = *t;
+    if (both) {
+        first.stop = nullptr;
+        second.start = nullptr;
+    }
+}
+}  // namespace
+
 void Plan::launchElem(int par, int sample, hipStream_t s, const KernelTiming* t) {
-    if (hex && (feat_union & kFeatPlastic))
-        launch_elem_hex_plastic(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block,
-                                par, sample, shape.xcd_remap, s, t);
+    const int n1 = static_cast<int>(elem_split), n2 = static_cast<int>(elem_tiles.size() - elem_split);
+    KernelTiming t1, t2;
+    splitTiming(t, n1 > 0 && n2 > 0, t1, t2);
+    const KernelTiming* p1 = t ? &t1 : nullptr;
+    const KernelTiming* p2 = t ? &t2 : nullptr;
+    if (n1 > 0)
+        launch_elem_tiled(d_recs, d_elem, n1, lmax, par, sample, shape.xcd_remap, feat_tiled, s, p1);
+    if (n2 == 0) return;
+    if (hex && (feat_slots & kFeatPlastic))
+        launch_elem_hex_plastic(d_recs, d_elem + n1, n2, shape.elem_block, par, sample, shape.xcd_remap, s, p2);
     else if (hex)
-        launch_elem_hex(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
-                        sample, shape.xcd_remap, s, t);
-    else if (shape.variant == kTiled)
-        launch_elem_tiled(d_recs, d_elem, static_cast<int>(elem_tiles.size()), lmax, par, sample,
-                          shape.xcd_remap, feat_union, s, t);
+        launch_elem_hex(d_recs, d_elem + n1, n2, shape.elem_block, par, sample, shape.xcd_remap, s, p2);
     else
-        launch_elem(d_recs, d_elem, static_cast<int>(elem_tiles.size()), shape.elem_block, par,
-                    sample, shape.xcd_remap, feat_union, s, t);
+        launch_elem(d_recs, d_elem + n1, n2, shape.elem_block, par, sample, shape.xcd_remap, feat_slots, s, p2);
 }
 
 void Plan::launchNode(int par, int sample, hipStream_t s, const KernelTiming* t) {
-    // a hex case's element pass always writes slots
-    launch_node(d_recs, d_node, static_cast<int>(node_tiles.size()), shape.node_block, par, sample,
-                shape.xcd_remap, feat_union, hex ? kSlots : shape.variant, s, t);
+    // the tiled group gathers tile partials; the slots group (and every hex case) slots
+    const int n1 = static_cast<int>(node_split), n2 = static_cast<int>(node_tiles.size() - node_split);
+    KernelTiming t1, t2;
+    splitTiming(t, n1 > 0 && n2 > 0, t1, t2);
+    if (n1 > 0)
+        launch_node(d_recs, d_node, n1, shape.node_block, par, sample, shape.xcd_remap, feat_tiled, kTiled, s,
+                    t ? &t1 : nullptr);
+    if (n2 > 0)
+        launch_node(d_recs, d_node + n1, n2, shape.node_block, par, sample, shape.xcd_remap, feat_slots, kSlots,
+                    s, t ? &t2 : nullptr);
 }
 
 void Plan::stepPlain(bool sample, hipStream_t s) {

@@ -190,12 +190,14 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, int6
             tr[k] = 2.0f * c.mu * (el[k] - vol * (1.0f / 3.0f));
             tr[3 + k] = c.mu * el[3 + k];
         }
-        const float nrm = sqrtf(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
-                                2.0f * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]));
+        // an elastic tet tests the yield surface by squared norms and takes no root
+        const float nrm2 = tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
+                           2.0f * (tr[3] * tr[3] + tr[4] * tr[4] + tr[5] * tr[5]);
         const float root23 = 0.81649658092772603f;
         const float radius = root23 * (c.yield + c.hard * fmaxf(ep[6], 0.0f));
         float sv[6];
-        if (nrm > radius) {
+        if (nrm2 > radius * radius) {
+            const float nrm = sqrtf(nrm2);
             const float gamma = (nrm - radius) / (2.0f * c.mu + (2.0f / 3.0f) * c.hard);
             const float inv = 1.0f / nrm;
             float d = 0.0f;
@@ -322,11 +324,13 @@ __device__ __forceinline__ void element_wave_sums(const CaseDev& c, uint32_t e, 
                                                   double diss, double se, double fract) {
     const uint32_t w = e >> 6;
     const bool writer = (threadIdx.x & 63u) == 0 && w < c.elem_waves;
-    if (PLASTIC && (c.features & kFeatPlastic)) {
+    // A wave none of whose tets yielded (or tore) adds +0 to its running sum, which leaves
+    // the sum's bits as they are: it skips the butterfly and the store.
+    if (PLASTIC && (c.features & kFeatPlastic) && __ballot(diss != 0.0)) {
         const double s = wave_sum(diss);
         if (writer) c.acc_e[kAccDiss * c.elem_waves + w] += s;
     }
-    if (DAMAGE && (c.features & kFeatDamage)) {
+    if (DAMAGE && (c.features & kFeatDamage) && __ballot(fract != 0.0)) {
         const double s = wave_sum(fract);
         if (writer) c.acc_e[kAccFract * c.elem_waves + w] += s;
     }
@@ -994,20 +998,21 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
 
     const uint32_t nw = c.node_waves;
     const bool writer = lane == 0 && w < nw;
-    if (c.features & (kFeatLoad | kFeatGravity)) {
+    // a running sum a wave adds only zeros to keeps its bits: such waves skip the butterfly
+    if ((c.features & (kFeatLoad | kFeatGravity)) && __ballot(a_ext != 0.0)) {
         const double s = wave_sum(a_ext);
         if (writer) c.acc_n[kAccWext * nw + w] += s;
     }
     if (EXTRA) {
-        if (c.features & kFeatContact) {
+        if ((c.features & kFeatContact) && __ballot(a_con != 0.0)) {
             const double s = wave_sum(a_con);
             if (writer) c.acc_n[kAccWcon * nw + w] += s;
         }
-        if (c.features & kFeatDrive) {
+        if ((c.features & kFeatDrive) && __ballot(a_bnd != 0.0)) {
             const double s = wave_sum(a_bnd);
             if (writer) c.acc_n[kAccWbnd * nw + w] += s;
         }
-        if (c.features & kFeatDamp) {
+        if ((c.features & kFeatDamp) && __ballot(a_damp != 0.0)) {
             const double s = wave_sum(a_damp);
             if (writer) c.acc_n[kAccDamp * nw + w] += s;
         }

@@ -481,10 +481,20 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
                 }
             std::sort(nodes.begin(), nodes.end());
             const uint32_t L = static_cast<uint32_t>(nodes.size());
-            for (uint32_t g : nodes)
-                if (deg[g] > 64)
-                    return "node " + std::to_string(externalNode(hc, g)) +
-                           " belongs to more than 64 tets of one 256-tet tile";
+            bool fits = true;
+            for (uint32_t g : nodes) fits = fits && deg[g] <= 64;
+            if (!fits) {
+                // a corner's rank in its node's run has 6 bits: a node in more than 64 tets of
+                // one tile cannot be tiled. The case keeps no tiles and every plan runs it on
+                // the slots variant (the reference's scheme, any node degree).
+                hc.elem_tiles = 0;
+                hc.P = hc.lmax = 0;
+                std::vector<uint32_t>().swap(hc.lconn);
+                std::vector<uint32_t>().swap(hc.tn_off);
+                std::vector<uint32_t>().swap(hc.tnode);
+                std::vector<uint16_t>().swap(hc.lofs);
+                break;
+            }
             for (uint32_t l = 0; l < L; ++l) loc[nodes[l]] = l;
             hc.tn_off[b + 1] = hc.tn_off[b] + L;
             hc.lmax = std::max(hc.lmax, L);
@@ -502,6 +512,9 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
                 hc.lconn[2ull * t + 1] = wv[2] | (wv[3] << 16);
             }
         }
+    }
+    if (corners == 4 && hc.elem_tiles > 0) {
+        const uint32_t nt = hc.elem_tiles;
         hc.P = hc.tn_off[nt];
         // node -> its partial slots in tile order, wave-transposed like the CSR slab
         std::vector<uint32_t> poff(N + 1, 0u);
@@ -611,6 +624,7 @@ uint64_t arenaBytes(const HostCase& hc) { return mapArena(hc).total; }
 
 PassBytes compulsoryBytes(const HostCase& hc, int variant) {
     PassBytes b;
+    if (hc.elem_tiles == 0) variant = kSlots;  // a case without tiles always runs slots
     const uint64_t T = hc.T, N = hc.N, Np = hc.Npad, P = hc.P, nt = hc.elem_tiles;
     const uint32_t f = hc.features;
     uint64_t freeNodes = 0;

@@ -627,8 +627,7 @@ shipfem_status submitCase(shipfem_server* s, const shipfem_case_desc* d, double 
     // a tet case is refused at once when its footprint bound exceeds the budget; a hex
     // case is measured after its set-up (a result with SHIPFEM_BUSY when it can never fit)
     if (!hex) {
-        const int32_t kind = d->material.plastic ? (d->material.failure_strain > 0.0 ? 2 : 1) : 0;
-        const uint64_t bound = shipfem_case_footprint(d->node_count, d->tet_count, kind, 512);
+        const uint64_t bound = shipfem_case_footprint_mesh(d, 512);
         if (bound > s->opts.memory_budget) {
             std::lock_guard<std::mutex> lk(s->m);
             ++s->st.refused;

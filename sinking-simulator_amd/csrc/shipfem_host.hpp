@@ -132,7 +132,13 @@ void setTimestep(DeviceCase& dc, const HostCase& hc, double dt);
 struct Plan {
     std::vector<CaseDev> recs;     // host copies
     CaseDev* d_recs = nullptr;     // device array
+    // Element and node tiles: the cases the tiled pass runs first (elem_split and node_split
+    // tiles), then those on the slots pass (every case of a SLOTS plan, a hex plan's cases,
+    // and tet cases whose mesh cannot be tiled). A case is in one group only, so each runs
+    // the same kernel and sum order whatever else is in the plan.
     std::vector<Tile> elem_tiles, node_tiles, omega_tiles;
+    size_t elem_split = 0, node_split = 0;
+    uint32_t feat_tiled = 0, feat_slots = 0;
     Tile* d_elem = nullptr;
     Tile* d_node = nullptr;
     Tile* d_omega = nullptr;
@@ -150,8 +156,11 @@ struct Plan {
     ~Plan();
     // (Re)builds tiles and the device record array for `recs` with `shape`.
     hipError_t build(hipStream_t stream);
-    // Uploads the host record copies only (after a scalar change).
+    // Uploads the host record copies only (after a scalar or feature change), and takes the
+    // groups' feature sets from them.
     hipError_t uploadRecords(hipStream_t stream);
+    bool inTiledGroup(const CaseDev& r) const;
+    void refreshFeatures();
     void dropGraph();
     // The element and node launches of one step at `parity` (no reduce).
     void launchStep(int parity, int sample, hipStream_t stream);

@@ -163,6 +163,7 @@ SIGNATURES = {
     "shipfem_batch_profile": (S, [VP, C.c_int32, D, D]),
     "shipfem_batch_traffic": (S, [VP, C.POINTER(C.c_uint64)]),
     "shipfem_batch_footprint": (S, [VP, C.POINTER(C.c_uint64)]),
+    "shipfem_case_footprint_mesh": (C.c_uint64, [C.POINTER(CaseDesc), C.c_int32]),
     "shipfem_solver_tears": (S, [VP, C.POINTER(Tear), C.c_uint32, C.POINTER(C.c_uint32)]),
     "shipfem_regularised_failure_strain": (C.c_double, [C.c_double] * 4),
     "shipfem_torn_groups": (S, [D, U32, C.c_uint32, U32, F, C.c_double, U32, C.c_uint32,
@@ -1009,3 +1010,9 @@ class Server:
 def case_footprint(N, T, plastic=False, records=64):
     # plastic: False/0 elastic, True/1 J2, 2 J2 with failure
     return lib().shipfem_case_footprint(N, T, int(plastic), records)
+
+
+def case_footprint_mesh(case, records=64):
+    """An upper bound of a tet case's device bytes from its own mesh (node degree)."""
+    d = case.desc()
+    return lib().shipfem_case_footprint_mesh(C.byref(d), records)
