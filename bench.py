@@ -182,8 +182,8 @@ def bench_serve(args, sf, info):
     rows = []
     total_tet_steps = 0.0
     total_wall = 0.0
-    for lam in rates:
-        srv = sf.Server(chunk=64)
+    for lam, policy in [(r, p) for p in policies for r in rates]:
+        srv = sf.Server(chunk=64, policy=policy)
         n = max(4, int(lam * args.duration))
         arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
         picks = rng.integers(0, len(pool), n)
@@ -202,7 +202,8 @@ def bench_serve(args, sf, info):
         lat = np.array([r["done_s"] - r["submit_s"] for r in res])
         qwait = np.array([r["admit_s"] - r["submit_s"] for r in res])
         ok = sum(1 for r in res if r["status"] == sf.OK)
-        rows.append(dict(rate=lam, requests=n, completed=ok, p50_s=float(np.percentile(lat, 50)),
+        rows.append(dict(rate=lam, policy="shortest" if policy == sf.ROUNDS_SHORTEST else "all", requests=n,
+                         completed=ok, p50_s=float(np.percentile(lat, 50)),
                          p99_s=float(np.percentile(lat, 99)),
                          p99_queue_s=float(np.percentile(qwait, 99)),
                          tet_steps_per_s=st["element_steps"] / wall,
@@ -647,7 +648,7 @@ def bench_headline(args, sf, info):
     emit(out)
 
 
-def serve_leg(sf, rates, duration, pool_n, seed=11):
+def serve_leg(sf, rates, duration, pool_n, seed=11, policies=None):
     """Poisson arrivals of stiffened-panel indentation cases (tets log-uniform in
     [5e3, 5e5], indenter 2-10 m/s, run to 2 ms): p50/p99 latency and throughput."""
     import math
@@ -661,12 +662,16 @@ def serve_leg(sf, rates, duration, pool_n, seed=11):
     for _ in range(pool_n):
         t = math.exp(rng.uniform(math.log(5e3), math.log(5e5)))
         pool.append(configs.served_case(t, float(rng.uniform(2.0, 10.0))))
+    policies = policies or [sf.ROUNDS_ALL, sf.ROUNDS_SHORTEST]
     rows = []
+    # each rate's arrivals and picks are drawn once: both policies serve the same stream
+    streams = {}
     for lam in rates:
-        srv = sf.Server(chunk=64)
         n = max(8, int(lam * duration))
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
-        picks = rng.integers(0, len(pool), n)
+        streams[lam] = (n, np.cumsum(rng.exponential(1.0 / lam, n)), rng.integers(0, len(pool), n))
+    for lam, policy in [(r, p) for p in policies for r in rates]:
+        srv = sf.Server(chunk=64, policy=policy)
+        n, arrivals, picks = streams[lam]
         t0 = time.perf_counter()
         for k in range(n):
             wait = arrivals[k] - (time.perf_counter() - t0)
@@ -681,7 +686,8 @@ def serve_leg(sf, rates, duration, pool_n, seed=11):
         srv.close()
         lat = np.array([r["done_s"] - r["submit_s"] for r in res])
         ok = sum(1 for r in res if r["status"] == sf.OK)
-        rows.append(dict(rate=lam, requests=n, completed=ok, p50_s=float(np.percentile(lat, 50)),
+        rows.append(dict(rate=lam, policy="shortest" if policy == sf.ROUNDS_SHORTEST else "all", requests=n,
+                         completed=ok, p50_s=float(np.percentile(lat, 50)),
                          p99_s=float(np.percentile(lat, 99)),
                          tet_steps_per_s=st["element_steps"] / wall,
                          device_busy=st["device_ms"] * 1e-3 / wall))

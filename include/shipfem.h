@@ -115,6 +115,14 @@ typedef struct shipfem_case_desc {
      * the density bound times sqrt(min over such nodes of mass / (density share)), which
      * bounds the lumped system's largest frequency from below. */
     const double* mass;
+    /* Solid-shell hexes with ductile failure: the local fracture strain eps_f
+     * (Failure::fractureStrain, plasticity.hpp:194-195). When > 0, every element resolves
+     * its own failure strain from its size, as initialisePlasticState does
+     * (solid_shell.cpp:808-815): material.failure_strain is then the uniform strain eps_g,
+     * and the element takes eps_g + (eps_f - eps_g) min(1, thickness / in-plane size)
+     * (shipfem_hex_failure_strain). 0: every element takes material.failure_strain. Tet cases
+     * take material.failure_strain as given and refuse a nonzero value. */
+    double fracture_strain;
 } shipfem_case_desc;
 
 /* Per-call step parameters of gpu::FemGpuSolver::run (fem_gpu.hpp:31-32). */
@@ -368,6 +376,15 @@ typedef struct shipfem_handoff {
     double work;          /* w_ext + w_contact + w_boundary */
     double w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture;
     uint32_t torn;        /* tets torn so far */
+    /* A plastic solid-shell hex keeps its history per integration point
+     * (solidshell::ElementPlasticState, engine/sim/solid_shell.hpp:436-441): */
+    double* hex_points;   /* [64T] per element, per point g = 0..7 (solid_shell.cpp's Gauss
+                           * order), 8 values: eps_p (6, Voigt, engineering shears,
+                           * co-rotated frame), eqps, damage (1 = the point has failed; an
+                           * element whose eight points have failed is torn). The back stress
+                           * is zero: hardening is isotropic (kinematicModulus 0). */
+    double* hex_enhanced; /* [7T] the enhanced-strain parameters alpha the next update
+                           * starts its Newton iterations from */
 } shipfem_handoff;
 
 /* ZoneGpuSolver::readback (engine/gpu/zone_gpu.hpp:205-207): the state and the
@@ -377,7 +394,10 @@ shipfem_status shipfem_solver_export(shipfem_solver* s, shipfem_handoff* out);
 /* zone::Solver::adopt (engine/sim/zone.hpp:963-969, zone.cpp:1321-1368) on this
  * solver: take a state computed elsewhere and continue as if this solver had produced
  * it. velocity and position (or displacement) are required; plastic (and with it tear_step/tear_stress)
- * is taken when non-NULL and the case is plastic. steps sets the step counter and time
+ * is taken when non-NULL and the case is a plastic tet case; for a plastic hex case
+ * hex_points (and with it hex_enhanced, zeros when NULL, and tear_step) takes its place,
+ * and a plastic hex exports its per-element summary in plastic (eps_p zero, the largest
+ * eqps and the smallest damage of its points: readback's eqps and damage). steps sets the step counter and time
  * must equal steps * dt (to 1e-9 relative), else SHIPFEM_INVALID_ARG. displacement,
  * when non-NULL, is taken instead of position - X (a double u of an FP32 + compensation
  * state comes back to the same two floats, so a run adopting its own export continues
@@ -446,6 +466,12 @@ uint32_t shipfem_case_blob_corners(const shipfem_case_blob* b);
  * SHIPFEM_INVALID_ARG for a degenerate or inverted element. */
 shipfem_status shipfem_hex_stiffness(const double nodes[24], double youngs_modulus,
                                      double poisson_ratio, double out[576]);
+/* The failure strain initialisePlasticState resolves for one element (solid_shell.cpp:808-815,
+ * elementSize :770-806, plasticity::regularisedFailureStrain plasticity.cpp:158-169): in-plane
+ * size l = sqrt(mid-surface area), thickness t = volume / area, and
+ * uniform + (fracture - uniform) min(1, t / l); fracture itself for a degenerate element. */
+shipfem_status shipfem_hex_failure_strain(const double nodes[24], double uniform, double fracture,
+                                          double* out);
 /* Lumped mass rho int N_a dV per node (solidshell::elementMass, solid_shell.hpp:330). */
 shipfem_status shipfem_hex_mass(const double nodes[24], double density, double out[8]);
 /* The rotation the element carries: the polar factor of the deformation gradient at the
@@ -558,11 +584,22 @@ void shipfem_batch_destroy(shipfem_batch* b);
 /* ---- resident serving loop with continuous batching ---------------------- */
 typedef struct shipfem_server shipfem_server;
 
+/* Which resident cases a round advances. ALL: every one, so each case's latency is its
+ * round count times rounds sized by everything resident. SHORTEST: the cases with the least
+ * remaining work (tets x steps left) first, until the round holds round_work tet-steps; a
+ * case left out of starve_rounds consecutive rounds of its kind goes first, so every case
+ * advances. The policy changes only when a case runs, never its arithmetic. */
+#define SHIPFEM_ROUNDS_ALL 0
+#define SHIPFEM_ROUNDS_SHORTEST 1
+
 typedef struct shipfem_server_opts {
     uint64_t memory_budget;  /* device bytes for case state; 0 => 60% of HBM */
     uint32_t max_active;     /* slots; 0 => 256 */
     int32_t chunk;           /* steps per device round between admissions; 0 => 64 */
     shipfem_launch_config launch; /* zeros => defaults */
+    int32_t policy;          /* SHIPFEM_ROUNDS_* ; 0 => ALL */
+    double round_work;       /* SHORTEST: tet-steps a round takes (at least one case); 0 => 5e7 */
+    int32_t starve_rounds;   /* SHORTEST: 0 => 8 */
 } shipfem_server_opts;
 
 typedef struct shipfem_result {

@@ -21,6 +21,9 @@ internalForce), restated in numpy over a batch of elements [E, 8, 3]:
     (solid_shell.cpp:160-179, 313-365);
   - K = Kuu - Kua Kaa^-1 Kua^T (solid_shell.cpp:384-458, 500-526);
   - mass rho sum_g w_g N_a(g) (solid_shell.cpp:655-663);
+  - element size and the regularised failure strain (elementSize solid_shell.cpp:770-806,
+    initialisePlasticState :808-815, regularisedFailureStrain plasticity.cpp:158-169):
+    in-plane size sqrt(mid-surface area as two triangles), thickness volume / area;
   - R the polar factor of F = J(x, 0) J(X, 0)^-1 by R <- (R + R^-T)/2 to convergence,
     f = -R K (R^T x - X) (solid_shell.cpp:680-766);
   - dt = safety * 2 / sqrt(lambda_max(M^-1/2 K M^-1/2)) (solid_shell.cpp:1088-1134).
@@ -157,6 +160,30 @@ def mass(X, rho):
         det = np.linalg.det(jacobian(X, dshape(xi, eta, zeta)))
         out += rho * det[:, None] * shape(xi, eta, zeta)[None, :]
     return out
+
+
+def element_size(X):
+    """(in-plane size, thickness) [E] each: sqrt of the mid-surface quad's area over its two
+    triangles (0-1-2, 0-2-3), and the Gauss volume over that area; zeros where the area
+    is not positive."""
+    X = np.asarray(X, np.float64).reshape(-1, 8, 3)
+    mid = 0.5 * (X[:, :4] + X[:, 4:])
+    area = np.zeros(X.shape[0])
+    for b, c in ((1, 2), (2, 3)):
+        area += 0.5 * np.linalg.norm(np.cross(mid[:, b] - mid[:, 0], mid[:, c] - mid[:, 0]), axis=1)
+    vol = mass(X, 1.0).sum(axis=1)
+    ok = area > 0
+    length = np.where(ok, np.sqrt(np.where(ok, area, 1.0)), 0.0)
+    thick = np.where(ok, vol / np.where(ok, area, 1.0), 0.0)
+    return length, thick
+
+
+def failure_strain(X, uniform, fracture):
+    """[E] the failure strain each element resolves: uniform + (fracture - uniform) *
+    min(1, t / l), fracture when the size is degenerate."""
+    length, thick = element_size(X)
+    share = np.where((length > 0) & (thick > 0), np.minimum(1.0, thick / np.where(length > 0, length, 1.0)), 1.0)
+    return uniform + (fracture - uniform) * share
 
 
 def rotation(X, x, tol=1e-16, iterations=64):

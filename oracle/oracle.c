@@ -1152,7 +1152,10 @@ static void run_elements(const ora_case* c, ora_state* st, int64_t e0, int64_t e
             }
             double stress[48], info[6], rel = 0.0;
             const double* w = &c->hex_w[e * 8];
-            const int now = hex_plastic(&c->hex_B[e * 1152], &c->hex_G[e * 336], w, X, x, &j2, point,
+            /* the element's own failure strain when the case resolves one per element */
+            ora_j2 je = j2;
+            if (c->hex_failure) je.failure_strain = c->hex_failure[e];
+            const int now = hex_plastic(&c->hex_B[e * 1152], &c->hex_G[e * 336], w, X, x, &je, point,
                                         &st->hex_enhanced[e * 7], f, stress, info, &rel);
             out->diss += info[0];
             out->fracture += rel;

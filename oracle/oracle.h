@@ -190,6 +190,9 @@ typedef struct ora_case {
     const double* hex_B;
     const double* hex_G;
     const double* hex_w;
+    /* plastic hexes: [T] each element's failure strain (initialisePlasticState); NULL:
+     * every element takes failure_strain */
+    const double* hex_failure;
 } ora_case;
 
 typedef struct ora_state {

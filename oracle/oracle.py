@@ -71,7 +71,7 @@ class CaseStruct(C.Structure):
                 ("ind_velocity", C.c_double * 3), ("ind_kappa", C.c_double), ("dt", C.c_double),
                 ("rest_inv", D), ("volume", D), ("mass", D), ("adj_offset", U32),
                 ("adj_entry", U32), ("corners", C.c_int), ("hex_k", D), ("hex_mass", D),
-                ("hex_B", D), ("hex_G", D), ("hex_w", D)]
+                ("hex_B", D), ("hex_G", D), ("hex_w", D), ("hex_failure", D)]
 
 
 class StateStruct(C.Structure):
@@ -308,6 +308,12 @@ class Case:
             f = dict(RICE_TRACEY)
             f.update(failure)
             s.failure_strain = f["failure_strain"]
+            if self.corners == 8 and plastic and f.get("fracture_strain", 0.0) > 0:
+                # every element resolves its own failure strain from its size
+                from oracle import hex as H
+                self.hex_failure = np.ascontiguousarray(
+                    H.failure_strain(self.X[self.index], f["failure_strain"], f["fracture_strain"]))
+                s.hex_failure = _p(self.hex_failure, C.c_double)
             s.sensitivity = f["sensitivity"]
             s.reference_triaxiality = f["reference_triaxiality"]
             s.cutoff_triaxiality = f["cutoff_triaxiality"]
@@ -375,9 +381,9 @@ class Case:
     def adopt(self, state):
         """Continue from a state computed elsewhere, as zone::Solver::adopt does
         (zone.cpp:1321-1368): a dict with displacement [N,3] (or position), velocity [N,3],
-        optional plastic
-        [T,8] (eps_p 6, eqps, damage) and tear_step [T], steps, and the account's running
-        sums w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture."""
+        optional plastic [T,8] (eps_p 6, eqps, damage) for tets or hex_points [T,8,8] and
+        hex_enhanced [T,7] for hexes, tear_step [T], steps, and the account's running sums
+        w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture."""
         if state.get("displacement") is not None:
             self.u[:] = np.asarray(state["displacement"], np.float64).reshape(self.N, 3)
         else:
@@ -388,6 +394,14 @@ class Case:
             self.eps_p[:] = pl[:, :6]
             self.eqps[:] = pl[:, 6]
             self.damage[:] = pl[:, 7]
+        if self.corners == 8 and state.get("hex_points") is not None:
+            # the per-point history of a solid-shell hex (ElementPlasticState,
+            # solid_shell.hpp:436-441); the failed flag follows from damage >= 1
+            hp = np.asarray(state["hex_points"], np.float64).reshape(self.T, 8, 8)
+            self.hex_point[:, :, :8] = hp
+            self.hex_point[:, :, 8] = (hp[:, :, 7] >= 1.0).astype(np.float64)
+            he = state.get("hex_enhanced")
+            self.hex_enhanced[:] = 0.0 if he is None else np.asarray(he, np.float64).reshape(self.T, 7)
         if state.get("tear_step") is not None:
             self.tear_step[:] = state["tear_step"]
         self.st.step = int(state["steps"])
@@ -401,6 +415,11 @@ class Case:
         out = dict(position=self.X + self.u, displacement=self.u.copy(), velocity=self.v.copy(),
                    plastic=np.concatenate([self.eps_p, self.eqps[:, None], self.damage[:, None]], 1),
                    tear_step=self.tear_step.copy(), steps=self.st.step)
+        if self.corners == 8:
+            out["hex_points"] = self.hex_point[:, :, :8].copy()
+            out["hex_enhanced"] = self.hex_enhanced.copy()
+            out["plastic"] = np.zeros((self.T, 8))
+            out["plastic"][:, 6], out["plastic"][:, 7] = self.hex_eqps, self.hex_damage
         for k in ("w_ext", "w_contact", "w_boundary", "damping_loss", "dissipation", "fracture"):
             out[k] = getattr(self.st, k)
         return out

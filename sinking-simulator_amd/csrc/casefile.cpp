@@ -25,8 +25,9 @@ enum Present : uint32_t {
     kHasV0 = 1u << 4,
     kHexElements = 1u << 5,  // solid-shell hexes: 8 node ids per element
     kHasMass = 1u << 6,      // the caller's lumped nodal masses
+    kHasFracture = 1u << 7,  // hexes: the local fracture strain (8 bytes after the arrays)
 };
-constexpr uint32_t kPresentMask = 0x7Fu;
+constexpr uint32_t kPresentMask = 0xFFu;
 
 struct Header {
     char magic[8];
@@ -64,7 +65,8 @@ uint64_t fnv1a(const unsigned char* p, uint64_t n) {
 uint32_t presentOf(const shipfem_case_desc& d, bool hex) {
     return (d.node_flags ? kHasFlags : 0u) | (d.load ? kHasLoad : 0u) |
            (d.prescribed ? kHasPrescribed : 0u) | (d.initial_displacement ? kHasU0 : 0u) |
-           (d.initial_velocity ? kHasV0 : 0u) | (hex ? kHexElements : 0u) | (d.mass ? kHasMass : 0u);
+           (d.initial_velocity ? kHasV0 : 0u) | (hex ? kHexElements : 0u) | (d.mass ? kHasMass : 0u) |
+           (d.fracture_strain != 0.0 ? kHasFracture : 0u);
 }
 
 uint64_t payloadBytes(uint64_t N, uint64_t T, uint32_t present) {
@@ -73,6 +75,7 @@ uint64_t payloadBytes(uint64_t N, uint64_t T, uint32_t present) {
     for (uint32_t bit : {kHasLoad, kHasPrescribed, kHasU0, kHasV0})
         if (present & bit) b += 24 * N;
     if (present & kHasMass) b += 8 * N;
+    if (present & kHasFracture) b += 8;
     return b;
 }
 
@@ -151,6 +154,7 @@ shipfem_status encode(const shipfem_case_desc* d, void* buf, uint64_t len, bool 
     if (d->initial_displacement) put(d->initial_displacement, 24 * N);
     if (d->initial_velocity) put(d->initial_velocity, 24 * N);
     if (d->mass) put(d->mass, 8 * N);
+    if (d->fracture_strain != 0.0) put(&d->fracture_strain, 8);
     h.checksum = fnv1a(body, h.payload);
     std::memcpy(buf, &h, sizeof h);
     return SHIPFEM_OK;
@@ -220,6 +224,11 @@ shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_b
     if (h.present & kHasU0) take(b->u0, 3 * N);
     if (h.present & kHasV0) take(b->v0, 3 * N);
     if (h.present & kHasMass) take(b->mass, N);
+    double fracture = 0.0;
+    if (h.present & kHasFracture) {
+        std::memcpy(&fracture, p, 8);
+        p += 8;
+    }
     for (uint32_t v : b->index)
         if (v >= N) return SHIPFEM_INVALID_ARG;
     *d = shipfem_case_desc{};
@@ -233,6 +242,7 @@ shipfem_status shipfem_case_decode(const void* buf, uint64_t len, shipfem_case_b
     d->initial_displacement = b->u0.empty() ? nullptr : b->u0.data();
     d->initial_velocity = b->v0.empty() ? nullptr : b->v0.data();
     d->mass = b->mass.empty() ? nullptr : b->mass.data();
+    d->fracture_strain = fracture;
     shipfem_material& m = d->material;
     m.youngs_modulus = s.E;
     m.poisson_ratio = s.nu;

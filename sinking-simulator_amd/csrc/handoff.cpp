@@ -5,6 +5,7 @@
 #include "shipfem.h"
 #include "shipfem_host.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -25,6 +26,8 @@ hipError_t toHost(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
 hipError_t toDevice(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
 }
+
+bool plasticHex(const HostCase& hc) { return hc.corners == 8 && (hc.features & kFeatPlastic); }
 
 uint32_t pstatePlanes(const HostCase& hc) {
     if (!(hc.features & kFeatPlastic) || hc.corners == 8) return 0;
@@ -87,9 +90,6 @@ extern "C" {
 shipfem_status shipfem_solver_export(shipfem_solver* s, shipfem_handoff* out) {
     if (!s || !out) return SHIPFEM_INVALID_ARG;
     const HostCase& hc = s->hc;
-    // a plastic hex keeps eight points of history per element: not the per-element shape
-    // of the hand-off (checkpoint and restore carry it)
-    if (hc.corners == 8 && (hc.features & kFeatPlastic)) return SHIPFEM_UNSUPPORTED;
     const CaseDev& r = s->dc.rec;
     const uint64_t N = hc.N, T = hc.T, Np = hc.Npad, Tp = hc.Tpad;
     const hipStream_t st = s->stream;
@@ -108,6 +108,10 @@ shipfem_status shipfem_solver_export(shipfem_solver* s, shipfem_handoff* out) {
     if (e == hipSuccess) e = toHost(an.data(), r.acc_n, 8 * an.size(), st);
     if (e == hipSuccess) e = toHost(tstep.data(), r.tear_step, 4 * tstep.size(), st);
     if (e == hipSuccess) e = toHost(tstress.data(), r.tear_stress, 4 * tstress.size(), st);
+    const bool hexp = plasticHex(hc);
+    std::vector<double> hp(hexp ? (uint64_t)kHexPoint * Tp : 0), he(hexp ? 7ull * Tp : 0);
+    if (e == hipSuccess) e = toHost(hp.data(), r.hpoint, 8 * hp.size(), st);
+    if (e == hipSuccess) e = toHost(he.data(), r.henh, 8 * he.size(), st);
     if (e == hipSuccess) e = toHost(step, r.step, 16, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return statusOf(e);
@@ -125,10 +129,28 @@ shipfem_status shipfem_solver_export(shipfem_solver* s, shipfem_handoff* out) {
             if (out->velocity) out->velocity[x * 3 + a] = (double)v[a * Np + n] - (double)cv[a * Np + n];
         }
     }
-    if (out->plastic)
+    if (out->plastic && !hexp)
         for (uint32_t t = 0; t < T; ++t)
             for (uint32_t k = 0; k < 8; ++k)
                 out->plastic[externalTet(hc, t) * 8ull + k] = k < pstatePlanes(hc) ? (double)ps[k * Tp + t] : 0.0;
+    if (hexp)
+        for (uint32_t t = 0; t < T; ++t) {
+            const uint64_t x = externalTet(hc, t);
+            double eq = 0.0, dmin = 1.0;
+            for (uint32_t g = 0; g < 8; ++g) {
+                eq = std::max(eq, hp[(8ull * g + 6) * Tp + t]);
+                dmin = std::min(dmin, hp[(8ull * g + 7) * Tp + t]);
+                if (out->hex_points)
+                    for (uint32_t k = 0; k < 8; ++k) out->hex_points[x * 64 + 8 * g + k] = hp[(8ull * g + k) * Tp + t];
+            }
+            if (out->plastic) {
+                for (uint32_t k = 0; k < 6; ++k) out->plastic[x * 8 + k] = 0.0;
+                out->plastic[x * 8 + 6] = eq;
+                out->plastic[x * 8 + 7] = dmin;
+            }
+            if (out->hex_enhanced)
+                for (uint32_t k = 0; k < 7; ++k) out->hex_enhanced[x * 7 + k] = he[k * Tp + t];
+        }
     uint32_t torn = 0;
     for (uint32_t t = 0; t < T; ++t) {
         const uint64_t x = externalTet(hc, t);
@@ -166,7 +188,6 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
         return SHIPFEM_INVALID_ARG;
     const HostCase& hc = s->hc;
     const CaseDev& r = s->dc.rec;
-    if (hc.corners == 8 && (hc.features & kFeatPlastic)) return SHIPFEM_UNSUPPORTED;
     const double want = static_cast<double>(in->steps) * s->dc.dt;
     if (!(std::fabs(in->time - want) <= 1e-9 * std::max(std::fabs(want), s->dc.dt)))
         return SHIPFEM_INVALID_ARG;
@@ -186,6 +207,24 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
             splitValue(vel, v[a * Np + n], cv[a * Np + n]);
         }
     }
+    // a plastic hex's point history and enhanced parameters, checked before anything is sent
+    const bool hexp = plasticHex(hc) && in->hex_points;
+    std::vector<double> hp(hexp ? (uint64_t)kHexPoint * Tp : 0, 0.0), he(hexp ? 7ull * Tp : 0, 0.0);
+    if (hexp)
+        for (uint32_t t = 0; t < T; ++t) {
+            const uint64_t x = externalTet(hc, t);
+            for (uint32_t q = 0; q < (uint32_t)kHexPoint; ++q) {
+                const double val = in->hex_points[x * 64 + q];
+                if (!std::isfinite(val)) return SHIPFEM_INVALID_ARG;
+                hp[q * Tp + t] = val;
+            }
+            if (in->hex_enhanced)
+                for (uint32_t k = 0; k < 7; ++k) {
+                    const double val = in->hex_enhanced[x * 7 + k];
+                    if (!std::isfinite(val)) return SHIPFEM_INVALID_ARG;
+                    he[k * Tp + t] = val;
+                }
+        }
     const hipStream_t st = s->stream;
     hipError_t e = toDevice(r.u4, u4.data(), 16 * N, st);
     if (e == hipSuccess) e = toDevice(r.cu, cu, 12 * Np, st);
@@ -212,6 +251,26 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
                 tstep[t] = static_cast<int32_t>(at + 1);
                 if (in->tear_stress)
                     for (int k = 0; k < 6; ++k) tstress[k * Tp + t] = static_cast<float>(in->tear_stress[x * 6 + k]);
+            }
+            if (e == hipSuccess) e = toDevice(r.tear_step, tstep.data(), 4 * Tp, st);
+            if (e == hipSuccess) e = toDevice(r.tear_stress, tstress.data(), 24 * Tp, st);
+        }
+    }
+    if (hexp) {
+        if (e == hipSuccess) e = toDevice(r.hpoint, hp.data(), 8 * hp.size(), st);
+        if (e == hipSuccess) e = toDevice(r.henh, he.data(), 8 * he.size(), st);
+        if (dmg) {
+            // an element whose eight points have failed is torn; one handed over without its
+            // tear step is recorded as torn at `steps` (the hex pass keeps no tear stress)
+            tstep.assign(Tp, 0);
+            tstress.assign(6 * Tp, 0.0f);
+            for (uint32_t t = 0; t < T; ++t) {
+                bool torn = true;
+                for (uint32_t g = 0; g < 8; ++g) torn = torn && hp[(8ull * g + 7) * Tp + t] >= 1.0;
+                if (!torn) continue;
+                const uint64_t x = externalTet(hc, t);
+                const int64_t at = in->tear_step && in->tear_step[x] >= 0 ? in->tear_step[x] : in->steps;
+                tstep[t] = static_cast<int32_t>(at + 1);
             }
             if (e == hipSuccess) e = toDevice(r.tear_step, tstep.data(), 4 * Tp, st);
             if (e == hipSuccess) e = toDevice(r.tear_stress, tstress.data(), 24 * Tp, st);

@@ -326,6 +326,42 @@ bool hexCentreGradient(const double* x, double b0[kN][3]) {
     return true;
 }
 
+// solidshell::elementSize (solid_shell.cpp:770-806): the in-plane size is the square root
+// of the mid-surface quad's area (two triangles, exact on a warped element), the thickness
+// the element's volume (its Gauss volumes) over that area. Zeros when either is not positive.
+void hexElementSize(const double* x, double& inPlane, double& thickness) {
+    inPlane = thickness = 0.0;
+    double mid[4][3];
+    for (int a = 0; a < 4; ++a)
+        for (int i = 0; i < 3; ++i) mid[a][i] = 0.5 * (x[3 * a + i] + x[3 * (a + 4) + i]);
+    double area = 0.0;
+    for (int t = 1; t <= 2; ++t) {
+        double e1[3], e2[3];
+        for (int i = 0; i < 3; ++i) {
+            e1[i] = mid[t][i] - mid[0][i];
+            e2[i] = mid[t + 1][i] - mid[0][i];
+        }
+        const double c0 = e1[1] * e2[2] - e1[2] * e2[1], c1 = e1[2] * e2[0] - e1[0] * e2[2],
+                     c2 = e1[0] * e2[1] - e1[1] * e2[0];
+        area += 0.5 * std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    }
+    if (!(area > 0.0)) return;
+    double m[kN];
+    if (!hexMass(x, 1.0, m)) return;
+    double volume = 0.0;
+    for (int a = 0; a < kN; ++a) volume += m[a];
+    inPlane = std::sqrt(area);
+    thickness = volume / area;
+}
+
+// plasticity::regularisedFailureStrain (plasticity.cpp:158-169) at an element's own size.
+double hexFailureStrain(const double* x, double uniform, double fracture) {
+    double l = 0.0, t = 0.0;
+    hexElementSize(x, l, t);
+    const double share = (l > 0.0 && t > 0.0) ? std::min(1.0, t / l) : 1.0;
+    return uniform + (fracture - uniform) * share;
+}
+
 double hexOmegaMax(const double K[kD * kD], const double m[kN]) {
     // largest omega^2 of K v = omega^2 M v with M the element's lumped mass, by power
     // iteration on M^-1/2 K M^-1/2 from a fixed start, to a relative 1e-12 change
@@ -416,6 +452,10 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
         hc.hk.assign(kHexPacked * tp, 0.0);
     }
     hc.hgeo.assign(kHexGeo * tp, 0.0f);
+    // per-element failure strain, resolved from each element's size (initialisePlasticState,
+    // solid_shell.cpp:808-815), when the case gives a local fracture strain
+    const bool perElement = plastic && d.material.failure_strain > 0.0 && d.fracture_strain > 0.0;
+    if (perElement) hc.hfail.assign(tp, 0.0);
     std::vector<double> em(8ull * T), omega(T, 0.0), volume(T, 0.0);
     std::vector<int> bad(T, 0);
     const double* X = d.rest_position;
@@ -436,6 +476,7 @@ std::string hexRestState(const shipfem_case_desc& d, HostCase& hc, std::vector<d
             }
             volume[e] /= hc.rho;
             omega[e] = hexOmegaMax(K, m);
+            if (perElement) hc.hfail[e] = hexFailureStrain(x, d.material.failure_strain, d.fracture_strain);
             // corners 1..7: rows and columns 3..23 of K
             int p = 0;
             for (int i = 0; i < kHexDof && !plastic; ++i)
@@ -485,6 +526,13 @@ shipfem_status shipfem_hex_stiffness(const double nodes[24], double youngs_modul
         return SHIPFEM_INVALID_ARG;
     return shipfem::hexStiffness(nodes, youngs_modulus, poisson_ratio, out) ? SHIPFEM_OK
                                                                            : SHIPFEM_INVALID_ARG;
+}
+
+shipfem_status shipfem_hex_failure_strain(const double nodes[24], double uniform, double fracture,
+                                          double* out) {
+    if (!nodes || !out || !std::isfinite(uniform) || !std::isfinite(fracture)) return SHIPFEM_INVALID_ARG;
+    *out = shipfem::hexFailureStrain(nodes, uniform, fracture);
+    return SHIPFEM_OK;
 }
 
 shipfem_status shipfem_hex_mass(const double nodes[24], double density, double out[8]) {

@@ -612,7 +612,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
                     cd[3 * mm + i] += W[0 * 3 + i] * y[0] + W[1 * 3 + i] * y[1] + W[2 * 3 + i] * y[2];
             }
             const hexp::Material m{c.mu_d, c.lam_d + (2.0 / 3.0) * c.mu_d, c.yield_d, c.hard_d,
-                                   c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage};
+                                   c.hfail ? c.hfail[e] : c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage};
             double fi[kHexDof];
             const hexp::Result res = hexp::update(m, cX, cd, c.hpoint + e, c.henh + e, tp, sample != 0, fi,
                                                   sh_eu + threadIdx.x, BLOCK);

@@ -34,7 +34,7 @@ struct Layout {
 
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
-    uint64_t conn, dmi, vol, force, hk, hgeo, hgeod, hpoint, henh, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
+    uint64_t conn, dmi, vol, force, hk, hgeo, hgeod, hpoint, henh, hfail, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
         ell_off, ell, lconn, tn_off, tnode, lofs, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
@@ -53,6 +53,7 @@ ArenaMap mapArena(const HostCase& hc) {
     const bool hexPlastic = hc.corners == 8 && (hc.features & kFeatPlastic);
     m.hpoint = L.take(hexPlastic ? 8ull * kHexPoint * Tp : 0);
     m.henh = L.take(hexPlastic ? 8ull * 7 * Tp : 0);
+    m.hfail = L.take(8ull * hc.hfail.size());
     const bool dmg = (hc.features & kFeatDamage) != 0;
     m.pstate = L.take((hc.features & kFeatPlastic) && !hexPlastic ? (dmg ? 32 : 28) * Tp : 0);
     m.tear_step = L.take(dmg ? 4 * Tp : 0);
@@ -223,6 +224,11 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
             !std::isfinite(mt.cutoff_triaxiality))
             return "triaxiality constants must be finite";
     }
+    if (!(d.fracture_strain >= 0.0) || !std::isfinite(d.fracture_strain))
+        return fmt("fracture_strain must be finite and >= 0", d.fracture_strain);
+    if (corners == 4 && d.fracture_strain > 0.0)
+        return "fracture_strain resolves per element for solid-shell hexes only; a tet case takes "
+               "material.failure_strain as given (see shipfem_regularised_failure_strain)";
     const double damping = d.damping == 0.0 ? 1.0 : d.damping;
     if (!(damping > 0 && damping <= 1.0)) return fmt("damping must lie in (0, 1]", d.damping);
     if (!(d.dt >= 0) || !std::isfinite(d.dt)) return fmt("dt must be >= 0", d.dt);
@@ -637,7 +643,7 @@ PassBytes compulsoryBytes(const HostCase& hc, int variant) {
         // history with the enhanced parameters (plastic), and the displacement with its
         // compensation of every node; the 24-float force slots out (and the enhanced
         // parameters, plastic)
-        b.elem_read = 32 * T + 28 * N +
+        b.elem_read = 32 * T + 28 * N + (hc.hfail.empty() ? 0 : 8 * T) +
                       (plastic ? (8ull * kHexGeo + 8ull * (kHexPoint + 7)) * T : (8ull * kHexPacked + 4ull * kHexGeo) * T);
         b.elem_write = 96 * T + (plastic ? 56 * T : 0);
     } else if (variant == kSlots) {
@@ -684,6 +690,7 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.hgeod = hexPlastic ? reinterpret_cast<const double*>(base + m.hgeod) : nullptr;
     r.hpoint = hexPlastic ? reinterpret_cast<double*>(base + m.hpoint) : nullptr;
     r.henh = hexPlastic ? reinterpret_cast<double*>(base + m.henh) : nullptr;
+    r.hfail = hc.hfail.empty() ? nullptr : reinterpret_cast<const double*>(base + m.hfail);
     r.dmi = reinterpret_cast<const float*>(base + m.dmi);
     r.vol = reinterpret_cast<const float*>(base + m.vol);
     r.force = reinterpret_cast<float*>(base + m.force);
@@ -758,6 +765,7 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.hk, hc.hk.data(), 8ull * hc.hk.size());
     up(m.hgeo, hc.hgeo.data(), 4ull * hc.hgeo.size());
     up(m.hgeod, hc.hgeod.data(), 8ull * hc.hgeod.size());
+    up(m.hfail, hc.hfail.data(), 8ull * hc.hfail.size());
     up(m.u4, hc.u4.data(), 4ull * hc.u4.size());
     up(m.cu, hc.cu.data(), 4ull * hc.cu.size());
     up(m.v, hc.v.data(), 4ull * hc.v.size());

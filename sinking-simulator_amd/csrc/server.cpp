@@ -113,6 +113,7 @@ struct Active {
     int64_t last = 0;        // step reached at the end of the last completed round
     int64_t predicted = 0;   // step it reaches when every launched round has run
     uint64_t last_round = 0; // the last launched round it belongs to (0: none)
+    int32_t left_out = 0;    // consecutive rounds of its kind it sat out (SHORTEST)
     bool dead = false;       // reported non-finite: joins no further round
 };
 
@@ -172,6 +173,9 @@ struct shipfem_server {
 
     void workerMain();
     void loopMain();
+    // SHORTEST: keeps the members a round takes, least remaining work first up to the
+    // round's work, starved cases first; counts the rounds the others sit out.
+    void pickShortest(std::vector<Active*>& members);
     bool admit();
     void finish(Active& a, hipError_t fault);
     hipError_t launchRound(const std::vector<Active*>& members, int kind);
@@ -508,6 +512,33 @@ void shipfem_server::failAll(hipError_t e) {
     cv_done.notify_all();
 }
 
+void shipfem_server::pickShortest(std::vector<Active*>& members) {
+    if (members.empty()) return;
+    const double budget = opts.round_work > 0 ? opts.round_work : 5e7;
+    const int32_t starve = opts.starve_rounds > 0 ? opts.starve_rounds : 8;
+    auto left = [](const Active* a) { return static_cast<double>(a->req->hc.T) * static_cast<double>(a->steps - a->predicted); };
+    std::stable_sort(members.begin(), members.end(), [&](const Active* x, const Active* y) {
+        const bool sx = x->left_out >= starve, sy = y->left_out >= starve;
+        if (sx != sy) return sx;
+        if (left(x) != left(y)) return left(x) < left(y);
+        return x->req->id < y->req->id;
+    });
+    std::vector<Active*> take;
+    double work = 0;
+    for (Active* a : members) {
+        const double w = static_cast<double>(a->req->hc.T) *
+                         static_cast<double>(std::min<int64_t>(opts.chunk, a->steps - a->predicted));
+        if (take.empty() || a->left_out >= starve || work + w <= budget) {
+            take.push_back(a);
+            work += w;
+            a->left_out = 0;
+        } else {
+            ++a->left_out;
+        }
+    }
+    members.swap(take);
+}
+
 void shipfem_server::loopMain() {
     for (;;) {
         bool stopping = false;
@@ -535,6 +566,7 @@ void shipfem_server::loopMain() {
                     if (!a->dead && a->predicted < a->steps && kindOf(a->req->hc) == kind)
                         members.push_back(a.get());
             }
+            if (opts.policy == SHIPFEM_ROUNDS_SHORTEST) pickShortest(members);
             if (!members.empty()) {
                 next_kind = (kind + 1) % kKinds;
                 const hipError_t e = launchRound(members, kind);
@@ -571,6 +603,11 @@ shipfem_status shipfem_server_create(const shipfem_server_opts* opts, shipfem_se
     if (opts) s->opts = *opts;
     if (s->opts.max_active == 0) s->opts.max_active = 256;
     if (s->opts.chunk == 0) s->opts.chunk = 64;
+    if (s->opts.policy != SHIPFEM_ROUNDS_ALL && s->opts.policy != SHIPFEM_ROUNDS_SHORTEST) {
+        setError(err, err_len, "policy must be SHIPFEM_ROUNDS_ALL or SHIPFEM_ROUNDS_SHORTEST");
+        delete s;
+        return SHIPFEM_INVALID_ARG;
+    }
     if (s->opts.chunk < 2 || (s->opts.chunk & 1)) {
         setError(err, err_len, "chunk must be an even number >= 2");
         delete s;

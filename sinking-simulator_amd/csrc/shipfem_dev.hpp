@@ -76,6 +76,7 @@ struct CaseDev {
                          // (xi, eta, zeta, xi eta, eta zeta, zeta xi, xi eta zeta), 3 m + i
     double* hpoint;      // plastic hex: [kHexPoint][Tpad], entry 8 g + k
     double* henh;        // plastic hex: [7][Tpad] enhanced parameters (warm start)
+    const double* hfail; // plastic hex: [Tpad] each element's failure strain, or null (fail_d)
     float* pstate;       // [7 or 8][Tpad] plastic strain (6, eng. shears), eqps, damage
     int32_t* tear_step;  // [Tpad] step + 1 at which the tet tore, 0 intact (kFeatDamage)
     float* tear_stress;  // [6][Tpad] world-frame stress at the tear (kFeatDamage)

@@ -39,6 +39,7 @@ struct HostCase {
     std::vector<uint32_t> conn;   // [corners T]
     std::vector<double> hk;       // hex: [kHexPacked][Tpad]
     std::vector<double> hgeod;    // plastic hex: [kHexGeo][Tpad]
+    std::vector<double> hfail;    // plastic hex with per-element failure: [Tpad] eps_f
     std::vector<float> hgeo;      // hex: [kHexGeo][Tpad]
     double hex_omega = 0;         // hex: largest element frequency (FP64, host)
     std::vector<double> hvol;     // hex: [T] element volumes (host only)

@@ -54,7 +54,7 @@ class CaseDesc(C.Structure):
                 ("indenter", Indenter), ("load_ramp", C.c_double), ("drive_ramp", C.c_double),
                 ("gravity", C.c_double), ("damping", C.c_double), ("dt", C.c_double),
                 ("dt_safety", C.c_double), ("sample_every", C.c_int32),
-                ("max_steps", C.c_int64), ("mass", D)]
+                ("max_steps", C.c_int64), ("mass", D), ("fracture_strain", C.c_double)]
 
 
 class StepParams(C.Structure):
@@ -88,9 +88,13 @@ class LaunchConfig(C.Structure):
                 ("chunk", C.c_int32), ("variant", C.c_int32)]
 
 
+ROUNDS_ALL, ROUNDS_SHORTEST = 0, 1
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("max_active", C.c_uint32), ("chunk", C.c_int32),
-                ("launch", LaunchConfig)]
+                ("launch", LaunchConfig), ("policy", C.c_int32), ("round_work", C.c_double),
+                ("starve_rounds", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -116,7 +120,8 @@ class Handoff(C.Structure):
                 ("steps", C.c_int64), ("time", C.c_double), ("penetration", C.c_double),
                 ("work", C.c_double), ("w_ext", C.c_double), ("w_contact", C.c_double),
                 ("w_boundary", C.c_double), ("damping_loss", C.c_double),
-                ("dissipation", C.c_double), ("fracture", C.c_double), ("torn", C.c_uint32)]
+                ("dissipation", C.c_double), ("fracture", C.c_double), ("torn", C.c_uint32),
+                ("hex_points", D), ("hex_enhanced", D)]
 
 
 ENERGY_FIELDS = ["strain", "kinetic", "dissipation", "w_ext", "w_contact", "w_boundary",
@@ -146,6 +151,7 @@ SIGNATURES = {
                                     C.c_char_p, C.c_size_t]),
     "shipfem_stable_timestep_noalloc": (S, [D, U32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                             D, C.c_char_p, C.c_size_t]),
+    "shipfem_hex_failure_strain": (S, [D, C.c_double, C.c_double, D]),
     "shipfem_cpu_tet_forces": (S, [F, D, U32, F, F, C.c_uint64, C.c_float, C.c_float, F]),
     "shipfem_cpu_step": (S, [C.c_uint32, C.c_uint32, F, F, F, U32, U32, F, F, F, U32, U32, D] +
                          [C.c_float] * 5),
@@ -386,6 +392,7 @@ class Case:
         d.initial_displacement = _p(self.u0, C.c_double)
         d.initial_velocity = _p(self.v0, C.c_double)
         d.mass = _p(self.mass, C.c_double)
+        d.fracture_strain = float((self.failure or {}).get("fracture_strain", 0.0))
         f = dict(RICE_TRACEY)
         f.update(self.failure or {})
         d.material = Material(self.E, self.nu, self.rho, 1 if self.plastic else 0, self.yield_,
@@ -428,6 +435,8 @@ class Case:
             failure = dict(failure_strain=m.failure_strain, sensitivity=m.triaxiality_sensitivity,
                            reference_triaxiality=m.reference_triaxiality,
                            cutoff_triaxiality=m.cutoff_triaxiality)
+            if d.fracture_strain > 0:
+                failure["fracture_strain"] = d.fracture_strain
         indenter = None
         if ind.enabled:
             indenter = dict(radius=ind.radius, center=tuple(ind.center), velocity=tuple(ind.velocity),
@@ -570,6 +579,15 @@ def hex_plastic_update(rest, current, point_state, enhanced, E=210e9, nu=0.3, yi
         _p(point_state, C.c_double), _p(enhanced, C.c_double), _p(out, C.c_double),
         C.byref(diss), C.byref(failed)), "hex_plastic_update")
     return out, diss.value, failed.value
+
+
+def hex_failure_strain(nodes, uniform, fracture):
+    """initialisePlasticState's regularised failure strain of one element
+    (shipfem_hex_failure_strain)."""
+    x = _nodes24(nodes)
+    out = C.c_double()
+    _check(lib().shipfem_hex_failure_strain(_p(x, C.c_double), uniform, fracture, C.byref(out)))
+    return out.value
 
 
 def hex_critical_timestep(nodes, E=210e9, nu=0.3, rho=7850.0, safety=0.9):
@@ -804,13 +822,20 @@ class Solver:
         """The state in the shape zone::Solver::adopt takes (ZoneGpuState): dict with
         position [N,3], displacement [N,3], velocity [N,3], plastic [T,8] (eps_p 6, eqps, damage), tear_step
         [T] (-1 intact), tear_stress [T,6] and the scalars steps, time, penetration, work,
-        w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture, torn."""
+        w_ext, w_contact, w_boundary, damping_loss, dissipation, fracture, torn. A plastic hex
+        case adds hex_points [T,8,8] (per point: eps_p 6, eqps, damage) and hex_enhanced
+        [T,7]; its plastic rows are the per-element summary (0 x 6, largest eqps, smallest
+        damage)."""
         N, T = self.case.N, self.case.T
         out = dict(position=np.zeros((N, 3)), displacement=np.zeros((N, 3)),
                    velocity=np.zeros((N, 3)),
                    plastic=np.zeros((T, 8)), tear_step=np.zeros(T, np.int64),
                    tear_stress=np.zeros((T, 6)))
         h = Handoff()
+        if isinstance(self.case, HexCase) and self.case.plastic:
+            out["hex_points"], out["hex_enhanced"] = np.zeros((T, 8, 8)), np.zeros((T, 7))
+            h.hex_points = _p(out["hex_points"], C.c_double)
+            h.hex_enhanced = _p(out["hex_enhanced"], C.c_double)
         h.position, h.velocity = _p(out["position"], C.c_double), _p(out["velocity"], C.c_double)
         h.displacement = _p(out["displacement"], C.c_double)
         h.plastic = _p(out["plastic"], C.c_double)
@@ -823,12 +848,14 @@ class Solver:
 
     def adopt(self, state):
         """zone::Solver::adopt on this solver: continue from a state made elsewhere
-        (a dict as export() returns; plastic, tear_step and tear_stress optional)."""
+        (a dict as export() returns; plastic, tear_step, tear_stress, hex_points and
+        hex_enhanced optional)."""
         keep = {}
         h = Handoff()
         for k, ct in (("position", C.c_double), ("displacement", C.c_double),
                       ("velocity", C.c_double), ("plastic", C.c_double),
-                      ("tear_step", C.c_int64), ("tear_stress", C.c_double)):
+                      ("tear_step", C.c_int64), ("tear_stress", C.c_double),
+                      ("hex_points", C.c_double), ("hex_enhanced", C.c_double)):
             a = state.get(k)
             if a is not None:
                 keep[k] = np.ascontiguousarray(a, np.int64 if k == "tear_step" else np.float64)
@@ -942,9 +969,10 @@ class Server:
     """Resident serving loop: submit cases at any time, poll results."""
 
     def __init__(self, memory_budget=0, max_active=0, chunk=0, elem_block=0, node_block=0,
-                 xcd_remap=1, variant=VARIANT_TILED):
+                 xcd_remap=1, variant=VARIANT_TILED, policy=ROUNDS_ALL, round_work=0.0, starve_rounds=0):
         opts = ServerOpts(memory_budget, max_active, chunk,
-                          LaunchConfig(elem_block, node_block, xcd_remap, 0, variant))
+                          LaunchConfig(elem_block, node_block, xcd_remap, 0, variant), policy, round_work,
+                          starve_rounds)
         err = C.create_string_buffer(512)
         h = C.c_void_p()
         _check(lib().shipfem_server_create(C.byref(opts), C.byref(h), err, 512), err.value.decode())

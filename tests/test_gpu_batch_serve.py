@@ -256,3 +256,29 @@ def test_fetch_returns_position_damage_and_plastic_strain_in_the_callers_order()
     with pytest.raises(sf.ShipfemError) as e:
         srv.fetch(ids[0], c)
     assert e.value.status == sf.NOT_FOUND
+
+
+def test_shortest_rounds_finish_small_cases_first_and_keep_every_bit():
+    # rounds of at most round_work tet-steps, least remaining work first: the small panels
+    # all finish before the large one, which still advances through the starvation bound;
+    # which round a case joins never changes its arithmetic
+    small = [small_panel(speed=float(s), nx=16) for s in (3.0, 5.0, 7.0, 9.0)]
+    large = small_panel(speed=6.0, nx=60)
+    cases = [large] + small
+    run_to = 4e-4
+    srv = sf.Server(memory_budget=2 << 30, chunk=32, policy=sf.ROUNDS_SHORTEST,
+                    round_work=1.5 * small[0].T * 32, starve_rounds=4)
+    ids = [srv.submit(c, run_to, keep=True) for c in cases]
+    assert srv.drain(120.0)
+    res = {r["request_id"]: r for r in srv.poll()}
+    assert all(res[i]["status"] == sf.OK for i in ids)
+    print("done (large first):", [round(res[i]["done_s"] - res[ids[0]]["submit_s"], 4) for i in ids], flush=True)
+    assert max(res[i]["done_s"] for i in ids[1:]) < res[ids[0]]["done_s"]
+    for k in (0, 2):
+        a = srv.fetch(ids[k], cases[k])
+        s = sf.Solver(cases[k])
+        s.run(a.step)
+        st = s.readback()
+        assert np.array_equal(a.u, st.u) and np.array_equal(a.eqps, st.eqps), k
+    with pytest.raises(sf.ShipfemError):
+        sf.Server(policy=7)

@@ -223,3 +223,55 @@ def test_tier1_edge_drive_matches_oracle():
     assert np.allclose(st.u[edge_x], disp[edge_x], rtol=0, atol=1e-5 * np.abs(disp).max())
     assert np.abs(st.u[y_edge]).max() == 0.0
     assert err <= REL_U and dw <= ENERGY and wb > 0
+
+
+def test_plastic_hex_history_crosses_the_boundary_both_ways():
+    # a plastic solid-shell zone that tears: every element's eight points (eps_p, eqps,
+    # damage) and enhanced parameters go out in the point order of the oracle
+    # (solid_shell.cpp's Gauss order), a device adopting its own export continues with the
+    # same bits, and each side continues the other's state at the parity tolerances
+    case = configs.hex_zone(lx=0.3, ly=0.15, nx=24, ny=12, radius=0.04, speed=10.0, plastic=True,
+                            yield_=355e6, hardening=1e9, failure=dict(failure_strain=0.06))
+    n, m = 700, 800
+    s = sf.Solver(case)
+    dt = s.dt
+    kw = case.oracle_kwargs()
+    full = O.Case(case.X, case.index, dt=dt, **kw)
+    full.run(n + m, threads=8)
+    half = O.Case(case.X, case.index, dt=dt, **kw)
+    half.run(n, threads=8)
+    s.run(n)
+    ex = s.export()
+    dq = float(np.abs(ex["hex_points"][:, :, 6] - half.hex_point[:, :, 6]).max())
+    dp = float(np.abs(ex["hex_points"][:, :, :6] - half.hex_point[:, :, :6]).max())
+    say(f"hex export at {n}: per-point eqps |d| {dq:.2e} (max {half.hex_point[:, :, 6].max():.3f}), "
+        f"eps_p |d| {dp:.2e}")
+    assert half.hex_point[:, :, 6].max() > 0.01 and dq <= ABS_EQPS and dp <= ABS_EQPS
+    r0 = s.readback()  # readback's per-element eqps and damage are the export's summary
+    assert np.allclose(ex["plastic"][:, 6], r0.eqps, rtol=1e-6, atol=0)
+    assert np.allclose(ex["plastic"][:, 7], r0.damage, rtol=1e-6, atol=0)
+    b = sf.Solver(case).adopt(ex)
+    cpu = O.Case(case.X, case.index, dt=dt, **kw).adopt(ex)
+    cpu.run(m, threads=8)
+    s.run(m)
+    b.run(m)
+    sa, sb = s.readback(), b.readback()
+    assert np.array_equal(sa.u, sb.u) and np.array_equal(sa.v, sb.v)
+    assert np.array_equal(sa.eqps, sb.eqps) and np.array_equal(sa.damage, sb.damage)
+    assert b.export()["tear_step"].tolist() == s.export()["tear_step"].tolist()
+    err = rel_l2(sa.u, cpu.u)
+    deq = float(np.abs(sa.eqps - cpu.hex_eqps).max())
+    torn_g, torn_c = set(np.nonzero(sa.damage >= 1)[0]), set(np.nonzero(cpu.hex_damage >= 1)[0])
+    say(f"hex GPU->CPU at {n}: u {err:.2e}, eqps {deq:.2e}, torn {len(torn_g)} / {len(torn_c)}")
+    assert len(torn_c) > 0 and err <= REL_U and deq <= ABS_EQPS
+    assert len(torn_g ^ torn_c) <= max(1, len(torn_c) // 50)
+
+    g = sf.Solver(case).adopt(dict(half.export(), time=n * dt))
+    g.run(m)
+    sg = g.readback()
+    err = rel_l2(sg.u, full.u)
+    deq = float(np.abs(sg.eqps - full.hex_eqps).max())
+    torn_g, torn_o = set(np.nonzero(sg.damage >= 1)[0]), set(np.nonzero(full.hex_damage >= 1)[0])
+    say(f"hex CPU->GPU at {n}: u {err:.2e}, eqps {deq:.2e}, torn {len(torn_g)} / {len(torn_o)}")
+    assert err <= REL_U and deq <= ABS_EQPS
+    assert len(torn_g ^ torn_o) <= max(1, len(torn_o) // 50)

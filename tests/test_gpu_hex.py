@@ -295,9 +295,12 @@ def test_plastic_punch_with_tearing_matches_oracle():
     r.run(200)
     a, b = s.readback(), r.readback()
     assert np.array_equal(a.u, b.u) and np.array_equal(a.eqps, b.eqps)
-    with pytest.raises(sf.ShipfemError) as e:
-        s.export()
-    assert e.value.status == sf.UNSUPPORTED
+    # the hand-off carries the same points: a torn element has eight failed points and
+    # its tear step
+    ex = s.export()
+    torn_x = np.nonzero((ex["hex_points"][:, :, 7] >= 1).all(1))[0]
+    assert set(torn_x) == set(np.nonzero(a.damage >= 1)[0])
+    assert {t: int(ex["tear_step"][t]) for t in torn_x} == {t: st_ for t, st_, _ in s.tears()}
 
 
 def jittered(case, amp, seed, lx, ly):
@@ -347,6 +350,40 @@ def test_jittered_plastic_zone_matches_oracle():
         f"(max {o.hex_eqps.max():.3f}), torn {len(torn_g)} / {len(torn_o)}")
     assert o.hex_eqps.max() > 0.01
     assert err <= REL_U and deq <= ABS_EQPS and len(torn_g ^ torn_o) <= max(1, len(torn_o) // 50)
+
+
+def test_graded_plate_tears_at_each_elements_own_failure_strain():
+    # initialisePlasticState (engine/sim/zone.cpp:1248-1280) gives every element its own
+    # failure strain from its size when a fracture strain is set: a plate graded along x
+    # (elements three times as long under the punch as at the clamped ends, thinner than
+    # they are wide) and jittered spreads it over the elements that tear
+    lx, ly, nx, ny = 0.3, 0.15, 20, 10
+    case = plastic_zone(10.0, failure=dict(failure_strain=0.04, fracture_strain=0.07), nx=nx, ny=ny)
+    s_ = case.X[:, 0] / lx
+    case.X[:, 0] = lx * (s_ - 0.5 * np.sin(2 * np.pi * s_) / (2 * np.pi))
+    case = jittered(case, 0.15 * ly / ny, 3, lx, ly)
+    ef = H.failure_strain(case.X[case.index], 0.04, 0.07)
+    s = sf.Solver(case)
+    o = oracle_for(case, s.dt)
+    ro = o.run(2500, sample_every=100, threads=8)
+    assert np.array_equal(o.hex_failure, ef)
+    s.run(2500)
+    st = s.readback()
+    torn_g = set(np.nonzero(st.damage >= 1)[0])
+    torn_o = set(np.nonzero(o.hex_damage >= 1)[0])
+    tg = {t: st_ for t, st_, _ in s.tears()}
+    same_steps = sum(1 for t in torn_g & torn_o if tg[t] == o.tear_step[t])
+    err = rel_l2(st.u, o.u)
+    deq = float(np.abs(st.eqps - o.hex_eqps).max())
+    torn_ef = ef[sorted(torn_o)] if torn_o else ef[:0]
+    say(f"graded plate, per-element failure strain {ef.min():.4f}-{ef.max():.4f}: torn {len(torn_g)} / "
+        f"{len(torn_o)} (same step {same_steps}; their strains {torn_ef.min() if len(torn_ef) else 0:.4f}-"
+        f"{torn_ef.max() if len(torn_ef) else 0:.4f}), u rel L2 {err:.3e}, eqps |d| {deq:.2e}, "
+        f"fracture {ro[-1]['fracture']:.3e} J")
+    assert len(torn_ef) > 4 and torn_ef.max() - torn_ef.min() > 0.002
+    assert len(torn_o) > 0 and len(torn_g ^ torn_o) <= max(1, len(torn_o) // 50)
+    assert same_steps >= len(torn_g & torn_o) - max(1, len(torn_o) // 50)
+    assert err <= REL_U and deq <= ABS_EQPS
 
 
 def test_plastic_hex_batch_matches_each_case_alone():

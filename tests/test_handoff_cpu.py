@@ -50,3 +50,31 @@ def test_oracle_adopting_its_own_state_continues_the_same_run():
     assert np.array_equal(c.eqps, a.eqps) and np.array_equal(c.damage, a.damage)
     assert rc == [r for r in ra if r["step"] >= 150]
     assert a.eqps.max() > 0  # the run yields, so the plastic state was carried
+
+
+def test_oracle_hex_adopt_carries_every_points_history():
+    # a plastic solid-shell zone that tears: adopt at step 600 and continue to 1200. The
+    # eight points' history and the enhanced parameters (the next update's Newton start)
+    # travel in hex_points / hex_enhanced; without the enhanced parameters the
+    # continuation is a different rounding of the same run, so identity needs them
+    case = configs.hex_zone(lx=0.3, ly=0.15, nx=12, ny=6, radius=0.04, speed=10.0, plastic=True,
+                            yield_=355e6, hardening=1e9, failure=dict(failure_strain=0.06))
+    dt = 1e-6
+    a = O.Case(case.X, case.index, dt=dt, **case.oracle_kwargs())
+    ra = a.run(1200, sample_every=100, threads=4)
+    b = O.Case(case.X, case.index, dt=dt, **case.oracle_kwargs())
+    b.run(600, threads=4)
+    ex = b.export()
+    assert ex["hex_points"].shape == (case.T, 8, 8) and ex["hex_enhanced"].shape == (case.T, 7)
+    assert np.array_equal(ex["plastic"][:, 6], b.hex_eqps) and np.array_equal(ex["plastic"][:, 7], b.hex_damage)
+    c = O.Case(case.X, case.index, dt=dt, **case.oracle_kwargs()).adopt(ex)
+    rc = c.run(600, sample_every=100, threads=4)
+    assert np.array_equal(c.u, a.u) and np.array_equal(c.v, a.v)
+    assert np.array_equal(c.hex_point, a.hex_point) and np.array_equal(c.tear_step, a.tear_step)
+    assert [r["step"] for r in rc] == [r["step"] for r in ra if r["step"] >= 600]
+    assert a.hex_eqps.max() > 0.01 and (a.hex_damage >= 1).any()
+    # the enhanced parameters matter: a cold start of the Newton iterations changes bits
+    d = O.Case(case.X, case.index, dt=dt, **case.oracle_kwargs()).adopt(
+        {k: v for k, v in ex.items() if k != "hex_enhanced"})
+    d.run(600, threads=4)
+    assert not np.array_equal(d.u, a.u)

@@ -363,3 +363,28 @@ def test_hex_cases_round_trip_through_the_wire_format(tmp_path):
     # a tet case stays a tet case
     t = sf.Case.decode(configs.plate(lx=0.2, ly=0.1, nx=4, ny=2).encode())
     assert type(t) is sf.Case and t.index.shape[1] == 4
+
+
+def test_element_failure_strain_matches_initialise_plastic_state():
+    # initialisePlasticState's regularised failure strain of the eight golden elements
+    # (uniform 0.12, fracture 0.8): the oracle's restatement and the library's host function
+    nodes = GOLD["el_nodes"]
+    want = GOLD["fs_strain"]
+    got_o = H.failure_strain(nodes, 0.12, 0.8)
+    assert np.allclose(got_o, want, rtol=1e-13, atol=0)
+    got_l = np.array([sf.hex_failure_strain(x, 0.12, 0.8) for x in nodes])
+    assert np.allclose(got_l, want, rtol=1e-13, atol=0)
+    # closed forms: an l x l x t box element reads share t / l; an element thicker than it
+    # is wide reads the fracture strain; a collapsed mid-surface falls to fracture
+    box = np.array([[0, 0, 0], [0.4, 0, 0], [0.4, 0.4, 0], [0, 0.4, 0],
+                    [0, 0, 0.01], [0.4, 0, 0.01], [0.4, 0.4, 0.01], [0, 0.4, 0.01]], np.float64)
+    assert sf.hex_failure_strain(box, 0.12, 0.8) == pytest.approx(0.12 + 0.68 * 0.01 / 0.4, rel=1e-13)
+    tall = box * np.array([1.0, 1.0, 100.0])
+    assert sf.hex_failure_strain(tall, 0.12, 0.8) == pytest.approx(0.8, rel=1e-15)
+    flat = box.copy()
+    flat[:, 1] = 0.0
+    assert sf.hex_failure_strain(flat, 0.12, 0.8) == 0.8
+    assert H.failure_strain(flat[None], 0.12, 0.8)[0] == 0.8
+    # a wrong share (thickness over area instead of over the in-plane size) would miss
+    assert not np.allclose(0.12 + 0.68 * np.minimum(1, H.element_size(nodes)[1] / H.element_size(nodes)[0] ** 2),
+                           want, rtol=1e-6)
