@@ -499,6 +499,19 @@ def counters_for(name, kernel_key):
     return dict(bytes=t[kernel_key], source=t.get("source", f"profiles/traffic_{name}.json"))
 
 
+def valu_issue(name, kernel_key, ms):
+    """The kernel's VALU issue time from its committed SQ_INSTS_VALU count (a wave64 FP32
+    instruction holds a 16-lane SIMD for 4 cycles; 1024 SIMDs at 2.4 GHz) and its fraction
+    of the measured duration."""
+    t = traffic_from_profiles(name)
+    key = f"{kernel_key}_insts_valu"
+    if not t or key not in t or not ms:
+        return None
+    issue_ms = t[key] * 4 / 1024 / 2.4e9 * 1e3
+    return dict(valu_issue_ms=round(issue_ms, 5), valu_issue_frac=round(issue_ms / ms, 4),
+                valu_per_wave=round(t[key] / t[f"{kernel_key}_waves"], 1))
+
+
 PRE_STEPS = 500  # untimed steps before the timed ones in the batched configuration
 
 
@@ -529,8 +542,10 @@ def bench_headline(args, sf, info):
     b = sf.Batch(cases, chunk=S)
     prep_s = time.perf_counter() - t0
     arena = b.footprint()  # creation copied this many bytes device to device once
-    # into the loaded regime (every case's load is at full ramp and the stiffest plates
-    # yield): the timed steps follow `pre` untimed ones, as in the middle of a 1000-step run
+    # past the start-up: the timed steps follow `pre` untimed ones, with every case's
+    # pressure ramping (a plate's bending period is tens of ms, so over these 0.1-0.2 ms
+    # of simulated time the plates load elastically: every tet's J2 trial is tested and
+    # none yields; `yielded_tets` reports it)
     pre = S * max(1, round(PRE_STEPS / S))
     b.run(pre)
     best = b.tune(steps_per_trial=max(S, 100))  # restores the state it started from
@@ -548,15 +563,21 @@ def bench_headline(args, sf, info):
     yielded = sum(int(np.count_nonzero(st.eqps)) for st in states)
     timed_from = int(states[0].step) - K
     del states
-    # per-kernel durations, live: the timed region's device time per step (HIP events around
-    # the graph replay on the batch's stream) apportioned between the two passes by their
-    # durations when each dispatch carries its own events (those read a few us long each:
-    # the event markers' own latency, which the ratio cancels)
+    # per-kernel durations, live, of the one-stream kernels (each over all 64 cases; with
+    # two streams the halves' kernels share the device and no kernel has a duration of its
+    # own): after the timed steps the batch switches to one stream and runs whole chunks;
+    # that segment's device time per step (HIP events around the graph replay on the
+    # batch's stream) is apportioned between the two passes by their durations when each
+    # dispatch carries its own events (those read a few us long each: the event markers'
+    # own latency, which the ratio cancels)
     b.run(-int(timed_from + K) % S)  # back onto a chunk boundary
+    b.set_config(best["elem_block"], best["node_block"], best["xcd_remap"], best["variant"], split=-1)
+    one = S * max(1, -(-200 // S))
+    b.run(S)  # captures the one-stream chunk before its timing starts
+    one_ms = b.run(one) / one
     ev_elem, ev_node = b.profile(S * max(1, -(-40 // S)))
-    step_ms = dev_ms / K
-    elem_ms = step_ms * ev_elem / (ev_elem + ev_node)
-    node_ms = step_ms * ev_node / (ev_elem + ev_node)
+    elem_ms = one_ms * ev_elem / (ev_elem + ev_node)
+    node_ms = one_ms * ev_node / (ev_elem + ev_node)
     tr = b.traffic()
     elem_need = tr["elem_read"] + tr["elem_write"]
     node_need = tr["node_read"] + tr["node_write"]
@@ -571,7 +592,11 @@ def bench_headline(args, sf, info):
     roofline.update(elem_ms=round(elem_ms, 5), node_ms=round(node_ms, 5),
                     event_elem_ms=round(ev_elem, 5), event_node_ms=round(ev_node, 5),
                     elem_bytes=int(elem_need), node_bytes=int(node_need),
-                    node_frac=round(node_need / (node_ms * 1e-3) / 8e12, 4))
+                    node_frac=round(node_need / (node_ms * 1e-3) / 8e12, 4),
+                    one_stream_ms_per_step=round(one_ms, 5), one_stream_steps=one)
+    valu = valu_issue("batch", "element", elem_ms)
+    if valu:
+        roofline.update(valu)
     step_need = elem_need + node_need
     # the ceiling of the batched rate: every step's compulsory bytes at the HBM peak
     batched_bound = tets / (step_need / 8e12)
@@ -587,6 +612,7 @@ def bench_headline(args, sf, info):
                batch_arena_bytes=arena,
                tuned=best, device_ms_per_step=dev_ms / K, timed_steps_from=timed_from,
                eqps_max=eqps_max, yielded_tets=yielded,
+               streams=2 if best.get("split", -1) == 0 else 1,
                step_bytes=int(step_need), step_hbm_frac=round(step_need * K / (dev_ms * 1e-3) / 8e12, 4),
                batched_bound_at_hbm_peak=batched_bound)
     if "plain" in legs:
@@ -637,7 +663,7 @@ def bench_headline(args, sf, info):
                             vs_reference_sustained=case.T * n / pw / REFERENCE_SUSTAINED)
         s.close()
     if "serve" in legs:
-        cfg["served"] = serve_leg(sf, rates=(20.0, 100.0), duration=args.duration, pool_n=12)
+        cfg["served"] = serve_leg(sf, rates=(20.0, 100.0, 150.0), duration=args.duration, pool_n=12)
     cpu = None
     if "cpu" in legs and not args.no_cpu:
         try:

@@ -264,6 +264,10 @@ typedef struct shipfem_launch_config {
     int32_t xcd_remap;  /* 1: give each XCD a contiguous range of tiles */
     int32_t chunk;      /* steps per captured graph (even) */
     int32_t variant;    /* SHIPFEM_VARIANT_* */
+    int32_t split;      /* a batch of two or more cases: 0 runs them as two halves (balanced
+                         * by element count) on two streams, each replaying its own captured
+                         * chunks, so one half's kernels fill the other's launch gaps and
+                         * tails (each case's arithmetic unchanged); -1: one stream */
 } shipfem_launch_config;
 
 /* ---- one case (replaces gpu::FemGpuSolver, fem_gpu.hpp:17-45) -------------- */
@@ -310,8 +314,10 @@ shipfem_status shipfem_solver_footprint(const shipfem_solver* s, uint64_t* bytes
 /* The HBM bytes one step's passes must move, by the layout of this case: out[0], out[1]
  * the element pass's reads and writes, out[2], out[3] the node pass's. Every plane a pass
  * reads is counted once (a node's displacement once, however many tiles gather it) and
- * every plane it writes once; not counted are the J2 history a yielding tet writes back and
- * the per-wave FP64 sums. The roofline of a pass is these bytes over its duration. */
+ * every plane it writes once. A J2 tet pass reads every tet's eqps and the plastic strain
+ * (and damage) only of tets that have yielded, counted on the device at the call. Not
+ * counted are the J2 history a yielding tet writes back and the per-wave FP64 sums. The
+ * roofline of a pass is these bytes over its duration. */
 shipfem_status shipfem_solver_traffic(const shipfem_solver* s, uint64_t out[4]);
 void shipfem_solver_destroy(shipfem_solver* s);
 
@@ -598,8 +604,8 @@ typedef struct shipfem_server_opts {
     int32_t chunk;           /* steps per device round between admissions; 0 => 64 */
     shipfem_launch_config launch; /* zeros => defaults */
     int32_t policy;          /* SHIPFEM_ROUNDS_* ; 0 => ALL */
-    double round_work;       /* SHORTEST: tet-steps a round takes (at least one case); 0 => 5e7 */
-    int32_t starve_rounds;   /* SHORTEST: 0 => 8 */
+    double round_work;       /* SHORTEST: tet-steps a round takes (at least one case); 0 => 2e8 */
+    int32_t starve_rounds;   /* SHORTEST: 0 => 4 */
 } shipfem_server_opts;
 
 typedef struct shipfem_result {

@@ -87,6 +87,8 @@ bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out) {
         if (cfg->elem_block || cfg->node_block) sh.xcd_remap = cfg->xcd_remap ? 1 : 0;
         if (cfg->variant != SHIPFEM_VARIANT_TILED && cfg->variant != SHIPFEM_VARIANT_SLOTS) return false;
         sh.variant = cfg->variant == SHIPFEM_VARIANT_SLOTS ? kSlots : kTiled;
+        if (cfg->split != 0 && cfg->split != -1) return false;
+        sh.split = cfg->split == 0 ? 1 : 0;
     }
     if (!blockOk(sh.elem_block) || !blockOk(sh.node_block)) return false;
     out = sh;
@@ -97,23 +99,86 @@ bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out) {
 struct shipfem_batch {
     std::vector<HostCase> hcs;
     std::vector<DeviceCase> dcs;
-    Plan plan;
+    Plan plan;  // every case: profiling, the tuner's reference, one-stream runs
+    // With shape.split and two or more cases: the cases in two halves, each advanced by its
+    // own plan on its own stream (half[0] on `stream`, half[1] on `side`), forked from and
+    // joined back into `stream`. Every plan keeps the same parity.
+    Plan half[2];
+    bool split = false;
     char* arena = nullptr;
     char* backup = nullptr;
     uint64_t bytes = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, fork = nullptr, join = nullptr;
     int64_t host_step = 0;
     int S = 100;
     ~shipfem_batch() {
         plan.dropGraph();
+        half[0].dropGraph();
+        half[1].dropGraph();
         if (arena) (void)hipFree(arena);
         if (backup) (void)hipFree(backup);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+namespace {
+// Splits the batch's cases into two halves of nearly equal element count (largest first,
+// each to the lighter half) and builds their plans, when the shape asks for it.
+hipError_t buildHalves(shipfem_batch* b) {
+    b->split = b->plan.shape.split && b->plan.recs.size() >= 2;
+    if (!b->split) {
+        b->half[0].dropGraph();
+        b->half[1].dropGraph();
+        return hipSuccess;
+    }
+    std::vector<size_t> order(b->plan.recs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return b->plan.recs[x].T > b->plan.recs[y].T; });
+    uint64_t load[2] = {0, 0};
+    std::vector<size_t> members[2];
+    for (size_t i : order) {
+        const int h = load[1] < load[0] ? 1 : 0;
+        members[h].push_back(i);
+        load[h] += b->plan.recs[i].T;
+    }
+    hipError_t e = hipSuccess;
+    for (int h = 0; h < 2 && e == hipSuccess; ++h) {
+        std::sort(members[h].begin(), members[h].end());
+        Plan& p = b->half[h];
+        p.recs.clear();
+        for (size_t i : members[h]) p.recs.push_back(b->plan.recs[i]);
+        p.shape = b->plan.shape;
+        p.parity = b->plan.parity;
+        e = p.build(b->stream);
+    }
+    return e;
+}
+
+// Advances every case `steps` steps: on `stream` alone, or the two halves side by side.
+hipError_t batchAdvance(shipfem_batch* b, int64_t steps) {
+    if (!b->split) {
+        const hipError_t e = advance(b->plan, b->host_step, steps, 0, b->S, b->stream);
+        b->half[0].parity = b->half[1].parity = b->plan.parity;
+        return e;
+    }
+    int64_t h1 = b->host_step;
+    hipError_t e = hipEventRecord(b->fork, b->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->fork, 0);
+    if (e == hipSuccess) e = advance(b->half[0], b->host_step, steps, 0, b->S, b->stream);
+    if (e == hipSuccess) e = advance(b->half[1], h1, steps, 0, b->S, b->side);
+    if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join, 0);
+    b->plan.parity = b->half[0].parity;
+    return e;
+}
+}  // namespace
 
 extern "C" {
 
@@ -422,7 +487,10 @@ shipfem_status shipfem_solver_profile(shipfem_solver* s, int32_t steps, double* 
 shipfem_status shipfem_solver_traffic(const shipfem_solver* s, uint64_t out[4]) {
     if (!s || !out) return SHIPFEM_INVALID_ARG;
     for (int k = 0; k < 4; ++k) out[k] = 0;
-    addBytes(compulsoryBytes(s->hc, s->plan.shape.variant), out);
+    uint64_t yielded = 0;
+    const hipError_t e = yieldedTets(s->dc, s->hc, s->stream, &yielded);
+    if (e != hipSuccess) return statusOf(e);
+    addBytes(compulsoryBytes(s->hc, s->plan.shape.variant, yielded), out);
     return SHIPFEM_OK;
 }
 
@@ -483,8 +551,11 @@ shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
     for (uint32_t i = 0; i < n; ++i) off[i + 1] = off[i] + (arenaBytes(b->hcs[i]) + 4095) / 4096 * 4096;
     b->bytes = off[n];
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->e0);
     if (e == hipSuccess) e = hipEventCreate(&b->e1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->arena), b->bytes);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->backup), b->bytes);
     for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
@@ -509,6 +580,7 @@ shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
         }
         e = b->plan.uploadRecords(b->stream);
     }
+    if (e == hipSuccess) e = buildHalves(b);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->backup, b->arena, b->bytes, hipMemcpyDeviceToDevice, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -559,7 +631,7 @@ shipfem_status shipfem_hex_batch_create(const shipfem_case_desc* cases, uint32_t
 shipfem_status shipfem_batch_run(shipfem_batch* b, int32_t steps, double* device_ms) {
     if (!b || steps < 0) return SHIPFEM_INVALID_ARG;
     hipError_t e = hipEventRecord(b->e0, b->stream);
-    if (e == hipSuccess) e = advance(b->plan, b->host_step, steps, 0, b->S, b->stream);
+    if (e == hipSuccess) e = batchAdvance(b, steps);
     if (e == hipSuccess) e = hipEventRecord(b->e1, b->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b->e1);
     float ms = 0;
@@ -573,7 +645,9 @@ shipfem_status shipfem_batch_profile(shipfem_batch* b, int32_t steps, double* el
     if (!b || steps < 1) return SHIPFEM_INVALID_ARG;
     // a captured chunk starts at parity 0 on a chunk boundary: profile whole chunks
     if (b->host_step % b->S != 0 || b->plan.parity != 0 || steps % b->S != 0) return SHIPFEM_INVALID_ARG;
-    return statusOf(profilePlan(b->plan, b->host_step, b->S, steps, b->stream, elem_ms, node_ms));
+    const hipError_t e = profilePlan(b->plan, b->host_step, b->S, steps, b->stream, elem_ms, node_ms);
+    b->half[0].parity = b->half[1].parity = b->plan.parity;
+    return statusOf(e);
 }
 
 shipfem_status shipfem_batch_footprint(const shipfem_batch* b, uint64_t* bytes) {
@@ -585,7 +659,12 @@ shipfem_status shipfem_batch_footprint(const shipfem_batch* b, uint64_t* bytes) 
 shipfem_status shipfem_batch_traffic(const shipfem_batch* b, uint64_t out[4]) {
     if (!b || !out) return SHIPFEM_INVALID_ARG;
     for (int k = 0; k < 4; ++k) out[k] = 0;
-    for (const HostCase& hc : b->hcs) addBytes(compulsoryBytes(hc, b->plan.shape.variant), out);
+    for (size_t i = 0; i < b->hcs.size(); ++i) {
+        uint64_t yielded = 0;
+        const hipError_t e = yieldedTets(b->dcs[i], b->hcs[i], b->stream, &yielded);
+        if (e != hipSuccess) return statusOf(e);
+        addBytes(compulsoryBytes(b->hcs[i], b->plan.shape.variant, yielded), out);
+    }
     return SHIPFEM_OK;
 }
 
@@ -595,7 +674,9 @@ shipfem_status shipfem_batch_set_config(shipfem_batch* b, const shipfem_launch_c
     if (!shapeFrom(cfg, sh)) return SHIPFEM_INVALID_ARG;
     if (b->host_step % b->S != 0 || b->plan.parity != 0) return SHIPFEM_INVALID_ARG;
     b->plan.shape = sh;
-    return statusOf(b->plan.build(b->stream));
+    hipError_t e = b->plan.build(b->stream);
+    if (e == hipSuccess) e = buildHalves(b);
+    return statusOf(e);
 }
 
 shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
@@ -618,9 +699,10 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
         b->plan.parity = 0;
         b->plan.shape = sh;
         if (r == hipSuccess) r = b->plan.build(b->stream);
-        if (r == hipSuccess) r = advance(b->plan, b->host_step, b->S, 0, b->S, b->stream);  // warm-up
+        if (r == hipSuccess) r = buildHalves(b);
+        if (r == hipSuccess) r = batchAdvance(b, b->S);  // warm-up
         if (r == hipSuccess) r = hipEventRecord(b->e0, b->stream);
-        if (r == hipSuccess) r = advance(b->plan, b->host_step, trial, 0, b->S, b->stream);
+        if (r == hipSuccess) r = batchAdvance(b, trial);
         if (r == hipSuccess) r = hipEventRecord(b->e1, b->stream);
         if (r == hipSuccess) r = hipEventSynchronize(b->e1);
         float f = 0;
@@ -628,26 +710,28 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
         ms = f / trial;
         return r;
     };
-    // coordinate descent: XCD remap, then element block, then node block
+    // coordinate descent: one stream or two, XCD remap, element block, node block
     LaunchShape cur = b->plan.shape;
     double curMs = 0;
     e = timeShape(cur, curMs);
     const int blocks[] = {64, 128, 256, 512};
-    for (int pass = 0; pass < 3 && e == hipSuccess; ++pass) {
+    for (int pass = -1; pass < 3 && e == hipSuccess; ++pass) {
+        if (pass == -1 && b->plan.recs.size() < 2) continue;
         // the tiled variant's element block is its 256-tet tile: not a free parameter (the
         // hex pass always takes one lane per element, so for hexes it is)
         if (pass == 1 && cur.variant == kTiled && !b->plan.hex) continue;
         std::vector<LaunchShape> cands;
-        for (int v : (pass == 0 ? std::vector<int>{0, 1} : std::vector<int>(blocks, blocks + 4))) {
+        for (int v : (pass <= 0 ? std::vector<int>{0, 1} : std::vector<int>(blocks, blocks + 4))) {
             LaunchShape c = cur;
-            if (pass == 0) c.xcd_remap = v;
+            if (pass == -1) c.split = v;
+            else if (pass == 0) c.xcd_remap = v;
             else if (pass == 1) c.elem_block = v;
             else c.node_block = v;
             cands.push_back(c);
         }
         for (const LaunchShape& c : cands) {
             if (c.elem_block == cur.elem_block && c.node_block == cur.node_block &&
-                c.xcd_remap == cur.xcd_remap)
+                c.xcd_remap == cur.xcd_remap && c.split == cur.split)
                 continue;
             double ms = 0;
             e = timeShape(c, ms);
@@ -663,6 +747,7 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
     b->plan.parity = 0;
     b->plan.shape = cur;
     if (e == hipSuccess) e = b->plan.build(b->stream);
+    if (e == hipSuccess) e = buildHalves(b);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     (void)hipFree(snap);
     if (e != hipSuccess) return statusOf(e);
@@ -672,6 +757,7 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
         best->xcd_remap = cur.xcd_remap;
         best->chunk = b->S;
         best->variant = cur.variant == kSlots ? SHIPFEM_VARIANT_SLOTS : SHIPFEM_VARIANT_TILED;
+        best->split = cur.split ? 0 : -1;
     }
     if (best_ms) *best_ms = curMs;
     return SHIPFEM_OK;
@@ -702,7 +788,7 @@ shipfem_status shipfem_batch_reset(shipfem_batch* b) {
     hipError_t e = hipMemcpyAsync(b->arena, b->backup, b->bytes, hipMemcpyDeviceToDevice, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     b->host_step = 0;
-    b->plan.parity = 0;
+    b->plan.parity = b->half[0].parity = b->half[1].parity = 0;
     return statusOf(e);
 }
 

@@ -239,6 +239,14 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
         for (uint32_t t = 0; t < T; ++t)
             for (uint32_t k = 0; k < planes; ++k)
                 ps[k * Tp + t] = static_cast<float>(in->plastic[externalTet(hc, t) * 8ull + k]);
+        // the element pass reads a tet's plastic strain and damage only when its eqps is not
+        // +0 (zero eqps means never yielded for any state the device makes); history handed
+        // over with zero eqps and something else not zero keeps its eqps as -0 so it is read
+        for (uint32_t t = 0; t < T; ++t) {
+            bool any = false;
+            for (uint32_t k = 0; k < planes; ++k) any = any || (k != 6 && ps[k * Tp + t] != 0.0f);
+            if (any && ps[6 * Tp + t] == 0.0f) ps[6 * Tp + t] = -0.0f;
+        }
         if (e == hipSuccess) e = toDevice(r.pstate, ps.data(), 4 * ps.size(), st);
         if (dmg) {
             // a torn tet handed over without its tear step is recorded as torn at `steps`

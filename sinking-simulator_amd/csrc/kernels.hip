@@ -99,7 +99,13 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, int6
                                               double& diss, double& se, double& fract,
                                               bool& tore) {
     tore = false;
-    if (DAMAGE && (c.features & kFeatDamage) && c.pstate[7 * c.Tpad + e] >= 1.0f) {
+    // eqps grows with every plastic increment, so a +0 eqps marks a tet that has never
+    // yielded: its plastic strain and damage are zero and are not read. (A state handed
+    // over from elsewhere with plastic strain but zero eqps carries -0, which reads.)
+    const bool plastic = PLASTIC && (c.features & kFeatPlastic);
+    const float eqps0 = plastic ? c.pstate[6 * c.Tpad + e] : 0.0f;
+    const bool history = plastic && __float_as_uint(eqps0) != 0u;
+    if (DAMAGE && (c.features & kFeatDamage) && history && c.pstate[7 * c.Tpad + e] >= 1.0f) {
         // torn: no stress, no strain energy, history frozen
 #pragma unroll
         for (int k = 0; k < 12; ++k) f[k] = 0.0f;
@@ -172,12 +178,18 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, int6
     const float e01 = 0.5f * (A[1] + A[3]), e12 = 0.5f * (A[5] + A[7]), e02 = 0.5f * (A[2] + A[6]);
 
     float s00, s11, s22, s01, s12, s02;
-    if (PLASTIC && (c.features & kFeatPlastic)) {
+    if (plastic) {
         // J2 radial return, linear isotropic hardening (plasticity.cpp:319-455)
         const uint32_t tp = c.Tpad;
         float ep[7];
+        ep[6] = eqps0;
+        if (history) {
 #pragma unroll
-        for (int k = 0; k < 7; ++k) ep[k] = c.pstate[k * tp + e];
+            for (int k = 0; k < 6; ++k) ep[k] = c.pstate[k * tp + e];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) ep[k] = 0.0f;
+        }
         const float st[6] = {e00, e11, e22, 2.0f * e01, 2.0f * e12, 2.0f * e02};
         float el[6];
 #pragma unroll
@@ -220,7 +232,7 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, int6
                 // the cutoff; eta = mean stress over von Mises of the returned stress
                 const float q = 1.22474487139158905f * (nrm - 2.0f * c.mu * gamma);
                 const float eta = q > 0.0f ? pr / q : 0.0f;
-                float dmg = c.pstate[7 * tp + e];
+                float dmg = history ? c.pstate[7 * tp + e] : 0.0f;
                 if (eta > c.fail_ecut)
                     dmg += root23 * gamma * __expf(c.fail_sens * (eta - c.fail_eref)) / c.fail_strain;
                 if (dmg >= 1.0f) {
@@ -672,17 +684,15 @@ __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
     return tiled_lds_offsets_offset() + ((lmax + 1) * sizeof(uint16_t) + 15) / 16 * 16;
 }
 
+// The element pass of tile `b` (one 256-thread block).
 template <bool PLASTIC, bool DAMAGE>
-__global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict__ cases,
-                                                      const Tile* __restrict__ tiles, int ntiles,
-                                                      uint32_t lmax, int parity, int sample,
-                                                      int xcd) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cases,
+                                                 const Tile* __restrict__ tiles, int b, uint32_t lmax,
+                                                 int parity, int sample, unsigned char* smem) {
     float* sh_f = reinterpret_cast<float*>(smem);
     uint16_t* sh_lo = reinterpret_cast<uint16_t*>(smem + tiled_lds_offsets_offset());
     float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
 
-    const int b = remap_block(blockIdx.x, gridDim.x, xcd);
     const Tile tile = tiles[b];
     const CaseDev& c = cases[tile.case_index];
     const uint32_t tb = tile.first / kTile;
@@ -777,12 +787,21 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
     }
 }
 
+template <bool PLASTIC, bool DAMAGE>
+__global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict__ cases,
+                                                      const Tile* __restrict__ tiles, int ntiles,
+                                                      uint32_t lmax, int parity, int sample,
+                                                      int xcd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    tiled_elem_block<PLASTIC, DAMAGE>(cases, tiles, remap_block(blockIdx.x, gridDim.x, xcd), lmax, parity,
+                                      sample, smem);
+}
+
 // ---------------------------------------------------------------- node pass
-template <int BLOCK, bool EXTRA, int VARIANT>
-__global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ cases,
-                                                const Tile* __restrict__ tiles, int ntiles,
-                                                int parity, int sample, int xcd) {
-    const int b = remap_block(blockIdx.x, gridDim.x, xcd);
+// The node pass of tile `b` (one block of as many threads as the tile has nodes).
+template <bool EXTRA, int VARIANT>
+__device__ __forceinline__ void node_block(const CaseDev* __restrict__ cases, const Tile* __restrict__ tiles,
+                                           int b, int parity, int sample) {
     const Tile tile = tiles[b];
     const CaseDev& c = cases[tile.case_index];
     const uint32_t i = tile.first + threadIdx.x;
@@ -1026,6 +1045,13 @@ __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ case
             if (writer) c.smp_n[k * nw + w] = s;
         }
     }
+}
+
+template <int BLOCK, bool EXTRA, int VARIANT>
+__global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ cases,
+                                                const Tile* __restrict__ tiles, int ntiles,
+                                                int parity, int sample, int xcd) {
+    node_block<EXTRA, VARIANT>(cases, tiles, remap_block(blockIdx.x, gridDim.x, xcd), parity, sample);
 }
 
 // ---------------------------------------------------------------- FP64 reductions

@@ -628,7 +628,7 @@ std::string prepareHexCase(const shipfem_case_desc& d, HostCase& hc, int32_t his
 
 uint64_t arenaBytes(const HostCase& hc) { return mapArena(hc).total; }
 
-PassBytes compulsoryBytes(const HostCase& hc, int variant) {
+PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded) {
     PassBytes b;
     if (hc.elem_tiles == 0) variant = kSlots;  // a case without tiles always runs slots
     const uint64_t T = hc.T, N = hc.N, Np = hc.Npad, P = hc.P, nt = hc.elem_tiles;
@@ -636,7 +636,9 @@ PassBytes compulsoryBytes(const HostCase& hc, int variant) {
     uint64_t freeNodes = 0;
     for (uint32_t n = 0; n < hc.N && n < hc.nflags.size(); ++n)
         freeNodes += (hc.nflags[n] & SHIPFEM_PIN_ALL) != SHIPFEM_PIN_ALL;
-    const uint64_t history = (f & kFeatPlastic) ? ((f & kFeatDamage) ? 32 : 28) * T : 0;
+    // a tet pass reads eqps of every tet and the plastic strain (and damage) of those that
+    // have yielded
+    const uint64_t history = (f & kFeatPlastic) ? 4 * T + ((f & kFeatDamage) ? 28 : 24) * yielded : 0;
     if (hc.corners == 8) {
         const bool plastic = (f & kFeatPlastic) != 0;
         // connectivity, the stiffness block (elastic) or the FP64 rest geometry and the point
@@ -667,6 +669,17 @@ PassBytes compulsoryBytes(const HostCase& hc, int variant) {
                   ((f & kFeatDrive) ? 12 * Np : 0);
     b.node_write = 52 * freeNodes;
     return b;
+}
+
+hipError_t yieldedTets(const DeviceCase& dc, const HostCase& hc, hipStream_t s, uint64_t* count) {
+    *count = 0;
+    if (hc.corners != 4 || !(hc.features & kFeatPlastic) || !dc.rec.pstate || hc.T == 0) return hipSuccess;
+    std::vector<uint32_t> q(hc.T);
+    hipError_t e = hipMemcpyAsync(q.data(), dc.rec.pstate + 6ull * hc.Tpad, 4ull * hc.T, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    for (uint32_t v : q) *count += v != 0u;  // eqps bits not +0
+    return hipSuccess;
 }
 
 hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_t s) {

@@ -512,10 +512,16 @@ void shipfem_server::failAll(hipError_t e) {
     cv_done.notify_all();
 }
 
+// SHORTEST's defaults: about 3 M tets at a 64-step chunk fill the device for a round (a
+// smaller round leaves CUs idle), and a case waits at most 4 rounds, so the largest keeps
+// at least a fifth of the rate it would have under ALL.
+constexpr double kShortestRoundWork = 2e8;
+constexpr int32_t kShortestStarveRounds = 4;
+
 void shipfem_server::pickShortest(std::vector<Active*>& members) {
     if (members.empty()) return;
-    const double budget = opts.round_work > 0 ? opts.round_work : 5e7;
-    const int32_t starve = opts.starve_rounds > 0 ? opts.starve_rounds : 8;
+    const double budget = opts.round_work > 0 ? opts.round_work : kShortestRoundWork;
+    const int32_t starve = opts.starve_rounds > 0 ? opts.starve_rounds : kShortestStarveRounds;
     auto left = [](const Active* a) { return static_cast<double>(a->req->hc.T) * static_cast<double>(a->steps - a->predicted); };
     std::stable_sort(members.begin(), members.end(), [&](const Active* x, const Active* y) {
         const bool sx = x->left_out >= starve, sy = y->left_out >= starve;

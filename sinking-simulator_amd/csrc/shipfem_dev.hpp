@@ -154,6 +154,7 @@ struct LaunchShape {
     int node_block = 256;
     int xcd_remap = 1;
     int variant = kTiled;
+    int split = 1;  // a batch: two halves on two streams
 };
 
 // Start/stop events recorded by the kernel dispatch itself (hipExtLaunchKernel), so

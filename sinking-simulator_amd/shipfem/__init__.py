@@ -85,7 +85,7 @@ VARIANT_TILED, VARIANT_SLOTS = 0, 1
 
 class LaunchConfig(C.Structure):
     _fields_ = [("elem_block", C.c_int32), ("node_block", C.c_int32), ("xcd_remap", C.c_int32),
-                ("chunk", C.c_int32), ("variant", C.c_int32)]
+                ("chunk", C.c_int32), ("variant", C.c_int32), ("split", C.c_int32)]
 
 
 ROUNDS_ALL, ROUNDS_SHORTEST = 0, 1
@@ -881,13 +881,14 @@ class Solver:
 
 
 class Batch:
-    """Many cases advancing in one launch per phase."""
+    """Many cases advancing in one launch per phase. split=0: two or more cases run as two
+    halves on two streams side by side; split=-1: one stream."""
 
     def __init__(self, cases, elem_block=256, node_block=256, xcd_remap=1, chunk=0,
-                 variant=VARIANT_TILED):
+                 variant=VARIANT_TILED, split=0):
         self.cases = list(cases)
         self._descs = (CaseDesc * len(self.cases))(*[c.desc() for c in self.cases])
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk, variant)
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk, variant, split)
         err = C.create_string_buffer(512)
         h = C.c_void_p()
         kinds = {c.corners for c in self.cases}
@@ -937,11 +938,11 @@ class Batch:
         ms = C.c_double()
         _check(lib().shipfem_batch_tune(self.h, steps_per_trial, C.byref(cfg), C.byref(ms)))
         return dict(elem_block=cfg.elem_block, node_block=cfg.node_block,
-                    xcd_remap=cfg.xcd_remap, chunk=cfg.chunk, variant=cfg.variant,
+                    xcd_remap=cfg.xcd_remap, chunk=cfg.chunk, variant=cfg.variant, split=cfg.split,
                     ms_per_step=ms.value)
 
-    def set_config(self, elem_block, node_block, xcd_remap, variant=VARIANT_TILED):
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant)
+    def set_config(self, elem_block, node_block, xcd_remap, variant=VARIANT_TILED, split=0):
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant, split)
         _check(lib().shipfem_batch_set_config(self.h, C.byref(cfg)))
 
     def reset(self):
