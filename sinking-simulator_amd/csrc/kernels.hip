@@ -60,6 +60,22 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const
     return reinterpret_cast<const __attribute__((address_space(1))) T*>(reinterpret_cast<uintptr_t>(p));
 }
 
+// A load at a wave-uniform base plus a 32-bit byte offset per lane: the address stays in
+// the load's scalar base + vector offset form, with no 64-bit address arithmetic per lane.
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* base, uint32_t byte_offset) {
+    return *gmem(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_offset));
+}
+
+// A wave-uniform pointer pinned in scalar registers: the compiler may not rewrite a set of
+// such bases into one per-lane base plus running 64-bit increments.
+template <typename T>
+__device__ __forceinline__ const T* sgpr_base(const T* p) {
+    uint64_t v = reinterpret_cast<uint64_t>(p);
+    asm volatile("" : "+s"(v));
+    return reinterpret_cast<const T*>(v);
+}
+
 // Vector-typed loads through the global address space (HIP's vector structs cannot be
 // copied from an address-space-qualified reference, their native vectors can).
 __device__ __forceinline__ float4 gload(const float4* p) {
@@ -103,9 +119,10 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, int6
     // yielded: its plastic strain and damage are zero and are not read. (A state handed
     // over from elsewhere with plastic strain but zero eqps carries -0, which reads.)
     const bool plastic = PLASTIC && (c.features & kFeatPlastic);
-    const float eqps0 = plastic ? c.pstate[6 * c.Tpad + e] : 0.0f;
+    const uint32_t e4 = 4u * e;  // a plane's byte offset (Tpad < 2^30)
+    const float eqps0 = plastic ? ld_at(c.pstate + 6ull * c.Tpad, e4) : 0.0f;
     const bool history = plastic && __float_as_uint(eqps0) != 0u;
-    if (DAMAGE && (c.features & kFeatDamage) && history && c.pstate[7 * c.Tpad + e] >= 1.0f) {
+    if (DAMAGE && (c.features & kFeatDamage) && history && ld_at(c.pstate + 7ull * c.Tpad, e4) >= 1.0f) {
         // torn: no stress, no strain energy, history frozen
 #pragma unroll
         for (int k = 0; k < 12; ++k) f[k] = 0.0f;
@@ -185,7 +202,7 @@ __device__ __forceinline__ void element_force(const CaseDev& c, uint32_t e, int6
         ep[6] = eqps0;
         if (history) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) ep[k] = c.pstate[k * tp + e];
+            for (int k = 0; k < 6; ++k) ep[k] = ld_at(sgpr_base(c.pstate + (uint64_t)k * tp), e4);
         } else {
 #pragma unroll
             for (int k = 0; k < 6; ++k) ep[k] = 0.0f;
@@ -668,8 +685,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 }
 
 // ---------------------------------------------------------------- element pass, tiled
-// One block per 256-tet tile. The tile's local nodes are numbered by descending corner
-// count and each has a contiguous run of 4-kTile corner slots, in (tet, corner) order,
+// One block per 256-tet tile. The tile's local nodes are numbered in ascending node id
+// and each has a contiguous run of 4-kTile corner slots, in (tet, corner) order,
 // starting at its offset lofs[l]. A tet's connectivity word carries, per corner, the
 // local node (10 bits) and the corner's rank in that node's run (6 bits), so a tet
 // lane writes each corner force straight into its slot and a node lane sums one
@@ -720,9 +737,10 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cas
     const uint32_t ec = min(e, c.T - 1);
     const uint2 lc = gload(c.lconn + ec);
     float B[9];
+    const uint32_t ec4 = 4u * ec;  // byte offset into a per-tet plane
 #pragma unroll
-    for (int k = 0; k < 9; ++k) B[k] = gmem(c.dmi)[k * c.Tpad + ec];
-    const float V = gmem(c.vol)[ec];
+    for (int k = 0; k < 9; ++k) B[k] = ld_at(sgpr_base(c.dmi + (uint64_t)k * c.Tpad), ec4);
+    const float V = ld_at(c.vol, ec4);
     const float4 ua = gload(c.u4 + ga);
     const float4 ub = gload(c.u4 + gb);
     // lanes past the tile's nodes rewrite the last entry with the value it already
@@ -762,8 +780,7 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cas
     __syncthreads();
 
     // one lane per local node sums its run of corner slots in order, four per round
-    // (all reads of a round in flight together). Local nodes are numbered by
-    // descending count, so the lanes of a wave run similar numbers of rounds.
+    // (all reads of a round in flight together)
     for (uint32_t l = threadIdx.x; l < L; l += kTile) {
         const uint32_t q0 = sh_lo[l], q1 = sh_lo[l + 1];
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
