@@ -32,11 +32,6 @@ HBM_PEAK_GBS = 8000.0
 # the element pass 12 (element); CSR offset 4 + X 12 + 1/m 4 + load 12 + state read
 # 48 + state write 48 (node). Sum: 168 T + 140 N (+ 56 T plastic) per step.
 ELEM_B_T, ELEM_B_N, NODE_B_T, NODE_B_N, PLASTIC_B_T = 104, 12, 64, 128, 56
-# Solid-shell hex pass, per element: connectivity 32 + the packed 21x21 stiffness block
-# 231 x 8 + centre gradients and rest offsets 42 x 4 + force-slot write 96; per node the
-# displacement and its compensation 28. Node pass per element: 8 slot reads of 12 + 8
-# gather entries of 4; per node as for tets.
-HEX_ELEM_B_T, HEX_ELEM_B_N, HEX_NODE_B_T = 32 + 231 * 8 + 42 * 4 + 96, 28, 8 * 16
 REFERENCE_SUSTAINED = 470e6  # tet-steps/s, GTX 1070 Ti, 491,520 tets (docs/07-fem-spike-findings.md:89)
 
 
@@ -307,14 +302,6 @@ def hex_cpu_baseline(budget_s=8.0, plastic=False):
                         f"{out['one'][0]:.3e} hex-steps/s over {out['one'][1]} steps"))
 
 
-# Plastic solid-shell pass, per element: connectivity 32 + rest geometry in FP64 42 x 8 (the
-# strain operator is formed from it in the pass) + point history and enhanced parameters
-# read 71 x 8 + enhanced parameters written 7 x 8 + force-slot write 96; per node 28. A point
-# writes its history back only when it yields (elsewhere it is unchanged): those writes are
-# not counted.
-HEX_PL_ELEM_B_T = 32 + 42 * 8 + 71 * 8 + 7 * 8 + 96
-
-
 def bench_hex(args, sf, info, rows, plastic=False):
     """The elastic solid-shell solve on the 1.2M-element hex plate, or (plastic) the plastic
     solve of a 12 x 6 m zone of 12 mm plating punched by a sphere."""
@@ -345,8 +332,11 @@ def bench_hex(args, sf, info, rows, plastic=False):
         raise SystemExit("state went non-finite")
     value = T * args.steps / wall
     elem_ms, node_ms = solver.profile(args.profile_steps)
-    elem_bytes = (HEX_PL_ELEM_B_T if plastic else HEX_ELEM_B_T) * T + HEX_ELEM_B_N * N
-    node_bytes = HEX_NODE_B_T * T + NODE_B_N * N
+    # the bytes each pass must move by the case's layout and state (a plastic point that has
+    # not yielded is read as its eqps alone)
+    tr = solver.traffic()
+    elem_bytes = tr["elem_read"] + tr["elem_write"]
+    node_bytes = tr["node_read"] + tr["node_write"]
     ek = "k_elem_hex_plastic" if plastic else "k_elem_hex"
     if elem_ms >= node_ms:
         dom, dom_ms, dom_bytes = f"element pass ({ek})", elem_ms, elem_bytes

@@ -218,6 +218,13 @@ shipfem_status shipfem_solver_adopt(shipfem_solver* s, const shipfem_handoff* in
                 if (!std::isfinite(val)) return SHIPFEM_INVALID_ARG;
                 hp[q * Tp + t] = val;
             }
+            // the pass reads a point's plastic strain and damage only when its eqps is not +0:
+            // a point handed over with zero eqps and something else not zero keeps eqps as -0
+            for (uint32_t g = 0; g < 8; ++g) {
+                bool any = false;
+                for (uint32_t k = 0; k < 8; ++k) any = any || (k != 6 && hp[(8ull * g + k) * Tp + t] != 0.0);
+                if (any && hp[(8ull * g + 6) * Tp + t] == 0.0) hp[(8ull * g + 6) * Tp + t] = -0.0;
+            }
             if (in->hex_enhanced)
                 for (uint32_t k = 0; k < 7; ++k) {
                     const double val = in->hex_enhanced[x * 7 + k];

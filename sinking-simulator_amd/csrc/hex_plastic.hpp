@@ -193,6 +193,23 @@ __host__ __device__ inline int modeRow(int k) { return k < 2 ? k : k < 5 ? 2 : 3
 constexpr double kShear[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
 constexpr double kCorner[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
 
+// eqps grows with every plastic increment, so a point whose eqps is +0 has never yielded:
+// its plastic strain and damage are zero and are not read. (A history handed over with
+// plastic strain but zero eqps carries -0, which reads.)
+__host__ __device__ inline bool hasHistory(double eqps) { return __builtin_bit_cast(uint64_t, eqps) != 0; }
+
+// Point g's history (plastic strain 6, eqps, damage) at the start of the step.
+__host__ __device__ inline void loadPoint(const double* hist, int g, uint32_t stride, double h0[8]) {
+    h0[6] = hist[(8 * g + 6) * stride];
+    if (hasHistory(h0[6])) {
+        for (int k = 0; k < 8; ++k)
+            if (k != 6) h0[k] = hist[(8 * g + k) * stride];
+    } else {
+        for (int k = 0; k < 8; ++k)
+            if (k != 6) h0[k] = 0.0;
+    }
+}
+
 __host__ __device__ inline double dot3(const double a[3], const double b[3]) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
@@ -331,7 +348,8 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
                                          uint32_t euStride) {
     Result res;
     for (int j = 0; j < 21; ++j) f[j] = 0.0;
-    for (int g = 0; g < 8; ++g) res.failedStart += hist[(8 * g + 7) * stride] >= 1.0;
+    for (int g = 0; g < 8; ++g)
+        res.failedStart += hasHistory(hist[(8 * g + 6) * stride]) && hist[(8 * g + 7) * stride] >= 1.0;
     if (m.damage && res.failedStart == 8) {
         res.failedEnd = 8;
         return res;
@@ -423,7 +441,7 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
         for (int g = 0; g < 8; ++g) {
             double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
             strainAt(g, eas, st);
-            for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
+            loadPoint(hist, g, stride, h0);
             const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
             if (onlyElastic && fl != 0) return false;
             if (fl & 1)  // only a yielding point changes its history
@@ -483,7 +501,7 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
             for (int g = 0; g < 8; ++g) {
                 double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
                 strainAt(g, eas, st);
-                for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
+                loadPoint(hist, g, stride, h0);
                 const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
                 died = died || (fl & 2);
                 linear = linear && fl == 0;

@@ -595,7 +595,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
         float4* out = reinterpret_cast<float4*>(c.force) + (size_t)e * 6;
         int failedStart = 0;
 #pragma unroll
-        for (int g = 0; g < 8; ++g) failedStart += c.hpoint[(8 * g + 7) * tp + e] >= 1.0;
+        for (int g = 0; g < 8; ++g)
+            failedStart += hexp::hasHistory(c.hpoint[(8 * g + 6) * tp + e]) && c.hpoint[(8 * g + 7) * tp + e] >= 1.0;
         if (damage && failedStart == 8) {
             // a torn element carries nothing and is not updated (zone.cpp:944-955)
 #pragma unroll

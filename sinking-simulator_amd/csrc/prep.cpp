@@ -645,8 +645,11 @@ PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded) {
         // history with the enhanced parameters (plastic), and the displacement with its
         // compensation of every node; the 24-float force slots out (and the enhanced
         // parameters, plastic)
+        // (plastic: every point's eqps, and the other seven words of each point that has
+        // yielded, `yielded` counting points)
         b.elem_read = 32 * T + 28 * N + (hc.hfail.empty() ? 0 : 8 * T) +
-                      (plastic ? (8ull * kHexGeo + 8ull * (kHexPoint + 7)) * T : (8ull * kHexPacked + 4ull * kHexGeo) * T);
+                      (plastic ? (8ull * kHexGeo + 8ull * (8 + 7)) * T + 56 * yielded
+                               : (8ull * kHexPacked + 4ull * kHexGeo) * T);
         b.elem_write = 96 * T + (plastic ? 56 * T : 0);
     } else if (variant == kSlots) {
         b.elem_read = 16 * T + 36 * T + 4 * T + 16 * N + history;
@@ -673,6 +676,18 @@ PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded) {
 
 hipError_t yieldedTets(const DeviceCase& dc, const HostCase& hc, hipStream_t s, uint64_t* count) {
     *count = 0;
+    if (hc.corners == 8 && (hc.features & kFeatPlastic) && dc.rec.hpoint && hc.T > 0) {
+        // a hex counts its integration points: eqps of point g is plane 8 g + 6
+        std::vector<uint64_t> q(hc.T);
+        for (int g = 0; g < 8; ++g) {
+            hipError_t e = hipMemcpyAsync(q.data(), dc.rec.hpoint + (8ull * g + 6) * hc.Tpad, 8ull * hc.T,
+                                          hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            for (uint64_t v : q) *count += v != 0u;
+        }
+        return hipSuccess;
+    }
     if (hc.corners != 4 || !(hc.features & kFeatPlastic) || !dc.rec.pstate || hc.T == 0) return hipSuccess;
     std::vector<uint32_t> q(hc.T);
     hipError_t e = hipMemcpyAsync(q.data(), dc.rec.pstate + 6ull * hc.Tpad, 4ull * hc.T, hipMemcpyDeviceToHost, s);

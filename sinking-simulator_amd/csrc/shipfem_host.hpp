@@ -112,8 +112,8 @@ uint64_t arenaBytes(const HostCase& hc);
 struct PassBytes {
     uint64_t elem_read = 0, elem_write = 0, node_read = 0, node_write = 0;
 };
-// The bytes one step's passes must move; `yielded` tets (eqps not +0) also have their
-// plastic strain read.
+// The bytes one step's passes must move; `yielded` tets (or, for a plastic hex case,
+// integration points) with eqps not +0 also have their plastic strain and damage read.
 PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded = 0);
 
 // One case's device memory: a single arena holding every plane of the case.
@@ -125,7 +125,8 @@ struct DeviceCase {
     double cfl = 0, dt = 0, omega = 0;
 };
 
-// Tets of a plastic tet case whose eqps is not +0 (read from the device).
+// Tets of a plastic tet case whose eqps is not +0, or integration points of a plastic hex
+// case whose eqps is not +0 (read from the device).
 hipError_t yieldedTets(const DeviceCase& dc, const HostCase& hc, hipStream_t s, uint64_t* count);
 
 // Lays out `hc` in the arena at `base` (which must hold arenaBytes(hc)) and fills
