@@ -142,7 +142,8 @@ def bench_batch(args, sf, info):
     bad = [i for i in range(len(cases)) if b.readback(i).status]
     if bad:
         raise SystemExit(f"non-finite cases {bad}")
-    step_bytes = sum((168 + 56) * c.T + 140 * c.N for c in cases)
+    tr = b.traffic()
+    step_bytes = tr["elem_read"] + tr["elem_write"] + tr["node_read"] + tr["node_write"]
     emit(dict(metric="tet-element-steps/s", value=rate, unit="tet-steps/s", n_gpus=1,
               steps=args.steps, warmup=args.warmup, ms_per_step=wall * 1e3 / args.steps,
               higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32",
@@ -155,8 +156,8 @@ def bench_batch(args, sf, info):
                           plain_slots_rate=plain_slots_rate,
                           batched_over_plain_slots=rate / plain_slots_rate,
                           device_ms_per_step=dev_ms / args.steps,
-                          step_hbm_gbs_model=step_bytes * args.steps / wall / 1e9,
-                          step_roofline_frac_model=step_bytes * args.steps / wall / 8e12)))
+                          step_bytes=step_bytes, step_hbm_gbs=step_bytes * args.steps / wall / 1e9,
+                          step_hbm_frac=step_bytes * args.steps / wall / 8e12)))
 
 
 def bench_serve(args, sf, info):
@@ -609,7 +610,7 @@ def bench_headline(args, sf, info):
         # the plain path on the same 64 cases: one solver per case, two plain launches per
         # step (the reference's dispatch pattern), the cases stepped one after another
         per = []
-        plain_time = 0.0
+        plain_time = graph_time = 0.0
         for c in cases:
             s = sf.Solver(c)
             s.run(timed_from - W)  # from the state the batched steps were timed from
@@ -619,8 +620,15 @@ def bench_headline(args, sf, info):
             dt_ = time.perf_counter() - t1
             plain_time += dt_
             per.append((c.T, c.T * K / dt_))
+            # the same case with its steps replayed from captured graphs (one submission of
+            # many steps, as the reference records a run into one command buffer)
+            s.run(S * -(-max(W, 1) // S), mode=0)
+            t1 = time.perf_counter()
+            s.run(K, mode=0)
+            graph_time += time.perf_counter() - t1
             s.close()
         plain_rate = tets * K / plain_time
+        plain_graph_rate = tets * K / graph_time
         bins = [(1e4, 3e4), (3e4, 1e5), (1e5, 3.01e5)]
         by_bin = []
         for lo, hi in bins:
@@ -631,7 +639,8 @@ def bench_headline(args, sf, info):
                                    rate=T_ / sum(t / r for t, r in xs),
                                    us_per_step=1e6 * sum(t / r for t, r in xs) / len(xs)))
         cfg.update(plain_rate=plain_rate, batched_over_plain=value / plain_rate,
-                   batched_over_plain_bound=batched_bound / plain_rate, plain_by_size=by_bin)
+                   batched_over_plain_bound=batched_bound / plain_rate, plain_by_size=by_bin,
+                   plain_graph_rate=plain_graph_rate, batched_over_plain_graph=value / plain_graph_rate)
     if "plate" in legs:
         case, desc = workload("plate")
         s = sf.Solver(case)
@@ -778,21 +787,22 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
 
     elem_ms, node_ms = solver.profile(args.profile_steps)
-    plastic = bool(case.plastic)
-    elem_bytes = (ELEM_B_T + (PLASTIC_B_T if plastic else 0)) * T + ELEM_B_N * N
-    node_bytes = NODE_B_T * T + NODE_B_N * N
+    # the bytes each pass must move by the case's layout and state; the survey's per-step
+    # model beside it, never as it
+    tr = solver.traffic()
+    elem_bytes = tr["elem_read"] + tr["elem_write"]
+    node_bytes = tr["node_read"] + tr["node_write"]
+    me, mn = model_bytes(case)
     ek = "k_elem_tiled" if args.variant == "tiled" else "k_elem"
+    # committed counter profiles exist for the tiled variant of some workloads only
+    prof = args.workload if args.variant == "tiled" else None
     if elem_ms >= node_ms:
-        dom, dom_ms, dom_bytes = f"element pass ({ek})", elem_ms, elem_bytes
+        roofline = roofline_entry(f"element pass ({ek})", elem_ms, elem_bytes, me,
+                                  counters_for(prof, "element") if prof else None)
     else:
-        dom, dom_ms, dom_bytes = "node pass (k_node)", node_ms, node_bytes
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles(args.workload)
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=(traffic or {}).get(dom.split()[0]), kernel=dom,
-                    kernel_ms=round(dom_ms, 5), algorithmic_bytes=dom_bytes,
-                    elem_ms=round(elem_ms, 5), node_ms=round(node_ms, 5))
+        roofline = roofline_entry("node pass (k_node)", node_ms, node_bytes, mn,
+                                  counters_for(prof, "node") if prof else None)
+    roofline.update(elem_ms=round(elem_ms, 5), node_ms=round(node_ms, 5))
     step_bytes = elem_bytes + node_bytes
     step_gbs = step_bytes * args.steps / wall / 1e9
     copy_probe = None
