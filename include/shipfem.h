@@ -268,6 +268,10 @@ typedef struct shipfem_launch_config {
                          * by element count) on two streams, each replaying its own captured
                          * chunks, so one half's kernels fill the other's launch gaps and
                          * tails (each case's arithmetic unchanged); -1: one stream */
+    int32_t rest_from_positions; /* tiled variant: 1 forms each tet's Dm^-1 and V in the pass from
+                         * its tile's rest positions (16 B per tile node read instead of 40 B
+                         * per tet, ~40 more FP32 operations a tet, results within ~1e-6 of the
+                         * stored planes'); 0 reads the stored planes */
 } shipfem_launch_config;
 
 /* ---- one case (replaces gpu::FemGpuSolver, fem_gpu.hpp:17-45) -------------- */

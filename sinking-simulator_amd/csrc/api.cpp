@@ -89,6 +89,8 @@ bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out) {
         sh.variant = cfg->variant == SHIPFEM_VARIANT_SLOTS ? kSlots : kTiled;
         if (cfg->split != 0 && cfg->split != -1) return false;
         sh.split = cfg->split == 0 ? 1 : 0;
+        if (cfg->rest_from_positions != 0 && cfg->rest_from_positions != 1) return false;
+        sh.formed = cfg->rest_from_positions;
     }
     if (!blockOk(sh.elem_block) || !blockOk(sh.node_block)) return false;
     out = sh;
@@ -490,7 +492,7 @@ shipfem_status shipfem_solver_traffic(const shipfem_solver* s, uint64_t out[4]) 
     uint64_t yielded = 0;
     const hipError_t e = yieldedTets(s->dc, s->hc, s->stream, &yielded);
     if (e != hipSuccess) return statusOf(e);
-    addBytes(compulsoryBytes(s->hc, s->plan.shape.variant, yielded), out);
+    addBytes(compulsoryBytes(s->hc, s->plan.shape.variant, yielded, s->plan.shape.formed != 0), out);
     return SHIPFEM_OK;
 }
 
@@ -597,6 +599,7 @@ shipfem_status createBatch(const shipfem_case_desc* cases, uint32_t n,
         std::vector<uint32_t>().swap(hc.conn);
         std::vector<uint32_t>().swap(hc.lconn);
         std::vector<uint32_t>().swap(hc.tnode);
+        std::vector<float>().swap(hc.trest);
         std::vector<uint32_t>().swap(hc.pell);
         std::vector<double>().swap(hc.hk);
         std::vector<float>().swap(hc.hgeo);
@@ -663,7 +666,7 @@ shipfem_status shipfem_batch_traffic(const shipfem_batch* b, uint64_t out[4]) {
         uint64_t yielded = 0;
         const hipError_t e = yieldedTets(b->dcs[i], b->hcs[i], b->stream, &yielded);
         if (e != hipSuccess) return statusOf(e);
-        addBytes(compulsoryBytes(b->hcs[i], b->plan.shape.variant, yielded), out);
+        addBytes(compulsoryBytes(b->hcs[i], b->plan.shape.variant, yielded, b->plan.shape.formed != 0), out);
     }
     return SHIPFEM_OK;
 }
@@ -758,6 +761,7 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
         best->chunk = b->S;
         best->variant = cur.variant == kSlots ? SHIPFEM_VARIANT_SLOTS : SHIPFEM_VARIANT_TILED;
         best->split = cur.split ? 0 : -1;
+        best->rest_from_positions = cur.formed;
     }
     if (best_ms) *best_ms = curMs;
     return SHIPFEM_OK;

@@ -144,7 +144,8 @@ void Plan::launchElem(int par, int sample, hipStream_t s, const KernelTiming* t)
     const KernelTiming* p1 = t ? &t1 : nullptr;
     const KernelTiming* p2 = t ? &t2 : nullptr;
     if (n1 > 0)
-        launch_elem_tiled(d_recs, d_elem, n1, lmax, par, sample, shape.xcd_remap, feat_tiled, s, p1);
+        launch_elem_tiled(d_recs, d_elem, n1, lmax, par, sample, shape.xcd_remap, feat_tiled, shape.formed != 0, s,
+                          p1);
     if (n2 == 0) return;
     if (hex && (feat_slots & kFeatPlastic))
         launch_elem_hex_plastic(d_recs, d_elem + n1, n2, shape.elem_block, par, sample, shape.xcd_remap, s, p2);

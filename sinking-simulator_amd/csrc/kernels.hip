@@ -693,23 +693,26 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 // lane writes each corner force straight into its slot and a node lane sums one
 // contiguous run: the tile needs no separate node->corner map.
 // LDS: corner-slot forces [3][4 kTile] floats (component planes), the run offsets
-// [lmax + 1] (16-bit), then the tile's local nodes' displacement float4[lmax]. Dm^-1 and
-// V stream as planes. Every global load a lane needs is issued before the first
-// barrier, so the staging gather and the streamed planes share one latency.
+// [lmax + 1] (16-bit), then the tile's local nodes' displacement float4[lmax] and rest
+// position about the tile's centroid float4[lmax]; each tet forms its Dm^-1 and V from
+// those. Every global load a lane needs is issued before the first barrier, so the
+// staging gathers share one latency.
 constexpr uint32_t kSlots4 = 4 * kTile;  // corner slots of a tile
 __host__ __device__ constexpr size_t tiled_lds_offsets_offset() { return 3 * kSlots4 * sizeof(float); }
 __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
     return tiled_lds_offsets_offset() + ((lmax + 1) * sizeof(uint16_t) + 15) / 16 * 16;
 }
 
-// The element pass of tile `b` (one 256-thread block).
-template <bool PLASTIC, bool DAMAGE>
+// The element pass of tile `b` (one 256-thread block). FORMED: each tet forms its Dm^-1 and
+// V from the tile's rest positions staged in LDS; else it reads the stored planes.
+template <bool PLASTIC, bool DAMAGE, bool FORMED>
 __device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cases,
                                                  const Tile* __restrict__ tiles, int b, uint32_t lmax,
                                                  int parity, int sample, unsigned char* smem) {
     float* sh_f = reinterpret_cast<float*>(smem);
     uint16_t* sh_lo = reinterpret_cast<uint16_t*>(smem + tiled_lds_offsets_offset());
     float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
+    float4* sh_x = sh_u + lmax;  // the local nodes' rest positions about the tile's centroid
 
     const Tile tile = tiles[b];
     const CaseDev& c = cases[tile.case_index];
@@ -737,21 +740,35 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cas
     const uint32_t ob = lo[min(lb, L)];
     const uint32_t ec = min(e, c.T - 1);
     const uint2 lc = gload(c.lconn + ec);
-    float B[9];
-    const uint32_t ec4 = 4u * ec;  // byte offset into a per-tet plane
+    float B[9], V = 0.0f;
+    if (!FORMED) {
+        const uint32_t ec4 = 4u * ec;  // byte offset into a per-tet plane
 #pragma unroll
-    for (int k = 0; k < 9; ++k) B[k] = ld_at(sgpr_base(c.dmi + (uint64_t)k * c.Tpad), ec4);
-    const float V = ld_at(c.vol, ec4);
+        for (int k = 0; k < 9; ++k) B[k] = ld_at(sgpr_base(c.dmi + (uint64_t)k * c.Tpad), ec4);
+        V = ld_at(c.vol, ec4);
+    }
+    float4 xa, xb;
+    if (FORMED) {
+        xa = gload(c.trest + p0 + min(la, L - 1));
+        xb = gload(c.trest + p0 + min(lb, L - 1));
+    }
     const float4 ua = gload(c.u4 + ga);
     const float4 ub = gload(c.u4 + gb);
     // lanes past the tile's nodes rewrite the last entry with the value it already
     // holds, so no store waits behind a branch
     sh_lo[min(la, L)] = static_cast<uint16_t>(oa);
     sh_lo[min(lb, L)] = static_cast<uint16_t>(ob);
+    if (FORMED) {
+        sh_x[min(la, L - 1)] = xa;
+        sh_x[min(lb, L - 1)] = xb;
+    }
     sh_u[min(la, L - 1)] = ua;
     sh_u[min(lb, L - 1)] = ub;
     for (uint32_t l = threadIdx.x + 2 * kTile; l <= L; l += kTile) sh_lo[l] = lo[l];
-    for (uint32_t l = threadIdx.x + 2 * kTile; l < L; l += kTile) sh_u[l] = c.u4[c.tnode[p0 + l]];
+    for (uint32_t l = threadIdx.x + 2 * kTile; l < L; l += kTile) {
+        if (FORMED) sh_x[l] = c.trest[p0 + l];
+        sh_u[l] = c.u4[c.tnode[p0 + l]];
+    }
     __syncthreads();
 
     double diss = 0.0, se = 0.0, fract = 0.0;
@@ -762,6 +779,30 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cas
         for (int q = 0; q < 4; ++q) {
             ln[q] = w4[q] & 1023u;
             slot[q] = sh_lo[ln[q]] + (w4[q] >> 10);
+        }
+        // Dm^-1 and V from the rest positions: Dm's column k is corner k+1 minus corner 0;
+        // the inverse by cofactors, a degenerate tet (|det| <= 1e-30) taking zero
+        // (fem.cpp:42-56)
+        if (FORMED) {
+            const float4 x0 = sh_x[ln[0]], x1 = sh_x[ln[1]], x2 = sh_x[ln[2]], x3 = sh_x[ln[3]];
+            const float D[3][3] = {{x1.x - x0.x, x2.x - x0.x, x3.x - x0.x},
+                                   {x1.y - x0.y, x2.y - x0.y, x3.y - x0.y},
+                                   {x1.z - x0.z, x2.z - x0.z, x3.z - x0.z}};
+            const float a00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+            const float a01 = D[0][2] * D[2][1] - D[0][1] * D[2][2];
+            const float a02 = D[0][1] * D[1][2] - D[0][2] * D[1][1];
+            const float det = D[0][0] * a00 + D[1][0] * a01 + D[2][0] * a02;
+            const float id = fabsf(det) > 1e-30f ? __builtin_amdgcn_rcpf(det) : 0.0f;
+            B[0] = a00 * id;
+            B[1] = a01 * id;
+            B[2] = a02 * id;
+            B[3] = (D[1][2] * D[2][0] - D[1][0] * D[2][2]) * id;
+            B[4] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
+            B[5] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
+            B[6] = (D[1][0] * D[2][1] - D[1][1] * D[2][0]) * id;
+            B[7] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * id;
+            B[8] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
+            V = fabsf(det) * (1.0f / 6.0f);
         }
         float f[12];
         bool tore;
@@ -805,14 +846,14 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cas
     }
 }
 
-template <bool PLASTIC, bool DAMAGE>
+template <bool PLASTIC, bool DAMAGE, bool FORMED>
 __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict__ cases,
                                                       const Tile* __restrict__ tiles, int ntiles,
                                                       uint32_t lmax, int parity, int sample,
                                                       int xcd) {
     extern __shared__ __align__(16) unsigned char smem[];
-    tiled_elem_block<PLASTIC, DAMAGE>(cases, tiles, remap_block(blockIdx.x, gridDim.x, xcd), lmax, parity,
-                                      sample, smem);
+    tiled_elem_block<PLASTIC, DAMAGE, FORMED>(cases, tiles, remap_block(blockIdx.x, gridDim.x, xcd), lmax,
+                                              parity, sample, smem);
 }
 
 // ---------------------------------------------------------------- node pass
@@ -1290,18 +1331,21 @@ void launch_elem_hex_plastic(const CaseDev* cases, const Tile* tiles, int ntiles
 }
 
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
-                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream,
+                       int parity, int sample, int xcd, uint32_t feat_union, bool formed, hipStream_t stream,
                        const KernelTiming* t) {
-    const size_t lds = tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax;
-    if (feat_union & kFeatDamage)
-        SHIPFEM_LAUNCH(t, (k_elem_tiled<true, true>), dim3(ntiles), dim3(kTile), lds, stream, cases,
-                       tiles, ntiles, lmax, parity, sample, xcd);
-    else if (feat_union & kFeatPlastic)
-        SHIPFEM_LAUNCH(t, (k_elem_tiled<true, false>), dim3(ntiles), dim3(kTile), lds, stream, cases,
-                       tiles, ntiles, lmax, parity, sample, xcd);
-    else
-        SHIPFEM_LAUNCH(t, (k_elem_tiled<false, false>), dim3(ntiles), dim3(kTile), lds, stream,
-                       cases, tiles, ntiles, lmax, parity, sample, xcd);
+    // the local nodes' displacements, and with `formed` their rest positions, after the slots
+    const size_t lds = tiled_lds_nodes_offset(lmax) + (formed ? 32 : 16) * (size_t)lmax;
+#define SHIPFEM_TILED(P, D, F)                                                                         \
+    SHIPFEM_LAUNCH(t, (k_elem_tiled<P, D, F>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles, ntiles, \
+                   lmax, parity, sample, xcd)
+    if (feat_union & kFeatDamage) {
+        if (formed) SHIPFEM_TILED(true, true, true); else SHIPFEM_TILED(true, true, false);
+    } else if (feat_union & kFeatPlastic) {
+        if (formed) SHIPFEM_TILED(true, false, true); else SHIPFEM_TILED(true, false, false);
+    } else {
+        if (formed) SHIPFEM_TILED(false, false, true); else SHIPFEM_TILED(false, false, false);
+    }
+#undef SHIPFEM_TILED
 }
 
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream) {

@@ -35,7 +35,7 @@ struct Layout {
 // Offsets of every array in a case arena; shared by arenaBytes and uploadCase.
 struct ArenaMap {
     uint64_t conn, dmi, vol, force, hk, hgeo, hgeod, hpoint, henh, hfail, pstate, tear_step, tear_stress, u4, cu, v, cv, invm, mass, nflags, load, xc, presc,
-        ell_off, ell, lconn, tn_off, tnode, lofs, partial, pell_off, pell, acc_e, acc_n,
+        ell_off, ell, lconn, tn_off, tnode, trest, lofs, partial, pell_off, pell, acc_e, acc_n,
         smp_e, smp_n, omega_w, step, status, hist_count, hist, total;
 };
 
@@ -73,6 +73,7 @@ ArenaMap mapArena(const HostCase& hc) {
     m.lconn = L.take(4ull * hc.lconn.size());
     m.tn_off = L.take(4 * (uint64_t)hc.tn_off.size());
     m.tnode = L.take(4 * (uint64_t)hc.tnode.size());
+    m.trest = L.take(4 * (uint64_t)hc.trest.size());
     m.lofs = L.take(2 * (uint64_t)hc.lofs.size());
     m.partial = L.take(16 * (uint64_t)hc.P);
     m.pell_off = L.take(4 * (uint64_t)hc.pell_off.size());
@@ -467,6 +468,7 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
         hc.lconn.assign(2ull * T, 0u);
         hc.tn_off.assign(nt + 1, 0u);
         hc.tnode.clear();
+        hc.trest.clear();
         hc.lofs.clear();
         std::vector<uint32_t> stamp(N, 0xFFFFFFFFu), loc(N, 0u), deg(N, 0u), seen(N, 0u);
         std::vector<uint32_t> nodes;
@@ -498,6 +500,7 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
                 std::vector<uint32_t>().swap(hc.lconn);
                 std::vector<uint32_t>().swap(hc.tn_off);
                 std::vector<uint32_t>().swap(hc.tnode);
+                std::vector<float>().swap(hc.trest);
                 std::vector<uint16_t>().swap(hc.lofs);
                 break;
             }
@@ -505,6 +508,16 @@ std::string prepareInternal(const shipfem_case_desc& d, HostCase& hc, int32_t hi
             hc.tn_off[b + 1] = hc.tn_off[b] + L;
             hc.lmax = std::max(hc.lmax, L);
             hc.tnode.insert(hc.tnode.end(), nodes.begin(), nodes.end());
+            // the tile's rest positions about its centroid: FP32 keeps ~1e-7 of a tile's
+            // extent, so edge vectors formed from them keep ~1e-6 of an edge
+            double ctr[3] = {0.0, 0.0, 0.0};
+            for (uint32_t g : nodes)
+                for (int a = 0; a < 3; ++a) ctr[a] += d.rest_position[3ull * g + a];
+            for (int a = 0; a < 3; ++a) ctr[a] /= static_cast<double>(L);
+            for (uint32_t g : nodes) {
+                for (int a = 0; a < 3; ++a) hc.trest.push_back(static_cast<float>(d.rest_position[3ull * g + a] - ctr[a]));
+                hc.trest.push_back(0.0f);
+            }
             cnt.assign(L + 1, 0);
             for (uint32_t l = 0; l < L; ++l) cnt[l + 1] = static_cast<uint16_t>(cnt[l] + deg[nodes[l]]);
             hc.lofs.insert(hc.lofs.end(), cnt.begin(), cnt.end());
@@ -628,7 +641,7 @@ std::string prepareHexCase(const shipfem_case_desc& d, HostCase& hc, int32_t his
 
 uint64_t arenaBytes(const HostCase& hc) { return mapArena(hc).total; }
 
-PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded) {
+PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded, bool formed) {
     PassBytes b;
     if (hc.elem_tiles == 0) variant = kSlots;  // a case without tiles always runs slots
     const uint64_t T = hc.T, N = hc.N, Np = hc.Npad, P = hc.P, nt = hc.elem_tiles;
@@ -655,9 +668,10 @@ PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded) {
         b.elem_read = 16 * T + 36 * T + 4 * T + 16 * N + history;
         b.elem_write = 48 * T;
     } else {
-        // tile offsets, each tile's node ids and run offsets, 16-bit local connectivity,
-        // Dm^-1 and V, every node's displacement once; one float4 partial per (tile, node)
-        b.elem_read = 4 * (nt + 1) + 4 * P + 2 * (P + nt) + 8 * T + 36 * T + 4 * T + 16 * N + history;
+        // tile offsets, each tile's node ids and run offsets, 16-bit local connectivity, Dm^-1
+        // and V (or, formed in the pass, each tile node's rest position), every node's
+        // displacement once; one float4 partial per (tile, node)
+        b.elem_read = 4 * (nt + 1) + 4 * P + 2 * (P + nt) + 8 * T + (formed ? 16 * P : 40 * T) + 16 * N + history;
         b.elem_write = 16 * P;
     }
     const bool slots = hc.corners == 8 || variant == kSlots;
@@ -741,6 +755,7 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.lconn = reinterpret_cast<const uint2*>(base + m.lconn);
     r.tn_off = reinterpret_cast<const uint32_t*>(base + m.tn_off);
     r.tnode = reinterpret_cast<const uint32_t*>(base + m.tnode);
+    r.trest = reinterpret_cast<const float4*>(base + m.trest);
     r.lofs = reinterpret_cast<const uint16_t*>(base + m.lofs);
     r.partial = reinterpret_cast<float4*>(base + m.partial);
     r.pell_off = reinterpret_cast<const uint32_t*>(base + m.pell_off);
@@ -809,6 +824,7 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     up(m.lconn, hc.lconn.data(), 4ull * hc.lconn.size());
     up(m.tn_off, hc.tn_off.data(), 4ull * hc.tn_off.size());
     up(m.tnode, hc.tnode.data(), 4ull * hc.tnode.size());
+    up(m.trest, hc.trest.data(), 4ull * hc.trest.size());
     up(m.lofs, hc.lofs.data(), 2ull * hc.lofs.size());
     up(m.pell_off, hc.pell_off.data(), 4ull * hc.pell_off.size());
     up(m.pell, hc.pell.data(), 4ull * hc.pell.size());

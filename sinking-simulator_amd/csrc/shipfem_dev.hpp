@@ -98,6 +98,7 @@ struct CaseDev {
                               // corner's place in that node's run of corner slots
     const uint32_t* tn_off;   // [elem_tiles + 1] first (tile, local node) slot of each tile
     const uint32_t* tnode;    // [P] global node id of each (tile, local node) slot
+    const float4* trest;      // [P] its rest position relative to the tile's centroid
     const uint16_t* lofs;     // [P + elem_tiles] per tile: L+1 run offsets into its
                               // 4 kTile corner slots
     float4* partial;          // [P] force sum of each (tile, local node)
@@ -154,7 +155,8 @@ struct LaunchShape {
     int node_block = 256;
     int xcd_remap = 1;
     int variant = kTiled;
-    int split = 1;  // a batch: two halves on two streams
+    int split = 1;   // a batch: two halves on two streams
+    int formed = 0;  // tiled pass: form Dm^-1 and V from the tile's rest positions
 };
 
 // Start/stop events recorded by the kernel dispatch itself (hipExtLaunchKernel), so
@@ -179,7 +181,7 @@ void launch_elem_hex_plastic(const CaseDev* cases, const Tile* tiles, int ntiles
                              int parity, int sample, int xcd, hipStream_t stream,
                              const KernelTiming* timing = nullptr);
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
-                       int parity, int sample, int xcd, uint32_t feat_union, hipStream_t stream,
+                       int parity, int sample, int xcd, uint32_t feat_union, bool formed, hipStream_t stream,
                        const KernelTiming* timing = nullptr);
 void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
                  int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream,

@@ -58,6 +58,8 @@ struct HostCase {
     std::vector<uint32_t> tn_off;   // [elem_tiles + 1]
     std::vector<uint32_t> tnode;    // [P]
     std::vector<uint16_t> lofs;     // [P + elem_tiles]
+    std::vector<float> trest;       // [4P] each (tile, local node)'s rest position relative to
+                                    // the tile's centroid (FP32 of the FP64 difference), w = 0
     std::vector<uint32_t> pell_off; // [node_waves + 1]
     std::vector<uint32_t> pell;
     std::vector<float> u4;          // [4N]
@@ -114,7 +116,8 @@ struct PassBytes {
 };
 // The bytes one step's passes must move; `yielded` tets (or, for a plastic hex case,
 // integration points) with eqps not +0 also have their plastic strain and damage read.
-PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded = 0);
+// `formed`: the tiled pass forms Dm^-1 and V from the tiles' rest positions.
+PassBytes compulsoryBytes(const HostCase& hc, int variant, uint64_t yielded = 0, bool formed = false);
 
 // One case's device memory: a single arena holding every plane of the case.
 struct DeviceCase {

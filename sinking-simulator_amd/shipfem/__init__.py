@@ -85,7 +85,8 @@ VARIANT_TILED, VARIANT_SLOTS = 0, 1
 
 class LaunchConfig(C.Structure):
     _fields_ = [("elem_block", C.c_int32), ("node_block", C.c_int32), ("xcd_remap", C.c_int32),
-                ("chunk", C.c_int32), ("variant", C.c_int32), ("split", C.c_int32)]
+                ("chunk", C.c_int32), ("variant", C.c_int32), ("split", C.c_int32),
+                ("rest_from_positions", C.c_int32)]
 
 
 ROUNDS_ALL, ROUNDS_SHORTEST = 0, 1
@@ -760,8 +761,9 @@ class Solver:
     def dt(self):
         return self.timestep()["dt"]
 
-    def set_launch(self, variant=VARIANT_TILED, node_block=256, elem_block=256, xcd_remap=1):
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant)
+    def set_launch(self, variant=VARIANT_TILED, node_block=256, elem_block=256, xcd_remap=1,
+                   rest_from_positions=0):
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant, 0, rest_from_positions)
         _check(lib().shipfem_solver_set_launch(self.h, C.byref(cfg)), "set_launch")
         return self
 
@@ -885,10 +887,10 @@ class Batch:
     halves on two streams side by side; split=-1: one stream."""
 
     def __init__(self, cases, elem_block=256, node_block=256, xcd_remap=1, chunk=0,
-                 variant=VARIANT_TILED, split=0):
+                 variant=VARIANT_TILED, split=0, rest_from_positions=0):
         self.cases = list(cases)
         self._descs = (CaseDesc * len(self.cases))(*[c.desc() for c in self.cases])
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk, variant, split)
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, chunk, variant, split, rest_from_positions)
         err = C.create_string_buffer(512)
         h = C.c_void_p()
         kinds = {c.corners for c in self.cases}
@@ -939,10 +941,12 @@ class Batch:
         _check(lib().shipfem_batch_tune(self.h, steps_per_trial, C.byref(cfg), C.byref(ms)))
         return dict(elem_block=cfg.elem_block, node_block=cfg.node_block,
                     xcd_remap=cfg.xcd_remap, chunk=cfg.chunk, variant=cfg.variant, split=cfg.split,
+                    rest_from_positions=cfg.rest_from_positions,
                     ms_per_step=ms.value)
 
-    def set_config(self, elem_block, node_block, xcd_remap, variant=VARIANT_TILED, split=0):
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant, split)
+    def set_config(self, elem_block, node_block, xcd_remap, variant=VARIANT_TILED, split=0,
+                   rest_from_positions=0):
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant, split, rest_from_positions)
         _check(lib().shipfem_batch_set_config(self.h, C.byref(cfg)))
 
     def reset(self):

@@ -416,3 +416,54 @@ def test_invalid_input_is_refused_with_a_reason():
     s = sf.Solver(sf.Case(X, idx))
     with pytest.raises(sf.ShipfemError):
         s.set_launch(variant=7)
+
+
+def test_dm_inverse_formed_in_the_pass_matches_oracle():
+    # rest_from_positions: each tet forms Dm^-1 and V from its tile's rest positions (FP32
+    # about the tile's centroid) instead of reading the stored FP64-formed planes. The
+    # jittered plate, the tearing indentation (torn set, steps) and the single tet, against
+    # the oracle at the same tolerances as the stored planes; a batch of such cases equals
+    # each case alone bit for bit.
+    plate = configs.plate(lx=1.0, ly=0.4, nx=35, ny=14, ramp=2e-4)
+    s = sf.Solver(plate).set_launch(rest_from_positions=1)
+    stored = sf.Solver(plate)
+    o = oracle_for(plate, s.dt)
+    ora = o.run(2000, sample_every=plate.sample_every)
+    s.run(2000)
+    stored.run(2000)
+    err = rel_l2(s.readback().u, o.u)
+    vs_stored = rel_l2(s.readback().u, stored.readback().u)
+    worst = check_energy(s.energy(), ora)
+    say(f"formed Dm^-1, jittered plate 2000 steps: u rel L2 {err:.3e} (vs stored planes {vs_stored:.2e}), "
+        f"energy {worst:.2e}")
+    assert err <= REL_U and worst <= ENERGY
+
+    case = small_panel(speed=10.0, failure=dict(failure_strain=0.08))
+    t = sf.Solver(case).set_launch(rest_from_positions=1)
+    ot = oracle_for(case, t.dt)
+    ot.run(1200)
+    t.run(1200)
+    st = t.readback()
+    torn_o = set(np.nonzero(ot.damage >= 1.0)[0].tolist())
+    torn_g = set(np.nonzero(st.damage >= 1.0)[0].tolist())
+    err_u = rel_l2(st.u, ot.u)
+    say(f"formed Dm^-1, tearing: torn {len(torn_g)} / {len(torn_o)}, u rel L2 {err_u:.2e}")
+    assert len(torn_o) > 10 and len(torn_o ^ torn_g) <= max(2, len(torn_o) // 50)
+    assert err_u <= REL_U
+
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.1
+    one = sf.Case(X, np.array([[0, 1, 2, 3]], np.uint32), flags=np.array([7, 7, 7, 0], np.uint32),
+                  load=np.array([[0, 0, 0]] * 3 + [[0, 0, 1e3]], float), damping=0.98)
+    u1 = sf.Solver(one).set_launch(rest_from_positions=1)
+    o1 = oracle_for(one, u1.dt)
+    u1.run(3000)
+    o1.run(3000)
+    assert rel_l2(u1.readback().u, o1.u) <= REL_U
+
+    cases = [configs.plate(lx=0.5, ly=0.2, nx=17, ny=7, ramp=2e-4), plate]
+    b = sf.Batch(cases, rest_from_positions=1)
+    b.run(400)
+    for i, c in enumerate(cases):
+        a = sf.Solver(c).set_launch(rest_from_positions=1)
+        a.run(400)
+        assert np.array_equal(a.readback().u, b.readback(i).u), i
