@@ -272,6 +272,10 @@ typedef struct shipfem_launch_config {
                          * its tile's rest positions (16 B per tile node read instead of 40 B
                          * per tet, ~40 more FP32 operations a tet, results within ~1e-6 of the
                          * stored planes'); 0 reads the stored planes */
+    int32_t resident;   /* 0: a plan whose tet cases each fit one block (at most 1024 tets in
+                         * four tiles, 2048 nodes) runs each step's passes inside one
+                         * 1024-thread block per case, a whole chunk per launch (same bits);
+                         * -1: two launches per step */
 } shipfem_launch_config;
 
 /* ---- one case (replaces gpu::FemGpuSolver, fem_gpu.hpp:17-45) -------------- */

@@ -91,6 +91,8 @@ bool shapeFrom(const shipfem_launch_config* cfg, LaunchShape& out) {
         sh.split = cfg->split == 0 ? 1 : 0;
         if (cfg->rest_from_positions != 0 && cfg->rest_from_positions != 1) return false;
         sh.formed = cfg->rest_from_positions;
+        if (cfg->resident != 0 && cfg->resident != -1) return false;
+        sh.resident = cfg->resident == 0 ? 1 : 0;
     }
     if (!blockOk(sh.elem_block) || !blockOk(sh.node_block)) return false;
     out = sh;
@@ -762,6 +764,7 @@ shipfem_status shipfem_batch_tune(shipfem_batch* b, int32_t steps_per_trial,
         best->variant = cur.variant == kSlots ? SHIPFEM_VARIANT_SLOTS : SHIPFEM_VARIANT_TILED;
         best->split = cur.split ? 0 : -1;
         best->rest_from_positions = cur.formed;
+        best->resident = cur.resident ? 0 : -1;
     }
     if (best_ms) *best_ms = curMs;
     return SHIPFEM_OK;

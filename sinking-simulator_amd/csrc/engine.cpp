@@ -66,6 +66,16 @@ hipError_t Plan::build(hipStream_t s) {
     }
     tilesFor(recs, 256, false, omega_tiles);
     refreshFeatures();
+    // a plan of small tiled cases steps whole chunks in one block per case (k_resident)
+    resident = shape.resident && !hex && !recs.empty() && elem_split == elem_tiles.size() &&
+               node_split == node_tiles.size();
+    for (const CaseDev& r : recs) resident = resident && r.elem_tiles > 0 && r.elem_tiles <= 4 && r.N <= 2048;
+    if (resident) {
+        int dev = 0, maxLds = 0;
+        resident = hipGetDevice(&dev) == hipSuccess &&
+                   hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+                   resident_lds(lmax, shape.formed != 0) <= static_cast<size_t>(maxLds);
+    }
     hipError_t e = hipSuccess;
     size_t need = std::max({elem_tiles.size(), node_tiles.size(), omega_tiles.size()});
     if (need > tile_cap || !d_elem) {
@@ -169,8 +179,13 @@ void Plan::launchNode(int par, int sample, hipStream_t s, const KernelTiming* t)
 }
 
 void Plan::stepPlain(bool sample, hipStream_t s) {
-    launchStep(parity, sample ? 1 : 0, s);
-    if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), parity, s);
+    if (resident) {
+        launch_resident(d_recs, static_cast<int>(recs.size()), lmax, 1, parity, sample ? 1 : 0, feat_tiled,
+                         shape.formed != 0, s);
+    } else {
+        launchStep(parity, sample ? 1 : 0, s);
+        if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), parity, s);
+    }
     parity ^= 1;
 }
 
@@ -179,12 +194,16 @@ hipError_t Plan::capture(int steps, hipStream_t s) {
     if (steps < 2 || (steps & 1)) return hipErrorInvalidValue;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    int par = 0;
-    for (int k = 0; k < steps; ++k) {
-        const int sample = k == 0 ? 1 : 0;
-        launchStep(par, sample, s);
-        if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), par, s);
-        par ^= 1;
+    if (resident) {
+        launch_resident(d_recs, static_cast<int>(recs.size()), lmax, steps, 0, 1, feat_tiled, shape.formed != 0, s);
+    } else {
+        int par = 0;
+        for (int k = 0; k < steps; ++k) {
+            const int sample = k == 0 ? 1 : 0;
+            launchStep(par, sample, s);
+            if (sample) launch_reduce(d_recs, static_cast<int>(recs.size()), par, s);
+            par ^= 1;
+        }
     }
     e = hipStreamEndCapture(s, &graph_src);
     if (e != hipSuccess) return e;

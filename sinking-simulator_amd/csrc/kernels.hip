@@ -703,71 +703,74 @@ __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
     return tiled_lds_offsets_offset() + ((lmax + 1) * sizeof(uint16_t) + 15) / 16 * 16;
 }
 
-// The element pass of tile `b` (one 256-thread block). FORMED: each tet forms its Dm^-1 and
-// V from the tile's rest positions staged in LDS; else it reads the stored planes.
+// The element pass of the 256-tet tile from `first`, at the case's step `n`. FORMED: each
+// tet forms its Dm^-1 and V from the tile's rest positions staged in LDS; else it reads the
+// stored planes. Its 256 threads (lane `lid`) all reach its two barriers; with `active`
+// false (no tile for them) they do nothing else.
 template <bool PLASTIC, bool DAMAGE, bool FORMED>
-__device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cases,
-                                                 const Tile* __restrict__ tiles, int b, uint32_t lmax,
-                                                 int parity, int sample, unsigned char* smem) {
+__device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t first, uint32_t lid, bool active,
+                                                 int64_t n, uint32_t lmax, int sample, unsigned char* smem) {
     float* sh_f = reinterpret_cast<float*>(smem);
     uint16_t* sh_lo = reinterpret_cast<uint16_t*>(smem + tiled_lds_offsets_offset());
     float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
     float4* sh_x = sh_u + lmax;  // the local nodes' rest positions about the tile's centroid
 
-    const Tile tile = tiles[b];
-    const CaseDev& c = cases[tile.case_index];
-    const uint32_t tb = tile.first / kTile;
-    const int64_t n = ld_uniform(c.step + parity);
-    const uint32_t p0 = ld_uniform(c.tn_off + tb);
-    const uint32_t L = ld_uniform(c.tn_off + tb + 1) - p0;
-    if (n >= c.step_end) return;  // the case has finished: whole block idles
-    const uint32_t e = tile.first + threadIdx.x;
-    const bool live = e < c.T;
+    const uint32_t tb = first / kTile;
+    uint32_t p0 = 0, L = 1;
+    if (active) {
+        p0 = ld_uniform(c.tn_off + tb);
+        L = ld_uniform(c.tn_off + tb + 1) - p0;
+    }
+    const uint32_t e = first + lid;
+    const bool live = active && e < c.T;
 
     // Loads in the order their chains need them: the staging gather's node ids head
     // the longest chain (ids, then displacements), then the run offsets, then the
     // streamed per-tet planes, so waiting for the ids never waits for the stream.
     // Indices are clamped into range instead of branched on, so every load is issued
     // unconditionally; values a lane does not own are discarded. A lane stages local
-    // nodes threadIdx.x and threadIdx.x + kTile; tiles with more (not seen in
-    // Kuhn-type meshes) stage the rest afterwards.
-    const uint32_t la = threadIdx.x, lb = threadIdx.x + kTile;
-    const auto* lo = gmem(c.lofs + p0 + tb);
-    const auto* tn = gmem(c.tnode + p0);
-    const uint32_t ga = tn[min(la, L - 1)];
-    const uint32_t gb = tn[min(lb, L - 1)];
-    const uint32_t oa = lo[min(la, L)];
-    const uint32_t ob = lo[min(lb, L)];
-    const uint32_t ec = min(e, c.T - 1);
-    const uint2 lc = gload(c.lconn + ec);
-    float B[9], V = 0.0f;
-    if (!FORMED) {
-        const uint32_t ec4 = 4u * ec;  // byte offset into a per-tet plane
+    // nodes lid and lid + kTile; tiles with more (not seen in Kuhn-type meshes) stage the
+    // rest afterwards.
+    uint2 lc = make_uint2(0u, 0u);
+    float B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, V = 0.0f;
+    if (active) {
+        const uint32_t la = lid, lb = lid + kTile;
+        const auto* lo = gmem(c.lofs + p0 + tb);
+        const auto* tn = gmem(c.tnode + p0);
+        const uint32_t ga = tn[min(la, L - 1)];
+        const uint32_t gb = tn[min(lb, L - 1)];
+        const uint32_t oa = lo[min(la, L)];
+        const uint32_t ob = lo[min(lb, L)];
+        const uint32_t ec = min(e, c.T - 1);
+        lc = gload(c.lconn + ec);
+        if (!FORMED) {
+            const uint32_t ec4 = 4u * ec;  // byte offset into a per-tet plane
 #pragma unroll
-        for (int k = 0; k < 9; ++k) B[k] = ld_at(sgpr_base(c.dmi + (uint64_t)k * c.Tpad), ec4);
-        V = ld_at(c.vol, ec4);
-    }
-    float4 xa, xb;
-    if (FORMED) {
-        xa = gload(c.trest + p0 + min(la, L - 1));
-        xb = gload(c.trest + p0 + min(lb, L - 1));
-    }
-    const float4 ua = gload(c.u4 + ga);
-    const float4 ub = gload(c.u4 + gb);
-    // lanes past the tile's nodes rewrite the last entry with the value it already
-    // holds, so no store waits behind a branch
-    sh_lo[min(la, L)] = static_cast<uint16_t>(oa);
-    sh_lo[min(lb, L)] = static_cast<uint16_t>(ob);
-    if (FORMED) {
-        sh_x[min(la, L - 1)] = xa;
-        sh_x[min(lb, L - 1)] = xb;
-    }
-    sh_u[min(la, L - 1)] = ua;
-    sh_u[min(lb, L - 1)] = ub;
-    for (uint32_t l = threadIdx.x + 2 * kTile; l <= L; l += kTile) sh_lo[l] = lo[l];
-    for (uint32_t l = threadIdx.x + 2 * kTile; l < L; l += kTile) {
-        if (FORMED) sh_x[l] = c.trest[p0 + l];
-        sh_u[l] = c.u4[c.tnode[p0 + l]];
+            for (int k = 0; k < 9; ++k) B[k] = ld_at(sgpr_base(c.dmi + (uint64_t)k * c.Tpad), ec4);
+            V = ld_at(c.vol, ec4);
+        }
+        float4 xa, xb;
+        if (FORMED) {
+            xa = gload(c.trest + p0 + min(la, L - 1));
+            xb = gload(c.trest + p0 + min(lb, L - 1));
+        }
+        const float4 ua = gload(c.u4 + ga);
+        const float4 ub = gload(c.u4 + gb);
+        // lanes past the tile's nodes rewrite the last entry with the value it already
+        // holds, so no store waits behind a branch
+        sh_lo[min(la, L)] = static_cast<uint16_t>(oa);
+        sh_lo[min(lb, L)] = static_cast<uint16_t>(ob);
+        if (FORMED) {
+            sh_x[min(la, L - 1)] = xa;
+            sh_x[min(lb, L - 1)] = xb;
+        }
+        sh_u[min(la, L - 1)] = ua;
+        sh_u[min(lb, L - 1)] = ub;
+        for (uint32_t l = lid + 2 * kTile; l <= L; l += kTile) sh_lo[l] = lo[l];
+        for (uint32_t l = lid + 2 * kTile; l < L; l += kTile) {
+            if (FORMED) sh_x[l] = c.trest[p0 + l];
+            sh_u[l] = c.u4[c.tnode[p0 + l]];
+        }
     }
     __syncthreads();
 
@@ -818,12 +821,12 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev* __restrict__ cas
 #pragma unroll
             for (int a = 0; a < 3; ++a) sh_f[a * kSlots4 + slot[q]] = f[q * 3 + a];
     }
-    element_wave_sums<PLASTIC, DAMAGE>(c, e, sample, diss, se, fract);
+    if (active) element_wave_sums<PLASTIC, DAMAGE>(c, e, sample, diss, se, fract);
     __syncthreads();
 
     // one lane per local node sums its run of corner slots in order, four per round
     // (all reads of a round in flight together)
-    for (uint32_t l = threadIdx.x; l < L; l += kTile) {
+    for (uint32_t l = lid; active && l < L; l += kTile) {
         const uint32_t q0 = sh_lo[l], q1 = sh_lo[l + 1];
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (uint32_t q = q0; q < q1; q += 4) {
@@ -852,22 +855,23 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
                                                       uint32_t lmax, int parity, int sample,
                                                       int xcd) {
     extern __shared__ __align__(16) unsigned char smem[];
-    tiled_elem_block<PLASTIC, DAMAGE, FORMED>(cases, tiles, remap_block(blockIdx.x, gridDim.x, xcd), lmax,
-                                              parity, sample, smem);
+    const Tile tile = tiles[remap_block(blockIdx.x, gridDim.x, xcd)];
+    const CaseDev& c = cases[tile.case_index];
+    const int64_t n = ld_uniform(c.step + parity);
+    if (n >= c.step_end) return;  // the case has finished: whole block idles
+    tiled_elem_block<PLASTIC, DAMAGE, FORMED>(c, tile.first, threadIdx.x, true, n, lmax, sample, smem);
 }
 
 // ---------------------------------------------------------------- node pass
-// The node pass of tile `b` (one block of as many threads as the tile has nodes).
+// The node pass of the nodes from `first` (a multiple of 64), lane `lid` taking node
+// first + lid, at step `n`.
 template <bool EXTRA, int VARIANT>
-__device__ __forceinline__ void node_block(const CaseDev* __restrict__ cases, const Tile* __restrict__ tiles,
-                                           int b, int parity, int sample) {
-    const Tile tile = tiles[b];
-    const CaseDev& c = cases[tile.case_index];
-    const uint32_t i = tile.first + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63u;
+__device__ __forceinline__ void node_block(const CaseDev& c, uint32_t first, uint32_t lid, int64_t n, int parity,
+                                           int sample) {
+    const uint32_t i = first + lid;
+    const uint32_t lane = lid & 63u;
     // tiles start on multiples of 64 nodes, so the wave index is wave-uniform
     const uint32_t w = __builtin_amdgcn_readfirstlane(i >> 6);
-    const int64_t n = ld_uniform(c.step + parity);
     // this wave's gather slab (fixed order: slots in CSR (tet, corner) order, tiled in
     // tile order). A block may overhang the case's last node: waves past its
     // node_waves slabs have no offsets (off holds node_waves + 1 entries) and gather
@@ -880,7 +884,7 @@ __device__ __forceinline__ void node_block(const CaseDev* __restrict__ cases, co
         width = (ld_uniform(off + w + 1) - base) >> 6;
     }
     const bool frozen = n >= c.step_end;
-    if (tile.first == 0 && threadIdx.x == 0) c.step[parity ^ 1] = frozen ? n : n + 1;
+    if (first == 0 && lid == 0) c.step[parity ^ 1] = frozen ? n : n + 1;
     if (frozen) return;
 
     const bool live = i < c.N;
@@ -1110,7 +1114,9 @@ template <int BLOCK, bool EXTRA, int VARIANT>
 __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ cases,
                                                 const Tile* __restrict__ tiles, int ntiles,
                                                 int parity, int sample, int xcd) {
-    node_block<EXTRA, VARIANT>(cases, tiles, remap_block(blockIdx.x, gridDim.x, xcd), parity, sample);
+    const Tile tile = tiles[remap_block(blockIdx.x, gridDim.x, xcd)];
+    const CaseDev& c = cases[tile.case_index];
+    node_block<EXTRA, VARIANT>(c, tile.first, threadIdx.x, ld_uniform(c.step + parity), parity, sample);
 }
 
 // ---------------------------------------------------------------- FP64 reductions
@@ -1122,24 +1128,22 @@ constexpr int kReduceBlock = 256;
 constexpr int kRecordBlock = 1024;
 constexpr int kQ = kAccElemSlots + kSmpElemSlots + kAccNodeSlots + kSmpNodeSlots;
 
-__global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restrict__ cases,
-                                                          int parity) {
-    __shared__ double sh[kRecordBlock];
-    const CaseDev& c = cases[blockIdx.x];
-    const int64_t n0 = c.step[parity];
-    if (c.step[parity ^ 1] != n0 + 1) return;  // the case did not step: no record
+// The energy record of the case's step n0 from its per-wave slots, by its 1024 threads
+// (lane `tid`) through `sh` (1024 doubles): every thread folds its strided share, then each
+// quantity goes through the same fixed tree. Ends with a barrier, so `sh` may be reused.
+__device__ __forceinline__ void reduce_record(const CaseDev& c, int64_t n0, uint32_t tid, double* sh) {
     const uint32_t ew = c.elem_waves, nw = c.node_waves;
     // q: element accumulators, the element sample, node accumulators, node samples
     constexpr int QS = kAccElemSlots, QN = QS + kSmpElemSlots, QM = QN + kAccNodeSlots;
     double q[kQ];
 #pragma unroll
     for (int k = 0; k < kQ; ++k) q[k] = 0.0;
-    for (uint32_t w = threadIdx.x; w < ew; w += kRecordBlock) {
+    for (uint32_t w = tid; w < ew; w += kRecordBlock) {
 #pragma unroll
         for (int k = 0; k < kAccElemSlots; ++k) q[k] += c.acc_e[k * ew + w];
         q[QS] += c.smp_e[kSmpStrain * ew + w];
     }
-    for (uint32_t w = threadIdx.x; w < nw; w += kRecordBlock) {
+    for (uint32_t w = tid; w < nw; w += kRecordBlock) {
 #pragma unroll
         for (int k = 0; k < kAccNodeSlots; ++k) q[QN + k] += c.acc_n[k * nw + w];
 #pragma unroll
@@ -1147,16 +1151,16 @@ __global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restri
     }
     double tot[kQ];
     for (int k = 0; k < kQ; ++k) {
-        sh[threadIdx.x] = q[k];
+        sh[tid] = q[k];
         __syncthreads();
         for (int s = kRecordBlock / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            if ((int)tid < s) sh[tid] += sh[tid + s];
             __syncthreads();
         }
         tot[k] = sh[0];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         const int32_t h = *c.hist_count;
         if (h < c.hist_cap) {
             const double* sn = tot + QM;
@@ -1178,6 +1182,49 @@ __global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restri
         }
     }
 }
+
+__global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restrict__ cases,
+                                                          int parity) {
+    __shared__ double sh[kRecordBlock];
+    const CaseDev& c = cases[blockIdx.x];
+    const int64_t n0 = c.step[parity];
+    if (c.step[parity ^ 1] != n0 + 1) return;  // the case did not step: no record
+    reduce_record(c, n0, threadIdx.x, sh);
+}
+
+// ---------------------------------------------------------------- a small case resident
+// One 1024-thread block per case of at most four tiles (1024 tets) and 2048 nodes runs
+// `steps` whole steps in one launch: the element pass of tile g by threads [256 g, 256 g +
+// 256) (each group its own LDS stage), a barrier, the node pass of 256-node tiles g, g + 4
+// by the same groups, a barrier; on the sampled first step the energy record. Each step
+// runs the separate passes' own code in their order, so a case ends with the bits the
+// two-launch step gives it. The step counter is carried in a register: within the launch
+// it is never read back from memory the launch has written.
+constexpr int kResidentBlock = 4 * kTile;
+
+template <bool PLASTIC, bool DAMAGE, bool FORMED, bool EXTRA>
+__global__ __launch_bounds__(kResidentBlock) void k_resident(const CaseDev* __restrict__ cases, uint32_t lmax,
+                                                            uint32_t tile_lds, int steps, int parity0,
+                                                            int sample_first) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const CaseDev& c = cases[blockIdx.x];
+    const uint32_t tid = threadIdx.x, g = tid / kTile, lid = tid % kTile;
+    const uint32_t tiles = c.elem_tiles, node_tiles = (c.N + kTile - 1) / kTile;
+    int64_t n = ld_uniform(c.step + parity0);  // written by earlier launches only
+    for (int k = 0; k < steps; ++k) {
+        if (n >= c.step_end) break;  // the case has finished (uniform)
+        const int parity = parity0 ^ (k & 1);
+        const int sample = k == 0 ? sample_first : 0;
+        tiled_elem_block<PLASTIC, DAMAGE, FORMED>(c, g * kTile, lid, g < tiles, n, lmax, sample,
+                                                  smem + (size_t)g * tile_lds);
+        __syncthreads();  // every tile's partials written
+        for (uint32_t t = g; t < node_tiles; t += 4) node_block<EXTRA, kTiled>(c, t * kTile, lid, n, parity, sample);
+        __syncthreads();  // u, v, the sums and the step counter written
+        if (sample) reduce_record(c, n, tid, reinterpret_cast<double*>(smem));
+        ++n;
+    }
+}
+
 
 // ---------------------------------------------------------------- FP64 stable step
 // The element bound of omega.hpp on the stored Dm^-1 planes.
@@ -1346,6 +1393,35 @@ void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint
         if (formed) SHIPFEM_TILED(false, false, true); else SHIPFEM_TILED(false, false, false);
     }
 #undef SHIPFEM_TILED
+}
+
+size_t resident_lds(uint32_t lmax, bool formed) {
+    const size_t tile = tiled_lds_nodes_offset(lmax) + (formed ? 32 : 16) * (size_t)lmax;
+    return std::max<size_t>(4 * ((tile + 15) / 16 * 16), 8 * kRecordBlock);
+}
+
+void launch_resident(const CaseDev* cases, int ncases, uint32_t lmax, int steps, int parity0, int sample_first,
+                     uint32_t feat_union, bool formed, hipStream_t stream) {
+    const uint32_t tile = (uint32_t)((tiled_lds_nodes_offset(lmax) + (formed ? 32 : 16) * (size_t)lmax + 15) / 16 * 16);
+    const size_t lds = resident_lds(lmax, formed);
+    const bool extra = (feat_union & (kFeatContact | kFeatDrive | kFeatDamp)) != 0;
+#define SHIPFEM_RES(P, D, F, X)                                                                           \
+    hipLaunchKernelGGL((k_resident<P, D, F, X>), dim3(ncases), dim3(kResidentBlock), lds, stream, cases, lmax, \
+                       tile, steps, parity0, sample_first)
+#define SHIPFEM_RES_X(P, D, F) \
+    if (extra) SHIPFEM_RES(P, D, F, true); else SHIPFEM_RES(P, D, F, false)
+#define SHIPFEM_RES_F(P, D) \
+    if (formed) { SHIPFEM_RES_X(P, D, true); } else { SHIPFEM_RES_X(P, D, false); }
+    if (feat_union & kFeatDamage) {
+        SHIPFEM_RES_F(true, true);
+    } else if (feat_union & kFeatPlastic) {
+        SHIPFEM_RES_F(true, false);
+    } else {
+        SHIPFEM_RES_F(false, false);
+    }
+#undef SHIPFEM_RES_F
+#undef SHIPFEM_RES_X
+#undef SHIPFEM_RES
 }
 
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream) {

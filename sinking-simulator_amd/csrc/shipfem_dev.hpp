@@ -156,7 +156,8 @@ struct LaunchShape {
     int xcd_remap = 1;
     int variant = kTiled;
     int split = 1;   // a batch: two halves on two streams
-    int formed = 0;  // tiled pass: form Dm^-1 and V from the tile's rest positions
+    int formed = 0;    // tiled pass: form Dm^-1 and V from the tile's rest positions
+    int resident = 1;  // plans of small tiled cases step whole chunks in one launch
 };
 
 // Start/stop events recorded by the kernel dispatch itself (hipExtLaunchKernel), so
@@ -187,6 +188,12 @@ void launch_node(const CaseDev* cases, const Tile* tiles, int ntiles, int block,
                  int sample, int xcd, uint32_t feat_union, int variant, hipStream_t stream,
                  const KernelTiming* timing = nullptr);
 void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t stream);
+// Small cases (at most 4 tiles and 2048 nodes each), one 1024-thread block per case: `steps`
+// whole steps in one launch, the first sampled when sample_first (k_resident).
+void launch_resident(const CaseDev* cases, int ncases, uint32_t lmax, int steps, int parity0, int sample_first,
+                     uint32_t feat_union, bool formed, hipStream_t stream);
+// Dynamic LDS bytes that launch takes.
+size_t resident_lds(uint32_t lmax, bool formed);
 // FP64 largest eigenvalue of B^T C B per element (the lumped-mass element frequency
 // is omega^2 = 4 lambda / rho), reduced to a per-wave max, then to one value per case.
 void launch_omega(const CaseDev* cases, const Tile* tiles, int ntiles, hipStream_t stream);

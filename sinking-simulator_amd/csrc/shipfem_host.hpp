@@ -157,6 +157,7 @@ struct Plan {
     uint32_t feat_union = 0;
     uint32_t lmax = 0;  // largest tile node count over the plan's cases (tiled LDS stage)
     bool hex = false;   // the plan's cases are solid-shell hexes (all or none)
+    bool resident = false;  // every case small enough for one block: whole chunks in one launch
     int parity = 0;
     int chunk = 0;
     hipGraphExec_t graph = nullptr;

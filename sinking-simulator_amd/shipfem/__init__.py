@@ -86,7 +86,7 @@ VARIANT_TILED, VARIANT_SLOTS = 0, 1
 class LaunchConfig(C.Structure):
     _fields_ = [("elem_block", C.c_int32), ("node_block", C.c_int32), ("xcd_remap", C.c_int32),
                 ("chunk", C.c_int32), ("variant", C.c_int32), ("split", C.c_int32),
-                ("rest_from_positions", C.c_int32)]
+                ("rest_from_positions", C.c_int32), ("resident", C.c_int32)]
 
 
 ROUNDS_ALL, ROUNDS_SHORTEST = 0, 1
@@ -762,8 +762,8 @@ class Solver:
         return self.timestep()["dt"]
 
     def set_launch(self, variant=VARIANT_TILED, node_block=256, elem_block=256, xcd_remap=1,
-                   rest_from_positions=0):
-        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant, 0, rest_from_positions)
+                   rest_from_positions=0, resident=0):
+        cfg = LaunchConfig(elem_block, node_block, xcd_remap, 0, variant, 0, rest_from_positions, resident)
         _check(lib().shipfem_solver_set_launch(self.h, C.byref(cfg)), "set_launch")
         return self
 

@@ -467,3 +467,43 @@ def test_dm_inverse_formed_in_the_pass_matches_oracle():
         a = sf.Solver(c).set_launch(rest_from_positions=1)
         a.run(400)
         assert np.array_equal(a.readback().u, b.readback(i).u), i
+
+
+def test_small_cases_resident_in_one_block_keep_every_bit():
+    # a case of at most four tiles and 2048 nodes runs whole chunks inside one block
+    # (k_resident): the same passes' code in the same order, so the same bits as two
+    # launches a step, plain steps and graph chunks alike, alone and in a batch of such
+    # cases, elastic and with J2 and tearing; and the oracle still agrees
+    elastic = configs.cantilever(nx=20, ny=4, nz=2, total_load=2000.0, ramp=2e-4)  # 960 tets
+    torn = configs.cantilever(nx=16, ny=4, nz=2, total_load=4e5, ramp=1e-4, plastic=True,
+                              yield_=235e6, hardening=1e9, failure=dict(failure_strain=0.02))  # 768 tets
+    for case in (elastic, torn):
+        assert case.T <= 1024 and case.N <= 2048
+        runs = {}
+        for res in (0, -1):
+            s = sf.Solver(case).set_launch(resident=res)
+            s.run(350, mode=0)
+            s.run(37, mode=1)
+            s.run(213, mode=0)
+            st = s.readback()
+            runs[res] = (st.u, st.v, st.eqps, st.damage, s.energy())
+        a, b = runs[0], runs[-1]
+        for k in range(4):
+            assert np.array_equal(a[k], b[k]), (case.T, k)
+        assert a[4] == b[4]
+        say(f"resident {case.T} tets, 600 steps: same bits as two launches a step; eqps max "
+            f"{a[2].max():.3e}, torn {int((a[3] >= 1).sum())}")
+        if case is torn:
+            assert a[2].max() > 0
+        else:
+            o = oracle_for(case, sf.Solver(case).dt)
+            o.run(600)
+            err = rel_l2(a[0], o.u)
+            say(f"resident cantilever vs oracle: u rel L2 {err:.2e}")
+            assert err <= REL_U
+    b = sf.Batch([elastic, torn, elastic], chunk=100)
+    b.run(600)
+    for i, c in enumerate((elastic, torn, elastic)):
+        s = sf.Solver(c).set_launch(resident=-1)
+        s.run(600)
+        assert np.array_equal(s.readback().u, b.readback(i).u), i
