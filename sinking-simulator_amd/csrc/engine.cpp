@@ -153,9 +153,10 @@ void Plan::launchElem(int par, int sample, hipStream_t s, const KernelTiming* t)
     splitTiming(t, n1 > 0 && n2 > 0, t1, t2);
     const KernelTiming* p1 = t ? &t1 : nullptr;
     const KernelTiming* p2 = t ? &t2 : nullptr;
+    // a plan of one case passes no tile table: tile b is item b * block of case 0
     if (n1 > 0)
-        launch_elem_tiled(d_recs, d_elem, n1, lmax, par, sample, shape.xcd_remap, feat_tiled, shape.formed != 0, s,
-                          p1);
+        launch_elem_tiled(d_recs, recs.size() == 1 ? nullptr : d_elem, n1, lmax, par, sample, shape.xcd_remap,
+                          feat_tiled, shape.formed != 0, s, p1);
     if (n2 == 0) return;
     if (hex && (feat_slots & kFeatPlastic))
         launch_elem_hex_plastic(d_recs, d_elem + n1, n2, shape.elem_block, par, sample, shape.xcd_remap, s, p2);
@@ -170,12 +171,13 @@ void Plan::launchNode(int par, int sample, hipStream_t s, const KernelTiming* t)
     const int n1 = static_cast<int>(node_split), n2 = static_cast<int>(node_tiles.size() - node_split);
     KernelTiming t1, t2;
     splitTiming(t, n1 > 0 && n2 > 0, t1, t2);
+    const bool one = recs.size() == 1;
     if (n1 > 0)
-        launch_node(d_recs, d_node, n1, shape.node_block, par, sample, shape.xcd_remap, feat_tiled, kTiled, s,
-                    t ? &t1 : nullptr);
+        launch_node(d_recs, one ? nullptr : d_node, n1, shape.node_block, par, sample, shape.xcd_remap, feat_tiled,
+                    kTiled, s, t ? &t1 : nullptr);
     if (n2 > 0)
-        launch_node(d_recs, d_node + n1, n2, shape.node_block, par, sample, shape.xcd_remap, feat_slots, kSlots,
-                    s, t ? &t2 : nullptr);
+        launch_node(d_recs, one ? nullptr : d_node + n1, n2, shape.node_block, par, sample, shape.xcd_remap,
+                    feat_slots, kSlots, s, t ? &t2 : nullptr);
 }
 
 void Plan::stepPlain(bool sample, hipStream_t s) {

@@ -45,6 +45,14 @@ __device__ __forceinline__ int remap_block(int b, int nb, int xcd) {
     return x * q + (x < r ? x : r) + j;
 }
 
+// This block's tile: from the table, or with no table (a plan of one case) tile b of case 0
+// starting at item b * block, which spares the kernel one dependent load before its first.
+__device__ __forceinline__ Tile tile_of(const Tile* tiles, int block, int xcd) {
+    const int b = remap_block(blockIdx.x, gridDim.x, xcd);
+    if (!tiles) return Tile{0, static_cast<uint32_t>(b) * static_cast<uint32_t>(block)};
+    return tiles[b];
+}
+
 // A load the wave issues on the scalar path (through the constant cache). Only for
 // wave-uniform addresses of data that no lane writes during the kernel; the cache is
 // invalidated at every kernel start, so values written by earlier launches are seen.
@@ -706,10 +714,13 @@ __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
 // The element pass of the 256-tet tile from `first`, at the case's step `n`. FORMED: each
 // tet forms its Dm^-1 and V from the tile's rest positions staged in LDS; else it reads the
 // stored planes. Its 256 threads (lane `lid`) all reach its two barriers; with `active`
-// false (no tile for them) they do nothing else.
+// false (no tile for them) they do nothing else. With `nptr` the step is read from there
+// instead, once the staging loads are in flight, and a finished case (step_end reached)
+// returns before its first store.
 template <bool PLASTIC, bool DAMAGE, bool FORMED>
 __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t first, uint32_t lid, bool active,
-                                                 int64_t n, uint32_t lmax, int sample, unsigned char* smem) {
+                                                 int64_t n, uint32_t lmax, int sample, unsigned char* smem,
+                                                 const int64_t* nptr = nullptr) {
     float* sh_f = reinterpret_cast<float*>(smem);
     uint16_t* sh_lo = reinterpret_cast<uint16_t*>(smem + tiled_lds_offsets_offset());
     float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
@@ -756,6 +767,10 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
         }
         const float4 ua = gload(c.u4 + ga);
         const float4 ub = gload(c.u4 + gb);
+        if (nptr) {
+            n = ld_uniform(nptr);
+            if (n >= c.step_end) return;  // the case has finished: the whole block idles
+        }
         // lanes past the tile's nodes rewrite the last entry with the value it already
         // holds, so no store waits behind a branch
         sh_lo[min(la, L)] = static_cast<uint16_t>(oa);
@@ -855,11 +870,10 @@ __global__ __launch_bounds__(kTile) void k_elem_tiled(const CaseDev* __restrict_
                                                       uint32_t lmax, int parity, int sample,
                                                       int xcd) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const Tile tile = tiles[remap_block(blockIdx.x, gridDim.x, xcd)];
+    const Tile tile = tile_of(tiles, kTile, xcd);
     const CaseDev& c = cases[tile.case_index];
-    const int64_t n = ld_uniform(c.step + parity);
-    if (n >= c.step_end) return;  // the case has finished: whole block idles
-    tiled_elem_block<PLASTIC, DAMAGE, FORMED>(c, tile.first, threadIdx.x, true, n, lmax, sample, smem);
+    tiled_elem_block<PLASTIC, DAMAGE, FORMED>(c, tile.first, threadIdx.x, true, 0, lmax, sample, smem,
+                                              c.step + parity);
 }
 
 // ---------------------------------------------------------------- node pass
@@ -883,10 +897,6 @@ __device__ __forceinline__ void node_block(const CaseDev& c, uint32_t first, uin
         base = ld_uniform(off + w);
         width = (ld_uniform(off + w + 1) - base) >> 6;
     }
-    const bool frozen = n >= c.step_end;
-    if (first == 0 && lid == 0) c.step[parity ^ 1] = frozen ? n : n + 1;
-    if (frozen) return;
-
     const bool live = i < c.N;
     const double dt = c.dt;
     const float dtf = c.dtf;
@@ -930,6 +940,10 @@ __device__ __forceinline__ void node_block(const CaseDev& c, uint32_t first, uin
         ld[a] = gmem(lplane)[a * np + ii];
     }
     const uint32_t fl = live ? flr : kPinAll;
+    // the step counter is needed only now: the loads above went out without waiting for it
+    const bool frozen = n >= c.step_end;
+    if (first == 0 && lid == 0) c.step[parity ^ 1] = frozen ? n : n + 1;
+    if (frozen) return;
 
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     auto gather = [&](uint32_t q0) {
@@ -1114,7 +1128,7 @@ template <int BLOCK, bool EXTRA, int VARIANT>
 __global__ __launch_bounds__(BLOCK) void k_node(const CaseDev* __restrict__ cases,
                                                 const Tile* __restrict__ tiles, int ntiles,
                                                 int parity, int sample, int xcd) {
-    const Tile tile = tiles[remap_block(blockIdx.x, gridDim.x, xcd)];
+    const Tile tile = tile_of(tiles, BLOCK, xcd);
     const CaseDev& c = cases[tile.case_index];
     node_block<EXTRA, VARIANT>(c, tile.first, threadIdx.x, ld_uniform(c.step + parity), parity, sample);
 }

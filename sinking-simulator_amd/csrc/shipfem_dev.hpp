@@ -169,7 +169,8 @@ struct KernelTiming {
 // Kernel launchers (kernels.hip). All launch asynchronously on `stream`; a non-null
 // `timing` times the launch (never used under graph capture).
 // For the tiled variant the tiles are 256-tet tiles, `block` is ignored and `lmax` sizes
-// the LDS node stage.
+// the LDS node stage. launch_elem_tiled and launch_node take a null `tiles` for a plan of
+// one case: block b then covers items [b * block, (b + 1) * block) of cases[0].
 void launch_elem(const CaseDev* cases, const Tile* tiles, int ntiles, int block, int parity,
                  int sample, int xcd, uint32_t feat_union, hipStream_t stream,
                  const KernelTiming* timing = nullptr);
