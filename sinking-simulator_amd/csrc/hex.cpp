@@ -673,9 +673,9 @@ shipfem_status shipfem_hex_plastic_update(const double rest[24], const double cu
         }
     const double mu = youngs_modulus / (2.0 * (1.0 + poisson_ratio));
     const double lam = youngs_modulus * poisson_ratio / ((1.0 + poisson_ratio) * (1.0 - 2.0 * poisson_ratio));
-    const shipfem::hexp::Material m{mu, lam + (2.0 / 3.0) * mu, yield_strength, hardening,
-                                    failure_strain, sensitivity, reference_triaxiality,
-                                    cutoff_triaxiality, failure_strain > 0.0};
+    const shipfem::hexp::Material m = shipfem::hexp::derive(
+        {mu, lam + (2.0 / 3.0) * mu, yield_strength, hardening, failure_strain, sensitivity,
+         reference_triaxiality, cutoff_triaxiality, failure_strain > 0.0});
     double cX[21], cd[21], fi[21], eu[48];
     shipfem::hexp::basis(X, cX);
     shipfem::hexp::basis(d, cd);

@@ -35,7 +35,44 @@ struct Material {
     double mu, kappa, yield, hard;
     double fail, sens, eref, ecut;  // failure strain (<= 0: none) and Rice-Tracey constants
     bool damage;
+    // constants of the return, formed once (derive()): 1 / (2 mu + 2/3 H),
+    // 1 / (1 + H / (3 mu)), 1 / (2 mu), 1 / (3 kappa), 1 / mu
+    double igam, thb0, i2mu, i3k, imu;
 };
+
+__host__ __device__ inline Material derive(Material m) {
+    m.igam = 1.0 / (2.0 * m.mu + (2.0 / 3.0) * m.hard);
+    m.thb0 = 1.0 / (1.0 + m.hard / (3.0 * m.mu));
+    m.i2mu = 1.0 / (2.0 * m.mu);
+    m.i3k = 1.0 / (3.0 * m.kappa);
+    m.imu = 1.0 / m.mu;
+    return m;
+}
+
+// 1/x and 1/sqrt(x) for the well-scaled arguments of this update (Jacobian determinants,
+// Cholesky pivots, stress norms). On the device: the hardware estimate and two Newton
+// steps, each of which at least squares the relative error, so the result is within an
+// ulp or so of the quotient without the IEEE division's scaling and fix-up sequence. On
+// the host: the plain quotient.
+__host__ __device__ inline double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+#else
+    return 1.0 / x;
+#endif
+}
+__host__ __device__ inline double invSqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(x);
+    // y <- y + y (1 - x y^2) / 2
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
 
 struct Result {
     double dissipation = 0.0;  // sum w sigma : d eps_p
@@ -71,11 +108,19 @@ __host__ __device__ inline void basis(const double* v, double c[21]) {
         }
 }
 
+// Coefficients held with a stride (a lane's column of a block-shared array on the device).
+struct Strided {
+    const double* p;
+    uint32_t s;
+    __host__ __device__ double operator[](int j) const { return p[j * s]; }
+};
+
 // Column l of the natural derivative at (xi, eta, zeta): out[i] = d v_i / d xi_l.
 //   l = 0: c_0 + eta c_3 + zeta c_5 + eta zeta c_6
 //   l = 1: c_1 + xi c_3 + zeta c_4 + xi zeta c_6
 //   l = 2: c_2 + eta c_4 + xi c_5 + xi eta c_6
-__host__ __device__ inline void column(const double* c, int l, double xi, double eta, double zeta,
+template <class C>
+__host__ __device__ inline void column(const C& c, int l, double xi, double eta, double zeta,
                                        double out[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -89,7 +134,8 @@ __host__ __device__ inline void column(const double* c, int l, double xi, double
 }
 
 // J[i][k] = d v_i / d xi_k at a point.
-__host__ __device__ inline void jacobian(const double* c, double xi, double eta, double zeta,
+template <class C>
+__host__ __device__ inline void jacobian(const C& c, double xi, double eta, double zeta,
                                          double J[3][3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -124,12 +170,12 @@ __host__ __device__ inline void addGrad(double F[21], const double P[3], int l, 
     }
 }
 
-__host__ __device__ inline double inverse3(const double m[3][3], double r[3][3]) {
+__host__ __device__ inline double inverse3(const double m[3][3], double r[3][3], double* inv = nullptr) {
     const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
     const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
     const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
     const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
-    const double id = 1.0 / det;
+    const double id = recip(det);
     r[0][0] = c00 * id;
     r[1][0] = c01 * id;
     r[2][0] = c02 * id;
@@ -139,6 +185,7 @@ __host__ __device__ inline double inverse3(const double m[3][3], double r[3][3])
     r[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
     r[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
     r[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+    if (inv) *inv = id;
     return det;
 }
 
@@ -235,7 +282,8 @@ __host__ __device__ inline void covariantStrain(const double J[3][3], const doub
 }
 
 // The samples of one zeta level for the displacement with coefficients cd.
-__host__ __device__ inline void samples(const double* cX, const double* cd, double zeta, double smp[8]) {
+template <class CX>
+__host__ __device__ inline void samples(const CX& cX, const double* cd, double zeta, double smp[8]) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const double xi = kShear[s][0], eta = kShear[s][1];
@@ -277,7 +325,7 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
     const double p = kappa * vol;
     double tr[6];
     for (int i = 0; i < 3; ++i) {
-        tr[i] = 2.0 * mu * (el[i] - vol / 3.0);
+        tr[i] = 2.0 * mu * (el[i] - vol * (1.0 / 3.0));
         tr[3 + i] = mu * el[3 + i];
     }
     const double n2 = tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2] +
@@ -285,7 +333,11 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
     const double root23 = 0.81649658092772603273;
     const double radius = root23 * (m.yield + m.hard * fmax(h0[6], 0.0));
     // inside the surface by the squared norms first: an elastic point takes no root
-    const double nrm = n2 > radius * radius ? sqrt(n2) : 0.0;
+    double nrm = 0.0, inrm = 0.0;
+    if (n2 > radius * radius) {
+        inrm = invSqrt(n2);
+        nrm = n2 * inrm;
+    }
     const double excess = nrm - radius;
     if (!(excess > 0.0)) {
         for (int i = 0; i < 3; ++i) {
@@ -294,8 +346,7 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
         }
         return 0;
     }
-    const double gamma = excess / (2.0 * mu + (2.0 / 3.0) * m.hard);
-    const double inrm = 1.0 / nrm;
+    const double gamma = excess * m.igam;
     double dep[6];
     for (int i = 0; i < 6; ++i) n[i] = tr[i] * inrm;
     for (int i = 0; i < 3; ++i) {
@@ -312,11 +363,11 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
     diss = dd;
     const double deq = root23 * gamma;
     h[6] = h0[6] + deq;
-    theta = 1.0 - 2.0 * mu * gamma / nrm;
-    thetabar = 1.0 / (1.0 + m.hard / (3.0 * mu)) - (1.0 - theta);
+    theta = 1.0 - 2.0 * mu * gamma * inrm;
+    thetabar = m.thb0 - (1.0 - theta);
     if (m.damage && m.fail > 0.0) {
         const double q = 1.22474487139158904909 * (nrm - 2.0 * mu * gamma);
-        const double eta = q > 0.0 ? p / q : 0.0;
+        const double eta = q > 0.0 ? p * recip(q) : 0.0;
         if (eta > m.ecut) {
             const double critical = m.fail * exp(-m.sens * (eta - m.eref));
             if (critical > 0.0 && critical <= 1.7976931348623157e308)  // positive and finite
@@ -324,10 +375,10 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
         }
         if (h[7] >= 1.0) {
             h[7] = 1.0;
-            const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
+            const double pm = (sig[0] + sig[1] + sig[2]) * (1.0 / 3.0);
             double u = 0.0;
-            for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * mu) + pm / (3.0 * kappa));
-            for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / mu);
+            for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) * m.i2mu + pm * m.i3k);
+            for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] * m.imu);
             released = u;
             for (int i = 0; i < 6; ++i) sig[i] = 0.0;
             return 1 | 2 | 4;
@@ -342,7 +393,8 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
 // enh[k * stride]. Writes the internal force of corners 1..7 (corner 0 carries minus their
 // sum) in the co-rotated frame. A torn element (all points failed) is left as it is and
 // returns a zero force.
-__host__ __device__ inline Result update(const Material& m, const double* cX, const double* cd,
+template <class CX>
+__host__ __device__ inline Result update(const Material& m, const CX& cX, const double* cd,
                                          double* hist, double* enh, uint32_t stride,
                                          bool wantStrain, double f[21], double* eu,
                                          uint32_t euStride) {
@@ -373,7 +425,7 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
     for (int p = 0; p < 4; ++p)
         for (int q = 0; q <= p; ++q) {
             double t = 0.0;
-            for (int i = 0; i < 3; ++i) t += 2.0 * t0[p][i] * (t0[q][i] - tr4[q] / 3.0);
+            for (int i = 0; i < 3; ++i) t += 2.0 * t0[p][i] * (t0[q][i] - tr4[q] * (1.0 / 3.0));
             for (int i = 3; i < 6; ++i) t += t0[p][i] * t0[q][i];
             D[p * (p + 1) / 2 + q] = t;
         }
@@ -387,8 +439,9 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
         for (int g = 0; g < 8; ++g) {
             double J[3][3], A[3][3], ec[6];
             jacobian(cX, gauss(g, 0), gauss(g, 1), gauss(g, 2), J);
-            w[g] = inverse3(J, A);
-            sc[g] = det0 / w[g];
+            double idg;
+            w[g] = inverse3(J, A, &idg);
+            sc[g] = det0 * idg;
             covariantStrain(J, cd, g, smp[(g >> 2) & 1], ec);
             double e6[6];
             push(A, ec, e6);
@@ -404,7 +457,7 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
             const double mk = monomial(k, gauss(g, 0), gauss(g, 1), gauss(g, 2));
             s += w[g] * sc[g] * sc[g] * mk * mk;
         }
-        inv[k] = c2 * s > 0.0 ? 1.0 / sqrt(c2 * s) : 1.0;
+        inv[k] = c2 * s > 0.0 ? invSqrt(c2 * s) : 1.0;
     }
     // Column k of G at point g is c_k(g) T0[:, r_k], c_k(g) = (det J0/det J)_g m_k(g) / |mode k|:
     // every column is a multiple of one of four vectors, so G alpha needs four sums and
@@ -450,10 +503,10 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
             dsum += w[g] * dg;
             rsum += w[g] * rel;
             if (wantStrain) {
-                const double pm = (sig[0] + sig[1] + sig[2]) / 3.0;
+                const double pm = (sig[0] + sig[1] + sig[2]) * (1.0 / 3.0);
                 double u = 0.0;
-                for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) / (2.0 * m.mu) + pm / (3.0 * m.kappa));
-                for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] / m.mu);
+                for (int i = 0; i < 3; ++i) u += 0.5 * sig[i] * ((sig[i] - pm) * m.i2mu + pm * m.i3k);
+                for (int i = 3; i < 6; ++i) u += 0.5 * sig[i] * (sig[i] * m.imu);
                 usum += w[g] * u;
             }
             if (fl & 4) continue;  // a failed point carries no stress
@@ -539,7 +592,8 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
             if (!eas) break;
             double nrm = 0.0;
             for (int k = 0; k < 7; ++k) nrm += r[k] * r[k];
-            if (sqrt(nrm) <= 1e-12 * scale) break;
+            const double tol = 1e-12 * scale;
+            if (nrm <= tol * tol) break;
             if (iter + 1 >= 40) break;
             // Cholesky with the pivots' reciprocals (one division per column)
             double L[28], id[7];
@@ -550,8 +604,9 @@ __host__ __device__ inline Result update(const Material& m, const double* cX, co
                     for (int k = 0; k < j; ++k) t -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
                     if (i == j) {
                         ok = ok && t > 0.0;
-                        L[i * (i + 1) / 2 + i] = sqrt(fmax(t, 1e-300));
-                        id[i] = 1.0 / L[i * (i + 1) / 2 + i];
+                        const double tt = fmax(t, 1e-300), y = invSqrt(tt);
+                        L[i * (i + 1) / 2 + i] = tt * y;
+                        id[i] = y;
                     } else {
                         L[i * (i + 1) / 2 + j] = t * id[j];
                     }

@@ -567,7 +567,7 @@ __device__ __forceinline__ void polar_perturbation_d(double W[9]) {
         const double dW = W[0] * a[0] + W[1] * a[3] + W[2] * a[6];
         const double den = 1.0 + (t + (i2 + dW));
         const bool ok = fabs(den) > 1e-300;
-        const double hid = ok ? 0.5 / den : 0.0, hnd = ok ? 0.0 : 0.5;
+        const double hid = ok ? 0.5 * hexp::recip(den) : 0.0, hnd = ok ? 0.0 : 0.5;
         const double sh = i2 + dW;
         double Wn[9];
 #pragma unroll
@@ -593,8 +593,11 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
     const CaseDev& c = cases[tile.case_index];
     if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
 
-    // each lane's base strains of its element's eight points, lane-interleaved
+    // each lane's base strains of its element's eight points and its element's rest
+    // coefficients, lane-interleaved
     __shared__ double sh_eu[48 * BLOCK];
+    __shared__ double sh_cx[kHexDof * BLOCK];
+    __shared__ double sh_w[9 * BLOCK];  // the element's rotation I + W
     const uint32_t e = tile.first + threadIdx.x;
     double diss = 0.0, se = 0.0, fract = 0.0;
     if (e < c.T) {
@@ -650,10 +653,15 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
                     cd[3 * mm + i] += W[0 * 3 + i] * y[0] + W[1 * 3 + i] * y[1] + W[2 * 3 + i] * y[2];
             }
             const hexp::Material m{c.mu_d, c.lam_d + (2.0 / 3.0) * c.mu_d, c.yield_d, c.hard_d,
-                                   c.hfail ? c.hfail[e] : c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage};
+                                   c.hfail ? c.hfail[e] : c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage,
+                                   c.hret_d[0], c.hret_d[1], c.hret_d[2], c.hret_d[3], c.hret_d[4]};
             double fi[kHexDof];
-            const hexp::Result res = hexp::update(m, cX, cd, c.hpoint + e, c.henh + e, tp, sample != 0, fi,
-                                                  sh_eu + threadIdx.x, BLOCK);
+#pragma unroll
+            for (int j = 0; j < kHexDof; ++j) sh_cx[j * BLOCK + threadIdx.x] = cX[j];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) sh_w[j * BLOCK + threadIdx.x] = W[j];
+            const hexp::Result res = hexp::update(m, hexp::Strided{sh_cx + threadIdx.x, BLOCK}, cd, c.hpoint + e,
+                                                  c.henh + e, tp, sample != 0, fi, sh_eu + threadIdx.x, BLOCK);
             diss = res.dissipation;
             fract = res.released;
             se = res.strain;
@@ -665,6 +673,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 #pragma unroll
                 for (int i = 0; i < 3; ++i) k0[i] -= fi[3 * a + i];
             float f[24];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) W[j] = sh_w[j * BLOCK + threadIdx.x];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const double v[3] = {a == 0 ? k0[0] : fi[3 * a - 3], a == 0 ? k0[1] : fi[3 * a - 2],

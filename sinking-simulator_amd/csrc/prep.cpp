@@ -1,6 +1,7 @@
 // prep.cpp -- case preparation (FP64 rest state on the host, FP32 SoA planes for the
 // device), the per-case device arena, upload and readback.
 #include "shipfem_host.hpp"
+#include "hex_plastic.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -786,6 +787,11 @@ hipError_t uploadCase(const HostCase& hc, char* base, DeviceCase& dc, hipStream_
     r.sens_d = hc.fail_sens;
     r.eref_d = hc.fail_eref;
     r.ecut_d = hc.fail_ecut;
+    {
+        const hexp::Material hm = hexp::derive(hexp::Material{hc.mu, hc.lam + (2.0 / 3.0) * hc.mu, hc.yield, hc.hard});
+        const double d[5] = {hm.igam, hm.thb0, hm.i2mu, hm.i3k, hm.imu};
+        std::copy(d, d + 5, r.hret_d);
+    }
     r.acc_e = reinterpret_cast<double*>(base + m.acc_e);
     r.acc_n = reinterpret_cast<double*>(base + m.acc_n);
     r.smp_e = reinterpret_cast<double*>(base + m.smp_e);

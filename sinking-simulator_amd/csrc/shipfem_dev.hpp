@@ -120,6 +120,7 @@ struct CaseDev {
     double lam_d, mu_d;       // FP64 Lame constants for the stable-step evaluation
     double yield_d, hard_d;   // FP64 flow curve and failure constants (plastic hexes)
     double fail_d, sens_d, eref_d, ecut_d;
+    double hret_d[5];         // plastic hexes: the return's constants (hexp::Material's derived)
     // ---- FP64 accounting
     double* acc_e;  // [kAccElemSlots][elem_waves]
     double* acc_n;  // [kAccNodeSlots][node_waves]
