@@ -234,6 +234,11 @@ __host__ __device__ inline double monomial(int k, double xi, double eta, double 
                                                               : k == 5 ? xi : eta;
 }
 __host__ __device__ inline int modeRow(int k) { return k < 2 ? k : k < 5 ? 2 : 3; }
+// the axes of mode k's monomial as bits (xi 1, eta 2, zeta 4) and its degree
+__host__ __device__ constexpr int modeAxes(int k) {
+    return k == 0 || k == 5 ? 1 : k == 1 || k == 6 ? 2 : k == 2 ? 4 : k == 3 ? 5 : 6;
+}
+__host__ __device__ constexpr int modeDegree(int k) { return k == 3 || k == 4 ? 2 : 1; }
 
 // Assumed-strain samples of one zeta level: 2E23 at (-1, 0), (+1, 0); 2E31 at (0, -1),
 // (0, +1); E33 at the corners (-1,-1), (1,-1), (1,1), (-1,1).
@@ -245,15 +250,14 @@ constexpr double kCorner[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
 // plastic strain but zero eqps carries -0, which reads.)
 __host__ __device__ inline bool hasHistory(double eqps) { return __builtin_bit_cast(uint64_t, eqps) != 0; }
 
-// Point g's history (plastic strain 6, eqps, damage) at the start of the step.
-__host__ __device__ inline void loadPoint(const double* hist, int g, uint32_t stride, double h0[8]) {
-    h0[6] = hist[(8 * g + 6) * stride];
-    if (hasHistory(h0[6])) {
-        for (int k = 0; k < 8; ++k)
-            if (k != 6) h0[k] = hist[(8 * g + k) * stride];
+// Point g's history (plastic strain 6, eqps, damage) at the start of the step. hmask has bit
+// g set when point g had yielded before the step (its eqps is not +0): the others have a zero
+// history and are not read.
+__host__ __device__ inline void loadPoint(const double* hist, int g, uint32_t stride, int hmask, double h0[8]) {
+    if ((hmask >> g) & 1) {
+        for (int k = 0; k < 8; ++k) h0[k] = hist[(8 * g + k) * stride];
     } else {
-        for (int k = 0; k < 8; ++k)
-            if (k != 6) h0[k] = 0.0;
+        for (int k = 0; k < 8; ++k) h0[k] = 0.0;
     }
 }
 
@@ -400,8 +404,12 @@ __host__ __device__ inline Result update(const Material& m, const CX& cX, const 
                                          uint32_t euStride) {
     Result res;
     for (int j = 0; j < 21; ++j) f[j] = 0.0;
+    int hmask = 0;  // the points with a history (read once here, not in every pass)
     for (int g = 0; g < 8; ++g)
-        res.failedStart += hasHistory(hist[(8 * g + 6) * stride]) && hist[(8 * g + 7) * stride] >= 1.0;
+        if (hasHistory(hist[(8 * g + 6) * stride])) {
+            hmask |= 1 << g;
+            res.failedStart += hist[(8 * g + 7) * stride] >= 1.0;
+        }
     if (m.damage && res.failedStart == 8) {
         res.failedEnd = 8;
         return res;
@@ -494,7 +502,7 @@ __host__ __device__ inline Result update(const Material& m, const CX& cX, const 
         for (int g = 0; g < 8; ++g) {
             double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
             strainAt(g, eas, st);
-            loadPoint(hist, g, stride, h0);
+            loadPoint(hist, g, stride, hmask, h0);
             const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
             if (onlyElastic && fl != 0) return false;
             if (fl & 1)  // only a yielding point changes its history
@@ -550,43 +558,72 @@ __host__ __device__ inline Result update(const Material& m, const CX& cX, const 
             for (int k = 0; k < 28; ++k) kaa[k] = 0.0;
             died = false;
             bool linear = true;
+            // moments of w sc^2 over the points that answered elastically: mo[b] is the sum of
+            // w sc^2 chi_b(g), chi_b(g) the product of point g's coordinate signs over the axes in b
+            double mo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            bool anyElastic = false;
 #pragma unroll 1
             for (int g = 0; g < 8; ++g) {
                 double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
                 strainAt(g, eas, st);
-                loadPoint(hist, g, stride, h0);
+                loadPoint(hist, g, stride, hmask, h0);
                 const int fl = pointReturn(m, st, h0, h, sig, dg, th, thb, n6, rel);
                 died = died || (fl & 2);
                 linear = linear && fl == 0;
                 if (!eas) continue;
-                double ts[4], nv[4], M[10];
+                double ts[4];
                 for (int q = 0; q < 4; ++q) {
-                    double t = 0.0, u = 0.0;
-                    for (int i = 0; i < 6; ++i) {
-                        t += t0[q][i] * sig[i];
-                        u += t0[q][i] * n6[i];
-                    }
+                    double t = 0.0;
+                    for (int i = 0; i < 6; ++i) t += t0[q][i] * sig[i];
                     ts[q] = t;
-                    nv[q] = u;
                 }
-                // (zero for a failed point)
-                const double ck = (fl & 4) ? 0.0 : m.kappa, cm = (fl & 4) ? 0.0 : m.mu * th,
-                             cn = (fl & 4) ? 0.0 : 2.0 * m.mu * thb;
-                for (int p = 0; p < 4; ++p)
-                    for (int q = 0; q <= p; ++q) {
-                        const int i = p * (p + 1) / 2 + q;
-                        M[i] = ck * tr4[p] * tr4[q] + cm * D[i] - cn * nv[p] * nv[q];
-                    }
                 double cg[7];
                 for (int k = 0; k < 7; ++k) {
                     cg[k] = coef(g, k);
                     r[k] += w[g] * cg[k] * ts[modeRow(k)];
                 }
+                if (fl == 0) {
+                    anyElastic = true;
+                    const double sg = w[g] * sc[g] * sc[g];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) mo[b] += __builtin_popcount(b & ~g) & 1 ? -sg : sg;
+                } else if (!(fl & 4)) {
+                    // a yielding point's own algorithmic tangent (a failed point adds none)
+                    double nv[4], M[10];
+                    for (int q = 0; q < 4; ++q) {
+                        double u = 0.0;
+                        for (int i = 0; i < 6; ++i) u += t0[q][i] * n6[i];
+                        nv[q] = u;
+                    }
+                    const double cm = m.mu * th, cn = 2.0 * m.mu * thb;
+                    for (int p = 0; p < 4; ++p)
+                        for (int q = 0; q <= p; ++q) {
+                            const int i = p * (p + 1) / 2 + q;
+                            M[i] = m.kappa * tr4[p] * tr4[q] + cm * D[i] - cn * nv[p] * nv[q];
+                        }
+                    for (int p = 0; p < 7; ++p)
+                        for (int q = 0; q <= p; ++q) {
+                            const int a = modeRow(p), b = modeRow(q);
+                            const double mpq = a >= b ? M[a * (a + 1) / 2 + b] : M[b * (b + 1) / 2 + a];
+                            kaa[p * (p + 1) / 2 + q] += w[g] * cg[p] * cg[q] * mpq;
+                        }
+                }
+            }
+            if (eas && anyElastic) {
+                // The elastic points' share of Kaa. Their tangent is the element's elastic form
+                // Me = kappa tr tr^T + mu D, so they add inv_p inv_q Me_ab sum_g w sc^2 m_p m_q,
+                // and at a Gauss point m_p m_q = q^(deg p + deg q) chi_(e_p ^ e_q)(g).
+                const double qq = 0.57735026918962576451 * 0.57735026918962576451;
+                const double qn[3] = {qq, qq * 0.57735026918962576451, qq * qq};  // q^2, q^3, q^4
+#pragma unroll
                 for (int p = 0; p < 7; ++p)
+#pragma unroll
                     for (int q = 0; q <= p; ++q) {
                         const int a = modeRow(p), b = modeRow(q);
-                        const double mpq = a >= b ? M[a * (a + 1) / 2 + b] : M[b * (b + 1) / 2 + a];
-                        kaa[p * (p + 1) / 2 + q] += w[g] * cg[p] * cg[q] * mpq;
+                        const int ab = a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a;
+                        const double me = m.kappa * tr4[a] * tr4[b] + m.mu * D[ab];
+                        kaa[p * (p + 1) / 2 + q] +=
+                            inv[p] * inv[q] * (qn[modeDegree(p) + modeDegree(q) - 2] * me) * mo[modeAxes(p) ^ modeAxes(q)];
                     }
             }
             if (!eas) break;
