@@ -565,7 +565,8 @@ __device__ __forceinline__ void polar_perturbation_d(double W[9]) {
         const double t = W[0] + W[4] + W[8];
         const double i2 = a[0] + a[4] + a[8];
         const double dW = W[0] * a[0] + W[1] * a[3] + W[2] * a[6];
-        const double den = 1.0 + (t + (i2 + dW));
+        const double q = t + (i2 + dW);  // det(I + W) - 1
+        const double den = 1.0 + q;
         const bool ok = fabs(den) > 1e-300;
         const double hid = ok ? 0.5 * hexp::recip(den) : 0.0, hnd = ok ? 0.0 : 0.5;
         const double sh = i2 + dW;
@@ -580,6 +581,10 @@ __device__ __forceinline__ void polar_perturbation_d(double W[9]) {
             }
 #pragma unroll
         for (int k = 0; k < 9; ++k) W[k] = Wn[k];
+        // From the second step on, the stretch error d left in I + W is positive
+        // semi-definite with trace ~ det(I + W) - 1, and the next step leaves d^2/2: below
+        // 2^-30 the step just taken has brought R to FP64 resolution.
+        if (it >= 1 && fabs(q) < 0x1p-30) break;
     }
 }
 
