@@ -391,25 +391,41 @@ __host__ __device__ inline int pointReturn(const Material& m, const double e[6],
     return 1;
 }
 
+// The element's points at the start of the step: which have a history (bit g) and how many
+// have failed. Read once per step: the Newton passes and the commit test the mask.
+struct Start {
+    int hmask = 0, failed = 0;
+};
+// (With the points' eqps already loaded: only a point with a history has its damage read.)
+__host__ __device__ inline Start startOf(const double eqps[8], const double* hist, uint32_t stride) {
+    Start s;
+    for (int g = 0; g < 8; ++g) s.hmask |= hasHistory(eqps[g]) ? 1 << g : 0;
+    if (s.hmask)
+        for (int g = 0; g < 8; ++g)
+            if ((s.hmask >> g) & 1) s.failed += hist[(8 * g + 7) * stride] >= 1.0;
+    return s;
+}
+__host__ __device__ inline Start startOf(const double* hist, uint32_t stride) {
+    double eqps[8];
+    for (int g = 0; g < 8; ++g) eqps[g] = hist[(8 * g + 6) * stride];
+    return startOf(eqps, hist, stride);
+}
+
 // The element update. cX: basis coefficients of the rest corner offsets (corner 0 at
 // zero); cd: those of the co-rotated displacement offsets; hist: 8 points x (plastic strain
 // 6, eqps, damage) at hist[(8 g + k) * stride]; enh: 7 enhanced parameters at
 // enh[k * stride]. Writes the internal force of corners 1..7 (corner 0 carries minus their
 // sum) in the co-rotated frame. A torn element (all points failed) is left as it is and
-// returns a zero force.
+// returns a zero force. start: startOf(hist, stride) at the step's start.
 template <class CX>
 __host__ __device__ inline Result update(const Material& m, const CX& cX, const double* cd,
                                          double* hist, double* enh, uint32_t stride,
                                          bool wantStrain, double f[21], double* eu,
-                                         uint32_t euStride) {
+                                         uint32_t euStride, const Start& start) {
     Result res;
     for (int j = 0; j < 21; ++j) f[j] = 0.0;
-    int hmask = 0;  // the points with a history (read once here, not in every pass)
-    for (int g = 0; g < 8; ++g)
-        if (hasHistory(hist[(8 * g + 6) * stride])) {
-            hmask |= 1 << g;
-            res.failedStart += hist[(8 * g + 7) * stride] >= 1.0;
-        }
+    const int hmask = start.hmask;
+    res.failedStart = start.failed;
     if (m.damage && res.failedStart == 8) {
         res.failedEnd = 8;
         return res;

@@ -96,6 +96,11 @@ __device__ __forceinline__ uint2 gload(const uint2* p) {
     const v2 x = *gmem(reinterpret_cast<const v2*>(p));
     return make_uint2(x.x, x.y);
 }
+__device__ __forceinline__ uint4 gload(const uint4* p) {
+    typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+    const v4 x = *gmem(reinterpret_cast<const v4*>(p));
+    return make_uint4(x.x, x.y, x.z, x.w);
+}
 
 __device__ __forceinline__ double wave_sum(double s) {
 #pragma unroll
@@ -596,7 +601,6 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
     const int b = remap_block(blockIdx.x, gridDim.x, xcd);
     const Tile tile = tiles[b];
     const CaseDev& c = cases[tile.case_index];
-    if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
 
     // each lane's base strains of its element's eight points and its element's rest
     // coefficients, lane-interleaved
@@ -604,44 +608,54 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
     __shared__ double sh_cx[kHexDof * BLOCK];
     __shared__ double sh_w[9 * BLOCK];  // the element's rotation I + W
     const uint32_t e = tile.first + threadIdx.x;
+    const bool live = e < c.T;
+    const uint32_t tp = c.Tpad, np = c.Npad;
+    // One wave per SIMD leaves nothing to hide a load chain behind, so every load that does
+    // not depend on another is issued first: the connectivity (which heads the gather), each
+    // point's eqps and the centre gradients; then the corner gather; the step counter is
+    // tested only then. Indices are clamped into range; a dead lane's values are discarded.
+    const uint32_t ec = live ? e : c.T - 1;
+    const uint4 q0 = gload(c.conn + 2 * ec), q1 = gload(c.conn + 2 * ec + 1);
+    double eqps[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) eqps[g] = *gmem(c.hpoint + (8 * g + 6) * tp + ec);
+    double b0[7][3];
+#pragma unroll
+    for (int a = 0; a < 7; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b0[a][j] = *gmem(c.hgeod + (3 * a + j) * tp + ec);
+    const uint32_t gi[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    double ux[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const float4 pv = gload(c.u4 + gi[a]);
+        ux[a][0] = (double)pv.x - (double)*gmem(c.cu + gi[a]);
+        ux[a][1] = (double)pv.y - (double)*gmem(c.cu + np + gi[a]);
+        ux[a][2] = (double)pv.z - (double)*gmem(c.cu + 2 * np + gi[a]);
+    }
+    if (c.step[parity] >= c.step_end) return;  // the case has finished: whole block idles
+
     double diss = 0.0, se = 0.0, fract = 0.0;
-    if (e < c.T) {
-        const uint32_t tp = c.Tpad, np = c.Npad;
+    if (live) {
         const bool damage = (c.features & kFeatDamage) != 0;
         float4* out = reinterpret_cast<float4*>(c.force) + (size_t)e * 6;
-        int failedStart = 0;
-#pragma unroll
-        for (int g = 0; g < 8; ++g)
-            failedStart += hexp::hasHistory(c.hpoint[(8 * g + 6) * tp + e]) && c.hpoint[(8 * g + 7) * tp + e] >= 1.0;
-        if (damage && failedStart == 8) {
+        const hexp::Start start = hexp::startOf(eqps, c.hpoint + e, tp);
+        if (damage && start.failed == 8) {
             // a torn element carries nothing and is not updated (zone.cpp:944-955)
 #pragma unroll
             for (int k = 0; k < 6; ++k) out[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         } else {
-            const uint4 q0 = c.conn[2 * e], q1 = c.conn[2 * e + 1];
-            const uint32_t gi[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-            double ux[8][3];
-#pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const float4 pv = c.u4[gi[a]];
-                ux[a][0] = (double)pv.x - (double)c.cu[gi[a]];
-                ux[a][1] = (double)pv.y - (double)c.cu[np + gi[a]];
-                ux[a][2] = (double)pv.z - (double)c.cu[2 * np + gi[a]];
-            }
             double r[7][3], W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int a = 0; a < 7; ++a)
 #pragma unroll
                 for (int i = 0; i < 3; ++i) r[a][i] = ux[a + 1][i] - ux[0][i];
 #pragma unroll
-            for (int a = 0; a < 7; ++a) {
-                const double b0[3] = {c.hgeod[(3 * a) * tp + e], c.hgeod[(3 * a + 1) * tp + e],
-                                      c.hgeod[(3 * a + 2) * tp + e]};
+            for (int a = 0; a < 7; ++a)
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) W[i * 3 + j] = fma(r[a][i], b0[j], W[i * 3 + j]);
-            }
+                    for (int j = 0; j < 3; ++j) W[i * 3 + j] = fma(r[a][i], b0[a][j], W[i * 3 + j]);
             polar_perturbation_d(W);
             // the co-rotated displacement d = r + W^T (X + r), taken to basis coefficients
             // (the map is linear): cd = cr + W^T (cX + cr)
@@ -666,7 +680,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 #pragma unroll
             for (int j = 0; j < 9; ++j) sh_w[j * BLOCK + threadIdx.x] = W[j];
             const hexp::Result res = hexp::update(m, hexp::Strided{sh_cx + threadIdx.x, BLOCK}, cd, c.hpoint + e,
-                                                  c.henh + e, tp, sample != 0, fi, sh_eu + threadIdx.x, BLOCK);
+                                                  c.henh + e, tp, sample != 0, fi, sh_eu + threadIdx.x, BLOCK,
+                                                  start);
             diss = res.dissipation;
             fract = res.released;
             se = res.strain;
