@@ -416,12 +416,14 @@ __host__ __device__ inline Start startOf(const double* hist, uint32_t stride) {
 // 6, eqps, damage) at hist[(8 g + k) * stride]; enh: 7 enhanced parameters at
 // enh[k * stride]. Writes the internal force of corners 1..7 (corner 0 carries minus their
 // sum) in the co-rotated frame. A torn element (all points failed) is left as it is and
-// returns a zero force. start: startOf(hist, stride) at the step's start.
+// returns a zero force. start: startOf(hist, stride) at the step's start; alpha0, when
+// given, the enhanced parameters as already read from enh.
 template <class CX>
 __host__ __device__ inline Result update(const Material& m, const CX& cX, const double* cd,
                                          double* hist, double* enh, uint32_t stride,
                                          bool wantStrain, double f[21], double* eu,
-                                         uint32_t euStride, const Start& start) {
+                                         uint32_t euStride, const Start& start,
+                                         const double* alpha0 = nullptr) {
     Result res;
     for (int j = 0; j < 21; ++j) f[j] = 0.0;
     const int hmask = start.hmask;
@@ -488,7 +490,7 @@ __host__ __device__ inline Result update(const Material& m, const CX& cX, const 
     // G^T C G needs only the 4 x 4 form T0[:, r]^T C T0[:, s] of each point.
     int eas = res.failedStart > 0 ? 0 : 7;
     double alpha[7];
-    for (int k = 0; k < 7; ++k) alpha[k] = eas ? enh[k * stride] : 0.0;
+    for (int k = 0; k < 7; ++k) alpha[k] = eas ? (alpha0 ? alpha0[k] : enh[k * stride]) : 0.0;
     auto coef = [&](int g, int k) {
         return sc[g] * monomial(k, gauss(g, 0), gauss(g, 1), gauss(g, 2)) * inv[k];
     };

@@ -624,6 +624,12 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
     for (int a = 0; a < 7; ++a)
 #pragma unroll
         for (int j = 0; j < 3; ++j) b0[a][j] = *gmem(c.hgeod + (3 * a + j) * tp + ec);
+    double cX[kHexDof], alpha0[7];
+#pragma unroll
+    for (int j = 0; j < kHexDof; ++j) cX[j] = *gmem(c.hgeod + (kHexDof + j) * tp + ec);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) alpha0[k] = *gmem(c.henh + k * tp + ec);
+    const double failE = c.hfail ? *gmem(c.hfail + ec) : c.fail_d;
     const uint32_t gi[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
     double ux[8][3];
 #pragma unroll
@@ -659,10 +665,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
             polar_perturbation_d(W);
             // the co-rotated displacement d = r + W^T (X + r), taken to basis coefficients
             // (the map is linear): cd = cr + W^T (cX + cr)
-            double cX[kHexDof], cd[kHexDof];
+            double cd[kHexDof];
             hexp::basis(&r[0][0], cd);
-#pragma unroll
-            for (int j = 0; j < kHexDof; ++j) cX[j] = c.hgeod[(kHexDof + j) * tp + e];
 #pragma unroll
             for (int mm = 0; mm < 7; ++mm) {
                 const double y[3] = {cX[3 * mm] + cd[3 * mm], cX[3 * mm + 1] + cd[3 * mm + 1],
@@ -672,7 +676,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
                     cd[3 * mm + i] += W[0 * 3 + i] * y[0] + W[1 * 3 + i] * y[1] + W[2 * 3 + i] * y[2];
             }
             const hexp::Material m{c.mu_d, c.lam_d + (2.0 / 3.0) * c.mu_d, c.yield_d, c.hard_d,
-                                   c.hfail ? c.hfail[e] : c.fail_d, c.sens_d, c.eref_d, c.ecut_d, damage,
+                                   failE, c.sens_d, c.eref_d, c.ecut_d, damage,
                                    c.hret_d[0], c.hret_d[1], c.hret_d[2], c.hret_d[3], c.hret_d[4]};
             double fi[kHexDof];
 #pragma unroll
@@ -681,7 +685,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
             for (int j = 0; j < 9; ++j) sh_w[j * BLOCK + threadIdx.x] = W[j];
             const hexp::Result res = hexp::update(m, hexp::Strided{sh_cx + threadIdx.x, BLOCK}, cd, c.hpoint + e,
                                                   c.henh + e, tp, sample != 0, fi, sh_eu + threadIdx.x, BLOCK,
-                                                  start);
+                                                  start, alpha0);
             diss = res.dissipation;
             fract = res.released;
             se = res.strain;
