@@ -680,7 +680,7 @@ shipfem_status shipfem_hex_plastic_update(const double rest[24], const double cu
     shipfem::hexp::basis(X, cX);
     shipfem::hexp::basis(d, cd);
     const shipfem::hexp::Result res =
-        shipfem::hexp::update(m, cX, cd, point_state, enhanced, 1, false, fi, eu, 1,
+        shipfem::hexp::update(m, cX, cX, cd, point_state, enhanced, 1, false, fi, eu, 1,
                               shipfem::hexp::startOf(point_state, 1));
     // the force on the nodes, -R f, corner 0 carrying minus the others
     double f0[3] = {0.0, 0.0, 0.0};

@@ -411,15 +411,16 @@ __host__ __device__ inline Start startOf(const double* hist, uint32_t stride) {
     return startOf(eqps, hist, stride);
 }
 
-// The element update. cX: basis coefficients of the rest corner offsets (corner 0 at
-// zero); cd: those of the co-rotated displacement offsets; hist: 8 points x (plastic strain
+// The element update. cXs, cX: basis coefficients of the rest corner offsets (corner 0 at
+// zero), the same values as the set-up reads them (an array) and as the later passes read
+// them (on the device, this lane's column of a block-shared array); cd: those of the co-rotated displacement offsets; hist: 8 points x (plastic strain
 // 6, eqps, damage) at hist[(8 g + k) * stride]; enh: 7 enhanced parameters at
 // enh[k * stride]. Writes the internal force of corners 1..7 (corner 0 carries minus their
 // sum) in the co-rotated frame. A torn element (all points failed) is left as it is and
 // returns a zero force. start: startOf(hist, stride) at the step's start; alpha0, when
 // given, the enhanced parameters as already read from enh.
 template <class CX>
-__host__ __device__ inline Result update(const Material& m, const CX& cX, const double* cd,
+__host__ __device__ inline Result update(const Material& m, const double* cXs, const CX& cX, const double* cd,
                                          double* hist, double* enh, uint32_t stride,
                                          bool wantStrain, double f[21], double* eu,
                                          uint32_t euStride, const Start& start,
@@ -435,7 +436,7 @@ __host__ __device__ inline Result update(const Material& m, const CX& cX, const 
     // the centre's map T0 (columns 0-3): the centre Jacobian's columns are c_0, c_1, c_2
     double J0[3][3], A0[3][3];
     for (int i = 0; i < 3; ++i)
-        for (int k = 0; k < 3; ++k) J0[i][k] = cX[3 * k + i];
+        for (int k = 0; k < 3; ++k) J0[i][k] = cXs[3 * k + i];
     const double det0 = inverse3(J0, A0);
     double t0[4][6];
     for (int r = 0; r < 4; ++r) {
@@ -460,11 +461,11 @@ __host__ __device__ inline Result update(const Material& m, const CX& cX, const 
     double w[8], sc[8];
     {
         double smp[2][8];
-        samples(cX, cd, -0.57735026918962576451, smp[0]);
-        samples(cX, cd, 0.57735026918962576451, smp[1]);
+        samples(cXs, cd, -0.57735026918962576451, smp[0]);
+        samples(cXs, cd, 0.57735026918962576451, smp[1]);
         for (int g = 0; g < 8; ++g) {
             double J[3][3], A[3][3], ec[6];
-            jacobian(cX, gauss(g, 0), gauss(g, 1), gauss(g, 2), J);
+            jacobian(cXs, gauss(g, 0), gauss(g, 1), gauss(g, 2), J);
             double idg;
             w[g] = inverse3(J, A, &idg);
             sc[g] = det0 * idg;

@@ -683,7 +683,7 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
             for (int j = 0; j < kHexDof; ++j) sh_cx[j * BLOCK + threadIdx.x] = cX[j];
 #pragma unroll
             for (int j = 0; j < 9; ++j) sh_w[j * BLOCK + threadIdx.x] = W[j];
-            const hexp::Result res = hexp::update(m, hexp::Strided{sh_cx + threadIdx.x, BLOCK}, cd, c.hpoint + e,
+            const hexp::Result res = hexp::update(m, cX, hexp::Strided{sh_cx + threadIdx.x, BLOCK}, cd, c.hpoint + e,
                                                   c.henh + e, tp, sample != 0, fi, sh_eu + threadIdx.x, BLOCK,
                                                   start, alpha0);
             diss = res.dissipation;
@@ -713,11 +713,13 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
     }
     const uint32_t w = e >> 6;
     const bool writer = (threadIdx.x & 63u) == 0 && w < c.elem_waves;
-    {
+    // a wave with no dissipation (or fracture) adds +0 to its running sum, which leaves the
+    // sum's bits as they are: it skips the butterfly, the read and the store
+    if (__ballot(diss != 0.0)) {
         const double s = wave_sum(diss);
         if (writer) c.acc_e[kAccDiss * c.elem_waves + w] += s;
     }
-    if (c.features & kFeatDamage) {
+    if ((c.features & kFeatDamage) && __ballot(fract != 0.0)) {
         const double s = wave_sum(fract);
         if (writer) c.acc_e[kAccFract * c.elem_waves + w] += s;
     }
