@@ -737,10 +737,12 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 // lane writes each corner force straight into its slot and a node lane sums one
 // contiguous run: the tile needs no separate node->corner map.
 // LDS: corner-slot forces [3][4 kTile] floats (component planes), the run offsets
-// [lmax + 1] (16-bit), then the tile's local nodes' displacement float4[lmax] and rest
-// position about the tile's centroid float4[lmax]; each tet forms its Dm^-1 and V from
-// those. Every global load a lane needs is issued before the first barrier, so the
-// staging gathers share one latency.
+// [lmax + 1] (16-bit), then the tile's local nodes' displacement float4[lmax]. Every
+// global load of the staging is issued before the first barrier, so its gathers share one
+// latency. LDS bandwidth, not HBM or the FP32 math, is what the pass spends most of its
+// time on (counters: the LDS busy most of the pass, nearly half of it in bank conflicts),
+// so a tet that forms its Dm^-1 reads its corners' rest positions through the vector L1
+// (the tile's few KB of them stay cached) rather than from a staged LDS copy.
 constexpr uint32_t kSlots4 = 4 * kTile;  // corner slots of a tile
 __host__ __device__ constexpr size_t tiled_lds_offsets_offset() { return 3 * kSlots4 * sizeof(float); }
 __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
@@ -748,8 +750,8 @@ __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
 }
 
 // The element pass of the 256-tet tile from `first`, at the case's step `n`. FORMED: each
-// tet forms its Dm^-1 and V from the tile's rest positions staged in LDS; else it reads the
-// stored planes. Its 256 threads (lane `lid`) all reach its two barriers; with `active`
+// tet forms its Dm^-1 and V from the rest positions of its corners (the tile's, about its
+// centroid); else it reads the stored planes. Its 256 threads (lane `lid`) all reach its two barriers; with `active`
 // false (no tile for them) they do nothing else. With `nptr` the step is read from there
 // instead, once the staging loads are in flight, and a finished case (step_end reached)
 // returns before its first store.
@@ -760,7 +762,6 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
     float* sh_f = reinterpret_cast<float*>(smem);
     uint16_t* sh_lo = reinterpret_cast<uint16_t*>(smem + tiled_lds_offsets_offset());
     float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
-    float4* sh_x = sh_u + lmax;  // the local nodes' rest positions about the tile's centroid
 
     const uint32_t tb = first / kTile;
     uint32_t p0 = 0, L = 1;
@@ -796,11 +797,6 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
             for (int k = 0; k < 9; ++k) B[k] = ld_at(sgpr_base(c.dmi + (uint64_t)k * c.Tpad), ec4);
             V = ld_at(c.vol, ec4);
         }
-        float4 xa, xb;
-        if (FORMED) {
-            xa = gload(c.trest + p0 + min(la, L - 1));
-            xb = gload(c.trest + p0 + min(lb, L - 1));
-        }
         const float4 ua = gload(c.u4 + ga);
         const float4 ub = gload(c.u4 + gb);
         if (nptr) {
@@ -811,15 +807,10 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
         // holds, so no store waits behind a branch
         sh_lo[min(la, L)] = static_cast<uint16_t>(oa);
         sh_lo[min(lb, L)] = static_cast<uint16_t>(ob);
-        if (FORMED) {
-            sh_x[min(la, L - 1)] = xa;
-            sh_x[min(lb, L - 1)] = xb;
-        }
         sh_u[min(la, L - 1)] = ua;
         sh_u[min(lb, L - 1)] = ub;
         for (uint32_t l = lid + 2 * kTile; l <= L; l += kTile) sh_lo[l] = lo[l];
         for (uint32_t l = lid + 2 * kTile; l < L; l += kTile) {
-            if (FORMED) sh_x[l] = c.trest[p0 + l];
             sh_u[l] = c.u4[c.tnode[p0 + l]];
         }
     }
@@ -838,7 +829,8 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
         // the inverse by cofactors, a degenerate tet (|det| <= 1e-30) taking zero
         // (fem.cpp:42-56)
         if (FORMED) {
-            const float4 x0 = sh_x[ln[0]], x1 = sh_x[ln[1]], x2 = sh_x[ln[2]], x3 = sh_x[ln[3]];
+            const auto* xr = c.trest + p0;
+            const float4 x0 = gload(xr + ln[0]), x1 = gload(xr + ln[1]), x2 = gload(xr + ln[2]), x3 = gload(xr + ln[3]);
             const float D[3][3] = {{x1.x - x0.x, x2.x - x0.x, x3.x - x0.x},
                                    {x1.y - x0.y, x2.y - x0.y, x3.y - x0.y},
                                    {x1.z - x0.z, x2.z - x0.z, x3.z - x0.z}};
@@ -1430,8 +1422,8 @@ void launch_elem_hex_plastic(const CaseDev* cases, const Tile* tiles, int ntiles
 void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint32_t lmax,
                        int parity, int sample, int xcd, uint32_t feat_union, bool formed, hipStream_t stream,
                        const KernelTiming* t) {
-    // the local nodes' displacements, and with `formed` their rest positions, after the slots
-    const size_t lds = tiled_lds_nodes_offset(lmax) + (formed ? 32 : 16) * (size_t)lmax;
+    // the local nodes' displacements after the slots and run offsets
+    const size_t lds = tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax;
 #define SHIPFEM_TILED(P, D, F)                                                                         \
     SHIPFEM_LAUNCH(t, (k_elem_tiled<P, D, F>), dim3(ntiles), dim3(kTile), lds, stream, cases, tiles, ntiles, \
                    lmax, parity, sample, xcd)
@@ -1445,14 +1437,14 @@ void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint
 #undef SHIPFEM_TILED
 }
 
-size_t resident_lds(uint32_t lmax, bool formed) {
-    const size_t tile = tiled_lds_nodes_offset(lmax) + (formed ? 32 : 16) * (size_t)lmax;
+size_t resident_lds(uint32_t lmax, bool) {
+    const size_t tile = tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax;
     return std::max<size_t>(4 * ((tile + 15) / 16 * 16), 8 * kRecordBlock);
 }
 
 void launch_resident(const CaseDev* cases, int ncases, uint32_t lmax, int steps, int parity0, int sample_first,
                      uint32_t feat_union, bool formed, hipStream_t stream) {
-    const uint32_t tile = (uint32_t)((tiled_lds_nodes_offset(lmax) + (formed ? 32 : 16) * (size_t)lmax + 15) / 16 * 16);
+    const uint32_t tile = (uint32_t)((tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax + 15) / 16 * 16);
     const size_t lds = resident_lds(lmax, formed);
     const bool extra = (feat_union & (kFeatContact | kFeatDrive | kFeatDamp)) != 0;
 #define SHIPFEM_RES(P, D, F, X)                                                                           \
