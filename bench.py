@@ -480,6 +480,12 @@ def roofline_entry(kernel, ms, need, model_bytes, counters=None):
     if counters:
         r["traffic"] = counters.get("bytes")
         r["traffic_source"] = counters.get("source")
+        # what the pass is bound by instead, where its counter profile says (busy fractions
+        # of the pass's cycles at 2.4 GHz, from the committed counter run)
+        if counters.get("lds_busy_cycles_per_cu"):
+            cyc = ms * 1e-3 * 2.4e9
+            r["lds_busy_frac"] = round(counters["lds_busy_cycles_per_cu"] / cyc, 3)
+            r["lds_bank_conflict_frac"] = round(counters["lds_conflict_cycles_per_cu"] / cyc, 3)
     return r
 
 
@@ -487,7 +493,11 @@ def counters_for(name, kernel_key):
     t = traffic_from_profiles(name)
     if not t or kernel_key not in t:
         return None
-    return dict(bytes=t[kernel_key], source=t.get("source", f"profiles/traffic_{name}.json"))
+    out = dict(bytes=t[kernel_key], source=t.get("source", f"profiles/traffic_{name}.json"))
+    if kernel_key == "element" and "element_lds_idx_active_per_cu" in t:
+        out.update(lds_busy_cycles_per_cu=t["element_lds_idx_active_per_cu"],
+                   lds_conflict_cycles_per_cu=t["element_lds_bank_conflict_per_cu"])
+    return out
 
 
 PRE_STEPS = 500  # untimed steps before the timed ones in the batched configuration

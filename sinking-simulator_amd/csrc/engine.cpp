@@ -74,7 +74,7 @@ hipError_t Plan::build(hipStream_t s) {
         int dev = 0, maxLds = 0;
         resident = hipGetDevice(&dev) == hipSuccess &&
                    hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
-                   resident_lds(lmax, shape.formed != 0) <= static_cast<size_t>(maxLds);
+                   resident_lds(lmax) <= static_cast<size_t>(maxLds);
     }
     hipError_t e = hipSuccess;
     size_t need = std::max({elem_tiles.size(), node_tiles.size(), omega_tiles.size()});

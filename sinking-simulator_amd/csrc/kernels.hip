@@ -1437,7 +1437,7 @@ void launch_elem_tiled(const CaseDev* cases, const Tile* tiles, int ntiles, uint
 #undef SHIPFEM_TILED
 }
 
-size_t resident_lds(uint32_t lmax, bool) {
+size_t resident_lds(uint32_t lmax) {
     const size_t tile = tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax;
     return std::max<size_t>(4 * ((tile + 15) / 16 * 16), 8 * kRecordBlock);
 }
@@ -1445,7 +1445,7 @@ size_t resident_lds(uint32_t lmax, bool) {
 void launch_resident(const CaseDev* cases, int ncases, uint32_t lmax, int steps, int parity0, int sample_first,
                      uint32_t feat_union, bool formed, hipStream_t stream) {
     const uint32_t tile = (uint32_t)((tiled_lds_nodes_offset(lmax) + 16 * (size_t)lmax + 15) / 16 * 16);
-    const size_t lds = resident_lds(lmax, formed);
+    const size_t lds = resident_lds(lmax);
     const bool extra = (feat_union & (kFeatContact | kFeatDrive | kFeatDamp)) != 0;
 #define SHIPFEM_RES(P, D, F, X)                                                                           \
     hipLaunchKernelGGL((k_resident<P, D, F, X>), dim3(ncases), dim3(kResidentBlock), lds, stream, cases, lmax, \

@@ -195,7 +195,7 @@ void launch_reduce(const CaseDev* cases, int ncases, int parity, hipStream_t str
 void launch_resident(const CaseDev* cases, int ncases, uint32_t lmax, int steps, int parity0, int sample_first,
                      uint32_t feat_union, bool formed, hipStream_t stream);
 // Dynamic LDS bytes that launch takes.
-size_t resident_lds(uint32_t lmax, bool formed);
+size_t resident_lds(uint32_t lmax);
 // FP64 largest eigenvalue of B^T C B per element (the lumped-mass element frequency
 // is omega^2 = 4 lambda / rho), reduced to a per-wave max, then to one value per case.
 void launch_omega(const CaseDev* cases, const Tile* tiles, int ntiles, hipStream_t stream);
