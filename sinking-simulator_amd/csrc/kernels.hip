@@ -736,7 +736,8 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 // local node (10 bits) and the corner's rank in that node's run (6 bits), so a tet
 // lane writes each corner force straight into its slot and a node lane sums one
 // contiguous run: the tile needs no separate node->corner map.
-// LDS: corner-slot forces [3][4 kTile] floats (component planes), the run offsets
+// LDS: corner-slot forces as an (x, y) float2 plane and a z plane of 4 kTile slots each (an
+// 8-byte and a 4-byte access per slot cost less LDS time than three 4-byte ones), the run offsets
 // [lmax + 1] (16-bit), then the tile's local nodes' displacement float4[lmax]. Every
 // global load of the staging is issued before the first barrier, so its gathers share one
 // latency. LDS bandwidth, not HBM or the FP32 math, is what the pass spends most of its
@@ -759,7 +760,8 @@ template <bool PLASTIC, bool DAMAGE, bool FORMED>
 __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t first, uint32_t lid, bool active,
                                                  int64_t n, uint32_t lmax, int sample, unsigned char* smem,
                                                  const int64_t* nptr = nullptr) {
-    float* sh_f = reinterpret_cast<float*>(smem);
+    float2* sh_fxy = reinterpret_cast<float2*>(smem);                 // corner slots' x, y
+    float* sh_fz = reinterpret_cast<float*>(smem + 2 * kSlots4 * sizeof(float));  // and z
     uint16_t* sh_lo = reinterpret_cast<uint16_t*>(smem + tiled_lds_offsets_offset());
     float4* sh_u = reinterpret_cast<float4*>(smem + tiled_lds_nodes_offset(lmax));
 
@@ -807,11 +809,14 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
         // holds, so no store waits behind a branch
         sh_lo[min(la, L)] = static_cast<uint16_t>(oa);
         sh_lo[min(lb, L)] = static_cast<uint16_t>(ob);
-        sh_u[min(la, L - 1)] = ua;
-        sh_u[min(lb, L - 1)] = ub;
+        // a local node's displacement with its run offset in the fourth word: a tet's one
+        // 16-byte read per corner gives both (a 12-byte LDS read costs about twice a 16-byte one)
+        if (la < L) sh_u[la] = make_float4(ua.x, ua.y, ua.z, __uint_as_float(oa));
+        if (lb < L) sh_u[lb] = make_float4(ub.x, ub.y, ub.z, __uint_as_float(ob));
         for (uint32_t l = lid + 2 * kTile; l <= L; l += kTile) sh_lo[l] = lo[l];
         for (uint32_t l = lid + 2 * kTile; l < L; l += kTile) {
-            sh_u[l] = c.u4[c.tnode[p0 + l]];
+            const float4 u = c.u4[c.tnode[p0 + l]];
+            sh_u[l] = make_float4(u.x, u.y, u.z, __uint_as_float((uint32_t)lo[l]));
         }
     }
     __syncthreads();
@@ -820,10 +825,12 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
     if (live) {
         const uint32_t w4[4] = {lc.x & 0xFFFFu, lc.x >> 16, lc.y & 0xFFFFu, lc.y >> 16};
         uint32_t ln[4], slot[4];
+        float4 uq[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             ln[q] = w4[q] & 1023u;
-            slot[q] = sh_lo[ln[q]] + (w4[q] >> 10);
+            uq[q] = sh_u[ln[q]];
+            slot[q] = __float_as_uint(uq[q].w) + (w4[q] >> 10);
         }
         // Dm^-1 and V from the rest positions: Dm's column k is corner k+1 minus corner 0;
         // the inverse by cofactors, a degenerate tet (|det| <= 1e-30) taking zero
@@ -852,17 +859,18 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
         }
         float f[12];
         bool tore;
-        element_force<PLASTIC, DAMAGE>(c, e, n, sh_u[ln[0]], sh_u[ln[1]], sh_u[ln[2]],
-                                       sh_u[ln[3]], B, V, sample, f, diss, se, fract, tore);
+        element_force<PLASTIC, DAMAGE>(c, e, n, uq[0], uq[1], uq[2], uq[3], B, V, sample, f, diss, se, fract,
+                                       tore);
         if (DAMAGE && tore) {
             const uint32_t g[4] = {c.tnode[p0 + ln[0]], c.tnode[p0 + ln[1]], c.tnode[p0 + ln[2]],
                                    c.tnode[p0 + ln[3]]};
             release_torn(c, g, f, fract);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int a = 0; a < 3; ++a) sh_f[a * kSlots4 + slot[q]] = f[q * 3 + a];
+        for (int q = 0; q < 4; ++q) {
+            sh_fxy[slot[q]] = make_float2(f[q * 3], f[q * 3 + 1]);
+            sh_fz[slot[q]] = f[q * 3 + 2];
+        }
     }
     if (active) element_wave_sums<PLASTIC, DAMAGE>(c, e, sample, diss, se, fract);
     __syncthreads();
@@ -877,8 +885,10 @@ __device__ __forceinline__ void tiled_elem_block(const CaseDev& c, uint32_t firs
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k) {
                 const uint32_t s = min(q + k, q1 - 1);
-#pragma unroll
-                for (int a = 0; a < 3; ++a) g[k][a] = sh_f[a * kSlots4 + s];
+                const float2 xy = sh_fxy[s];
+                g[k][0] = xy.x;
+                g[k][1] = xy.y;
+                g[k][2] = sh_fz[s];
             }
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k)
