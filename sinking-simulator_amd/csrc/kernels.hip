@@ -740,10 +740,9 @@ __global__ __launch_bounds__(BLOCK) void k_elem_hex_plastic(const CaseDev* __res
 // 8-byte and a 4-byte access per slot cost less LDS time than three 4-byte ones), the run offsets
 // [lmax + 1] (16-bit), then the tile's local nodes' displacement float4[lmax]. Every
 // global load of the staging is issued before the first barrier, so its gathers share one
-// latency. LDS bandwidth, not HBM or the FP32 math, is what the pass spends most of its
-// time on (counters: the LDS busy most of the pass, nearly half of it in bank conflicts),
-// so a tet that forms its Dm^-1 reads its corners' rest positions through the vector L1
-// (the tile's few KB of them stay cached) rather than from a staged LDS copy.
+// latency. A tet that forms its Dm^-1 reads its corners' rest positions through the vector
+// L1 (the tile's few KB of them stay cached) rather than from a staged LDS copy: it keeps
+// the LDS traffic, most of it the tets' scattered accesses, to the displacements and slots.
 constexpr uint32_t kSlots4 = 4 * kTile;  // corner slots of a tile
 __host__ __device__ constexpr size_t tiled_lds_offsets_offset() { return 3 * kSlots4 * sizeof(float); }
 __host__ __device__ constexpr size_t tiled_lds_nodes_offset(uint32_t lmax) {
