@@ -1180,8 +1180,10 @@ constexpr int kRecordBlock = 1024;
 constexpr int kQ = kAccElemSlots + kSmpElemSlots + kAccNodeSlots + kSmpNodeSlots;
 
 // The energy record of the case's step n0 from its per-wave slots, by its 1024 threads
-// (lane `tid`) through `sh` (1024 doubles): every thread folds its strided share, then each
-// quantity goes through the same fixed tree. Ends with a barrier, so `sh` may be reused.
+// (lane `tid`) through `sh` (1024 doubles): every thread folds its strided share, each wave
+// folds its 64 lanes by the xor butterfly, and one lane per quantity sums the 16 waves'
+// partials in wave order: a fixed tree for every quantity, with two barriers instead of one
+// per tree level and quantity. Ends with a barrier, so `sh` may be reused.
 __device__ __forceinline__ void reduce_record(const CaseDev& c, int64_t n0, uint32_t tid, double* sh) {
     const uint32_t ew = c.elem_waves, nw = c.node_waves;
     // q: element accumulators, the element sample, node accumulators, node samples
@@ -1200,18 +1202,24 @@ __device__ __forceinline__ void reduce_record(const CaseDev& c, int64_t n0, uint
 #pragma unroll
         for (int k = 0; k < kSmpNodeSlots; ++k) q[QM + k] += c.smp_n[k * nw + w];
     }
-    double tot[kQ];
-    for (int k = 0; k < kQ; ++k) {
-        sh[tid] = q[k];
-        __syncthreads();
-        for (int s = kRecordBlock / 2; s > 0; s >>= 1) {
-            if ((int)tid < s) sh[tid] += sh[tid + s];
-            __syncthreads();
-        }
-        tot[k] = sh[0];
-        __syncthreads();
+    constexpr uint32_t kWaves = kRecordBlock / 64;
+    static_assert(kWaves * kQ + kQ <= kRecordBlock, "record partials fit the shared buffer");
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) q[k] = wave_sum(q[k]);
+    if ((tid & 63u) == 0)
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) sh[(tid >> 6) * kQ + k] = q[k];
+    __syncthreads();
+    if (tid < (uint32_t)kQ) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < kWaves; ++w) t += sh[w * kQ + tid];
+        sh[kWaves * kQ + tid] = t;
     }
+    __syncthreads();
     if (tid == 0) {
+        double tot[kQ];
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) tot[k] = sh[kWaves * kQ + k];
         const int32_t h = *c.hist_count;
         if (h < c.hist_cap) {
             const double* sn = tot + QM;
@@ -1232,6 +1240,7 @@ __device__ __forceinline__ void reduce_record(const CaseDev& c, int64_t n0, uint
             *c.hist_count = h + 1;
         }
     }
+    __syncthreads();
 }
 
 __global__ __launch_bounds__(kRecordBlock) void k_reduce(const CaseDev* __restrict__ cases,
