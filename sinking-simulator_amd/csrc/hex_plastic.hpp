@@ -517,8 +517,12 @@ __host__ __device__ inline Result update(const Material& m, const double* cXs, c
             for (int s = 0; s < 8; ++s) cs[l][s] = 0.0;
         double dsum = 0.0, rsum = 0.0, usum = 0.0;
         int failed = 0;
+        // the points level by level, so each level's assumed-row sums cs[lv] are fixed registers
+#pragma unroll
+        for (int lv = 0; lv < 2; ++lv)
 #pragma unroll 1
-        for (int g = 0; g < 8; ++g) {
+        for (int gq = 0; gq < 4; ++gq) {
+            const int g = 4 * lv + gq;
             double st[6], h0[8], h[8], sig[6], n6[6], dg, th, thb, rel;
             strainAt(g, eas, st);
             loadPoint(hist, g, stride, hmask, h0);
@@ -551,7 +555,6 @@ __host__ __device__ inline Result update(const Material& m, const double* cXs, c
             }
             addGrad(F, p0, 0, xi, eta, zeta);
             addGrad(F, p1, 1, xi, eta, zeta);
-            const int lv = (g >> 2) & 1;
             cs[lv][0] += w[g] * tau[4] * 0.5 * (1 - xi);
             cs[lv][1] += w[g] * tau[4] * 0.5 * (1 + xi);
             cs[lv][2] += w[g] * tau[5] * 0.5 * (1 - eta);
