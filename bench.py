@@ -8,7 +8,8 @@ Default workload (c4): the batched configuration, 64 independent clamped 20 mm s
 FP32 + Kahan state), one step = one element pass + one node pass over every case, replayed
 from a captured graph. `value` is batched tet-steps/s over exactly K timed steps. Beside it,
 measured in the same run on the same 64 cases: the plain path (one case per solver, two plain
-launches per step) and the ratio with its HBM bound; the roofline of the dominant kernel on
+launches per step; with this library's kernels, and with the reference's slot scheme) and the
+ratios with the HBM bound; the roofline of the dominant kernel on
 the bytes its layout must move, timed live with HIP events on the batch's stream; the large
 plate (1,209,600 tets) on the plain path; a short served leg (Poisson arrivals of stiffened
 panels) with p50/p99 latency; and the reference's FP32 CPU step on this host's cores.
@@ -606,7 +607,7 @@ def bench_headline(args, sf, info):
         # the plain path on the same 64 cases: one solver per case, two plain launches per
         # step (the reference's dispatch pattern), the cases stepped one after another
         per = []
-        plain_time = graph_time = 0.0
+        plain_time = graph_time = slots_time = 0.0
         for c in cases:
             s = sf.Solver(c)
             s.run(timed_from - W)  # from the state the batched steps were timed from
@@ -622,9 +623,17 @@ def bench_headline(args, sf, info):
             t1 = time.perf_counter()
             s.run(K, mode=0)
             graph_time += time.perf_counter() - t1
+            # and with the reference's own kernel scheme: one 48-byte force slot per tet,
+            # gathered through the CSR node map (tet_forces.comp + node_integrate.comp)
+            s.set_launch(sf.VARIANT_SLOTS)
+            s.run(W, mode=1)
+            t1 = time.perf_counter()
+            s.run(K, mode=1)
+            slots_time += time.perf_counter() - t1
             s.close()
         plain_rate = tets * K / plain_time
         plain_graph_rate = tets * K / graph_time
+        plain_slots_rate = tets * K / slots_time
         bins = [(1e4, 3e4), (3e4, 1e5), (1e5, 3.01e5)]
         by_bin = []
         for lo, hi in bins:
@@ -636,7 +645,8 @@ def bench_headline(args, sf, info):
                                    us_per_step=1e6 * sum(t / r for t, r in xs) / len(xs)))
         cfg.update(plain_rate=plain_rate, batched_over_plain=value / plain_rate,
                    batched_over_plain_bound=batched_bound / plain_rate, plain_by_size=by_bin,
-                   plain_graph_rate=plain_graph_rate, batched_over_plain_graph=value / plain_graph_rate)
+                   plain_graph_rate=plain_graph_rate, batched_over_plain_graph=value / plain_graph_rate,
+                   plain_slots_rate=plain_slots_rate, batched_over_plain_slots=value / plain_slots_rate)
     if "plate" in legs:
         case, desc = workload("plate")
         s = sf.Solver(case)
