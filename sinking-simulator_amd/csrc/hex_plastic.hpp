@@ -413,9 +413,9 @@ __host__ __device__ inline Start startOf(const double* hist, uint32_t stride) {
 
 // The element update. cXs, cX: basis coefficients of the rest corner offsets (corner 0 at
 // zero), the same values as the set-up reads them (an array) and as the later passes read
-// them (on the device, this lane's column of a block-shared array); cd: those of the co-rotated displacement offsets; hist: 8 points x (plastic strain
-// 6, eqps, damage) at hist[(8 g + k) * stride]; enh: 7 enhanced parameters at
-// enh[k * stride]. Writes the internal force of corners 1..7 (corner 0 carries minus their
+// them (on the device, this lane's column of a block-shared array); cd: those of the
+// co-rotated displacement offsets; hist: 8 points x (plastic strain 6, eqps, damage) at
+// hist[(8 g + k) * stride]; enh: 7 enhanced parameters at enh[k * stride]. Writes the internal force of corners 1..7 (corner 0 carries minus their
 // sum) in the co-rotated frame. A torn element (all points failed) is left as it is and
 // returns a zero force. start: startOf(hist, stride) at the step's start; alpha0, when
 // given, the enhanced parameters as already read from enh.
