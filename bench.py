@@ -554,7 +554,7 @@ def bench_headline(args, sf, info):
     timed_from = int(states[0].step) - K
     del states
     # per-kernel durations, live, of the one-stream kernels (each over all 64 cases; with
-    # two streams the halves' kernels share the device and no kernel has a duration of its
+    # several streams the parts' kernels share the device and no kernel has a duration of its
     # own): after the timed steps the batch switches to one stream and runs whole chunks;
     # that segment's device time per step (HIP events around the graph replay on the
     # batch's stream) is apportioned between the two passes by their durations when each
@@ -600,7 +600,7 @@ def bench_headline(args, sf, info):
                batch_arena_bytes=arena,
                tuned=best, device_ms_per_step=dev_ms / K, timed_steps_from=timed_from,
                eqps_max=eqps_max, yielded_tets=yielded,
-               streams=2 if best.get("split", -1) == 0 else 1,
+               streams=min(4, len(cases)) if best.get("split", -1) == 0 else 1,
                step_bytes=int(step_need), step_hbm_frac=round(step_need * K / (dev_ms * 1e-3) / 8e12, 4),
                batched_bound_at_hbm_peak=batched_bound)
     if "plain" in legs:

@@ -264,10 +264,10 @@ typedef struct shipfem_launch_config {
     int32_t xcd_remap;  /* 1: give each XCD a contiguous range of tiles */
     int32_t chunk;      /* steps per captured graph (even) */
     int32_t variant;    /* SHIPFEM_VARIANT_* */
-    int32_t split;      /* a batch of two or more cases: 0 runs them as two halves (balanced
-                         * by element count) on two streams, each replaying its own captured
-                         * chunks, so one half's kernels fill the other's launch gaps and
-                         * tails (each case's arithmetic unchanged); -1: one stream */
+    int32_t split;      /* a batch of two or more cases: 0 runs them as up to four parts
+                         * (balanced by element count) on as many streams, each replaying its
+                         * own captured chunks, so each part's kernels fill the others' launch
+                         * gaps and tails (each case's arithmetic unchanged); -1: one stream */
     int32_t rest_from_positions; /* tiled variant: 1 forms each tet's Dm^-1 and V in the pass from
                          * its tile's rest positions (16 B per tile node read instead of 40 B
                          * per tet, ~40 more FP32 operations a tet, results within ~1e-6 of the

@@ -156,7 +156,7 @@ struct LaunchShape {
     int node_block = 256;
     int xcd_remap = 1;
     int variant = kTiled;
-    int split = 1;   // a batch: two halves on two streams
+    int split = 1;   // a batch: up to four parts on as many streams
     int formed = 0;    // tiled pass: form Dm^-1 and V from the tile's rest positions
     int resident = 1;  // plans of small tiled cases step whole chunks in one launch
 };

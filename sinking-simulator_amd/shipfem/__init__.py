@@ -883,8 +883,9 @@ class Solver:
 
 
 class Batch:
-    """Many cases advancing in one launch per phase. split=0: two or more cases run as two
-    halves on two streams side by side; split=-1: one stream."""
+    """Many cases advancing in one launch per phase. split=0: two or more cases run as parts
+    (up to four, balanced by element count) on as many streams side by side; split=-1: one
+    stream."""
 
     def __init__(self, cases, elem_block=256, node_block=256, xcd_remap=1, chunk=0,
                  variant=VARIANT_TILED, split=0, rest_from_positions=0):
