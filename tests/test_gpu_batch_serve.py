@@ -107,6 +107,19 @@ def test_launch_shape_does_not_change_any_bit(shape, variant):
         assert same(state(ref, i), state(other, i)), (shape, i)
 
 
+@pytest.mark.parametrize("n", [2, 3, 6])
+def test_parts_on_their_own_streams_keep_every_bit(n):
+    # split=0 runs min(n, 4) parts, one stream each (six cases: parts of one and two cases)
+    cases = (mix() + [configs.plate(lx=0.5, ly=0.2, nx=18, ny=8, ramp=1e-4), small_panel(speed=6.0)])[:n]
+    one = sf.Batch(cases, split=-1)
+    parts = sf.Batch(cases, split=0)
+    for steps in (250, 51):  # whole chunks, then a ragged tail of plain steps
+        one.run(steps)
+        parts.run(steps)
+    for i in range(n):
+        assert same(state(one, i), state(parts, i)), (n, i)
+
+
 def test_tuner_picks_a_shape_and_restores_the_state():
     cases = mix()
     b = sf.Batch(cases)
